@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Benchmark: 2D Euler isentropic vortex, P3, 256x256x2 = 131072 jittered periodic
+triangles on [0,10]^2, local Lax-Friedrichs flux, RK4 at CFL 0.1.
+
+One "step" is one RK4 step (4 stages, each a fused residual + stage-update
+kernel) over the whole mesh, replayed from a hipGraph; inputs are resident in
+HBM before the timed region.  The headline value is the reduced form (FP32
+volume/face arithmetic, FP64 stage accumulation and inverse mass); the plain
+FP64 form is timed beside it, with the density L2 error of both against the
+exact vortex and against each other.  The CPU baseline is the FP64 CPU oracle
+(a restatement of the reference algorithm) run on a bounded sample on this
+host's cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "dg-parallel_amd"))
+
+NX = 256
+ORDER = 3
+CFL = 0.1
+SEED = 20240611
+VORTEX = (5.0, 5.0, 5.0, 1.0, 1.0, 10.0)
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_dof_stage(fp32, nf, np_):
+    """Algorithmic HBM bytes per DOF and RK4 stage of the fused stage kernel:
+    stage input read once, FP64 step-start state and accumulator, stage output,
+    plus the element table (|J| J^-1: 4 doubles, |J|, 3 int32 links) per element."""
+    s = 4 if fp32 else 8
+    st = [8 + 8 + s,            # stage 1: read U^n, write acc, write U1
+          s + 8 + 16 + s,       # stage 2: read U1, U^n, acc; write acc, U2
+          s + 8 + 16 + s,       # stage 3
+          s + 8 + 8 + 8]        # stage 4: read U3, U^n, acc; write U^{n+1}
+    table = (4 * 8 + 8 + 3 * 4) / (nf * np_)
+    return sum(st) / 4.0 + table
+
+
+def run_form(dgpar, mesh, precision, steps, warmup):
+    import torch
+    law = dgpar.EulerLaw(dgpar.FLUX_LF, 1.4)
+    g = dgpar.DGAnalysis(mesh, law, ORDER, precision=precision, use_graph=True, device=0)
+    g.project(dgpar.FIELD_VORTEX, VORTEX)
+    dt = g.computeDt(CFL)
+    if warmup > 0:
+        g.time_advance(dgpar.RK4, dt, warmup)
+    g.project(dgpar.FIELD_VORTEX, VORTEX)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms = g.time_advance(dgpar.RK4, dt, steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    err = g.l2_error(dgpar.FIELD_VORTEX, VORTEX, steps * dt)
+    state = g.get_state()
+    ndof = g.ndof
+    g.close()
+    return dict(ms=ms, wall=wall, dt=dt, err=err, state=state, ndof=ndof)
+
+
+def cpu_baseline(mesh, steps=5):
+    from oracle import pyoracle as po
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    o = po.Oracle(2, mesh.vids, mesh.xyz, ORDER, po.LAW_EULER2D, po.FLUX_LF, gamma=1.4)
+    U = o.project(1, VORTEX)
+    dt = o.dt(U, CFL)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        U, _ = o.step(U, po.RK4, s * dt, dt, nthreads=threads)
+    el = time.perf_counter() - t0
+    return dict(value=o.ndof * 4 * steps / el, unit="DOF*stage/s", cores=threads, kind="port",
+                sample="%d RK4 steps of the full 131072-triangle P3 vortex mesh (FP64 CPU oracle, %d threads, %.1f s)"
+                % (steps, threads, el))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path; use --gpus 1", file=sys.stderr)
+        return 2
+    import dgpar
+
+    mesh = dgpar.mesh_tri_structured(NX, NX, lx=10.0, ly=10.0, jitter=0.1, seed=SEED)
+    E = mesh.n_elems
+    r32 = run_form(dgpar, mesh, dgpar.PREC_FP32, args.steps, args.warmup)
+    r64 = run_form(dgpar, mesh, dgpar.PREC_FP64, args.steps, args.warmup)
+    ndof = r32["ndof"]
+    np_ = (ORDER + 1) * (ORDER + 2) // 2
+    X = mesh.xyz
+    detj = np.abs((X[:, 1, 0] - X[:, 0, 0]) * (X[:, 2, 1] - X[:, 0, 1]) -
+                  (X[:, 2, 0] - X[:, 0, 0]) * (X[:, 1, 1] - X[:, 0, 1]))
+    d = (r32["state"] - r64["state"]).reshape(E, 4, np_)[:, 0, :]
+    l2_rho_diff = float(np.sqrt((d ** 2 * detj[:, None]).sum()))
+
+    def thr(r):
+        return ndof * 4 * args.steps / (r["ms"] / 1e3)
+
+    v32, v64 = thr(r32), thr(r64)
+    launch_ms = r32["ms"] / (4 * args.steps)
+    b32 = bytes_per_dof_stage(True, 4, np_)
+    b64 = bytes_per_dof_stage(False, 4, np_)
+    ach = b32 * ndof / (launch_ms / 1e3) / 1e9
+    ach64 = b64 * ndof / (r64["ms"] / (4 * args.steps) / 1e3) / 1e9
+    out = {
+        "metric": "DOF*stage/s of the FP32-arithmetic / FP64-accumulation DG residual + RK4 stage (vs FP64 plain)",
+        "value": v32,
+        "unit": "DOF*stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 stage accumulation)",
+        "data": "synthetic: isentropic vortex projected on a seeded jittered mesh",
+        "config": {"workload": "euler2d-isentropic-vortex P3 LF RK4 CFL0.1", "elements": E, "dof": ndof,
+                   "mesh": "256x256x2 periodic [0,10]^2 jitter 0.1h seed %d" % SEED, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "tri_stage_kernel<3,EulerLawD,float,*>",
+                     "bytes_per_dof_stage": b32},
+        "fp64_plain": {"value": v64, "ms_per_step": r64["ms"] / args.steps, "achieved_GBs": ach64,
+                       "frac": ach64 / HBM_PEAK_GBS, "bytes_per_dof_stage": b64},
+        "ratio_fp32_over_fp64": v32 / v64,
+        "l2_rho_vs_exact": {"fp32": float(r32["err"][0]), "fp64": float(r64["err"][0])},
+        "l2_rho_fp32_vs_fp64": l2_rho_diff,
+        "wall_s_timed": r32["wall"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mesh)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,670 @@
+// C ABI implementation: host-side table construction (FP64), device buffers,
+// kernel launches and hipGraph replay of Runge-Kutta steps.
+//
+// Host setup replaces what DGAnalysis's constructor builds per cell and per face
+// (DGAnalysis.cc:133-302, DGCell.cc:127-182, 423-555) with structure-of-arrays
+// tables: per element |J| J^-1, |J|, size and three face links; per order a
+// block of basis values at quadrature points shared by all elements.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/dg_capi.h"
+#include "dg_fields.h"
+#include "dg_kernels.h"
+#include "dg_tables.h"
+
+using namespace dgp;
+
+struct dg_ctx {
+  std::string err;
+  int dim = 2, E = 0, Nf = 0, P = 0, NP = 0;
+  int law = 0, prec = 0, use_graph = 0, device = 0;
+  LawParams lp{};
+  size_t ndof = 0;
+  std::vector<int32_t> perm;  // internal index -> caller element
+  std::vector<int32_t> faces; // (eL, lL, eR, lR) caller numbering
+  std::vector<double> xyz;    // caller corner coordinates [E][3][2]
+  hipStream_t stream = nullptr;
+  // device memory
+  double *Un = nullptr, *acc = nullptr, *A64 = nullptr, *B64 = nullptr, *R = nullptr;
+  float *A32 = nullptr, *B32 = nullptr;
+  double *mg = nullptr, *detj = nullptr, *size = nullptr, *tab = nullptr, *vtx = nullptr, *part = nullptr;
+  float* tabf = nullptr;
+  int32_t* link = nullptr;
+  std::vector<void*> allocs;
+  // graph cache
+  hipGraphExec_t gexec = nullptr;
+  int g_scheme = -1;
+  double g_dt = 0.0;
+};
+
+static dg_status fail(dg_ctx* c, dg_status s, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return s;
+}
+
+#define HIPCHECK(c, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) return fail((c), DG_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static dg_status dalloc(dg_ctx* c, T** p, size_t n) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, DG_ERR_OOM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  }
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return DG_OK;
+}
+
+static void free_all(dg_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  for (void* p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+}
+
+static int nfields(int law) {
+  switch (law) {
+    case DG_LAW_SCALAR: return 1;
+    case DG_LAW_EULER2D: return 4;
+    case DG_LAW_MAXWELL3D: return 6;
+  }
+  return -1;
+}
+
+namespace {
+struct FaceKey {
+  int32_t k[2];
+  int32_t e, l;
+  bool operator<(const FaceKey& o) const {
+    if (k[0] != o.k[0]) return k[0] < o.k[0];
+    if (k[1] != o.k[1]) return k[1] < o.k[1];
+    if (e != o.e) return e < o.e;
+    return l < o.l;
+  }
+};
+}  // namespace
+
+extern "C" {
+
+const char* dg_last_error(const dg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+size_t dg_ndof(const dg_ctx* c) { return c ? c->ndof : 0; }
+int dg_nfaces(const dg_ctx* c) { return c ? (int)(c->faces.size() / 4) : 0; }
+
+dg_status dg_faces(const dg_ctx* c, int32_t* out4, size_t n) {
+  if (!c || !out4 || n < c->faces.size()) return DG_ERR_ARG;
+  std::memcpy(out4, c->faces.data(), c->faces.size() * sizeof(int32_t));
+  return DG_OK;
+}
+
+dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, dg_ctx** out) {
+  if (!out) return DG_ERR_ARG;
+  *out = nullptr;
+  dg_ctx* c = new (std::nothrow) dg_ctx();
+  if (!c) return DG_ERR_OOM;
+  auto bail = [&](dg_status s) {
+    // keep the context alive only to report the error through dg_last_error
+    free_all(c);
+    *out = c;
+    return s;
+  };
+  if (!m || !law || !opt || !m->vids || !m->xyz || m->n_elems <= 0) return bail(fail(c, DG_ERR_ARG, "null mesh/law/options"));
+  if (m->dim != 2) return bail(fail(c, DG_ERR_UNSUPPORTED, "only triangles (dim 2) are built"));
+  c->dim = m->dim;
+  c->E = m->n_elems;
+  c->law = law->law;
+  c->Nf = nfields(law->law);
+  if (c->Nf < 0 || c->law == DG_LAW_MAXWELL3D) return bail(fail(c, DG_ERR_UNSUPPORTED, "law %d not built for triangles", law->law));
+  if (c->law == DG_LAW_EULER2D && law->flux != DG_FLUX_LF && law->flux != DG_FLUX_ROE)
+    return bail(fail(c, DG_ERR_UNSUPPORTED, "Euler flux %d not built", law->flux));
+  if (c->law == DG_LAW_SCALAR && law->flux != DG_FLUX_UPWIND)
+    return bail(fail(c, DG_ERR_UNSUPPORTED, "scalar flux %d not built", law->flux));
+  int P = opt->order;
+  if (m->order) {
+    P = m->order[0];
+    for (int e = 1; e < c->E; e++)
+      if (m->order[e] != P) return bail(fail(c, DG_ERR_UNSUPPORTED, "mixed orders are not built in this entry point"));
+  }
+  if (P < 1 || P > MAXP_TRI) return bail(fail(c, DG_ERR_UNSUPPORTED, "order %d outside 1..%d", P, MAXP_TRI));
+  if (opt->precision != DG_PREC_FP64 && opt->precision != DG_PREC_FP32) return bail(fail(c, DG_ERR_ARG, "bad precision"));
+  c->P = P;
+  c->NP = np_tri(P);
+  c->prec = opt->precision;
+  c->use_graph = opt->use_graph;
+  c->device = opt->device;
+  c->lp.gamma = law->gamma;
+  for (int i = 0; i < 3; i++) c->lp.vel[i] = law->vel[i];
+  for (int i = 0; i < 8; i++) c->lp.farfield[i] = law->farfield[i];
+  c->lp.flux = law->flux;
+  const int E = c->E, NF = c->Nf, NP = c->NP;
+  c->ndof = (size_t)E * NF * NP;
+  c->xyz.assign(m->xyz, m->xyz + (size_t)E * 6);
+
+  // ---- connectivity (mMesh.cc:321-378, translation-periodic by vertex ids) ----
+  std::vector<FaceKey> keys((size_t)E * 3);
+  for (int e = 0; e < E; e++)
+    for (int l = 0; l < 3; l++) {
+      int32_t a = m->vids[e * 3 + l], b = m->vids[e * 3 + (l + 1) % 3];
+      if (a == b) return bail(fail(c, DG_ERR_ARG, "degenerate edge in element %d", e));
+      FaceKey& k = keys[(size_t)e * 3 + l];
+      k.k[0] = std::min(a, b);
+      k.k[1] = std::max(a, b);
+      k.e = e;
+      k.l = l;
+    }
+  std::sort(keys.begin(), keys.end());
+  std::vector<int32_t> nbe((size_t)E * 3, -1), nbl((size_t)E * 3, -1), same((size_t)E * 3, 0);
+  for (size_t i = 0; i < keys.size();) {
+    size_t j = i + 1;
+    while (j < keys.size() && keys[j].k[0] == keys[i].k[0] && keys[j].k[1] == keys[i].k[1]) j++;
+    if (j - i > 2) return bail(fail(c, DG_ERR_ARG, "edge (%d,%d) shared by %zu elements", keys[i].k[0], keys[i].k[1], j - i));
+    const FaceKey& L = keys[i];
+    if (j - i == 2) {
+      const FaceKey& Rk = keys[i + 1];
+      nbe[L.e * 3 + L.l] = Rk.e;
+      nbl[L.e * 3 + L.l] = Rk.l;
+      nbe[Rk.e * 3 + Rk.l] = L.e;
+      nbl[Rk.e * 3 + Rk.l] = L.l;
+      int s = m->vids[L.e * 3 + L.l] == m->vids[Rk.e * 3 + Rk.l];
+      same[L.e * 3 + L.l] = same[Rk.e * 3 + Rk.l] = s;
+      c->faces.insert(c->faces.end(), {L.e, L.l, Rk.e, Rk.l});
+    } else {
+      c->faces.insert(c->faces.end(), {L.e, L.l, -1, -1});
+    }
+    i = j;
+  }
+
+  // ---- internal order: group elements whose face links share (nb local face,
+  //      orientation, boundary code) so a wavefront takes one path ------------
+  auto sig = [&](int e) {
+    int s = 0;
+    for (int l = 0; l < 3; l++) {
+      int v;
+      if (nbe[e * 3 + l] >= 0)
+        v = nbl[e * 3 + l] | (same[e * 3 + l] << 2);
+      else {
+        int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
+        v = 8 + (tag == DG_TAG_WALL ? BC_WALL : BC_FARFIELD);
+      }
+      s = s * 16 + v;
+    }
+    return s;
+  };
+  c->perm.resize(E);
+  for (int e = 0; e < E; e++) c->perm[e] = e;
+  std::vector<int> sg(E);
+  for (int e = 0; e < E; e++) sg[e] = sig(e);
+  std::stable_sort(c->perm.begin(), c->perm.end(), [&](int a, int b) { return sg[a] < sg[b]; });
+  std::vector<int32_t> iperm(E);
+  for (int i = 0; i < E; i++) iperm[c->perm[i]] = i;
+
+  // ---- geometry (Mapping.cc:442-566, 687-712; DGCell.cc:1209-1229) ---------------
+  std::vector<double> h_mg((size_t)4 * E), h_detj(E), h_size(E);
+  std::vector<int32_t> h_link((size_t)3 * E);
+  for (int i = 0; i < E; i++) {
+    int e = c->perm[i];
+    const double* X = m->xyz + (size_t)e * 6;
+    double J00 = X[2] - X[0], J01 = X[4] - X[0];  // dx/du, dx/dv
+    double J10 = X[3] - X[1], J11 = X[5] - X[1];  // dy/du, dy/dv
+    double det = J00 * J11 - J01 * J10;
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return bail(fail(c, DG_ERR_ARG, "degenerate element %d", e));
+    double ad = std::fabs(det);
+    // |J| * (J^-1)[r][d]
+    double inv[2][2] = {{J11 / det, -J01 / det}, {-J10 / det, J00 / det}};
+    for (int r = 0; r < 2; r++)
+      for (int d = 0; d < 2; d++) h_mg[(size_t)(r * 2 + d) * E + i] = ad * inv[r][d];
+    h_detj[i] = ad;
+    double a = std::sqrt((X[2] - X[0]) * (X[2] - X[0]) + (X[3] - X[1]) * (X[3] - X[1]));
+    double b = std::sqrt((X[4] - X[2]) * (X[4] - X[2]) + (X[5] - X[3]) * (X[5] - X[3]));
+    double cc = std::sqrt((X[0] - X[4]) * (X[0] - X[4]) + (X[1] - X[5]) * (X[1] - X[5]));
+    double hp = 0.5 * (a + b + cc);
+    h_size[i] = std::sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
+    for (int l = 0; l < 3; l++) {
+      int32_t v;
+      if (nbe[e * 3 + l] >= 0)
+        v = (iperm[nbe[e * 3 + l]] << 3) | (same[e * 3 + l] << 2) | nbl[e * 3 + l];
+      else {
+        int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
+        v = -(tag == DG_TAG_WALL ? BC_WALL : BC_FARFIELD);
+      }
+      h_link[(size_t)l * E + i] = v;
+    }
+  }
+
+  // ---- tables ---------------------------------------------------------------------
+  std::vector<double> h_tab(TriBlock::size(P));
+  try {
+    build_tri_block(P, h_tab.data());
+  } catch (const std::exception& ex) {
+    return bail(fail(c, DG_ERR_ARG, "%s", ex.what()));
+  }
+  std::vector<float> h_tabf(h_tab.begin(), h_tab.end());
+  std::vector<double> h_vtx((size_t)4 * NP);
+  {
+    double z[3] = {0, 0, 0}, phi[64];
+    basis(2, 0, z, phi, nullptr);
+    h_vtx[0] = phi[0];  // computeMean: coefficient 0 times phi_0 (DGCell.cc:1133-1137)
+    static const double V[3][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+    for (int v = 0; v < 3; v++) {
+      basis(2, P, V[v], phi, nullptr);
+      for (int k = 0; k < NP; k++) h_vtx[(1 + v) * NP + k] = phi[k];
+    }
+  }
+
+  // ---- device ---------------------------------------------------------------------
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return bail(fail(c, DG_ERR_HIP, "no HIP device"));
+  }
+  if (c->device < 0 || c->device >= ndev) return bail(fail(c, DG_ERR_ARG, "device %d out of range", c->device));
+  if (hipSetDevice(c->device) != hipSuccess) return bail(fail(c, DG_ERR_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(c, DG_ERR_HIP, "hipStreamCreate failed"));
+  dg_status s;
+  const size_t nd = c->ndof;
+  if ((s = dalloc(c, &c->Un, nd)) || (s = dalloc(c, &c->acc, nd)) || (s = dalloc(c, &c->R, nd)) ||
+      (s = dalloc(c, &c->mg, (size_t)4 * E)) || (s = dalloc(c, &c->detj, E)) || (s = dalloc(c, &c->size, E)) ||
+      (s = dalloc(c, &c->link, (size_t)3 * E)) || (s = dalloc(c, &c->tab, h_tab.size())) ||
+      (s = dalloc(c, &c->tabf, h_tabf.size())) || (s = dalloc(c, &c->vtx, h_vtx.size())) ||
+      (s = dalloc(c, &c->part, (size_t)tri_stage_blocks(E) + 1)))
+    return bail(s);
+  if (c->prec == DG_PREC_FP64) {
+    if ((s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd))) return bail(s);
+  } else {
+    if ((s = dalloc(c, &c->A32, nd)) || (s = dalloc(c, &c->B32, nd))) return bail(s);
+  }
+  auto up = [&](void* d, const void* h, size_t bytes) {
+    return hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up(c->mg, h_mg.data(), h_mg.size() * 8) || !up(c->detj, h_detj.data(), E * 8) ||
+      !up(c->size, h_size.data(), E * 8) || !up(c->link, h_link.data(), h_link.size() * 4) ||
+      !up(c->tab, h_tab.data(), h_tab.size() * 8) || !up(c->tabf, h_tabf.data(), h_tabf.size() * 4) ||
+      !up(c->vtx, h_vtx.data(), h_vtx.size() * 8))
+    return bail(fail(c, DG_ERR_HIP, "table upload failed"));
+  if (hipMemset(c->Un, 0, nd * 8) != hipSuccess || hipMemset(c->acc, 0, nd * 8) != hipSuccess)
+    return bail(fail(c, DG_ERR_HIP, "memset failed"));
+  *out = c;
+  return DG_OK;
+}
+
+dg_status dg_destroy(dg_ctx* c) {
+  if (!c) return DG_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  free_all(c);
+  delete c;
+  return DG_OK;
+}
+
+static bool ready(const dg_ctx* c) { return c && c->stream && c->Un; }
+
+dg_status dg_set_state(dg_ctx* c, const double* U, size_t n) {
+  if (!ready(c) || !U || n != c->ndof) return fail(c, DG_ERR_ARG, "dg_set_state: need %zu doubles", c ? c->ndof : 0);
+  const int E = c->E, BS = c->Nf * c->NP;
+  std::vector<double> h(n);
+  for (int i = 0; i < E; i++) {
+    const double* src = U + (size_t)c->perm[i] * BS;
+    for (int j = 0; j < BS; j++) {
+      if (!std::isfinite(src[j])) return fail(c, DG_ERR_NONFINITE, "non-finite coefficient in element %d", c->perm[i]);
+      h[(size_t)j * E + i] = src[j];
+    }
+  }
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipMemcpyAsync(c->Un, h.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return DG_OK;
+}
+
+static dg_status fetch_soa(dg_ctx* c, const double* dsrc, double* U) {
+  const int E = c->E, BS = c->Nf * c->NP;
+  std::vector<double> h(c->ndof);
+  HIPCHECK(c, hipMemcpyAsync(h.data(), dsrc, c->ndof * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < E; i++) {
+    double* dst = U + (size_t)c->perm[i] * BS;
+    for (int j = 0; j < BS; j++) dst[j] = h[(size_t)j * E + i];
+  }
+  return DG_OK;
+}
+
+dg_status dg_get_state(dg_ctx* c, double* U, size_t n) {
+  if (!ready(c) || !U || n != c->ndof) return fail(c, DG_ERR_ARG, "dg_get_state: need %zu doubles", c ? c->ndof : 0);
+  HIPCHECK(c, hipSetDevice(c->device));
+  return fetch_soa(c, c->Un, U);
+}
+
+static StageArgs base_args(dg_ctx* c) {
+  StageArgs a{};
+  a.E = c->E;
+  a.P = c->P;
+  a.tab = c->tab;
+  a.tabf = c->tabf;
+  a.el.mg = c->mg;
+  a.el.detj = c->detj;
+  a.el.size = c->size;
+  a.el.link = c->link;
+  a.law = c->lp;
+  a.Un = c->Un;
+  a.acc = c->acc;
+  return a;
+}
+
+dg_status dg_residual(dg_ctx* c, double t, double* R_out) {
+  (void)t;  // no law here has a time-dependent source or boundary value
+  if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
+  HIPCHECK(c, hipSetDevice(c->device));
+  StageArgs a = base_args(c);
+  a.Uin = c->Un;
+  a.R = c->R;
+  a.scheme = MODE_RESIDUAL;
+  int variant = c->prec == DG_PREC_FP64 ? 0 : 1;
+  if (launch_tri_stage(a, c->law, variant, c->stream)) return fail(c, DG_ERR_HIP, "residual launch failed: %s", hipGetErrorString(hipGetLastError()));
+  if (R_out) return fetch_soa(c, c->R, R_out);
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return DG_OK;
+}
+
+dg_status dg_compute_dt(dg_ctx* c, double cfl, double* dt_out) {
+  if (!ready(c) || !dt_out || !(cfl > 0)) return fail(c, DG_ERR_ARG, "dg_compute_dt: bad argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  DtArgs a{};
+  a.E = c->E;
+  a.P = c->P;
+  a.U = c->Un;
+  a.vtx = c->vtx;
+  a.el.mg = c->mg;
+  a.el.detj = c->detj;
+  a.el.size = c->size;
+  a.el.link = c->link;
+  a.law = c->lp;
+  a.cfl = cfl;
+  a.part = c->part;
+  if (launch_tri_dt(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "dt launch failed");
+  int nb = tri_dt_blocks(c->E);
+  std::vector<double> h(nb);
+  HIPCHECK(c, hipMemcpyAsync(h.data(), c->part, nb * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  double dt = 1e300;
+  for (double v : h) dt = std::min(dt, v);
+  if (!std::isfinite(dt) || !(dt > 0)) return fail(c, DG_ERR_NONFINITE, "time step is %g", dt);
+  *dt_out = dt;
+  return DG_OK;
+}
+
+static int nstages(int scheme) {
+  switch (scheme) {
+    case DG_RK_FE: return 1;
+    case DG_RK_TVD2: return 2;
+    case DG_RK_TVD3: return 3;
+    case DG_RK4: return 4;
+  }
+  return -1;
+}
+
+// enqueue one full step on the stream; returns 0 on success
+static int enqueue_step(dg_ctx* c, int scheme, double dt, bool want_resid) {
+  const int ns = nstages(scheme);
+  StageArgs a = base_args(c);
+  a.dt = dt;
+  a.scheme = scheme;
+  const bool f64 = c->prec == DG_PREC_FP64;
+  void* bufA = f64 ? (void*)c->A64 : (void*)c->A32;
+  void* bufB = f64 ? (void*)c->B64 : (void*)c->B32;
+  for (int st = 0; st < ns; st++) {
+    a.stage = st;
+    const bool first = st == 0, last = st == ns - 1;
+    a.Uin = first ? (const void*)c->Un : ((st % 2) ? bufA : bufB);
+    a.Uout = last ? (void*)c->Un : ((st % 2) ? bufB : bufA);
+    if (first && last) a.Uout = bufA;  // forward Euler: never write the array being read
+    a.resid_part = (last && want_resid) ? c->part : nullptr;
+    int variant;
+    if (f64)
+      variant = 0;
+    else
+      variant = first ? 1 : (last ? 3 : 2);
+    if (launch_tri_stage(a, c->law, variant, c->stream)) return -1;
+  }
+  if (ns == 1) {
+    if (hipMemcpyAsync(c->Un, c->A64, c->ndof * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) return -1;
+  }
+  return 0;
+}
+
+static dg_status check_scheme(dg_ctx* c, int scheme) {
+  if (nstages(scheme) < 0) return fail(c, DG_ERR_ARG, "unknown RK scheme %d", scheme);
+  if (scheme == DG_RK_FE && c->prec != DG_PREC_FP64)
+    return fail(c, DG_ERR_UNSUPPORTED, "forward Euler is built for the FP64 form only");
+  return DG_OK;
+}
+
+static dg_status run_steps(dg_ctx* c, int scheme, double dt, int nsteps) {
+  if (nsteps <= 0) return DG_OK;
+  if (c->use_graph && scheme != DG_RK_FE) {
+    if (!c->gexec || c->g_scheme != scheme || c->g_dt != dt) {
+      if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+      hipGraph_t g;
+      HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_step(c, scheme, dt, false);
+      hipError_t ec = hipStreamEndCapture(c->stream, &g);
+      if (rc || ec != hipSuccess) return fail(c, DG_ERR_HIP, "graph capture failed");
+      hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) return fail(c, DG_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+      c->g_scheme = scheme;
+      c->g_dt = dt;
+    }
+    for (int s = 0; s < nsteps; s++) HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
+  } else {
+    for (int s = 0; s < nsteps; s++)
+      if (enqueue_step(c, scheme, dt, false)) return fail(c, DG_ERR_HIP, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
+  }
+  return DG_OK;
+}
+
+dg_status dg_advance(dg_ctx* c, int scheme, double t, double dt, int nsteps, double* resid_out) {
+  (void)t;
+  if (!ready(c) || nsteps < 0 || !(dt > 0)) return fail(c, DG_ERR_ARG, "dg_advance: bad argument");
+  dg_status s = check_scheme(c, scheme);
+  if (s) return s;
+  HIPCHECK(c, hipSetDevice(c->device));
+  int body = resid_out ? nsteps - 1 : nsteps;
+  if ((s = run_steps(c, scheme, dt, body))) return s;
+  if (resid_out && nsteps > 0) {
+    if (enqueue_step(c, scheme, dt, true)) return fail(c, DG_ERR_HIP, "stage launch failed");
+    int nb = tri_stage_blocks(c->E);
+    std::vector<double> h(nb);
+    HIPCHECK(c, hipMemcpyAsync(h.data(), c->part, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    double r = 0.0;
+    for (double v : h) r += v;
+    *resid_out = r;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return DG_OK;
+}
+
+dg_status dg_time_advance(dg_ctx* c, int scheme, double dt, int nsteps, float* ms_out) {
+  if (!ready(c) || !ms_out || nsteps <= 0) return fail(c, DG_ERR_ARG, "dg_time_advance: bad argument");
+  dg_status s = check_scheme(c, scheme);
+  if (s) return s;
+  HIPCHECK(c, hipSetDevice(c->device));
+  hipEvent_t e0, e1;
+  HIPCHECK(c, hipEventCreate(&e0));
+  HIPCHECK(c, hipEventCreate(&e1));
+  // build the graph (if any) outside the timed region
+  if (c->use_graph && scheme != DG_RK_FE && (!c->gexec || c->g_scheme != scheme || c->g_dt != dt)) {
+    if ((s = run_steps(c, scheme, dt, 1))) return s;
+  }
+  HIPCHECK(c, hipEventRecord(e0, c->stream));
+  s = run_steps(c, scheme, dt, nsteps);
+  HIPCHECK(c, hipEventRecord(e1, c->stream));
+  HIPCHECK(c, hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHECK(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_out = ms;
+  return s;
+}
+
+dg_status dg_set_state_device(dg_ctx* c, const double* dU) {
+  if (!ready(c) || !dU) return fail(c, DG_ERR_ARG, "bad argument");
+  std::vector<double> h(c->ndof);
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipMemcpy(h.data(), dU, c->ndof * 8, hipMemcpyDeviceToHost));
+  return dg_set_state(c, h.data(), c->ndof);
+}
+
+dg_status dg_get_state_device(dg_ctx* c, double* dU) {
+  if (!ready(c) || !dU) return fail(c, DG_ERR_ARG, "bad argument");
+  std::vector<double> h(c->ndof);
+  dg_status s = dg_get_state(c, h.data(), c->ndof);
+  if (s) return s;
+  HIPCHECK(c, hipMemcpy(dU, h.data(), c->ndof * 8, hipMemcpyHostToDevice));
+  return DG_OK;
+}
+
+dg_status dg_project(dg_ctx* c, int field, const double* par, double t) {
+  if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
+  double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (!par) par = zero;
+  const int E = c->E, NF = c->Nf, NP = c->NP, P = c->P;
+  auto rule = tri_rule(2 * P);
+  std::vector<double> phi(rule.size() * NP);
+  for (size_t q = 0; q < rule.size(); q++) basis(2, P, rule[q].x, &phi[q * NP], nullptr);
+  std::vector<double> U(c->ndof, 0.0);
+  for (int e = 0; e < E; e++) {
+    const double* X = &c->xyz[(size_t)e * 6];
+    double det = (X[2] - X[0]) * (X[5] - X[1]) - (X[4] - X[0]) * (X[3] - X[1]);
+    double ad = std::fabs(det);
+    double* Ue = &U[(size_t)e * NF * NP];
+    for (size_t q = 0; q < rule.size(); q++) {
+      const double u = rule[q].x[0], v = rule[q].x[1];
+      double x[3] = {X[0] + (X[2] - X[0]) * u + (X[4] - X[0]) * v, X[1] + (X[3] - X[1]) * u + (X[5] - X[1]) * v, 0.0};
+      double val[8];
+      if (exact_field(field, par, c->lp, x, t, val)) return fail(c, DG_ERR_ARG, "unknown field %d", field);
+      const double jw = ad * rule[q].w;
+      for (int f = 0; f < NF; f++)
+        for (int k = 0; k < NP; k++) Ue[k + NP * f] += val[f] * phi[q * NP + k] * jw;
+    }
+    const double inv = 1. / ad;
+    for (int i = 0; i < NF * NP; i++) Ue[i] *= inv;
+  }
+  return dg_set_state(c, U.data(), U.size());
+}
+
+dg_status dg_l2_error(dg_ctx* c, int field, const double* par, double t, double* err) {
+  if (!ready(c) || !err) return fail(c, DG_ERR_ARG, "bad argument");
+  double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (!par) par = zero;
+  const int E = c->E, NF = c->Nf, NP = c->NP, P = c->P;
+  std::vector<double> U(c->ndof);
+  dg_status s = dg_get_state(c, U.data(), U.size());
+  if (s) return s;
+  auto rule = tri_rule(std::min(2 * (P + 1), 10));
+  std::vector<double> phi(rule.size() * NP);
+  for (size_t q = 0; q < rule.size(); q++) basis(2, P, rule[q].x, &phi[q * NP], nullptr);
+  std::vector<double> acc(NF, 0.0);
+  for (int e = 0; e < E; e++) {
+    const double* X = &c->xyz[(size_t)e * 6];
+    double ad = std::fabs((X[2] - X[0]) * (X[5] - X[1]) - (X[4] - X[0]) * (X[3] - X[1]));
+    const double* Ue = &U[(size_t)e * NF * NP];
+    for (size_t q = 0; q < rule.size(); q++) {
+      const double u = rule[q].x[0], v = rule[q].x[1];
+      double x[3] = {X[0] + (X[2] - X[0]) * u + (X[4] - X[0]) * v, X[1] + (X[3] - X[1]) * u + (X[5] - X[1]) * v, 0.0};
+      double val[8];
+      if (exact_field(field, par, c->lp, x, t, val)) return fail(c, DG_ERR_ARG, "unknown field %d", field);
+      for (int f = 0; f < NF; f++) {
+        double s2 = 0;
+        for (int k = 0; k < NP; k++) s2 += Ue[f * NP + k] * phi[q * NP + k];
+        const double d = s2 - val[f];
+        acc[f] += d * d * ad * rule[q].w;
+      }
+    }
+  }
+  for (int f = 0; f < NF; f++) err[f] = std::sqrt(acc[f]);
+  return DG_OK;
+}
+
+// ---------------------------------------------------------------------------
+static uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static double unit_rand(uint64_t x) { return (double)(splitmix64(x) >> 11) * (1.0 / 9007199254740992.0); }
+
+dg_status dg_mesh_tri_structured(int nx, int ny, double x0, double y0, double lx, double ly, int periodic_x,
+                                 int periodic_y, double jitter, uint64_t seed, const int32_t side_tags[4],
+                                 int32_t* vids, double* xyz, int32_t* bc) {
+  if (nx < 1 || ny < 1 || !(lx > 0) || !(ly > 0) || !vids || !xyz || jitter < 0 || jitter >= 0.25) return DG_ERR_ARG;
+  if ((periodic_x && nx < 3) || (periodic_y && ny < 3)) return DG_ERR_ARG;
+  const double hx = lx / nx, hy = ly / ny;
+  auto lattice = [&](int i, int j, double* p) {
+    p[0] = x0 + i * hx;
+    p[1] = y0 + j * hy;
+    if (i > 0 && i < nx && j > 0 && j < ny && jitter > 0) {
+      uint64_t vid = (uint64_t)j * (nx + 1) + i;
+      p[0] += (2.0 * unit_rand(seed ^ (2 * vid)) - 1.0) * jitter * hx;
+      p[1] += (2.0 * unit_rand(seed ^ (2 * vid + 1)) - 1.0) * jitter * hy;
+    }
+  };
+  auto topo = [&](int i, int j) {
+    if (periodic_x && i == nx) i = 0;
+    if (periodic_y && j == ny) j = 0;
+    return (int32_t)(j * (nx + 1) + i);
+  };
+  // side of a lattice edge: 0 bottom, 1 right, 2 top, 3 left, -1 interior
+  auto side = [&](int i0, int j0, int i1, int j1) {
+    if (j0 == 0 && j1 == 0) return 0;
+    if (i0 == nx && i1 == nx) return 1;
+    if (j0 == ny && j1 == ny) return 2;
+    if (i0 == 0 && i1 == 0) return 3;
+    return -1;
+  };
+  size_t e = 0;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      const int tri[2][3][2] = {{{i, j}, {i + 1, j}, {i + 1, j + 1}}, {{i, j}, {i + 1, j + 1}, {i, j + 1}}};
+      for (int t = 0; t < 2; t++, e++) {
+        for (int v = 0; v < 3; v++) {
+          vids[e * 3 + v] = topo(tri[t][v][0], tri[t][v][1]);
+          lattice(tri[t][v][0], tri[t][v][1], xyz + e * 6 + v * 2);
+        }
+        if (bc)
+          for (int l = 0; l < 3; l++) {
+            int sd = side(tri[t][l][0], tri[t][l][1], tri[t][(l + 1) % 3][0], tri[t][(l + 1) % 3][1]);
+            bool per = (sd == 0 || sd == 2) ? periodic_y : (sd == 1 || sd == 3) ? periodic_x : true;
+            bc[e * 3 + l] = (sd >= 0 && !per) ? (side_tags ? side_tags[sd] : DG_TAG_FARFIELD) : 0;
+          }
+      }
+    }
+  return DG_OK;
+}
+
+}  // extern "C"

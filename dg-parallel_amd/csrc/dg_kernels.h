@@ -1,0 +1,60 @@
+// Launch interface between the host context (dg_capi.cc) and the HIP kernels
+// (dg_kernels.hip).  Plain structs of device pointers; no torch types.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dg_laws.h"
+
+namespace dgp {
+
+// Element table of one order bucket of triangles, structure-of-arrays over
+// the bucket's E elements (element index fastest => coalesced per wavefront).
+struct TriElemTable {
+  const double* mg;     // [4][E]  mg[r*2+d] = |J| d xi_r / d x_d
+  const double* detj;   // [E]     |J| (the inverse mass of the orthonormal basis is 1/|J|)
+  const double* size;   // [E]     2 x inradius (DGCell.cc:1209-1229)
+  const int32_t* link;  // [3][E]  neighbour across local face l:
+                        //         >= 0: (nb << 3) | (same_orientation << 2) | nb_local_face
+                        //         <  0: -(boundary code)
+};
+
+// stage modes
+enum { MODE_RESIDUAL = -1 };
+
+struct StageArgs {
+  int E;
+  int P;
+  const void* Uin;       // stage input, SoA [NF*NP][E] (double or float)
+  const double* Un;      // step-start state, SoA, FP64
+  double* acc;           // RK4 accumulator, FP64
+  void* Uout;            // stage output (double or float)
+  double* R;             // MODE_RESIDUAL output, SoA FP64
+  const double* tab;     // table block of order P (FP64)
+  const float* tabf;     // same block in FP32
+  TriElemTable el;
+  LawParams law;
+  double dt;
+  int scheme;            // DG_RK_* or MODE_RESIDUAL
+  int stage;
+  double* resid_part;    // optional per-block partial sums of the step residual norm
+};
+
+// variant: 0 = FP64 throughout; 1 = FP32 arithmetic, FP64 input, FP32 output;
+// 2 = FP32 in/out; 3 = FP32 input, FP64 output
+int launch_tri_stage(const StageArgs& a, int law, int variant, hipStream_t s);
+int tri_stage_blocks(int E);
+
+struct DtArgs {
+  int E, P;
+  const double* U;       // SoA FP64
+  const double* vtx;     // [4][NP]: phi at the centroid slot 0 (phi_0 only) then 3 vertices
+  TriElemTable el;
+  LawParams law;
+  double cfl;
+  double* part;          // per-block minima
+};
+int launch_tri_dt(const DtArgs& a, int law, hipStream_t s);
+int tri_dt_blocks(int E);
+
+}  // namespace dgp

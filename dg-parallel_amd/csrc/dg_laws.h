@@ -1,0 +1,259 @@
+// Conservation-law arithmetic shared by the host classes and the HIP kernels.
+//
+// Each law is a set of static inline functions templated on the arithmetic
+// type T (double for the plain form, float for the reduced form).  No virtual
+// dispatch reaches the device: kernels are instantiated per law.
+//
+//   ScalarLawD   ScalarLaw::Fi / riemannSolver / boundaryFlux   (ScalarLaw.cc:29-88)
+//   EulerLawD    Euler2d::Fi (EulerLaw.cc:47-69), Euler2dRoe::riemannSolver
+//                (EulerLaw.cc:913-1010), wall ghost (EulerLaw.cc:280-291), far-field
+//                ghost (EulerLaw.cc:293-295), maximumEigenValue (EulerLaw.cc:1012-1034),
+//                isPhysical (EulerLaw.cc:1398-1404); local Lax-Friedrichs flux is new.
+//   MaxwellLawD  3D Maxwell, eps = mu = 1, upwind/central flux, PEC ghost (new).
+#pragma once
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define DG_HD __host__ __device__ __forceinline__
+#else
+#define DG_HD inline
+#endif
+
+#include <math.h>
+
+namespace dgp {
+
+enum { LAW_SCALAR = 0, LAW_EULER2D = 1, LAW_MAXWELL3D = 2 };
+enum { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2, FLUX_CENTRAL = 3 };
+// boundary codes stored in the element table
+enum { BC_FARFIELD = 1, BC_WALL = 2, BC_PEC = 3 };
+
+struct LawParams {
+  double gamma;
+  double vel[3];
+  double farfield[8];
+  int flux;
+  int pad;
+};
+
+template <typename T>
+DG_HD T dg_sqrt(T x) { return sqrt(x); }
+template <>
+DG_HD float dg_sqrt<float>(float x) { return sqrtf(x); }
+template <typename T>
+DG_HD T dg_abs(T x) { return x < T(0) ? -x : x; }
+
+// ---------------------------------------------------------------------------
+struct ScalarLawD {
+  static constexpr int NF = 1;
+  static constexpr int DIM = 2;
+  template <typename T>
+  DG_HD static void flux(const LawParams& L, const T* q, T F[][DIM]) {
+    F[0][0] = T(L.vel[0]) * q[0];
+    F[0][1] = T(L.vel[1]) * q[0];
+  }
+  template <typename T>
+  DG_HD static void riemann(const LawParams& L, const T* n, const T* uL, const T* uR, T* out) {
+    T vx = T(L.vel[0]), vy = T(L.vel[1]);
+    T vn = n[0] * vx + n[1] * vy;
+    T u = vn > T(0) ? uL[0] : uR[0];
+    out[0] = (vx * u) * n[0] + (vy * u) * n[1];
+  }
+  template <typename T>
+  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* q, T* out) {
+    (void)code;
+    T vn = n[0] * T(L.vel[0]) + n[1] * T(L.vel[1]);
+    out[0] = vn < T(0) ? vn * T(L.farfield[0]) : vn * q[0];
+  }
+  template <typename T>
+  DG_HD static bool physical(const LawParams&, const T*) { return true; }
+  template <typename T>
+  DG_HD static T maxev(const LawParams& L, const T*) {
+    return T(fabs(L.vel[0]) + fabs(L.vel[1]) + fabs(L.vel[2]));
+  }
+};
+
+// ---------------------------------------------------------------------------
+struct EulerLawD {
+  static constexpr int NF = 4;
+  static constexpr int DIM = 2;
+  template <typename T>
+  DG_HD static void flux(const LawParams& L, const T* Q, T F[][DIM]) {
+    T g = T(L.gamma);
+    T invrho = T(1) / Q[0];
+    T Q1 = Q[1], Q2 = Q[2];
+    T q12 = Q1 * Q2 * invrho, q11 = Q1 * Q1 * invrho, q22 = Q2 * Q2 * invrho;
+    T p = (g - T(1)) * (Q[3] - T(0.5) * (q11 + q22));
+    T qq = invrho * (Q[3] + p);
+    F[0][0] = Q1;      F[0][1] = Q2;
+    F[1][0] = q11 + p; F[1][1] = q12;
+    F[2][0] = q12;     F[2][1] = q22 + p;
+    F[3][0] = Q1 * qq; F[3][1] = Q2 * qq;
+  }
+  template <typename T>
+  DG_HD static T wave(const LawParams& L, const T* n, const T* u) {
+    T g = T(L.gamma);
+    T ri = T(1) / u[0];
+    T p = (g - T(1)) * (u[3] - T(0.5) * (u[1] * u[1] + u[2] * u[2]) * ri);
+    T c = dg_sqrt(g * p * ri);
+    return c + dg_abs((u[1] * n[0] + u[2] * n[1]) * ri);
+  }
+  template <typename T>
+  DG_HD static void lf(const LawParams& L, const T* n, const T* uL, const T* uR, T* out) {
+    T FL[4][2], FR[4][2];
+    flux(L, uL, FL);
+    flux(L, uR, FR);
+    T l1 = wave(L, n, uL), l2 = wave(L, n, uR);
+    T lam = l1 > l2 ? l1 : l2;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      T fn = (FL[i][0] + FR[i][0]) * n[0] + (FL[i][1] + FR[i][1]) * n[1];
+      out[i] = T(0.5) * (fn - lam * (uR[i] - uL[i]));
+    }
+  }
+  template <typename T>
+  DG_HD static void roe(const LawParams& L, const T* n, const T* u0, const T* u1, T* flux_out) {
+    T gamma = T(L.gamma);
+    T rho_0 = u0[0], rho_1 = u1[0];
+    T sqrt_rho_0 = dg_sqrt(rho_0), sqrt_rho_1 = dg_sqrt(rho_1);
+    T inv_rho_0 = T(1) / rho_0, inv_rho_1 = T(1) / rho_1;
+    T rho_A = sqrt_rho_0 * sqrt_rho_1;
+    T inv_denom = T(1) / (sqrt_rho_0 + sqrt_rho_1);
+    T u_0 = u0[1] * inv_rho_0, u_1 = u1[1] * inv_rho_1;
+    T u_A = (u_0 * sqrt_rho_0 + u_1 * sqrt_rho_1) * inv_denom;
+    T v_0 = u0[2] * inv_rho_0, v_1 = u1[2] * inv_rho_1;
+    T v_A = (v_0 * sqrt_rho_0 + v_1 * sqrt_rho_1) * inv_denom;
+    T p_0 = (gamma - T(1)) * (u0[3] - T(0.5) * (u0[1] * u0[1] + u0[2] * u0[2]) * inv_rho_0);
+    T p_1 = (gamma - T(1)) * (u1[3] - T(0.5) * (u1[1] * u1[1] + u1[2] * u1[2]) * inv_rho_1);
+    T H_0 = (u0[3] + p_0) * inv_rho_0, H_1 = (u1[3] + p_1) * inv_rho_1;
+    T H_A = (H_0 * sqrt_rho_0 + H_1 * sqrt_rho_1) * inv_denom;
+    T qq = H_A - T(0.5) * (u_A * u_A + v_A * v_A);
+    T roe[4] = {T(0), T(0), T(0), T(0)};
+    if (qq > T(0)) {
+      T c_A = dg_sqrt((gamma - T(1)) * qq);
+      T vn = u_A * n[0] + v_A * n[1];
+      T eg0 = dg_abs(vn), eg2 = dg_abs(vn + c_A), eg3 = dg_abs(vn - c_A);
+      T k = T(0.5) * rho_A / c_A;
+      T inv_c_A = T(1) / c_A;
+      T drho = rho_1 - rho_0, dp = p_1 - p_0, du = u_1 - u_0, dv = v_1 - v_0;
+      T ch0 = drho - dp * inv_c_A * inv_c_A;
+      T ch1 = n[1] * du - n[0] * dv;
+      T ch2 = n[0] * du + n[1] * dv + dp / rho_A * inv_c_A;
+      T ch3 = -n[0] * du - n[1] * dv + dp / rho_A * inv_c_A;
+      T a0 = eg0 * ch0, a1 = eg0 * ch1, a2 = eg2 * ch2, a3 = eg3 * ch3;
+      // sum_i eval_i charv_i evec_i[j] with the eigenvectors of EulerLaw.cc:955-976
+      roe[0] = a0 + a2 * k + a3 * k;
+      roe[1] = a0 * u_A + a1 * (n[1] * rho_A) + a2 * (k * (u_A + c_A * n[0])) + a3 * (k * (u_A - c_A * n[0]));
+      roe[2] = a0 * v_A + a1 * (-n[0] * rho_A) + a2 * (k * (v_A + c_A * n[1])) + a3 * (k * (v_A - c_A * n[1]));
+      roe[3] = a0 * (T(0.5) * (u_A * u_A + v_A * v_A)) + a1 * (rho_A * (u_A * n[1] - v_A * n[0])) +
+               a2 * (k * (H_A + c_A * vn)) + a3 * (k * (H_A - c_A * vn));
+    }
+    T f1[4][2], f2[4][2];
+    flux(L, u0, f1);
+    flux(L, u1, f2);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      flux_out[i] = T(0.5) * ((f1[i][0] + f2[i][0]) * n[0] + (f1[i][1] + f2[i][1]) * n[1] - roe[i]);
+  }
+  template <typename T>
+  DG_HD static void riemann(const LawParams& L, const T* n, const T* uL, const T* uR, T* out) {
+    if (L.flux == FLUX_ROE)
+      roe(L, n, uL, uR, out);
+    else
+      lf(L, n, uL, uR, out);
+  }
+  template <typename T>
+  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* Q, T* out) {
+    T g[4];
+    if (code == BC_WALL) {
+      T vn = -(Q[1] * n[0] + Q[2] * n[1]);
+      T vt = Q[1] * n[1] - Q[2] * n[0];
+      g[0] = Q[0];
+      g[1] = vn * n[0] + vt * n[1];
+      g[2] = vn * n[1] - vt * n[0];
+      g[3] = Q[3];
+    } else {
+      g[0] = T(L.farfield[0]);
+      g[1] = T(L.farfield[1]);
+      g[2] = T(L.farfield[2]);
+      g[3] = T(L.farfield[3]);
+    }
+    riemann(L, n, Q, g, out);
+  }
+  template <typename T>
+  DG_HD static bool physical(const LawParams& L, const T* Q) {
+    if (Q[0] < T(0)) return false;
+    if ((T(L.gamma) - T(1)) * (Q[3] - T(0.5) * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]) < T(0)) return false;
+    return true;
+  }
+  template <typename T>
+  DG_HD static T maxev(const LawParams& L, const T* Q) {
+    T rho_inv = T(1) / Q[0];
+    T v2 = (Q[1] * Q[1] + Q[2] * Q[2]) * rho_inv;
+    T p = (T(L.gamma) - T(1)) * (Q[3] - T(0.5) * v2);
+    v2 *= rho_inv;
+    T c = dg_sqrt(T(L.gamma) * p * rho_inv);
+    return c + dg_sqrt(v2);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Maxwell in 3D with eps = mu = 1: dE/dt = curl H, dH/dt = -curl E, written
+// as dq/dt + div F(q) = 0 with (F.n)_E = -n x H and (F.n)_H = n x E.
+struct MaxwellLawD {
+  static constexpr int NF = 6;
+  static constexpr int DIM = 3;
+  template <typename T>
+  DG_HD static void flux(const LawParams&, const T* Q, T F[][DIM]) {
+    const T *E = Q, *H = Q + 3;
+    F[0][0] = T(0);  F[0][1] = -H[2]; F[0][2] = H[1];
+    F[1][0] = H[2];  F[1][1] = T(0);  F[1][2] = -H[0];
+    F[2][0] = -H[1]; F[2][1] = H[0];  F[2][2] = T(0);
+    F[3][0] = T(0);  F[3][1] = E[2];  F[3][2] = -E[1];
+    F[4][0] = -E[2]; F[4][1] = T(0);  F[4][2] = E[0];
+    F[5][0] = E[1];  F[5][1] = -E[0]; F[5][2] = T(0);
+  }
+  template <typename T>
+  DG_HD static void riemann(const LawParams& L, const T* n, const T* uL, const T* uR, T* out) {
+    T Ea[3], Ha[3], dE[3], dH[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      Ea[i] = T(0.5) * (uL[i] + uR[i]);
+      Ha[i] = T(0.5) * (uL[3 + i] + uR[3 + i]);
+      dE[i] = uR[i] - uL[i];
+      dH[i] = uR[3 + i] - uL[3 + i];
+    }
+    out[0] = -(n[1] * Ha[2] - n[2] * Ha[1]);
+    out[1] = -(n[2] * Ha[0] - n[0] * Ha[2]);
+    out[2] = -(n[0] * Ha[1] - n[1] * Ha[0]);
+    out[3] = n[1] * Ea[2] - n[2] * Ea[1];
+    out[4] = n[2] * Ea[0] - n[0] * Ea[2];
+    out[5] = n[0] * Ea[1] - n[1] * Ea[0];
+    if (L.flux == FLUX_UPWIND) {
+      T ndE = n[0] * dE[0] + n[1] * dE[1] + n[2] * dE[2];
+      T ndH = n[0] * dH[0] + n[1] * dH[1] + n[2] * dH[2];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        out[i] += T(0.5) * (n[i] * ndE - dE[i]);
+        out[3 + i] += T(0.5) * (n[i] * ndH - dH[i]);
+      }
+    }
+  }
+  template <typename T>
+  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* Q, T* out) {
+    (void)code;
+    T g[6];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      g[i] = -Q[i];
+      g[3 + i] = Q[3 + i];
+    }
+    riemann(L, n, Q, g, out);
+  }
+  template <typename T>
+  DG_HD static bool physical(const LawParams&, const T*) { return true; }
+  template <typename T>
+  DG_HD static T maxev(const LawParams&, const T*) { return T(1); }
+};
+
+}  // namespace dgp

@@ -1,0 +1,234 @@
+"""Python binding of libdgpar.so (the C ABI declared in include/dg_capi.h).
+
+A thin ctypes layer: numpy arrays in, numpy arrays out.  The GPU path is the
+only path -- if the shared library is missing this module raises at import,
+and every compute call goes through the HIP kernels.
+
+The class names mirror the reference's driver surface:
+    DGAnalysis(mesh, law, order, ...)      DGAnalysis.h:110-135
+    .residual(t)                           TimeIntegrator::assembleVolume/Boundary
+    .advanceInTime(t, dt, scheme, nsteps)  TimeIntegrator::advanceInTime (TimeIntegrator.h:28)
+    .computeDt(cfl)                        DGCell::adaptTimeStep min (DGCell.cc:1155-1163)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdgpar.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libdgpar.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+DG_OK, DG_ERR_ARG, DG_ERR_OOM, DG_ERR_HIP, DG_ERR_NONFINITE, DG_ERR_NONPHYSICAL, DG_ERR_UNSUPPORTED = range(7)
+LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
+FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL = 0, 1, 2, 3
+RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
+PREC_FP64, PREC_FP32 = 0, 1
+TAG_FARFIELD, TAG_WALL = 30000, 40000
+FIELD_GAUSSIAN, FIELD_VORTEX, FIELD_SOD, FIELD_CAVITY = 0, 1, 2, 3
+
+STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_OOM", 3: "DG_ERR_HIP", 4: "DG_ERR_NONFINITE",
+                5: "DG_ERR_NONPHYSICAL", 6: "DG_ERR_UNSUPPORTED"}
+
+_p_i32 = ctypes.POINTER(ctypes.c_int32)
+_p_f64 = ctypes.POINTER(ctypes.c_double)
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("n_elems", ctypes.c_int), ("vids", _p_i32), ("xyz", _p_f64),
+                ("bc", _p_i32), ("order", _p_i32)]
+
+
+class LawDesc(ctypes.Structure):
+    _fields_ = [("law", ctypes.c_int), ("flux", ctypes.c_int), ("gamma", ctypes.c_double),
+                ("vel", ctypes.c_double * 3), ("farfield", ctypes.c_double * 8)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int), ("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("device", ctypes.c_int), ("reserved", ctypes.c_int * 8)]
+
+
+class DGError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_vp = ctypes.c_void_p
+_create = _sig("dg_create", ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(LawDesc),
+                                            ctypes.POINTER(Options), ctypes.POINTER(_vp)])
+_destroy = _sig("dg_destroy", ctypes.c_int, [_vp])
+_last_error = _sig("dg_last_error", ctypes.c_char_p, [_vp])
+_ndof = _sig("dg_ndof", ctypes.c_size_t, [_vp])
+_nfaces = _sig("dg_nfaces", ctypes.c_int, [_vp])
+_faces = _sig("dg_faces", ctypes.c_int, [_vp, _p_i32, ctypes.c_size_t])
+_set_state = _sig("dg_set_state", ctypes.c_int, [_vp, _p_f64, ctypes.c_size_t])
+_get_state = _sig("dg_get_state", ctypes.c_int, [_vp, _p_f64, ctypes.c_size_t])
+_residual = _sig("dg_residual", ctypes.c_int, [_vp, ctypes.c_double, _p_f64])
+_compute_dt = _sig("dg_compute_dt", ctypes.c_int, [_vp, ctypes.c_double, _p_f64])
+_advance = _sig("dg_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _p_f64])
+_time_advance = _sig("dg_time_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                       ctypes.POINTER(ctypes.c_float)])
+_project = _sig("dg_project", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double])
+_l2_error = _sig("dg_l2_error", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64])
+_mesh_tri = _sig("dg_mesh_tri_structured", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _p_i32, _p_i32, _p_f64, _p_i32])
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class Mesh:
+    """Element-wise mesh description handed to the library (dg_mesh_desc)."""
+
+    def __init__(self, dim, vids, xyz, bc=None, order=None):
+        self.dim = int(dim)
+        self.vids = np.ascontiguousarray(vids, dtype=np.int32)
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.bc = None if bc is None else np.ascontiguousarray(bc, dtype=np.int32)
+        self.order = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+        self.n_elems = self.vids.shape[0]
+
+    def desc(self):
+        return MeshDesc(self.dim, self.n_elems, _ptr(self.vids, _p_i32), _ptr(self.xyz, _p_f64),
+                        _ptr(self.bc, _p_i32), _ptr(self.order, _p_i32))
+
+
+def mesh_tri_structured(nx, ny, x0=0.0, y0=0.0, lx=1.0, ly=1.0, periodic_x=True, periodic_y=True,
+                        jitter=0.0, seed=0, side_tags=(0, 0, 0, 0)):
+    """Structured, optionally jittered, triangle mesh (dg_mesh_tri_structured)."""
+    E = 2 * nx * ny
+    vids = np.zeros((E, 3), np.int32)
+    xyz = np.zeros((E, 3, 2), np.float64)
+    bc = np.zeros((E, 3), np.int32)
+    tags = np.asarray(side_tags, np.int32)
+    st = _mesh_tri(nx, ny, x0, y0, lx, ly, int(periodic_x), int(periodic_y), jitter, seed, _ptr(tags, _p_i32),
+                   _ptr(vids, _p_i32), _ptr(xyz, _p_f64), _ptr(bc, _p_i32))
+    if st != DG_OK:
+        raise DGError(st, "dg_mesh_tri_structured rejected its arguments")
+    return Mesh(2, vids, xyz, bc)
+
+
+class Law:
+    """Conservation-law selection (dg_law_desc), mirroring the ConservationLaw subclasses."""
+
+    def __init__(self, law, flux, gamma=1.4, vel=(0.0, 0.0, 0.0), farfield=()):
+        self.law, self.flux, self.gamma = law, flux, gamma
+        self.vel = tuple(vel) + (0.0,) * (3 - len(vel))
+        ff = list(farfield) + [0.0] * (8 - len(farfield))
+        self.farfield = tuple(ff[:8])
+
+    def desc(self):
+        return LawDesc(self.law, self.flux, self.gamma, (ctypes.c_double * 3)(*self.vel),
+                       (ctypes.c_double * 8)(*self.farfield))
+
+    @property
+    def nfields(self):
+        return {LAW_SCALAR: 1, LAW_EULER2D: 4, LAW_MAXWELL3D: 6}[self.law]
+
+
+def ScalarLaw(vel):
+    return Law(LAW_SCALAR, FLUX_UPWIND, vel=vel)
+
+
+def EulerLaw(flux=FLUX_LF, gamma=1.4, farfield=(1.0, 0.0, 0.0, 2.5)):
+    return Law(LAW_EULER2D, flux, gamma=gamma, farfield=farfield)
+
+
+class DGAnalysis:
+    """One context: HBM-resident tables and state of a mesh + law + order."""
+
+    def __init__(self, mesh, law, order, precision=PREC_FP64, use_graph=True, device=0):
+        self.mesh, self.law, self.order = mesh, law, order
+        md, ld = mesh.desc(), law.desc()
+        opt = Options(order, precision, int(use_graph), device, (ctypes.c_int * 8)())
+        h = _vp()
+        st = _create(ctypes.byref(md), ctypes.byref(ld), ctypes.byref(opt), ctypes.byref(h))
+        self._h = h
+        if st != DG_OK:
+            msg = _last_error(h).decode() if h.value else "dg_create failed"
+            if h.value:
+                _destroy(h)
+            self._h = None
+            raise DGError(st, msg)
+        self.ndof = _ndof(h)
+
+    def _check(self, st):
+        if st != DG_OK:
+            raise DGError(st, _last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            _destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def faces(self):
+        n = _nfaces(self._h)
+        out = np.zeros((n, 4), np.int32)
+        self._check(_faces(self._h, _ptr(out, _p_i32), out.size))
+        return out
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).ravel()
+        self._check(_set_state(self._h, _ptr(U, _p_f64), U.size))
+
+    def get_state(self):
+        out = np.zeros(self.ndof, np.float64)
+        self._check(_get_state(self._h, _ptr(out, _p_f64), out.size))
+        return out
+
+    def residual(self, t=0.0):
+        out = np.zeros(self.ndof, np.float64)
+        self._check(_residual(self._h, t, _ptr(out, _p_f64)))
+        return out
+
+    def computeDt(self, cfl):
+        dt = ctypes.c_double()
+        self._check(_compute_dt(self._h, cfl, ctypes.byref(dt)))
+        return dt.value
+
+    def advanceInTime(self, t, dt, scheme=RK4, nsteps=1, want_resid=True):
+        r = ctypes.c_double()
+        self._check(_advance(self._h, scheme, t, dt, nsteps, ctypes.byref(r) if want_resid else None))
+        return r.value
+
+    def project(self, field, par=(), t=0.0):
+        par = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)
+        self._check(_project(self._h, field, _ptr(par, _p_f64), t))
+
+    def l2_error(self, field, par=(), t=0.0):
+        par = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)
+        err = np.zeros(self.law.nfields)
+        self._check(_l2_error(self._h, field, _ptr(par, _p_f64), t, _ptr(err, _p_f64)))
+        return err
+
+    def time_advance(self, scheme, dt, nsteps):
+        ms = ctypes.c_float()
+        self._check(_time_advance(self._h, scheme, dt, nsteps, ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,151 @@
+/*
+ * dg_capi.h -- C ABI of the MI355X (gfx950) DG residual / Runge-Kutta library.
+ *
+ * The drop-in boundary for DG-Parallel's explicit DG hot path: everything the
+ * reference reaches through DGAnalysis (DGAnalysis.h:110-135),
+ * TimeIntegrator::advanceInTime (TimeIntegrator.h:28) and the ConservationLaw
+ * plug-in surface (ConservationLaw.h:12-60) is served here by hand-written HIP
+ * kernels on one GPU.  Plain C types only: pointers, sizes, enums.
+ *
+ * Conventions
+ *  - Caller owns host buffers; the context owns all device memory and one HIP
+ *    stream.  Calls on one context are serialised on that stream; a context is
+ *    used from one host thread at a time.
+ *  - No C++ exception crosses the ABI and nothing calls exit(): every entry
+ *    point returns a dg_status; dg_last_error() gives the message.
+ *  - State layout at the boundary: element-major, field, mode -- U[e][f][k]
+ *    with Np(e) modes, elements in the caller's order; the reference's cell
+ *    coefficient block `i*FS+j` (DGCell.h:31) concatenated over cells.
+ */
+#ifndef DG_CAPI_H
+#define DG_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dg_ctx dg_ctx;
+
+typedef enum {
+  DG_OK = 0,
+  DG_ERR_ARG = 1,          /* bad argument / malformed mesh */
+  DG_ERR_OOM = 2,          /* device allocation failed (nothing leaked) */
+  DG_ERR_HIP = 3,          /* HIP runtime error */
+  DG_ERR_NONFINITE = 4,    /* NaN / Inf in a state or residual */
+  DG_ERR_NONPHYSICAL = 5,  /* negative density/pressure at a quadrature point */
+  DG_ERR_UNSUPPORTED = 6   /* law / order / element combination not built */
+} dg_status;
+
+enum { DG_LAW_SCALAR = 0, DG_LAW_EULER2D = 1, DG_LAW_MAXWELL3D = 2 };
+enum { DG_FLUX_UPWIND = 0, DG_FLUX_LF = 1, DG_FLUX_ROE = 2, DG_FLUX_CENTRAL = 3 };
+enum { DG_RK_FE = 0, DG_RK_TVD2 = 1, DG_RK_TVD3 = 2, DG_RK4 = 3 };
+enum { DG_PREC_FP64 = 0, DG_PREC_FP32 = 1 };
+/* boundary tags (the reference's geometric classification ids) */
+enum { DG_TAG_FARFIELD = 30000, DG_TAG_WALL = 40000 };
+
+/* Mesh handed over by the caller (replaces mDGMesh + mImportExport for the path:
+ * mMesh.h:52-139).  Faces are matched by their sorted topological vertex ids;
+ * elements that share a vertex id through a periodic image are neighbours
+ * (translation periodicity). */
+typedef struct {
+  int dim;                /* 2: triangles, 3: tetrahedra */
+  int n_elems;
+  const int32_t* vids;    /* [E][dim+1] topological vertex ids */
+  const double* xyz;      /* [E][dim+1][dim] corner coordinates of each element */
+  const int32_t* bc;      /* [E][dim+1] tag of faces with no neighbour, NULL: far field.
+                             Local face l of a triangle is edge (v_l, v_{l+1}) (Face.cc:114-148) */
+  const int32_t* order;   /* [E] polynomial order per element, NULL: options.order */
+} dg_mesh_desc;
+
+/* Law selection (replaces the ConservationLaw subclass instance; the host C++
+ * classes in dg-parallel_amd/host map themselves onto this). */
+typedef struct {
+  int law;                /* DG_LAW_* */
+  int flux;               /* DG_FLUX_*: Scalar upwind; Euler LF or Roe; Maxwell upwind or central */
+  double gamma;           /* Euler */
+  double vel[3];          /* ScalarLaw constant velocity field */
+  double farfield[8];     /* ghost state of far-field boundary faces */
+} dg_law_desc;
+
+typedef struct {
+  int order;              /* polynomial order when mesh.order is NULL */
+  int precision;          /* DG_PREC_FP64: plain form; DG_PREC_FP32: FP32 residual, FP64 stage accumulation */
+  int use_graph;          /* capture each dg_advance step in a hipGraph and replay it */
+  int device;             /* HIP device ordinal */
+  int reserved[8];
+} dg_options;
+
+/* DGAnalysis::DGAnalysis(mesh, law, limiter, order) (DGAnalysis.cc:133-302):
+ * builds element/face tables on the host in FP64 and uploads them. */
+dg_status dg_create(const dg_mesh_desc* mesh, const dg_law_desc* law, const dg_options* opt, dg_ctx** out);
+/* DGAnalysis::~DGAnalysis (DGAnalysis.cc:304-307) */
+dg_status dg_destroy(dg_ctx* ctx);
+const char* dg_last_error(const dg_ctx* ctx);
+/* sum_e Nf*Np(e): the length of every state / residual buffer */
+size_t dg_ndof(const dg_ctx* ctx);
+int dg_nfaces(const dg_ctx* ctx);
+/* faces as (elemL, localL, elemR, localR) in caller element numbering, elemL <
+ * elemR; boundary faces carry elemR = -1.  Order: sorted by sorted vertex ids. */
+dg_status dg_faces(const dg_ctx* ctx, int32_t* out4, size_t n);
+
+/* DGCell::theFieldsCoefficients write/read for all cells (DGCell.h:22-40) */
+dg_status dg_set_state(dg_ctx* ctx, const double* U, size_t n);
+dg_status dg_get_state(dg_ctx* ctx, double* U, size_t n);
+
+/* One residual evaluation R = int F.grad(phi) - oint Fhat phi for all cells:
+ * TimeIntegrator::assembleVolume + assembleBoundary (TimeIntegrator.cc:1332-1412)
+ * = DGCell::computeVolumeContribution (DGCell.cc:184-249) over cells plus
+ * DGBoundaryCell::computeBoundaryContributions (DGCell.cc:639-732) over faces.
+ * R_out: host buffer of dg_ndof() doubles (NULL: keep on device). */
+dg_status dg_residual(dg_ctx* ctx, double t, double* R_out);
+
+/* DGCell::adaptTimeStep min over cells (DGCell.cc:1155-1262, DGAnalysis.cc:510-540):
+ * dt = min_e CFL*size_e / ((2p_e+1) lambda_e), without the TEND snapping. */
+dg_status dg_compute_dt(dg_ctx* ctx, double cfl, double* dt_out);
+
+/* nsteps calls of TimeIntegrator::advanceInTime(t, dt) (TimeIntegrator.h:28) with
+ * the scheme chosen by rk_scheme (ForwardEuler, RungeKuttaTVD2, RungeKuttaTVD3,
+ * RungeKutta4: TimeIntegrator.cc:81-275,279-438,838-990,993-1048).  resid_out
+ * (optional) receives the last step's residual norm as the reference returns it. */
+dg_status dg_advance(dg_ctx* ctx, int rk_scheme, double t, double dt, int nsteps, double* resid_out);
+
+/* Device-resident variants for callers that keep state on the GPU (torch tensors):
+ * pointers are device pointers holding dg_ndof() doubles in the boundary layout. */
+dg_status dg_set_state_device(dg_ctx* ctx, const double* dU);
+dg_status dg_get_state_device(dg_ctx* ctx, double* dU);
+
+/* Timing helper for benchmarks: replays nsteps of dg_advance on the context's
+ * stream between two HIP events and returns the elapsed milliseconds. */
+dg_status dg_time_advance(dg_ctx* ctx, int rk_scheme, double dt, int nsteps, float* ms_out);
+
+/* Built-in exact fields: 0 Gaussian pulse carried by the scalar velocity on the
+ * periodic unit square; 1 isentropic vortex, par = {beta, xc, yc, u0, v0, box
+ * length}; 2 Sod tube (Examples.cc:62-83), par = {interface x}; 3 PEC cavity mode. */
+enum { DG_FIELD_GAUSSIAN = 0, DG_FIELD_VORTEX = 1, DG_FIELD_SOD = 2, DG_FIELD_CAVITY = 3 };
+
+/* DGCell::L2ProjInitial of an exact field at time t over all cells
+ * (DGCell.cc:1458-1483; volume rule of degree 2p); becomes the current state. */
+dg_status dg_project(dg_ctx* ctx, int field, const double* par, double t);
+
+/* Per-field L2 norm of (state - exact field at t), rule of degree 2(p+1)
+ * (DGCell::densityErrorL2, DGError.cc:805-831); err holds Nf doubles. */
+dg_status dg_l2_error(dg_ctx* ctx, int field, const double* par, double t, double* err);
+
+/* Structured triangle mesh (no counterpart in the reference, which reads Gmsh
+ * files, mImportExport.cc:77-170): nx*ny squares on [x0,x0+lx]x[y0,y0+ly], each
+ * split by the diagonal (i,j)-(i+1,j+1) into two counter-clockwise triangles,
+ * elements row-major.  Interior lattice vertices move by a uniform +-jitter*h in
+ * x and y drawn from splitmix64(seed ^ (2*vid)) and splitmix64(seed ^ (2*vid+1)).
+ * side_tags: boundary tags of the bottom, right, top, left sides when not periodic.
+ * Output buffers: vids [2*nx*ny*3], xyz [2*nx*ny*3*2], bc [2*nx*ny*3] (may be NULL). */
+dg_status dg_mesh_tri_structured(int nx, int ny, double x0, double y0, double lx, double ly,
+                                 int periodic_x, int periodic_y, double jitter, uint64_t seed,
+                                 const int32_t side_tags[4], int32_t* vids, double* xyz, int32_t* bc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

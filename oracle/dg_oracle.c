@@ -1,0 +1,1200 @@
+/*
+ * dg_oracle.c -- CPU FP64 restatement of DG-Parallel's explicit DG residual,
+ * Runge-Kutta stages and time-step rule.  TEST INFRASTRUCTURE: the parity
+ * checker and the CPU baseline; the product never links or calls it.
+ *
+ * What each part restates and what pins it (tests/test_oracle_*.py):
+ *   basis            ShapeFunctionList.cc:120-175,311-349,666-706 (G-S of Pascal
+ *                    monomials).  Own long-double modified Gram-Schmidt.  Pinned by
+ *                    orthonormality under an exact rule, positive leading
+ *                    coefficients, and the reference tables OrthoSFTri.h /
+ *                    TetOrtho.h read as text.
+ *   quadrature       Integrator.cc:31-101, GaussQuadratureTri.cc:138-167,
+ *                    GaussQuadratureTet.cc:349-380, GaussLegendre1D.cc:271-274.
+ *                    Rule data is the shared generated table; pinned by monomial
+ *                    exactness and by the reference tables read as text.
+ *   geometry         Mapping.cc:369-433,442-566,568-610,660-712.  Pinned by areas,
+ *                    unit outward normals, sum_faces n|f| = 0.
+ *   connectivity     mMesh.cc:321-378 + translation periodicity.  Pinned by
+ *                    Euler characteristic, 2 elements per interior face.
+ *   volume term      DGCell.cc:184-249.  face term DGCell.cc:639-732.
+ *                    Pinned by free-stream preservation, conservation, exact
+ *                    polynomial advection, and (where built) oracle/_ref.
+ *   fluxes           ScalarLaw.cc:29-97; EulerLaw.cc:47-69,182-296,913-1034; LF is
+ *                    new (closed form); Maxwell 3D is new (closed form).
+ *   RK stages        TimeIntegrator.cc:81-275 (RK4), 279-438 (TVD2), 838-990 (TVD3),
+ *                    993-1048 (FE).  Pinned by order of accuracy on linear ODE modes.
+ *   dt               DGCell.cc:1155-1262.  Closed form.
+ *
+ * Face accumulation is race-free: each face writes its own contribution block,
+ * then every element gathers its faces in local-face order.  Results do not
+ * depend on the thread count.
+ */
+#include "dg_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../dg-parallel_amd/csrc/dg_rules_data.h"
+
+#define MAXNF 6
+#define MAXNP 35
+#define MAXQ 64
+
+/* ------------------------------------------------------------------------- */
+/* basis: own Gram-Schmidt in long double                                     */
+/* ------------------------------------------------------------------------- */
+static const int tri_mono_exp[28][2] = {
+    {0, 0}, {1, 0}, {0, 1}, {2, 0}, {1, 1}, {0, 2}, {3, 0}, {2, 1}, {1, 2}, {0, 3},
+    {4, 0}, {3, 1}, {2, 2}, {1, 3}, {0, 4}, {5, 0}, {4, 1}, {3, 2}, {2, 3}, {1, 4},
+    {0, 5}, {6, 0}, {5, 1}, {4, 2}, {3, 3}, {2, 4}, {1, 5}, {0, 6}};
+/* ShapeFunctionList.cc:666-706 */
+static const int tet_mono_exp[35][3] = {
+    {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2},
+    {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0},
+    {2, 0, 1}, {1, 2, 0}, {0, 2, 1}, {1, 0, 2}, {0, 1, 2}, {1, 1, 1}, {4, 0, 0},
+    {0, 4, 0}, {0, 0, 4}, {3, 1, 0}, {3, 0, 1}, {1, 3, 0}, {0, 3, 1}, {1, 0, 3},
+    {0, 1, 3}, {2, 2, 0}, {2, 0, 2}, {0, 2, 2}, {2, 1, 1}, {1, 2, 1}, {1, 1, 2}};
+
+static long double lfact(int n) {
+  long double r = 1;
+  for (int i = 2; i <= n; i++) r *= i;
+  return r;
+}
+
+static double gs_tri[28][28];
+static double gs_tet[35][35];
+static int gs_ready = 0;
+static pthread_mutex_t gs_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static void gram_schmidt(int n, int dim, double* out /* n*n */) {
+  long double G[35][35];
+  long double C[35][35];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      if (dim == 2) {
+        int a = tri_mono_exp[i][0] + tri_mono_exp[j][0];
+        int b = tri_mono_exp[i][1] + tri_mono_exp[j][1];
+        G[i][j] = lfact(a) * lfact(b) / lfact(a + b + 2);
+      } else {
+        int a = tet_mono_exp[i][0] + tet_mono_exp[j][0];
+        int b = tet_mono_exp[i][1] + tet_mono_exp[j][1];
+        int c = tet_mono_exp[i][2] + tet_mono_exp[j][2];
+        G[i][j] = lfact(a) * lfact(b) * lfact(c) / lfact(a + b + c + 3);
+      }
+    }
+  /* modified Gram-Schmidt on coefficient vectors in the monomial basis */
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) C[i][j] = (i == j) ? 1.0L : 0.0L;
+    for (int k = 0; k < i; k++) {
+      long double ip = 0;
+      for (int a = 0; a <= i; a++)
+        for (int b = 0; b <= k; b++) ip += C[i][a] * G[a][b] * C[k][b];
+      for (int a = 0; a <= k; a++) C[i][a] -= ip * C[k][a];
+    }
+    long double nn = 0;
+    for (int a = 0; a <= i; a++)
+      for (int b = 0; b <= i; b++) nn += C[i][a] * G[a][b] * C[i][b];
+    long double s = 1.0L / sqrtl(nn);
+    for (int a = 0; a <= i; a++) C[i][a] *= s;
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) out[i * n + j] = (double)C[i][j];
+}
+
+static void gs_init(void) {
+  pthread_mutex_lock(&gs_lock);
+  if (!gs_ready) {
+    gram_schmidt(28, 2, &gs_tri[0][0]);
+    gram_schmidt(35, 3, &gs_tet[0][0]);
+    gs_ready = 1;
+  }
+  pthread_mutex_unlock(&gs_lock);
+}
+
+static int np_of(int dim, int p) {
+  return dim == 2 ? (p + 1) * (p + 2) / 2 : (p + 1) * (p + 2) * (p + 3) / 6;
+}
+
+static double ipow(double x, int n) {
+  double r = 1.0;
+  for (int i = 0; i < n; i++) r *= x;
+  return r;
+}
+
+/* phi_i = sum_j C[i][j] m_j (ShapeFunctionList.cc:120-126); gradients likewise
+ * (ShapeFunctionList.cc:152-175) */
+static void basis_eval(int dim, int p, const double* x, double* phi, double* dphi) {
+  int n = np_of(dim, p);
+  double m[35], dm[35][3];
+  for (int j = 0; j < n; j++) {
+    if (dim == 2) {
+      int a = tri_mono_exp[j][0], b = tri_mono_exp[j][1];
+      m[j] = ipow(x[0], a) * ipow(x[1], b);
+      dm[j][0] = a ? a * ipow(x[0], a - 1) * ipow(x[1], b) : 0.0;
+      dm[j][1] = b ? b * ipow(x[0], a) * ipow(x[1], b - 1) : 0.0;
+      dm[j][2] = 0.0;
+    } else {
+      int a = tet_mono_exp[j][0], b = tet_mono_exp[j][1], c = tet_mono_exp[j][2];
+      m[j] = ipow(x[0], a) * ipow(x[1], b) * ipow(x[2], c);
+      dm[j][0] = a ? a * ipow(x[0], a - 1) * ipow(x[1], b) * ipow(x[2], c) : 0.0;
+      dm[j][1] = b ? b * ipow(x[0], a) * ipow(x[1], b - 1) * ipow(x[2], c) : 0.0;
+      dm[j][2] = c ? c * ipow(x[0], a) * ipow(x[1], b) * ipow(x[2], c - 1) : 0.0;
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    const double* C = dim == 2 ? gs_tri[i] : gs_tet[i];
+    double r = 0, g0 = 0, g1 = 0, g2 = 0;
+    for (int j = 0; j <= i; j++) {
+      r += C[j] * m[j];
+      g0 += C[j] * dm[j][0];
+      g1 += C[j] * dm[j][1];
+      g2 += C[j] * dm[j][2];
+    }
+    if (phi) phi[i] = r;
+    if (dphi) {
+      dphi[i * dim + 0] = g0;
+      dphi[i * dim + 1] = g1;
+      if (dim == 3) dphi[i * dim + 2] = g2;
+    }
+  }
+}
+
+int or_basis(int dim, int p, const double* uvw, double* phi, double* dphi) {
+  if ((dim != 2 && dim != 3) || p < 0 || (dim == 2 && p > 6) || (dim == 3 && p > 4)) return -1;
+  gs_init();
+  basis_eval(dim, p, uvw, phi, dphi);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* quadrature selection (Integrator.cc:31-101)                                */
+/* ------------------------------------------------------------------------- */
+static int vol_rule(int dim, int degree, double (*pts)[4]) {
+  if (degree < 1) degree = 1;
+  if (dim == 2) {
+    if (degree > DG_TRI_RULE_MAXDEG) return -1;
+    int o = dg_tri_rule_off[degree], n = dg_tri_rule_npt[degree];
+    for (int q = 0; q < n; q++) {
+      pts[q][0] = dg_tri_rule_pts[o + q][0];
+      pts[q][1] = dg_tri_rule_pts[o + q][1];
+      pts[q][2] = 0.0;
+      pts[q][3] = 0.5 * dg_tri_rule_pts[o + q][2];
+    }
+    return n;
+  }
+  if (degree > DG_TET_RULE_MAXDEG) return -1;
+  int o = dg_tet_rule_off[degree], n = dg_tet_rule_npt[degree];
+  for (int q = 0; q < n; q++) {
+    pts[q][0] = dg_tet_rule_pts[o + q][0];
+    pts[q][1] = dg_tet_rule_pts[o + q][1];
+    pts[q][2] = dg_tet_rule_pts[o + q][2];
+    pts[q][3] = dg_tet_rule_pts[o + q][3] / 6.0;
+  }
+  return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* context                                                                    */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int eL, lL, eR, lR; /* eR = -1: boundary */
+  int tag;
+  int nq;
+  int off_pts;        /* into face point arrays */
+  double n[3];        /* unit outward normal of eL */
+} or_face;
+
+struct or_ctx {
+  int dim, nv, nfl, E, nf;
+  or_law law;
+  int32_t* order;
+  long* uoff;          /* offset of element e's block in U */
+  long ndof;
+  double* x0;          /* [E][3] vertex 0 */
+  double* Jm;          /* [E][9] d x / d xi  (column d: dx/dxi_d) */
+  double* Ji;          /* [E][9] d xi / d x */
+  double* detj;        /* [E] |det J| */
+  double* xyz;         /* copy of corner coordinates [E][nv][dim] */
+  int32_t* vids;
+  or_face* faces;
+  int* ef;             /* [E][nfl] face index; sign via eside */
+  int* eside;          /* [E][nfl] 0 = left side, 1 = right side */
+  /* face point data */
+  double* fx;          /* [P][3] physical point (left frame) */
+  double* fjw;         /* [P] |J_face| w */
+  double* fphiL;       /* [P][MAXNP] */
+  double* fphiR;       /* [P][MAXNP] */
+  int npts_face;
+  /* per-order volume data on the reference element */
+  int nqv[8];
+  double vq[8][MAXQ][4];
+  double vphi[8][MAXQ][MAXNP];
+  double vdphi[8][MAXQ][MAXNP][3];
+  /* scratch for face contributions */
+  double* fcontrib;    /* [F][2][Nf*MAXNP] */
+};
+
+static int nfields_of(const or_law* l) {
+  switch (l->law) {
+    case OR_LAW_SCALAR: return 1;
+    case OR_LAW_EULER2D: return 4;
+    case OR_LAW_MAXWELL3D: return 6;
+  }
+  return -1;
+}
+
+/* local face l of an element: 2D edge l = (v_l, v_{l+1}) (Face.cc:114-148);
+ * 3D face l = the three vertices other than vertex (3-l) ... fixed table */
+static const int tet_face_v[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+static void face_local_verts(int dim, int l, int* v) {
+  if (dim == 2) {
+    v[0] = l;
+    v[1] = (l + 1) % 3;
+  } else {
+    v[0] = tet_face_v[l][0];
+    v[1] = tet_face_v[l][1];
+    v[2] = tet_face_v[l][2];
+  }
+}
+
+typedef struct {
+  int32_t k[3];
+  int e, l;
+} fkey;
+
+static int fkey_cmp(const void* a, const void* b) {
+  const fkey* x = (const fkey*)a;
+  const fkey* y = (const fkey*)b;
+  for (int i = 0; i < 3; i++) {
+    if (x->k[i] < y->k[i]) return -1;
+    if (x->k[i] > y->k[i]) return 1;
+  }
+  if (x->e != y->e) return x->e < y->e ? -1 : 1;
+  return x->l - y->l;
+}
+
+static void sort3(int32_t* k, int n) {
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++)
+      if (k[j] < k[i]) {
+        int32_t t = k[i];
+        k[i] = k[j];
+        k[j] = t;
+      }
+}
+
+static void elem_map(const or_ctx* c, int e, const double* xi, double* x) {
+  for (int d = 0; d < c->dim; d++) {
+    double s = c->x0[e * 3 + d];
+    for (int k = 0; k < c->dim; k++) s += c->Jm[e * 9 + d * 3 + k] * xi[k];
+    x[d] = s;
+  }
+  if (c->dim == 2) x[2] = 0.0;
+}
+
+/* physical -> reference (Mapping.cc:383-433; affine, so the Newton iteration
+ * reduces to one linear solve) */
+static void elem_invert(const or_ctx* c, int e, const double* x, double* xi) {
+  double dx[3];
+  for (int d = 0; d < c->dim; d++) dx[d] = x[d] - c->x0[e * 3 + d];
+  for (int k = 0; k < c->dim; k++) {
+    double s = 0;
+    for (int d = 0; d < c->dim; d++) s += c->Ji[e * 9 + k * 3 + d] * dx[d];
+    xi[k] = s;
+  }
+  if (c->dim == 2) xi[2] = 0.0;
+}
+
+void or_destroy(or_ctx* c) {
+  if (!c) return;
+  free(c->order);
+  free(c->uoff);
+  free(c->x0);
+  free(c->Jm);
+  free(c->Ji);
+  free(c->detj);
+  free(c->xyz);
+  free(c->vids);
+  free(c->faces);
+  free(c->ef);
+  free(c->eside);
+  free(c->fx);
+  free(c->fjw);
+  free(c->fphiL);
+  free(c->fphiR);
+  free(c->fcontrib);
+  free(c);
+}
+
+or_ctx* or_create(const or_mesh* m, const or_law* law) {
+  gs_init();
+  if (!m || !law || (m->dim != 2 && m->dim != 3) || m->n_elems <= 0) return NULL;
+  int Nf = nfields_of(law);
+  if (Nf < 0) return NULL;
+  or_ctx* c = (or_ctx*)calloc(1, sizeof(or_ctx));
+  c->dim = m->dim;
+  c->nv = m->dim + 1;
+  c->nfl = m->dim + 1;
+  c->E = m->n_elems;
+  c->law = *law;
+  int E = c->E, nv = c->nv, dim = c->dim;
+  c->order = (int32_t*)malloc(sizeof(int32_t) * E);
+  c->uoff = (long*)malloc(sizeof(long) * (E + 1));
+  c->x0 = (double*)calloc(E * 3, sizeof(double));
+  c->Jm = (double*)calloc(E * 9, sizeof(double));
+  c->Ji = (double*)calloc(E * 9, sizeof(double));
+  c->detj = (double*)calloc(E, sizeof(double));
+  c->xyz = (double*)malloc(sizeof(double) * E * nv * dim);
+  c->vids = (int32_t*)malloc(sizeof(int32_t) * E * nv);
+  memcpy(c->xyz, m->xyz, sizeof(double) * E * nv * dim);
+  memcpy(c->vids, m->vids, sizeof(int32_t) * E * nv);
+  long off = 0;
+  for (int e = 0; e < E; e++) {
+    int p = m->order[e];
+    if (p < 0 || p > (dim == 2 ? 6 : 3)) {
+      or_destroy(c);
+      return NULL;
+    }
+    c->order[e] = p;
+    c->uoff[e] = off;
+    off += (long)Nf * np_of(dim, p);
+  }
+  c->uoff[E] = off;
+  c->ndof = off;
+  /* geometry: x = x0 + sum_k (x_k - x0) xi_k (Mapping.cc:153-170 shape functions
+   * N0 = 1-u-v(-w), N1 = u, N2 = v, N3 = w) */
+  for (int e = 0; e < E; e++) {
+    const double* X = m->xyz + (long)e * nv * dim;
+    for (int d = 0; d < dim; d++) c->x0[e * 3 + d] = X[d];
+    double J[3][3] = {{0}};
+    for (int d = 0; d < dim; d++)
+      for (int k = 0; k < dim; k++) J[d][k] = X[(k + 1) * dim + d] - X[d];
+    double det, Ji[3][3] = {{0}};
+    if (dim == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      Ji[0][0] = J[1][1] / det;
+      Ji[0][1] = -J[0][1] / det;
+      Ji[1][0] = -J[1][0] / det;
+      Ji[1][1] = J[0][0] / det;
+    } else {
+      det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+      Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+      Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+      Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+      Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+      Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+      Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+      Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+      Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    }
+    if (det == 0.0 || !isfinite(det)) {
+      or_destroy(c);
+      return NULL;
+    }
+    for (int d = 0; d < 3; d++)
+      for (int k = 0; k < 3; k++) {
+        c->Jm[e * 9 + d * 3 + k] = J[d][k];
+        c->Ji[e * 9 + d * 3 + k] = Ji[d][k];
+      }
+    c->detj[e] = fabs(det); /* ConstantMeshMapping: det < 0 -> -det (Mapping.cc:687-697) */
+  }
+  /* per-order volume tables (DGCell.cc:127-182: rule of degree 2p) */
+  for (int p = 0; p <= (dim == 2 ? 6 : 3); p++) {
+    int n = vol_rule(dim, 2 * p, c->vq[p]);
+    if (n < 0) n = 0;
+    c->nqv[p] = n;
+    for (int q = 0; q < n; q++) {
+      double dphi[MAXNP * 3];
+      basis_eval(dim, p, c->vq[p][q], c->vphi[p][q], dphi);
+      for (int k = 0; k < np_of(dim, p); k++)
+        for (int d = 0; d < dim; d++) c->vdphi[p][q][k][d] = dphi[k * dim + d];
+    }
+  }
+  /* connectivity: faces keyed by sorted topological vertex ids (mMesh.cc:321-378) */
+  int nfl = c->nfl, nfv = dim;
+  fkey* keys = (fkey*)malloc(sizeof(fkey) * (long)E * nfl);
+  for (int e = 0; e < E; e++)
+    for (int l = 0; l < nfl; l++) {
+      int lv[3];
+      face_local_verts(dim, l, lv);
+      fkey* k = &keys[e * nfl + l];
+      k->k[2] = -1;
+      for (int i = 0; i < nfv; i++) k->k[i] = m->vids[e * nv + lv[i]];
+      sort3(k->k, nfv);
+      k->e = e;
+      k->l = l;
+    }
+  qsort(keys, (size_t)E * nfl, sizeof(fkey), fkey_cmp);
+  c->faces = (or_face*)malloc(sizeof(or_face) * (long)E * nfl);
+  c->ef = (int*)malloc(sizeof(int) * E * nfl);
+  c->eside = (int*)malloc(sizeof(int) * E * nfl);
+  int nf = 0;
+  for (long i = 0; i < (long)E * nfl;) {
+    long j = i + 1;
+    while (j < (long)E * nfl && memcmp(keys[j].k, keys[i].k, sizeof(keys[i].k)) == 0) j++;
+    if (j - i > 2) { /* non-manifold */
+      free(keys);
+      or_destroy(c);
+      return NULL;
+    }
+    or_face* f = &c->faces[nf];
+    f->eL = keys[i].e;
+    f->lL = keys[i].l;
+    c->ef[f->eL * nfl + f->lL] = nf;
+    c->eside[f->eL * nfl + f->lL] = 0;
+    if (j - i == 2) {
+      f->eR = keys[i + 1].e;
+      f->lR = keys[i + 1].l;
+      c->ef[f->eR * nfl + f->lR] = nf;
+      c->eside[f->eR * nfl + f->lR] = 1;
+      f->tag = 0;
+    } else {
+      f->eR = -1;
+      f->lR = -1;
+      f->tag = m->bc ? m->bc[f->eL * nfl + f->lL] : 0;
+    }
+    nf++;
+    i = j;
+  }
+  free(keys);
+  c->nf = nf;
+  /* face quadrature (DGCell.cc:408-555): degree 2*max(pL,pR)+1 */
+  long tot = 0;
+  for (int fi = 0; fi < nf; fi++) {
+    or_face* f = &c->faces[fi];
+    int p = c->order[f->eL];
+    if (f->eR >= 0 && c->order[f->eR] > p) p = c->order[f->eR];
+    int deg = 2 * p + 1;
+    if (dim == 2) {
+      f->nq = (deg % 2) ? (deg + 1) / 2 : deg / 2 + 1; /* getNGQ1DPts */
+    } else {
+      if (deg > DG_TRI_RULE_MAXDEG) {
+        or_destroy(c);
+        return NULL;
+      }
+      f->nq = dg_tri_rule_npt[deg];
+    }
+    f->off_pts = (int)tot;
+    tot += f->nq;
+  }
+  c->npts_face = (int)tot;
+  c->fx = (double*)malloc(sizeof(double) * tot * 3);
+  c->fjw = (double*)malloc(sizeof(double) * tot);
+  c->fphiL = (double*)malloc(sizeof(double) * tot * MAXNP);
+  c->fphiR = (double*)calloc(tot * MAXNP, sizeof(double));
+  for (int fi = 0; fi < nf; fi++) {
+    or_face* f = &c->faces[fi];
+    int lv[3];
+    face_local_verts(dim, f->lL, lv);
+    const double* XL = m->xyz + (long)f->eL * nv * dim;
+    /* corners of the face in the left frame, in left local order */
+    double A[3][3] = {{0}};
+    int32_t fv[3];
+    for (int i = 0; i < nfv; i++) {
+      fv[i] = m->vids[f->eL * nv + lv[i]];
+      for (int d = 0; d < dim; d++) A[i][d] = XL[lv[i] * dim + d];
+    }
+    /* the same corners in the right element's frame (translation periodicity) */
+    double B[3][3] = {{0}};
+    if (f->eR >= 0) {
+      const double* XR = m->xyz + (long)f->eR * nv * dim;
+      for (int i = 0; i < nfv; i++) {
+        int found = -1;
+        for (int v = 0; v < nv; v++)
+          if (m->vids[f->eR * nv + v] == fv[i]) found = v;
+        for (int d = 0; d < dim; d++) B[i][d] = XR[found * dim + d];
+      }
+    }
+    /* opposite vertex of the left element */
+    int opp = -1;
+    for (int v = 0; v < nv; v++) {
+      int on = 0;
+      for (int i = 0; i < nfv; i++)
+        if (lv[i] == v) on = 1;
+      if (!on) opp = v;
+    }
+    double n[3] = {0, 0, 0}, area;
+    if (dim == 2) {
+      double tx = A[1][0] - A[0][0], ty = A[1][1] - A[0][1];
+      area = sqrt(tx * tx + ty * ty);
+      n[0] = ty / area;
+      n[1] = -tx / area;
+    } else {
+      double a[3], b[3];
+      for (int d = 0; d < 3; d++) {
+        a[d] = A[1][d] - A[0][d];
+        b[d] = A[2][d] - A[0][d];
+      }
+      n[0] = a[1] * b[2] - a[2] * b[1];
+      n[1] = a[2] * b[0] - a[0] * b[2];
+      n[2] = a[0] * b[1] - a[1] * b[0];
+      double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      area = nn; /* |J_face| of the reference-triangle map = 2*area */
+      for (int d = 0; d < 3; d++) n[d] /= nn;
+    }
+    double s = 0;
+    for (int d = 0; d < dim; d++) s += n[d] * (A[0][d] - XL[opp * dim + d]);
+    if (s < 0)
+      for (int d = 0; d < 3; d++) n[d] = -n[d];
+    for (int d = 0; d < 3; d++) f->n[d] = n[d];
+    for (int q = 0; q < f->nq; q++) {
+      double lam[3], w;
+      if (dim == 2) {
+        int o = dg_gl_off[f->nq - 1];
+        double sq = dg_gl_pts[o + q][0];
+        w = dg_gl_pts[o + q][1];
+        lam[0] = 0.5 * (1.0 + sq); /* node 0 <-> (1+u)/2 (Mapping.cc:568-579) */
+        lam[1] = 0.5 * (1.0 - sq);
+        c->fjw[f->off_pts + q] = 0.5 * area * w;
+      } else {
+        int deg = 2 * c->order[f->eL];
+        if (f->eR >= 0 && 2 * c->order[f->eR] > deg) deg = 2 * c->order[f->eR];
+        deg += 1;
+        int o = dg_tri_rule_off[deg];
+        double u = dg_tri_rule_pts[o + q][0], v = dg_tri_rule_pts[o + q][1];
+        w = 0.5 * dg_tri_rule_pts[o + q][2];
+        lam[0] = 1.0 - u - v;
+        lam[1] = u;
+        lam[2] = v;
+        c->fjw[f->off_pts + q] = area * w;
+      }
+      double xL[3] = {0, 0, 0}, xR[3] = {0, 0, 0}, xi[3];
+      for (int i = 0; i < nfv; i++)
+        for (int d = 0; d < dim; d++) {
+          xL[d] += lam[i] * A[i][d];
+          xR[d] += lam[i] * B[i][d];
+        }
+      for (int d = 0; d < 3; d++) c->fx[(f->off_pts + q) * 3 + d] = xL[d];
+      elem_invert(c, f->eL, xL, xi);
+      basis_eval(dim, c->order[f->eL], xi, c->fphiL + (long)(f->off_pts + q) * MAXNP, NULL);
+      if (f->eR >= 0) {
+        elem_invert(c, f->eR, xR, xi);
+        basis_eval(dim, c->order[f->eR], xi, c->fphiR + (long)(f->off_pts + q) * MAXNP, NULL);
+      }
+    }
+  }
+  c->fcontrib = (double*)malloc(sizeof(double) * (long)nf * 2 * Nf * MAXNP);
+  return c;
+}
+
+int or_nfields(const or_ctx* c) { return nfields_of(&c->law); }
+long or_ndof(const or_ctx* c) { return c->ndof; }
+int or_nfaces(const or_ctx* c) { return c->nf; }
+void or_faces(const or_ctx* c, int32_t* out) {
+  for (int i = 0; i < c->nf; i++) {
+    out[4 * i + 0] = c->faces[i].eL;
+    out[4 * i + 1] = c->faces[i].lL;
+    out[4 * i + 2] = c->faces[i].eR;
+    out[4 * i + 3] = c->faces[i].lR;
+  }
+}
+void or_detj(const or_ctx* c, double* out) { memcpy(out, c->detj, sizeof(double) * c->E); }
+
+/* ------------------------------------------------------------------------- */
+/* laws                                                                       */
+/* ------------------------------------------------------------------------- */
+/* physical flux F[f][d] (EulerLaw.cc:47-69, ScalarLaw.cc:29-37) */
+static void law_flux(const or_law* L, const double* Q, double F[MAXNF][3]) {
+  if (L->law == OR_LAW_SCALAR) {
+    for (int d = 0; d < 3; d++) F[0][d] = L->vel[d] * Q[0];
+  } else if (L->law == OR_LAW_EULER2D) {
+    double g = L->gamma;
+    double invrho = 1. / Q[0];
+    double Q1 = Q[1], Q2 = Q[2];
+    double q12 = Q1 * Q2 * invrho, q11 = Q1 * Q1 * invrho, q22 = Q2 * Q2 * invrho;
+    double p = (g - 1.) * (Q[3] - 0.5 * (q11 + q22));
+    double qq = invrho * (Q[3] + p);
+    F[0][0] = Q1; F[0][1] = Q2; F[0][2] = 0;
+    F[1][0] = q11 + p; F[1][1] = q12; F[1][2] = 0;
+    F[2][0] = q12; F[2][1] = q22 + p; F[2][2] = 0;
+    F[3][0] = Q1 * qq; F[3][1] = Q2 * qq; F[3][2] = 0;
+  } else {
+    /* Maxwell, eps = mu = 1: dE/dt = curl H, dH/dt = -curl E.  With
+     * dq/dt + div F = 0: F_E[i][j] = -eps_ijk H_k, F_H[i][j] = eps_ijk E_k. */
+    const double *E = Q, *H = Q + 3;
+    F[0][0] = 0;     F[0][1] = -H[2]; F[0][2] = H[1];
+    F[1][0] = H[2];  F[1][1] = 0;     F[1][2] = -H[0];
+    F[2][0] = -H[1]; F[2][1] = H[0];  F[2][2] = 0;
+    F[3][0] = 0;     F[3][1] = E[2];  F[3][2] = -E[1];
+    F[4][0] = -E[2]; F[4][1] = 0;     F[4][2] = E[0];
+    F[5][0] = E[1];  F[5][1] = -E[0]; F[5][2] = 0;
+  }
+}
+
+static int law_physical(const or_law* L, const double* Q) {
+  if (L->law != OR_LAW_EULER2D) return 1;
+  /* EulerLaw.cc:1398-1404 */
+  if (Q[0] < 0) return 0;
+  if ((L->gamma - 1.) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]) < 0.0) return 0;
+  return 1;
+}
+
+static void euler_lf(const or_law* L, const double* n, const double* uL, const double* uR, double* out) {
+  double g = L->gamma, FL[MAXNF][3], FR[MAXNF][3];
+  law_flux(L, uL, FL);
+  law_flux(L, uR, FR);
+  /* wave speed as Euler2d::maximumEigenValue(n, uL, uR) (EulerLaw.cc:1022-1034) */
+  double ri = 1. / uL[0];
+  double p = (g - 1.) * (uL[3] - 0.5 * (uL[1] * uL[1] + uL[2] * uL[2]) * ri);
+  double cs = sqrt(g * p * ri);
+  double l1 = cs + fabs((uL[1] * n[0] + uL[2] * n[1]) * ri);
+  ri = 1. / uR[0];
+  p = (g - 1.) * (uR[3] - 0.5 * (uR[1] * uR[1] + uR[2] * uR[2]) * ri);
+  cs = sqrt(g * p * ri);
+  double l2 = cs + fabs((uR[1] * n[0] + uR[2] * n[1]) * ri);
+  double lam = l1 > l2 ? l1 : l2;
+  for (int i = 0; i < 4; i++) {
+    double fn = (FL[i][0] + FR[i][0]) * n[0] + (FL[i][1] + FR[i][1]) * n[1];
+    out[i] = 0.5 * (fn - lam * (uR[i] - uL[i]));
+  }
+}
+
+/* Euler2dRoe::riemannSolver (EulerLaw.cc:913-1010) */
+static void euler_roe(const or_law* L, const double* n, const double* u0, const double* u1, double* flux) {
+  double gamma = L->gamma;
+  double rho_0 = u0[0], rho_1 = u1[0];
+  double sqrt_rho_0 = sqrt(rho_0), sqrt_rho_1 = sqrt(rho_1);
+  double inv_rho_0 = 1. / rho_0, inv_rho_1 = 1. / rho_1;
+  double rho_A = sqrt_rho_0 * sqrt_rho_1;
+  double inv_denom = 1. / (sqrt_rho_0 + sqrt_rho_1);
+  double u_0 = u0[1] * inv_rho_0, u_1 = u1[1] * inv_rho_1;
+  double u_A = (u_0 * sqrt_rho_0 + u_1 * sqrt_rho_1) * inv_denom;
+  double v_0 = u0[2] * inv_rho_0, v_1 = u1[2] * inv_rho_1;
+  double v_A = (v_0 * sqrt_rho_0 + v_1 * sqrt_rho_1) * inv_denom;
+  double p_0 = (gamma - 1.) * (u0[3] - 0.5 * (u0[1] * u0[1] + u0[2] * u0[2]) * inv_rho_0);
+  double p_1 = (gamma - 1.) * (u1[3] - 0.5 * (u1[1] * u1[1] + u1[2] * u1[2]) * inv_rho_1);
+  double H_0 = (u0[3] + p_0) * inv_rho_0, H_1 = (u1[3] + p_1) * inv_rho_1;
+  double H_A = (H_0 * sqrt_rho_0 + H_1 * sqrt_rho_1) * inv_denom;
+  double qq = H_A - 0.5 * (u_A * u_A + v_A * v_A);
+  double roe[4] = {0, 0, 0, 0};
+  if (qq > 0.0) {
+    double c_A = sqrt((gamma - 1.0) * qq);
+    double vn = u_A * n[0] + v_A * n[1];
+    double ev[4][4], eg[4];
+    eg[0] = fabs(vn);
+    eg[1] = fabs(vn);
+    eg[2] = fabs(vn + c_A);
+    eg[3] = fabs(vn - c_A);
+    ev[0][0] = 1.; ev[0][1] = u_A; ev[0][2] = v_A; ev[0][3] = 0.5 * (u_A * u_A + v_A * v_A);
+    ev[1][0] = 0.; ev[1][1] = n[1] * rho_A; ev[1][2] = -n[0] * rho_A;
+    ev[1][3] = rho_A * (u_A * n[1] - v_A * n[0]);
+    double k = 0.5 * rho_A / c_A;
+    ev[2][0] = k; ev[2][1] = k * (u_A + c_A * n[0]); ev[2][2] = k * (v_A + c_A * n[1]);
+    ev[2][3] = k * (H_A + c_A * vn);
+    ev[3][0] = k; ev[3][1] = k * (u_A - c_A * n[0]); ev[3][2] = k * (v_A - c_A * n[1]);
+    ev[3][3] = k * (H_A - c_A * vn);
+    double inv_c_A = 1. / c_A;
+    double drho = rho_1 - rho_0, dp = p_1 - p_0, du = u_1 - u_0, dv = v_1 - v_0;
+    double ch[4];
+    ch[0] = drho - dp * inv_c_A * inv_c_A;
+    ch[1] = n[1] * du - n[0] * dv;
+    ch[2] = n[0] * du + n[1] * dv + dp / rho_A * inv_c_A;
+    ch[3] = -n[0] * du - n[1] * dv + dp / rho_A * inv_c_A;
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) roe[j] += eg[i] * ch[i] * ev[i][j];
+  }
+  double f1[MAXNF][3], f2[MAXNF][3];
+  law_flux(L, u0, f1);
+  law_flux(L, u1, f2);
+  for (int i = 0; i < 4; i++)
+    flux[i] = 0.5 * ((f1[i][0] + f2[i][0]) * n[0] + (f1[i][1] + f2[i][1]) * n[1] +
+                     (f1[i][2] + f2[i][2]) * n[2] - roe[i]);
+}
+
+/* Maxwell upwind / central flux for eps = mu = 1 (Hesthaven & Warburton 2008,
+ * sec. 6.5): F*.n = Fc.n - alpha/2 * jump terms */
+static void maxwell_flux(const or_law* L, const double* n, const double* uL, const double* uR, double* out) {
+  const double *EL = uL, *HL = uL + 3, *ER = uR, *HR = uR + 3;
+  double Ea[3], Ha[3], dE[3], dH[3];
+  for (int i = 0; i < 3; i++) {
+    Ea[i] = 0.5 * (EL[i] + ER[i]);
+    Ha[i] = 0.5 * (HL[i] + HR[i]);
+    dE[i] = ER[i] - EL[i];
+    dH[i] = HR[i] - HL[i];
+  }
+  /* central part: (F(q).n)_E = -n x H, (F(q).n)_H = n x E */
+  double nxH[3] = {n[1] * Ha[2] - n[2] * Ha[1], n[2] * Ha[0] - n[0] * Ha[2], n[0] * Ha[1] - n[1] * Ha[0]};
+  double nxE[3] = {n[1] * Ea[2] - n[2] * Ea[1], n[2] * Ea[0] - n[0] * Ea[2], n[0] * Ea[1] - n[1] * Ea[0]};
+  for (int i = 0; i < 3; i++) {
+    out[i] = -nxH[i];
+    out[3 + i] = nxE[i];
+  }
+  if (L->flux == OR_FLUX_UPWIND) {
+    /* dissipation: -1/2 n x (n x [E]) on E, -1/2 n x (n x [H]) on H, with
+     * n x (n x a) = n (n.a) - a */
+    double ndE = n[0] * dE[0] + n[1] * dE[1] + n[2] * dE[2];
+    double ndH = n[0] * dH[0] + n[1] * dH[1] + n[2] * dH[2];
+    for (int i = 0; i < 3; i++) {
+      out[i] -= -0.5 * (n[i] * ndE - dE[i]);
+      out[3 + i] -= -0.5 * (n[i] * ndH - dH[i]);
+    }
+  }
+}
+
+/* numerical flux Fhat(uL, uR).n (riemannSolver) */
+static void law_riemann(const or_law* L, const double* n, const double* uL, const double* uR, double* out) {
+  if (L->law == OR_LAW_SCALAR) {
+    /* ScalarLaw::riemannSolver (ScalarLaw.cc:69-88) */
+    double vn = n[0] * L->vel[0] + n[1] * L->vel[1] + n[2] * L->vel[2];
+    double u = vn > 0 ? uL[0] : uR[0];
+    out[0] = (L->vel[0] * u) * n[0] + (L->vel[1] * u) * n[1] + (L->vel[2] * u) * n[2];
+  } else if (L->law == OR_LAW_EULER2D) {
+    if (L->flux == OR_FLUX_ROE)
+      euler_roe(L, n, uL, uR, out);
+    else
+      euler_lf(L, n, uL, uR, out);
+  } else {
+    maxwell_flux(L, n, uL, uR, out);
+  }
+}
+
+/* boundary flux for faces without a neighbour */
+static void law_boundary(const or_law* L, int tag, const double* n, const double* x, double T, const double* Q,
+                         double* out) {
+  int Nf = nfields_of(L);
+  double ghost[MAXNF];
+  if (L->law == OR_LAW_SCALAR) {
+    /* ScalarLaw::boundaryFlux (ScalarLaw.cc:39-59) */
+    double vn = n[0] * L->vel[0] + n[1] * L->vel[1] + n[2] * L->vel[2];
+    out[0] = vn < 0 ? vn * L->farfield[0] : vn * Q[0];
+    return;
+  }
+  if (L->law == OR_LAW_EULER2D) {
+    if (tag == 40000) {
+      /* reflective wall ghost (EulerLaw.cc:280-291) */
+      double vn = -(Q[1] * n[0] + Q[2] * n[1]);
+      double vt = Q[1] * n[1] - Q[2] * n[0];
+      ghost[0] = Q[0];
+      ghost[1] = vn * n[0] + vt * n[1];
+      ghost[2] = vn * n[1] - vt * n[0];
+      ghost[3] = Q[3];
+    } else {
+      /* far-field ghost (EulerLaw.cc:293-295) */
+      (void)x;
+      (void)T;
+      for (int i = 0; i < Nf; i++) ghost[i] = L->farfield[i];
+    }
+    law_riemann(L, n, Q, ghost, out);
+    return;
+  }
+  /* Maxwell PEC: E_ghost = -E, H_ghost = H (new; no reference counterpart) */
+  for (int i = 0; i < 3; i++) {
+    ghost[i] = -Q[i];
+    ghost[3 + i] = Q[3 + i];
+  }
+  law_riemann(L, n, Q, ghost, out);
+}
+
+/* ------------------------------------------------------------------------- */
+/* residual                                                                   */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  or_ctx* c;
+  const double* U;
+  double* R;
+  double t;
+  int lo, hi, phase;
+  int nonphys;
+} work_t;
+
+static void volume_elem(or_ctx* c, const double* Ue, double* Re, int e, double t) {
+  (void)t;
+  int dim = c->dim, p = c->order[e], Np = np_of(dim, p), Nf = nfields_of(&c->law);
+  const double* Ji = c->Ji + e * 9;
+  double dj = c->detj[e];
+  for (int i = 0; i < Nf * Np; i++) Re[i] = 0.0;
+  for (int q = 0; q < c->nqv[p]; q++) {
+    double val[MAXNF];
+    /* DGCell::interpolate (DGCell.h:217-227) */
+    for (int f = 0; f < Nf; f++) {
+      double s = 0;
+      for (int k = 0; k < Np; k++) s += Ue[f * Np + k] * c->vphi[p][q][k];
+      val[f] = s;
+    }
+    double F[MAXNF][3];
+    law_flux(&c->law, val, F);
+    double jw = dj * c->vq[p][q][3];
+    /* grads = (grad_ref phi . J^-1) |J| w (DGCell.cc:156-158, FunctionSpace.cc:7-16) */
+    for (int k = 0; k < Np; k++) {
+      double g[3] = {0, 0, 0};
+      for (int d = 0; d < dim; d++) {
+        double s = 0;
+        for (int r = 0; r < dim; r++) s += c->vdphi[p][q][k][r] * Ji[r * 3 + d];
+        g[d] = s * jw;
+      }
+      for (int f = 0; f < Nf; f++) {
+        double s = 0;
+        for (int d = 0; d < dim; d++) s += F[f][d] * g[d];
+        Re[f * Np + k] += s; /* source terms are zero for every law here */
+      }
+    }
+  }
+}
+
+static int face_flux(or_ctx* c, const double* U, int fi, double t) {
+  or_face* f = &c->faces[fi];
+  int dim = c->dim, Nf = nfields_of(&c->law);
+  int pL = c->order[f->eL], NpL = np_of(dim, pL);
+  int pR = f->eR >= 0 ? c->order[f->eR] : 0, NpR = f->eR >= 0 ? np_of(dim, pR) : 0;
+  const double* UL = U + c->uoff[f->eL];
+  const double* UR = f->eR >= 0 ? U + c->uoff[f->eR] : NULL;
+  double* cL = c->fcontrib + (long)fi * 2 * Nf * MAXNP;
+  double* cR = cL + Nf * MAXNP;
+  int nonphys = 0;
+  for (int i = 0; i < Nf * MAXNP; i++) cL[i] = cR[i] = 0.0;
+  for (int q = 0; q < f->nq; q++) {
+    int P = f->off_pts + q;
+    const double* phL = c->fphiL + (long)P * MAXNP;
+    const double* phR = c->fphiR + (long)P * MAXNP;
+    double uL[MAXNF], uR[MAXNF], Fh[MAXNF];
+    for (int k = 0; k < Nf; k++) {
+      double s = 0;
+      for (int j = 0; j < NpL; j++) s += UL[k * NpL + j] * phL[j];
+      uL[k] = s;
+    }
+    if (!law_physical(&c->law, uL)) nonphys |= 1;
+    if (UR) {
+      for (int k = 0; k < Nf; k++) {
+        double s = 0;
+        for (int j = 0; j < NpR; j++) s += UR[k * NpR + j] * phR[j];
+        uR[k] = s;
+      }
+      if (!law_physical(&c->law, uR)) nonphys |= 2;
+      law_riemann(&c->law, f->n, uL, uR, Fh);
+    } else {
+      law_boundary(&c->law, f->tag, f->n, c->fx + (long)P * 3, t, uL, Fh);
+    }
+    double jw = c->fjw[P];
+    /* DGCell.cc:709-728 */
+    for (int k = 0; k < Nf; k++) {
+      for (int j = 0; j < NpL; j++) cL[k * NpL + j] -= Fh[k] * phL[j] * jw;
+      for (int j = 0; j < NpR; j++) cR[k * NpR + j] += Fh[k] * phR[j] * jw;
+    }
+  }
+  return nonphys;
+}
+
+static void* worker(void* arg) {
+  work_t* w = (work_t*)arg;
+  or_ctx* c = w->c;
+  int Nf = nfields_of(&c->law);
+  if (w->phase == 0) {
+    for (int e = w->lo; e < w->hi; e++) volume_elem(c, w->U + c->uoff[e], w->R + c->uoff[e], e, w->t);
+  } else if (w->phase == 1) {
+    for (int fi = w->lo; fi < w->hi; fi++) w->nonphys |= face_flux(c, w->U, fi, w->t);
+  } else {
+    for (int e = w->lo; e < w->hi; e++) {
+      int Np = np_of(c->dim, c->order[e]);
+      double* Re = w->R + c->uoff[e];
+      for (int l = 0; l < c->nfl; l++) {
+        int fi = c->ef[e * c->nfl + l];
+        const double* src = c->fcontrib + (long)fi * 2 * Nf * MAXNP + c->eside[e * c->nfl + l] * Nf * MAXNP;
+        for (int i = 0; i < Nf * Np; i++) Re[i] += src[i];
+      }
+    }
+  }
+  return NULL;
+}
+
+static int run_phase(or_ctx* c, const double* U, double* R, double t, int phase, int n, int nthreads) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  work_t w[256];
+  pthread_t th[256];
+  for (int i = 0; i < nthreads; i++) {
+    w[i].c = c;
+    w[i].U = U;
+    w[i].R = R;
+    w[i].t = t;
+    w[i].phase = phase;
+    w[i].lo = (int)((long)n * i / nthreads);
+    w[i].hi = (int)((long)n * (i + 1) / nthreads);
+    w[i].nonphys = 0;
+  }
+  for (int i = 1; i < nthreads; i++) pthread_create(&th[i], NULL, worker, &w[i]);
+  worker(&w[0]);
+  int np = w[0].nonphys;
+  for (int i = 1; i < nthreads; i++) {
+    pthread_join(th[i], NULL);
+    np |= w[i].nonphys;
+  }
+  return np;
+}
+
+int or_residual(or_ctx* c, const double* U, double t, double* R, int nthreads) {
+  run_phase(c, U, R, t, 0, c->E, nthreads);
+  int np = run_phase(c, U, R, t, 1, c->nf, nthreads);
+  run_phase(c, U, R, t, 2, c->E, nthreads);
+  return np;
+}
+
+/* ------------------------------------------------------------------------- */
+/* RK stages                                                                  */
+/* ------------------------------------------------------------------------- */
+double or_step(or_ctx* c, int scheme, double* U, double t, double dt, int nthreads) {
+  long n = c->ndof;
+  double* R = (double*)malloc(sizeof(double) * n);
+  double* s0 = (double*)malloc(sizeof(double) * n);
+  double* acc = (double*)malloc(sizeof(double) * n);
+  double* dtj = (double*)malloc(sizeof(double) * n);
+  for (int e = 0; e < c->E; e++)
+    for (long i = c->uoff[e]; i < c->uoff[e + 1]; i++) dtj[i] = dt / c->detj[e];
+  double resid = 0.0;
+  memcpy(s0, U, sizeof(double) * n);
+  if (scheme == OR_RK4) {
+    /* TimeIntegrator.cc:81-275 */
+    or_residual(c, U, t, R, nthreads);
+    for (long i = 0; i < n; i++) {
+      acc[i] = R[i] * dtj[i];
+      U[i] += 0.5 * R[i] * dtj[i];
+    }
+    or_residual(c, U, t + dt * 0.5, R, nthreads);
+    for (long i = 0; i < n; i++) {
+      acc[i] += 2. * dtj[i] * R[i];
+      U[i] = s0[i] + 0.5 * dtj[i] * R[i];
+    }
+    or_residual(c, U, t + dt * 0.5, R, nthreads);
+    for (long i = 0; i < n; i++) {
+      acc[i] += 2. * dtj[i] * R[i];
+      U[i] = s0[i] + dtj[i] * R[i];
+    }
+    or_residual(c, U, t + dt, R, nthreads);
+    for (long i = 0; i < n; i++) {
+      acc[i] += dtj[i] * R[i];
+      U[i] = s0[i] + acc[i] / 6.0;
+      resid += acc[i] * acc[i] / 36.0;
+    }
+  } else if (scheme == OR_RK_TVD3) {
+    /* TimeIntegrator.cc:838-990 */
+    or_residual(c, U, t, R, nthreads);
+    for (long i = 0; i < n; i++) U[i] += R[i] * dtj[i];
+    or_residual(c, U, t + dt, R, nthreads);
+    for (long i = 0; i < n; i++) U[i] = 0.25 * (U[i] + R[i] * dtj[i] + 3. * s0[i]);
+    or_residual(c, U, t + dt * 0.5, R, nthreads);
+    for (long i = 0; i < n; i++) {
+      U[i] = (2.0 * (U[i] + R[i] * dtj[i]) + s0[i]) / 3.0;
+      double d = U[i] - s0[i];
+      resid += d * d;
+    }
+  } else if (scheme == OR_RK_TVD2) {
+    /* TimeIntegrator.cc:279-438 with parallel.cc:78-186 */
+    or_residual(c, U, t, R, nthreads);
+    for (long i = 0; i < n; i++) U[i] += R[i] * dtj[i];
+    or_residual(c, U, t + dt, R, nthreads);
+    for (long i = 0; i < n; i++) {
+      U[i] = (U[i] + R[i] * dtj[i] + s0[i]) * 0.5;
+      double d = U[i] - s0[i];
+      resid += d * d;
+    }
+  } else {
+    /* TimeIntegrator.cc:993-1048 */
+    or_residual(c, U, t, R, nthreads);
+    for (long i = 0; i < n; i++) {
+      double d = U[i];
+      U[i] += R[i] * dtj[i];
+      d -= U[i];
+      resid += d * d;
+    }
+  }
+  free(R);
+  free(s0);
+  free(acc);
+  free(dtj);
+  return resid;
+}
+
+/* ------------------------------------------------------------------------- */
+/* time step (DGCell.cc:1155-1262)                                            */
+/* ------------------------------------------------------------------------- */
+static double law_maxev(const or_law* L, const double* Q) {
+  if (L->law == OR_LAW_SCALAR) return fabs(L->vel[0]) + fabs(L->vel[1]) + fabs(L->vel[2]);
+  if (L->law == OR_LAW_EULER2D) {
+    /* EulerLaw.cc:1012-1020 */
+    double rho_inv = 1. / Q[0];
+    double v2 = (Q[1] * Q[1] + Q[2] * Q[2]) * rho_inv;
+    double p = (L->gamma - 1.) * (Q[3] - 0.5 * v2);
+    v2 *= rho_inv;
+    double cs = sqrt(L->gamma * p * rho_inv);
+    return cs + sqrt(v2);
+  }
+  return 1.0; /* Maxwell with eps = mu = 1: light speed */
+}
+
+double or_dt(or_ctx* c, const double* U, double cfl) {
+  int dim = c->dim, Nf = nfields_of(&c->law);
+  double dt = 1e300;
+  for (int e = 0; e < c->E; e++) {
+    int p = c->order[e], Np = np_of(dim, p);
+    const double* Ue = U + c->uoff[e];
+    const double* X = c->xyz + (long)e * c->nv * dim;
+    double size;
+    if (dim == 2) {
+      double a = hypot(X[2] - X[0], X[3] - X[1]);
+      double b = hypot(X[4] - X[2], X[5] - X[3]);
+      double cc = hypot(X[0] - X[4], X[1] - X[5]);
+      double hp = 0.5 * (a + b + cc);
+      size = sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
+    } else {
+      static const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+      size = 1e300;
+      for (int i = 0; i < 6; i++) {
+        double s = 0;
+        for (int d = 0; d < 3; d++) {
+          double v = X[ed[i][1] * 3 + d] - X[ed[i][0] * 3 + d];
+          s += v * v;
+        }
+        s = sqrt(s);
+        if (s < size) size = s;
+      }
+    }
+    double mean[MAXNF], phi[MAXNP];
+    double phi0;
+    {
+      double z[3] = {0, 0, 0};
+      basis_eval(dim, 0, z, &phi0, NULL);
+    }
+    for (int f = 0; f < Nf; f++) mean[f] = Ue[f * Np] * phi0; /* computeMean (DGCell.cc:1130-1153) */
+    double a = law_maxev(&c->law, mean);
+    if (dim == 2) {
+      static const double corner[3][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+      for (int v = 0; v < 3; v++) {
+        double val[MAXNF];
+        basis_eval(dim, p, corner[v], phi, NULL);
+        for (int f = 0; f < Nf; f++) {
+          double s = 0;
+          for (int k = 0; k < Np; k++) s += Ue[f * Np + k] * phi[k];
+          val[f] = s;
+        }
+        double b = law_maxev(&c->law, val);
+        a = b > a ? b : a;
+      }
+    }
+    double d = size * cfl / (a * (2. * p + 1.0));
+    if (dt > d) dt = d;
+  }
+  return dt;
+}
+
+/* ------------------------------------------------------------------------- */
+/* exact fields, projection, errors                                           */
+/* ------------------------------------------------------------------------- */
+static double wrap(double x, double L) {
+  double r = fmod(x, L);
+  if (r < 0) r += L;
+  return r;
+}
+
+int or_exact(int field, const double* par, const or_law* law, const double* x, double t, double* val) {
+  if (field == 0) {
+    /* Gaussian exp(-((x-.5)^2+(y-.5)^2)/0.01) advected with vel, unit square periodic */
+    double cx = wrap(x[0] - law->vel[0] * t, 1.0), cy = wrap(x[1] - law->vel[1] * t, 1.0);
+    double dx = cx - 0.5, dy = cy - 0.5;
+    val[0] = exp(-(dx * dx + dy * dy) / 0.01);
+    return 0;
+  }
+  if (field == 1) {
+    double g = law->gamma, beta = par[0], L = par[5];
+    double u0 = par[3], v0 = par[4];
+    double xs = wrap(x[0] - u0 * t - par[1] + 0.5 * L, L) - 0.5 * L;
+    double ys = wrap(x[1] - v0 * t - par[2] + 0.5 * L, L) - 0.5 * L;
+    double r2 = xs * xs + ys * ys;
+    double pi = 3.14159265358979323846;
+    double T = 1.0 - (g - 1.0) * beta * beta * exp(1.0 - r2) / (8.0 * g * pi * pi);
+    double rho = pow(T, 1.0 / (g - 1.0));
+    double du = beta / (2.0 * pi) * exp(0.5 * (1.0 - r2));
+    double u = u0 - du * ys, v = v0 + du * xs;
+    double p = pow(rho, g);
+    val[0] = rho;
+    val[1] = rho * u;
+    val[2] = rho * v;
+    val[3] = p / (g - 1.0) + 0.5 * rho * (u * u + v * v);
+    return 0;
+  }
+  if (field == 2) {
+    /* Sod (Examples.cc:62-83) */
+    double g = law->gamma;
+    int left = x[0] < par[0];
+    double rho = left ? 1.0 : 0.125, p = left ? 1.0 : 0.1;
+    val[0] = rho;
+    val[1] = 0.0;
+    val[2] = 0.0;
+    val[3] = p / (g - 1.0);
+    return 0;
+  }
+  if (field == 3) {
+    /* PEC cavity mode (1,1,1) on the unit cube, E amplitude (1,-1,0) */
+    double pi = 3.14159265358979323846, w = pi * sqrt(3.0);
+    double cx = cos(pi * x[0]), sx = sin(pi * x[0]);
+    double cy = cos(pi * x[1]), sy = sin(pi * x[1]);
+    double cz = cos(pi * x[2]), sz = sin(pi * x[2]);
+    double ct = cos(w * t), st = sin(w * t);
+    double ax = 1.0, ay = -1.0, az = 0.0;
+    val[0] = ax * cx * sy * sz * ct;
+    val[1] = ay * sx * cy * sz * ct;
+    val[2] = az * sx * sy * cz * ct;
+    /* H from dH/dt = -curl E:  H = -(1/w) curl(E_space) sin(wt) */
+    double cEx = pi * (az * sx * cy * cz - ay * sx * cy * cz);
+    double cEy = pi * (ax * cx * sy * cz - az * cx * sy * cz);
+    double cEz = pi * (ay * cx * cy * sz - ax * cx * cy * sz);
+    val[3] = -cEx / w * st;
+    val[4] = -cEy / w * st;
+    val[5] = -cEz / w * st;
+    return 0;
+  }
+  return -1;
+}
+
+int or_project(or_ctx* c, int field, const double* par, double t, double* U) {
+  int dim = c->dim, Nf = nfields_of(&c->law);
+  for (int e = 0; e < c->E; e++) {
+    int p = c->order[e], Np = np_of(dim, p);
+    double* Ue = U + c->uoff[e];
+    for (int i = 0; i < Nf * Np; i++) Ue[i] = 0.0;
+    for (int q = 0; q < c->nqv[p]; q++) {
+      double x[3], val[MAXNF];
+      elem_map(c, e, c->vq[p][q], x);
+      if (or_exact(field, par, &c->law, x, t, val)) return -1;
+      double jw = c->detj[e] * c->vq[p][q][3];
+      for (int f = 0; f < Nf; f++)
+        for (int k = 0; k < Np; k++) Ue[k + Np * f] += val[f] * c->vphi[p][q][k] * jw;
+    }
+    double inv = 1. / c->detj[e];
+    for (int i = 0; i < Nf * Np; i++) Ue[i] *= inv;
+  }
+  return 0;
+}
+
+int or_l2_error(or_ctx* c, const double* U, int field, const double* par, double t, double* err) {
+  int dim = c->dim, Nf = nfields_of(&c->law);
+  for (int f = 0; f < Nf; f++) err[f] = 0.0;
+  double (*pts)[4] = (double (*)[4])malloc(sizeof(double) * 4 * 64);
+  for (int e = 0; e < c->E; e++) {
+    int p = c->order[e], Np = np_of(dim, p);
+    int deg = 2 * (p + 1);
+    if (deg > (dim == 2 ? DG_TRI_RULE_MAXDEG : DG_TET_RULE_MAXDEG)) deg = dim == 2 ? DG_TRI_RULE_MAXDEG : DG_TET_RULE_MAXDEG;
+    int nq = vol_rule(dim, deg, pts);
+    const double* Ue = U + c->uoff[e];
+    for (int q = 0; q < nq; q++) {
+      double x[3], val[MAXNF], phi[MAXNP];
+      elem_map(c, e, pts[q], x);
+      or_exact(field, par, &c->law, x, t, val);
+      basis_eval(dim, p, pts[q], phi, NULL);
+      double jw = c->detj[e] * pts[q][3];
+      for (int f = 0; f < Nf; f++) {
+        double s = 0;
+        for (int k = 0; k < Np; k++) s += Ue[f * Np + k] * phi[k];
+        double d = s - val[f];
+        err[f] += d * d * jw;
+      }
+    }
+  }
+  free(pts);
+  for (int f = 0; f < Nf; f++) err[f] = sqrt(err[f]);
+  return 0;
+}

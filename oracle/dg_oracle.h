@@ -1,0 +1,85 @@
+/*
+ * dg_oracle -- CPU FP64 restatement of DG-Parallel's explicit DG residual and
+ * Runge-Kutta step.  TEST INFRASTRUCTURE ONLY: it is the parity checker and the
+ * reported CPU baseline.  Nothing in the product links, loads or calls it; only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * Every routine names the reference file:line it restates.  Parity status of
+ * each function (what pins it) is listed in the header of dg_oracle.c.
+ *
+ * State layout (both input and output): element-major, then field, then mode,
+ * i.e. U[e][f][k] with Np(e) modes -- the reference's per-cell coefficient
+ * block `i*FS+j` (DGCell.h:31), concatenated over elements in input order.
+ */
+#ifndef DG_ORACLE_H
+#define DG_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_LAW_SCALAR = 0, OR_LAW_EULER2D = 1, OR_LAW_MAXWELL3D = 2 };
+enum { OR_FLUX_UPWIND = 0, OR_FLUX_LF = 1, OR_FLUX_ROE = 2, OR_FLUX_CENTRAL = 3 };
+enum { OR_RK_FE = 0, OR_RK_TVD2 = 1, OR_RK_TVD3 = 2, OR_RK4 = 3 };
+
+typedef struct {
+  int dim;                 /* 2: triangles, 3: tetrahedra */
+  int n_elems;
+  const int32_t* vids;     /* [E][dim+1] topological vertex ids (periodic images share an id) */
+  const double* xyz;       /* [E][dim+1][dim] corner coordinates of each element */
+  const int32_t* bc;       /* [E][dim+1] tag for faces without a neighbour (NULL: 0) */
+  const int32_t* order;    /* [E] polynomial order of each element */
+} or_mesh;
+
+typedef struct {
+  int law;                 /* OR_LAW_* */
+  int flux;                /* OR_FLUX_* */
+  double gamma;            /* Euler */
+  double vel[3];           /* scalar advection velocity */
+  double farfield[8];      /* ghost state for boundary faces with tag != 40000 */
+} or_law;
+
+typedef struct or_ctx or_ctx;
+
+/* builds connectivity, geometry and quadrature tables; NULL on bad input */
+or_ctx* or_create(const or_mesh* m, const or_law* law);
+void or_destroy(or_ctx* c);
+int or_nfields(const or_ctx* c);
+long or_ndof(const or_ctx* c);                 /* sum_e Nf*Np(e) */
+int or_nfaces(const or_ctx* c);
+/* faces as (elemL, localL, elemR, localR); elemR = -1 on boundary faces;
+ * elemL < elemR for interior faces */
+void or_faces(const or_ctx* c, int32_t* out4);
+
+/* R = int F.grad(phi) + S phi - oint Fhat phi  (DGCell.cc:184-249, 639-732) */
+int or_residual(or_ctx* c, const double* U, double t, double* R, int nthreads);
+/* one step of the chosen scheme in place; returns the residual norm the
+ * reference's advanceInTime returns (TimeIntegrator.cc:81-275, 279-438, 838-990, 993-1048) */
+double or_step(or_ctx* c, int scheme, double* U, double t, double dt, int nthreads);
+/* global min of CFL*size/((2p+1) lambda) (DGCell.cc:1155-1262) */
+double or_dt(or_ctx* c, const double* U, double cfl);
+/* per-element detJ (reference-to-physical) and mean of field f */
+void or_detj(const or_ctx* c, double* out);
+
+/* basis values/gradients of order p on the reference simplex, from this
+ * file's own long-double Gram-Schmidt (ShapeFunctionList.cc:120-175) */
+int or_basis(int dim, int p, const double* uvw, double* phi, double* dphi);
+
+/* L2 projection of a built-in exact field (DGCell.cc:1458-1483):
+ * field 0: Gaussian pulse translated by vel*t (periodic unit square)
+ * field 1: isentropic vortex (gamma, beta=par[0], centre par[1],par[2], free stream
+ *          (1, par[3], par[4], 1), periodic box [0,par[5]]^2) at time t
+ * field 2: Sod tube, interface x = par[0]
+ * field 3: PEC cavity mode (1,1,1) with E amplitude (1,-1,0) at time t */
+int or_project(or_ctx* c, int field, const double* par, double t, double* U);
+/* evaluates the same exact fields at a point */
+int or_exact(int field, const double* par, const or_law* law, const double* x, double t, double* val);
+/* per-field L2 error of U against an exact field, rule of degree 2(p+1)
+ * (DGError.cc:805-831) */
+int or_l2_error(or_ctx* c, const double* U, int field, const double* par, double t, double* err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

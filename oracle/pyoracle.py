@@ -1,0 +1,140 @@
+"""ctypes access to the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY:
+used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+
+LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
+FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL = 0, 1, 2, 3
+RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
+
+_p_i32 = ctypes.POINTER(ctypes.c_int32)
+_p_f64 = ctypes.POINTER(ctypes.c_double)
+
+
+class _Mesh(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("n_elems", ctypes.c_int), ("vids", _p_i32), ("xyz", _p_f64),
+                ("bc", _p_i32), ("order", _p_i32)]
+
+
+class _Law(ctypes.Structure):
+    _fields_ = [("law", ctypes.c_int), ("flux", ctypes.c_int), ("gamma", ctypes.c_double),
+                ("vel", ctypes.c_double * 3), ("farfield", ctypes.c_double * 8)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB):
+        build()
+    lib = ctypes.CDLL(LIB)
+    vp = ctypes.c_void_p
+    lib.or_create.restype = vp
+    lib.or_create.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_Law)]
+    lib.or_destroy.argtypes = [vp]
+    lib.or_nfields.argtypes = [vp]
+    lib.or_ndof.restype = ctypes.c_long
+    lib.or_ndof.argtypes = [vp]
+    lib.or_nfaces.argtypes = [vp]
+    lib.or_faces.argtypes = [vp, _p_i32]
+    lib.or_residual.argtypes = [vp, _p_f64, ctypes.c_double, _p_f64, ctypes.c_int]
+    lib.or_step.restype = ctypes.c_double
+    lib.or_step.argtypes = [vp, ctypes.c_int, _p_f64, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    lib.or_dt.restype = ctypes.c_double
+    lib.or_dt.argtypes = [vp, _p_f64, ctypes.c_double]
+    lib.or_detj.argtypes = [vp, _p_f64]
+    lib.or_basis.argtypes = [ctypes.c_int, ctypes.c_int, _p_f64, _p_f64, _p_f64]
+    lib.or_project.argtypes = [vp, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64]
+    lib.or_exact.argtypes = [ctypes.c_int, _p_f64, ctypes.POINTER(_Law), _p_f64, ctypes.c_double, _p_f64]
+    lib.or_l2_error.argtypes = [vp, _p_f64, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64]
+    return lib
+
+
+_lib = _load()
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def basis(dim, p, x):
+    n = (p + 1) * (p + 2) // 2 if dim == 2 else (p + 1) * (p + 2) * (p + 3) // 6
+    x = np.ascontiguousarray(np.resize(np.asarray(x, np.float64), 3))
+    phi = np.zeros(n)
+    dphi = np.zeros(n * dim)
+    if _lib.or_basis(dim, p, _p(x, _p_f64), _p(phi, _p_f64), _p(dphi, _p_f64)) != 0:
+        raise ValueError("bad basis request")
+    return phi, dphi.reshape(n, dim)
+
+
+class Oracle:
+    def __init__(self, dim, vids, xyz, order, law, flux, gamma=1.4, vel=(0, 0, 0), farfield=(), bc=None):
+        self.vids = np.ascontiguousarray(vids, np.int32)
+        self.xyz = np.ascontiguousarray(xyz, np.float64)
+        E = self.vids.shape[0]
+        if np.isscalar(order):
+            order = np.full(E, order, np.int32)
+        self.order = np.ascontiguousarray(order, np.int32)
+        self.bc = None if bc is None else np.ascontiguousarray(bc, np.int32)
+        m = _Mesh(dim, E, _p(self.vids, _p_i32), _p(self.xyz, _p_f64), _p(self.bc, _p_i32), _p(self.order, _p_i32))
+        ff = list(farfield) + [0.0] * (8 - len(farfield))
+        v = list(vel) + [0.0] * (3 - len(vel))
+        self._law = _Law(law, flux, gamma, (ctypes.c_double * 3)(*v), (ctypes.c_double * 8)(*ff[:8]))
+        self._h = _lib.or_create(ctypes.byref(m), ctypes.byref(self._law))
+        if not self._h:
+            raise ValueError("oracle rejected the mesh")
+        self.ndof = _lib.or_ndof(self._h)
+        self.nfields = _lib.or_nfields(self._h)
+        self.E = E
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.or_destroy(self._h)
+            self._h = None
+
+    def faces(self):
+        n = _lib.or_nfaces(self._h)
+        out = np.zeros((n, 4), np.int32)
+        _lib.or_faces(self._h, _p(out, _p_i32))
+        return out
+
+    def detj(self):
+        out = np.zeros(self.E)
+        _lib.or_detj(self._h, _p(out, _p_f64))
+        return out
+
+    def residual(self, U, t=0.0, nthreads=1):
+        U = np.ascontiguousarray(U, np.float64)
+        R = np.zeros(self.ndof)
+        flag = _lib.or_residual(self._h, _p(U, _p_f64), t, _p(R, _p_f64), nthreads)
+        return R, flag
+
+    def step(self, U, scheme, t, dt, nthreads=1):
+        U = np.array(U, np.float64, copy=True)
+        r = _lib.or_step(self._h, scheme, _p(U, _p_f64), t, dt, nthreads)
+        return U, r
+
+    def dt(self, U, cfl):
+        U = np.ascontiguousarray(U, np.float64)
+        return _lib.or_dt(self._h, _p(U, _p_f64), cfl)
+
+    def project(self, field, par=(), t=0.0):
+        par = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)
+        U = np.zeros(self.ndof)
+        if _lib.or_project(self._h, field, _p(par, _p_f64), t, _p(U, _p_f64)) != 0:
+            raise ValueError("bad field")
+        return U
+
+    def l2_error(self, U, field, par=(), t=0.0):
+        par = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)
+        U = np.ascontiguousarray(U, np.float64)
+        err = np.zeros(self.nfields)
+        _lib.or_l2_error(self._h, _p(U, _p_f64), field, _p(par, _p_f64), t, _p(err, _p_f64))
+        return err

@@ -1,0 +1,101 @@
+"""Test-side helpers: an independent numpy mesh generator (same specification as
+dg_mesh_tri_structured, written separately so the product generator is checked
+against it) and state builders."""
+import math
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REFERENCE = "/root/reference"
+
+M64 = (1 << 64) - 1
+
+
+def splitmix64(x):
+    z = (x + 0x9E3779B97F4A7C15) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def unit_rand(x):
+    return (splitmix64(x & M64) >> 11) * (1.0 / 9007199254740992.0)
+
+
+def mesh_tri(nx, ny, x0=0.0, y0=0.0, lx=1.0, ly=1.0, periodic_x=True, periodic_y=True, jitter=0.0, seed=0,
+             side_tags=(0, 0, 0, 0)):
+    hx, hy = lx / nx, ly / ny
+
+    def lattice(i, j):
+        x, y = x0 + i * hx, y0 + j * hy
+        if 0 < i < nx and 0 < j < ny and jitter > 0:
+            vid = j * (nx + 1) + i
+            x += (2.0 * unit_rand(seed ^ (2 * vid)) - 1.0) * jitter * hx
+            y += (2.0 * unit_rand(seed ^ (2 * vid + 1)) - 1.0) * jitter * hy
+        return x, y
+
+    def topo(i, j):
+        if periodic_x and i == nx:
+            i = 0
+        if periodic_y and j == ny:
+            j = 0
+        return j * (nx + 1) + i
+
+    def side(a, b):
+        (i0, j0), (i1, j1) = a, b
+        if j0 == 0 and j1 == 0:
+            return 0
+        if i0 == nx and i1 == nx:
+            return 1
+        if j0 == ny and j1 == ny:
+            return 2
+        if i0 == 0 and i1 == 0:
+            return 3
+        return -1
+
+    E = 2 * nx * ny
+    vids = np.zeros((E, 3), np.int32)
+    xyz = np.zeros((E, 3, 2))
+    bc = np.zeros((E, 3), np.int32)
+    e = 0
+    for j in range(ny):
+        for i in range(nx):
+            for tri in (((i, j), (i + 1, j), (i + 1, j + 1)), ((i, j), (i + 1, j + 1), (i, j + 1))):
+                for v in range(3):
+                    vids[e, v] = topo(*tri[v])
+                    xyz[e, v] = lattice(*tri[v])
+                for l in range(3):
+                    sd = side(tri[l], tri[(l + 1) % 3])
+                    per = periodic_y if sd in (0, 2) else periodic_x if sd in (1, 3) else True
+                    bc[e, l] = side_tags[sd] if (sd >= 0 and not per) else 0
+                e += 1
+    return vids, xyz, bc
+
+
+def np_tri(p):
+    return (p + 1) * (p + 2) // 2
+
+
+def random_modal_perturbation(U, ndof_per_elem, scale, seed):
+    rng = np.random.default_rng(seed)
+    return U + scale * rng.standard_normal(U.shape)
+
+
+VORTEX = (5.0, 5.0, 5.0, 1.0, 1.0, 10.0)  # beta, xc, yc, u0, v0, L
+
+
+def collapsed_rule(n):
+    """Collapsed (Duffy) Gauss-Legendre rule on the reference triangle, n x n points,
+    exact for polynomials of degree 2n-2; independent of the generated tables."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    t = 0.5 * (x + 1)
+    wt = 0.5 * w
+    pts, wts = [], []
+    for a, wa in zip(t, wt):
+        for b, wb in zip(t, wt):
+            u = a
+            v = (1 - a) * b
+            pts.append((u, v))
+            wts.append(wa * wb * (1 - a))
+    return np.array(pts), np.array(wts)

@@ -1,0 +1,188 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same meshes and states.  FP64 plain form: residuals and RK steps within 1e-12
+relative (max-norm over all DOFs), time steps within 1e-14, connectivity
+bit-exact as a set.  FP32 reduced form: error budget against the FP64 path."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import VORTEX, mesh_tri, np_tri
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+FREESTREAM = (1.0, 1.0, 1.0, 3.5)  # rho, rho u, rho v, E of (rho,u,v,p) = (1,1,1,1), gamma 1.4
+
+
+def announce(name):
+    print("\n[gpu case] %s" % name, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def make_pair(law, flux, p, nx=8, periodic=True, jitter=0.1, seed=3, precision=0, use_graph=True):
+    import dgpar
+    if law == "scalar":
+        vids, xyz, bc = mesh_tri(nx, nx, periodic_x=periodic, periodic_y=periodic, jitter=jitter, seed=seed,
+                                 side_tags=(30000,) * 4)
+        o = po.Oracle(2, vids, xyz, p, po.LAW_SCALAR, po.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.0,), bc=bc)
+        lw = dgpar.Law(dgpar.LAW_SCALAR, dgpar.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.0,))
+        U = o.project(0)
+    else:
+        tags = (40000, 30000, 40000, 30000)
+        vids, xyz, bc = mesh_tri(nx, nx, lx=10, ly=10, periodic_x=periodic, periodic_y=periodic, jitter=jitter,
+                                 seed=seed, side_tags=tags)
+        ofl = po.FLUX_LF if flux == "lf" else po.FLUX_ROE
+        o = po.Oracle(2, vids, xyz, p, po.LAW_EULER2D, ofl, gamma=1.4, farfield=FREESTREAM, bc=bc)
+        lw = dgpar.Law(dgpar.LAW_EULER2D, dgpar.FLUX_LF if flux == "lf" else dgpar.FLUX_ROE, gamma=1.4,
+                       farfield=FREESTREAM)
+        U = o.project(1, VORTEX)
+    mesh = dgpar.Mesh(2, vids, xyz, bc)
+    g = dgpar.DGAnalysis(mesh, lw, p, precision=precision, use_graph=use_graph)
+    g.set_state(U)
+    return g, o, U
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_scalar_residual(p):
+    announce("scalar residual P%d" % p)
+    g, o, U = make_pair("scalar", None, p)
+    Rg = g.residual()
+    Ro, _ = o.residual(U)
+    assert rel(Rg, Ro) < 1e-12
+
+
+@pytest.mark.parametrize("flux", ["lf", "roe"])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_euler_residual_periodic(p, flux):
+    announce("euler %s residual P%d periodic" % (flux, p))
+    g, o, U = make_pair("euler", flux, p)
+    assert rel(g.residual(), o.residual(U)[0]) < 1e-12
+
+
+@pytest.mark.parametrize("flux", ["lf", "roe"])
+@pytest.mark.parametrize("p", [1, 3])
+def test_euler_residual_walls_and_farfield(p, flux):
+    announce("euler %s residual P%d walls" % (flux, p))
+    g, o, U = make_pair("euler", flux, p, periodic=False)
+    assert rel(g.residual(), o.residual(U)[0]) < 1e-12
+
+
+def test_state_roundtrip_and_connectivity():
+    announce("state roundtrip + connectivity")
+    g, o, U = make_pair("euler", "lf", 3, nx=6)
+    assert np.array_equal(g.get_state(), U)
+    fg = {tuple(r) for r in g.faces()}
+    fo = {tuple(r) for r in o.faces()}
+    assert fg == fo
+
+
+@pytest.mark.parametrize("law,flux,p,scheme", [("scalar", None, 2, po.RK_TVD3), ("scalar", None, 1, po.RK_FE),
+                                               ("euler", "lf", 3, po.RK4), ("euler", "roe", 2, po.RK_TVD2),
+                                               ("euler", "roe", 3, po.RK_TVD3)])
+def test_steps(law, flux, p, scheme):
+    announce("steps %s %s P%d scheme %d" % (law, flux, p, scheme))
+    g, o, U = make_pair(law, flux, p, periodic=(law == "scalar"))
+    dt = o.dt(U, 0.1)
+    t = 0.0
+    Uo = U
+    for s in range(3):
+        Uo, ro = o.step(Uo, scheme, t, dt)
+        t += dt
+    rg = 0.0
+    for s in range(3):
+        rg = g.advanceInTime(s * dt, dt, scheme, 1)
+    Ug = g.get_state()
+    assert rel(Ug, Uo) < 1e-12
+    assert abs(rg - ro) <= 1e-10 * abs(ro) + 1e-300
+
+
+@pytest.mark.parametrize("law,flux,p", [("scalar", None, 2), ("euler", "lf", 3), ("euler", "roe", 4)])
+def test_dt(law, flux, p):
+    announce("dt %s P%d" % (law, p))
+    g, o, U = make_pair(law, flux, p)
+    assert abs(g.computeDt(0.1) - o.dt(U, 0.1)) <= 1e-14 * o.dt(U, 0.1)
+
+
+def test_deterministic_and_graph_equals_eager():
+    announce("determinism + graph replay")
+    g, o, U = make_pair("euler", "lf", 3, nx=12)
+    r1 = g.residual()
+    r2 = g.residual()
+    assert np.array_equal(r1, r2)
+    dt = o.dt(U, 0.1)
+    g.advanceInTime(0, dt, po.RK4, 5, want_resid=False)
+    a = g.get_state()
+    g2, _, _ = make_pair("euler", "lf", 3, nx=12, use_graph=False)
+    g2.advanceInTime(0, dt, po.RK4, 5, want_resid=False)
+    b = g2.get_state()
+    assert np.array_equal(a, b)
+    # graph replays interleaved with eager host work stay identical
+    g.set_state(U)
+    for _ in range(3):
+        g.advanceInTime(0, dt, po.RK4, 1, want_resid=True)
+        g.residual()
+    g2.set_state(U)
+    g2.advanceInTime(0, dt, po.RK4, 3, want_resid=False)
+    assert np.array_equal(g.get_state(), g2.get_state())
+
+
+def test_fp32_form_error_budget():
+    announce("fp32 form budget")
+    import dgpar
+    g64, o, U = make_pair("euler", "lf", 3, nx=16)
+    g32, _, _ = make_pair("euler", "lf", 3, nx=16, precision=dgpar.PREC_FP32)
+    R64, R32 = g64.residual(), g32.residual()
+    assert rel(R32, R64) < 1e-5
+    dt = o.dt(U, 0.1)
+    g64.advanceInTime(0, dt, po.RK4, 20, want_resid=False)
+    g32.advanceInTime(0, dt, po.RK4, 20, want_resid=False)
+    e64 = o.l2_error(g64.get_state(), 1, VORTEX, 20 * dt)
+    e32 = o.l2_error(g32.get_state(), 1, VORTEX, 20 * dt)
+    d = g32.get_state() - g64.get_state()
+    # density L2 difference between the forms, measured with the oracle's norm
+    Np = np_tri(3)
+    dd = d.reshape(-1, 4, Np)[:, 0, :]
+    l2 = np.sqrt((dd ** 2 * o.detj()[:, None]).sum())
+    assert l2 < 1e-5
+    assert abs(e32[0] - e64[0]) < 1e-5
+
+
+def test_error_paths():
+    announce("error paths")
+    import dgpar
+    g, o, U = make_pair("scalar", None, 1, nx=4)
+    bad = U.copy()
+    bad[3] = np.nan
+    with pytest.raises(dgpar.DGError) as ei:
+        g.set_state(bad)
+    assert ei.value.status == dgpar.DG_ERR_NONFINITE
+    with pytest.raises(dgpar.DGError) as ei:
+        g.set_state(U[:-1])
+    assert ei.value.status == dgpar.DG_ERR_ARG
+    vids, xyz, bc = mesh_tri(4, 4)
+    with pytest.raises(dgpar.DGError) as ei:
+        dgpar.DGAnalysis(dgpar.Mesh(2, vids, xyz), dgpar.ScalarLaw((1, 0)), 7)
+    assert ei.value.status == dgpar.DG_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("field", ["vortex", "gaussian"])
+def test_projection_and_l2_error(field):
+    announce("projection + l2 error %s" % field)
+    import dgpar
+    law = "euler" if field == "vortex" else "scalar"
+    g, o, U = make_pair(law, "lf", 3)
+    if field == "vortex":
+        g.project(dgpar.FIELD_VORTEX, VORTEX)
+        fid, par = 1, VORTEX
+    else:
+        g.project(dgpar.FIELD_GAUSSIAN)
+        fid, par = 0, ()
+    assert rel(g.get_state(), U) < 1e-13
+    eg = g.l2_error(fid, par, 0.3)
+    eo = o.l2_error(U, fid, par, 0.3)
+    assert np.abs(eg - eo).max() <= 1e-12 * max(1.0, np.abs(eo).max())
