@@ -197,6 +197,12 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_opti
     i = j;
   }
 
+  if (m->bc)
+    for (int e = 0; e < E; e++)
+      for (int l = 0; l < 3; l++)
+        if (nbe[e * 3 + l] < 0 && m->bc[e * 3 + l] == DG_TAG_CHARACTERISTIC)
+          return bail(fail(c, DG_ERR_UNSUPPORTED, "characteristic far-field boundary (tag 30000) is not built"));
+
   // ---- internal order: group elements whose face links share (nb local face,
   //      orientation, boundary code) so a wavefront takes one path ------------
   auto sig = [&](int e) {

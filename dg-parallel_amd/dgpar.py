@@ -28,7 +28,7 @@ LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
 FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL = 0, 1, 2, 3
 RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
-TAG_FARFIELD, TAG_WALL = 30000, 40000
+TAG_FARFIELD, TAG_CHARACTERISTIC, TAG_WALL = 1, 30000, 40000
 FIELD_GAUSSIAN, FIELD_VORTEX, FIELD_SOD, FIELD_CAVITY = 0, 1, 2, 3
 
 STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_OOM", 3: "DG_ERR_HIP", 4: "DG_ERR_NONFINITE",

@@ -43,8 +43,11 @@ enum { DG_LAW_SCALAR = 0, DG_LAW_EULER2D = 1, DG_LAW_MAXWELL3D = 2 };
 enum { DG_FLUX_UPWIND = 0, DG_FLUX_LF = 1, DG_FLUX_ROE = 2, DG_FLUX_CENTRAL = 3 };
 enum { DG_RK_FE = 0, DG_RK_TVD2 = 1, DG_RK_TVD3 = 2, DG_RK4 = 3 };
 enum { DG_PREC_FP64 = 0, DG_PREC_FP32 = 1 };
-/* boundary tags (the reference's geometric classification ids) */
-enum { DG_TAG_FARFIELD = 30000, DG_TAG_WALL = 40000 };
+/* Boundary tags (the reference's classification ids, EulerLaw.cc:182-296):
+ * 40000 reflective wall; 30000 characteristic far field (not built: rejected);
+ * any other tag on a face without neighbour: Riemann flux against the law's
+ * far-field state (the reference's default branch, EulerLaw.cc:293-295). */
+enum { DG_TAG_FARFIELD = 1, DG_TAG_CHARACTERISTIC = 30000, DG_TAG_WALL = 40000 };
 
 /* Mesh handed over by the caller (replaces mDGMesh + mImportExport for the path:
  * mMesh.h:52-139).  Faces are matched by their sorted topological vertex ids;

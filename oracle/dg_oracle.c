@@ -169,6 +169,36 @@ int or_basis(int dim, int p, const double* uvw, double* phi, double* dphi) {
   return 0;
 }
 
+/* Test hook: replace the triangle basis coefficients or a triangle rule by
+ * externally supplied tables (used to evaluate the restatement with the
+ * reference's own printed tables). */
+static double tri_ovr[11][32][3];
+static int tri_ovr_n[11];
+
+int or_set_tri_basis(int n, const double* C) {
+  gs_init();
+  if (n < 1 || n > 28) return -1;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) gs_tri[i][j] = C[i * n + j];
+  return 0;
+}
+
+int or_set_tri_rule(int degree, int n, const double* pts) {
+  if (degree < 0 || degree > 10 || n < 0 || n > 32) return -1;
+  for (int q = 0; q < n; q++)
+    for (int k = 0; k < 3; k++) tri_ovr[degree][q][k] = pts[q * 3 + k];
+  tri_ovr_n[degree] = n;
+  return 0;
+}
+
+void or_reset_tables(void) {
+  pthread_mutex_lock(&gs_lock);
+  gs_ready = 0;
+  pthread_mutex_unlock(&gs_lock);
+  gs_init();
+  for (int d = 0; d <= 10; d++) tri_ovr_n[d] = 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* quadrature selection (Integrator.cc:31-101)                                */
 /* ------------------------------------------------------------------------- */
@@ -176,6 +206,15 @@ static int vol_rule(int dim, int degree, double (*pts)[4]) {
   if (degree < 1) degree = 1;
   if (dim == 2) {
     if (degree > DG_TRI_RULE_MAXDEG) return -1;
+    if (tri_ovr_n[degree]) {
+      for (int q = 0; q < tri_ovr_n[degree]; q++) {
+        pts[q][0] = tri_ovr[degree][q][0];
+        pts[q][1] = tri_ovr[degree][q][1];
+        pts[q][2] = 0.0;
+        pts[q][3] = 0.5 * tri_ovr[degree][q][2];
+      }
+      return tri_ovr_n[degree];
+    }
     int o = dg_tri_rule_off[degree], n = dg_tri_rule_npt[degree];
     for (int q = 0; q < n; q++) {
       pts[q][0] = dg_tri_rule_pts[o + q][0];

@@ -79,6 +79,14 @@ int or_exact(int field, const double* par, const or_law* law, const double* x, d
  * (DGError.cc:805-831) */
 int or_l2_error(or_ctx* c, const double* U, int field, const double* par, double t, double* err);
 
+/* test hooks: substitute the triangle basis coefficient matrix (n x n, row i =
+ * coefficients of phi_i over the Pascal monomials) or the triangle rule of one
+ * degree ({u, v, weight} with weights summing to 1); or_reset_tables() restores
+ * the oracle's own.  Affects contexts created afterwards. */
+int or_set_tri_basis(int n, const double* C);
+int or_set_tri_rule(int degree, int n, const double* pts);
+void or_reset_tables(void);
+
 #ifdef __cplusplus
 }
 #endif

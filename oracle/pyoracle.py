@@ -54,10 +54,26 @@ def _load():
     lib.or_project.argtypes = [vp, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64]
     lib.or_exact.argtypes = [ctypes.c_int, _p_f64, ctypes.POINTER(_Law), _p_f64, ctypes.c_double, _p_f64]
     lib.or_l2_error.argtypes = [vp, _p_f64, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64]
+    lib.or_set_tri_basis.argtypes = [ctypes.c_int, _p_f64]
+    lib.or_set_tri_rule.argtypes = [ctypes.c_int, ctypes.c_int, _p_f64]
     return lib
 
 
 _lib = _load()
+
+
+def set_tri_basis(C):
+    C = np.ascontiguousarray(C, np.float64)
+    assert _lib.or_set_tri_basis(C.shape[0], _p(C, _p_f64)) == 0
+
+
+def set_tri_rule(degree, pts):
+    pts = np.ascontiguousarray(pts, np.float64)
+    assert _lib.or_set_tri_rule(degree, pts.shape[0], _p(pts, _p_f64)) == 0
+
+
+def reset_tables():
+    _lib.or_reset_tables()
 
 
 def _p(a, t):

@@ -1,0 +1,200 @@
+// ref_driver.cc -- a serial driver for the reference's own DG classes, compiled
+// against the reference sources (oracle/Makefile.ref) into oracle/_ref/.  TEST
+// INFRASTRUCTURE ONLY: it produces reference residuals / RK steps so the oracle
+// can be checked against the real reference code.  It plays the role of the
+// reference's main.cc (main.cc:55-159) and scenario functions (Lilia.cc:84-108):
+//
+//   ref_driver MESH.msh LAW FLUX ORDER OP STATE_IN OUT [DT NSTEPS]
+//     LAW   scalar | euler      FLUX  upwind | exact | roe | lf
+//     OP    residual | fe | tvd2 | tvd3 | rk4
+//     STATE_IN: one line per cell: 3 vertex ids (any order), then Nf*Np coefficients
+//     OUT: one line per cell in the reference's iteration order: its 3 vertex ids
+//          in the reference's own vertex order, then the residual or new state
+//   scalar velocity (VX VY) and far-field state are read from the environment
+//   (REF_VEL="vx vy", REF_FARFIELD="q0 q1 q2 q3").
+//
+// Only the serial code paths run (numThreads = 1): the threaded face assembly
+// races (parallel.cc:46-76), the serial one does not.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <set>
+#include <vector>
+
+#include "ConservationLaw.h"
+#include "DGAnalysis.h"
+#include "DGCell.h"
+#include "EulerLaw.h"
+#include "FieldEvaluator.h"
+#include "ScalarLaw.h"
+#include "TimeIntegrator.h"
+#include "Vertex.h"
+#include "mDGMesh.h"
+#include "mEntity.h"
+#include "mImportExport.h"
+#include "mPoint.h"
+#include "mVector.h"
+#include "parallel.h"
+
+// globals the reference's main.cc defines (main.cc:32-36)
+unsigned int numCPU = 1;
+unsigned int numThreads = 1;
+vector<infoWrapper> threadInfo[3];
+double timeRK = 0;
+timespec timer1load, timer2load;
+timespec timer_start, timer_stop;
+
+static double g_vel[3] = {1.0, 0.5, 0.0};
+static double g_ff[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+class ConstField : public FieldEvaluator {
+  const double* v;
+  int n;
+
+ public:
+  ConstField(const double* vals, int nv) : v(vals), n(nv) {}
+  virtual void eval(const mPoint&, double, double* val) const {
+    for (int i = 0; i < n; i++) val[i] = v[i];
+  }
+  virtual int order() const { return 0; }
+};
+
+// Local Lax-Friedrichs flux plugged into the reference's Euler2d law through
+// the riemannSolver virtual (ConservationLaw.h:29-33): the only part of the
+// path the reference does not provide.  Wave speed as maximumEigenValue(n,.,.)
+// (EulerLaw.cc:1022-1034).
+class Euler2dLF : public Euler2d {
+ public:
+  Euler2dLF(set<int>& w, FieldEvaluator* f, double g) : Euler2d(w, f, g) {}
+  virtual void riemannSolver(mVector& n, const mPoint& p, double* uL, double* uR, double* flux) const {
+    mVector fL[4], fR[4];
+    Fi(p, uL, fL);
+    Fi(p, uR, fR);
+    double lam = maximumEigenValue(n, uL, uR);
+    for (int i = 0; i < 4; i++) flux[i] = 0.5 * ((fL[i] + fR[i]) * n - lam * (uR[i] - uL[i]));
+  }
+};
+
+static int np_tri(int p) { return (p + 1) * (p + 2) / 2; }
+
+int main(int argc, char** argv) {
+  if (argc < 8) {
+    fprintf(stderr, "usage: %s MESH LAW FLUX ORDER OP STATE_IN OUT [DT NSTEPS]\n", argv[0]);
+    return 2;
+  }
+  const char* law_s = argv[2];
+  const char* flux_s = argv[3];
+  int order = atoi(argv[4]);
+  const char* op = argv[5];
+  double dt = argc > 8 ? atof(argv[8]) : 0.0;
+  int nsteps = argc > 9 ? atoi(argv[9]) : 1;
+  if (getenv("REF_VEL")) sscanf(getenv("REF_VEL"), "%lf %lf", &g_vel[0], &g_vel[1]);
+  if (getenv("REF_FARFIELD"))
+    sscanf(getenv("REF_FARFIELD"), "%lf %lf %lf %lf", &g_ff[0], &g_ff[1], &g_ff[2], &g_ff[3]);
+
+  mImportExport io;
+  mDGMesh* mesh = new mDGMesh;
+  io.import(argv[1], mesh);
+  const int dim = 2;
+  mesh->createCellBoundaries(dim, dim - 1);
+  mesh->createConnections(dim - 1, dim);
+  mesh->createConnections(0, dim - 1);
+  mesh->setPeriodicBC(dim);
+  for (int i = 0; i < 3; i++) threadInfo[i].push_back(infoWrapper(mesh->begin(i), mesh->end(i), 0, 0));
+
+  ConservationLaw* law = 0;
+  ConstField vel(g_vel, 3), ff(g_ff, 4);
+  set<int> walls;
+  walls.insert(40000);
+  int nf;
+  if (!strcmp(law_s, "scalar")) {
+    law = new ScalarLaw(&ff, &vel, &ff);
+    nf = 1;
+  } else {
+    if (!strcmp(flux_s, "roe"))
+      law = new Euler2dRoe(walls, &ff, 1.4);
+    else if (!strcmp(flux_s, "lf"))
+      law = new Euler2dLF(walls, &ff, 1.4);
+    else
+      law = new Euler2d(walls, &ff, 1.4);
+    nf = 4;
+  }
+  const int NP = np_tri(order);
+  DGAnalysis* an = new DGAnalysis(mesh, law, 0, order);
+  (void)an;
+
+  // read the state keyed by sorted vertex ids
+  std::map<std::vector<int>, std::vector<double> > state;
+  FILE* fi = fopen(argv[6], "r");
+  if (!fi) return 3;
+  int v[3];
+  while (fscanf(fi, "%d %d %d", &v[0], &v[1], &v[2]) == 3) {
+    std::vector<int> key(v, v + 3);
+    std::sort(key.begin(), key.end());
+    std::vector<double> c(nf * NP);
+    for (int i = 0; i < nf * NP; i++)
+      if (fscanf(fi, "%lf", &c[i]) != 1) return 4;
+    state[key] = c;
+  }
+  fclose(fi);
+  int DOF = 0;
+  for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it) {
+    mEntity* m = *it;
+    DGCell* cell = (DGCell*)m->getCell();
+    std::vector<int> key;
+    for (int k = 0; k < 3; k++) key.push_back(m->get(0, k)->getId());
+    std::sort(key.begin(), key.end());
+    if (!state.count(key)) {
+      fprintf(stderr, "cell not in state file\n");
+      return 5;
+    }
+    const std::vector<double>& c = state[key];
+    for (int f = 0; f < nf; f++)
+      for (int k = 0; k < NP; k++) cell->theFieldsCoefficients->get(f, k) = c[f * NP + k];
+    cell->computeMean();
+    cell->ZeroRHS();
+    cell->setTimeStep(dt);
+    DOF += nf * NP;
+  }
+
+  const bool residual = !strcmp(op, "residual");
+  if (residual) {
+    // serial assembly as TimeIntegrator::assembleVolume/assembleBoundary do with one thread
+    for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it)
+      ((DGCell*)(*it)->getCell())->computeVolumeContribution(0.0);
+    for (mMesh::iter it = mesh->begin(dim - 1); it != mesh->end(dim - 1); ++it)
+      ((DGBoundaryCell*)(*it)->getCell())->computeBoundaryContributions(0.0);
+  } else {
+    TimeIntegrator* ti;
+    if (!strcmp(op, "rk4"))
+      ti = new RungeKutta4(mesh, 0, nf, DOF);
+    else if (!strcmp(op, "tvd3"))
+      ti = new RungeKuttaTVD3(mesh, 0, nf, DOF);
+    else if (!strcmp(op, "tvd2"))
+      ti = new RungeKuttaTVD2(mesh, 0, nf, DOF);
+    else
+      ti = new ForwardEuler(mesh, 0, nf, DOF);
+    double t = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+      double r = ti->advanceInTime(t, dt);
+      printf("resid %.17e\n", r);
+      t += dt;
+    }
+  }
+
+  FILE* fo = fopen(argv[7], "w");
+  if (!fo) return 6;
+  for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it) {
+    mEntity* m = *it;
+    DGCell* cell = (DGCell*)m->getCell();
+    for (int k = 0; k < 3; k++) fprintf(fo, "%d ", m->get(0, k)->getId());
+    for (int f = 0; f < nf; f++)
+      for (int k = 0; k < NP; k++)
+        fprintf(fo, " %.17e", residual ? cell->theRightHandSide[f * NP + k] : cell->theFieldsCoefficients->get(f, k));
+    fprintf(fo, "\n");
+  }
+  fclose(fo);
+  return 0;
+}

@@ -38,9 +38,9 @@ def test_python_binding_imports():
 def test_mesh_generator_matches_independent_generator(periodic):
     import dgpar
     m = dgpar.mesh_tri_structured(7, 5, x0=-1, y0=2, lx=3.0, ly=2.0, periodic_x=periodic, periodic_y=periodic,
-                                  jitter=0.1, seed=12345, side_tags=(40000, 30000, 40000, 30000))
+                                  jitter=0.1, seed=12345, side_tags=(40000, 7, 40000, 7))
     vids, xyz, bc = mesh_tri(7, 5, x0=-1, y0=2, lx=3.0, ly=2.0, periodic_x=periodic, periodic_y=periodic,
-                             jitter=0.1, seed=12345, side_tags=(40000, 30000, 40000, 30000))
+                             jitter=0.1, seed=12345, side_tags=(40000, 7, 40000, 7))
     assert np.array_equal(m.vids, vids)
     assert np.array_equal(m.xyz, xyz)
     assert np.array_equal(m.bc, bc)

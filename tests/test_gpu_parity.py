@@ -28,12 +28,12 @@ def make_pair(law, flux, p, nx=8, periodic=True, jitter=0.1, seed=3, precision=0
     import dgpar
     if law == "scalar":
         vids, xyz, bc = mesh_tri(nx, nx, periodic_x=periodic, periodic_y=periodic, jitter=jitter, seed=seed,
-                                 side_tags=(30000,) * 4)
+                                 side_tags=(7,) * 4)
         o = po.Oracle(2, vids, xyz, p, po.LAW_SCALAR, po.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.0,), bc=bc)
         lw = dgpar.Law(dgpar.LAW_SCALAR, dgpar.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.0,))
         U = o.project(0)
     else:
-        tags = (40000, 30000, 40000, 30000)
+        tags = (40000, 7, 40000, 7)
         vids, xyz, bc = mesh_tri(nx, nx, lx=10, ly=10, periodic_x=periodic, periodic_y=periodic, jitter=jitter,
                                  seed=seed, side_tags=tags)
         ofl = po.FLUX_LF if flux == "lf" else po.FLUX_ROE

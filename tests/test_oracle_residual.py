@@ -14,7 +14,7 @@ from oracle import pyoracle as po
 
 def scalar_oracle(nx, p, jitter=0.1, periodic=True, vel=(1.0, 0.5), bc=None):
     vids, xyz, b = mesh_tri(nx, nx, periodic_x=periodic, periodic_y=periodic, jitter=jitter, seed=7,
-                            side_tags=(30000,) * 4)
+                            side_tags=(7,) * 4)
     return po.Oracle(2, vids, xyz, p, po.LAW_SCALAR, po.FLUX_UPWIND, vel=vel, farfield=(0.0,), bc=b), vids, xyz
 
 
@@ -78,7 +78,7 @@ def test_euler_wall_free_stream(p):
     # a fluid at rest next to walls stays at rest; a far-field ghost equal to the
     # state gives no flux jump
     vids, xyz, bc = mesh_tri(5, 4, periodic_x=False, periodic_y=False, jitter=0.1, seed=2,
-                             side_tags=(40000, 30000, 40000, 30000))
+                             side_tags=(40000, 7, 40000, 7))
     q = [0.9, 0.0, 0.0, 2.0]
     o = po.Oracle(2, vids, xyz, p, po.LAW_EULER2D, po.FLUX_ROE, gamma=1.4, farfield=q, bc=bc)
     R, _ = o.residual(_constant_state(o, p, q))
@@ -110,7 +110,7 @@ def test_scalar_polynomial_advection_exact(p):
     # boundary equals the L2 projection of -v.grad u (upwind flux of a continuous trace)
     vel = (0.7, -0.4)
     nx = 6
-    vids, xyz, bc = mesh_tri(nx, nx, periodic_x=False, periodic_y=False, jitter=0.1, seed=4, side_tags=(30000,) * 4)
+    vids, xyz, bc = mesh_tri(nx, nx, periodic_x=False, periodic_y=False, jitter=0.1, seed=4, side_tags=(7,) * 4)
     o = po.Oracle(2, vids, xyz, p, po.LAW_SCALAR, po.FLUX_UPWIND, vel=vel, farfield=(0.0,), bc=bc)
     c = np.random.default_rng(p).standard_normal(10)
 
