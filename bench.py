@@ -46,10 +46,10 @@ def bytes_per_dof_stage(fp32, nf, np_):
     return sum(st) / 4.0 + table
 
 
-def run_form(dgpar, mesh, precision, steps, warmup):
+def run_form(dgpar, mesh, precision, steps, warmup, kernel=0):
     import torch
     law = dgpar.EulerLaw(dgpar.FLUX_LF, 1.4)
-    g = dgpar.DGAnalysis(mesh, law, ORDER, precision=precision, use_graph=True, device=0)
+    g = dgpar.DGAnalysis(mesh, law, ORDER, precision=precision, use_graph=True, device=0, kernel=kernel)
     g.project(dgpar.FIELD_VORTEX, VORTEX)
     dt = g.computeDt(CFL)
     if warmup > 0:
@@ -88,16 +88,26 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernel", type=int, default=0, help="stage kernel: 0 tiled (default), 1 one element per lane")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a plain 256 MiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path; use --gpus 1", file=sys.stderr)
         return 2
     import dgpar
 
+    if args.copy_probe:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     mesh = dgpar.mesh_tri_structured(NX, NX, lx=10.0, ly=10.0, jitter=0.1, seed=SEED)
     E = mesh.n_elems
-    r32 = run_form(dgpar, mesh, dgpar.PREC_FP32, args.steps, args.warmup)
-    r64 = run_form(dgpar, mesh, dgpar.PREC_FP64, args.steps, args.warmup)
+    r32 = run_form(dgpar, mesh, dgpar.PREC_FP32, args.steps, args.warmup, args.kernel)
+    r64 = run_form(dgpar, mesh, dgpar.PREC_FP64, args.steps, args.warmup, args.kernel)
     ndof = r32["ndof"]
     np_ = (ORDER + 1) * (ORDER + 2) // 2
     X = mesh.xyz
@@ -131,7 +141,8 @@ def main():
         "config": {"workload": "euler2d-isentropic-vortex P3 LF RK4 CFL0.1", "elements": E, "dof": ndof,
                    "mesh": "256x256x2 periodic [0,10]^2 jitter 0.1h seed %d" % SEED, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "tri_stage_kernel<3,EulerLawD,float,*>",
+                     "traffic": None,
+                     "kernel": ("tri_stage_tiled" if args.kernel == 0 else "tri_stage_kernel") + "<3,EulerLawD,float,*>",
                      "bytes_per_dof_stage": b32},
         "fp64_plain": {"value": v64, "ms_per_step": r64["ms"] / args.steps, "achieved_GBs": ach64,
                        "frac": ach64 / HBM_PEAK_GBS, "bytes_per_dof_stage": b64},

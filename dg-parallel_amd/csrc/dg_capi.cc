@@ -19,6 +19,7 @@
 #include "../../include/dg_capi.h"
 #include "dg_fields.h"
 #include "dg_kernels.h"
+#include "dg_setup.h"
 #include "dg_tables.h"
 
 using namespace dgp;
@@ -27,6 +28,7 @@ struct dg_ctx {
   std::string err;
   int dim = 2, E = 0, Nf = 0, P = 0, NP = 0;
   int law = 0, prec = 0, use_graph = 0, device = 0;
+  int kernel = 0;  // 0: tiled LDS-staged stage kernel, 1: one element per work-item
   LawParams lp{};
   size_t ndof = 0;
   std::vector<int32_t> perm;  // internal index -> caller element
@@ -95,18 +97,6 @@ static int nfields(int law) {
   return -1;
 }
 
-namespace {
-struct FaceKey {
-  int32_t k[2];
-  int32_t e, l;
-  bool operator<(const FaceKey& o) const {
-    if (k[0] != o.k[0]) return k[0] < o.k[0];
-    if (k[1] != o.k[1]) return k[1] < o.k[1];
-    if (e != o.e) return e < o.e;
-    return l < o.l;
-  }
-};
-}  // namespace
 
 extern "C" {
 
@@ -131,154 +121,33 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_opti
     *out = c;
     return s;
   };
-  if (!m || !law || !opt || !m->vids || !m->xyz || m->n_elems <= 0) return bail(fail(c, DG_ERR_ARG, "null mesh/law/options"));
-  if (m->dim != 2) return bail(fail(c, DG_ERR_UNSUPPORTED, "only triangles (dim 2) are built"));
-  c->dim = m->dim;
-  c->E = m->n_elems;
-  c->law = law->law;
-  c->Nf = nfields(law->law);
-  if (c->Nf < 0 || c->law == DG_LAW_MAXWELL3D) return bail(fail(c, DG_ERR_UNSUPPORTED, "law %d not built for triangles", law->law));
-  if (c->law == DG_LAW_EULER2D && law->flux != DG_FLUX_LF && law->flux != DG_FLUX_ROE)
-    return bail(fail(c, DG_ERR_UNSUPPORTED, "Euler flux %d not built", law->flux));
-  if (c->law == DG_LAW_SCALAR && law->flux != DG_FLUX_UPWIND)
-    return bail(fail(c, DG_ERR_UNSUPPORTED, "scalar flux %d not built", law->flux));
-  int P = opt->order;
-  if (m->order) {
-    P = m->order[0];
-    for (int e = 1; e < c->E; e++)
-      if (m->order[e] != P) return bail(fail(c, DG_ERR_UNSUPPORTED, "mixed orders are not built in this entry point"));
-  }
-  if (P < 1 || P > MAXP_TRI) return bail(fail(c, DG_ERR_UNSUPPORTED, "order %d outside 1..%d", P, MAXP_TRI));
-  if (opt->precision != DG_PREC_FP64 && opt->precision != DG_PREC_FP32) return bail(fail(c, DG_ERR_ARG, "bad precision"));
-  c->P = P;
-  c->NP = np_tri(P);
+  TriSetup S;
+  std::string err;
+  dg_status st = build_tri_setup(m, law, opt, S, err);
+  if (st != DG_OK) return bail(fail(c, st, "%s", err.c_str()));
+  c->dim = 2;
+  c->E = S.E;
+  c->law = S.law;
+  c->Nf = S.NF;
+  c->P = S.P;
+  c->NP = S.NP;
   c->prec = opt->precision;
   c->use_graph = opt->use_graph;
   c->device = opt->device;
-  c->lp.gamma = law->gamma;
-  for (int i = 0; i < 3; i++) c->lp.vel[i] = law->vel[i];
-  for (int i = 0; i < 8; i++) c->lp.farfield[i] = law->farfield[i];
-  c->lp.flux = law->flux;
-  const int E = c->E, NF = c->Nf, NP = c->NP;
-  c->ndof = (size_t)E * NF * NP;
-  c->xyz.assign(m->xyz, m->xyz + (size_t)E * 6);
-
-  // ---- connectivity (mMesh.cc:321-378, translation-periodic by vertex ids) ----
-  std::vector<FaceKey> keys((size_t)E * 3);
-  for (int e = 0; e < E; e++)
-    for (int l = 0; l < 3; l++) {
-      int32_t a = m->vids[e * 3 + l], b = m->vids[e * 3 + (l + 1) % 3];
-      if (a == b) return bail(fail(c, DG_ERR_ARG, "degenerate edge in element %d", e));
-      FaceKey& k = keys[(size_t)e * 3 + l];
-      k.k[0] = std::min(a, b);
-      k.k[1] = std::max(a, b);
-      k.e = e;
-      k.l = l;
-    }
-  std::sort(keys.begin(), keys.end());
-  std::vector<int32_t> nbe((size_t)E * 3, -1), nbl((size_t)E * 3, -1), same((size_t)E * 3, 0);
-  for (size_t i = 0; i < keys.size();) {
-    size_t j = i + 1;
-    while (j < keys.size() && keys[j].k[0] == keys[i].k[0] && keys[j].k[1] == keys[i].k[1]) j++;
-    if (j - i > 2) return bail(fail(c, DG_ERR_ARG, "edge (%d,%d) shared by %zu elements", keys[i].k[0], keys[i].k[1], j - i));
-    const FaceKey& L = keys[i];
-    if (j - i == 2) {
-      const FaceKey& Rk = keys[i + 1];
-      nbe[L.e * 3 + L.l] = Rk.e;
-      nbl[L.e * 3 + L.l] = Rk.l;
-      nbe[Rk.e * 3 + Rk.l] = L.e;
-      nbl[Rk.e * 3 + Rk.l] = L.l;
-      int s = m->vids[L.e * 3 + L.l] == m->vids[Rk.e * 3 + Rk.l];
-      same[L.e * 3 + L.l] = same[Rk.e * 3 + Rk.l] = s;
-      c->faces.insert(c->faces.end(), {L.e, L.l, Rk.e, Rk.l});
-    } else {
-      c->faces.insert(c->faces.end(), {L.e, L.l, -1, -1});
-    }
-    i = j;
-  }
-
-  if (m->bc)
-    for (int e = 0; e < E; e++)
-      for (int l = 0; l < 3; l++)
-        if (nbe[e * 3 + l] < 0 && m->bc[e * 3 + l] == DG_TAG_CHARACTERISTIC)
-          return bail(fail(c, DG_ERR_UNSUPPORTED, "characteristic far-field boundary (tag 30000) is not built"));
-
-  // ---- internal order: group elements whose face links share (nb local face,
-  //      orientation, boundary code) so a wavefront takes one path ------------
-  auto sig = [&](int e) {
-    int s = 0;
-    for (int l = 0; l < 3; l++) {
-      int v;
-      if (nbe[e * 3 + l] >= 0)
-        v = nbl[e * 3 + l] | (same[e * 3 + l] << 2);
-      else {
-        int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
-        v = 8 + (tag == DG_TAG_WALL ? BC_WALL : BC_FARFIELD);
-      }
-      s = s * 16 + v;
-    }
-    return s;
-  };
-  c->perm.resize(E);
-  for (int e = 0; e < E; e++) c->perm[e] = e;
-  std::vector<int> sg(E);
-  for (int e = 0; e < E; e++) sg[e] = sig(e);
-  std::stable_sort(c->perm.begin(), c->perm.end(), [&](int a, int b) { return sg[a] < sg[b]; });
-  std::vector<int32_t> iperm(E);
-  for (int i = 0; i < E; i++) iperm[c->perm[i]] = i;
-
-  // ---- geometry (Mapping.cc:442-566, 687-712; DGCell.cc:1209-1229) ---------------
-  std::vector<double> h_mg((size_t)4 * E), h_detj(E), h_size(E);
-  std::vector<int32_t> h_link((size_t)3 * E);
-  for (int i = 0; i < E; i++) {
-    int e = c->perm[i];
-    const double* X = m->xyz + (size_t)e * 6;
-    double J00 = X[2] - X[0], J01 = X[4] - X[0];  // dx/du, dx/dv
-    double J10 = X[3] - X[1], J11 = X[5] - X[1];  // dy/du, dy/dv
-    double det = J00 * J11 - J01 * J10;
-    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return bail(fail(c, DG_ERR_ARG, "degenerate element %d", e));
-    double ad = std::fabs(det);
-    // |J| * (J^-1)[r][d]
-    double inv[2][2] = {{J11 / det, -J01 / det}, {-J10 / det, J00 / det}};
-    for (int r = 0; r < 2; r++)
-      for (int d = 0; d < 2; d++) h_mg[(size_t)(r * 2 + d) * E + i] = ad * inv[r][d];
-    h_detj[i] = ad;
-    double a = std::sqrt((X[2] - X[0]) * (X[2] - X[0]) + (X[3] - X[1]) * (X[3] - X[1]));
-    double b = std::sqrt((X[4] - X[2]) * (X[4] - X[2]) + (X[5] - X[3]) * (X[5] - X[3]));
-    double cc = std::sqrt((X[0] - X[4]) * (X[0] - X[4]) + (X[1] - X[5]) * (X[1] - X[5]));
-    double hp = 0.5 * (a + b + cc);
-    h_size[i] = std::sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
-    for (int l = 0; l < 3; l++) {
-      int32_t v;
-      if (nbe[e * 3 + l] >= 0)
-        v = (iperm[nbe[e * 3 + l]] << 3) | (same[e * 3 + l] << 2) | nbl[e * 3 + l];
-      else {
-        int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
-        v = -(tag == DG_TAG_WALL ? BC_WALL : BC_FARFIELD);
-      }
-      h_link[(size_t)l * E + i] = v;
-    }
-  }
-
-  // ---- tables ---------------------------------------------------------------------
-  std::vector<double> h_tab(TriBlock::size(P));
-  try {
-    build_tri_block(P, h_tab.data());
-  } catch (const std::exception& ex) {
-    return bail(fail(c, DG_ERR_ARG, "%s", ex.what()));
-  }
-  std::vector<float> h_tabf(h_tab.begin(), h_tab.end());
-  std::vector<double> h_vtx((size_t)4 * NP);
-  {
-    double z[3] = {0, 0, 0}, phi[64];
-    basis(2, 0, z, phi, nullptr);
-    h_vtx[0] = phi[0];  // computeMean: coefficient 0 times phi_0 (DGCell.cc:1133-1137)
-    static const double V[3][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
-    for (int v = 0; v < 3; v++) {
-      basis(2, P, V[v], phi, nullptr);
-      for (int k = 0; k < NP; k++) h_vtx[(1 + v) * NP + k] = phi[k];
-    }
-  }
+  c->kernel = opt->reserved[0] == 1 ? 1 : 0;
+  c->lp = S.lp;
+  c->ndof = (size_t)S.E * S.NF * S.NP;
+  c->perm = S.perm;
+  c->faces = S.faces;
+  c->xyz = S.xyz;
+  const int E = c->E;
+  std::vector<double>& h_mg = S.mg;
+  std::vector<double>& h_detj = S.detj;
+  std::vector<double>& h_size = S.size;
+  std::vector<int32_t>& h_link = S.link;
+  std::vector<double>& h_tab = S.tab;
+  std::vector<float>& h_tabf = S.tabf;
+  std::vector<double>& h_vtx = S.vtx;
 
   // ---- device ---------------------------------------------------------------------
   int ndev = 0;
@@ -296,7 +165,7 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_opti
       (s = dalloc(c, &c->mg, (size_t)4 * E)) || (s = dalloc(c, &c->detj, E)) || (s = dalloc(c, &c->size, E)) ||
       (s = dalloc(c, &c->link, (size_t)3 * E)) || (s = dalloc(c, &c->tab, h_tab.size())) ||
       (s = dalloc(c, &c->tabf, h_tabf.size())) || (s = dalloc(c, &c->vtx, h_vtx.size())) ||
-      (s = dalloc(c, &c->part, (size_t)tri_stage_blocks(E) + 1)))
+      (s = dalloc(c, &c->part, (size_t)std::max(tri_stage_blocks(E), tri_tiled_blocks(E)) + 1)))
     return bail(s);
   if (c->prec == DG_PREC_FP64) {
     if ((s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd))) return bail(s);
@@ -362,6 +231,15 @@ dg_status dg_get_state(dg_ctx* c, double* U, size_t n) {
   return fetch_soa(c, c->Un, U);
 }
 
+static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
+  return c->kernel == 1 ? launch_tri_stage(a, c->law, variant, c->stream)
+                        : launch_tri_stage_tiled(a, c->law, variant, c->stream);
+}
+
+static int stage_blocks(const dg_ctx* c) {
+  return c->kernel == 1 ? tri_stage_blocks(c->E) : tri_tiled_blocks(c->E);
+}
+
 static StageArgs base_args(dg_ctx* c) {
   StageArgs a{};
   a.E = c->E;
@@ -387,7 +265,7 @@ dg_status dg_residual(dg_ctx* c, double t, double* R_out) {
   a.R = c->R;
   a.scheme = MODE_RESIDUAL;
   int variant = c->prec == DG_PREC_FP64 ? 0 : 1;
-  if (launch_tri_stage(a, c->law, variant, c->stream)) return fail(c, DG_ERR_HIP, "residual launch failed: %s", hipGetErrorString(hipGetLastError()));
+  if (launch_stage(c, a, variant)) return fail(c, DG_ERR_HIP, "residual launch failed: %s", hipGetErrorString(hipGetLastError()));
   if (R_out) return fetch_soa(c, c->R, R_out);
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return DG_OK;
@@ -451,7 +329,7 @@ static int enqueue_step(dg_ctx* c, int scheme, double dt, bool want_resid) {
       variant = 0;
     else
       variant = first ? 1 : (last ? 3 : 2);
-    if (launch_tri_stage(a, c->law, variant, c->stream)) return -1;
+    if (launch_stage(c, a, variant)) return -1;
   }
   if (ns == 1) {
     if (hipMemcpyAsync(c->Un, c->A64, c->ndof * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) return -1;
@@ -501,7 +379,7 @@ dg_status dg_advance(dg_ctx* c, int scheme, double t, double dt, int nsteps, dou
   if ((s = run_steps(c, scheme, dt, body))) return s;
   if (resid_out && nsteps > 0) {
     if (enqueue_step(c, scheme, dt, true)) return fail(c, DG_ERR_HIP, "stage launch failed");
-    int nb = tri_stage_blocks(c->E);
+    int nb = stage_blocks(c);
     std::vector<double> h(nb);
     HIPCHECK(c, hipMemcpyAsync(h.data(), c->part, nb * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));

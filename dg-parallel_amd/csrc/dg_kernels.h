@@ -44,6 +44,9 @@ struct StageArgs {
 // 2 = FP32 in/out; 3 = FP32 input, FP64 output
 int launch_tri_stage(const StageArgs& a, int law, int variant, hipStream_t s);
 int tri_stage_blocks(int E);
+// tiled kernel (64 elements per workgroup, LDS-staged); same arguments
+int launch_tri_stage_tiled(const StageArgs& a, int law, int variant, hipStream_t s);
+int tri_tiled_blocks(int E);
 
 struct DtArgs {
   int E, P;

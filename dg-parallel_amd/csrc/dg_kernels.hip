@@ -159,7 +159,6 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(StageArgs a) {
       }
       const CT len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
       const CT n[2] = {nl0 / len, nl1 / len};
-      const CT half = CT(0.5) * len;
       CT uR[NQF][NF];
       if (link >= 0) nb_traces_any<P, LAW, CT, IT>(Uin, E, link, tb, uR);
 #pragma unroll
@@ -177,7 +176,7 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(StageArgs a) {
           LAW::riemann(law, n, uL, uR[q], Fh);
         else
           LAW::boundary(law, -link, n, uL, Fh);
-        const CT sw = half * tb[TriBlock::off_wf(P) + q];
+        const CT sw = len * tb[TriBlock::off_wf(P) + q];  // |J_face| w = (len/2) w
 #pragma unroll
         for (int f = 0; f < NF; f++) {
           const CT fs = Fh[f] * sw;

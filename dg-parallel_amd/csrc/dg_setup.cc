@@ -1,0 +1,193 @@
+// Host-side FP64 table construction for the triangle kernels.
+#include "dg_setup.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <stdexcept>
+
+#include "dg_tables.h"
+
+namespace dgp {
+
+static dg_status failS(std::string& err, dg_status s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  err = buf;
+  return s;
+}
+
+namespace {
+struct FaceKey {
+  int32_t k[2];
+  int32_t e, l;
+  bool operator<(const FaceKey& o) const {
+    if (k[0] != o.k[0]) return k[0] < o.k[0];
+    if (k[1] != o.k[1]) return k[1] < o.k[1];
+    if (e != o.e) return e < o.e;
+    return l < o.l;
+  }
+};
+}  // namespace
+
+dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TriSetup& S,
+                          std::string& err) {
+  if (!m || !law || !opt || !m->vids || !m->xyz || m->n_elems <= 0) return failS(err, DG_ERR_ARG, "null mesh/law/options");
+  if (m->dim != 2) return failS(err, DG_ERR_UNSUPPORTED, "only triangles (dim 2) are built");
+  S.E = m->n_elems;
+  S.law = law->law;
+  S.NF = law->law == DG_LAW_SCALAR ? 1 : law->law == DG_LAW_EULER2D ? 4 : -1;
+  if (S.NF < 0) return failS(err, DG_ERR_UNSUPPORTED, "law %d not built for triangles", law->law);
+  if (S.law == DG_LAW_EULER2D && law->flux != DG_FLUX_LF && law->flux != DG_FLUX_ROE)
+    return failS(err, DG_ERR_UNSUPPORTED, "Euler flux %d not built", law->flux);
+  if (S.law == DG_LAW_SCALAR && law->flux != DG_FLUX_UPWIND)
+    return failS(err, DG_ERR_UNSUPPORTED, "scalar flux %d not built", law->flux);
+  int P = opt->order;
+  if (m->order) {
+    P = m->order[0];
+    for (int e = 1; e < S.E; e++)
+      if (m->order[e] != P) return failS(err, DG_ERR_UNSUPPORTED, "mixed orders are not built in this entry point");
+  }
+  if (P < 1 || P > MAXP_TRI) return failS(err, DG_ERR_UNSUPPORTED, "order %d outside 1..%d", P, MAXP_TRI);
+  if (opt->precision != DG_PREC_FP64 && opt->precision != DG_PREC_FP32) return failS(err, DG_ERR_ARG, "bad precision");
+  S.P = P;
+  S.NP = np_tri(P);
+  S.lp.gamma = law->gamma;
+  for (int i = 0; i < 3; i++) S.lp.vel[i] = law->vel[i];
+  for (int i = 0; i < 8; i++) S.lp.farfield[i] = law->farfield[i];
+  S.lp.flux = law->flux;
+  const int E = S.E, NP = S.NP;
+  S.xyz.assign(m->xyz, m->xyz + (size_t)E * 6);
+
+  // ---- connectivity (mMesh.cc:321-378, translation-periodic by vertex ids) ----
+  std::vector<FaceKey> keys((size_t)E * 3);
+  for (int e = 0; e < E; e++)
+    for (int l = 0; l < 3; l++) {
+      int32_t a = m->vids[e * 3 + l], b = m->vids[e * 3 + (l + 1) % 3];
+      if (a == b) return failS(err, DG_ERR_ARG, "degenerate edge in element %d", e);
+      FaceKey& k = keys[(size_t)e * 3 + l];
+      k.k[0] = std::min(a, b);
+      k.k[1] = std::max(a, b);
+      k.e = e;
+      k.l = l;
+    }
+  std::sort(keys.begin(), keys.end());
+  std::vector<int32_t> nbe((size_t)E * 3, -1), nbl((size_t)E * 3, -1), same((size_t)E * 3, 0);
+  for (size_t i = 0; i < keys.size();) {
+    size_t j = i + 1;
+    while (j < keys.size() && keys[j].k[0] == keys[i].k[0] && keys[j].k[1] == keys[i].k[1]) j++;
+    if (j - i > 2) return failS(err, DG_ERR_ARG, "edge (%d,%d) shared by %zu elements", keys[i].k[0], keys[i].k[1], j - i);
+    const FaceKey& L = keys[i];
+    if (j - i == 2) {
+      const FaceKey& Rk = keys[i + 1];
+      nbe[L.e * 3 + L.l] = Rk.e;
+      nbl[L.e * 3 + L.l] = Rk.l;
+      nbe[Rk.e * 3 + Rk.l] = L.e;
+      nbl[Rk.e * 3 + Rk.l] = L.l;
+      int s = m->vids[L.e * 3 + L.l] == m->vids[Rk.e * 3 + Rk.l];
+      same[L.e * 3 + L.l] = same[Rk.e * 3 + Rk.l] = s;
+      S.faces.insert(S.faces.end(), {L.e, L.l, Rk.e, Rk.l});
+    } else {
+      S.faces.insert(S.faces.end(), {L.e, L.l, -1, -1});
+    }
+    i = j;
+  }
+
+  if (m->bc)
+    for (int e = 0; e < E; e++)
+      for (int l = 0; l < 3; l++)
+        if (nbe[e * 3 + l] < 0 && m->bc[e * 3 + l] == DG_TAG_CHARACTERISTIC)
+          return failS(err, DG_ERR_UNSUPPORTED, "characteristic far-field boundary (tag 30000) is not built");
+
+  // ---- internal order: group elements whose face links share (nb local face,
+  //      orientation, boundary code) so a wavefront takes one path ------------
+  auto sig = [&](int e) {
+    int s = 0;
+    for (int l = 0; l < 3; l++) {
+      int v;
+      if (nbe[e * 3 + l] >= 0)
+        v = nbl[e * 3 + l] | (same[e * 3 + l] << 2);
+      else {
+        int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
+        v = 8 + (tag == DG_TAG_WALL ? BC_WALL : BC_FARFIELD);
+      }
+      s = s * 16 + v;
+    }
+    return s;
+  };
+  S.perm.resize(E);
+  for (int e = 0; e < E; e++) S.perm[e] = e;
+  std::vector<int> sg(E);
+  for (int e = 0; e < E; e++) sg[e] = sig(e);
+  std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) { return sg[a] < sg[b]; });
+  std::vector<int32_t> iperm(E);
+  for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
+
+  // ---- geometry (Mapping.cc:442-566, 687-712; DGCell.cc:1209-1229) ---------------
+  S.mg.assign((size_t)4 * E, 0.0);
+  S.detj.assign(E, 0.0);
+  S.size.assign(E, 0.0);
+  S.link.assign((size_t)3 * E, 0);
+  std::vector<double>& h_mg = S.mg;
+  std::vector<double>& h_detj = S.detj;
+  std::vector<double>& h_size = S.size;
+  std::vector<int32_t>& h_link = S.link;
+  for (int i = 0; i < E; i++) {
+    int e = S.perm[i];
+    const double* X = m->xyz + (size_t)e * 6;
+    double J00 = X[2] - X[0], J01 = X[4] - X[0];  // dx/du, dx/dv
+    double J10 = X[3] - X[1], J11 = X[5] - X[1];  // dy/du, dy/dv
+    double det = J00 * J11 - J01 * J10;
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return failS(err, DG_ERR_ARG, "degenerate element %d", e);
+    double ad = std::fabs(det);
+    // |J| * (J^-1)[r][d]
+    double inv[2][2] = {{J11 / det, -J01 / det}, {-J10 / det, J00 / det}};
+    for (int r = 0; r < 2; r++)
+      for (int d = 0; d < 2; d++) h_mg[(size_t)(r * 2 + d) * E + i] = ad * inv[r][d];
+    h_detj[i] = ad;
+    double a = std::sqrt((X[2] - X[0]) * (X[2] - X[0]) + (X[3] - X[1]) * (X[3] - X[1]));
+    double b = std::sqrt((X[4] - X[2]) * (X[4] - X[2]) + (X[5] - X[3]) * (X[5] - X[3]));
+    double cc = std::sqrt((X[0] - X[4]) * (X[0] - X[4]) + (X[1] - X[5]) * (X[1] - X[5]));
+    double hp = 0.5 * (a + b + cc);
+    h_size[i] = std::sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
+    for (int l = 0; l < 3; l++) {
+      int32_t v;
+      if (nbe[e * 3 + l] >= 0)
+        v = (iperm[nbe[e * 3 + l]] << 3) | (same[e * 3 + l] << 2) | nbl[e * 3 + l];
+      else {
+        int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
+        v = -(tag == DG_TAG_WALL ? BC_WALL : BC_FARFIELD);
+      }
+      h_link[(size_t)l * E + i] = v;
+    }
+  }
+
+  // ---- tables ---------------------------------------------------------------------
+  S.tab.assign(TriBlock::size(P), 0.0);
+  try {
+    build_tri_block(P, S.tab.data());
+  } catch (const std::exception& ex) {
+    return failS(err, DG_ERR_ARG, "%s", ex.what());
+  }
+  S.tabf.assign(S.tab.begin(), S.tab.end());
+  S.vtx.assign((size_t)4 * NP, 0.0);
+  std::vector<double>& h_vtx = S.vtx;
+  {
+    double z[3] = {0, 0, 0}, phi[64];
+    basis(2, 0, z, phi, nullptr);
+    h_vtx[0] = phi[0];  // computeMean: coefficient 0 times phi_0 (DGCell.cc:1133-1137)
+    static const double V[3][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+    for (int v = 0; v < 3; v++) {
+      basis(2, P, V[v], phi, nullptr);
+      for (int k = 0; k < NP; k++) h_vtx[(1 + v) * NP + k] = phi[k];
+    }
+  }
+
+  return DG_OK;
+}
+
+}  // namespace dgp

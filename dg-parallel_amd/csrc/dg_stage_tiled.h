@@ -1,0 +1,311 @@
+// Tiled residual + RK stage for triangles, written as a sequence of phases
+// separated by workgroup barriers.  Each phase is a __host__ __device__ function
+// of (thread id, LDS tile), so the same code runs in the HIP kernel
+// (dg_stage_tiled.hip) and, thread by thread, in a CPU emulation used by the
+// tests.
+//
+// One workgroup (4 wavefronts) owns a tile of 64 elements, one element per
+// lane.  Every phase maps a wave-uniform work index (quadrature point, face,
+// output row) to the wave and the element to the lane, so basis tables are
+// scalar loads and LDS accesses are lane-contiguous.
+//   A  stage input tile, |J| J^-1 and face links -> LDS
+//   B  volume points (wave = point q): interpolate, physical flux, contravariant
+//      flux G = w_q |J| J^-1 F -> LDS                      (DGCell.cc:193-236)
+//   D1 output rows (wave = row f*NP+k): R = sum_q dphi_k(q) . G(q)   (registers)
+//   C1 neighbour traces (wave = (face, field)): coefficients of the neighbour
+//      read from the stage input in global memory (L2-resident), evaluated at
+//      the face points in the neighbour's frame -> LDS
+//   C2 face points (wave = (face, point)): own trace, numerical flux, weight
+//      -> LDS                                               (DGCell.cc:647-705)
+//   D2 output rows: R -= sum_{l,q} phi_k(l,q) Fhat(l,q)        (DGCell.cc:709-728)
+//      then the stage update with the FP64 step-start state and accumulator
+// Each face flux is computed by both of its elements (no face buffer, no
+// atomics; results independent of scheduling).
+#pragma once
+
+#include "dg_kernels.h"
+#include "dg_laws.h"
+#include "dg_tables.h"
+
+namespace dgp {
+
+constexpr int TILE_THREADS = 256;
+constexpr int TILE_ELEMS = 64;
+
+DG_HD int wave_uniform(int x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_readfirstlane(x);
+#else
+  return x;
+#endif
+}
+
+DG_HD double stage_value_hd(int sc, double v, double un, double us, double* acc, double& rsum) {
+  double o;
+  switch (sc) {
+    case 3 * 4 + 0:  // RK4 (TimeIntegrator.cc:101-137)
+      *acc = v;
+      o = un + 0.5 * v;
+      break;
+    case 3 * 4 + 1:  // (:149-181)
+      *acc += 2. * v;
+      o = un + 0.5 * v;
+      break;
+    case 3 * 4 + 2:  // (:192-222)
+      *acc += 2. * v;
+      o = un + v;
+      break;
+    case 3 * 4 + 3: {  // (:235-270)
+      const double s = *acc + v;
+      o = un + s / 6.0;
+      rsum += s * s / 36.0;
+      break;
+    }
+    case 2 * 4 + 0:  // SSP-RK3 (:850-880)
+      o = un + v;
+      break;
+    case 2 * 4 + 1:  // (:889-925)
+      o = 0.25 * (us + v + 3. * un);
+      break;
+    case 2 * 4 + 2: {  // (:934-987)
+      o = (2.0 * (us + v) + un) / 3.0;
+      const double d = o - un;
+      rsum += d * d;
+      break;
+    }
+    case 1 * 4 + 0:  // SSP-RK2 (parallel.cc:78-131)
+      o = un + v;
+      break;
+    case 1 * 4 + 1: {  // (parallel.cc:133-186)
+      o = (us + v + un) * 0.5;
+      const double d = o - un;
+      rsum += d * d;
+      break;
+    }
+    default: {  // forward Euler (TimeIntegrator.cc:993-1048)
+      o = un + v;
+      const double d = un - o;
+      rsum += d * d;
+      break;
+    }
+  }
+  return o;
+}
+
+template <int P, class LAW, typename CT, typename IT, typename OT>
+struct TiledStage {
+  static constexpr int NP = np_tri(P), NQ = nqv_tri(P), NQF = nqf_tri(P), NF = LAW::NF;
+  static constexpr int T = TILE_ELEMS;
+  static constexpr int NROW = NF * NP;
+  static constexpr int RPW = (NROW + 3) / 4;
+  static constexpr int GW = NQ * NF * 2;   // volume-flux rows
+  static constexpr int FW = 3 * NQF * NF;  // trace / face-flux rows
+  static constexpr int WORK = GW > 2 * FW ? GW : 2 * FW;
+
+  struct Smem {
+    CT Us[NROW][T];
+    CT Ms[4][T];
+    int Ls[3][T];
+    CT W[WORK][T];
+    double red[TILE_THREADS];
+  };
+
+  struct Regs {
+    CT R[RPW];
+    double rsum;
+  };
+
+  DG_HD static const CT* table(const StageArgs& a) {
+    return sizeof(CT) == 8 ? (const CT*)(const void*)a.tab : (const CT*)(const void*)a.tabf;
+  }
+
+  // ---- A ---------------------------------------------------------------------
+  DG_HD static void phaseA(const StageArgs& a, Smem& s, int tid, int tile) {
+    const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+    const int E = a.E, e = tile * T + t;
+    const bool valid = e < E;
+    const IT* Uin = (const IT*)a.Uin;
+    for (int r = wave; r < NROW; r += 4) s.Us[r][t] = valid ? CT(Uin[(long)r * E + e]) : CT(r == 0 ? 1 : 0);
+    if (wave < 4) s.Ms[wave][t] = valid ? CT(a.el.mg[(long)wave * E + e]) : CT(wave == 0 || wave == 3 ? 1 : 0);
+    if (wave < 3) s.Ls[wave][t] = valid ? a.el.link[(long)wave * E + e] : -BC_FARFIELD;
+  }
+
+  // ---- B ---------------------------------------------------------------------
+  DG_HD static void phaseB(const StageArgs& a, Smem& s, int tid) {
+    const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+    const CT* tb = table(a);
+    for (int q = wave; q < NQ; q += 4) {
+      CT u[NF];
+      for (int f = 0; f < NF; f++) {
+        CT acc = CT(0);
+#pragma unroll
+        for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phi(P) + q * NP + k] * s.Us[f * NP + k][t];
+        u[f] = acc;
+      }
+      CT F[NF][2];
+      LAW::flux(a.law, u, F);
+      const CT w = tb[TriBlock::off_wv(P) + q];
+      const CT m00 = s.Ms[0][t], m01 = s.Ms[1][t], m10 = s.Ms[2][t], m11 = s.Ms[3][t];
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        s.W[(q * NF + f) * 2 + 0][t] = w * (m00 * F[f][0] + m01 * F[f][1]);
+        s.W[(q * NF + f) * 2 + 1][t] = w * (m10 * F[f][0] + m11 * F[f][1]);
+      }
+    }
+  }
+
+  // ---- D1 --------------------------------------------------------------------
+  DG_HD static void phaseD1(const StageArgs& a, Smem& s, int tid, Regs& g) {
+    const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+    const CT* tb = table(a);
+#pragma unroll
+    for (int j = 0; j < RPW; j++) {
+      const int row = wave + 4 * j;
+      CT acc = CT(0);
+      if (row < NROW) {
+        const int f = row / NP, k = row - f * NP;
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+          acc += tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 0] * s.W[(q * NF + f) * 2 + 0][t] +
+                 tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 1] * s.W[(q * NF + f) * 2 + 1][t];
+      }
+      g.R[j] = acc;
+    }
+  }
+
+  // ---- C1 --------------------------------------------------------------------
+  template <int LB, bool SAME>
+  DG_HD static void nb_trace(const StageArgs& a, int nb, int f, CT* out) {
+    constexpr int OFF = TriBlock::off_phif(P);
+    const CT* tb = table(a);
+    const IT* Uin = (const IT*)a.Uin;
+    CT acc[NQF];
+#pragma unroll
+    for (int q = 0; q < NQF; q++) acc[q] = CT(0);
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      const CT v = CT(Uin[(long)(f * NP + k) * a.E + nb]);
+#pragma unroll
+      for (int q = 0; q < NQF; q++) {
+        const int qq = SAME ? q : NQF - 1 - q;
+        acc[q] += tb[OFF + (LB * NQF + qq) * NP + k] * v;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NQF; q++) out[q] = acc[q];
+  }
+
+  DG_HD static void phaseC1(const StageArgs& a, Smem& s, int tid, int tile) {
+    const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+    const bool valid = tile * T + t < a.E;
+    for (int it = wave; it < 3 * NF; it += 4) {
+      const int l = it / NF, f = it - l * NF;
+      const int link = s.Ls[l][t];
+      CT out[NQF];
+      if (valid && link >= 0) {
+        const int nb = link >> 3, lb = link & 3;
+        const bool same = (link >> 2) & 1;
+        if (!same) {
+          if (lb == 0)
+            nb_trace<0, false>(a, nb, f, out);
+          else if (lb == 1)
+            nb_trace<1, false>(a, nb, f, out);
+          else
+            nb_trace<2, false>(a, nb, f, out);
+        } else {
+          if (lb == 0)
+            nb_trace<0, true>(a, nb, f, out);
+          else if (lb == 1)
+            nb_trace<1, true>(a, nb, f, out);
+          else
+            nb_trace<2, true>(a, nb, f, out);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < NQF; q++) out[q] = CT(0);
+      }
+#pragma unroll
+      for (int q = 0; q < NQF; q++) s.W[(l * NQF + q) * NF + f][t] = out[q];
+    }
+  }
+
+  // ---- C2 --------------------------------------------------------------------
+  DG_HD static void phaseC2(const StageArgs& a, Smem& s, int tid) {
+    const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+    const CT* tb = table(a);
+    for (int it = wave; it < 3 * NQF; it += 4) {
+      const int l = it / NQF, q = it - l * NQF;
+      const int link = s.Ls[l][t];
+      // outward normal x edge length = |J| J^-T nhat |fhat|, with nhat |fhat| of the
+      // reference edges (0,-1), (1,1), (-1,0)
+      CT nl0, nl1;
+      if (l == 0) {
+        nl0 = -s.Ms[2][t];
+        nl1 = -s.Ms[3][t];
+      } else if (l == 1) {
+        nl0 = s.Ms[0][t] + s.Ms[2][t];
+        nl1 = s.Ms[1][t] + s.Ms[3][t];
+      } else {
+        nl0 = -s.Ms[0][t];
+        nl1 = -s.Ms[1][t];
+      }
+      const CT len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
+      const CT n[2] = {nl0 / len, nl1 / len};
+      CT uL[NF], uR[NF], Fh[NF];
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        CT acc = CT(0);
+#pragma unroll
+        for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * s.Us[f * NP + k][t];
+        uL[f] = acc;
+        uR[f] = s.W[(l * NQF + q) * NF + f][t];
+      }
+      if (link >= 0)
+        LAW::riemann(a.law, n, uL, uR, Fh);
+      else
+        LAW::boundary(a.law, -link, n, uL, Fh);
+      // |J_face| w_q = (len / 2) w_q; the table holds w_q / 2
+      const CT sw = len * tb[TriBlock::off_wf(P) + q];
+#pragma unroll
+      for (int f = 0; f < NF; f++) s.W[FW + (l * NQF + q) * NF + f][t] = Fh[f] * sw;
+    }
+  }
+
+  // ---- D2 + stage update ----------------------------------------------------
+  DG_HD static void phaseD2(const StageArgs& a, Smem& s, int tid, int tile, Regs& g) {
+    const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+    const int E = a.E, e = tile * T + t;
+    const bool valid = e < E;
+    const CT* tb = table(a);
+    const IT* Uin = (const IT*)a.Uin;
+    const int sc = a.scheme * 4 + a.stage;
+    const double dtj = valid ? a.dt / a.el.detj[e] : 0.0;
+    g.rsum = 0.0;
+#pragma unroll
+    for (int j = 0; j < RPW; j++) {
+      const int row = wave + 4 * j;
+      if (row < NROW) {
+        const int f = row / NP, k = row - f * NP;
+        CT acc = g.R[j];
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+#pragma unroll
+          for (int q = 0; q < NQF; q++)
+            acc -= tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * s.W[FW + (l * NQF + q) * NF + f][t];
+        if (valid) {
+          const long i = (long)row * E + e;
+          if (a.scheme == MODE_RESIDUAL) {
+            a.R[i] = double(acc);
+          } else {
+            const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
+            const double us = reads_input ? double(Uin[i]) : 0.0;
+            const double o = stage_value_hd(sc, double(acc) * dtj, a.Un[i], us, a.acc + i, g.rsum);
+            ((OT*)a.Uout)[i] = OT(o);
+          }
+        }
+      }
+    }
+  }
+};
+
+}  // namespace dgp

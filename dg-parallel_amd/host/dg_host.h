@@ -1,0 +1,200 @@
+// Host C++ mirror of the reference's call surface for the DG hot path:
+//   ConservationLaw and its Euler / Scalar / Maxwell / Burgers subclasses
+//     (ConservationLaw.h:12-60, EulerLaw.h, ScalarLaw.h, MaxwellLaw.h, Burgers.h),
+//   DGAnalysis (DGAnalysis.h:110-135),
+//   TimeIntegrator::advanceInTime and its RK subclasses (TimeIntegrator.h:8-91).
+// The same method names and argument meanings; the work is done on the GPU
+// through the C ABI (include/dg_capi.h).  The law classes keep their pointwise
+// methods (Fi, riemannSolver, boundaryFlux, maximumEigenValue, isPhysical)
+// evaluated on the host with the same arithmetic the kernels use, and describe
+// themselves to the device as a dg_law_desc -- there is no virtual dispatch on
+// the device.
+#pragma once
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dg_capi.h"
+
+namespace dgpar {
+
+// point and vector in the reference's sense (mPoint.h, mVector.h)
+struct mPoint {
+  double p[3] = {0, 0, 0};
+  mPoint() = default;
+  mPoint(double x, double y, double z = 0) : p{x, y, z} {}
+  double operator()(int i) const { return p[i]; }
+  double& operator()(int i) { return p[i]; }
+};
+
+struct mVector {
+  double v[3] = {0, 0, 0};
+  mVector() = default;
+  mVector(double x, double y, double z = 0) : v{x, y, z} {}
+  double operator()(int i) const { return v[i]; }
+  double& operator()(int i) { return v[i]; }
+  double operator*(const mVector& o) const { return v[0] * o.v[0] + v[1] * o.v[1] + v[2] * o.v[2]; }
+};
+
+// FieldEvaluator.h:9-14
+class FieldEvaluator {
+ public:
+  virtual ~FieldEvaluator() = default;
+  virtual void eval(const mPoint& space, double time, double* val) const = 0;
+  virtual int order() const { return 0; }
+};
+
+class ConstantField : public FieldEvaluator {
+  std::vector<double> v_;
+
+ public:
+  explicit ConstantField(std::vector<double> v) : v_(std::move(v)) {}
+  void eval(const mPoint&, double, double* val) const override {
+    for (size_t i = 0; i < v_.size(); i++) val[i] = v_[i];
+  }
+};
+
+class DGError : public std::runtime_error {
+ public:
+  dg_status status;
+  DGError(dg_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+// ---------------------------------------------------------------------------
+class ConservationLaw {
+ protected:
+  FieldEvaluator* farField = nullptr;
+  FieldEvaluator* exactField = nullptr;
+
+ public:
+  virtual ~ConservationLaw() = default;
+  FieldEvaluator* getFarField() { return farField; }
+  FieldEvaluator* getExactSolution() { return exactField; }
+  virtual int getNbFields() const = 0;
+  virtual void Fi(const mPoint& position, double* Q, mVector* flux) const = 0;
+  virtual void RHS(double*, const mPoint&, double, double* rhs) const {
+    for (int i = 0; i < getNbFields(); i++) rhs[i] = 0.0;
+  }
+  virtual void riemannSolver(mVector& n, const mPoint& position, double* left, double* right, double* flux) const = 0;
+  virtual void boundaryFlux(mVector& n, int BoundaryId, const mPoint& position, double* field, double* flux,
+                            double T) const = 0;
+  virtual double maximumEigenValue(const double* field, const mPoint& position) const = 0;
+  virtual bool isPhysical(double* val) const { return true; }
+  // the device-side description of this law (no counterpart in the reference)
+  virtual dg_law_desc describe() const = 0;
+};
+
+// ScalarLaw (ScalarLaw.cc:19-97) with a constant velocity field
+class ScalarLaw : public ConservationLaw {
+  double vel_[3];
+
+ public:
+  ScalarLaw(FieldEvaluator* far, const double vel[3], FieldEvaluator* exact = nullptr);
+  int getNbFields() const override { return 1; }
+  void Fi(const mPoint&, double* Q, mVector* flux) const override;
+  void riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* flux) const override;
+  void boundaryFlux(mVector& n, int, const mPoint&, double* Q, double* flux, double) const override;
+  double maximumEigenValue(const double*, const mPoint&) const override;
+  dg_law_desc describe() const override;
+};
+
+// Euler2d (EulerLaw.cc:47-69, 182-296, 1012-1034, 1398-1404); the numerical flux
+// is local Lax-Friedrichs here, Roe in Euler2dRoe
+class Euler2d : public ConservationLaw {
+ protected:
+  double gamma;
+  int flux_kind = DG_FLUX_LF;
+
+ public:
+  Euler2d(FieldEvaluator* far, double Gamma = 1.4);
+  int getNbFields() const override { return 4; }
+  void Fi(const mPoint&, double* Q, mVector* flux) const override;
+  void riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* flux) const override;
+  void boundaryFlux(mVector& n, int BoundaryId, const mPoint&, double* Q, double* flux, double T) const override;
+  double maximumEigenValue(const double* Q, const mPoint&) const override;
+  bool isPhysical(double* Q) const override;
+  dg_law_desc describe() const override;
+};
+
+class Euler2dRoe : public Euler2d {
+ public:
+  Euler2dRoe(FieldEvaluator* far, double Gamma = 1.4) : Euler2d(far, Gamma) { flux_kind = DG_FLUX_ROE; }
+};
+
+// ---------------------------------------------------------------------------
+// A mesh handed to DGAnalysis (the path's view of mDGMesh)
+struct Mesh {
+  int dim = 2;
+  std::vector<int32_t> vids, bc, order;
+  std::vector<double> xyz;
+  int n_elems() const { return (int)(vids.size() / (dim + 1)); }
+  static Mesh structuredTri(int nx, int ny, double x0, double y0, double lx, double ly, bool periodic_x,
+                            bool periodic_y, double jitter, uint64_t seed, const int32_t side_tags[4]);
+};
+
+class DGAnalysis;
+
+// TimeIntegrator (TimeIntegrator.h:8-29): advanceInTime returns the residual norm
+class TimeIntegrator {
+ protected:
+  DGAnalysis* an_;
+  int scheme_;
+
+ public:
+  TimeIntegrator(DGAnalysis* a, int scheme) : an_(a), scheme_(scheme) {}
+  virtual ~TimeIntegrator() = default;
+  virtual double advanceInTime(double t, double dt);
+  int scheme() const { return scheme_; }
+};
+class ForwardEuler : public TimeIntegrator {
+ public:
+  explicit ForwardEuler(DGAnalysis* a) : TimeIntegrator(a, DG_RK_FE) {}
+};
+class RungeKuttaTVD2 : public TimeIntegrator {
+ public:
+  explicit RungeKuttaTVD2(DGAnalysis* a) : TimeIntegrator(a, DG_RK_TVD2) {}
+};
+class RungeKuttaTVD3 : public TimeIntegrator {
+ public:
+  explicit RungeKuttaTVD3(DGAnalysis* a) : TimeIntegrator(a, DG_RK_TVD3) {}
+};
+class RungeKutta4 : public TimeIntegrator {
+ public:
+  explicit RungeKutta4(DGAnalysis* a) : TimeIntegrator(a, DG_RK4) {}
+};
+
+// DGAnalysis (DGAnalysis.h:110-135): owns the GPU context and the integrator,
+// chosen from the order as the reference does (DGAnalysis.cc:239-242).
+class DGAnalysis {
+  dg_ctx* ctx_ = nullptr;
+  ConservationLaw* law_;
+  TimeIntegrator* integrator_ = nullptr;
+  int order_;
+
+ public:
+  double TACT = 0.0, TEND = 0.2, CFLMAX = 0.5;
+
+  DGAnalysis(const Mesh& mesh, ConservationLaw* law, void* limiter, int order, int precision = DG_PREC_FP64);
+  ~DGAnalysis();
+  DGAnalysis(const DGAnalysis&) = delete;
+  DGAnalysis& operator=(const DGAnalysis&) = delete;
+
+  dg_ctx* context() { return ctx_; }
+  TimeIntegrator* integrator() { return integrator_; }
+  size_t ndof() const { return dg_ndof(ctx_); }
+  void check(dg_status s) const;
+
+  void setState(const std::vector<double>& U);
+  std::vector<double> getState() const;
+  // L2 projection of a built-in exact field (DGCell::L2ProjInitial)
+  void project(int field, const double* par, double t = 0.0);
+  std::vector<double> residual(double t = 0.0);
+  double computeDt(double cfl);
+  // the reference's time loop (commented out in DGAnalysis.cc:480-731): dt from the
+  // CFL rule, N = ceil((TEND-TACT)/dt) steps of dt = (TEND-TACT)/N; returns N
+  int run();
+};
+
+}  // namespace dgpar

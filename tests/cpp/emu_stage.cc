@@ -1,0 +1,164 @@
+// CPU emulation of the tiled triangle stage kernel: runs the kernel's own phase
+// functions (dg-parallel_amd/csrc/dg_stage_tiled.h) thread by thread on the host,
+// with the same host-built tables the library uploads.  Test infrastructure:
+// lets the CPU test suite check the kernel arithmetic and indexing against the
+// oracle without a GPU.
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../dg-parallel_amd/csrc/dg_setup.h"
+#include "../../dg-parallel_amd/csrc/dg_stage_tiled.h"
+
+using namespace dgp;
+
+namespace {
+
+template <int P, class LAW, typename CT, typename IT, typename OT>
+void run_tiles(const StageArgs& a) {
+  using K = TiledStage<P, LAW, CT, IT, OT>;
+  std::unique_ptr<typename K::Smem> sm(new typename K::Smem());
+  std::vector<typename K::Regs> g(TILE_THREADS);
+  const int ntiles = (a.E + TILE_ELEMS - 1) / TILE_ELEMS;
+  for (int tile = 0; tile < ntiles; tile++) {
+    std::memset(sm.get(), 0, sizeof(typename K::Smem));
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, *sm, t, tile);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, *sm, t);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseD1(a, *sm, t, g[t]);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, *sm, t, tile);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t);
+    double r = 0;
+    for (int t = 0; t < TILE_THREADS; t++) {
+      K::phaseD2(a, *sm, t, tile, g[t]);
+      r += g[t].rsum;
+    }
+    if (a.resid_part) a.resid_part[tile] = r;
+  }
+}
+
+template <class LAW, typename CT, typename IT, typename OT>
+int run_law(const StageArgs& a) {
+  switch (a.P) {
+    case 1: run_tiles<1, LAW, CT, IT, OT>(a); return 0;
+    case 2: run_tiles<2, LAW, CT, IT, OT>(a); return 0;
+    case 3: run_tiles<3, LAW, CT, IT, OT>(a); return 0;
+    case 4: run_tiles<4, LAW, CT, IT, OT>(a); return 0;
+  }
+  return -1;
+}
+
+template <typename CT, typename IT, typename OT>
+int run(const StageArgs& a, int law) {
+  if (law == LAW_SCALAR) return run_law<ScalarLawD, CT, IT, OT>(a);
+  if (law == LAW_EULER2D) return run_law<EulerLawD, CT, IT, OT>(a);
+  return -1;
+}
+
+int run_variant(const StageArgs& a, int law, int variant) {
+  switch (variant) {
+    case 0: return run<double, double, double>(a, law);
+    case 1: return run<float, double, float>(a, law);
+    case 2: return run<float, float, float>(a, law);
+    case 3: return run<float, float, double>(a, law);
+  }
+  return -1;
+}
+
+struct Emu {
+  TriSetup S;
+  std::vector<double> Un, acc, R, A64, B64, part;
+  std::vector<float> A32, B32;
+  StageArgs base() {
+    StageArgs a{};
+    a.E = S.E;
+    a.P = S.P;
+    a.tab = S.tab.data();
+    a.tabf = S.tabf.data();
+    a.el.mg = S.mg.data();
+    a.el.detj = S.detj.data();
+    a.el.size = S.size.data();
+    a.el.link = S.link.data();
+    a.law = S.lp;
+    a.Un = Un.data();
+    a.acc = acc.data();
+    return a;
+  }
+  void to_soa(const double* U) {
+    const int BS = S.NF * S.NP;
+    for (int i = 0; i < S.E; i++)
+      for (int j = 0; j < BS; j++) Un[(size_t)j * S.E + i] = U[(size_t)S.perm[i] * BS + j];
+  }
+  void from_soa(const double* src, double* U) {
+    const int BS = S.NF * S.NP;
+    for (int i = 0; i < S.E; i++)
+      for (int j = 0; j < BS; j++) U[(size_t)S.perm[i] * BS + j] = src[(size_t)j * S.E + i];
+  }
+};
+
+int make(Emu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int order) {
+  dg_options opt{};
+  opt.order = order;
+  std::string err;
+  if (build_tri_setup(mesh, law, &opt, m.S, err) != DG_OK) return -1;
+  size_t n = (size_t)m.S.E * m.S.NF * m.S.NP;
+  m.Un.assign(n, 0.0);
+  m.acc.assign(n, 0.0);
+  m.R.assign(n, 0.0);
+  m.A64.assign(n, 0.0);
+  m.B64.assign(n, 0.0);
+  m.A32.assign(n, 0.0f);
+  m.B32.assign(n, 0.0f);
+  m.part.assign((m.S.E + TILE_ELEMS - 1) / TILE_ELEMS + 1, 0.0);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int emu_tri_residual(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int fp32,
+                                const double* U, double* R) {
+  Emu m;
+  if (make(m, mesh, law, order)) return -1;
+  m.to_soa(U);
+  StageArgs a = m.base();
+  a.Uin = m.Un.data();
+  a.R = m.R.data();
+  a.scheme = MODE_RESIDUAL;
+  if (run_variant(a, m.S.law, fp32 ? 1 : 0)) return -2;
+  m.from_soa(m.R.data(), R);
+  return 0;
+}
+
+// nsteps steps of the given scheme in place, with the stage / buffer sequence of
+// the library (dg_capi.cc enqueue_step); resid: last step's residual norm
+extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int fp32, int scheme,
+                             double dt, int nsteps, double* U, double* resid) {
+  Emu m;
+  if (make(m, mesh, law, order)) return -1;
+  m.to_soa(U);
+  const int ns = scheme == 0 ? 1 : scheme == 1 ? 2 : scheme == 2 ? 3 : 4;
+  if (fp32 && ns == 1) return -3;
+  void* bufA = fp32 ? (void*)m.A32.data() : (void*)m.A64.data();
+  void* bufB = fp32 ? (void*)m.B32.data() : (void*)m.B64.data();
+  for (int s = 0; s < nsteps; s++) {
+    StageArgs a = m.base();
+    a.dt = dt;
+    a.scheme = scheme;
+    for (int st = 0; st < ns; st++) {
+      a.stage = st;
+      const bool first = st == 0, last = st == ns - 1;
+      a.Uin = first ? (const void*)m.Un.data() : ((st % 2) ? bufA : bufB);
+      a.Uout = last ? (void*)m.Un.data() : ((st % 2) ? bufB : bufA);
+      if (first && last) a.Uout = bufA;
+      a.resid_part = last ? m.part.data() : nullptr;
+      int variant = fp32 ? (first ? 1 : (last ? 3 : 2)) : 0;
+      if (run_variant(a, m.S.law, variant)) return -2;
+    }
+    if (ns == 1) m.Un = m.A64;
+  }
+  double r = 0;
+  for (int i = 0; i < (m.S.E + TILE_ELEMS - 1) / TILE_ELEMS; i++) r += m.part[i];
+  if (resid) *resid = r;
+  m.from_soa(m.Un.data(), U);
+  return 0;
+}

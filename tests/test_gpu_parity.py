@@ -24,7 +24,7 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-def make_pair(law, flux, p, nx=8, periodic=True, jitter=0.1, seed=3, precision=0, use_graph=True):
+def make_pair(law, flux, p, nx=8, periodic=True, jitter=0.1, seed=3, precision=0, use_graph=True, kernel=0):
     import dgpar
     if law == "scalar":
         vids, xyz, bc = mesh_tri(nx, nx, periodic_x=periodic, periodic_y=periodic, jitter=jitter, seed=seed,
@@ -42,33 +42,39 @@ def make_pair(law, flux, p, nx=8, periodic=True, jitter=0.1, seed=3, precision=0
                        farfield=FREESTREAM)
         U = o.project(1, VORTEX)
     mesh = dgpar.Mesh(2, vids, xyz, bc)
-    g = dgpar.DGAnalysis(mesh, lw, p, precision=precision, use_graph=use_graph)
+    g = dgpar.DGAnalysis(mesh, lw, p, precision=precision, use_graph=use_graph, kernel=kernel)
     g.set_state(U)
     return g, o, U
 
 
+KERNELS = [0, 1]  # tiled LDS-staged kernel, one element per work-item
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("p", [1, 2, 3, 4])
-def test_scalar_residual(p):
-    announce("scalar residual P%d" % p)
-    g, o, U = make_pair("scalar", None, p)
+def test_scalar_residual(p, kernel):
+    announce("scalar residual P%d kernel %d" % (p, kernel))
+    g, o, U = make_pair("scalar", None, p, kernel=kernel)
     Rg = g.residual()
     Ro, _ = o.residual(U)
     assert rel(Rg, Ro) < 1e-12
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("flux", ["lf", "roe"])
 @pytest.mark.parametrize("p", [1, 2, 3, 4])
-def test_euler_residual_periodic(p, flux):
-    announce("euler %s residual P%d periodic" % (flux, p))
-    g, o, U = make_pair("euler", flux, p)
+def test_euler_residual_periodic(p, flux, kernel):
+    announce("euler %s residual P%d periodic kernel %d" % (flux, p, kernel))
+    g, o, U = make_pair("euler", flux, p, kernel=kernel)
     assert rel(g.residual(), o.residual(U)[0]) < 1e-12
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("flux", ["lf", "roe"])
 @pytest.mark.parametrize("p", [1, 3])
-def test_euler_residual_walls_and_farfield(p, flux):
-    announce("euler %s residual P%d walls" % (flux, p))
-    g, o, U = make_pair("euler", flux, p, periodic=False)
+def test_euler_residual_walls_and_farfield(p, flux, kernel):
+    announce("euler %s residual P%d walls kernel %d" % (flux, p, kernel))
+    g, o, U = make_pair("euler", flux, p, periodic=False, kernel=kernel)
     assert rel(g.residual(), o.residual(U)[0]) < 1e-12
 
 
@@ -84,9 +90,10 @@ def test_state_roundtrip_and_connectivity():
 @pytest.mark.parametrize("law,flux,p,scheme", [("scalar", None, 2, po.RK_TVD3), ("scalar", None, 1, po.RK_FE),
                                                ("euler", "lf", 3, po.RK4), ("euler", "roe", 2, po.RK_TVD2),
                                                ("euler", "roe", 3, po.RK_TVD3)])
-def test_steps(law, flux, p, scheme):
-    announce("steps %s %s P%d scheme %d" % (law, flux, p, scheme))
-    g, o, U = make_pair(law, flux, p, periodic=(law == "scalar"))
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_steps(law, flux, p, scheme, kernel):
+    announce("steps %s %s P%d scheme %d kernel %d" % (law, flux, p, scheme, kernel))
+    g, o, U = make_pair(law, flux, p, periodic=(law == "scalar"), kernel=kernel)
     dt = o.dt(U, 0.1)
     t = 0.0
     Uo = U
@@ -106,6 +113,14 @@ def test_dt(law, flux, p):
     announce("dt %s P%d" % (law, p))
     g, o, U = make_pair(law, flux, p)
     assert abs(g.computeDt(0.1) - o.dt(U, 0.1)) <= 1e-14 * o.dt(U, 0.1)
+
+
+def test_kernels_agree_bitwise_or_to_rounding():
+    announce("tiled vs per-element kernels")
+    g0, o, U = make_pair("euler", "roe", 3, nx=12, kernel=0)
+    g1, _, _ = make_pair("euler", "roe", 3, nx=12, kernel=1)
+    r0, r1 = g0.residual(), g1.residual()
+    assert rel(r0, r1) < 1e-14
 
 
 def test_deterministic_and_graph_equals_eager():
@@ -131,11 +146,12 @@ def test_deterministic_and_graph_equals_eager():
     assert np.array_equal(g.get_state(), g2.get_state())
 
 
-def test_fp32_form_error_budget():
-    announce("fp32 form budget")
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_fp32_form_error_budget(kernel):
+    announce("fp32 form budget kernel %d" % kernel)
     import dgpar
-    g64, o, U = make_pair("euler", "lf", 3, nx=16)
-    g32, _, _ = make_pair("euler", "lf", 3, nx=16, precision=dgpar.PREC_FP32)
+    g64, o, U = make_pair("euler", "lf", 3, nx=16, kernel=kernel)
+    g32, _, _ = make_pair("euler", "lf", 3, nx=16, precision=dgpar.PREC_FP32, kernel=kernel)
     R64, R32 = g64.residual(), g32.residual()
     assert rel(R32, R64) < 1e-5
     dt = o.dt(U, 0.1)

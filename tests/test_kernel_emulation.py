@@ -1,0 +1,104 @@
+"""CPU emulation of the tiled HIP stage kernel (its own phase functions run
+thread by thread on the host, tests/cpp/emu_stage.cc) against the oracle:
+FP64 residuals and RK steps within 1e-12; the FP32 form within its budget.
+Catches arithmetic and indexing errors in the kernel code before it reaches a
+GPU."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import dgpar
+from helpers import ROOT, VORTEX, mesh_tri
+from oracle import pyoracle as po
+
+LIB = os.path.join(ROOT, "tests", "cpp", "libemu.so")
+pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="emulator not built")
+
+
+def _lib():
+    lib = ctypes.CDLL(LIB)
+    p_md, p_ld = ctypes.POINTER(dgpar.MeshDesc), ctypes.POINTER(dgpar.LawDesc)
+    pd = ctypes.POINTER(ctypes.c_double)
+    lib.emu_tri_residual.argtypes = [p_md, p_ld, ctypes.c_int, ctypes.c_int, pd, pd]
+    lib.emu_tri_steps.argtypes = [p_md, p_ld, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_int, pd, pd]
+    return lib
+
+
+LIBEMU = _lib() if os.path.exists(LIB) else None
+FF = (1.0, 1.0, 1.0, 3.5)
+
+
+def case(law, flux, p, periodic=True, nx=8):
+    tags = (40000, 7, 40000, 7)
+    if law == "scalar":
+        vids, xyz, bc = mesh_tri(nx, nx, periodic_x=periodic, periodic_y=periodic, jitter=0.1, seed=3, side_tags=tags)
+        o = po.Oracle(2, vids, xyz, p, po.LAW_SCALAR, po.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.2,), bc=bc)
+        lw = dgpar.Law(dgpar.LAW_SCALAR, dgpar.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.2,))
+        U = o.project(0)
+    else:
+        vids, xyz, bc = mesh_tri(nx, nx, lx=10, ly=10, periodic_x=periodic, periodic_y=periodic, jitter=0.1, seed=3,
+                                 side_tags=tags)
+        o = po.Oracle(2, vids, xyz, p, po.LAW_EULER2D, po.FLUX_LF if flux == "lf" else po.FLUX_ROE, gamma=1.4,
+                      farfield=FF, bc=bc)
+        lw = dgpar.Law(dgpar.LAW_EULER2D, dgpar.FLUX_LF if flux == "lf" else dgpar.FLUX_ROE, gamma=1.4, farfield=FF)
+        U = o.project(1, VORTEX)
+    return dgpar.Mesh(2, vids, xyz, bc), lw, o, U
+
+
+def emu_residual(mesh, lw, p, U, fp32=0):
+    md, ld = mesh.desc(), lw.desc()
+    R = np.zeros_like(U)
+    U = np.ascontiguousarray(U)
+    pd = ctypes.POINTER(ctypes.c_double)
+    assert LIBEMU.emu_tri_residual(ctypes.byref(md), ctypes.byref(ld), p, fp32, U.ctypes.data_as(pd),
+                                   R.ctypes.data_as(pd)) == 0
+    return R
+
+
+def emu_steps(mesh, lw, p, U, scheme, dt, n, fp32=0):
+    md, ld = mesh.desc(), lw.desc()
+    U = np.array(U, copy=True)
+    r = ctypes.c_double()
+    pd = ctypes.POINTER(ctypes.c_double)
+    assert LIBEMU.emu_tri_steps(ctypes.byref(md), ctypes.byref(ld), p, fp32, scheme, dt, n, U.ctypes.data_as(pd),
+                                ctypes.byref(r)) == 0
+    return U, r.value
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("law,flux,periodic", [("scalar", None, True), ("scalar", None, False), ("euler", "lf", True),
+                                               ("euler", "roe", True), ("euler", "lf", False), ("euler", "roe", False)])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_emulated_residual(law, flux, periodic, p):
+    mesh, lw, o, U = case(law, flux, p, periodic)
+    assert rel(emu_residual(mesh, lw, p, U), o.residual(U)[0]) < 1e-12
+
+
+@pytest.mark.parametrize("law,flux,p,scheme", [("scalar", None, 2, po.RK_TVD3), ("scalar", None, 1, po.RK_FE),
+                                               ("euler", "lf", 3, po.RK4), ("euler", "roe", 2, po.RK_TVD2)])
+def test_emulated_steps(law, flux, p, scheme):
+    mesh, lw, o, U = case(law, flux, p, periodic=(law == "scalar"))
+    dt = o.dt(U, 0.1)
+    Uo, t = U, 0.0
+    for s in range(3):
+        Uo, ro = o.step(Uo, scheme, t, dt)
+        t += dt
+    Ue, re = emu_steps(mesh, lw, p, U, scheme, dt, 3)
+    assert rel(Ue, Uo) < 1e-12
+    assert abs(re - ro) <= 1e-10 * abs(ro)
+
+
+def test_emulated_fp32_form():
+    mesh, lw, o, U = case("euler", "lf", 3, nx=12)
+    R64 = o.residual(U)[0]
+    assert rel(emu_residual(mesh, lw, 3, U, fp32=1), R64) < 1e-5
+    dt = o.dt(U, 0.1)
+    U64, _ = emu_steps(mesh, lw, 3, U, po.RK4, dt, 5)
+    U32, _ = emu_steps(mesh, lw, 3, U, po.RK4, dt, 5, fp32=1)
+    assert rel(U32, U64) < 1e-5
