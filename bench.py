@@ -47,7 +47,6 @@ def bytes_per_dof_stage(fp32, nf, np_):
 
 
 def run_form(dgpar, mesh, precision, steps, warmup, kernel=0):
-    import torch
     law = dgpar.EulerLaw(dgpar.FLUX_LF, 1.4)
     g = dgpar.DGAnalysis(mesh, law, ORDER, precision=precision, use_graph=True, device=0, kernel=kernel)
     g.project(dgpar.FIELD_VORTEX, VORTEX)
@@ -55,10 +54,12 @@ def run_form(dgpar, mesh, precision, steps, warmup, kernel=0):
     if warmup > 0:
         g.time_advance(dgpar.RK4, dt, warmup)
     g.project(dgpar.FIELD_VORTEX, VORTEX)
-    torch.cuda.synchronize()
+    # bracket the timed steps with device-wide synchronisation on the library's
+    # HIP runtime (one process, one GPU: no barrier needed)
+    dgpar.device_synchronize()
     t0 = time.perf_counter()
     ms = g.time_advance(dgpar.RK4, dt, steps)
-    torch.cuda.synchronize()
+    dgpar.device_synchronize()
     wall = time.perf_counter() - t0
     err = g.l2_error(dgpar.FIELD_VORTEX, VORTEX, steps * dt)
     state = g.get_state()
@@ -98,12 +99,7 @@ def main():
     import dgpar
 
     if args.copy_probe:
-        import torch
-        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        dgpar.copy_probe(64 << 20)
     mesh = dgpar.mesh_tri_structured(NX, NX, lx=10.0, ly=10.0, jitter=0.1, seed=SEED)
     E = mesh.n_elems
     r32 = run_form(dgpar, mesh, dgpar.PREC_FP32, args.steps, args.warmup, args.kernel)

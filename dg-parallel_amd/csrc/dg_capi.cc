@@ -494,6 +494,36 @@ dg_status dg_l2_error(dg_ctx* c, int field, const double* par, double t, double*
   return DG_OK;
 }
 
+dg_status dg_copy_probe(int device, size_t n_doubles, float* ms_out) {
+  if (n_doubles < 4 || !ms_out) return DG_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return DG_ERR_HIP;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n_doubles * 8) != hipSuccess) return DG_ERR_OOM;
+  if (hipMalloc(&b, n_doubles * 8) != hipSuccess) {
+    (void)hipFree(a);
+    return DG_ERR_OOM;
+  }
+  hipEvent_t e0, e1;
+  dg_status st = DG_OK;
+  if (hipMemset(a, 0, n_doubles * 8) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    st = DG_ERR_HIP;
+  } else {
+    (void)hipEventRecord(e0, nullptr);
+    if (launch_copy_probe(a, b, (long)n_doubles, nullptr)) st = DG_ERR_HIP;
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_out = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return st;
+}
+
 // ---------------------------------------------------------------------------
 static uint64_t splitmix64(uint64_t x) {
   uint64_t z = x + 0x9E3779B97F4A7C15ull;

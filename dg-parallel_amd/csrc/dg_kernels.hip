@@ -378,3 +378,19 @@ int launch_tri_dt(const DtArgs& a, int law, hipStream_t s) {
 }
 
 }  // namespace dgp
+
+namespace dgp {
+
+// plain streaming copy: the reference point for HBM byte counters
+__global__ __launch_bounds__(256) void copy_probe_kernel(const double4* __restrict__ src, double4* __restrict__ dst,
+                                                          long n4) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n4; i += (long)gridDim.x * 256) dst[i] = src[i];
+}
+
+int launch_copy_probe(const void* src, void* dst, long n_doubles, hipStream_t s) {
+  long n4 = n_doubles / 4;
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(4096), dim3(256), 0, s, (const double4*)src, (double4*)dst, n4);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace dgp

@@ -22,6 +22,16 @@ if not os.path.exists(LIB_PATH):
     raise ImportError("libdgpar.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
 
 _lib = ctypes.CDLL(LIB_PATH)
+# the HIP runtime libdgpar.so is linked against (already loaded by the line above)
+_hip = ctypes.CDLL("libamdhip64.so.7")
+_hip.hipDeviceSynchronize.restype = ctypes.c_int
+
+
+def device_synchronize():
+    """hipDeviceSynchronize on the runtime this library uses."""
+    st = _hip.hipDeviceSynchronize()
+    if st != 0:
+        raise RuntimeError("hipDeviceSynchronize failed with %d" % st)
 
 DG_OK, DG_ERR_ARG, DG_ERR_OOM, DG_ERR_HIP, DG_ERR_NONFINITE, DG_ERR_NONPHYSICAL, DG_ERR_UNSUPPORTED = range(7)
 LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
@@ -81,11 +91,21 @@ _compute_dt = _sig("dg_compute_dt", ctypes.c_int, [_vp, ctypes.c_double, _p_f64]
 _advance = _sig("dg_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _p_f64])
 _time_advance = _sig("dg_time_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                        ctypes.POINTER(ctypes.c_float)])
+_copy_probe = _sig("dg_copy_probe", ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_float)])
 _project = _sig("dg_project", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double])
 _l2_error = _sig("dg_l2_error", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64])
 _mesh_tri = _sig("dg_mesh_tri_structured", ctypes.c_int,
                  [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                   ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _p_i32, _p_i32, _p_f64, _p_i32])
+
+
+def copy_probe(n_doubles, device=0):
+    """Time one streaming device copy of n_doubles (HBM counter reference point)."""
+    ms = ctypes.c_float()
+    st = _copy_probe(device, n_doubles, ctypes.byref(ms))
+    if st != DG_OK:
+        raise DGError(st, "dg_copy_probe failed")
+    return ms.value
 
 
 def _ptr(a, t):

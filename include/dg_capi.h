@@ -124,6 +124,11 @@ dg_status dg_get_state_device(dg_ctx* ctx, double* dU);
  * stream between two HIP events and returns the elapsed milliseconds. */
 dg_status dg_time_advance(dg_ctx* ctx, int rk_scheme, double dt, int nsteps, float* ms_out);
 
+/* Measurement helper (no counterpart in the reference): one streaming copy of
+ * n_doubles between two fresh device buffers; the reference point that scales
+ * the HBM byte counters of the profiler.  Returns the kernel time. */
+dg_status dg_copy_probe(int device, size_t n_doubles, float* ms_out);
+
 /* Built-in exact fields: 0 Gaussian pulse carried by the scalar velocity on the
  * periodic unit square; 1 isentropic vortex, par = {beta, xc, yc, u0, v0, box
  * length}; 2 Sod tube (Examples.cc:62-83), par = {interface x}; 3 PEC cavity mode. */
