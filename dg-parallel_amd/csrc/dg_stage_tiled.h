@@ -8,7 +8,7 @@
 // lane.  Every phase maps a wave-uniform work index (quadrature point, face,
 // output row) to the wave and the element to the lane, so basis tables are
 // scalar loads and LDS accesses are lane-contiguous.
-//   A  stage input tile, |J| J^-1 and face links -> LDS
+//   A  own element's coefficients, |J| J^-1 and face links -> registers
 //   B  volume points (wave = point q): interpolate, physical flux, contravariant
 //      flux G = w_q |J| J^-1 F -> LDS                      (DGCell.cc:193-236)
 //   D1 output rows (wave = row f*NP+k): R = sum_q dphi_k(q) . G(q)   (registers)
@@ -97,83 +97,107 @@ struct TiledStage {
   static constexpr int NP = np_tri(P), NQ = nqv_tri(P), NQF = nqf_tri(P), NF = LAW::NF;
   static constexpr int T = TILE_ELEMS;
   static constexpr int NROW = NF * NP;
-  static constexpr int RPW = (NROW + 3) / 4;
   static constexpr int GW = NQ * NF * 2;   // volume-flux rows
   static constexpr int FW = 3 * NQF * NF;  // trace / face-flux rows
   static constexpr int WORK = GW > 2 * FW ? GW : 2 * FW;
+  // output rows of a wave: with four or more fields a wave owns whole fields
+  // (f = wave, wave+4, ..; every mode k), else it owns modes k = wave, wave+4, ..
+  // of the single field -- either way the rows of one field share every LDS read
+  static constexpr bool BY_FIELD = NF >= 4;
+  static constexpr int FPW = BY_FIELD ? (NF + 3) / 4 : NF;
+  static constexpr int KPW = BY_FIELD ? NP : (NP + 3) / 4;
+  static constexpr int QPW = (NQ + 3) / 4;
+  static constexpr int CPW = (3 * NF + 3) / 4;
+  static constexpr int FPPW = (3 * NQF + 3) / 4;
 
   struct Smem {
-    CT Us[NROW][T];
-    CT Ms[4][T];
-    int Ls[3][T];
     CT W[WORK][T];
     double red[TILE_THREADS];
   };
 
+  // per-thread state of its own element (all four waves of the tile hold it)
   struct Regs {
-    CT R[RPW];
+    CT U[NROW];
+    CT M[4];
+    int L[3];
+    CT R[FPW * KPW];
     double rsum;
   };
 
   DG_HD static const CT* table(const StageArgs& a) {
     return sizeof(CT) == 8 ? (const CT*)(const void*)a.tab : (const CT*)(const void*)a.tabf;
   }
+  DG_HD static int row_f(int wave, int fi) { return BY_FIELD ? wave + 4 * fi : fi; }
+  DG_HD static int row_k(int wave, int ki) { return BY_FIELD ? ki : wave + 4 * ki; }
 
-  // ---- A ---------------------------------------------------------------------
-  DG_HD static void phaseA(const StageArgs& a, Smem& s, int tid, int tile) {
-    const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+  // ---- A: own element's coefficients, geometry and links into registers ----
+  DG_HD static void phaseA(const StageArgs& a, Smem&, int tid, int tile, Regs& g) {
+    const int t = tid & (T - 1);
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
     const IT* Uin = (const IT*)a.Uin;
-    for (int r = wave; r < NROW; r += 4) s.Us[r][t] = valid ? CT(Uin[(long)r * E + e]) : CT(r == 0 ? 1 : 0);
-    if (wave < 4) s.Ms[wave][t] = valid ? CT(a.el.mg[(long)wave * E + e]) : CT(wave == 0 || wave == 3 ? 1 : 0);
-    if (wave < 3) s.Ls[wave][t] = valid ? a.el.link[(long)wave * E + e] : -BC_FARFIELD;
+#pragma unroll
+    for (int r = 0; r < NROW; r++) g.U[r] = valid ? CT(Uin[(long)r * E + e]) : CT(r == 0 ? 1 : 0);
+#pragma unroll
+    for (int r = 0; r < 4; r++) g.M[r] = valid ? CT(a.el.mg[(long)r * E + e]) : CT(r == 0 || r == 3 ? 1 : 0);
+#pragma unroll
+    for (int l = 0; l < 3; l++) g.L[l] = valid ? a.el.link[(long)l * E + e] : -BC_FARFIELD;
   }
 
-  // ---- B ---------------------------------------------------------------------
-  DG_HD static void phaseB(const StageArgs& a, Smem& s, int tid) {
+  // ---- B: volume flux at the quadrature points ---------------------------------
+  DG_HD static void phaseB(const StageArgs& a, Smem& s, int tid, const Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const CT* tb = table(a);
-    for (int q = wave; q < NQ; q += 4) {
-      CT u[NF];
-      for (int f = 0; f < NF; f++) {
-        CT acc = CT(0);
 #pragma unroll
-        for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phi(P) + q * NP + k] * s.Us[f * NP + k][t];
-        u[f] = acc;
-      }
-      CT F[NF][2];
-      LAW::flux(a.law, u, F);
-      const CT w = tb[TriBlock::off_wv(P) + q];
-      const CT m00 = s.Ms[0][t], m01 = s.Ms[1][t], m10 = s.Ms[2][t], m11 = s.Ms[3][t];
+    for (int j = 0; j < QPW; j++) {
+      const int q = wave + 4 * j;
+      if (q < NQ) {
+        CT u[NF];
 #pragma unroll
-      for (int f = 0; f < NF; f++) {
-        s.W[(q * NF + f) * 2 + 0][t] = w * (m00 * F[f][0] + m01 * F[f][1]);
-        s.W[(q * NF + f) * 2 + 1][t] = w * (m10 * F[f][0] + m11 * F[f][1]);
+        for (int f = 0; f < NF; f++) {
+          CT acc = CT(0);
+#pragma unroll
+          for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phi(P) + q * NP + k] * g.U[f * NP + k];
+          u[f] = acc;
+        }
+        CT F[NF][2];
+        LAW::flux(a.law, u, F);
+        const CT w = tb[TriBlock::off_wv(P) + q];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          s.W[(q * NF + f) * 2 + 0][t] = w * (g.M[0] * F[f][0] + g.M[1] * F[f][1]);
+          s.W[(q * NF + f) * 2 + 1][t] = w * (g.M[2] * F[f][0] + g.M[3] * F[f][1]);
+        }
       }
     }
   }
 
-  // ---- D1 --------------------------------------------------------------------
+  // ---- D1: contraction with the basis gradients ----------------------------
   DG_HD static void phaseD1(const StageArgs& a, Smem& s, int tid, Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const CT* tb = table(a);
 #pragma unroll
-    for (int j = 0; j < RPW; j++) {
-      const int row = wave + 4 * j;
-      CT acc = CT(0);
-      if (row < NROW) {
-        const int f = row / NP, k = row - f * NP;
+    for (int i = 0; i < FPW * KPW; i++) g.R[i] = CT(0);
 #pragma unroll
-        for (int q = 0; q < NQ; q++)
-          acc += tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 0] * s.W[(q * NF + f) * 2 + 0][t] +
-                 tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 1] * s.W[(q * NF + f) * 2 + 1][t];
+    for (int fi = 0; fi < FPW; fi++) {
+      const int f = row_f(wave, fi);
+      if (f < NF) {
+        for (int q = 0; q < NQ; q++) {
+          const CT G0 = s.W[(q * NF + f) * 2 + 0][t];
+          const CT G1 = s.W[(q * NF + f) * 2 + 1][t];
+#pragma unroll
+          for (int ki = 0; ki < KPW; ki++) {
+            const int k = row_k(wave, ki);
+            if (k < NP)
+              g.R[fi * KPW + ki] += tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 0] * G0 +
+                                    tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 1] * G1;
+          }
+        }
       }
-      g.R[j] = acc;
     }
   }
 
-  // ---- C1 --------------------------------------------------------------------
+  // ---- C1: neighbour traces ---------------------------------------------------------
   template <int LB, bool SAME>
   DG_HD static void nb_trace(const StageArgs& a, int nb, int f, CT* out) {
     constexpr int OFF = TriBlock::off_phif(P);
@@ -195,79 +219,87 @@ struct TiledStage {
     for (int q = 0; q < NQF; q++) out[q] = acc[q];
   }
 
-  DG_HD static void phaseC1(const StageArgs& a, Smem& s, int tid, int tile) {
+  DG_HD static void phaseC1(const StageArgs& a, Smem& s, int tid, int tile, const Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const bool valid = tile * T + t < a.E;
-    for (int it = wave; it < 3 * NF; it += 4) {
-      const int l = it / NF, f = it - l * NF;
-      const int link = s.Ls[l][t];
-      CT out[NQF];
-      if (valid && link >= 0) {
-        const int nb = link >> 3, lb = link & 3;
-        const bool same = (link >> 2) & 1;
-        if (!same) {
-          if (lb == 0)
-            nb_trace<0, false>(a, nb, f, out);
-          else if (lb == 1)
-            nb_trace<1, false>(a, nb, f, out);
-          else
-            nb_trace<2, false>(a, nb, f, out);
+#pragma unroll
+    for (int j = 0; j < CPW; j++) {
+      const int it = wave + 4 * j;
+      if (it < 3 * NF) {
+        const int l = it / NF, f = it - l * NF;
+        const int link = l == 0 ? g.L[0] : l == 1 ? g.L[1] : g.L[2];
+        CT out[NQF];
+        if (valid && link >= 0) {
+          const int nb = link >> 3, lb = link & 3;
+          const bool same = (link >> 2) & 1;
+          if (!same) {
+            if (lb == 0)
+              nb_trace<0, false>(a, nb, f, out);
+            else if (lb == 1)
+              nb_trace<1, false>(a, nb, f, out);
+            else
+              nb_trace<2, false>(a, nb, f, out);
+          } else {
+            if (lb == 0)
+              nb_trace<0, true>(a, nb, f, out);
+            else if (lb == 1)
+              nb_trace<1, true>(a, nb, f, out);
+            else
+              nb_trace<2, true>(a, nb, f, out);
+          }
         } else {
-          if (lb == 0)
-            nb_trace<0, true>(a, nb, f, out);
-          else if (lb == 1)
-            nb_trace<1, true>(a, nb, f, out);
-          else
-            nb_trace<2, true>(a, nb, f, out);
+#pragma unroll
+          for (int q = 0; q < NQF; q++) out[q] = CT(0);
         }
-      } else {
 #pragma unroll
-        for (int q = 0; q < NQF; q++) out[q] = CT(0);
+        for (int q = 0; q < NQF; q++) s.W[(l * NQF + q) * NF + f][t] = out[q];
       }
-#pragma unroll
-      for (int q = 0; q < NQF; q++) s.W[(l * NQF + q) * NF + f][t] = out[q];
     }
   }
 
-  // ---- C2 --------------------------------------------------------------------
-  DG_HD static void phaseC2(const StageArgs& a, Smem& s, int tid) {
+  // ---- C2: numerical flux at the face points ----------------------------------
+  DG_HD static void phaseC2(const StageArgs& a, Smem& s, int tid, const Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const CT* tb = table(a);
-    for (int it = wave; it < 3 * NQF; it += 4) {
-      const int l = it / NQF, q = it - l * NQF;
-      const int link = s.Ls[l][t];
-      // outward normal x edge length = |J| J^-T nhat |fhat|, with nhat |fhat| of the
-      // reference edges (0,-1), (1,1), (-1,0)
-      CT nl0, nl1;
-      if (l == 0) {
-        nl0 = -s.Ms[2][t];
-        nl1 = -s.Ms[3][t];
-      } else if (l == 1) {
-        nl0 = s.Ms[0][t] + s.Ms[2][t];
-        nl1 = s.Ms[1][t] + s.Ms[3][t];
-      } else {
-        nl0 = -s.Ms[0][t];
-        nl1 = -s.Ms[1][t];
+#pragma unroll
+    for (int j = 0; j < FPPW; j++) {
+      const int it = wave + 4 * j;
+      if (it < 3 * NQF) {
+        const int l = it / NQF, q = it - l * NQF;
+        const int link = l == 0 ? g.L[0] : l == 1 ? g.L[1] : g.L[2];
+        // outward normal x edge length = |J| J^-T nhat |fhat|, with nhat |fhat| of
+        // the reference edges (0,-1), (1,1), (-1,0)
+        CT nl0, nl1;
+        if (l == 0) {
+          nl0 = -g.M[2];
+          nl1 = -g.M[3];
+        } else if (l == 1) {
+          nl0 = g.M[0] + g.M[2];
+          nl1 = g.M[1] + g.M[3];
+        } else {
+          nl0 = -g.M[0];
+          nl1 = -g.M[1];
+        }
+        const CT len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
+        const CT n[2] = {nl0 / len, nl1 / len};
+        CT uL[NF], uR[NF], Fh[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          CT acc = CT(0);
+#pragma unroll
+          for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * g.U[f * NP + k];
+          uL[f] = acc;
+          uR[f] = s.W[(l * NQF + q) * NF + f][t];
+        }
+        if (link >= 0)
+          LAW::riemann(a.law, n, uL, uR, Fh);
+        else
+          LAW::boundary(a.law, -link, n, uL, Fh);
+        // |J_face| w_q = (len / 2) w_q; the table holds w_q / 2
+        const CT sw = len * tb[TriBlock::off_wf(P) + q];
+#pragma unroll
+        for (int f = 0; f < NF; f++) s.W[FW + (l * NQF + q) * NF + f][t] = Fh[f] * sw;
       }
-      const CT len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
-      const CT n[2] = {nl0 / len, nl1 / len};
-      CT uL[NF], uR[NF], Fh[NF];
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-        CT acc = CT(0);
-#pragma unroll
-        for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * s.Us[f * NP + k][t];
-        uL[f] = acc;
-        uR[f] = s.W[(l * NQF + q) * NF + f][t];
-      }
-      if (link >= 0)
-        LAW::riemann(a.law, n, uL, uR, Fh);
-      else
-        LAW::boundary(a.law, -link, n, uL, Fh);
-      // |J_face| w_q = (len / 2) w_q; the table holds w_q / 2
-      const CT sw = len * tb[TriBlock::off_wf(P) + q];
-#pragma unroll
-      for (int f = 0; f < NF; f++) s.W[FW + (l * NQF + q) * NF + f][t] = Fh[f] * sw;
     }
   }
 
@@ -282,25 +314,31 @@ struct TiledStage {
     const double dtj = valid ? a.dt / a.el.detj[e] : 0.0;
     g.rsum = 0.0;
 #pragma unroll
-    for (int j = 0; j < RPW; j++) {
-      const int row = wave + 4 * j;
-      if (row < NROW) {
-        const int f = row / NP, k = row - f * NP;
-        CT acc = g.R[j];
+    for (int fi = 0; fi < FPW; fi++) {
+      const int f = row_f(wave, fi);
+      if (f < NF) {
+        for (int lq = 0; lq < 3 * NQF; lq++) {
+          const CT Fq = s.W[FW + lq * NF + f][t];
 #pragma unroll
-        for (int l = 0; l < 3; l++)
+          for (int ki = 0; ki < KPW; ki++) {
+            const int k = row_k(wave, ki);
+            if (k < NP) g.R[fi * KPW + ki] -= tb[TriBlock::off_phif(P) + lq * NP + k] * Fq;
+          }
+        }
 #pragma unroll
-          for (int q = 0; q < NQF; q++)
-            acc -= tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * s.W[FW + (l * NQF + q) * NF + f][t];
-        if (valid) {
-          const long i = (long)row * E + e;
-          if (a.scheme == MODE_RESIDUAL) {
-            a.R[i] = double(acc);
-          } else {
-            const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
-            const double us = reads_input ? double(Uin[i]) : 0.0;
-            const double o = stage_value_hd(sc, double(acc) * dtj, a.Un[i], us, a.acc + i, g.rsum);
-            ((OT*)a.Uout)[i] = OT(o);
+        for (int ki = 0; ki < KPW; ki++) {
+          const int k = row_k(wave, ki);
+          if (k < NP && valid) {
+            const long i = (long)(f * NP + k) * E + e;
+            const CT acc = g.R[fi * KPW + ki];
+            if (a.scheme == MODE_RESIDUAL) {
+              a.R[i] = double(acc);
+            } else {
+              const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
+              const double us = reads_input ? double(Uin[i]) : 0.0;
+              const double o = stage_value_hd(sc, double(acc) * dtj, a.Un[i], us, a.acc + i, g.rsum);
+              ((OT*)a.Uout)[i] = OT(o);
+            }
           }
         }
       }

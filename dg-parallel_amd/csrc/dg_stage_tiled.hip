@@ -20,15 +20,14 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_stage_tiled(StageArgs a) {
   typename K::Regs g;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
-  K::phaseA(a, sm, tid, tile);
-  __syncthreads();
-  K::phaseB(a, sm, tid);
+  K::phaseA(a, sm, tid, tile, g);
+  K::phaseB(a, sm, tid, g);
   __syncthreads();
   K::phaseD1(a, sm, tid, g);
   __syncthreads();
-  K::phaseC1(a, sm, tid, tile);
+  K::phaseC1(a, sm, tid, tile, g);
   __syncthreads();
-  K::phaseC2(a, sm, tid);
+  K::phaseC2(a, sm, tid, g);
   __syncthreads();
   K::phaseD2(a, sm, tid, tile, g);
   if (a.resid_part) {
