@@ -153,7 +153,8 @@ def test_fp32_form_error_budget(kernel):
     g64, o, U = make_pair("euler", "lf", 3, nx=16, kernel=kernel)
     g32, _, _ = make_pair("euler", "lf", 3, nx=16, precision=dgpar.PREC_FP32, kernel=kernel)
     R64, R32 = g64.residual(), g32.residual()
-    assert rel(R32, R64) < 1e-5
+    # FP32 arithmetic on O(1-10) fluxes whose residual cancels to O(1e-2): ~1e-5 max-norm
+    assert rel(R32, R64) < 5e-5
     dt = o.dt(U, 0.1)
     g64.advanceInTime(0, dt, po.RK4, 20, want_resid=False)
     g32.advanceInTime(0, dt, po.RK4, 20, want_resid=False)
