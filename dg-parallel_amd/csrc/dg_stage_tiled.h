@@ -8,7 +8,7 @@
 // lane.  Every phase maps a wave-uniform work index (quadrature point, face,
 // output row) to the wave and the element to the lane, so basis tables are
 // scalar loads and LDS accesses are lane-contiguous.
-//   A  own element's coefficients, |J| J^-1 and face links -> registers
+//   A  stage-input tile, |J| J^-1 and face links -> LDS
 //   B  volume points (wave = point q): interpolate, physical flux, contravariant
 //      flux G = w_q |J| J^-1 F -> LDS                      (DGCell.cc:193-236)
 //   D1 output rows (wave = row f*NP+k): R = sum_q dphi_k(q) . G(q)   (registers)
@@ -111,15 +111,15 @@ struct TiledStage {
   static constexpr int FPPW = (3 * NQF + 3) / 4;
 
   struct Smem {
+    CT Us[NROW][T];
+    CT Ms[4][T];
+    int Ls[3][T];
     CT W[WORK][T];
     double red[TILE_THREADS];
   };
 
-  // per-thread state of its own element (all four waves of the tile hold it)
+  // per-thread output rows of its own element
   struct Regs {
-    CT U[NROW];
-    CT M[4];
-    int L[3];
     CT R[FPW * KPW];
     double rsum;
   };
@@ -130,25 +130,23 @@ struct TiledStage {
   DG_HD static int row_f(int wave, int fi) { return BY_FIELD ? wave + 4 * fi : fi; }
   DG_HD static int row_k(int wave, int ki) { return BY_FIELD ? ki : wave + 4 * ki; }
 
-  // ---- A: own element's coefficients, geometry and links into registers ----
-  DG_HD static void phaseA(const StageArgs& a, Smem&, int tid, int tile, Regs& g) {
-    const int t = tid & (T - 1);
+  // ---- A: stage-input tile -> LDS (rows split over the waves); own element's
+  //      geometry and links -> registers ----------------------------------------
+  DG_HD static void phaseA(const StageArgs& a, Smem& s, int tid, int tile, Regs& g) {
+    const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
     const IT* Uin = (const IT*)a.Uin;
-#pragma unroll
-    for (int r = 0; r < NROW; r++) g.U[r] = valid ? CT(Uin[(long)r * E + e]) : CT(r == 0 ? 1 : 0);
-#pragma unroll
-    for (int r = 0; r < 4; r++) g.M[r] = valid ? CT(a.el.mg[(long)r * E + e]) : CT(r == 0 || r == 3 ? 1 : 0);
-#pragma unroll
-    for (int l = 0; l < 3; l++) g.L[l] = valid ? a.el.link[(long)l * E + e] : -BC_FARFIELD;
+    for (int r = wave; r < NROW; r += 4) s.Us[r][t] = valid ? CT(Uin[(long)r * E + e]) : CT(r == 0 ? 1 : 0);
+    if (wave < 4) s.Ms[wave][t] = valid ? CT(a.el.mg[(long)wave * E + e]) : CT(wave == 0 || wave == 3 ? 1 : 0);
+    if (wave < 3) s.Ls[wave][t] = valid ? a.el.link[(long)wave * E + e] : -BC_FARFIELD;
   }
 
   // ---- B: volume flux at the quadrature points ---------------------------------
   DG_HD static void phaseB(const StageArgs& a, Smem& s, int tid, const Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const CT* tb = table(a);
-#pragma unroll
+#pragma unroll 1
     for (int j = 0; j < QPW; j++) {
       const int q = wave + 4 * j;
       if (q < NQ) {
@@ -157,16 +155,17 @@ struct TiledStage {
         for (int f = 0; f < NF; f++) {
           CT acc = CT(0);
 #pragma unroll
-          for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phi(P) + q * NP + k] * g.U[f * NP + k];
+          for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phi(P) + q * NP + k] * s.Us[f * NP + k][t];
           u[f] = acc;
         }
         CT F[NF][2];
         LAW::flux(a.law, u, F);
         const CT w = tb[TriBlock::off_wv(P) + q];
+        const CT m00 = s.Ms[0][t], m01 = s.Ms[1][t], m10 = s.Ms[2][t], m11 = s.Ms[3][t];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-          s.W[(q * NF + f) * 2 + 0][t] = w * (g.M[0] * F[f][0] + g.M[1] * F[f][1]);
-          s.W[(q * NF + f) * 2 + 1][t] = w * (g.M[2] * F[f][0] + g.M[3] * F[f][1]);
+          s.W[(q * NF + f) * 2 + 0][t] = w * (m00 * F[f][0] + m01 * F[f][1]);
+          s.W[(q * NF + f) * 2 + 1][t] = w * (m10 * F[f][0] + m11 * F[f][1]);
         }
       }
     }
@@ -219,15 +218,15 @@ struct TiledStage {
     for (int q = 0; q < NQF; q++) out[q] = acc[q];
   }
 
-  DG_HD static void phaseC1(const StageArgs& a, Smem& s, int tid, int tile, const Regs& g) {
+  DG_HD static void phaseC1(const StageArgs& a, Smem& s, int tid, int tile, const Regs&) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const bool valid = tile * T + t < a.E;
-#pragma unroll
+#pragma unroll 1
     for (int j = 0; j < CPW; j++) {
       const int it = wave + 4 * j;
       if (it < 3 * NF) {
         const int l = it / NF, f = it - l * NF;
-        const int link = l == 0 ? g.L[0] : l == 1 ? g.L[1] : g.L[2];
+        const int link = s.Ls[l][t];
         CT out[NQF];
         if (valid && link >= 0) {
           const int nb = link >> 3, lb = link & 3;
@@ -258,27 +257,27 @@ struct TiledStage {
   }
 
   // ---- C2: numerical flux at the face points ----------------------------------
-  DG_HD static void phaseC2(const StageArgs& a, Smem& s, int tid, const Regs& g) {
+  DG_HD static void phaseC2(const StageArgs& a, Smem& s, int tid, const Regs&) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const CT* tb = table(a);
-#pragma unroll
+#pragma unroll 1
     for (int j = 0; j < FPPW; j++) {
       const int it = wave + 4 * j;
       if (it < 3 * NQF) {
         const int l = it / NQF, q = it - l * NQF;
-        const int link = l == 0 ? g.L[0] : l == 1 ? g.L[1] : g.L[2];
+        const int link = s.Ls[l][t];
         // outward normal x edge length = |J| J^-T nhat |fhat|, with nhat |fhat| of
         // the reference edges (0,-1), (1,1), (-1,0)
         CT nl0, nl1;
         if (l == 0) {
-          nl0 = -g.M[2];
-          nl1 = -g.M[3];
+          nl0 = -s.Ms[2][t];
+          nl1 = -s.Ms[3][t];
         } else if (l == 1) {
-          nl0 = g.M[0] + g.M[2];
-          nl1 = g.M[1] + g.M[3];
+          nl0 = s.Ms[0][t] + s.Ms[2][t];
+          nl1 = s.Ms[1][t] + s.Ms[3][t];
         } else {
-          nl0 = -g.M[0];
-          nl1 = -g.M[1];
+          nl0 = -s.Ms[0][t];
+          nl1 = -s.Ms[1][t];
         }
         const CT len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
         const CT n[2] = {nl0 / len, nl1 / len};
@@ -287,7 +286,7 @@ struct TiledStage {
         for (int f = 0; f < NF; f++) {
           CT acc = CT(0);
 #pragma unroll
-          for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * g.U[f * NP + k];
+          for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * s.Us[f * NP + k][t];
           uL[f] = acc;
           uR[f] = s.W[(l * NQF + q) * NF + f][t];
         }

@@ -21,6 +21,7 @@ __global__ __launch_bounds__(TILE_THREADS) void tri_stage_tiled(StageArgs a) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   K::phaseA(a, sm, tid, tile, g);
+  __syncthreads();
   K::phaseB(a, sm, tid, g);
   __syncthreads();
   K::phaseD1(a, sm, tid, g);

@@ -23,10 +23,8 @@ void run_tiles(const StageArgs& a) {
   const int ntiles = (a.E + TILE_ELEMS - 1) / TILE_ELEMS;
   for (int tile = 0; tile < ntiles; tile++) {
     std::memset(sm.get(), 0, sizeof(typename K::Smem));
-    for (int t = 0; t < TILE_THREADS; t++) {
-      K::phaseA(a, *sm, t, tile, g[t]);
-      K::phaseB(a, *sm, t, g[t]);
-    }
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, *sm, t, tile, g[t]);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, *sm, t, g[t]);
     for (int t = 0; t < TILE_THREADS; t++) K::phaseD1(a, *sm, t, g[t]);
     for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, *sm, t, tile, g[t]);
     for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t, g[t]);
