@@ -42,6 +42,18 @@ template <>
 DG_HD float dg_sqrt<float>(float x) { return sqrtf(x); }
 template <typename T>
 DG_HD T dg_abs(T x) { return x < T(0) ? -x : x; }
+// reciprocal: exact division in FP64; the hardware reciprocal (1 ulp) in the
+// FP32 form
+template <typename T>
+DG_HD T dg_rcp(T x) { return T(1) / x; }
+template <>
+DG_HD float dg_rcp<float>(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
 
 // ---------------------------------------------------------------------------
 struct ScalarLawD {
@@ -80,7 +92,7 @@ struct EulerLawD {
   template <typename T>
   DG_HD static void flux(const LawParams& L, const T* Q, T F[][DIM]) {
     T g = T(L.gamma);
-    T invrho = T(1) / Q[0];
+    T invrho = dg_rcp(Q[0]);
     T Q1 = Q[1], Q2 = Q[2];
     T q12 = Q1 * Q2 * invrho, q11 = Q1 * Q1 * invrho, q22 = Q2 * Q2 * invrho;
     T p = (g - T(1)) * (Q[3] - T(0.5) * (q11 + q22));
@@ -90,6 +102,20 @@ struct EulerLawD {
     F[2][0] = q12;     F[2][1] = q22 + p;
     F[3][0] = Q1 * qq; F[3][1] = Q2 * qq;
   }
+  // F(u).n together with the normal velocity and the sound speed, sharing 1/rho
+  template <typename T>
+  DG_HD static void normal_flux(const LawParams& L, const T* n, const T* Q, T* Fn, T& vn, T& c) {
+    const T g = T(L.gamma);
+    const T ri = dg_rcp(Q[0]);
+    const T mn = Q[1] * n[0] + Q[2] * n[1];
+    const T p = (g - T(1)) * (Q[3] - T(0.5) * (Q[1] * Q[1] + Q[2] * Q[2]) * ri);
+    vn = mn * ri;
+    c = dg_sqrt(g * p * ri);
+    Fn[0] = mn;
+    Fn[1] = Q[1] * vn + p * n[0];
+    Fn[2] = Q[2] * vn + p * n[1];
+    Fn[3] = (Q[3] + p) * vn;
+  }
   template <typename T>
   DG_HD static T wave(const LawParams& L, const T* n, const T* u) {
     T g = T(L.gamma);
@@ -98,18 +124,17 @@ struct EulerLawD {
     T c = dg_sqrt(g * p * ri);
     return c + dg_abs((u[1] * n[0] + u[2] * n[1]) * ri);
   }
+  // local Lax-Friedrichs: (F(uL).n + F(uR).n)/2 - lam (uR - uL)/2 with
+  // lam = max(|vn| + c) over both states (EulerLaw.cc:1022-1034)
   template <typename T>
   DG_HD static void lf(const LawParams& L, const T* n, const T* uL, const T* uR, T* out) {
-    T FL[4][2], FR[4][2];
-    flux(L, uL, FL);
-    flux(L, uR, FR);
-    T l1 = wave(L, n, uL), l2 = wave(L, n, uR);
-    T lam = l1 > l2 ? l1 : l2;
+    T FL[4], FR[4], vL, cL, vR, cR;
+    normal_flux(L, n, uL, FL, vL, cL);
+    normal_flux(L, n, uR, FR, vR, cR);
+    const T l1 = cL + dg_abs(vL), l2 = cR + dg_abs(vR);
+    const T lam = l1 > l2 ? l1 : l2;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      T fn = (FL[i][0] + FR[i][0]) * n[0] + (FL[i][1] + FR[i][1]) * n[1];
-      out[i] = T(0.5) * (fn - lam * (uR[i] - uL[i]));
-    }
+    for (int i = 0; i < 4; i++) out[i] = T(0.5) * ((FL[i] + FR[i]) - lam * (uR[i] - uL[i]));
   }
   template <typename T>
   DG_HD static void roe(const LawParams& L, const T* n, const T* u0, const T* u1, T* flux_out) {

@@ -113,6 +113,7 @@ struct TiledStage {
   struct Smem {
     CT Us[NROW][T];
     CT Ms[4][T];
+    CT Ns[3][3][T];  // per face: unit outward normal (2) and edge length
     int Ls[3][T];
     CT W[WORK][T];
     double red[TILE_THREADS];
@@ -139,7 +140,30 @@ struct TiledStage {
     const IT* Uin = (const IT*)a.Uin;
     for (int r = wave; r < NROW; r += 4) s.Us[r][t] = valid ? CT(Uin[(long)r * E + e]) : CT(r == 0 ? 1 : 0);
     if (wave < 4) s.Ms[wave][t] = valid ? CT(a.el.mg[(long)wave * E + e]) : CT(wave == 0 || wave == 3 ? 1 : 0);
-    if (wave < 3) s.Ls[wave][t] = valid ? a.el.link[(long)wave * E + e] : -BC_FARFIELD;
+    if (wave < 3) {
+      const int l = wave;
+      s.Ls[l][t] = valid ? a.el.link[(long)l * E + e] : -BC_FARFIELD;
+      // outward normal x edge length = |J| J^-T nhat |fhat|, with nhat |fhat| of
+      // the reference edges (0,-1), (1,1), (-1,0)
+      CT m[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) m[r] = valid ? CT(a.el.mg[(long)r * E + e]) : CT(r == 0 || r == 3 ? 1 : 0);
+      CT nl0, nl1;
+      if (l == 0) {
+        nl0 = -m[2];
+        nl1 = -m[3];
+      } else if (l == 1) {
+        nl0 = m[0] + m[2];
+        nl1 = m[1] + m[3];
+      } else {
+        nl0 = -m[0];
+        nl1 = -m[1];
+      }
+      const CT len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
+      s.Ns[l][0][t] = nl0 / len;
+      s.Ns[l][1][t] = nl1 / len;
+      s.Ns[l][2][t] = len;
+    }
   }
 
   // ---- B: volume flux at the quadrature points ---------------------------------
@@ -187,9 +211,10 @@ struct TiledStage {
 #pragma unroll
           for (int ki = 0; ki < KPW; ki++) {
             const int k = row_k(wave, ki);
-            if (k < NP)
-              g.R[fi * KPW + ki] += tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 0] * G0 +
-                                    tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 1] * G1;
+            if (k < NP) {
+              g.R[fi * KPW + ki] += tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 0] * G0;
+              g.R[fi * KPW + ki] += tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 1] * G1;
+            }
           }
         }
       }
@@ -266,21 +291,8 @@ struct TiledStage {
       if (it < 3 * NQF) {
         const int l = it / NQF, q = it - l * NQF;
         const int link = s.Ls[l][t];
-        // outward normal x edge length = |J| J^-T nhat |fhat|, with nhat |fhat| of
-        // the reference edges (0,-1), (1,1), (-1,0)
-        CT nl0, nl1;
-        if (l == 0) {
-          nl0 = -s.Ms[2][t];
-          nl1 = -s.Ms[3][t];
-        } else if (l == 1) {
-          nl0 = s.Ms[0][t] + s.Ms[2][t];
-          nl1 = s.Ms[1][t] + s.Ms[3][t];
-        } else {
-          nl0 = -s.Ms[0][t];
-          nl1 = -s.Ms[1][t];
-        }
-        const CT len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
-        const CT n[2] = {nl0 / len, nl1 / len};
+        const CT n[2] = {s.Ns[l][0][t], s.Ns[l][1][t]};
+        const CT len = s.Ns[l][2][t];
         CT uL[NF], uR[NF], Fh[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
