@@ -222,54 +222,64 @@ struct TiledStage {
   }
 
   // ---- C1: neighbour traces ---------------------------------------------------------
+  // the neighbour's coefficients of one field evaluated at this face's points,
+  // in the neighbour's local-face frame
   template <int LB, bool SAME>
-  DG_HD static void nb_trace(const StageArgs& a, int nb, int f, CT* out) {
+  DG_HD static void nb_combine(const CT* tb, const CT* v, CT* out) {
     constexpr int OFF = TriBlock::off_phif(P);
-    const CT* tb = table(a);
-    const IT* Uin = (const IT*)a.Uin;
-    CT acc[NQF];
 #pragma unroll
-    for (int q = 0; q < NQF; q++) acc[q] = CT(0);
+    for (int q = 0; q < NQF; q++) {
+      const int qq = SAME ? q : NQF - 1 - q;
+      CT acc = CT(0);
 #pragma unroll
-    for (int k = 0; k < NP; k++) {
-      const CT v = CT(Uin[(long)(f * NP + k) * a.E + nb]);
-#pragma unroll
-      for (int q = 0; q < NQF; q++) {
-        const int qq = SAME ? q : NQF - 1 - q;
-        acc[q] += tb[OFF + (LB * NQF + qq) * NP + k] * v;
-      }
+      for (int k = 0; k < NP; k++) acc += tb[OFF + (LB * NQF + qq) * NP + k] * v[k];
+      out[q] = acc;
     }
-#pragma unroll
-    for (int q = 0; q < NQF; q++) out[q] = acc[q];
   }
 
   DG_HD static void phaseC1(const StageArgs& a, Smem& s, int tid, int tile, const Regs&) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const bool valid = tile * T + t < a.E;
-#pragma unroll 1
+    const int e = tile * T + t;
+    const bool valid = e < a.E;
+    const CT* tb = table(a);
+    const IT* Uin = (const IT*)a.Uin;
+    // issue every neighbour load of this thread before any arithmetic (boundary and
+    // padding lanes read their own element, in bounds, and discard it)
+    CT v[CPW][NP];
+    int lk[CPW];
+#pragma unroll
+    for (int j = 0; j < CPW; j++) {
+      const int it = wave + 4 * j;
+      const int l = it < 3 * NF ? it / NF : 0, f = it < 3 * NF ? it - l * NF : 0;
+      lk[j] = s.Ls[l][t];
+      const int src = (valid && lk[j] >= 0) ? (lk[j] >> 3) : (valid ? e : 0);
+#pragma unroll
+      for (int k = 0; k < NP; k++) v[j][k] = CT(Uin[(long)(f * NP + k) * a.E + src]);
+    }
+#pragma unroll
     for (int j = 0; j < CPW; j++) {
       const int it = wave + 4 * j;
       if (it < 3 * NF) {
         const int l = it / NF, f = it - l * NF;
-        const int link = s.Ls[l][t];
+        const int link = lk[j];
         CT out[NQF];
         if (valid && link >= 0) {
-          const int nb = link >> 3, lb = link & 3;
+          const int lb = link & 3;
           const bool same = (link >> 2) & 1;
           if (!same) {
             if (lb == 0)
-              nb_trace<0, false>(a, nb, f, out);
+              nb_combine<0, false>(tb, v[j], out);
             else if (lb == 1)
-              nb_trace<1, false>(a, nb, f, out);
+              nb_combine<1, false>(tb, v[j], out);
             else
-              nb_trace<2, false>(a, nb, f, out);
+              nb_combine<2, false>(tb, v[j], out);
           } else {
             if (lb == 0)
-              nb_trace<0, true>(a, nb, f, out);
+              nb_combine<0, true>(tb, v[j], out);
             else if (lb == 1)
-              nb_trace<1, true>(a, nb, f, out);
+              nb_combine<1, true>(tb, v[j], out);
             else
-              nb_trace<2, true>(a, nb, f, out);
+              nb_combine<2, true>(tb, v[j], out);
           }
         } else {
 #pragma unroll
@@ -319,15 +329,29 @@ struct TiledStage {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
+    const int es = valid ? e : 0;
     const CT* tb = table(a);
     const IT* Uin = (const IT*)a.Uin;
     const int sc = a.scheme * 4 + a.stage;
+    const bool residual_only = a.scheme == MODE_RESIDUAL;
+    const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
+    const bool reads_acc = sc == 3 * 4 + 1 || sc == 3 * 4 + 2 || sc == 3 * 4 + 3;
     const double dtj = valid ? a.dt / a.el.detj[e] : 0.0;
     g.rsum = 0.0;
 #pragma unroll
     for (int fi = 0; fi < FPW; fi++) {
       const int f = row_f(wave, fi);
       if (f < NF) {
+        // issue the step-start / accumulator loads first; the lift below hides them
+        double un[KPW], ac[KPW], us[KPW];
+#pragma unroll
+        for (int ki = 0; ki < KPW; ki++) {
+          const int k = row_k(wave, ki);
+          const long i = (long)(f * NP + (k < NP ? k : 0)) * E + es;
+          un[ki] = residual_only ? 0.0 : a.Un[i];
+          ac[ki] = reads_acc ? a.acc[i] : 0.0;
+          us[ki] = reads_input ? double(Uin[i]) : 0.0;
+        }
         for (int lq = 0; lq < 3 * NQF; lq++) {
           const CT Fq = s.W[FW + lq * NF + f][t];
 #pragma unroll
@@ -341,13 +365,13 @@ struct TiledStage {
           const int k = row_k(wave, ki);
           if (k < NP && valid) {
             const long i = (long)(f * NP + k) * E + e;
-            const CT acc = g.R[fi * KPW + ki];
-            if (a.scheme == MODE_RESIDUAL) {
-              a.R[i] = double(acc);
+            const CT r = g.R[fi * KPW + ki];
+            if (residual_only) {
+              a.R[i] = double(r);
             } else {
-              const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
-              const double us = reads_input ? double(Uin[i]) : 0.0;
-              const double o = stage_value_hd(sc, double(acc) * dtj, a.Un[i], us, a.acc + i, g.rsum);
+              double accv = ac[ki];
+              const double o = stage_value_hd(sc, double(r) * dtj, un[ki], us[ki], &accv, g.rsum);
+              if (sc >= 3 * 4 && sc < 3 * 4 + 3) a.acc[i] = accv;
               ((OT*)a.Uout)[i] = OT(o);
             }
           }
