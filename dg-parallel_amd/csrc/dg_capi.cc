@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/dg_capi.h"
+#include "dg_adjoint.h"
 #include "dg_fields.h"
 #include "dg_kernels.h"
 #include "dg_setup.h"
@@ -46,6 +47,16 @@ struct dg_ctx {
   hipGraphExec_t gexec = nullptr;
   int g_scheme = -1;
   double g_dt = 0.0;
+  // discrete adjoint (FP64, RK4): checkpoints every adj_every steps, the stage
+  // states of one segment, and the adjoint work vectors
+  double *ckpt = nullptr, *seg = nullptr, *lam = nullptr, *mu = nullptr, *za = nullptr, *zb = nullptr,
+         *wv = nullptr;
+  int adj_nsteps = 0, adj_every = 0, adj_nck = 0;
+  size_t adj_ck_cap = 0, adj_seg_cap = 0;  // doubles allocated for ckpt and seg
+  double adj_dt = 0.0;
+  bool adj_ready = false;
+  std::vector<double> adj_seed;  // dJ/dU(T) of the density-misfit objective, caller layout
+  float adj_fwd_ms = 0.f, adj_bwd_ms = 0.f;
 };
 
 static dg_status fail(dg_ctx* c, dg_status s, const char* fmt, ...) {
@@ -77,6 +88,17 @@ static dg_status dalloc(dg_ctx* c, T** p, size_t n) {
   c->allocs.push_back(q);
   *p = (T*)q;
   return DG_OK;
+}
+
+// free one buffer obtained from dalloc (synchronises the device)
+static void release(dg_ctx* c, void* p) {
+  for (size_t i = 0; i < c->allocs.size(); i++)
+    if (c->allocs[i] == p) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipFree(p);
+      c->allocs.erase(c->allocs.begin() + i);
+      return;
+    }
 }
 
 static void free_all(dg_ctx* c) {
@@ -491,6 +513,237 @@ dg_status dg_l2_error(dg_ctx* c, int field, const double* par, double t, double*
     }
   }
   for (int f = 0; f < NF; f++) err[f] = std::sqrt(acc[f]);
+  return DG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// residual derivatives and the discrete adjoint of RK4 (no counterpart in the
+// reference)
+
+static void to_soa(const dg_ctx* c, const double* U, std::vector<double>& h) {
+  const int E = c->E, BS = c->Nf * c->NP;
+  h.resize(c->ndof);
+  for (int i = 0; i < E; i++)
+    for (int j = 0; j < BS; j++) h[(size_t)j * E + i] = U[(size_t)c->perm[i] * BS + j];
+}
+
+static dg_status upload_soa(dg_ctx* c, const double* U, double* dst) {
+  std::vector<double> h;
+  to_soa(c, U, h);
+  HIPCHECK(c, hipMemcpyAsync(dst, h.data(), c->ndof * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return DG_OK;
+}
+
+static dg_status adj_buffers(dg_ctx* c) {
+  if (c->lam) return DG_OK;
+  dg_status s;
+  const size_t nd = c->ndof;
+  if ((s = dalloc(c, &c->lam, nd)) || (s = dalloc(c, &c->mu, nd)) || (s = dalloc(c, &c->za, nd)) ||
+      (s = dalloc(c, &c->zb, nd)) || (s = dalloc(c, &c->wv, nd)))
+    return s;
+  return DG_OK;
+}
+
+static AdjArgs adj_args(dg_ctx* c) {
+  AdjArgs a{};
+  a.E = c->E;
+  a.P = c->P;
+  a.tab = c->tab;
+  a.el.mg = c->mg;
+  a.el.detj = c->detj;
+  a.el.size = c->size;
+  a.el.link = c->link;
+  a.law = c->lp;
+  a.mode = ADJ_PLAIN;
+  return a;
+}
+
+dg_status dg_residual_jvp(dg_ctx* c, const double* U, const double* dU, double* dR) {
+  if (!ready(c) || !U || !dU || !dR) return fail(c, DG_ERR_ARG, "dg_residual_jvp: bad argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  dg_status s = adj_buffers(c);
+  if (s || (s = upload_soa(c, U, c->lam)) || (s = upload_soa(c, dU, c->za))) return s;
+  AdjArgs a = adj_args(c);
+  a.U = c->lam;
+  a.Z = c->za;
+  a.W = c->wv;
+  if (launch_tri_jvp(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "jvp launch failed");
+  return fetch_soa(c, c->wv, dR);
+}
+
+dg_status dg_residual_vjp(dg_ctx* c, const double* U, const double* z, double* w) {
+  if (!ready(c) || !U || !z || !w) return fail(c, DG_ERR_ARG, "dg_residual_vjp: bad argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  dg_status s = adj_buffers(c);
+  if (s || (s = upload_soa(c, U, c->lam)) || (s = upload_soa(c, z, c->za))) return s;
+  AdjArgs a = adj_args(c);
+  a.U = c->lam;
+  a.Z = c->za;
+  a.W = c->wv;
+  if (launch_tri_vjp(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "vjp launch failed");
+  return fetch_soa(c, c->wv, w);
+}
+
+// one forward RK4 step from seg slot (j,0) writing the stage inputs (j,1..3) and
+// the next step's state to `next`
+static int enqueue_rk4_stored(dg_ctx* c, double dt, double* s0, double* s1, double* s2, double* s3, double* next) {
+  StageArgs a = base_args(c);
+  a.dt = dt;
+  a.scheme = DG_RK4;
+  a.Un = s0;
+  double* in[4] = {s0, s1, s2, s3};
+  double* out[4] = {s1, s2, s3, next};
+  for (int st = 0; st < 4; st++) {
+    if (!out[st]) break;
+    a.stage = st;
+    a.Uin = in[st];
+    a.Uout = out[st];
+    a.resid_part = nullptr;
+    if (launch_stage(c, a, 0)) return -1;
+  }
+  return 0;
+}
+
+// mass-weighted density misfit J = 1/2 sum_e |J_e| sum_k (rho_ek - rho*_ek)^2 and its
+// gradient |J_e| (rho_ek - rho*_ek) (zero on the other fields), caller layout
+static double density_misfit(const dg_ctx* c, const double* U, const double* target, double* grad) {
+  const int BS = c->Nf * c->NP;
+  double J = 0.0;
+  std::fill(grad, grad + c->ndof, 0.0);
+  for (int e = 0; e < c->E; e++) {
+    const double* X = &c->xyz[(size_t)e * 6];
+    const double ad = std::fabs((X[2] - X[0]) * (X[5] - X[1]) - (X[4] - X[0]) * (X[3] - X[1]));
+    for (int k = 0; k < c->NP; k++) {
+      const size_t i = (size_t)e * BS + k;
+      const double d = U[i] - target[i];
+      J += 0.5 * ad * d * d;
+      grad[i] = ad * d;
+    }
+  }
+  return J;
+}
+
+dg_status dg_adjoint_forward(dg_ctx* c, double dt, int nsteps, int ckpt_every, const double* target, double* J_out) {
+  if (!ready(c) || !(dt > 0) || nsteps <= 0 || ckpt_every <= 0)
+    return fail(c, DG_ERR_ARG, "dg_adjoint_forward: bad argument");
+  if (c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the FP64 form");
+  HIPCHECK(c, hipSetDevice(c->device));
+  c->adj_ready = false;
+  dg_status s = adj_buffers(c);
+  if (s) return s;
+  const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
+  const size_t need_ck = c->ndof * (size_t)nck, need_seg = c->ndof * (size_t)4 * std::min(ckpt_every, nsteps);
+  if (need_ck > c->adj_ck_cap) {
+    if (c->ckpt) release(c, c->ckpt);
+    c->ckpt = nullptr;
+    if ((s = dalloc(c, &c->ckpt, need_ck))) return s;
+    c->adj_ck_cap = need_ck;
+  }
+  if (need_seg > c->adj_seg_cap) {
+    if (c->seg) release(c, c->seg);
+    c->seg = nullptr;
+    if ((s = dalloc(c, &c->seg, need_seg))) return s;
+    c->adj_seg_cap = need_seg;
+  }
+  c->adj_nck = nck;
+  c->adj_every = ckpt_every;
+  c->adj_nsteps = nsteps;
+  c->adj_dt = dt;
+  hipEvent_t e0, e1;
+  HIPCHECK(c, hipEventCreate(&e0));
+  HIPCHECK(c, hipEventCreate(&e1));
+  HIPCHECK(c, hipEventRecord(e0, c->stream));
+  for (int st = 0; st < nsteps; st++) {
+    if (st % ckpt_every == 0)
+      HIPCHECK(c, hipMemcpyAsync(c->ckpt + (size_t)(st / ckpt_every) * c->ndof, c->Un, c->ndof * 8,
+                                 hipMemcpyDeviceToDevice, c->stream));
+    if (enqueue_step(c, DG_RK4, dt, false)) return fail(c, DG_ERR_HIP, "forward step failed");
+  }
+  HIPCHECK(c, hipEventRecord(e1, c->stream));
+  HIPCHECK(c, hipEventSynchronize(e1));
+  HIPCHECK(c, hipEventElapsedTime(&c->adj_fwd_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->adj_seed.clear();
+  if (target) {
+    std::vector<double> U(c->ndof);
+    if ((s = dg_get_state(c, U.data(), U.size()))) return s;
+    c->adj_seed.resize(c->ndof);
+    const double J = density_misfit(c, U.data(), target, c->adj_seed.data());
+    if (J_out) *J_out = J;
+  } else if (J_out) {
+    *J_out = 0.0;
+  }
+  c->adj_ready = true;
+  return DG_OK;
+}
+
+dg_status dg_adjoint_backward(dg_ctx* c, const double* dJdUT, double* dJdU0) {
+  if (!ready(c) || !dJdU0) return fail(c, DG_ERR_ARG, "dg_adjoint_backward: bad argument");
+  if (!c->adj_ready) return fail(c, DG_ERR_ARG, "dg_adjoint_backward needs a preceding dg_adjoint_forward");
+  if (!dJdUT && c->adj_seed.empty())
+    return fail(c, DG_ERR_ARG, "dg_adjoint_backward: no seed (pass dJdUT or a target to the forward)");
+  HIPCHECK(c, hipSetDevice(c->device));
+  dg_status s = upload_soa(c, dJdUT ? dJdUT : c->adj_seed.data(), c->lam);
+  if (s) return s;
+  const size_t nd = c->ndof;
+  const double dt = c->adj_dt;
+  const int C = c->adj_every, N = c->adj_nsteps;
+  hipEvent_t e0, e1;
+  HIPCHECK(c, hipEventCreate(&e0));
+  HIPCHECK(c, hipEventCreate(&e1));
+  HIPCHECK(c, hipEventRecord(e0, c->stream));
+  double* lam = c->lam;
+  double* mu = c->mu;
+  for (int ck = c->adj_nck - 1; ck >= 0; ck--) {
+    const int m = std::min(N, ck * C + C) - ck * C;
+    // recompute the segment's stage states from its checkpoint (the last step's
+    // output is not needed)
+    HIPCHECK(c, hipMemcpyAsync(c->seg, c->ckpt + (size_t)ck * nd, nd * 8, hipMemcpyDeviceToDevice, c->stream));
+    for (int j = 0; j < m; j++) {
+      double* b = c->seg + (size_t)j * 4 * nd;
+      double* next = (j + 1 < m) ? c->seg + (size_t)(j + 1) * 4 * nd : nullptr;
+      if (enqueue_rk4_stored(c, dt, b, b + nd, b + 2 * nd, b + 3 * nd, next))
+        return fail(c, DG_ERR_HIP, "recompute failed");
+    }
+    // reverse sweep: z_3 = dt/|J| lam/6, then stages 3..0 of the fused adjoint kernel
+    for (int j = m - 1; j >= 0; j--) {
+      double* b = c->seg + (size_t)j * 4 * nd;
+      if (launch_tri_adj_scale(c->E, c->Nf * c->NP, 1.0 / 6.0, dt, c->detj, lam, c->za, c->stream))
+        return fail(c, DG_ERR_HIP, "adjoint seed failed");
+      AdjArgs a = adj_args(c);
+      a.mode = ADJ_RK4;
+      a.dt = dt;
+      a.lam = lam;
+      a.mu = mu;
+      double* zin = c->za;
+      double* zout = c->zb;
+      for (int st = 3; st >= 0; st--) {
+        a.stage = st;
+        a.U = b + (size_t)st * nd;
+        a.Z = zin;
+        a.znext = zout;
+        if (launch_tri_vjp(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "adjoint stage failed");
+        std::swap(zin, zout);
+      }
+      std::swap(lam, mu);
+    }
+  }
+  HIPCHECK(c, hipEventRecord(e1, c->stream));
+  HIPCHECK(c, hipEventSynchronize(e1));
+  HIPCHECK(c, hipEventElapsedTime(&c->adj_bwd_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->lam = lam;
+  c->mu = mu;
+  return fetch_soa(c, lam, dJdU0);
+}
+
+dg_status dg_adjoint_timing(const dg_ctx* c, float* fwd_ms, float* bwd_ms) {
+  if (!c || !fwd_ms || !bwd_ms) return DG_ERR_ARG;
+  *fwd_ms = c->adj_fwd_ms;
+  *bwd_ms = c->adj_bwd_ms;
   return DG_OK;
 }
 

@@ -91,6 +91,12 @@ _compute_dt = _sig("dg_compute_dt", ctypes.c_int, [_vp, ctypes.c_double, _p_f64]
 _advance = _sig("dg_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _p_f64])
 _time_advance = _sig("dg_time_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                        ctypes.POINTER(ctypes.c_float)])
+_jvp = _sig("dg_residual_jvp", ctypes.c_int, [_vp, _p_f64, _p_f64, _p_f64])
+_vjp = _sig("dg_residual_vjp", ctypes.c_int, [_vp, _p_f64, _p_f64, _p_f64])
+_adj_fwd = _sig("dg_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _p_f64, _p_f64])
+_adj_bwd = _sig("dg_adjoint_backward", ctypes.c_int, [_vp, _p_f64, _p_f64])
+_adj_timing = _sig("dg_adjoint_timing", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float),
+                                                       ctypes.POINTER(ctypes.c_float)])
 _copy_probe = _sig("dg_copy_probe", ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_float)])
 _project = _sig("dg_project", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double])
 _l2_error = _sig("dg_l2_error", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64])
@@ -252,3 +258,39 @@ class DGAnalysis:
         ms = ctypes.c_float()
         self._check(_time_advance(self._h, scheme, dt, nsteps, ctypes.byref(ms)))
         return ms.value
+
+    # ---- derivatives (FP64) ----
+    def residual_jvp(self, U, dU):
+        """dR = (dR/dU)(U) dU, boundary layout"""
+        U, dU = np.ascontiguousarray(U, np.float64), np.ascontiguousarray(dU, np.float64)
+        out = np.zeros(self.ndof)
+        self._check(_jvp(self._h, _ptr(U, _p_f64), _ptr(dU, _p_f64), _ptr(out, _p_f64)))
+        return out
+
+    def residual_vjp(self, U, z):
+        """w = (dR/dU)(U)^T z, boundary layout"""
+        U, z = np.ascontiguousarray(U, np.float64), np.ascontiguousarray(z, np.float64)
+        out = np.zeros(self.ndof)
+        self._check(_vjp(self._h, _ptr(U, _p_f64), _ptr(z, _p_f64), _ptr(out, _p_f64)))
+        return out
+
+    def adjoint_forward(self, dt, nsteps, ckpt_every, target=None):
+        """nsteps RK4 steps from the current state with checkpoints; returns the
+        density misfit J against target (0.0 without one)"""
+        J = ctypes.c_double()
+        t = None if target is None else np.ascontiguousarray(target, np.float64)
+        self._check(_adj_fwd(self._h, dt, nsteps, ckpt_every, _ptr(t, _p_f64), ctypes.byref(J)))
+        return J.value
+
+    def adjoint_backward(self, seed=None):
+        """dJ/dU(0) for the seed dJ/dU(T) (None: the gradient of the forward's misfit)"""
+        s = None if seed is None else np.ascontiguousarray(seed, np.float64)
+        out = np.zeros(self.ndof)
+        self._check(_adj_bwd(self._h, _ptr(s, _p_f64), _ptr(out, _p_f64)))
+        return out
+
+    def adjoint_timing(self):
+        """device milliseconds of the last adjoint_forward and adjoint_backward"""
+        f, b = ctypes.c_float(), ctypes.c_float()
+        self._check(_adj_timing(self._h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value

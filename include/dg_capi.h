@@ -124,6 +124,25 @@ dg_status dg_get_state_device(dg_ctx* ctx, double* dU);
  * stream between two HIP events and returns the elapsed milliseconds. */
 dg_status dg_time_advance(dg_ctx* ctx, int rk_scheme, double dt, int nsteps, float* ms_out);
 
+/* ---- derivatives (no counterpart in the reference) ------------------------
+ * Tangent-linear and adjoint residual products at a state U (boundary layout,
+ * host buffers, FP64): dR = (dR/dU) dU and w = (dR/dU)^T z.  For callers that
+ * differentiate the residual (tests, optimisation drivers). */
+dg_status dg_residual_jvp(dg_ctx* ctx, const double* U, const double* dU, double* dR);
+dg_status dg_residual_vjp(dg_ctx* ctx, const double* U, const double* z, double* w);
+
+/* Discrete adjoint of nsteps RK4 steps (TimeIntegrator.cc:81-275) of the
+ * unlimited residual, FP64 form.  dg_adjoint_forward advances the current state
+ * nsteps steps of dt, keeping a checkpoint every ckpt_every steps on the device;
+ * with a target state (boundary layout) it returns the objective
+ * J = 1/2 sum_e |J_e| sum_k (rho_ek - rho*_ek)^2.  dg_adjoint_backward returns
+ * dJ/dU(0) for the seed dJ/dU(T) (NULL: the gradient of that objective),
+ * recomputing each segment's stage states from its checkpoint. */
+dg_status dg_adjoint_forward(dg_ctx* ctx, double dt, int nsteps, int ckpt_every, const double* target, double* J_out);
+dg_status dg_adjoint_backward(dg_ctx* ctx, const double* dJdUT, double* dJdU0);
+/* device times of the last forward and backward calls (HIP events) */
+dg_status dg_adjoint_timing(const dg_ctx* ctx, float* fwd_ms, float* bwd_ms);
+
 /* Measurement helper (no counterpart in the reference): one streaming copy of
  * n_doubles between two fresh device buffers; the reference point that scales
  * the HBM byte counters of the profiler.  Returns the kernel time. */

@@ -1,5 +1,6 @@
-// CPU emulation of the tiled triangle stage kernel: runs the kernel's own phase
-// functions (dg-parallel_amd/csrc/dg_stage_tiled.h) thread by thread on the host,
+// CPU emulation of the tiled triangle stage kernel and of the residual-derivative
+// kernels: runs the kernels' own phase functions (dg-parallel_amd/csrc/
+// dg_stage_tiled.h, dg_adjoint.h) thread by thread on the host,
 // with the same host-built tables the library uploads.  Test infrastructure:
 // lets the CPU test suite check the kernel arithmetic and indexing against the
 // oracle without a GPU.
@@ -8,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "../../dg-parallel_amd/csrc/dg_adjoint.h"
 #include "../../dg-parallel_amd/csrc/dg_setup.h"
 #include "../../dg-parallel_amd/csrc/dg_stage_tiled.h"
 
@@ -84,10 +86,24 @@ struct Emu {
     a.acc = acc.data();
     return a;
   }
-  void to_soa(const double* U) {
+  AdjArgs adj() {
+    AdjArgs a{};
+    a.E = S.E;
+    a.P = S.P;
+    a.tab = S.tab.data();
+    a.el.mg = S.mg.data();
+    a.el.detj = S.detj.data();
+    a.el.size = S.size.data();
+    a.el.link = S.link.data();
+    a.law = S.lp;
+    a.mode = ADJ_PLAIN;
+    return a;
+  }
+  void to_soa(const double* U) { to_soa(U, Un.data()); }
+  void to_soa(const double* U, double* dst) {
     const int BS = S.NF * S.NP;
     for (int i = 0; i < S.E; i++)
-      for (int j = 0; j < BS; j++) Un[(size_t)j * S.E + i] = U[(size_t)S.perm[i] * BS + j];
+      for (int j = 0; j < BS; j++) dst[(size_t)j * S.E + i] = U[(size_t)S.perm[i] * BS + j];
   }
   void from_soa(const double* src, double* U) {
     const int BS = S.NF * S.NP;
@@ -95,6 +111,58 @@ struct Emu {
       for (int j = 0; j < BS; j++) U[(size_t)S.perm[i] * BS + j] = src[(size_t)j * S.E + i];
   }
 };
+
+template <int P, class LAW>
+void run_vjp_tiles(const AdjArgs& a) {
+  using K = TiledVjp<P, LAW>;
+  std::unique_ptr<typename K::Smem> sm(new typename K::Smem());
+  std::vector<typename K::Regs> g(TILE_THREADS);
+  const int ntiles = (a.E + TILE_ELEMS - 1) / TILE_ELEMS;
+  for (int tile = 0; tile < ntiles; tile++) {
+    std::memset(sm.get(), 0, sizeof(typename K::Smem));
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, *sm, t, tile);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, *sm, t);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseD1(a, *sm, t, g[t]);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, *sm, t, tile);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseD2(a, *sm, t, tile, g[t]);
+  }
+}
+
+template <class LAW>
+int run_vjp_law(const AdjArgs& a) {
+  switch (a.P) {
+    case 1: run_vjp_tiles<1, LAW>(a); return 0;
+    case 2: run_vjp_tiles<2, LAW>(a); return 0;
+    case 3: run_vjp_tiles<3, LAW>(a); return 0;
+    case 4: run_vjp_tiles<4, LAW>(a); return 0;
+  }
+  return -1;
+}
+
+int run_vjp(const AdjArgs& a, int law) {
+  if (law == LAW_SCALAR) return run_vjp_law<ScalarLawD>(a);
+  if (law == LAW_EULER2D) return run_vjp_law<EulerLawD>(a);
+  return -1;
+}
+
+template <class LAW>
+int run_jvp_law(const AdjArgs& a) {
+  for (int e = 0; e < a.E; e++) switch (a.P) {
+      case 1: jvp_element<1, LAW>(a, e); break;
+      case 2: jvp_element<2, LAW>(a, e); break;
+      case 3: jvp_element<3, LAW>(a, e); break;
+      case 4: jvp_element<4, LAW>(a, e); break;
+      default: return -1;
+    }
+  return 0;
+}
+
+int run_jvp(const AdjArgs& a, int law) {
+  if (law == LAW_SCALAR) return run_jvp_law<ScalarLawD>(a);
+  if (law == LAW_EULER2D) return run_jvp_law<EulerLawD>(a);
+  return -1;
+}
 
 int make(Emu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int order) {
   dg_options opt{};
@@ -160,5 +228,74 @@ extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, i
   for (int i = 0; i < (m.S.E + TILE_ELEMS - 1) / TILE_ELEMS; i++) r += m.part[i];
   if (resid) *resid = r;
   m.from_soa(m.Un.data(), U);
+  return 0;
+}
+
+// tangent-linear (jvp=1: dR = A dU) or adjoint (jvp=0: w = A^T z) residual product
+extern "C" int emu_tri_derivative(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int jvp,
+                                  const double* U, const double* V, double* W) {
+  Emu m;
+  if (make(m, mesh, law, order)) return -1;
+  const size_t n = m.Un.size();
+  std::vector<double> u(n), v(n), w(n, 0.0);
+  m.to_soa(U, u.data());
+  m.to_soa(V, v.data());
+  AdjArgs a = m.adj();
+  a.U = u.data();
+  a.Z = v.data();
+  a.W = w.data();
+  if ((jvp ? run_jvp(a, m.S.law) : run_vjp(a, m.S.law))) return -2;
+  m.from_soa(w.data(), W);
+  return 0;
+}
+
+// gradient dJ/dU(0) of <seed, U(T)> over nsteps RK4 steps: the forward stage
+// kernel with every stage state stored, then the fused reverse stages (the
+// library's dg_adjoint_backward sequence without checkpointing)
+extern "C" int emu_tri_adjoint_rk4(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, double dt,
+                                   int nsteps, const double* U0, const double* seed, double* UT, double* grad) {
+  Emu m;
+  if (make(m, mesh, law, order)) return -1;
+  const size_t n = m.Un.size();
+  std::vector<double> st((size_t)nsteps * 4 * n + n);
+  m.to_soa(U0, st.data());
+  for (int j = 0; j < nsteps; j++) {
+    double* b = st.data() + (size_t)j * 4 * n;
+    StageArgs a = m.base();
+    a.dt = dt;
+    a.scheme = 3;
+    a.Un = b;
+    for (int s = 0; s < 4; s++) {
+      a.stage = s;
+      a.Uin = b + (size_t)s * n;
+      a.Uout = s < 3 ? b + (size_t)(s + 1) * n : b + 4 * n;
+      if (run_variant(a, m.S.law, 0)) return -2;
+    }
+  }
+  m.from_soa(st.data() + (size_t)nsteps * 4 * n, UT);
+  std::vector<double> lam(n), mu(n), za(n), zb(n);
+  m.to_soa(seed, lam.data());
+  const int BS = m.S.NF * m.S.NP;
+  for (int j = nsteps - 1; j >= 0; j--) {
+    for (int r = 0; r < BS; r++)
+      for (int e = 0; e < m.S.E; e++) za[(size_t)r * m.S.E + e] = 1.0 / 6.0 * dt / m.S.detj[e] * lam[(size_t)r * m.S.E + e];
+    AdjArgs a = m.adj();
+    a.mode = ADJ_RK4;
+    a.dt = dt;
+    a.lam = lam.data();
+    a.mu = mu.data();
+    double* zin = za.data();
+    double* zout = zb.data();
+    for (int s = 3; s >= 0; s--) {
+      a.stage = s;
+      a.U = st.data() + ((size_t)j * 4 + s) * n;
+      a.Z = zin;
+      a.znext = zout;
+      if (run_vjp(a, m.S.law)) return -2;
+      std::swap(zin, zout);
+    }
+    std::swap(lam, mu);
+  }
+  m.from_soa(lam.data(), grad);
   return 0;
 }

@@ -99,3 +99,27 @@ def collapsed_rule(n):
             pts.append((u, v))
             wts.append(wa * wb * (1 - a))
     return np.array(pts), np.array(wts)
+
+
+# far-field state off the vortex's free stream: where the two agree, the
+# Lax-Friedrichs wave-speed maximum ties on the boundary faces and the discrete
+# residual has a kink within reach of the finite-difference step
+FF_OFF = (1.0, 0.8, 0.6, 3.0)
+
+
+def roughen(U, p, seed=7):
+    """the projected field plus a seeded perturbation of its three linear
+    coefficients per element and field (0.5% of its scale): the face jumps and
+    wave speeds of neighbouring traces then differ by far more than a
+    finite-difference step, so no maximum or upwind switch ties within reach of
+    it (as it would in an exactly uniform free stream)"""
+    np_ = (p + 1) * (p + 2) // 2
+    V = U.reshape(-1, np_).copy()
+    V[:, :3] += 0.005 * np.abs(U).max() * np.random.default_rng(seed).standard_normal((V.shape[0], 3))
+    return V.reshape(U.shape)
+
+
+def direction(U, seed):
+    """a random direction scaled per entry to the state (keeps the Euler states physical)"""
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(U.shape) * (0.05 * np.abs(U).max())

@@ -31,7 +31,7 @@ LIBEMU = _lib() if os.path.exists(LIB) else None
 FF = (1.0, 1.0, 1.0, 3.5)
 
 
-def case(law, flux, p, periodic=True, nx=8):
+def case(law, flux, p, periodic=True, nx=8, ff=FF):
     tags = (40000, 7, 40000, 7)
     if law == "scalar":
         vids, xyz, bc = mesh_tri(nx, nx, periodic_x=periodic, periodic_y=periodic, jitter=0.1, seed=3, side_tags=tags)
@@ -42,8 +42,8 @@ def case(law, flux, p, periodic=True, nx=8):
         vids, xyz, bc = mesh_tri(nx, nx, lx=10, ly=10, periodic_x=periodic, periodic_y=periodic, jitter=0.1, seed=3,
                                  side_tags=tags)
         o = po.Oracle(2, vids, xyz, p, po.LAW_EULER2D, po.FLUX_LF if flux == "lf" else po.FLUX_ROE, gamma=1.4,
-                      farfield=FF, bc=bc)
-        lw = dgpar.Law(dgpar.LAW_EULER2D, dgpar.FLUX_LF if flux == "lf" else dgpar.FLUX_ROE, gamma=1.4, farfield=FF)
+                      farfield=ff, bc=bc)
+        lw = dgpar.Law(dgpar.LAW_EULER2D, dgpar.FLUX_LF if flux == "lf" else dgpar.FLUX_ROE, gamma=1.4, farfield=ff)
         U = o.project(1, VORTEX)
     return dgpar.Mesh(2, vids, xyz, bc), lw, o, U
 
