@@ -50,6 +50,21 @@ def bytes_per_dof_stage(fp32, nf, np_):
     return sum(st) / 4.0 + table
 
 
+def measured_traffic(form):
+    """HBM bytes per launch of the stage kernel for this workload, from the two
+    rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE, each scaled by what the
+    same counter reads for a 512 MiB device copy) summarised under profiles/."""
+    path = os.path.join(ROOT, "profiles", "hbm_counters_tri_stage_tiled_euler_vortex_p3.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch_%s:" % form):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def run_form(dgpar, mesh, precision, steps, warmup, kernel=0):
     law = dgpar.EulerLaw(dgpar.FLUX_LF, 1.4)
     g = dgpar.DGAnalysis(mesh, law, ORDER, precision=precision, use_graph=True, device=0, kernel=kernel)
@@ -172,11 +187,12 @@ def main():
         "config": {"workload": "euler2d-isentropic-vortex P3 LF RK4 CFL0.1", "elements": E, "dof": ndof,
                    "mesh": "256x256x2 periodic [0,10]^2 jitter 0.1h seed %d" % SEED, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": measured_traffic("fp32") if args.kernel == 0 else None,
                      "kernel": ("tri_stage_tiled" if args.kernel == 0 else "tri_stage_kernel") + "<3,EulerLawD,float,*>",
                      "bytes_per_dof_stage": b32},
         "fp64_plain": {"value": v64, "ms_per_step": r64["ms"] / args.steps, "achieved_GBs": ach64,
-                       "frac": ach64 / HBM_PEAK_GBS, "bytes_per_dof_stage": b64},
+                       "frac": ach64 / HBM_PEAK_GBS, "bytes_per_dof_stage": b64,
+                       "traffic": measured_traffic("fp64") if args.kernel == 0 else None},
         "ratio_fp32_over_fp64": v32 / v64,
         "l2_rho_vs_exact": {"fp32": float(r32["err"][0]), "fp64": float(r64["err"][0])},
         "l2_rho_fp32_vs_fp64": l2_rho_diff,

@@ -40,16 +40,23 @@ struct AdjArgs {
   double* znext;      // dt/|J| * kbar of the next (earlier) stage
 };
 
-template <int P, class LAW>
+// ND: tangent directions carried per flux evaluation (a divisor of NF).  The
+// flux Jacobian at a point is built in NF/ND passes of Dual<double, ND>: fewer
+// directions per pass re-evaluate the flux more often but keep fewer registers
+// live (the Euler flux with 4 directions needs more than 256 VGPRs).
+template <int P, class LAW, int ND = LAW::NF>
 struct TiledVjp {
   static constexpr int NP = np_tri(P), NQ = nqv_tri(P), NQF = nqf_tri(P), NF = LAW::NF;
+  static_assert(NF % ND == 0, "ND must divide NF");
   static constexpr int T = TILE_ELEMS;
   static constexpr int NROW = NF * NP;
   static constexpr int RPW = (NROW + 3) / 4;
-  static constexpr int FW = 3 * NQF * NF;
+  static constexpr int FW = NQF * NF;  // one face: points x fields
   static constexpr int WORK = NQ * NF > 2 * FW ? NQ * NF : 2 * FW;
-  using D = Dual<double, NF>;
+  using D = Dual<double, ND>;
 
+  // the face term runs one local face at a time (C1 -> C2 -> lift, per face), so
+  // the work area holds one face's traces: 67 KB at P3 Euler, two workgroups per CU
   struct Smem {
     double Us[NROW][T];
     double Zs[NROW][T];
@@ -78,8 +85,9 @@ struct TiledVjp {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const double* tb = a.tab;
     for (int q = wave; q < NQ; q += 4) {
-      D u[NF];
-      double Z0[NF], Z1[NF];
+      double uq[NF], Y0[NF], Y1[NF];
+      const double m00 = s.Ms[0][t], m01 = s.Ms[1][t], m10 = s.Ms[2][t], m11 = s.Ms[3][t];
+#pragma unroll
       for (int f = 0; f < NF; f++) {
         double acc = 0, z0 = 0, z1 = 0;
 #pragma unroll
@@ -88,27 +96,31 @@ struct TiledVjp {
           z0 += tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 0] * s.Zs[f * NP + k][t];
           z1 += tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 1] * s.Zs[f * NP + k][t];
         }
-        u[f] = D(acc);
-        u[f].d[f] = 1.0;
-        Z0[f] = z0;
-        Z1[f] = z1;
+        uq[f] = acc;
+        Y0[f] = m00 * z0 + m10 * z1;
+        Y1[f] = m01 * z0 + m11 * z1;
       }
-      D F[NF][2];
-      LAW::flux(a.law, u, F);
       const double w = tb[TriBlock::off_wv(P) + q];
-      const double m00 = s.Ms[0][t], m01 = s.Ms[1][t], m10 = s.Ms[2][t], m11 = s.Ms[3][t];
-      double ub[NF];
+#pragma unroll 1
+      for (int j0 = 0; j0 < NF; j0 += ND) {
+        D u[NF];
 #pragma unroll
-      for (int j = 0; j < NF; j++) ub[j] = 0.0;
+        for (int f = 0; f < NF; f++) {
+          u[f] = D(uq[f]);
 #pragma unroll
-      for (int f = 0; f < NF; f++) {
-        const double Y0 = m00 * Z0[f] + m10 * Z1[f];
-        const double Y1 = m01 * Z0[f] + m11 * Z1[f];
+          for (int i = 0; i < ND; i++)
+            if (f == j0 + i) u[f].d[i] = 1.0;
+        }
+        D F[NF][2];
+        LAW::flux(a.law, u, F);
 #pragma unroll
-        for (int j = 0; j < NF; j++) ub[j] += Y0 * F[f][0].d[j] + Y1 * F[f][1].d[j];
+        for (int i = 0; i < ND; i++) {
+          double ub = 0.0;
+#pragma unroll
+          for (int f = 0; f < NF; f++) ub += Y0[f] * F[f][0].d[i] + Y1[f] * F[f][1].d[i];
+          s.W[q * NF + j0 + i][t] = w * ub;
+        }
       }
-#pragma unroll
-      for (int j = 0; j < NF; j++) s.W[q * NF + j][t] = w * ub[j];
     }
   }
 
@@ -131,35 +143,33 @@ struct TiledVjp {
   DG_HD static void nb_trace2(const AdjArgs& a, int nb, int f, double* ou, double* oz) {
     constexpr int OFF = TriBlock::off_phif(P);
     const double* tb = a.tab;
-    double au[NQF], az[NQF];
-#pragma unroll
-    for (int q = 0; q < NQF; q++) au[q] = az[q] = 0.0;
+    double vu[NP], vz[NP];
 #pragma unroll
     for (int k = 0; k < NP; k++) {
-      const double vu = a.U[(long)(f * NP + k) * a.E + nb];
-      const double vz = a.Z[(long)(f * NP + k) * a.E + nb];
-#pragma unroll
-      for (int q = 0; q < NQF; q++) {
-        const int qq = SAME ? q : NQF - 1 - q;
-        const double phi = tb[OFF + (LB * NQF + qq) * NP + k];
-        au[q] += phi * vu;
-        az[q] += phi * vz;
-      }
+      vu[k] = a.U[(long)(f * NP + k) * a.E + nb];
+      vz[k] = a.Z[(long)(f * NP + k) * a.E + nb];
     }
 #pragma unroll
     for (int q = 0; q < NQF; q++) {
-      ou[q] = au[q];
-      oz[q] = az[q];
+      const int qq = SAME ? q : NQF - 1 - q;
+      double au = 0.0, az = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; k++) {
+        const double phi = tb[OFF + (LB * NQF + qq) * NP + k];
+        au += phi * vu[k];
+        az += phi * vz[k];
+      }
+      ou[q] = au;
+      oz[q] = az;
     }
   }
 
-  // neighbour traces of the state (rows [0, FW)) and of the seed (rows [FW, 2FW))
-  DG_HD static void phaseC1(const AdjArgs& a, Smem& s, int tid, int tile) {
+  // face l: neighbour traces of the state (rows [0, FW)) and of the seed (rows [FW, 2FW))
+  DG_HD static void phaseC1(const AdjArgs& a, Smem& s, int tid, int tile, int l) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const bool valid = tile * T + t < a.E;
-    for (int it = wave; it < 3 * NF; it += 4) {
-      const int l = it / NF, f = it - l * NF;
-      const int link = s.Ls[l][t];
+    const int link = s.Ls[l][t];
+    for (int f = wave; f < NF; f += 4) {
       double ou[NQF], oz[NQF];
       if (valid && link >= 0) {
         const int nb = link >> 3, lb = link & 3;
@@ -185,35 +195,34 @@ struct TiledVjp {
       }
 #pragma unroll
       for (int q = 0; q < NQF; q++) {
-        s.W[(l * NQF + q) * NF + f][t] = ou[q];
-        s.W[FW + (l * NQF + q) * NF + f][t] = oz[q];
+        s.W[q * NF + f][t] = ou[q];
+        s.W[FW + q * NF + f][t] = oz[q];
       }
     }
   }
 
-  // face points: ubar = J^T (zbar_own - zbar_nb), written over the state traces
-  DG_HD static void phaseC2(const AdjArgs& a, Smem& s, int tid) {
+  // face l points: ubar = J^T (zbar_own - zbar_nb), written over the state traces
+  // (each point's rows are read and rewritten by the one thread that owns it)
+  DG_HD static void phaseC2(const AdjArgs& a, Smem& s, int tid, int l) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const double* tb = a.tab;
-    for (int it = wave; it < 3 * NQF; it += 4) {
-      const int l = it / NQF, q = it - l * NQF;
-      const int link = s.Ls[l][t];
-      double nl0, nl1;
-      if (l == 0) {
-        nl0 = -s.Ms[2][t];
-        nl1 = -s.Ms[3][t];
-      } else if (l == 1) {
-        nl0 = s.Ms[0][t] + s.Ms[2][t];
-        nl1 = s.Ms[1][t] + s.Ms[3][t];
-      } else {
-        nl0 = -s.Ms[0][t];
-        nl1 = -s.Ms[1][t];
-      }
-      const double len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
+    const int link = s.Ls[l][t];
+    double nl0, nl1;
+    if (l == 0) {
+      nl0 = -s.Ms[2][t];
+      nl1 = -s.Ms[3][t];
+    } else if (l == 1) {
+      nl0 = s.Ms[0][t] + s.Ms[2][t];
+      nl1 = s.Ms[1][t] + s.Ms[3][t];
+    } else {
+      nl0 = -s.Ms[0][t];
+      nl1 = -s.Ms[1][t];
+    }
+    const double len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
+    for (int q = wave; q < NQF; q += 4) {
       const D n[2] = {D(nl0 / len), D(nl1 / len)};
       const double sw = len * tb[TriBlock::off_wf(P) + q];
-      D uL[NF], uR[NF], Fh[NF];
-      double zb[NF];
+      double uLq[NF], uRq[NF], zb[NF];
 #pragma unroll
       for (int f = 0; f < NF; f++) {
         double au = 0, az = 0;
@@ -223,42 +232,64 @@ struct TiledVjp {
           au += phi * s.Us[f * NP + k][t];
           az += phi * s.Zs[f * NP + k][t];
         }
-        uL[f] = D(au);
-        uL[f].d[f] = 1.0;
-        uR[f] = D(s.W[(l * NQF + q) * NF + f][t]);
-        const double znb = link >= 0 ? s.W[FW + (l * NQF + q) * NF + f][t] : 0.0;
+        uLq[f] = au;
+        uRq[f] = s.W[q * NF + f][t];
+        const double znb = link >= 0 ? s.W[FW + q * NF + f][t] : 0.0;
         zb[f] = -sw * (az - znb);
       }
-      if (link >= 0)
+      // the point's rows are rewritten in place: its traces are in registers now
+#pragma unroll 1
+      for (int j0 = 0; j0 < NF; j0 += ND) {
+        D uL[NF], uR[NF], Fh[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          uL[f] = D(uLq[f]);
+          uR[f] = D(uRq[f]);
+#pragma unroll
+          for (int i = 0; i < ND; i++)
+            if (f == j0 + i) uL[f].d[i] = 1.0;
+        }
+        if (link < 0) LAW::ghost(a.law, -link, n, uL, uR);
         LAW::riemann(a.law, n, uL, uR, Fh);
-      else
-        LAW::boundary(a.law, -link, n, uL, Fh);
 #pragma unroll
-      for (int j = 0; j < NF; j++) {
-        double acc = 0.0;
+        for (int i = 0; i < ND; i++) {
+          double acc = 0.0;
 #pragma unroll
-        for (int f = 0; f < NF; f++) acc += Fh[f].d[j] * zb[f];
-        s.W[(l * NQF + q) * NF + j][t] = acc;
+          for (int f = 0; f < NF; f++) acc += Fh[f].d[i] * zb[f];
+          s.W[q * NF + j0 + i][t] = acc;
+        }
       }
     }
   }
 
-  DG_HD static void phaseD2(const AdjArgs& a, Smem& s, int tid, int tile, Regs& g) {
+  // face l: lift transpose into the register rows
+  DG_HD static void phaseLift(const AdjArgs& a, Smem& s, int tid, int l, Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const int E = a.E, e = tile * T + t;
-    const bool valid = e < E;
     const double* tb = a.tab;
-    const double dtj = valid ? a.dt / a.el.detj[e] : 0.0;
 #pragma unroll
     for (int j = 0; j < RPW; j++) {
       const int row = wave + 4 * j;
-      if (row < NROW && valid) {
+      if (row < NROW) {
         const int f = row / NP, k = row - f * NP;
         double acc = g.R[j];
-        for (int l = 0; l < 3; l++)
 #pragma unroll
-          for (int q = 0; q < NQF; q++)
-            acc += tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * s.W[(l * NQF + q) * NF + f][t];
+        for (int q = 0; q < NQF; q++) acc += tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * s.W[q * NF + f][t];
+        g.R[j] = acc;
+      }
+    }
+  }
+
+  // output: plain w = A^T z, or one reverse RK4 stage
+  DG_HD static void phaseOut(const AdjArgs& a, int tid, int tile, const Regs& g) {
+    const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+    const int E = a.E, e = tile * T + t;
+    if (e >= E) return;
+    const double dtj = a.dt / a.el.detj[e];
+#pragma unroll
+    for (int j = 0; j < RPW; j++) {
+      const int row = wave + 4 * j;
+      if (row < NROW) {
+        const double acc = g.R[j];
         const long i = (long)row * E + e;
         if (a.mode == ADJ_PLAIN) {
           a.W[i] = acc;
@@ -269,13 +300,13 @@ struct TiledVjp {
           const double lam = a.lam[i];
           if (a.stage == 3) {
             a.mu[i] = lam + acc;
-            a.znext[i] = dtj * (lam / 3.0 + acc);
+            a.znext[i] = dtj * (lam * (1.0 / 3.0) + acc);
           } else if (a.stage == 2) {
             a.mu[i] += acc;
-            a.znext[i] = dtj * (lam / 3.0 + 0.5 * acc);
+            a.znext[i] = dtj * (lam * (1.0 / 3.0) + 0.5 * acc);
           } else if (a.stage == 1) {
             a.mu[i] += acc;
-            a.znext[i] = dtj * (lam / 6.0 + 0.5 * acc);
+            a.znext[i] = dtj * (lam * (1.0 / 6.0) + 0.5 * acc);
           } else {
             a.mu[i] += acc;
           }
@@ -358,10 +389,10 @@ DG_HD void jvp_element(const AdjArgs& a, int e) {
           }
           uR[f] = s;
         }
-        LAW::riemann(a.law, n, uL, uR, Fh);
       } else {
-        LAW::boundary(a.law, -link, n, uL, Fh);
+        LAW::ghost(a.law, -link, n, uL, uR);
       }
+      LAW::riemann(a.law, n, uL, uR, Fh);
       const double sw = len * tb[TriBlock::off_wf(P) + q];
       for (int f = 0; f < NF; f++)
         for (int k = 0; k < NP; k++) R[f][k] -= (sw * tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k]) * Fh[f];

@@ -30,6 +30,7 @@ struct dg_ctx {
   int dim = 2, E = 0, Nf = 0, P = 0, NP = 0;
   int law = 0, prec = 0, use_graph = 0, device = 0;
   int kernel = 0;  // 0: tiled LDS-staged stage kernel, 1: one element per work-item
+  int adj_nd = 0;  // adjoint kernel: flux tangent directions per pass (0: default)
   LawParams lp{};
   size_t ndof = 0;
   std::vector<int32_t> perm;  // internal index -> caller element
@@ -157,6 +158,9 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_opti
   c->use_graph = opt->use_graph;
   c->device = opt->device;
   c->kernel = opt->reserved[0] == 1 ? 1 : 0;
+  c->adj_nd = opt->reserved[1];
+  if (c->adj_nd != 0 && c->adj_nd != 1 && c->adj_nd != 2 && c->adj_nd != 4)
+    return bail(fail(c, DG_ERR_ARG, "options.reserved[1] (adjoint directions per pass) must be 0, 1, 2 or 4"));
   c->lp = S.lp;
   c->ndof = (size_t)S.E * S.NF * S.NP;
   c->perm = S.perm;
@@ -581,7 +585,7 @@ dg_status dg_residual_vjp(dg_ctx* c, const double* U, const double* z, double* w
   a.U = c->lam;
   a.Z = c->za;
   a.W = c->wv;
-  if (launch_tri_vjp(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "vjp launch failed");
+  if (launch_tri_vjp(a, c->law, c->adj_nd, c->stream)) return fail(c, DG_ERR_HIP, "vjp launch failed");
   return fetch_soa(c, c->wv, w);
 }
 
@@ -724,7 +728,7 @@ dg_status dg_adjoint_backward(dg_ctx* c, const double* dJdUT, double* dJdU0) {
         a.U = b + (size_t)st * nd;
         a.Z = zin;
         a.znext = zout;
-        if (launch_tri_vjp(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "adjoint stage failed");
+        if (launch_tri_vjp(a, c->law, c->adj_nd, c->stream)) return fail(c, DG_ERR_HIP, "adjoint stage failed");
         std::swap(zin, zout);
       }
       std::swap(lam, mu);

@@ -51,7 +51,7 @@ int tri_tiled_blocks(int E);
 int launch_copy_probe(const void* src, void* dst, long n_doubles, hipStream_t s);
 
 struct AdjArgs;
-int launch_tri_vjp(const AdjArgs& a, int law, hipStream_t s);
+int launch_tri_vjp(const AdjArgs& a, int law, int nd, hipStream_t s);
 int launch_tri_jvp(const AdjArgs& a, int law, hipStream_t s);
 int launch_tri_adj_scale(int E, int nrow, double c, double dt, const double* detj, const double* in, double* out,
                          hipStream_t s);

@@ -172,10 +172,8 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(StageArgs a) {
           uL[f] = s;
         }
         CT Fh[NF];
-        if (link >= 0)
-          LAW::riemann(law, n, uL, uR[q], Fh);
-        else
-          LAW::boundary(law, -link, n, uL, Fh);
+        if (link < 0) LAW::ghost(law, -link, n, uL, uR[q]);
+        LAW::riemann(law, n, uL, uR[q], Fh);
         const CT sw = len * tb[TriBlock::off_wf(P) + q];  // |J_face| w = (len/2) w
 #pragma unroll
         for (int f = 0; f < NF; f++) {

@@ -71,11 +71,21 @@ struct ScalarLawD {
     T u = vn > T(0) ? uL[0] : uR[0];
     out[0] = (vx * u) * n[0] + (vy * u) * n[1];
   }
+  // ghost state of a boundary face: the far-field value (ScalarLaw.cc:39-59 takes
+  // the exterior value on inflow and the interior value on outflow, which is the
+  // upwind flux against this ghost)
+  template <typename T>
+  DG_HD static void ghost(const LawParams& L, int code, const T* n, const T* q, T* g) {
+    (void)code;
+    (void)n;
+    (void)q;
+    g[0] = T(L.farfield[0]);
+  }
   template <typename T>
   DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* q, T* out) {
-    (void)code;
-    T vn = n[0] * T(L.vel[0]) + n[1] * T(L.vel[1]);
-    out[0] = vn < T(0) ? vn * T(L.farfield[0]) : vn * q[0];
+    T g[1];
+    ghost(L, code, n, q, g);
+    riemann(L, n, q, g, out);
   }
   template <typename T>
   DG_HD static bool physical(const LawParams&, const T*) { return true; }
@@ -187,9 +197,10 @@ struct EulerLawD {
     else
       lf(L, n, uL, uR, out);
   }
+  // ghost state of a boundary face (EulerLaw.cc:280-295): the wall mirrors the
+  // normal momentum, every other tag takes the far-field state
   template <typename T>
-  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* Q, T* out) {
-    T g[4];
+  DG_HD static void ghost(const LawParams& L, int code, const T* n, const T* Q, T* g) {
     if (code == BC_WALL) {
       T vn = -(Q[1] * n[0] + Q[2] * n[1]);
       T vt = Q[1] * n[1] - Q[2] * n[0];
@@ -203,6 +214,11 @@ struct EulerLawD {
       g[2] = T(L.farfield[2]);
       g[3] = T(L.farfield[3]);
     }
+  }
+  template <typename T>
+  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* Q, T* out) {
+    T g[4];
+    ghost(L, code, n, Q, g);
     riemann(L, n, Q, g, out);
   }
   template <typename T>
@@ -264,15 +280,22 @@ struct MaxwellLawD {
       }
     }
   }
+  // perfect electric conductor: E -> -E, H -> H in the ghost
   template <typename T>
-  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* Q, T* out) {
+  DG_HD static void ghost(const LawParams& L, int code, const T* n, const T* Q, T* g) {
+    (void)L;
     (void)code;
-    T g[6];
+    (void)n;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       g[i] = -Q[i];
       g[3 + i] = Q[3 + i];
     }
+  }
+  template <typename T>
+  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* Q, T* out) {
+    T g[6];
+    ghost(L, code, n, Q, g);
     riemann(L, n, Q, g, out);
   }
   template <typename T>

@@ -103,8 +103,12 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
         if (nbe[e * 3 + l] < 0 && m->bc[e * 3 + l] == DG_TAG_CHARACTERISTIC)
           return failS(err, DG_ERR_UNSUPPORTED, "characteristic far-field boundary (tag 30000) is not built");
 
-  // ---- internal order: group elements whose face links share (nb local face,
-  //      orientation, boundary code) so a wavefront takes one path ------------
+  // ---- internal order: within consecutive blocks of 2 x 64 caller elements,
+  //      group elements whose face links share (nb local face, orientation,
+  //      boundary code) so a wavefront takes one path.  Blocks keep face
+  //      neighbours in nearby tiles (the same XCD's L2); on the structured
+  //      meshes each block holds the 64 lower and 64 upper triangles of 64 cells,
+  //      one tile each. ---------------------------------------------------------
   auto sig = [&](int e) {
     int s = 0;
     for (int l = 0; l < 3; l++) {
@@ -123,7 +127,11 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   for (int e = 0; e < E; e++) S.perm[e] = e;
   std::vector<int> sg(E);
   for (int e = 0; e < E; e++) sg[e] = sig(e);
-  std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) { return sg[a] < sg[b]; });
+  const int BLK = 2 * 64;
+  std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) {
+    if (a / BLK != b / BLK) return a / BLK < b / BLK;
+    return sg[a] < sg[b];
+  });
   std::vector<int32_t> iperm(E);
   for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
 

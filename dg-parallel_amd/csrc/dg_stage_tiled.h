@@ -40,7 +40,11 @@ DG_HD int wave_uniform(int x) {
 #endif
 }
 
+// Divisions by 3, 6 and 36 are multiplications by the rounded reciprocals (one
+// rounding more than the reference's division, far inside the 1e-12 parity
+// bound; a full FP64 division is a 10-instruction sequence per value).
 DG_HD double stage_value_hd(int sc, double v, double un, double us, double* acc, double& rsum) {
+  constexpr double third = 1.0 / 3.0, sixth = 1.0 / 6.0, r36 = 1.0 / 36.0;
   double o;
   switch (sc) {
     case 3 * 4 + 0:  // RK4 (TimeIntegrator.cc:101-137)
@@ -57,8 +61,8 @@ DG_HD double stage_value_hd(int sc, double v, double un, double us, double* acc,
       break;
     case 3 * 4 + 3: {  // (:235-270)
       const double s = *acc + v;
-      o = un + s / 6.0;
-      rsum += s * s / 36.0;
+      o = un + s * sixth;
+      rsum += s * s * r36;
       break;
     }
     case 2 * 4 + 0:  // SSP-RK3 (:850-880)
@@ -68,7 +72,7 @@ DG_HD double stage_value_hd(int sc, double v, double un, double us, double* acc,
       o = 0.25 * (us + v + 3. * un);
       break;
     case 2 * 4 + 2: {  // (:934-987)
-      o = (2.0 * (us + v) + un) / 3.0;
+      o = (2.0 * (us + v) + un) * third;
       const double d = o - un;
       rsum += d * d;
       break;
@@ -312,10 +316,10 @@ struct TiledStage {
           uL[f] = acc;
           uR[f] = s.W[(l * NQF + q) * NF + f][t];
         }
-        if (link >= 0)
-          LAW::riemann(a.law, n, uL, uR, Fh);
-        else
-          LAW::boundary(a.law, -link, n, uL, Fh);
+        // boundary faces: the law's ghost state, then the same two-point flux (one
+        // inlined flux for interior and boundary lanes alike)
+        if (link < 0) LAW::ghost(a.law, -link, n, uL, uR);
+        LAW::riemann(a.law, n, uL, uR, Fh);
         // |J_face| w_q = (len / 2) w_q; the table holds w_q / 2
         const CT sw = len * tb[TriBlock::off_wf(P) + q];
 #pragma unroll
@@ -352,6 +356,9 @@ struct TiledStage {
           ac[ki] = reads_acc ? a.acc[i] : 0.0;
           us[ki] = reads_input ? double(Uin[i]) : 0.0;
         }
+        // one face's points per unrolled block: unrolling all 3*NQF points hoists
+        // every table row into SGPRs and spills them to VGPR lanes
+#pragma unroll NQF
         for (int lq = 0; lq < 3 * NQF; lq++) {
           const CT Fq = s.W[FW + lq * NF + f][t];
 #pragma unroll

@@ -178,10 +178,10 @@ def EulerLaw(flux=FLUX_LF, gamma=1.4, farfield=(1.0, 0.0, 0.0, 2.5)):
 class DGAnalysis:
     """One context: HBM-resident tables and state of a mesh + law + order."""
 
-    def __init__(self, mesh, law, order, precision=PREC_FP64, use_graph=True, device=0, kernel=0):
+    def __init__(self, mesh, law, order, precision=PREC_FP64, use_graph=True, device=0, kernel=0, adj_nd=0):
         self.mesh, self.law, self.order = mesh, law, order
         md, ld = mesh.desc(), law.desc()
-        opt = Options(order, precision, int(use_graph), device, (ctypes.c_int * 8)(kernel))
+        opt = Options(order, precision, int(use_graph), device, (ctypes.c_int * 8)(kernel, adj_nd))
         h = _vp()
         st = _create(ctypes.byref(md), ctypes.byref(ld), ctypes.byref(opt), ctypes.byref(h))
         self._h = h

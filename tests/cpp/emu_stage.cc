@@ -112,9 +112,9 @@ struct Emu {
   }
 };
 
-template <int P, class LAW>
+template <int P, class LAW, int ND>
 void run_vjp_tiles(const AdjArgs& a) {
-  using K = TiledVjp<P, LAW>;
+  using K = TiledVjp<P, LAW, ND>;
   std::unique_ptr<typename K::Smem> sm(new typename K::Smem());
   std::vector<typename K::Regs> g(TILE_THREADS);
   const int ntiles = (a.E + TILE_ELEMS - 1) / TILE_ELEMS;
@@ -123,26 +123,34 @@ void run_vjp_tiles(const AdjArgs& a) {
     for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, *sm, t, tile);
     for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, *sm, t);
     for (int t = 0; t < TILE_THREADS; t++) K::phaseD1(a, *sm, t, g[t]);
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, *sm, t, tile);
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t);
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseD2(a, *sm, t, tile, g[t]);
+    for (int l = 0; l < 3; l++) {
+      for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, *sm, t, tile, l);
+      for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t, l);
+      for (int t = 0; t < TILE_THREADS; t++) K::phaseLift(a, *sm, t, l, g[t]);
+    }
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseOut(a, t, tile, g[t]);
   }
 }
 
-template <class LAW>
+template <class LAW, int ND>
 int run_vjp_law(const AdjArgs& a) {
   switch (a.P) {
-    case 1: run_vjp_tiles<1, LAW>(a); return 0;
-    case 2: run_vjp_tiles<2, LAW>(a); return 0;
-    case 3: run_vjp_tiles<3, LAW>(a); return 0;
-    case 4: run_vjp_tiles<4, LAW>(a); return 0;
+    case 1: run_vjp_tiles<1, LAW, ND>(a); return 0;
+    case 2: run_vjp_tiles<2, LAW, ND>(a); return 0;
+    case 3: run_vjp_tiles<3, LAW, ND>(a); return 0;
+    case 4: run_vjp_tiles<4, LAW, ND>(a); return 0;
   }
   return -1;
 }
 
-int run_vjp(const AdjArgs& a, int law) {
-  if (law == LAW_SCALAR) return run_vjp_law<ScalarLawD>(a);
-  if (law == LAW_EULER2D) return run_vjp_law<EulerLawD>(a);
+// nd: flux tangent directions per pass (0: all fields at once)
+int run_vjp(const AdjArgs& a, int law, int nd = 0) {
+  if (law == LAW_SCALAR) return run_vjp_law<ScalarLawD, 1>(a);
+  if (law == LAW_EULER2D) {
+    if (nd == 1) return run_vjp_law<EulerLawD, 1>(a);
+    if (nd == 2) return run_vjp_law<EulerLawD, 2>(a);
+    return run_vjp_law<EulerLawD, 4>(a);
+  }
   return -1;
 }
 
@@ -231,7 +239,8 @@ extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, i
   return 0;
 }
 
-// tangent-linear (jvp=1: dR = A dU) or adjoint (jvp=0: w = A^T z) residual product
+// tangent-linear (jvp=1: dR = A dU) or adjoint (jvp=0: w = A^T z; jvp=-1, -2: the
+// adjoint kernel with 1 or 2 flux tangent directions per pass) residual product
 extern "C" int emu_tri_derivative(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int jvp,
                                   const double* U, const double* V, double* W) {
   Emu m;
@@ -244,7 +253,7 @@ extern "C" int emu_tri_derivative(const dg_mesh_desc* mesh, const dg_law_desc* l
   a.U = u.data();
   a.Z = v.data();
   a.W = w.data();
-  if ((jvp ? run_jvp(a, m.S.law) : run_vjp(a, m.S.law))) return -2;
+  if ((jvp > 0 ? run_jvp(a, m.S.law) : run_vjp(a, m.S.law, -jvp))) return -2;
   m.from_soa(w.data(), W);
   return 0;
 }

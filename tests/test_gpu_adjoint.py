@@ -28,7 +28,7 @@ def announce(name):
     sys.stdout.flush()
 
 
-def make(law, flux, p, periodic, nx=6):
+def make(law, flux, p, periodic, nx=6, adj_nd=0):
     import dgpar
     if law == "scalar":
         vids, xyz, bc = mesh_tri(nx, nx, periodic_x=periodic, periodic_y=periodic, jitter=0.1, seed=3,
@@ -44,7 +44,7 @@ def make(law, flux, p, periodic, nx=6):
         lw = dgpar.Law(dgpar.LAW_EULER2D, dgpar.FLUX_LF if flux == "lf" else dgpar.FLUX_ROE, gamma=1.4,
                        farfield=FF_OFF)
         U = o.project(1, VORTEX)
-    g = dgpar.DGAnalysis(dgpar.Mesh(2, vids, xyz, bc), lw, p)
+    g = dgpar.DGAnalysis(dgpar.Mesh(2, vids, xyz, bc), lw, p, adj_nd=adj_nd)
     return g, o, roughen(U, p)
 
 
@@ -153,6 +153,20 @@ def test_gpu_adjoint_checkpoint_spacing_and_objective():
     g.set_state(U0)
     g.adjoint_forward(dt, 3, 3)
     assert np.array_equal(g.adjoint_backward(np.ones_like(U0)), g2)
+
+
+@pytest.mark.parametrize("p", [1, 3, 4])
+def test_gpu_adjoint_directions_per_pass_agree(p):
+    announce("adjoint directions per pass P%d" % p)
+    grads = []
+    for nd in (0, 1, 2, 4):
+        g, o, U0 = make("euler", "roe", p, False, adj_nd=nd)
+        dt = o.dt(U0, 0.1)
+        g.set_state(U0)
+        g.adjoint_forward(dt, 3, 2)
+        grads.append(g.adjoint_backward(np.random.default_rng(3).standard_normal(U0.shape)))
+    for gr in grads[1:]:
+        assert np.abs(gr - grads[0]).max() <= 1e-14 * np.abs(grads[0]).max()
 
 
 def test_gpu_adjoint_rejects_bad_use():
