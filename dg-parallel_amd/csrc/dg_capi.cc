@@ -20,6 +20,7 @@
 #include "dg_adjoint.h"
 #include "dg_fields.h"
 #include "dg_kernels.h"
+#include "dg_mixed.h"
 #include "dg_setup.h"
 #include "dg_tables.h"
 
@@ -36,6 +37,23 @@ struct dg_ctx {
   std::vector<int32_t> perm;  // internal index -> caller element
   std::vector<int32_t> faces; // (eL, lL, eR, lR) caller numbering
   std::vector<double> xyz;    // caller corner coordinates [E][3][2]
+  // order buckets (a uniform mesh is one bucket at offset 0)
+  bool mixed = false;
+  int bstart[6] = {0, 0, 0, 0, 0, 0};
+  long boff[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<int8_t> order;  // internal order per element
+  std::vector<long> coff;     // caller element -> first coefficient (E+1)
+  int moff[6] = {0, 0, 0, 0, 0, 0};
+  int ftoff[6][6];
+  int nchunks = 0;
+  // what dg_create was given, kept to rebuild the buckets when orders change
+  std::vector<int32_t> m_vids, m_bc, m_order;
+  dg_law_desc m_law{};
+  dg_options m_opt{};
+  // limiter (applied after every RK stage when on)
+  int lim_on = 0;
+  double lim_m = 0.0, lim_nu = 1.0;
+  double lim_midv[3][15], lim_pinv[2][3];
   hipStream_t stream = nullptr;
   // device memory
   double *Un = nullptr, *acc = nullptr, *A64 = nullptr, *B64 = nullptr, *R = nullptr;
@@ -43,6 +61,9 @@ struct dg_ctx {
   double *mg = nullptr, *detj = nullptr, *size = nullptr, *tab = nullptr, *vtx = nullptr, *part = nullptr;
   float* tabf = nullptr;
   int32_t* link = nullptr;
+  double *mtab = nullptr, *glw = nullptr, *mvtx = nullptr;
+  int* chunks = nullptr;
+  uint8_t *flags = nullptr, *sens = nullptr;
   std::vector<void*> allocs;
   // graph cache
   hipGraphExec_t gexec = nullptr;
@@ -133,17 +154,8 @@ dg_status dg_faces(const dg_ctx* c, int32_t* out4, size_t n) {
   return DG_OK;
 }
 
-dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, dg_ctx** out) {
-  if (!out) return DG_ERR_ARG;
-  *out = nullptr;
-  dg_ctx* c = new (std::nothrow) dg_ctx();
-  if (!c) return DG_ERR_OOM;
-  auto bail = [&](dg_status s) {
-    // keep the context alive only to report the error through dg_last_error
-    free_all(c);
-    *out = c;
-    return s;
-  };
+static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt) {
+  auto bail = [&](dg_status s) { return s; };
   TriSetup S;
   std::string err;
   dg_status st = build_tri_setup(m, law, opt, S, err);
@@ -162,7 +174,24 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_opti
   if (c->adj_nd != 0 && c->adj_nd != 1 && c->adj_nd != 2 && c->adj_nd != 4)
     return bail(fail(c, DG_ERR_ARG, "options.reserved[1] (adjoint directions per pass) must be 0, 1, 2 or 4"));
   c->lp = S.lp;
-  c->ndof = (size_t)S.E * S.NF * S.NP;
+  c->mixed = S.mixed;
+  for (int p = 0; p < 6; p++) {
+    c->bstart[p] = S.bstart[p];
+    c->boff[p] = S.boff[p];
+    c->moff[p] = S.moff[p];
+    for (int q = 0; q < 6; q++) c->ftoff[p][q] = S.ftoff[p][q];
+  }
+  c->order = S.order;
+  c->coff = S.coff;
+  c->ndof = (size_t)S.boff[5];
+  if (c->mixed && c->kernel == 1) return bail(fail(c, DG_ERR_UNSUPPORTED, "mixed orders run the persistent mixed kernel only"));
+  c->m_vids.assign(m->vids, m->vids + (size_t)S.E * 3);
+  c->m_bc.clear();
+  if (m->bc) c->m_bc.assign(m->bc, m->bc + (size_t)S.E * 3);
+  c->m_order.resize(S.E);
+  for (int i = 0; i < S.E; i++) c->m_order[S.perm[i]] = S.order[i];
+  c->m_law = *law;
+  c->m_opt = *opt;
   c->perm = S.perm;
   c->faces = S.faces;
   c->xyz = S.xyz;
@@ -198,6 +227,47 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_opti
   } else {
     if ((s = dalloc(c, &c->A32, nd)) || (s = dalloc(c, &c->B32, nd))) return bail(s);
   }
+  // every order's tables (limiter, sensor, mixed kernels) and the chunk list of
+  // the persistent mixed launch (heaviest order first)
+  std::vector<int> chunks;
+  for (int p = MAXP_TRI; p >= 1; p--)
+    for (int i = S.bstart[p]; i < S.bstart[p + 1]; i += MIX_CHUNK) {
+      chunks.push_back(p);
+      chunks.push_back(i);
+    }
+  c->nchunks = (int)chunks.size() / 2;
+  if ((s = dalloc(c, &c->mtab, S.mtab.size())) || (s = dalloc(c, &c->glw, S.glw.size())) ||
+      (s = dalloc(c, &c->mvtx, S.mvtx.size())) || (s = dalloc(c, &c->chunks, chunks.size() + 2)) ||
+      (s = dalloc(c, &c->flags, (size_t)E)) || (s = dalloc(c, &c->sens, (size_t)E)))
+    return bail(s);
+  if (hipMemcpy(c->mtab, S.mtab.data(), S.mtab.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->glw, S.glw.data(), S.glw.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->mvtx, S.mvtx.data(), S.mvtx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->chunks, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(c->flags, 0, E) != hipSuccess)
+    return bail(fail(c, DG_ERR_HIP, "table upload failed"));
+  // limiter constants: phi_1, phi_2 at the edge midpoints and the least-squares
+  // inverse that rebuilds (U_1, U_2) from midpoint deviations
+  {
+    static const double mid[3][2] = {{0.5, 0.0}, {0.5, 0.5}, {0.0, 0.5}};
+    double M[3][2], phi[64], xi[3] = {0, 0, 0};
+    for (int l = 0; l < 3; l++) {
+      xi[0] = mid[l][0];
+      xi[1] = mid[l][1];
+      basis(2, MAXP_TRI, xi, phi, nullptr);
+      for (int k = 0; k < 15; k++) c->lim_midv[l][k] = phi[k];
+      M[l][0] = phi[1];
+      M[l][1] = phi[2];
+    }
+    double A[2][2] = {{0, 0}, {0, 0}};
+    for (int l = 0; l < 3; l++)
+      for (int r = 0; r < 2; r++)
+        for (int q = 0; q < 2; q++) A[r][q] += M[l][r] * M[l][q];
+    const double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    const double Ai[2][2] = {{A[1][1] / det, -A[0][1] / det}, {-A[1][0] / det, A[0][0] / det}};
+    for (int r = 0; r < 2; r++)
+      for (int l = 0; l < 3; l++) c->lim_pinv[r][l] = Ai[r][0] * M[l][0] + Ai[r][1] * M[l][1];
+  }
   auto up = [&](void* d, const void* h, size_t bytes) {
     return hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) == hipSuccess;
   };
@@ -208,8 +278,19 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_opti
     return bail(fail(c, DG_ERR_HIP, "table upload failed"));
   if (hipMemset(c->Un, 0, nd * 8) != hipSuccess || hipMemset(c->acc, 0, nd * 8) != hipSuccess)
     return bail(fail(c, DG_ERR_HIP, "memset failed"));
-  *out = c;
   return DG_OK;
+}
+
+dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, dg_ctx** out) {
+  if (!out) return DG_ERR_ARG;
+  *out = nullptr;
+  dg_ctx* c = new (std::nothrow) dg_ctx();
+  if (!c) return DG_ERR_OOM;
+  dg_status s = ctx_init(c, m, law, opt);
+  // on failure the context is kept (device memory released) to report the error
+  if (s != DG_OK) free_all(c);
+  *out = c;
+  return s;
 }
 
 dg_status dg_destroy(dg_ctx* c) {
@@ -222,17 +303,29 @@ dg_status dg_destroy(dg_ctx* c) {
 
 static bool ready(const dg_ctx* c) { return c && c->stream && c->Un; }
 
+// caller layout [e][f][k] (Np of each element's order, elements in caller order)
+// <-> bucketed structure of arrays; visit(i, caller index, soa index)
+extern "C++" {
+template <class F>
+static void for_each_coeff(const dg_ctx* c, F visit) {
+  for (int i = 0; i < c->E; i++) {
+    const int p = c->order[i], n = c->Nf * np_tri(p), Ep = c->bstart[p + 1] - c->bstart[p];
+    const long cb = c->coff[c->perm[i]];
+    const long sb = c->boff[p] + (i - c->bstart[p]);
+    for (int j = 0; j < n; j++) visit(i, cb + j, sb + (long)j * Ep);
+  }
+}
+}
+
 dg_status dg_set_state(dg_ctx* c, const double* U, size_t n) {
   if (!ready(c) || !U || n != c->ndof) return fail(c, DG_ERR_ARG, "dg_set_state: need %zu doubles", c ? c->ndof : 0);
-  const int E = c->E, BS = c->Nf * c->NP;
   std::vector<double> h(n);
-  for (int i = 0; i < E; i++) {
-    const double* src = U + (size_t)c->perm[i] * BS;
-    for (int j = 0; j < BS; j++) {
-      if (!std::isfinite(src[j])) return fail(c, DG_ERR_NONFINITE, "non-finite coefficient in element %d", c->perm[i]);
-      h[(size_t)j * E + i] = src[j];
-    }
-  }
+  int bad = -1;
+  for_each_coeff(c, [&](int i, long ci, long si) {
+    if (!std::isfinite(U[ci]) && bad < 0) bad = c->perm[i];
+    h[si] = U[ci];
+  });
+  if (bad >= 0) return fail(c, DG_ERR_NONFINITE, "non-finite coefficient in element %d", bad);
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipMemcpyAsync(c->Un, h.data(), n * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -240,14 +333,10 @@ dg_status dg_set_state(dg_ctx* c, const double* U, size_t n) {
 }
 
 static dg_status fetch_soa(dg_ctx* c, const double* dsrc, double* U) {
-  const int E = c->E, BS = c->Nf * c->NP;
   std::vector<double> h(c->ndof);
   HIPCHECK(c, hipMemcpyAsync(h.data(), dsrc, c->ndof * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  for (int i = 0; i < E; i++) {
-    double* dst = U + (size_t)c->perm[i] * BS;
-    for (int j = 0; j < BS; j++) dst[j] = h[(size_t)j * E + i];
-  }
+  for_each_coeff(c, [&](int, long ci, long si) { U[ci] = h[si]; });
   return DG_OK;
 }
 
@@ -257,12 +346,79 @@ dg_status dg_get_state(dg_ctx* c, double* U, size_t n) {
   return fetch_soa(c, c->Un, U);
 }
 
+static MixedArgs mixed_args(const dg_ctx* c) {
+  MixedArgs m{};
+  m.E = c->E;
+  for (int p = 0; p < 6; p++) {
+    m.bstart[p] = c->bstart[p];
+    m.boff[p] = c->boff[p];
+    m.moff[p] = c->moff[p];
+    for (int q = 0; q < 6; q++) m.ftoff[p][q] = c->ftoff[p][q];
+  }
+  m.tab = c->mtab;
+  m.glw = c->glw;
+  m.mvtx = c->mvtx;
+  m.el.mg = c->mg;
+  m.el.detj = c->detj;
+  m.el.size = c->size;
+  m.el.link = c->link;
+  m.law = c->lp;
+  m.chunks = c->chunks;
+  m.nchunks = c->nchunks;
+  return m;
+}
+
+static LimiterArgs limiter_args(const dg_ctx* c, double* U) {
+  LimiterArgs l{};
+  l.E = c->E;
+  for (int p = 0; p < 6; p++) {
+    l.bstart[p] = c->bstart[p];
+    l.boff[p] = c->boff[p];
+  }
+  l.el.mg = c->mg;
+  l.el.detj = c->detj;
+  l.el.size = c->size;
+  l.el.link = c->link;
+  l.law = c->lp;
+  l.U = U;
+  l.mvtx = c->mvtx;
+  for (int i = 0; i < 3; i++) {
+    for (int k = 0; k < MIX_MAXNP; k++) l.midv[i][k] = c->lim_midv[i][k];
+    for (int j = 0; j < 2; j++) l.pinv[j][i] = c->lim_pinv[j][i];
+  }
+  l.tvb_m = c->lim_m;
+  l.nu = c->lim_nu;
+  l.flags = c->flags;
+  return l;
+}
+
+// one stage kernel (uniform tiled / per-element, or the persistent mixed-order
+// launch), then the limiter on its output when it is on
 static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
-  return c->kernel == 1 ? launch_tri_stage(a, c->law, variant, c->stream)
-                        : launch_tri_stage_tiled(a, c->law, variant, c->stream);
+  int r;
+  if (c->mixed) {
+    MixedArgs m = mixed_args(c);
+    m.Uin = (const double*)a.Uin;
+    m.Un = a.Un;
+    m.acc = a.acc;
+    m.Uout = (double*)a.Uout;
+    m.R = a.R;
+    m.dt = a.dt;
+    m.scheme = a.scheme;
+    m.stage = a.stage;
+    m.resid_part = a.resid_part;
+    r = launch_mixed_stage(m, c->law, c->stream);
+  } else {
+    r = c->kernel == 1 ? launch_tri_stage(a, c->law, variant, c->stream)
+                       : launch_tri_stage_tiled(a, c->law, variant, c->stream);
+  }
+  if (r == 0 && c->lim_on && a.scheme != MODE_RESIDUAL)
+    r = launch_limiter(limiter_args(c, (double*)a.Uout), c->law, c->stream);
+  return r;
 }
 
 static int stage_blocks(const dg_ctx* c) {
+  if (c->mixed) return mixed_grid(c->nchunks);
   return c->kernel == 1 ? tri_stage_blocks(c->E) : tri_tiled_blocks(c->E);
 }
 
@@ -312,8 +468,16 @@ dg_status dg_compute_dt(dg_ctx* c, double cfl, double* dt_out) {
   a.law = c->lp;
   a.cfl = cfl;
   a.part = c->part;
-  if (launch_tri_dt(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "dt launch failed");
-  int nb = tri_dt_blocks(c->E);
+  int nb;
+  if (c->mixed) {
+    MixedArgs m = mixed_args(c);
+    m.Uin = c->Un;
+    if (launch_mixed_dt(m, c->law, cfl, c->part, c->stream)) return fail(c, DG_ERR_HIP, "dt launch failed");
+    nb = (c->E + MIX_CHUNK - 1) / MIX_CHUNK;
+  } else {
+    if (launch_tri_dt(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "dt launch failed");
+    nb = tri_dt_blocks(c->E);
+  }
   std::vector<double> h(nb);
   HIPCHECK(c, hipMemcpyAsync(h.data(), c->part, nb * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -462,16 +626,24 @@ dg_status dg_project(dg_ctx* c, int field, const double* par, double t) {
   if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
   double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (!par) par = zero;
-  const int E = c->E, NF = c->Nf, NP = c->NP, P = c->P;
-  auto rule = tri_rule(2 * P);
-  std::vector<double> phi(rule.size() * NP);
-  for (size_t q = 0; q < rule.size(); q++) basis(2, P, rule[q].x, &phi[q * NP], nullptr);
+  const int E = c->E, NF = c->Nf;
+  // per order: the rule of degree 2p and the basis at its points (DGCell.cc:1458-1483)
+  std::vector<QPoint> rules[MAXP_TRI + 1];
+  std::vector<double> phis[MAXP_TRI + 1];
+  for (int p = 1; p <= MAXP_TRI; p++) {
+    rules[p] = tri_rule(2 * p);
+    phis[p].resize(rules[p].size() * np_tri(p));
+    for (size_t q = 0; q < rules[p].size(); q++) basis(2, p, rules[p][q].x, &phis[p][q * np_tri(p)], nullptr);
+  }
   std::vector<double> U(c->ndof, 0.0);
   for (int e = 0; e < E; e++) {
+    const int P = c->m_order[e], NP = np_tri(P);
+    const auto& rule = rules[P];
+    const auto& phi = phis[P];
     const double* X = &c->xyz[(size_t)e * 6];
     double det = (X[2] - X[0]) * (X[5] - X[1]) - (X[4] - X[0]) * (X[3] - X[1]);
     double ad = std::fabs(det);
-    double* Ue = &U[(size_t)e * NF * NP];
+    double* Ue = &U[c->coff[e]];
     for (size_t q = 0; q < rule.size(); q++) {
       const double u = rule[q].x[0], v = rule[q].x[1];
       double x[3] = {X[0] + (X[2] - X[0]) * u + (X[4] - X[0]) * v, X[1] + (X[3] - X[1]) * u + (X[5] - X[1]) * v, 0.0};
@@ -491,18 +663,26 @@ dg_status dg_l2_error(dg_ctx* c, int field, const double* par, double t, double*
   if (!ready(c) || !err) return fail(c, DG_ERR_ARG, "bad argument");
   double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (!par) par = zero;
-  const int E = c->E, NF = c->Nf, NP = c->NP, P = c->P;
+  const int E = c->E, NF = c->Nf;
   std::vector<double> U(c->ndof);
   dg_status s = dg_get_state(c, U.data(), U.size());
   if (s) return s;
-  auto rule = tri_rule(std::min(2 * (P + 1), 10));
-  std::vector<double> phi(rule.size() * NP);
-  for (size_t q = 0; q < rule.size(); q++) basis(2, P, rule[q].x, &phi[q * NP], nullptr);
+  // per order: the rule of degree 2(p+1) (DGError.cc:805-831)
+  std::vector<QPoint> rules[MAXP_TRI + 1];
+  std::vector<double> phis[MAXP_TRI + 1];
+  for (int p = 1; p <= MAXP_TRI; p++) {
+    rules[p] = tri_rule(std::min(2 * (p + 1), 10));
+    phis[p].resize(rules[p].size() * np_tri(p));
+    for (size_t q = 0; q < rules[p].size(); q++) basis(2, p, rules[p][q].x, &phis[p][q * np_tri(p)], nullptr);
+  }
   std::vector<double> acc(NF, 0.0);
   for (int e = 0; e < E; e++) {
+    const int P = c->m_order[e], NP = np_tri(P);
+    const auto& rule = rules[P];
+    const auto& phi = phis[P];
     const double* X = &c->xyz[(size_t)e * 6];
     double ad = std::fabs((X[2] - X[0]) * (X[5] - X[1]) - (X[4] - X[0]) * (X[3] - X[1]));
-    const double* Ue = &U[(size_t)e * NF * NP];
+    const double* Ue = &U[c->coff[e]];
     for (size_t q = 0; q < rule.size(); q++) {
       const double u = rule[q].x[0], v = rule[q].x[1];
       double x[3] = {X[0] + (X[2] - X[0]) * u + (X[4] - X[0]) * v, X[1] + (X[3] - X[1]) * u + (X[5] - X[1]) * v, 0.0};
@@ -521,14 +701,124 @@ dg_status dg_l2_error(dg_ctx* c, int field, const double* par, double t, double*
 }
 
 // ---------------------------------------------------------------------------
+// limiter and order selection (new designs for triangles)
+
+dg_status dg_set_limiter(dg_ctx* c, int enabled, double tvb_m, double nu) {
+  if (!ready(c) || !(tvb_m >= 0) || !(nu > 0)) return fail(c, DG_ERR_ARG, "dg_set_limiter: bad argument");
+  if (enabled && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the limiter runs in the FP64 form only");
+  c->lim_on = enabled ? 1 : 0;
+  c->lim_m = tvb_m;
+  c->lim_nu = nu;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured steps change
+  c->gexec = nullptr;
+  return DG_OK;
+}
+
+static dg_status fetch_flags(dg_ctx* c, const uint8_t* dsrc, uint8_t* out) {
+  std::vector<uint8_t> h(c->E);
+  HIPCHECK(c, hipMemcpyAsync(h.data(), dsrc, c->E, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->E; i++) out[c->perm[i]] = h[i];
+  return DG_OK;
+}
+
+dg_status dg_limit(dg_ctx* c, uint8_t* flags_out) {
+  if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
+  if (c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the limiter runs in the FP64 form only");
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (launch_limiter(limiter_args(c, c->Un), c->law, c->stream)) return fail(c, DG_ERR_HIP, "limiter launch failed");
+  if (flags_out) return fetch_flags(c, c->flags, flags_out);
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return DG_OK;
+}
+
+dg_status dg_limiter_flags(dg_ctx* c, uint8_t* flags_out) {
+  if (!ready(c) || !flags_out) return fail(c, DG_ERR_ARG, "bad argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  return fetch_flags(c, c->flags, flags_out);
+}
+
+dg_status dg_get_orders(dg_ctx* c, int32_t* orders_out) {
+  if (!c || !orders_out || c->m_order.empty()) return fail(c, DG_ERR_ARG, "bad argument");
+  std::memcpy(orders_out, c->m_order.data(), c->m_order.size() * 4);
+  return DG_OK;
+}
+
+// rebuild the buckets for new per-element orders, carrying the state over
+// (orthonormal hierarchical modes: raising an order appends zero modes,
+// lowering it drops the top ones)
+static dg_status rebucket(dg_ctx* c, const std::vector<int32_t>& neword) {
+  std::vector<double> U(c->ndof);
+  dg_status s = dg_get_state(c, U.data(), U.size());
+  if (s) return s;
+  std::vector<long> noff(c->E + 1, 0);
+  for (int e = 0; e < c->E; e++) noff[e + 1] = noff[e] + (long)c->Nf * np_tri(neword[e]);
+  std::vector<double> V(noff[c->E], 0.0);
+  for (int e = 0; e < c->E; e++) {
+    const int np0 = np_tri(c->m_order[e]), np1 = np_tri(neword[e]), nk = std::min(np0, np1);
+    for (int f = 0; f < c->Nf; f++)
+      for (int k = 0; k < nk; k++) V[noff[e] + f * np1 + k] = U[c->coff[e] + f * np0 + k];
+  }
+  dg_ctx* t = new (std::nothrow) dg_ctx();
+  if (!t) return fail(c, DG_ERR_OOM, "out of memory");
+  dg_mesh_desc m{};
+  m.dim = 2;
+  m.n_elems = c->E;
+  m.vids = c->m_vids.data();
+  m.xyz = c->xyz.data();
+  m.bc = c->m_bc.empty() ? nullptr : c->m_bc.data();
+  m.order = neword.data();
+  s = ctx_init(t, &m, &c->m_law, &c->m_opt);
+  if (s != DG_OK) {
+    c->err = t->err;
+    free_all(t);
+    delete t;
+    return s;
+  }
+  t->lim_on = c->lim_on;
+  t->lim_m = c->lim_m;
+  t->lim_nu = c->lim_nu;
+  t->adj_nd = c->adj_nd;
+  std::swap(*c, *t);
+  free_all(t);
+  delete t;
+  return dg_set_state(c, V.data(), V.size());
+}
+
+dg_status dg_select_orders(dg_ctx* c, const dg_sensor_opts* o, int apply, int32_t* orders_out) {
+  if (!ready(c) || !o) return fail(c, DG_ERR_ARG, "dg_select_orders: bad argument");
+  if (o->pmin < 1 || o->pmax > MAXP_TRI || o->pmin > o->pmax) return fail(c, DG_ERR_ARG, "orders must lie in 1..%d", MAXP_TRI);
+  if (apply && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "order changes run in the FP64 form only");
+  HIPCHECK(c, hipSetDevice(c->device));
+  SensorArgs a{};
+  a.E = c->E;
+  for (int p = 0; p < 6; p++) {
+    a.bstart[p] = c->bstart[p];
+    a.boff[p] = c->boff[p];
+    a.s_hi[p] = p >= 1 && p <= MAXP_TRI ? o->s_hi[p - 1] : 0.0;
+    a.s_lo[p] = p >= 1 && p <= MAXP_TRI ? o->s_lo[p - 1] : 0.0;
+  }
+  a.pmin = o->pmin;
+  a.pmax = o->pmax;
+  a.U = c->Un;
+  a.orders = c->sens;
+  if (launch_sensor(a, c->stream)) return fail(c, DG_ERR_HIP, "sensor launch failed");
+  std::vector<uint8_t> h(c->E);
+  dg_status s = fetch_flags(c, c->sens, h.data());
+  if (s) return s;
+  std::vector<int32_t> neword(h.begin(), h.end());
+  if (orders_out) std::memcpy(orders_out, neword.data(), neword.size() * 4);
+  if (apply && neword != c->m_order) return rebucket(c, neword);
+  return DG_OK;
+}
+
+// ---------------------------------------------------------------------------
 // residual derivatives and the discrete adjoint of RK4 (no counterpart in the
 // reference)
 
 static void to_soa(const dg_ctx* c, const double* U, std::vector<double>& h) {
-  const int E = c->E, BS = c->Nf * c->NP;
   h.resize(c->ndof);
-  for (int i = 0; i < E; i++)
-    for (int j = 0; j < BS; j++) h[(size_t)j * E + i] = U[(size_t)c->perm[i] * BS + j];
+  for_each_coeff(c, [&](int, long ci, long si) { h[si] = U[ci]; });
 }
 
 static dg_status upload_soa(dg_ctx* c, const double* U, double* dst) {
@@ -565,6 +855,7 @@ static AdjArgs adj_args(dg_ctx* c) {
 
 dg_status dg_residual_jvp(dg_ctx* c, const double* U, const double* dU, double* dR) {
   if (!ready(c) || !U || !dU || !dR) return fail(c, DG_ERR_ARG, "dg_residual_jvp: bad argument");
+  if (c->mixed) return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform orders");
   HIPCHECK(c, hipSetDevice(c->device));
   dg_status s = adj_buffers(c);
   if (s || (s = upload_soa(c, U, c->lam)) || (s = upload_soa(c, dU, c->za))) return s;
@@ -578,6 +869,7 @@ dg_status dg_residual_jvp(dg_ctx* c, const double* U, const double* dU, double* 
 
 dg_status dg_residual_vjp(dg_ctx* c, const double* U, const double* z, double* w) {
   if (!ready(c) || !U || !z || !w) return fail(c, DG_ERR_ARG, "dg_residual_vjp: bad argument");
+  if (c->mixed) return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform orders");
   HIPCHECK(c, hipSetDevice(c->device));
   dg_status s = adj_buffers(c);
   if (s || (s = upload_soa(c, U, c->lam)) || (s = upload_soa(c, z, c->za))) return s;
@@ -632,6 +924,7 @@ dg_status dg_adjoint_forward(dg_ctx* c, double dt, int nsteps, int ckpt_every, c
   if (!ready(c) || !(dt > 0) || nsteps <= 0 || ckpt_every <= 0)
     return fail(c, DG_ERR_ARG, "dg_adjoint_forward: bad argument");
   if (c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the FP64 form");
+  if (c->mixed || c->lim_on) return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the unlimited uniform-order residual");
   HIPCHECK(c, hipSetDevice(c->device));
   c->adj_ready = false;
   dg_status s = adj_buffers(c);

@@ -24,9 +24,11 @@
 namespace dgp {
 
 enum { LAW_SCALAR = 0, LAW_EULER2D = 1, LAW_MAXWELL3D = 2 };
-enum { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2, FLUX_CENTRAL = 3 };
-// boundary codes stored in the element table
-enum { BC_FARFIELD = 1, BC_WALL = 2, BC_PEC = 3 };
+enum { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2, FLUX_CENTRAL = 3, FLUX_ROE_EF = 4 };
+// boundary codes: far field (farfield[0..NF)), wall, PEC, second far field
+// (farfield[4..8), Euler: a far-field state that differs by boundary, as the
+// reference's far-field FieldEvaluator evaluated at the boundary point)
+enum { BC_FARFIELD = 1, BC_WALL = 2, BC_PEC = 3, BC_FARFIELD2 = 4 };
 
 struct LawParams {
   double gamma;
@@ -168,6 +170,15 @@ struct EulerLawD {
       T c_A = dg_sqrt((gamma - T(1)) * qq);
       T vn = u_A * n[0] + v_A * n[1];
       T eg0 = dg_abs(vn), eg2 = dg_abs(vn + c_A), eg3 = dg_abs(vn - c_A);
+      if (L.flux == FLUX_ROE_EF) {
+        // Harten's entropy fix (Harten & Hyman 1983), delta = 0.2 c: a new option;
+        // the reference's Roe flux has none, and its zero dissipation of the
+        // linear waves on faces with vn ~ 0 lets a shock-tube run blow up
+        const T dl = T(0.2) * c_A, inv2 = T(0.5) / dl;
+        if (eg0 < dl) eg0 = (eg0 * eg0 + dl * dl) * inv2;
+        if (eg2 < dl) eg2 = (eg2 * eg2 + dl * dl) * inv2;
+        if (eg3 < dl) eg3 = (eg3 * eg3 + dl * dl) * inv2;
+      }
       T k = T(0.5) * rho_A / c_A;
       T inv_c_A = T(1) / c_A;
       T drho = rho_1 - rho_0, dp = p_1 - p_0, du = u_1 - u_0, dv = v_1 - v_0;
@@ -192,7 +203,7 @@ struct EulerLawD {
   }
   template <typename T>
   DG_HD static void riemann(const LawParams& L, const T* n, const T* uL, const T* uR, T* out) {
-    if (L.flux == FLUX_ROE)
+    if (L.flux == FLUX_ROE || L.flux == FLUX_ROE_EF)
       roe(L, n, uL, uR, out);
     else
       lf(L, n, uL, uR, out);
@@ -209,10 +220,11 @@ struct EulerLawD {
       g[2] = vn * n[1] - vt * n[0];
       g[3] = Q[3];
     } else {
-      g[0] = T(L.farfield[0]);
-      g[1] = T(L.farfield[1]);
-      g[2] = T(L.farfield[2]);
-      g[3] = T(L.farfield[3]);
+      const int o = code == BC_FARFIELD2 ? 4 : 0;
+      g[0] = T(L.farfield[o + 0]);
+      g[1] = T(L.farfield[o + 1]);
+      g[2] = T(L.farfield[o + 2]);
+      g[3] = T(L.farfield[o + 3]);
     }
   }
   template <typename T>

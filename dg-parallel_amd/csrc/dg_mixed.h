@@ -1,0 +1,389 @@
+// Mixed-order (P1-P4) triangle kernels as per-element __host__ __device__
+// functions, shared by the HIP kernels (dg_mixed.hip) and the CPU emulation:
+//
+//   mixed_stage_element  residual + one RK stage of an element of order P whose
+//                        neighbours have any order: each face is integrated with
+//                        the rule of the larger order, nq = max(P, Pnb) + 1
+//                        Gauss-Legendre points (degree 2 max(pL, pR) + 1,
+//                        DGCell.cc:408-423), the neighbour's trace taken with its
+//                        own basis (DGCell.cc:639-732);
+//   limit_element        triangle TVB limiter (new design; the reference's
+//                        DGMomentLimiter::limit, DGLimiter.cc:963-1133, indexes
+//                        quadrilateral tensor modes only);
+//   sensor_element       modal-decay shock sensor choosing the next order (new
+//                        design in place of the face-jump bins of DGPref.cc:24-60,
+//                        194-228, 602-630);
+//   dt_element           dt_e = CFL size_e / ((2p_e+1) lambda_e) (DGCell.cc:1155-1262).
+//
+// State layout: bucket p (order p) holds internal elements [bstart[p], bstart[p+1])
+// and its coefficients from double boff[p] on, [row = f*Np(p)+k][element].
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#include "dg_kernels.h"
+#include "dg_laws.h"
+#include "dg_stage_tiled.h"
+#include "dg_tables.h"
+
+namespace dgp {
+
+constexpr int MIX_CHUNK = 256;  // elements per work item of the persistent launch
+constexpr int MIX_MAXNP = 15;   // np_tri(4)
+
+struct MixedArgs {
+  int E;
+  int bstart[6];
+  long boff[6];
+  const double* tab;  // blocks of orders 1..4
+  int moff[6];
+  int ftoff[6][6];    // [p][nq]: trace table [3][nq][Np(p)] (absolute offset into tab)
+  const double* glw;  // Gauss-Legendre half weights at GLW_OFF(nq)
+  const double* mvtx; // [p][4][15]: phi_0 at [p][0][0], then the 3 vertices
+  TriElemTable el;
+  LawParams law;
+  const double* Uin;
+  const double* Un;
+  double* acc;
+  double* Uout;
+  double* R;
+  double dt;
+  int scheme, stage;  // scheme MODE_RESIDUAL: write R only
+  double* resid_part; // per workgroup of the persistent launch (or null)
+  const int* chunks;  // work items: (order, first internal element) pairs
+  int nchunks;
+};
+
+DG_HD int mixed_order_of(const MixedArgs& a, int i) {
+  return 1 + (i >= a.bstart[2]) + (i >= a.bstart[3]) + (i >= a.bstart[4]);
+}
+DG_HD long mixed_index(const MixedArgs& a, int p, int i, int row) {
+  return a.boff[p] + (long)row * (a.bstart[p + 1] - a.bstart[p]) + (i - a.bstart[p]);
+}
+
+// outward unit normal and length of local edge l from |J| J^-1 (as the uniform kernels)
+DG_HD void edge_normal(const double mg[2][2], int l, double n[2], double& len) {
+  double nl0, nl1;
+  if (l == 0) {
+    nl0 = -mg[1][0];
+    nl1 = -mg[1][1];
+  } else if (l == 1) {
+    nl0 = mg[0][0] + mg[1][0];
+    nl1 = mg[0][1] + mg[1][1];
+  } else {
+    nl0 = -mg[0][0];
+    nl1 = -mg[0][1];
+  }
+  len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
+  n[0] = nl0 / len;
+  n[1] = nl1 / len;
+}
+
+template <int P, class LAW>
+DG_HD void mixed_stage_element(const MixedArgs& a, int i, double& rsum) {
+  constexpr int NP = np_tri(P), NQ = nqv_tri(P), NF = LAW::NF;
+  const int E = a.E;
+  const double* tb = a.tab + a.moff[P];
+  const double* Uin = a.Uin;
+  double U[NF][NP], R[NF][NP];
+  for (int f = 0; f < NF; f++)
+    for (int k = 0; k < NP; k++) {
+      U[f][k] = Uin[mixed_index(a, P, i, f * NP + k)];
+      R[f][k] = 0.0;
+    }
+  double mg[2][2];
+  for (int r = 0; r < 2; r++)
+    for (int d = 0; d < 2; d++) mg[r][d] = a.el.mg[(long)(r * 2 + d) * E + i];
+
+  // volume term (DGCell.cc:184-249)
+  for (int q = 0; q < NQ; q++) {
+    double u[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; k++) s += tb[TriBlock::off_phi(P) + q * NP + k] * U[f][k];
+      u[f] = s;
+    }
+    double F[NF][2];
+    LAW::flux(a.law, u, F);
+    const double w = tb[TriBlock::off_wv(P) + q];
+    double G[NF][2];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      G[f][0] = w * (mg[0][0] * F[f][0] + mg[0][1] * F[f][1]);
+      G[f][1] = w * (mg[1][0] * F[f][0] + mg[1][1] * F[f][1]);
+    }
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      const double g0 = tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 0];
+      const double g1 = tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 1];
+#pragma unroll
+      for (int f = 0; f < NF; f++) R[f][k] += g0 * G[f][0] + g1 * G[f][1];
+    }
+  }
+
+  // face terms (DGCell.cc:639-732) with the rule of the larger order
+  for (int l = 0; l < 3; l++) {
+    const int link = a.el.link[(long)l * E + i];
+    double n[2], len;
+    edge_normal(mg, l, n, len);
+    int nq = P + 1;
+    double uR[MAXP_TRI + 1][NF];
+    if (link >= 0) {
+      const int nb = link >> 3, lb = link & 3;
+      const bool same = (link >> 2) & 1;
+      const int pn = mixed_order_of(a, nb), npn = (pn + 1) * (pn + 2) / 2;
+      nq = (pn > P ? pn : P) + 1;
+      const double* tn = a.tab + a.ftoff[pn][nq];
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        double v[MIX_MAXNP];
+#pragma unroll
+        for (int k = 0; k < MIX_MAXNP; k++) v[k] = k < npn ? a.Uin[mixed_index(a, pn, nb, f * npn + k)] : 0.0;
+#pragma unroll
+        for (int q = 0; q <= MAXP_TRI; q++) {
+          double s = 0.0;
+          if (q < nq) {
+            const int qq = same ? q : nq - 1 - q;
+#pragma unroll
+            for (int k = 0; k < MIX_MAXNP; k++)
+              if (k < npn) s += tn[(lb * nq + qq) * npn + k] * v[k];
+          }
+          uR[q][f] = s;
+        }
+      }
+    }
+    const double* to = a.tab + a.ftoff[P][nq];
+    const double* wq = a.glw + GLW_OFF(nq);
+#pragma unroll
+    for (int q = 0; q <= MAXP_TRI; q++) {
+      if (q < nq) {
+        double uL[NF], Fh[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < NP; k++) s += to[(l * nq + q) * NP + k] * U[f][k];
+          uL[f] = s;
+        }
+        if (link < 0) LAW::ghost(a.law, -link, n, uL, uR[q]);
+        LAW::riemann(a.law, n, uL, uR[q], Fh);
+        const double sw = len * wq[q];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          const double fs = Fh[f] * sw;
+#pragma unroll
+          for (int k = 0; k < NP; k++) R[f][k] -= fs * to[(l * nq + q) * NP + k];
+        }
+      }
+    }
+  }
+
+  // stage update (TimeIntegrator.cc:81-275, 279-438, 838-1048)
+  if (a.scheme == MODE_RESIDUAL) {
+    for (int f = 0; f < NF; f++)
+      for (int k = 0; k < NP; k++) a.R[mixed_index(a, P, i, f * NP + k)] = R[f][k];
+    return;
+  }
+  const double dtj = a.dt / a.el.detj[i];
+  const int sc = a.scheme * 4 + a.stage;
+  const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
+  const bool reads_acc = sc == 3 * 4 + 1 || sc == 3 * 4 + 2 || sc == 3 * 4 + 3;
+  for (int f = 0; f < NF; f++)
+    for (int k = 0; k < NP; k++) {
+      const long idx = mixed_index(a, P, i, f * NP + k);
+      double accv = reads_acc ? a.acc[idx] : 0.0;
+      const double o = stage_value_hd(sc, R[f][k] * dtj, a.Un[idx], reads_input ? Uin[idx] : 0.0, &accv, rsum);
+      if (sc >= 3 * 4 && sc < 3 * 4 + 3) a.acc[idx] = accv;
+      a.Uout[idx] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Triangle TVB limiter (new design), per element, Jacobi semantics: it reads the
+// means (mode 0, never modified) of its neighbours and rewrites only its own
+// modes 1.. in place, so no ordering between elements matters.
+//
+// For field f, with the element mean m = U_0 phi_0 and the face-neighbour means
+// m_l (boundary faces: the law's ghost state of the mean), at the midpoint c_l
+// of edge l:
+//   u_l   = u(c_l) - m                          the whole polynomial's deviation
+//   d_l   = U_1 phi_1(c_l) + U_2 phi_2(c_l)    the linear part's deviation
+//   D_l   = m_l - m
+//   the field is troubled when some |u_l| > tol and minmod(u_l, nu D_l) != u_l,
+//   tol = max(M h^2, 1e-10 |m|) (h = 2 x inradius; M = 0 is the TVD limiter;
+//   Cockburn & Shu 1998, sec. 5.3, for P^k);
+// a troubled field keeps its mean, takes the linear part rebuilt from the
+// limited deviations dl_l = minmod(d_l, nu D_l) (after the Cockburn-Shu
+// positive/negative rescaling that makes them sum to zero) and drops every
+// mode of degree >= 2.  Bit f of the returned flag is set when field f was limited.
+struct LimiterArgs {
+  int E;
+  int bstart[6];
+  long boff[6];
+  TriElemTable el;
+  LawParams law;
+  double* U;          // limited in place
+  const double* mvtx; // phi_0 at [p*60]
+  double midv[3][MIX_MAXNP];  // the basis (hierarchical: every order's prefix) at the three edge midpoints
+  double pinv[2][3];  // linear coefficients from midpoint deviations
+  double tvb_m, nu;
+  uint8_t* flags;     // internal order
+};
+
+DG_HD double lim_minmod(double a, double b) {
+  if (a > 0.0 && b > 0.0) return a < b ? a : b;
+  if (a < 0.0 && b < 0.0) return a > b ? a : b;
+  return 0.0;
+}
+
+template <class LAW>
+DG_HD void limit_element(const LimiterArgs& a, int i) {
+  constexpr int NF = LAW::NF;
+  MixedArgs ix{};
+  for (int p = 0; p < 6; p++) {
+    ix.bstart[p] = a.bstart[p];
+    ix.boff[p] = a.boff[p];
+  }
+  const int E = a.E, p = mixed_order_of(ix, i), np = (p + 1) * (p + 2) / 2;
+  const double phi0 = a.mvtx[p * 60];
+  double mean[NF], c1[NF], c2[NF], full[3][NF];
+  for (int f = 0; f < NF; f++) {
+    mean[f] = a.U[mixed_index(ix, p, i, f * np)] * phi0;
+    c1[f] = a.U[mixed_index(ix, p, i, f * np + 1)];
+    c2[f] = a.U[mixed_index(ix, p, i, f * np + 2)];
+    for (int l = 0; l < 3; l++) full[l][f] = c1[f] * a.midv[l][1];
+    for (int k = 2; k < np; k++) {
+      const double u = a.U[mixed_index(ix, p, i, f * np + k)];
+      for (int l = 0; l < 3; l++) full[l][f] = fma(u, a.midv[l][k], full[l][f]);
+    }
+  }
+  double mg[2][2];
+  for (int r = 0; r < 2; r++)
+    for (int d = 0; d < 2; d++) mg[r][d] = a.el.mg[(long)(r * 2 + d) * E + i];
+  double D[3][NF];
+  for (int l = 0; l < 3; l++) {
+    const int link = a.el.link[(long)l * E + i];
+    double g[NF];
+    if (link >= 0) {
+      const int nb = link >> 3, pn = mixed_order_of(ix, nb), npn = (pn + 1) * (pn + 2) / 2;
+      for (int f = 0; f < NF; f++) g[f] = a.U[mixed_index(ix, pn, nb, f * npn)] * a.mvtx[pn * 60];
+    } else {
+      double n[2], len;
+      edge_normal(mg, l, n, len);
+      LAW::ghost(a.law, -link, n, mean, g);
+    }
+    for (int f = 0; f < NF; f++) D[l][f] = g[f] - mean[f];
+  }
+  const double h = a.el.size[i];
+  uint8_t flag = 0;
+  for (int f = 0; f < NF; f++) {
+    double d[3];
+    bool troubled = false;
+    double tol = a.tvb_m * h * h;
+    const double tr = 1e-10 * (mean[f] < 0.0 ? -mean[f] : mean[f]);
+    if (tr > tol) tol = tr;
+    for (int l = 0; l < 3; l++) {
+      const double u = full[l][f], au = u < 0.0 ? -u : u;
+      if (au > tol && lim_minmod(u, a.nu * D[l][f]) != u) troubled = true;
+      d[l] = fma(c2[f], a.midv[l][2], c1[f] * a.midv[l][1]);
+    }
+    if (!troubled) continue;
+    flag |= (uint8_t)(1u << f);
+    double dl[3], pos = 0.0, neg = 0.0;
+    for (int l = 0; l < 3; l++) {
+      dl[l] = lim_minmod(d[l], a.nu * D[l][f]);
+      pos += dl[l] > 0.0 ? dl[l] : 0.0;
+      neg += dl[l] < 0.0 ? -dl[l] : 0.0;
+    }
+    if (pos != neg) {
+      const double tp = pos > 0.0 ? (neg < pos ? neg / pos : 1.0) : 0.0;
+      const double tn = neg > 0.0 ? (pos < neg ? pos / neg : 1.0) : 0.0;
+      for (int l = 0; l < 3; l++) dl[l] = dl[l] > 0.0 ? tp * dl[l] : tn * dl[l];
+    }
+    const double n1 = fma(a.pinv[0][2], dl[2], fma(a.pinv[0][1], dl[1], a.pinv[0][0] * dl[0]));
+    const double n2 = fma(a.pinv[1][2], dl[2], fma(a.pinv[1][1], dl[1], a.pinv[1][0] * dl[0]));
+    a.U[mixed_index(ix, p, i, f * np + 1)] = n1;
+    a.U[mixed_index(ix, p, i, f * np + 2)] = n2;
+    for (int k = 3; k < np; k++) a.U[mixed_index(ix, p, i, f * np + k)] = 0.0;
+  }
+  a.flags[i] = flag;
+}
+
+// ---------------------------------------------------------------------------
+// Modal-decay shock sensor (new design).  On field 0 (density for Euler), with
+// the orthonormal modes of the element's order p:
+//   top = sum of U_k^2 over the degree-p modes, all = sum over every mode,
+//   high = all - U_0^2;
+// the next order is p - 1 (not below pmin) when top > s_hi[p] all, p + 1 (not
+// above pmax) when top < s_lo[p] all, and p otherwise; an element whose modes
+// above the mean carry no energy (high <= 1e-24 all) keeps its order.  Sums run
+// in mode order with fused multiply-adds so the oracle can reproduce them bit
+// for bit.
+struct SensorArgs {
+  int E;
+  int bstart[6];
+  long boff[6];
+  const double* U;
+  double s_hi[6], s_lo[6];
+  int pmin, pmax;
+  uint8_t* orders;  // internal order
+};
+
+DG_HD void sensor_element(const SensorArgs& a, int i) {
+  MixedArgs ix{};
+  for (int p = 0; p < 6; p++) {
+    ix.bstart[p] = a.bstart[p];
+    ix.boff[p] = a.boff[p];
+  }
+  const int p = mixed_order_of(ix, i), np = (p + 1) * (p + 2) / 2, npl = p * (p + 1) / 2;
+  double all = 0.0, top = 0.0, high = 0.0;
+  for (int k = 0; k < np; k++) {
+    const double u = a.U[mixed_index(ix, p, i, k)];
+    all = fma(u, u, all);
+    if (k >= 1) high = fma(u, u, high);
+    if (k >= npl) top = fma(u, u, top);
+  }
+  int next = p;
+  if (high > 1e-24 * all) {
+    if (top > a.s_hi[p] * all)
+      next = p - 1 < a.pmin ? a.pmin : p - 1;
+    else if (top < a.s_lo[p] * all)
+      next = p + 1 > a.pmax ? a.pmax : p + 1;
+  }
+  a.orders[i] = (uint8_t)next;
+}
+
+// ---------------------------------------------------------------------------
+// time-step candidate of one element of any order (DGCell.cc:1155-1207)
+template <class LAW>
+DG_HD double dt_element(const MixedArgs& a, int i, double cfl) {
+  constexpr int NF = LAW::NF;
+  const int p = mixed_order_of(a, i), np = (p + 1) * (p + 2) / 2;
+  const double* vt = a.mvtx + p * 60;
+  double mean[NF];
+  for (int f = 0; f < NF; f++) mean[f] = a.Uin[mixed_index(a, p, i, f * np)] * vt[0];
+  double lam = LAW::maxev(a.law, mean);
+  for (int v = 0; v < 3; v++) {
+    double val[NF];
+    for (int f = 0; f < NF; f++) {
+      double s = 0.0;
+      for (int k = 0; k < np; k++) s += a.Uin[mixed_index(a, p, i, f * np + k)] * vt[(1 + v) * 15 + k];
+      val[f] = s;
+    }
+    const double b = LAW::maxev(a.law, val);
+    lam = b > lam ? b : lam;
+  }
+  return a.el.size[i] * cfl / (lam * (2. * p + 1.0));
+}
+
+// launches (dg_mixed.hip); part holds one value per workgroup of the launch
+int mixed_grid(int nchunks);
+int launch_mixed_stage(const MixedArgs& a, int law, hipStream_t s);
+int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipStream_t s);
+int launch_limiter(const LimiterArgs& a, int law, hipStream_t s);
+int launch_sensor(const SensorArgs& a, hipStream_t s);
+
+}  // namespace dgp

@@ -1,0 +1,110 @@
+// HIP kernels of the mixed-order path (element functions in dg_mixed.h).
+//
+// tri_mixed_stage is one persistent launch for every order bucket: the host
+// splits each bucket into chunks of MIX_CHUNK elements, heaviest order first,
+// and each workgroup walks the chunk list with a stride of the grid size.  A
+// chunk never spans buckets, so the order switch is uniform per workgroup and
+// every wave runs one specialised path.  All workgroups reach the end of the
+// list; the residual norm is reduced per workgroup in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "dg_mixed.h"
+
+namespace dgp {
+
+constexpr int MIX_BLOCK = MIX_CHUNK;
+
+int mixed_grid(int nchunks) { return nchunks < 2048 ? nchunks : 2048; }
+
+template <class LAW>
+__global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_stage(MixedArgs a) {
+  double rsum = 0.0;
+  for (int w = blockIdx.x; w < a.nchunks; w += gridDim.x) {
+    const int p = a.chunks[2 * w], i = a.chunks[2 * w + 1] + threadIdx.x;
+    if (i < a.bstart[p + 1]) {
+      switch (p) {
+        case 1: mixed_stage_element<1, LAW>(a, i, rsum); break;
+        case 2: mixed_stage_element<2, LAW>(a, i, rsum); break;
+        case 3: mixed_stage_element<3, LAW>(a, i, rsum); break;
+        default: mixed_stage_element<4, LAW>(a, i, rsum); break;
+      }
+    }
+  }
+  if (a.resid_part) {
+    __shared__ double red[MIX_BLOCK];
+    red[threadIdx.x] = rsum;
+    __syncthreads();
+    for (int s = MIX_BLOCK / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.resid_part[blockIdx.x] = red[0];
+  }
+}
+
+template <class LAW>
+__global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_dt(MixedArgs a, double cfl, double* part) {
+  const int i = blockIdx.x * MIX_BLOCK + threadIdx.x;
+  const double dt = i < a.E ? dt_element<LAW>(a, i, cfl) : 1e300;
+  __shared__ double red[MIX_BLOCK];
+  red[threadIdx.x] = dt;
+  __syncthreads();
+  for (int s = MIX_BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+template <class LAW>
+__global__ __launch_bounds__(256) void tri_limit(LimiterArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < a.E) limit_element<LAW>(a, i);
+}
+
+__global__ __launch_bounds__(256) void tri_sensor(SensorArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < a.E) sensor_element(a, i);
+}
+
+int launch_mixed_stage(const MixedArgs& a, int law, hipStream_t s) {
+  if (a.nchunks <= 0) return -1;
+  dim3 grid(mixed_grid(a.nchunks)), block(MIX_BLOCK);
+  if (law == LAW_SCALAR)
+    hipLaunchKernelGGL(tri_mixed_stage<ScalarLawD>, grid, block, 0, s, a);
+  else if (law == LAW_EULER2D)
+    hipLaunchKernelGGL(tri_mixed_stage<EulerLawD>, grid, block, 0, s, a);
+  else
+    return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipStream_t s) {
+  dim3 grid((a.E + MIX_BLOCK - 1) / MIX_BLOCK), block(MIX_BLOCK);
+  if (law == LAW_SCALAR)
+    hipLaunchKernelGGL(tri_mixed_dt<ScalarLawD>, grid, block, 0, s, a, cfl, part);
+  else if (law == LAW_EULER2D)
+    hipLaunchKernelGGL(tri_mixed_dt<EulerLawD>, grid, block, 0, s, a, cfl, part);
+  else
+    return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_limiter(const LimiterArgs& a, int law, hipStream_t s) {
+  dim3 grid((a.E + 255) / 256), block(256);
+  if (law == LAW_SCALAR)
+    hipLaunchKernelGGL(tri_limit<ScalarLawD>, grid, block, 0, s, a);
+  else if (law == LAW_EULER2D)
+    hipLaunchKernelGGL(tri_limit<EulerLawD>, grid, block, 0, s, a);
+  else
+    return -1;
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_sensor(const SensorArgs& a, hipStream_t s) {
+  dim3 grid((a.E + 255) / 256), block(256);
+  hipLaunchKernelGGL(tri_sensor, grid, block, 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace dgp

@@ -42,26 +42,33 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   S.law = law->law;
   S.NF = law->law == DG_LAW_SCALAR ? 1 : law->law == DG_LAW_EULER2D ? 4 : -1;
   if (S.NF < 0) return failS(err, DG_ERR_UNSUPPORTED, "law %d not built for triangles", law->law);
-  if (S.law == DG_LAW_EULER2D && law->flux != DG_FLUX_LF && law->flux != DG_FLUX_ROE)
+  if (S.law == DG_LAW_EULER2D && law->flux != DG_FLUX_LF && law->flux != DG_FLUX_ROE && law->flux != DG_FLUX_ROE_EF)
     return failS(err, DG_ERR_UNSUPPORTED, "Euler flux %d not built", law->flux);
   if (S.law == DG_LAW_SCALAR && law->flux != DG_FLUX_UPWIND)
     return failS(err, DG_ERR_UNSUPPORTED, "scalar flux %d not built", law->flux);
-  int P = opt->order;
-  if (m->order) {
-    P = m->order[0];
-    for (int e = 1; e < S.E; e++)
-      if (m->order[e] != P) return failS(err, DG_ERR_UNSUPPORTED, "mixed orders are not built in this entry point");
-  }
-  if (P < 1 || P > MAXP_TRI) return failS(err, DG_ERR_UNSUPPORTED, "order %d outside 1..%d", P, MAXP_TRI);
+  // per-element orders (mesh->order) or one order for all (opt->order)
+  std::vector<int> ord(S.E, opt->order);
+  if (m->order)
+    for (int e = 0; e < S.E; e++) ord[e] = m->order[e];
+  for (int e = 0; e < S.E; e++)
+    if (ord[e] < 1 || ord[e] > MAXP_TRI)
+      return failS(err, DG_ERR_UNSUPPORTED, "order %d of element %d outside 1..%d", ord[e], e, MAXP_TRI);
+  int P = ord[0];
+  for (int e = 1; e < S.E; e++)
+    if (ord[e] != P) S.mixed = true;
   if (opt->precision != DG_PREC_FP64 && opt->precision != DG_PREC_FP32) return failS(err, DG_ERR_ARG, "bad precision");
-  S.P = P;
-  S.NP = np_tri(P);
+  if (S.mixed && opt->precision != DG_PREC_FP64)
+    return failS(err, DG_ERR_UNSUPPORTED, "mixed orders run in the FP64 form only");
+  S.P = S.mixed ? 0 : P;
+  S.NP = S.mixed ? 0 : np_tri(P);
   S.lp.gamma = law->gamma;
   for (int i = 0; i < 3; i++) S.lp.vel[i] = law->vel[i];
   for (int i = 0; i < 8; i++) S.lp.farfield[i] = law->farfield[i];
   S.lp.flux = law->flux;
-  const int E = S.E, NP = S.NP;
+  const int E = S.E;
   S.xyz.assign(m->xyz, m->xyz + (size_t)E * 6);
+  S.coff.assign(E + 1, 0);
+  for (int e = 0; e < E; e++) S.coff[e + 1] = S.coff[e] + (long)S.NF * np_tri(ord[e]);
 
   // ---- connectivity (mMesh.cc:321-378, translation-periodic by vertex ids) ----
   std::vector<FaceKey> keys((size_t)E * 3);
@@ -117,7 +124,7 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
         v = nbl[e * 3 + l] | (same[e * 3 + l] << 2);
       else {
         int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
-        v = 8 + (tag == DG_TAG_WALL ? BC_WALL : BC_FARFIELD);
+        v = 8 + (tag == DG_TAG_WALL ? BC_WALL : tag == DG_TAG_FARFIELD2 ? BC_FARFIELD2 : BC_FARFIELD);
       }
       s = s * 16 + v;
     }
@@ -129,11 +136,24 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   for (int e = 0; e < E; e++) sg[e] = sig(e);
   const int BLK = 2 * 64;
   std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) {
+    if (ord[a] != ord[b]) return ord[a] < ord[b];  // order buckets
     if (a / BLK != b / BLK) return a / BLK < b / BLK;
     return sg[a] < sg[b];
   });
   std::vector<int32_t> iperm(E);
   for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
+  S.order.resize(E);
+  for (int i = 0; i < E; i++) S.order[i] = (int8_t)ord[S.perm[i]];
+  {
+    int cnt[6] = {0, 0, 0, 0, 0, 0};
+    for (int e = 0; e < E; e++) cnt[ord[e]]++;
+    S.bstart[1] = 0;
+    S.boff[1] = 0;
+    for (int p = 1; p <= MAXP_TRI; p++) {
+      S.bstart[p + 1] = S.bstart[p] + cnt[p];
+      S.boff[p + 1] = S.boff[p] + (long)S.NF * np_tri(p) * cnt[p];
+    }
+  }
 
   // ---- geometry (Mapping.cc:442-566, 687-712; DGCell.cc:1209-1229) ---------------
   S.mg.assign((size_t)4 * E, 0.0);
@@ -168,13 +188,47 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
         v = (iperm[nbe[e * 3 + l]] << 3) | (same[e * 3 + l] << 2) | nbl[e * 3 + l];
       else {
         int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
-        v = -(tag == DG_TAG_WALL ? BC_WALL : BC_FARFIELD);
+        v = -(tag == DG_TAG_WALL ? BC_WALL : tag == DG_TAG_FARFIELD2 ? BC_FARFIELD2 : BC_FARFIELD);
       }
       h_link[(size_t)l * E + i] = v;
     }
   }
 
   // ---- tables ---------------------------------------------------------------------
+  {
+    // every order's block (mixed-order kernels, limiter, sensor, dt)
+    int o = 0;
+    for (int p = 1; p <= MAXP_TRI; p++) {
+      S.moff[p] = o;
+      o += TriBlock::size(p);
+    }
+    S.mtab.assign(o, 0.0);
+    for (int p = 1; p <= MAXP_TRI; p++) {
+      try {
+        build_tri_block(p, S.mtab.data() + S.moff[p]);
+      } catch (const std::exception& ex) {
+        return failS(err, DG_ERR_ARG, "%s", ex.what());
+      }
+      for (int nq = 0; nq < 6; nq++)
+        S.ftoff[p][nq] = nq < p + 1 ? -1 : S.moff[p] + (nq == p + 1 ? TriBlock::off_phif(p) : TriBlock::off_phifx(p, nq));
+    }
+    for (int nq = 0; nq < 6; nq++) S.ftoff[0][nq] = -1;
+    S.glw.assign(GLW_OFF(MAXP_TRI + 2), 0.0);
+    build_glw(S.glw.data());
+    S.mvtx.assign((size_t)(MAXP_TRI + 1) * 4 * 15, 0.0);
+    double z[3] = {0, 0, 0}, phi[64];
+    static const double V[3][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};
+    for (int p = 1; p <= MAXP_TRI; p++) {
+      basis(2, 0, z, phi, nullptr);
+      S.mvtx[(size_t)p * 60] = phi[0];
+      for (int v = 0; v < 3; v++) {
+        basis(2, p, V[v], phi, nullptr);
+        for (int k = 0; k < np_tri(p); k++) S.mvtx[(size_t)p * 60 + (1 + v) * 15 + k] = phi[k];
+      }
+    }
+  }
+  if (S.mixed) return DG_OK;
+  const int NP = S.NP;
   S.tab.assign(TriBlock::size(P), 0.0);
   try {
     build_tri_block(P, S.tab.data());

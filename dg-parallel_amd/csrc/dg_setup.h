@@ -14,6 +14,22 @@ namespace dgp {
 
 struct TriSetup {
   int E = 0, P = 0, NP = 0, NF = 0, law = 0;
+  // Elements are bucketed by order: internal elements [bstart[p], bstart[p+1])
+  // have order p (p = 1..4) and their coefficients start at double boff[p] of
+  // the state arrays, stored [row][element] within the bucket.  A uniform-order
+  // mesh is one bucket at offset 0 (P, NP describe it); mixed is set otherwise.
+  bool mixed = false;
+  int bstart[6] = {0, 0, 0, 0, 0, 0};
+  long boff[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<int8_t> order;    // internal order of each element
+  std::vector<long> coff;       // caller element -> first coefficient in the caller layout (E+1)
+  // mixed-order tables: the blocks of orders 1..4 at moff[p]; ftoff[p][nq] is the
+  // trace table [3][nq][Np(p)] of order p for the face rule of nq points;
+  // glw the Gauss-Legendre half weights (GLW_OFF); mvtx[p][4][15] the mean and
+  // vertex basis values of order p
+  std::vector<double> mtab, glw, mvtx;
+  int moff[6] = {0, 0, 0, 0, 0, 0};
+  int ftoff[6][6];
   LawParams lp{};
   std::vector<int32_t> perm;   // internal index -> caller element
   std::vector<int32_t> faces;  // (eL, lL, eR, lR) in caller numbering

@@ -35,10 +35,10 @@ def device_synchronize():
 
 DG_OK, DG_ERR_ARG, DG_ERR_OOM, DG_ERR_HIP, DG_ERR_NONFINITE, DG_ERR_NONPHYSICAL, DG_ERR_UNSUPPORTED = range(7)
 LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
-FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL = 0, 1, 2, 3
+FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL, FLUX_ROE_EF = 0, 1, 2, 3, 4
 RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
-TAG_FARFIELD, TAG_CHARACTERISTIC, TAG_WALL = 1, 30000, 40000
+TAG_FARFIELD, TAG_FARFIELD2, TAG_CHARACTERISTIC, TAG_WALL = 1, 2, 30000, 40000
 FIELD_GAUSSIAN, FIELD_VORTEX, FIELD_SOD, FIELD_CAVITY = 0, 1, 2, 3
 
 STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_OOM", 3: "DG_ERR_HIP", 4: "DG_ERR_NONFINITE",
@@ -91,6 +91,17 @@ _compute_dt = _sig("dg_compute_dt", ctypes.c_int, [_vp, ctypes.c_double, _p_f64]
 _advance = _sig("dg_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _p_f64])
 _time_advance = _sig("dg_time_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                        ctypes.POINTER(ctypes.c_float)])
+class SensorOpts(ctypes.Structure):
+    _fields_ = [("s_hi", ctypes.c_double * 4), ("s_lo", ctypes.c_double * 4), ("pmin", ctypes.c_int),
+                ("pmax", ctypes.c_int)]
+
+
+_p_u8 = ctypes.POINTER(ctypes.c_uint8)
+_set_limiter = _sig("dg_set_limiter", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double])
+_limit = _sig("dg_limit", ctypes.c_int, [_vp, _p_u8])
+_limiter_flags = _sig("dg_limiter_flags", ctypes.c_int, [_vp, _p_u8])
+_select_orders = _sig("dg_select_orders", ctypes.c_int, [_vp, ctypes.POINTER(SensorOpts), ctypes.c_int, _p_i32])
+_get_orders = _sig("dg_get_orders", ctypes.c_int, [_vp, _p_i32])
 _jvp = _sig("dg_residual_jvp", ctypes.c_int, [_vp, _p_f64, _p_f64, _p_f64])
 _vjp = _sig("dg_residual_vjp", ctypes.c_int, [_vp, _p_f64, _p_f64, _p_f64])
 _adj_fwd = _sig("dg_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _p_f64, _p_f64])
@@ -192,6 +203,7 @@ class DGAnalysis:
             self._h = None
             raise DGError(st, msg)
         self.ndof = _ndof(h)
+        self.mesh_E = md.n_elems
 
     def _check(self, st):
         if st != DG_OK:
@@ -294,3 +306,33 @@ class DGAnalysis:
         f, b = ctypes.c_float(), ctypes.c_float()
         self._check(_adj_timing(self._h, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+    # ---- limiter and order selection (FP64 form) ----
+    def set_limiter(self, enabled, tvb_m=0.0, nu=1.0):
+        """limit after every stage of advanceInTime / time_advance"""
+        self._check(_set_limiter(self._h, 1 if enabled else 0, tvb_m, nu))
+
+    def limit(self):
+        """limit the current state once; per-element flags (bit f: field f limited)"""
+        out = np.zeros(self.mesh_E, np.uint8)
+        self._check(_limit(self._h, out.ctypes.data_as(_p_u8)))
+        return out
+
+    def limiter_flags(self):
+        out = np.zeros(self.mesh_E, np.uint8)
+        self._check(_limiter_flags(self._h, out.ctypes.data_as(_p_u8)))
+        return out
+
+    def select_orders(self, s_hi, s_lo, pmin=1, pmax=4, apply=False):
+        """sensor orders per element; with apply the context re-buckets (ndof changes)"""
+        o = SensorOpts((ctypes.c_double * 4)(*s_hi), (ctypes.c_double * 4)(*s_lo), pmin, pmax)
+        out = np.zeros(self.mesh_E, np.int32)
+        self._check(_select_orders(self._h, ctypes.byref(o), 1 if apply else 0, _ptr(out, _p_i32)))
+        if apply:
+            self.ndof = _ndof(self._h)
+        return out
+
+    def orders(self):
+        out = np.zeros(self.mesh_E, np.int32)
+        self._check(_get_orders(self._h, _ptr(out, _p_i32)))
+        return out

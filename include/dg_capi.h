@@ -40,14 +40,19 @@ typedef enum {
 } dg_status;
 
 enum { DG_LAW_SCALAR = 0, DG_LAW_EULER2D = 1, DG_LAW_MAXWELL3D = 2 };
-enum { DG_FLUX_UPWIND = 0, DG_FLUX_LF = 1, DG_FLUX_ROE = 2, DG_FLUX_CENTRAL = 3 };
+/* DG_FLUX_ROE is Euler2dRoe::riemannSolver (EulerLaw.cc:913-1010); DG_FLUX_ROE_EF
+ * adds Harten's entropy fix (delta = 0.2 c), which the reference lacks */
+enum { DG_FLUX_UPWIND = 0, DG_FLUX_LF = 1, DG_FLUX_ROE = 2, DG_FLUX_CENTRAL = 3, DG_FLUX_ROE_EF = 4 };
 enum { DG_RK_FE = 0, DG_RK_TVD2 = 1, DG_RK_TVD3 = 2, DG_RK4 = 3 };
 enum { DG_PREC_FP64 = 0, DG_PREC_FP32 = 1 };
 /* Boundary tags (the reference's classification ids, EulerLaw.cc:182-296):
  * 40000 reflective wall; 30000 characteristic far field (not built: rejected);
  * any other tag on a face without neighbour: Riemann flux against the law's
- * far-field state (the reference's default branch, EulerLaw.cc:293-295). */
-enum { DG_TAG_FARFIELD = 1, DG_TAG_CHARACTERISTIC = 30000, DG_TAG_WALL = 40000 };
+ * far-field state (the reference's default branch, EulerLaw.cc:293-295).  The
+ * reference evaluates its far-field FieldEvaluator at the boundary point; tag
+ * DG_TAG_FARFIELD2 selects a second constant state (Euler: farfield[4..8)), so
+ * two boundaries can hold different states (the two ends of a shock tube). */
+enum { DG_TAG_FARFIELD = 1, DG_TAG_FARFIELD2 = 2, DG_TAG_CHARACTERISTIC = 30000, DG_TAG_WALL = 40000 };
 
 /* Mesh handed over by the caller (replaces mDGMesh + mImportExport for the path:
  * mMesh.h:52-139).  Faces are matched by their sorted topological vertex ids;
@@ -60,7 +65,10 @@ typedef struct {
   const double* xyz;      /* [E][dim+1][dim] corner coordinates of each element */
   const int32_t* bc;      /* [E][dim+1] tag of faces with no neighbour, NULL: far field.
                              Local face l of a triangle is edge (v_l, v_{l+1}) (Face.cc:114-148) */
-  const int32_t* order;   /* [E] polynomial order per element, NULL: options.order */
+  const int32_t* order;   /* [E] polynomial order per element (1..4), NULL: options.order.
+                             Mixed orders (DGCell::adaptOrder, DGCell.cc:1542-1602) run
+                             the FP64 form in one persistent launch over order buckets;
+                             the state layout is then [e][f][k] with Np(p_e) modes */
 } dg_mesh_desc;
 
 /* Law selection (replaces the ConservationLaw subclass instance; the host C++
@@ -124,6 +132,40 @@ dg_status dg_get_state_device(dg_ctx* ctx, double* dU);
 /* Timing helper for benchmarks: replays nsteps of dg_advance on the context's
  * stream between two HIP events and returns the elapsed milliseconds. */
 dg_status dg_time_advance(dg_ctx* ctx, int rk_scheme, double dt, int nsteps, float* ms_out);
+
+/* ---- limiter and order selection (triangles; FP64 form) -------------------
+ * The reference's limiter interface is DGLimiter::limit(DGCell*, time)
+ * (DGLimiter.h:12), applied to every cell after each RK stage
+ * (TimeIntegrator::limit, TimeIntegrator.cc:1414-1434; calls at :139-272,
+ * :359-429, :850-987).  Its moment limiter (DGLimiter.cc:963-1133) indexes
+ * quadrilateral tensor modes; this build limits triangles with a TVB minmod of
+ * the linear part at the edge midpoints against the neighbour means (M: TVB
+ * constant, 0 = TVD; nu: slope factor), dropping the modes of degree >= 2 of a
+ * limited field.  Flags: one byte per element (caller order), bit f set when
+ * field f was limited.  When enabled, dg_advance limits after every stage. */
+dg_status dg_set_limiter(dg_ctx* ctx, int enabled, double tvb_m, double nu);
+/* limit the current state once (TimeIntegrator::limit); flags_out may be NULL */
+dg_status dg_limit(dg_ctx* ctx, uint8_t* flags_out);
+/* flags of the last limiter pass (dg_limit or the last stage of dg_advance) */
+dg_status dg_limiter_flags(dg_ctx* ctx, uint8_t* flags_out);
+
+/* Order selection (replaces DGAnalysis::adaptP / modifyP, DGPref.cc:24-60,
+ * 602-630, whose face-jump bins choose orders 1..3): a modal-decay sensor on
+ * field 0.  With top = energy of the degree-p modes and all = energy of every
+ * mode of an element of order p, the next order is max(pmin, p-1) when
+ * top > s_hi[p-1] all, min(pmax, p+1) when top < s_lo[p-1] all, else p
+ * (elements with no energy above the mean keep p).  orders_out (caller order)
+ * may be NULL; with apply the context re-buckets its elements by the new orders
+ * and carries the state over (raised orders get zero modes, lowered orders drop
+ * the top modes: DGCell::adaptOrder). */
+typedef struct {
+  double s_hi[4];         /* per current order 1..4 */
+  double s_lo[4];
+  int pmin, pmax;
+} dg_sensor_opts;
+dg_status dg_select_orders(dg_ctx* ctx, const dg_sensor_opts* opts, int apply, int32_t* orders_out);
+/* current per-element orders, caller order */
+dg_status dg_get_orders(dg_ctx* ctx, int32_t* orders_out);
 
 /* ---- derivatives (no counterpart in the reference) ------------------------
  * Tangent-linear and adjoint residual products at a state U (boundary layout,

@@ -274,6 +274,9 @@ struct or_ctx {
   double vdphi[8][MAXQ][MAXNP][3];
   /* scratch for face contributions */
   double* fcontrib;    /* [F][2][Nf*MAXNP] */
+  /* limiter applied after every RK stage when on (or_set_limiter) */
+  int lim_on;
+  double lim_m, lim_nu;
 };
 
 static int nfields_of(const or_law* l) {
@@ -720,6 +723,14 @@ static void euler_roe(const or_law* L, const double* n, const double* u0, const 
     eg[1] = fabs(vn);
     eg[2] = fabs(vn + c_A);
     eg[3] = fabs(vn - c_A);
+    if (L->flux == OR_FLUX_ROE_EF) {
+      /* Harten's entropy fix (Harten & Hyman 1983), delta = 0.2 c: new option; the
+       * reference's Roe flux has none, and its zero dissipation of the linear
+       * waves on faces with vn ~ 0 lets a shock-tube run blow up */
+      double dl = 0.2 * c_A;
+      for (int i = 0; i < 4; i++)
+        if (eg[i] < dl) eg[i] = (eg[i] * eg[i] + dl * dl) * (0.5 / dl);
+    }
     ev[0][0] = 1.; ev[0][1] = u_A; ev[0][2] = v_A; ev[0][3] = 0.5 * (u_A * u_A + v_A * v_A);
     ev[1][0] = 0.; ev[1][1] = n[1] * rho_A; ev[1][2] = -n[0] * rho_A;
     ev[1][3] = rho_A * (u_A * n[1] - v_A * n[0]);
@@ -784,7 +795,7 @@ static void law_riemann(const or_law* L, const double* n, const double* uL, cons
     double u = vn > 0 ? uL[0] : uR[0];
     out[0] = (L->vel[0] * u) * n[0] + (L->vel[1] * u) * n[1] + (L->vel[2] * u) * n[2];
   } else if (L->law == OR_LAW_EULER2D) {
-    if (L->flux == OR_FLUX_ROE)
+    if (L->flux == OR_FLUX_ROE || L->flux == OR_FLUX_ROE_EF)
       euler_roe(L, n, uL, uR, out);
     else
       euler_lf(L, n, uL, uR, out);
@@ -814,10 +825,10 @@ static void law_boundary(const or_law* L, int tag, const double* n, const double
       ghost[2] = vn * n[1] - vt * n[0];
       ghost[3] = Q[3];
     } else {
-      /* far-field ghost (EulerLaw.cc:293-295) */
+      /* far-field ghost (EulerLaw.cc:293-295); tag 2 takes the second state */
       (void)x;
       (void)T;
-      for (int i = 0; i < Nf; i++) ghost[i] = L->farfield[i];
+      for (int i = 0; i < Nf; i++) ghost[i] = L->farfield[(tag == 2 ? 4 : 0) + i];
     }
     law_riemann(L, n, Q, ghost, out);
     return;
@@ -976,6 +987,178 @@ int or_residual(or_ctx* c, const double* U, double t, double* R, int nthreads) {
 /* ------------------------------------------------------------------------- */
 /* RK stages                                                                  */
 /* ------------------------------------------------------------------------- */
+/* ------------------------------------------------------------------------- */
+/* limiter and sensor (triangles)                                             */
+/* ------------------------------------------------------------------------- */
+/* Ghost state of a boundary face for the limiter's neighbour mean: Euler wall
+ * mirror (EulerLaw.cc:280-291), far field (EulerLaw.cc:293-295), scalar far field
+ * (ScalarLaw.cc:39-59). */
+static void law_ghost(const or_law* L, int tag, const double* n, const double* Q, double* g) {
+  int Nf = nfields_of(L);
+  if (L->law == OR_LAW_EULER2D && tag == 40000) {
+    double vn = -(Q[1] * n[0] + Q[2] * n[1]);
+    double vt = Q[1] * n[1] - Q[2] * n[0];
+    g[0] = Q[0];
+    g[1] = vn * n[0] + vt * n[1];
+    g[2] = vn * n[1] - vt * n[0];
+    g[3] = Q[3];
+    return;
+  }
+  for (int i = 0; i < Nf; i++) g[i] = L->farfield[(L->law == OR_LAW_EULER2D && tag == 2 ? 4 : 0) + i];
+}
+
+static double mm2(double a, double b) {
+  if (a > 0.0 && b > 0.0) return a < b ? a : b;
+  if (a < 0.0 && b < 0.0) return a > b ? a : b;
+  return 0.0;
+}
+
+static double tri_size(const or_ctx* c, int e) {
+  const double* X = c->xyz + (long)e * 6;
+  double a = hypot(X[2] - X[0], X[3] - X[1]);
+  double b = hypot(X[4] - X[2], X[5] - X[3]);
+  double cc = hypot(X[0] - X[4], X[1] - X[5]);
+  double hp = 0.5 * (a + b + cc);
+  return sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
+}
+
+/* TVB limiter on triangles, new design (the reference's DGMomentLimiter,
+ * DGLimiter.cc:963-1133, indexes quadrilateral tensor modes; its driver
+ * TimeIntegrator::limit, TimeIntegrator.cc:1414-1434, limits every cell from
+ * the unlimited neighbours, then copies back: Jacobi semantics, kept here by
+ * reading neighbour means only, which the limiter never changes).
+ * Per field, at the edge midpoints c_l: u_l = u(c_l) - mean (whole polynomial),
+ * D_l = (neighbour or ghost mean) - mean; troubled when some |u_l| > tol and
+ * minmod(u_l, nu D_l) != u_l, tol = max(M h^2, 1e-10 |mean|) (Cockburn & Shu
+ * 1998, sec. 5.3, for P^k); the linear part's deviations d_l = U1 phi1(c_l) +
+ * U2 phi2(c_l) are then limited to minmod(d_l, nu D_l); a troubled field
+ * keeps its mean, gets the linear part of the limited deviations (rescaled to
+ * sum to zero, Cockburn & Shu 1998, sec. 5.2) and loses its modes of degree >= 2. */
+int or_limit(or_ctx* c, double* U, double tvb_m, double nu, uint8_t* flags) {
+  if (c->dim != 2) return -1;
+  int Nf = nfields_of(&c->law);
+  static const double mid[3][3] = {{0.5, 0.0, 0.0}, {0.5, 0.5, 0.0}, {0.0, 0.5, 0.0}};
+  double m1[3], m2[3], midv[3][MAXNP], phi0;
+  {
+    double z[3] = {0, 0, 0};
+    basis_eval(2, 0, z, &phi0, NULL);
+  }
+  for (int l = 0; l < 3; l++) {
+    basis_eval(2, 4, mid[l], midv[l], NULL);
+    m1[l] = midv[l][1];
+    m2[l] = midv[l][2];
+  }
+  /* least-squares inverse of [phi1(c_l) phi2(c_l)] */
+  double A00 = 0, A01 = 0, A11 = 0;
+  for (int l = 0; l < 3; l++) {
+    A00 += m1[l] * m1[l];
+    A01 += m1[l] * m2[l];
+    A11 += m2[l] * m2[l];
+  }
+  double det = A00 * A11 - A01 * A01;
+  double I00 = A11 / det, I01 = -A01 / det, I10 = -A01 / det, I11 = A00 / det;
+  double pinv[2][3];
+  for (int l = 0; l < 3; l++) {
+    pinv[0][l] = I00 * m1[l] + I01 * m2[l];
+    pinv[1][l] = I10 * m1[l] + I11 * m2[l];
+  }
+  double* out = (double*)malloc(sizeof(double) * c->ndof);
+  memcpy(out, U, sizeof(double) * c->ndof);
+  for (int e = 0; e < c->E; e++) {
+    int p = c->order[e], Np = np_of(2, p);
+    const double* Ue = U + c->uoff[e];
+    double mean[MAXNF], D[3][MAXNF];
+    for (int f = 0; f < Nf; f++) mean[f] = Ue[f * Np] * phi0;
+    for (int l = 0; l < 3; l++) {
+      const or_face* fc = &c->faces[c->ef[e * c->nfl + l]];
+      double g[MAXNF];
+      int other = c->eside[e * c->nfl + l] == 0 ? fc->eR : fc->eL;
+      if (other >= 0) {
+        int Npo = np_of(2, c->order[other]);
+        for (int f = 0; f < Nf; f++) g[f] = U[c->uoff[other] + f * Npo] * phi0;
+      } else {
+        law_ghost(&c->law, fc->tag, fc->n, mean, g);
+      }
+      for (int f = 0; f < Nf; f++) D[l][f] = g[f] - mean[f];
+    }
+    double h = tri_size(c, e);
+    uint8_t flag = 0;
+    for (int f = 0; f < Nf; f++) {
+      double d[3], tol = tvb_m * h * h, tr = 1e-10 * fabs(mean[f]);
+      if (tr > tol) tol = tr;
+      int troubled = 0;
+      for (int l = 0; l < 3; l++) {
+        /* detection on the whole polynomial: u(c_l) - mean */
+        double full = Ue[f * Np + 1] * midv[l][1];
+        for (int k = 2; k < Np; k++) full = fma(Ue[f * Np + k], midv[l][k], full);
+        if (fabs(full) > tol && mm2(full, nu * D[l][f]) != full) troubled = 1;
+        d[l] = fma(Ue[f * Np + 2], m2[l], Ue[f * Np + 1] * m1[l]);
+      }
+      if (!troubled) continue;
+      flag |= (uint8_t)(1u << f);
+      double dl[3], pos = 0, neg = 0;
+      for (int l = 0; l < 3; l++) {
+        dl[l] = mm2(d[l], nu * D[l][f]);
+        if (dl[l] > 0) pos += dl[l];
+        if (dl[l] < 0) neg += -dl[l];
+      }
+      if (pos != neg) {
+        double tp = pos > 0 ? (neg < pos ? neg / pos : 1.0) : 0.0;
+        double tn = neg > 0 ? (pos < neg ? pos / neg : 1.0) : 0.0;
+        for (int l = 0; l < 3; l++) dl[l] = dl[l] > 0 ? tp * dl[l] : tn * dl[l];
+      }
+      double* Oe = out + c->uoff[e];
+      Oe[f * Np + 1] = fma(pinv[0][2], dl[2], fma(pinv[0][1], dl[1], pinv[0][0] * dl[0]));
+      Oe[f * Np + 2] = fma(pinv[1][2], dl[2], fma(pinv[1][1], dl[1], pinv[1][0] * dl[0]));
+      for (int k = 3; k < Np; k++) Oe[f * Np + k] = 0.0;
+    }
+    if (flags) flags[e] = flag;
+  }
+  memcpy(U, out, sizeof(double) * c->ndof);
+  free(out);
+  return 0;
+}
+
+/* modal-decay sensor, new design (in place of DGAnalysis::adaptP / modifyP,
+ * DGPref.cc:24-60, 602-630): energy of the degree-p modes of field 0 against
+ * the energy of all modes decides p-1, p+1 or p; sums in mode order with fused
+ * multiply-adds */
+int or_sensor(or_ctx* c, const double* U, const double* s_hi, const double* s_lo, int pmin, int pmax,
+              int32_t* orders) {
+  for (int e = 0; e < c->E; e++) {
+    int p = c->order[e], Np = np_of(c->dim, p), Npl = np_of(c->dim, p - 1);
+    const double* Ue = U + c->uoff[e];
+    double all = 0, top = 0, high = 0;
+    for (int k = 0; k < Np; k++) {
+      all = fma(Ue[k], Ue[k], all);
+      if (k >= 1) high = fma(Ue[k], Ue[k], high);
+      if (k >= Npl) top = fma(Ue[k], Ue[k], top);
+    }
+    int next = p;
+    if (high > 1e-24 * all) {
+      if (top > s_hi[p - 1] * all)
+        next = p - 1 < pmin ? pmin : p - 1;
+      else if (top < s_lo[p - 1] * all)
+        next = p + 1 > pmax ? pmax : p + 1;
+    }
+    orders[e] = next;
+  }
+  return 0;
+}
+
+void or_set_limiter(or_ctx* c, int on, double tvb_m, double nu) {
+  c->lim_on = on;
+  c->lim_m = tvb_m;
+  c->lim_nu = nu;
+}
+
+/* the limiter after every stage, as TimeIntegrator's integrators call
+ * TimeIntegrator::limit (TimeIntegrator.cc:139-272, 359-429, 850-987) */
+#define LIMIT()                                                          \
+  do {                                                                   \
+    if (c->lim_on) or_limit(c, U, c->lim_m, c->lim_nu, NULL);            \
+  } while (0)
+
 double or_step(or_ctx* c, int scheme, double* U, double t, double dt, int nthreads) {
   long n = c->ndof;
   double* R = (double*)malloc(sizeof(double) * n);
@@ -993,44 +1176,53 @@ double or_step(or_ctx* c, int scheme, double* U, double t, double dt, int nthrea
       acc[i] = R[i] * dtj[i];
       U[i] += 0.5 * R[i] * dtj[i];
     }
+    LIMIT();
     or_residual(c, U, t + dt * 0.5, R, nthreads);
     for (long i = 0; i < n; i++) {
       acc[i] += 2. * dtj[i] * R[i];
       U[i] = s0[i] + 0.5 * dtj[i] * R[i];
     }
+    LIMIT();
     or_residual(c, U, t + dt * 0.5, R, nthreads);
     for (long i = 0; i < n; i++) {
       acc[i] += 2. * dtj[i] * R[i];
       U[i] = s0[i] + dtj[i] * R[i];
     }
+    LIMIT();
     or_residual(c, U, t + dt, R, nthreads);
     for (long i = 0; i < n; i++) {
       acc[i] += dtj[i] * R[i];
       U[i] = s0[i] + acc[i] / 6.0;
       resid += acc[i] * acc[i] / 36.0;
     }
+    LIMIT();
   } else if (scheme == OR_RK_TVD3) {
     /* TimeIntegrator.cc:838-990 */
     or_residual(c, U, t, R, nthreads);
     for (long i = 0; i < n; i++) U[i] += R[i] * dtj[i];
+    LIMIT();
     or_residual(c, U, t + dt, R, nthreads);
     for (long i = 0; i < n; i++) U[i] = 0.25 * (U[i] + R[i] * dtj[i] + 3. * s0[i]);
+    LIMIT();
     or_residual(c, U, t + dt * 0.5, R, nthreads);
     for (long i = 0; i < n; i++) {
       U[i] = (2.0 * (U[i] + R[i] * dtj[i]) + s0[i]) / 3.0;
       double d = U[i] - s0[i];
       resid += d * d;
     }
+    LIMIT();
   } else if (scheme == OR_RK_TVD2) {
     /* TimeIntegrator.cc:279-438 with parallel.cc:78-186 */
     or_residual(c, U, t, R, nthreads);
     for (long i = 0; i < n; i++) U[i] += R[i] * dtj[i];
+    LIMIT();
     or_residual(c, U, t + dt, R, nthreads);
     for (long i = 0; i < n; i++) {
       U[i] = (U[i] + R[i] * dtj[i] + s0[i]) * 0.5;
       double d = U[i] - s0[i];
       resid += d * d;
     }
+    LIMIT();
   } else {
     /* TimeIntegrator.cc:993-1048 */
     or_residual(c, U, t, R, nthreads);
@@ -1040,6 +1232,7 @@ double or_step(or_ctx* c, int scheme, double* U, double t, double dt, int nthrea
       d -= U[i];
       resid += d * d;
     }
+    LIMIT();
   }
   free(R);
   free(s0);
@@ -1047,6 +1240,7 @@ double or_step(or_ctx* c, int scheme, double* U, double t, double dt, int nthrea
   free(dtj);
   return resid;
 }
+#undef LIMIT
 
 /* ------------------------------------------------------------------------- */
 /* time step (DGCell.cc:1155-1262)                                            */

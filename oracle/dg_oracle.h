@@ -20,7 +20,7 @@ extern "C" {
 #endif
 
 enum { OR_LAW_SCALAR = 0, OR_LAW_EULER2D = 1, OR_LAW_MAXWELL3D = 2 };
-enum { OR_FLUX_UPWIND = 0, OR_FLUX_LF = 1, OR_FLUX_ROE = 2, OR_FLUX_CENTRAL = 3 };
+enum { OR_FLUX_UPWIND = 0, OR_FLUX_LF = 1, OR_FLUX_ROE = 2, OR_FLUX_CENTRAL = 3, OR_FLUX_ROE_EF = 4 };
 enum { OR_RK_FE = 0, OR_RK_TVD2 = 1, OR_RK_TVD3 = 2, OR_RK4 = 3 };
 
 typedef struct {
@@ -78,6 +78,15 @@ int or_exact(int field, const double* par, const or_law* law, const double* x, d
 /* per-field L2 error of U against an exact field, rule of degree 2(p+1)
  * (DGError.cc:805-831) */
 int or_l2_error(or_ctx* c, const double* U, int field, const double* par, double t, double* err);
+
+/* limiter (triangles, new design; the reference's DGMomentLimiter is
+ * quad-only): limits U in place, flags[e] bit f set when field f was limited
+ * (flags may be NULL).  or_set_limiter makes or_step limit after every stage. */
+int or_limit(or_ctx* c, double* U, double tvb_m, double nu, uint8_t* flags);
+void or_set_limiter(or_ctx* c, int on, double tvb_m, double nu);
+/* modal-decay sensor: next order per element (s_hi, s_lo indexed by order-1) */
+int or_sensor(or_ctx* c, const double* U, const double* s_hi, const double* s_lo, int pmin, int pmax,
+              int32_t* orders);
 
 /* test hooks: substitute the triangle basis coefficient matrix (n x n, row i =
  * coefficients of phi_i over the Pascal monomials) or the triangle rule of one

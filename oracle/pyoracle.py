@@ -10,7 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "liboracle.so")
 
 LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
-FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL = 0, 1, 2, 3
+FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL, FLUX_ROE_EF = 0, 1, 2, 3, 4
 RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
 
 _p_i32 = ctypes.POINTER(ctypes.c_int32)
@@ -54,6 +54,9 @@ def _load():
     lib.or_project.argtypes = [vp, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64]
     lib.or_exact.argtypes = [ctypes.c_int, _p_f64, ctypes.POINTER(_Law), _p_f64, ctypes.c_double, _p_f64]
     lib.or_l2_error.argtypes = [vp, _p_f64, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64]
+    lib.or_limit.argtypes = [vp, _p_f64, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_uint8)]
+    lib.or_set_limiter.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    lib.or_sensor.argtypes = [vp, _p_f64, _p_f64, _p_f64, ctypes.c_int, ctypes.c_int, _p_i32]
     lib.or_set_tri_basis.argtypes = [ctypes.c_int, _p_f64]
     lib.or_set_tri_rule.argtypes = [ctypes.c_int, ctypes.c_int, _p_f64]
     return lib
@@ -147,6 +150,26 @@ class Oracle:
         if _lib.or_project(self._h, field, _p(par, _p_f64), t, _p(U, _p_f64)) != 0:
             raise ValueError("bad field")
         return U
+
+    def limit(self, U, tvb_m=0.0, nu=1.0):
+        """limited copy of U and the per-element flags (bit f: field f limited)"""
+        U = np.array(U, np.float64, copy=True)
+        flags = np.zeros(self.E, np.uint8)
+        if _lib.or_limit(self._h, _p(U, _p_f64), tvb_m, nu, flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))):
+            raise ValueError("limiter is built for triangles")
+        return U, flags
+
+    def set_limiter(self, on, tvb_m=0.0, nu=1.0):
+        """limit after every stage of step()"""
+        _lib.or_set_limiter(self._h, 1 if on else 0, tvb_m, nu)
+
+    def sensor(self, U, s_hi, s_lo, pmin=1, pmax=4):
+        U = np.ascontiguousarray(U, np.float64)
+        hi = np.ascontiguousarray(s_hi, np.float64)
+        lo = np.ascontiguousarray(s_lo, np.float64)
+        out = np.zeros(self.E, np.int32)
+        _lib.or_sensor(self._h, _p(U, _p_f64), _p(hi, _p_f64), _p(lo, _p_f64), pmin, pmax, _p(out, _p_i32))
+        return out
 
     def l2_error(self, U, field, par=(), t=0.0):
         par = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)

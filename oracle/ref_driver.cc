@@ -7,7 +7,11 @@
 //   ref_driver MESH.msh LAW FLUX ORDER OP STATE_IN OUT [DT NSTEPS]
 //     LAW   scalar | euler      FLUX  upwind | exact | roe | lf
 //     OP    residual | fe | tvd2 | tvd3 | rk4
-//     STATE_IN: one line per cell: 3 vertex ids (any order), then Nf*Np coefficients
+//     STATE_IN: one line per cell: 3 vertex ids (any order), then Nf*Np coefficients;
+//          with ORDER 0 each line carries the cell's order after the ids, and the
+//          driver gives every cell that order through DGCell::adaptOrder
+//          (DGCell.cc:1542-1602), which re-initialises the adjacent faces with the
+//          rule of degree 2 max(pL, pR) + 1 (DGCell.cc:408-423)
 //     OUT: one line per cell in the reference's iteration order: its 3 vertex ids
 //          in the reference's own vertex order, then the residual or new state
 //   scalar velocity (VX VY) and far-field state are read from the environment
@@ -121,20 +125,24 @@ int main(int argc, char** argv) {
       law = new Euler2d(walls, &ff, 1.4);
     nf = 4;
   }
-  const int NP = np_tri(order);
-  DGAnalysis* an = new DGAnalysis(mesh, law, 0, order);
+  const bool mixed = order == 0;
+  DGAnalysis* an = new DGAnalysis(mesh, law, 0, mixed ? 1 : order);
   (void)an;
 
   // read the state keyed by sorted vertex ids
   std::map<std::vector<int>, std::vector<double> > state;
+  std::map<std::vector<int>, int> orders;
   FILE* fi = fopen(argv[6], "r");
   if (!fi) return 3;
   int v[3];
   while (fscanf(fi, "%d %d %d", &v[0], &v[1], &v[2]) == 3) {
     std::vector<int> key(v, v + 3);
     std::sort(key.begin(), key.end());
-    std::vector<double> c(nf * NP);
-    for (int i = 0; i < nf * NP; i++)
+    int p = order;
+    if (mixed && fscanf(fi, "%d", &p) != 1) return 4;
+    orders[key] = p;
+    std::vector<double> c(nf * np_tri(p));
+    for (size_t i = 0; i < c.size(); i++)
       if (fscanf(fi, "%lf", &c[i]) != 1) return 4;
     state[key] = c;
   }
@@ -142,7 +150,6 @@ int main(int argc, char** argv) {
   int DOF = 0;
   for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it) {
     mEntity* m = *it;
-    DGCell* cell = (DGCell*)m->getCell();
     std::vector<int> key;
     for (int k = 0; k < 3; k++) key.push_back(m->get(0, k)->getId());
     std::sort(key.begin(), key.end());
@@ -150,7 +157,16 @@ int main(int argc, char** argv) {
       fprintf(stderr, "cell not in state file\n");
       return 5;
     }
+    if (mixed) ((DGCell*)m->getCell())->adaptOrder(orders[key]);
+  }
+  for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it) {
+    mEntity* m = *it;
+    DGCell* cell = (DGCell*)m->getCell();
+    std::vector<int> key;
+    for (int k = 0; k < 3; k++) key.push_back(m->get(0, k)->getId());
+    std::sort(key.begin(), key.end());
     const std::vector<double>& c = state[key];
+    const int NP = np_tri(orders[key]);
     for (int f = 0; f < nf; f++)
       for (int k = 0; k < NP; k++) cell->theFieldsCoefficients->get(f, k) = c[f * NP + k];
     cell->computeMean();
@@ -189,6 +205,10 @@ int main(int argc, char** argv) {
   for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it) {
     mEntity* m = *it;
     DGCell* cell = (DGCell*)m->getCell();
+    std::vector<int> key;
+    for (int k = 0; k < 3; k++) key.push_back(m->get(0, k)->getId());
+    std::sort(key.begin(), key.end());
+    const int NP = np_tri(orders[key]);
     for (int k = 0; k < 3; k++) fprintf(fo, "%d ", m->get(0, k)->getId());
     for (int f = 0; f < nf; f++)
       for (int k = 0; k < NP; k++)

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../dg-parallel_amd/csrc/dg_adjoint.h"
+#include "../../dg-parallel_amd/csrc/dg_mixed.h"
 #include "../../dg-parallel_amd/csrc/dg_setup.h"
 #include "../../dg-parallel_amd/csrc/dg_stage_tiled.h"
 
@@ -306,5 +307,222 @@ extern "C" int emu_tri_adjoint_rk4(const dg_mesh_desc* mesh, const dg_law_desc* 
     std::swap(lam, mu);
   }
   m.from_soa(lam.data(), grad);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// mixed orders, limiter, sensor: the element functions of dg_mixed.h per element
+
+namespace {
+
+struct MixEmu {
+  TriSetup S;
+  std::vector<double> U, Un, acc, R, A, B;
+  std::vector<uint8_t> flags, orders;
+  double midv[3][15], pinv[2][3];
+  void to_soa(const double* Uc, double* dst) const {
+    for (int i = 0; i < S.E; i++) {
+      const int p = S.order[i], n = S.NF * np_tri(p), Ep = S.bstart[p + 1] - S.bstart[p];
+      for (int j = 0; j < n; j++)
+        dst[S.boff[p] + (long)j * Ep + (i - S.bstart[p])] = Uc[S.coff[S.perm[i]] + j];
+    }
+  }
+  void from_soa(const double* src, double* Uc) const {
+    for (int i = 0; i < S.E; i++) {
+      const int p = S.order[i], n = S.NF * np_tri(p), Ep = S.bstart[p + 1] - S.bstart[p];
+      for (int j = 0; j < n; j++)
+        Uc[S.coff[S.perm[i]] + j] = src[S.boff[p] + (long)j * Ep + (i - S.bstart[p])];
+    }
+  }
+  MixedArgs args() const {
+    MixedArgs m{};
+    m.E = S.E;
+    for (int p = 0; p < 6; p++) {
+      m.bstart[p] = S.bstart[p];
+      m.boff[p] = S.boff[p];
+      m.moff[p] = S.moff[p];
+      for (int q = 0; q < 6; q++) m.ftoff[p][q] = S.ftoff[p][q];
+    }
+    m.tab = S.mtab.data();
+    m.glw = S.glw.data();
+    m.mvtx = S.mvtx.data();
+    m.el.mg = S.mg.data();
+    m.el.detj = S.detj.data();
+    m.el.size = S.size.data();
+    m.el.link = S.link.data();
+    m.law = S.lp;
+    return m;
+  }
+  LimiterArgs lim(double* u, double M, double nu) {
+    LimiterArgs l{};
+    l.E = S.E;
+    for (int p = 0; p < 6; p++) {
+      l.bstart[p] = S.bstart[p];
+      l.boff[p] = S.boff[p];
+    }
+    l.el.mg = S.mg.data();
+    l.el.detj = S.detj.data();
+    l.el.size = S.size.data();
+    l.el.link = S.link.data();
+    l.law = S.lp;
+    l.U = u;
+    l.mvtx = S.mvtx.data();
+    for (int i = 0; i < 3; i++) {
+      for (int k = 0; k < 15; k++) l.midv[i][k] = midv[i][k];
+      for (int j = 0; j < 2; j++) l.pinv[j][i] = pinv[j][i];
+    }
+    l.tvb_m = M;
+    l.nu = nu;
+    l.flags = flags.data();
+    return l;
+  }
+};
+
+int mix_make(MixEmu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int order) {
+  dg_options opt{};
+  opt.order = order;
+  std::string err;
+  if (build_tri_setup(mesh, law, &opt, m.S, err) != DG_OK) return -1;
+  const size_t n = (size_t)m.S.boff[5];
+  m.U.assign(n, 0.0);
+  m.Un.assign(n, 0.0);
+  m.acc.assign(n, 0.0);
+  m.R.assign(n, 0.0);
+  m.A.assign(n, 0.0);
+  m.B.assign(n, 0.0);
+  m.flags.assign(m.S.E, 0);
+  m.orders.assign(m.S.E, 0);
+  // limiter constants, as the library computes them
+  static const double mid[3][2] = {{0.5, 0.0}, {0.5, 0.5}, {0.0, 0.5}};
+  double M[3][2], phi[64], xi[3] = {0, 0, 0};
+  for (int l = 0; l < 3; l++) {
+    xi[0] = mid[l][0];
+    xi[1] = mid[l][1];
+    basis(2, MAXP_TRI, xi, phi, nullptr);
+    for (int k = 0; k < 15; k++) m.midv[l][k] = phi[k];
+    M[l][0] = phi[1];
+    M[l][1] = phi[2];
+  }
+  double A[2][2] = {{0, 0}, {0, 0}};
+  for (int l = 0; l < 3; l++)
+    for (int r = 0; r < 2; r++)
+      for (int q = 0; q < 2; q++) A[r][q] += M[l][r] * M[l][q];
+  const double det = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+  const double Ai[2][2] = {{A[1][1] / det, -A[0][1] / det}, {-A[1][0] / det, A[0][0] / det}};
+  for (int r = 0; r < 2; r++)
+    for (int l = 0; l < 3; l++) m.pinv[r][l] = Ai[r][0] * M[l][0] + Ai[r][1] * M[l][1];
+  return 0;
+}
+
+template <class LAW>
+void mix_stage_all(const MixedArgs& a, double& rsum) {
+  for (int i = 0; i < a.E; i++) {
+    const int p = mixed_order_of(a, i);
+    switch (p) {
+      case 1: mixed_stage_element<1, LAW>(a, i, rsum); break;
+      case 2: mixed_stage_element<2, LAW>(a, i, rsum); break;
+      case 3: mixed_stage_element<3, LAW>(a, i, rsum); break;
+      default: mixed_stage_element<4, LAW>(a, i, rsum); break;
+    }
+  }
+}
+
+void mix_stage(const MixedArgs& a, int law, double& rsum) {
+  if (law == LAW_SCALAR)
+    mix_stage_all<ScalarLawD>(a, rsum);
+  else
+    mix_stage_all<EulerLawD>(a, rsum);
+}
+
+void mix_limit(const LimiterArgs& l, int law) {
+  for (int i = 0; i < l.E; i++) {
+    if (law == LAW_SCALAR)
+      limit_element<ScalarLawD>(l, i);
+    else
+      limit_element<EulerLawD>(l, i);
+  }
+}
+
+}  // namespace
+
+// residual of a mesh with per-element orders (mesh->order) through the mixed kernel
+extern "C" int emu_mixed_residual(const dg_mesh_desc* mesh, const dg_law_desc* law, const double* U, double* R) {
+  MixEmu m;
+  if (mix_make(m, mesh, law, 1)) return -1;
+  m.to_soa(U, m.U.data());
+  MixedArgs a = m.args();
+  a.Uin = m.U.data();
+  a.R = m.R.data();
+  a.scheme = MODE_RESIDUAL;
+  double r = 0;
+  mix_stage(a, m.S.law, r);
+  m.from_soa(m.R.data(), R);
+  return 0;
+}
+
+// nsteps of a scheme with the library's buffer sequence, the limiter after every
+// stage when lim_on; flags: the last limiter pass (caller order)
+extern "C" int emu_mixed_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, int scheme, double dt, int nsteps,
+                               int lim_on, double M, double nu, double* U, double* resid, uint8_t* flags) {
+  MixEmu m;
+  if (mix_make(m, mesh, law, 1)) return -1;
+  m.to_soa(U, m.Un.data());
+  const int ns = scheme == 0 ? 1 : scheme == 1 ? 2 : scheme == 2 ? 3 : 4;
+  double r = 0;
+  for (int st = 0; st < nsteps; st++) {
+    MixedArgs a = m.args();
+    a.Un = m.Un.data();
+    a.acc = m.acc.data();
+    a.dt = dt;
+    a.scheme = scheme;
+    r = 0;
+    for (int s = 0; s < ns; s++) {
+      a.stage = s;
+      const bool first = s == 0, last = s == ns - 1;
+      a.Uin = first ? m.Un.data() : ((s % 2) ? m.A.data() : m.B.data());
+      a.Uout = last ? m.Un.data() : ((s % 2) ? m.B.data() : m.A.data());
+      if (first && last) a.Uout = m.A.data();
+      mix_stage(a, m.S.law, r);
+      if (lim_on) mix_limit(m.lim(a.Uout, M, nu), m.S.law);
+    }
+    if (ns == 1) m.Un = m.A;
+  }
+  if (resid) *resid = r;
+  m.from_soa(m.Un.data(), U);
+  if (flags)
+    for (int i = 0; i < m.S.E; i++) flags[m.S.perm[i]] = m.flags[i];
+  return 0;
+}
+
+extern "C" int emu_limit(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, double M, double nu, double* U,
+                         uint8_t* flags) {
+  MixEmu m;
+  if (mix_make(m, mesh, law, order)) return -1;
+  m.to_soa(U, m.U.data());
+  mix_limit(m.lim(m.U.data(), M, nu), m.S.law);
+  m.from_soa(m.U.data(), U);
+  for (int i = 0; i < m.S.E; i++) flags[m.S.perm[i]] = m.flags[i];
+  return 0;
+}
+
+extern "C" int emu_sensor(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, const double* U,
+                          const double* s_hi, const double* s_lo, int pmin, int pmax, int32_t* orders) {
+  MixEmu m;
+  if (mix_make(m, mesh, law, order)) return -1;
+  m.to_soa(U, m.U.data());
+  SensorArgs a{};
+  a.E = m.S.E;
+  for (int p = 0; p < 6; p++) {
+    a.bstart[p] = m.S.bstart[p];
+    a.boff[p] = m.S.boff[p];
+    a.s_hi[p] = p >= 1 && p <= 4 ? s_hi[p - 1] : 0.0;
+    a.s_lo[p] = p >= 1 && p <= 4 ? s_lo[p - 1] : 0.0;
+  }
+  a.pmin = pmin;
+  a.pmax = pmax;
+  a.U = m.U.data();
+  a.orders = m.orders.data();
+  for (int i = 0; i < a.E; i++) sensor_element(a, i);
+  for (int i = 0; i < m.S.E; i++) orders[m.S.perm[i]] = m.orders[i];
   return 0;
 }

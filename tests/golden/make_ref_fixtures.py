@@ -42,13 +42,21 @@ def main():
             cases.append((law, flux, periodic, p, "residual", 0))
     cases += [("euler", "lf", True, 3, "rk4", 3), ("scalar", "upwind", True, 2, "tvd3", 3),
               ("euler", "roe", False, 2, "tvd2", 3), ("scalar", "upwind", False, 1, "fe", 3)]
+    # mixed orders P1-P4 (p = 0: a seeded per-element order map), bounded meshes
+    cases += [("euler", "roe", False, 0, "residual", 0), ("scalar", "upwind", False, 0, "residual", 0),
+              ("euler", "roe", False, 0, "tvd3", 3)]
     for law, flux, periodic, p, op, nsteps in cases:
         m = rh.square_mesh(4, jitter=0.12, seed=100 + k, periodic=periodic, side_tags=(40000, 7, 40000, 7))
         E = len(m["lat"])
-        U = state(law, p, E, k)
+        if p == 0:
+            orders = np.random.default_rng(500 + k).integers(1, 5, E).astype(np.int32)
+            U = np.concatenate([state(law, int(q), 1, 1000 * k + e) for e, q in enumerate(orders)])
+        else:
+            U = state(law, p, E, k)
         dt = 0.002 if op != "residual" else 0.0
         ff = (0.3, 0, 0, 0) if law == "scalar" else FF
-        R, vorder, resid = rh.run(m, law, flux, p, op, U, dt=dt, nsteps=max(nsteps, 1), vel=(1.0, 0.5), farfield=ff)
+        R, vorder, resid = rh.run(m, law, flux, orders if p == 0 else p, op, U, dt=dt, nsteps=max(nsteps, 1),
+                                  vel=(1.0, 0.5), farfield=ff)
         assert np.array_equal(vorder, m["lat"])
         pre = "c%02d_" % k
         out[pre + "meta"] = np.array([law == "euler", {"upwind": 0, "lf": 1, "roe": 2}[flux], p,
@@ -61,6 +69,8 @@ def main():
         out[pre + "dt"] = np.array([dt])
         out[pre + "farfield"] = np.array(ff)
         out[pre + "resid"] = np.array(resid if resid else [0.0])
+        if p == 0:
+            out[pre + "orders"] = orders
         k += 1
     np.savez_compressed(os.path.join(HERE, "ref_fixtures.npz"), **out)
     print("wrote %d cases" % k)

@@ -85,19 +85,26 @@ def write_gmsh(mesh, path):
 
 
 def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfield=(0, 0, 0, 0)):
-    """Returns (values in the caller's element order, reference vertex order per element, resid list)."""
+    """Returns (values in the caller's element order, reference vertex order per element, resid list).
+    order: one order for every cell, or a per-element array (mixed orders; U is then
+    the concatenation of the per-element blocks of Nf*Np(p_e) coefficients)."""
     E = len(mesh["lat"])
-    U = np.asarray(U).reshape(E, -1)
+    U = np.asarray(U, np.float64).ravel()
+    mixed = not np.isscalar(order)
+    orders = np.asarray(order, np.int64) if mixed else np.full(E, int(order))
+    nf = 1 if law == "scalar" else 4
+    sizes = nf * (orders + 1) * (orders + 2) // 2
+    offs = np.concatenate([[0], np.cumsum(sizes)])
     with tempfile.TemporaryDirectory() as d:
         mp, sp, op_ = os.path.join(d, "m.msh"), os.path.join(d, "s.txt"), os.path.join(d, "o.txt")
         write_gmsh(mesh, mp)
         with open(sp, "w") as f:
             for e in range(E):
-                f.write(" ".join(str(v + 1) for v in mesh["lat"][e]) + " " +
-                        " ".join("%.17e" % x for x in U[e]) + "\n")
+                f.write(" ".join(str(v + 1) for v in mesh["lat"][e]) + (" %d" % orders[e] if mixed else "") + " " +
+                        " ".join("%.17e" % x for x in U[offs[e]:offs[e + 1]]) + "\n")
         env = dict(os.environ, REF_VEL="%r %r" % tuple(vel[:2]), REF_FARFIELD=" ".join("%r" % x for x in farfield))
-        r = subprocess.run([DRIVER, mp, law, flux, str(order), op, sp, op_, "%r" % dt, str(nsteps)], cwd=d,
-                           env=env, capture_output=True, text=True, timeout=600)
+        r = subprocess.run([DRIVER, mp, law, flux, "0" if mixed else str(order), op, sp, op_, "%r" % dt,
+                            str(nsteps)], cwd=d, env=env, capture_output=True, text=True, timeout=600)
         if r.returncode != 0:
             raise RuntimeError("ref_driver failed: %s\n%s" % (r.stdout[-2000:], r.stderr[-2000:]))
         resid = [float(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("resid ")]
@@ -108,6 +115,6 @@ def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfiel
     for row in rows:
         v = [int(x) for x in row[:3]]
         e = key[tuple(sorted(v))]
-        out[e] = [float(x) for x in row[3:]]
+        out[offs[e]:offs[e + 1]] = [float(x) for x in row[3:]]
         order_v[e] = v
-    return out.ravel(), order_v - 1, resid
+    return out, order_v - 1, resid
