@@ -154,6 +154,22 @@ dg_status dg_faces(const dg_ctx* c, int32_t* out4, size_t n) {
   return DG_OK;
 }
 
+// chunks of the persistent mixed-order launch: one per MIX_CHUNK elements of a bucket
+static int mixed_chunks(const TriSetup& S) {
+  int n = 0;
+  for (int p = 1; p <= MAXP_TRI; p++) n += (S.bstart[p + 1] - S.bstart[p] + MIX_CHUNK - 1) / MIX_CHUNK;
+  return n;
+}
+
+// per-workgroup partial sums / minima: one slot per workgroup of the largest launch
+static size_t part_size(const TriSetup& S) {
+  const int E = S.E;
+  int n = std::max(tri_stage_blocks(E), tri_tiled_blocks(E));
+  n = std::max(n, mixed_grid(mixed_chunks(S)));
+  n = std::max(n, (E + MIX_CHUNK - 1) / MIX_CHUNK);
+  return (size_t)n + 1;
+}
+
 static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt) {
   auto bail = [&](dg_status s) { return s; };
   TriSetup S;
@@ -220,7 +236,7 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
       (s = dalloc(c, &c->mg, (size_t)4 * E)) || (s = dalloc(c, &c->detj, E)) || (s = dalloc(c, &c->size, E)) ||
       (s = dalloc(c, &c->link, (size_t)3 * E)) || (s = dalloc(c, &c->tab, h_tab.size())) ||
       (s = dalloc(c, &c->tabf, h_tabf.size())) || (s = dalloc(c, &c->vtx, h_vtx.size())) ||
-      (s = dalloc(c, &c->part, (size_t)std::max(tri_stage_blocks(E), tri_tiled_blocks(E)) + 1)))
+      (s = dalloc(c, &c->part, part_size(S))))
     return bail(s);
   if (c->prec == DG_PREC_FP64) {
     if ((s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd))) return bail(s);
@@ -236,6 +252,7 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
       chunks.push_back(i);
     }
   c->nchunks = (int)chunks.size() / 2;
+  if (c->nchunks != mixed_chunks(S)) return bail(fail(c, DG_ERR_ARG, "chunk count mismatch"));
   if ((s = dalloc(c, &c->mtab, S.mtab.size())) || (s = dalloc(c, &c->glw, S.glw.size())) ||
       (s = dalloc(c, &c->mvtx, S.mvtx.size())) || (s = dalloc(c, &c->chunks, chunks.size() + 2)) ||
       (s = dalloc(c, &c->flags, (size_t)E)) || (s = dalloc(c, &c->sens, (size_t)E)))

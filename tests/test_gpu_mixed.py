@@ -2,7 +2,7 @@
 triangle limiter and the order sensor, through the C ABI, against the oracle.
 
 - mixed-order residual, dt and limited SSP-RK3 / RK4 steps within 1e-12
-  (1e-14 for dt) of the oracle;
+  (1e-13 for dt) of the oracle;
 - limiter flags and sensor orders bit for bit against the oracle on the same
   state; the limited state within 1e-14;
 - re-bucketing by new orders carries the state over (raised orders get zero
@@ -77,7 +77,7 @@ def test_mixed_residual_and_dt(flux, seed):
     U = smooth_state(orders, seed)
     g.set_state(U)
     assert rel(g.residual(), o.residual(U)[0]) < 1e-12
-    assert abs(g.computeDt(0.1) - o.dt(U, 0.1)) <= 1e-14 * o.dt(U, 0.1)
+    assert abs(g.computeDt(0.1) - o.dt(U, 0.1)) <= 1e-13 * o.dt(U, 0.1)  # vertex sums of up to 15 modes
 
 
 def test_mixed_scalar_residual():
