@@ -21,6 +21,7 @@
 #include "dg_fields.h"
 #include "dg_kernels.h"
 #include "dg_mixed.h"
+#include "dg_tet.h"
 #include "dg_setup.h"
 #include "dg_tables.h"
 
@@ -30,6 +31,7 @@ struct dg_ctx {
   std::string err;
   int dim = 2, E = 0, Nf = 0, P = 0, NP = 0;
   int law = 0, prec = 0, use_graph = 0, device = 0;
+  double hmin = 0.0;  // tetrahedra: shortest edge (the Maxwell dt needs no state)
   int kernel = 0;  // 0: tiled LDS-staged stage kernel, 1: one element per work-item
   int adj_nd = 0;  // adjoint kernel: flux tangent directions per pass (0: default)
   LawParams lp{};
@@ -170,8 +172,70 @@ static size_t part_size(const TriSetup& S) {
   return (size_t)n + 1;
 }
 
+// tetrahedra: one order, Maxwell, FP64; one order bucket
+static dg_status ctx_init_tet(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt) {
+  TetSetup S;
+  std::string err;
+  dg_status st = build_tet_setup(m, law, opt, S, err);
+  if (st != DG_OK) return fail(c, st, "%s", err.c_str());
+  c->dim = 3;
+  c->E = S.E;
+  c->law = S.law;
+  c->Nf = S.NF;
+  c->P = S.P;
+  c->NP = S.NP;
+  c->prec = DG_PREC_FP64;
+  c->use_graph = opt->use_graph;
+  c->device = opt->device;
+  c->lp = S.lp;
+  c->mixed = false;
+  for (int p = 0; p < 6; p++) {
+    c->bstart[p] = p <= S.P ? 0 : S.E;
+    c->boff[p] = p <= S.P ? 0 : (long)S.E * S.NF * S.NP;
+  }
+  c->order.assign(S.E, (int8_t)S.P);
+  c->coff.resize(S.E + 1);
+  for (int e = 0; e <= S.E; e++) c->coff[e] = (long)e * S.NF * S.NP;
+  c->ndof = (size_t)S.E * S.NF * S.NP;
+  c->m_vids.assign(m->vids, m->vids + (size_t)S.E * 4);
+  c->m_order.assign(S.E, S.P);
+  c->m_law = *law;
+  c->m_opt = *opt;
+  c->perm = S.perm;
+  c->faces = S.faces;
+  c->xyz = S.xyz;
+  c->hmin = 1e300;
+  for (double h : S.size) c->hmin = std::min(c->hmin, h);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return fail(c, DG_ERR_HIP, "no HIP device");
+  }
+  if (c->device < 0 || c->device >= ndev) return fail(c, DG_ERR_ARG, "device %d out of range", c->device);
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, DG_ERR_HIP, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(c, DG_ERR_HIP, "hipStreamCreate failed");
+  dg_status s;
+  const size_t nd = c->ndof;
+  const int E = c->E;
+  if ((s = dalloc(c, &c->Un, nd)) || (s = dalloc(c, &c->acc, nd)) || (s = dalloc(c, &c->R, nd)) ||
+      (s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd)) || (s = dalloc(c, &c->mg, (size_t)9 * E)) ||
+      (s = dalloc(c, &c->detj, E)) || (s = dalloc(c, &c->size, E)) || (s = dalloc(c, &c->link, (size_t)4 * E)) ||
+      (s = dalloc(c, &c->tab, S.tab.size())) || (s = dalloc(c, &c->part, (size_t)tet_tiles(E) + 1)))
+    return s;
+  if (hipMemcpy(c->mg, S.mg.data(), S.mg.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->detj, S.detj.data(), E * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->size, S.size.data(), E * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->link, S.link.data(), S.link.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->tab, S.tab.data(), S.tab.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(c->Un, 0, nd * 8) != hipSuccess || hipMemset(c->acc, 0, nd * 8) != hipSuccess)
+    return fail(c, DG_ERR_HIP, "table upload failed");
+  return DG_OK;
+}
+
 static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt) {
   auto bail = [&](dg_status s) { return s; };
+  if (m && m->dim == 3) return ctx_init_tet(c, m, law, opt);
   TriSetup S;
   std::string err;
   dg_status st = build_tri_setup(m, law, opt, S, err);
@@ -326,7 +390,7 @@ extern "C++" {
 template <class F>
 static void for_each_coeff(const dg_ctx* c, F visit) {
   for (int i = 0; i < c->E; i++) {
-    const int p = c->order[i], n = c->Nf * np_tri(p), Ep = c->bstart[p + 1] - c->bstart[p];
+    const int p = c->order[i], n = c->Nf * (c->dim == 3 ? np_tet(p) : np_tri(p)), Ep = c->bstart[p + 1] - c->bstart[p];
     const long cb = c->coff[c->perm[i]];
     const long sb = c->boff[p] + (i - c->bstart[p]);
     for (int j = 0; j < n; j++) visit(i, cb + j, sb + (long)j * Ep);
@@ -413,6 +477,26 @@ static LimiterArgs limiter_args(const dg_ctx* c, double* U) {
 // launch), then the limiter on its output when it is on
 static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
   int r;
+  if (c->dim == 3) {
+    TetArgs t{};
+    t.E = c->E;
+    t.P = c->P;
+    t.tab = c->tab;
+    t.mg = c->mg;
+    t.detj = c->detj;
+    t.link = c->link;
+    t.law = c->lp;
+    t.Uin = (const double*)a.Uin;
+    t.Un = a.Un;
+    t.acc = a.acc;
+    t.Uout = (double*)a.Uout;
+    t.R = a.R;
+    t.dt = a.dt;
+    t.scheme = a.scheme;
+    t.stage = a.stage;
+    t.resid_part = a.resid_part;
+    return launch_tet_stage(t, c->stream);
+  }
   if (c->mixed) {
     MixedArgs m = mixed_args(c);
     m.Uin = (const double*)a.Uin;
@@ -435,6 +519,7 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
 }
 
 static int stage_blocks(const dg_ctx* c) {
+  if (c->dim == 3) return tet_tiles(c->E);
   if (c->mixed) return mixed_grid(c->nchunks);
   return c->kernel == 1 ? tri_stage_blocks(c->E) : tri_tiled_blocks(c->E);
 }
@@ -472,6 +557,12 @@ dg_status dg_residual(dg_ctx* c, double t, double* R_out) {
 
 dg_status dg_compute_dt(dg_ctx* c, double cfl, double* dt_out) {
   if (!ready(c) || !dt_out || !(cfl > 0)) return fail(c, DG_ERR_ARG, "dg_compute_dt: bad argument");
+  if (c->dim == 3) {
+    // Maxwell's largest eigenvalue is the light speed 1 everywhere (DGCell.cc:1155-1207
+    // with the TET size: the shortest edge, DGCell.cc:1230-1262)
+    *dt_out = cfl * c->hmin / (2. * c->P + 1.0);
+    return DG_OK;
+  }
   HIPCHECK(c, hipSetDevice(c->device));
   DtArgs a{};
   a.E = c->E;
@@ -639,10 +730,72 @@ dg_status dg_get_state_device(dg_ctx* c, double* dU) {
   return DG_OK;
 }
 
+// tetrahedra: x(xi) = X0 + sum_c xi_c (X_{c+1} - X0)
+static void tet_map(const double* X, const double* xi, double* x) {
+  for (int d = 0; d < 3; d++) x[d] = X[d] + xi[0] * (X[3 + d] - X[d]) + xi[1] * (X[6 + d] - X[d]) + xi[2] * (X[9 + d] - X[d]);
+}
+static double tet_detj(const double* X) {
+  double J[3][3];
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < 3; k++) J[d][k] = X[(k + 1) * 3 + d] - X[d];
+  return std::fabs(J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]));
+}
+
+static dg_status project_tet(dg_ctx* c, int field, const double* par, double t) {
+  const int P = c->P, NP = c->NP, NF = c->Nf;
+  auto rule = tet_rule(2 * P);
+  std::vector<double> phi(rule.size() * NP);
+  for (size_t q = 0; q < rule.size(); q++) basis(3, P, rule[q].x, &phi[q * NP], nullptr);
+  std::vector<double> U(c->ndof, 0.0);
+  for (int e = 0; e < c->E; e++) {
+    const double* X = &c->xyz[(size_t)e * 12];
+    double* Ue = &U[c->coff[e]];
+    for (size_t q = 0; q < rule.size(); q++) {
+      double x[3], val[8];
+      tet_map(X, rule[q].x, x);
+      if (exact_field(field, par, c->lp, x, t, val)) return fail(c, DG_ERR_ARG, "unknown field %d", field);
+      for (int f = 0; f < NF; f++)
+        for (int k = 0; k < NP; k++) Ue[f * NP + k] += val[f] * phi[q * NP + k] * rule[q].w;  // |J| cancels
+    }
+  }
+  return dg_set_state(c, U.data(), U.size());
+}
+
+static dg_status l2_error_tet(dg_ctx* c, int field, const double* par, double t, double* err) {
+  const int P = c->P, NP = c->NP, NF = c->Nf;
+  std::vector<double> U(c->ndof);
+  dg_status s = dg_get_state(c, U.data(), U.size());
+  if (s) return s;
+  auto rule = tet_rule(std::min(2 * (P + 1), 6));
+  std::vector<double> phi(rule.size() * NP);
+  for (size_t q = 0; q < rule.size(); q++) basis(3, P, rule[q].x, &phi[q * NP], nullptr);
+  std::vector<double> acc(NF, 0.0);
+  for (int e = 0; e < c->E; e++) {
+    const double* X = &c->xyz[(size_t)e * 12];
+    const double ad = tet_detj(X);
+    const double* Ue = &U[c->coff[e]];
+    for (size_t q = 0; q < rule.size(); q++) {
+      double x[3], val[8];
+      tet_map(X, rule[q].x, x);
+      if (exact_field(field, par, c->lp, x, t, val)) return fail(c, DG_ERR_ARG, "unknown field %d", field);
+      for (int f = 0; f < NF; f++) {
+        double s2 = 0;
+        for (int k = 0; k < NP; k++) s2 += Ue[f * NP + k] * phi[q * NP + k];
+        const double d = s2 - val[f];
+        acc[f] += d * d * ad * rule[q].w;
+      }
+    }
+  }
+  for (int f = 0; f < NF; f++) err[f] = std::sqrt(acc[f]);
+  return DG_OK;
+}
+
 dg_status dg_project(dg_ctx* c, int field, const double* par, double t) {
   if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
   double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (!par) par = zero;
+  if (c->dim == 3) return project_tet(c, field, par, t);
   const int E = c->E, NF = c->Nf;
   // per order: the rule of degree 2p and the basis at its points (DGCell.cc:1458-1483)
   std::vector<QPoint> rules[MAXP_TRI + 1];
@@ -680,6 +833,7 @@ dg_status dg_l2_error(dg_ctx* c, int field, const double* par, double t, double*
   if (!ready(c) || !err) return fail(c, DG_ERR_ARG, "bad argument");
   double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (!par) par = zero;
+  if (c->dim == 3) return l2_error_tet(c, field, par, t, err);
   const int E = c->E, NF = c->Nf;
   std::vector<double> U(c->ndof);
   dg_status s = dg_get_state(c, U.data(), U.size());
@@ -722,6 +876,7 @@ dg_status dg_l2_error(dg_ctx* c, int field, const double* par, double t, double*
 
 dg_status dg_set_limiter(dg_ctx* c, int enabled, double tvb_m, double nu) {
   if (!ready(c) || !(tvb_m >= 0) || !(nu > 0)) return fail(c, DG_ERR_ARG, "dg_set_limiter: bad argument");
+  if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "built for triangles");
   if (enabled && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the limiter runs in the FP64 form only");
   c->lim_on = enabled ? 1 : 0;
   c->lim_m = tvb_m;
@@ -741,6 +896,7 @@ static dg_status fetch_flags(dg_ctx* c, const uint8_t* dsrc, uint8_t* out) {
 
 dg_status dg_limit(dg_ctx* c, uint8_t* flags_out) {
   if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
+  if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "built for triangles");
   if (c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the limiter runs in the FP64 form only");
   HIPCHECK(c, hipSetDevice(c->device));
   if (launch_limiter(limiter_args(c, c->Un), c->law, c->stream)) return fail(c, DG_ERR_HIP, "limiter launch failed");
@@ -804,6 +960,7 @@ static dg_status rebucket(dg_ctx* c, const std::vector<int32_t>& neword) {
 
 dg_status dg_select_orders(dg_ctx* c, const dg_sensor_opts* o, int apply, int32_t* orders_out) {
   if (!ready(c) || !o) return fail(c, DG_ERR_ARG, "dg_select_orders: bad argument");
+  if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "built for triangles");
   if (o->pmin < 1 || o->pmax > MAXP_TRI || o->pmin > o->pmax) return fail(c, DG_ERR_ARG, "orders must lie in 1..%d", MAXP_TRI);
   if (apply && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "order changes run in the FP64 form only");
   HIPCHECK(c, hipSetDevice(c->device));
@@ -872,7 +1029,7 @@ static AdjArgs adj_args(dg_ctx* c) {
 
 dg_status dg_residual_jvp(dg_ctx* c, const double* U, const double* dU, double* dR) {
   if (!ready(c) || !U || !dU || !dR) return fail(c, DG_ERR_ARG, "dg_residual_jvp: bad argument");
-  if (c->mixed) return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform orders");
+  if (c->mixed || c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform-order triangles");
   HIPCHECK(c, hipSetDevice(c->device));
   dg_status s = adj_buffers(c);
   if (s || (s = upload_soa(c, U, c->lam)) || (s = upload_soa(c, dU, c->za))) return s;
@@ -886,7 +1043,7 @@ dg_status dg_residual_jvp(dg_ctx* c, const double* U, const double* dU, double* 
 
 dg_status dg_residual_vjp(dg_ctx* c, const double* U, const double* z, double* w) {
   if (!ready(c) || !U || !z || !w) return fail(c, DG_ERR_ARG, "dg_residual_vjp: bad argument");
-  if (c->mixed) return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform orders");
+  if (c->mixed || c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform-order triangles");
   HIPCHECK(c, hipSetDevice(c->device));
   dg_status s = adj_buffers(c);
   if (s || (s = upload_soa(c, U, c->lam)) || (s = upload_soa(c, z, c->za))) return s;
@@ -941,7 +1098,8 @@ dg_status dg_adjoint_forward(dg_ctx* c, double dt, int nsteps, int ckpt_every, c
   if (!ready(c) || !(dt > 0) || nsteps <= 0 || ckpt_every <= 0)
     return fail(c, DG_ERR_ARG, "dg_adjoint_forward: bad argument");
   if (c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the FP64 form");
-  if (c->mixed || c->lim_on) return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the unlimited uniform-order residual");
+  if (c->mixed || c->lim_on || c->dim != 2)
+    return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the unlimited uniform-order triangle residual");
   HIPCHECK(c, hipSetDevice(c->device));
   c->adj_ready = false;
   dg_status s = adj_buffers(c);
@@ -1145,6 +1303,31 @@ dg_status dg_mesh_tri_structured(int nx, int ny, double x0, double y0, double lx
           }
       }
     }
+  return DG_OK;
+}
+
+// Kuhn subdivision of an nx x ny x nz lattice of boxes on [0,lx]x[0,ly]x[0,lz]:
+// every box is cut into the 6 tetrahedra that share its main diagonal
+// (v000 -> v000+e_a -> v000+e_a+e_b -> v111 for the permutations (a,b,c)), so the
+// faces of neighbouring boxes match.  Vertex ids are lattice ids.
+dg_status dg_mesh_tet_structured(int nx, int ny, int nz, double lx, double ly, double lz, int32_t* vids,
+                                 double* xyz) {
+  if (nx < 1 || ny < 1 || nz < 1 || !(lx > 0) || !(ly > 0) || !(lz > 0) || !vids || !xyz) return DG_ERR_ARG;
+  const double h[3] = {lx / nx, ly / ny, lz / nz};
+  static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  size_t e = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        for (int t = 0; t < 6; t++, e++) {
+          int c[3] = {i, j, k};
+          for (int v = 0; v < 4; v++) {
+            if (v > 0 && v < 3) c[perm[t][v - 1]] += 1;
+            if (v == 3) c[perm[t][2]] += 1;
+            vids[e * 4 + v] = (int32_t)(((long)c[2] * (ny + 1) + c[1]) * (nx + 1) + c[0]);
+            for (int d = 0; d < 3; d++) xyz[e * 12 + v * 3 + d] = c[d] * h[d];
+          }
+        }
   return DG_OK;
 }
 

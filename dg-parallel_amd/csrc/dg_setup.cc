@@ -252,4 +252,161 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   return DG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// tetrahedra (mMesh.cc:321-378 with faces keyed by their sorted vertex triples;
+// Mapping.cc:442-566 for the affine Jacobian)
+dg_status build_tet_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TetSetup& S,
+                          std::string& err) {
+  if (!m || !law || !opt || !m->vids || !m->xyz || m->n_elems <= 0) return failS(err, DG_ERR_ARG, "null mesh/law/options");
+  if (law->law != DG_LAW_MAXWELL3D) return failS(err, DG_ERR_UNSUPPORTED, "tetrahedra run the Maxwell law only");
+  if (law->flux != DG_FLUX_UPWIND && law->flux != DG_FLUX_CENTRAL)
+    return failS(err, DG_ERR_UNSUPPORTED, "Maxwell flux %d not built", law->flux);
+  if (opt->precision != DG_PREC_FP64) return failS(err, DG_ERR_UNSUPPORTED, "tetrahedra run the FP64 form only");
+  int P = opt->order;
+  if (m->order)
+    for (int e = 0; e < m->n_elems; e++)
+      if (m->order[e] != m->order[0]) return failS(err, DG_ERR_UNSUPPORTED, "mixed orders are built for triangles only");
+  if (m->order) P = m->order[0];
+  if (P < 1 || P > MAXP_TET) return failS(err, DG_ERR_UNSUPPORTED, "tet order %d outside 1..%d", P, MAXP_TET);
+  S.E = m->n_elems;
+  S.P = P;
+  S.NP = np_tet(P);
+  S.NF = 6;
+  S.law = law->law;
+  S.lp.gamma = law->gamma;
+  for (int i = 0; i < 3; i++) S.lp.vel[i] = law->vel[i];
+  for (int i = 0; i < 8; i++) S.lp.farfield[i] = law->farfield[i];
+  S.lp.flux = law->flux;
+  const int E = S.E;
+  S.xyz.assign(m->xyz, m->xyz + (size_t)E * 12);
+
+  struct FK {
+    int32_t k[3];
+    int32_t e, l;
+    bool operator<(const FK& o) const {
+      for (int i = 0; i < 3; i++)
+        if (k[i] != o.k[i]) return k[i] < o.k[i];
+      if (e != o.e) return e < o.e;
+      return l < o.l;
+    }
+  };
+  std::vector<FK> keys((size_t)E * 4);
+  for (int e = 0; e < E; e++)
+    for (int l = 0; l < 4; l++) {
+      FK& k = keys[(size_t)e * 4 + l];
+      for (int i = 0; i < 3; i++) k.k[i] = m->vids[e * 4 + TET_FACE_V[l][i]];
+      std::sort(k.k, k.k + 3);
+      if (k.k[0] == k.k[1] || k.k[1] == k.k[2]) return failS(err, DG_ERR_ARG, "degenerate face in element %d", e);
+      k.e = e;
+      k.l = l;
+    }
+  std::sort(keys.begin(), keys.end());
+  std::vector<int32_t> nbe((size_t)E * 4, -1), nbl((size_t)E * 4, -1), nbo((size_t)E * 4, 0);
+  auto orient = [&](int e, int l, int n, int ln) {
+    // permutation o with vid_n[f(ln, PERM3[o][i])] == vid_e[f(l, i)]
+    for (int o = 0; o < 6; o++) {
+      bool ok = true;
+      for (int i = 0; i < 3; i++)
+        if (m->vids[n * 4 + TET_FACE_V[ln][PERM3[o][i]]] != m->vids[e * 4 + TET_FACE_V[l][i]]) ok = false;
+      if (ok) return o;
+    }
+    return -1;
+  };
+  for (size_t i = 0; i < keys.size();) {
+    size_t j = i + 1;
+    while (j < keys.size() && keys[j].k[0] == keys[i].k[0] && keys[j].k[1] == keys[i].k[1] && keys[j].k[2] == keys[i].k[2]) j++;
+    if (j - i > 2) return failS(err, DG_ERR_ARG, "face shared by %zu elements", j - i);
+    const FK& L = keys[i];
+    if (j - i == 2) {
+      const FK& R = keys[i + 1];
+      nbe[L.e * 4 + L.l] = R.e;
+      nbl[L.e * 4 + L.l] = R.l;
+      nbe[R.e * 4 + R.l] = L.e;
+      nbl[R.e * 4 + R.l] = L.l;
+      int o1 = orient(L.e, L.l, R.e, R.l), o2 = orient(R.e, R.l, L.e, L.l);
+      if (o1 < 0 || o2 < 0) return failS(err, DG_ERR_ARG, "unmatched face vertices");
+      nbo[L.e * 4 + L.l] = o1;
+      nbo[R.e * 4 + R.l] = o2;
+      S.faces.insert(S.faces.end(), {L.e, L.l, R.e, R.l});
+    } else {
+      S.faces.insert(S.faces.end(), {L.e, L.l, -1, -1});
+    }
+    i = j;
+  }
+  // internal order: block-local grouping by link signature (as for triangles)
+  auto sig = [&](int e) {
+    int s = 0;
+    for (int l = 0; l < 4; l++) s = s * 32 + (nbe[e * 4 + l] >= 0 ? (nbo[e * 4 + l] * 4 + nbl[e * 4 + l]) : 31);
+    return s;
+  };
+  S.perm.resize(E);
+  for (int e = 0; e < E; e++) S.perm[e] = e;
+  std::vector<int> sg(E);
+  for (int e = 0; e < E; e++) sg[e] = sig(e);
+  const int BLK = 6 * 64;
+  std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) {
+    if (a / BLK != b / BLK) return a / BLK < b / BLK;
+    return sg[a] < sg[b];
+  });
+  std::vector<int32_t> iperm(E);
+  for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
+  if ((long)E >= (1L << 26)) return failS(err, DG_ERR_ARG, "too many elements for the link encoding");
+
+  S.mg.assign((size_t)9 * E, 0.0);
+  S.detj.assign(E, 0.0);
+  S.size.assign(E, 0.0);
+  S.link.assign((size_t)4 * E, 0);
+  for (int i = 0; i < E; i++) {
+    const int e = S.perm[i];
+    const double* X = m->xyz + (size_t)e * 12;
+    double J[3][3];
+    for (int d = 0; d < 3; d++)
+      for (int c = 0; c < 3; c++) J[d][c] = X[(c + 1) * 3 + d] - X[d];  // dx_d / dxi_c
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return failS(err, DG_ERR_ARG, "degenerate element %d", e);
+    // |J| (J^-1)[r][d] = sign(det) * cofactor(J)[d][r]
+    double cof[3][3];
+    cof[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    cof[0][1] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+    cof[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    cof[1][0] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+    cof[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    cof[1][2] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+    cof[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    cof[2][1] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+    cof[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double sgn = det > 0 ? 1.0 : -1.0;
+    for (int r = 0; r < 3; r++)
+      for (int d = 0; d < 3; d++) S.mg[(size_t)(r * 3 + d) * E + i] = sgn * cof[d][r];
+    S.detj[i] = std::fabs(det);
+    static const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+    double h = 1e300;
+    for (int k = 0; k < 6; k++) {
+      double s2 = 0;
+      for (int d = 0; d < 3; d++) {
+        const double v = X[ed[k][1] * 3 + d] - X[ed[k][0] * 3 + d];
+        s2 += v * v;
+      }
+      h = std::min(h, std::sqrt(s2));
+    }
+    S.size[i] = h;
+    for (int l = 0; l < 4; l++) {
+      int32_t v;
+      if (nbe[e * 4 + l] >= 0)
+        v = (iperm[nbe[e * 4 + l]] << 5) | (nbo[e * 4 + l] << 2) | nbl[e * 4 + l];
+      else
+        v = -BC_PEC;  // every boundary face of a Maxwell mesh is a perfect conductor
+      S.link[(size_t)l * E + i] = v;
+    }
+  }
+  S.tab.assign(TetBlock::size(P), 0.0);
+  try {
+    build_tet_block(P, S.tab.data());
+  } catch (const std::exception& ex) {
+    return failS(err, DG_ERR_ARG, "%s", ex.what());
+  }
+  return DG_OK;
+}
+
 }  // namespace dgp

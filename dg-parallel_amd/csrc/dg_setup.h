@@ -45,4 +45,22 @@ struct TriSetup {
 dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TriSetup& S,
                           std::string& err);
 
+// Tetrahedra (Maxwell, one order): the same roles as TriSetup.  mg holds
+// |J| J^-1 as [9][E] (row-major r*3+d), size the shortest edge (DGCell.cc:1230-1262),
+// link[4][E] = (nb << 5) | (perm << 2) | nb_local_face with perm indexing
+// PERM3 (the neighbour's face vertex PERM3[perm][i] holds the own face vertex i),
+// or -(boundary code).
+struct TetSetup {
+  int E = 0, P = 0, NP = 0, NF = 0, law = 0;
+  LawParams lp{};
+  std::vector<int32_t> perm, faces;
+  std::vector<double> xyz;  // [E][4][3]
+  std::vector<double> mg, detj, size;
+  std::vector<int32_t> link;
+  std::vector<double> tab;
+};
+
+dg_status build_tet_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TetSetup& S,
+                          std::string& err);
+
 }  // namespace dgp

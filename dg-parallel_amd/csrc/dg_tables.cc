@@ -127,4 +127,43 @@ void build_glw(double* out) {
   }
 }
 
+// local face l = vertices {0,1,2}, {0,1,3}, {1,2,3}, {0,2,3} (the oracle's and the
+// mesh generator's convention; face l is opposite vertex 3, 2, 0, 1)
+const int TET_FACE_V[4][3] = {{0, 1, 2}, {0, 1, 3}, {1, 2, 3}, {0, 2, 3}};
+const int PERM3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+void build_tet_block(int P, double* out) {
+  const int NP = np_tet(P), NQF = nqf_tet(P);
+  std::memset(out, 0, sizeof(double) * TetBlock::size(P));
+  static const double V[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double phi[64], dphi[192];
+  // stiffness operators with the rule of degree 2P (integrand degree 2P-1)
+  auto vr = tet_rule(2 * P);
+  for (const auto& q : vr) {
+    basis(3, P, q.x, phi, dphi);
+    for (int r = 0; r < 3; r++)
+      for (int k = 0; k < NP; k++)
+        for (int j = 0; j < NP; j++) out[TetBlock::off_K(P) + (r * NP + k) * NP + j] += q.w * dphi[k * 3 + r] * phi[j];
+  }
+  auto fr = tri_rule(2 * P + 1);
+  if ((int)fr.size() != NQF) throw std::runtime_error("tet face rule size mismatch");
+  for (int q = 0; q < NQF; q++) out[TetBlock::off_wf(P) + q] = 2.0 * fr[q].w;
+  for (int l = 0; l < 4; l++)
+    for (int q = 0; q < NQF; q++) {
+      const double lam[3] = {1.0 - fr[q].x[0] - fr[q].x[1], fr[q].x[0], fr[q].x[1]};
+      double xi[3] = {0, 0, 0};
+      for (int i = 0; i < 3; i++)
+        for (int d = 0; d < 3; d++) xi[d] += lam[i] * V[TET_FACE_V[l][i]][d];
+      basis(3, P, xi, phi, nullptr);
+      for (int k = 0; k < NP; k++) out[TetBlock::off_phif(P) + (l * NQF + q) * NP + k] = phi[k];
+      for (int o = 0; o < 6; o++) {
+        double xn[3] = {0, 0, 0};
+        for (int i = 0; i < 3; i++)
+          for (int d = 0; d < 3; d++) xn[d] += lam[i] * V[TET_FACE_V[l][PERM3[o][i]]][d];
+        basis(3, P, xn, phi, nullptr);
+        for (int k = 0; k < NP; k++) out[TetBlock::off_phin(P) + ((l * 6 + o) * NQF + q) * NP + k] = phi[k];
+      }
+    }
+}
+
 }  // namespace dgp

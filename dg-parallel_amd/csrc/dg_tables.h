@@ -61,6 +61,30 @@ struct TriBlock {
 // Gauss-Legendre weights (x 1/2) for every rule size used by faces
 constexpr int GLW_OFF(int nq) { return (nq - 1) * nq / 2; }
 
+// ---------------------------------------------------------------------------
+// Per-order block of the tetrahedron kernels (Maxwell: linear flux, so the
+// volume term is the exact operator form of the degree-2P quadrature).  With
+// NP = np_tet(P), NQF = nqf_tet(P) (triangle rule of degree 2P+1 on each face):
+//   K    [3][NP][NP]       K_r[k][j] = int d phi_k/d xi_r phi_j over the reference tet
+//   phif [4][NQF][NP]      traces on local face l at face point q; the point is
+//                          sum_i lambda_i(q) v_{f(l,i)} with f(l,.) the face's local
+//                          vertices and lambda = (1-u-v, u, v) of the rule point (u,v)
+//   wf   [NQF]             face weights, summing to 1 (physical area x wf)
+//   phin [4][6][NQF][NP]   the neighbour's traces at the same points: its local
+//                          face l' whose vertex f(l',perm_o(i)) holds the own
+//                          face's vertex i, for the six permutations perm_o
+struct TetBlock {
+  static constexpr int off_K(int) { return 0; }
+  static constexpr int off_phif(int P) { return 3 * np_tet(P) * np_tet(P); }
+  static constexpr int off_wf(int P) { return off_phif(P) + 4 * nqf_tet(P) * np_tet(P); }
+  static constexpr int off_phin(int P) { return off_wf(P) + nqf_tet(P); }
+  static constexpr int size(int P) { return off_phin(P) + 4 * 6 * nqf_tet(P) * np_tet(P); }
+};
+// local faces of a tet (vertex triples) and the six permutations of a face's vertices
+extern const int TET_FACE_V[4][3];
+extern const int PERM3[6][3];
+void build_tet_block(int P, double* out);
+
 // fill one block (doubles); out must hold TriBlock::size(P)
 void build_tri_block(int P, double* out);
 // Gauss-Legendre half-weights for nq = 1..MAXP_TRI+1, concatenated

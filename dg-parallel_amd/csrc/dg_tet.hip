@@ -1,0 +1,59 @@
+// HIP kernel wrapper of the tetrahedron stage (phases in dg_tet.h).
+#include <hip/hip_runtime.h>
+
+#include "dg_tet.h"
+
+namespace dgp {
+
+__device__ __forceinline__ int xcd_tile_tet(int b, int nb) {
+  int full = (nb / 8) * 8;
+  if (b >= full) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
+template <int P>
+__global__ __launch_bounds__(TILE_THREADS) void tet_stage_tiled(TetArgs a) {
+  using K = TiledTet<P>;
+  __shared__ typename K::Smem sm;
+  typename K::Regs g;
+  const int tile = xcd_tile_tet(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  K::phaseA(a, sm, tid, tile);
+  __syncthreads();
+  K::phaseB(a, sm, tid, g);
+#pragma unroll 1
+  for (int l = 0; l < 4; l++) {
+    __syncthreads();
+    K::phaseC1(a, sm, tid, tile, l);
+    __syncthreads();
+    K::phaseC2(a, sm, tid, l);
+    __syncthreads();
+    K::phaseLift(a, sm, tid, l, g);
+  }
+  K::phaseD(a, tid, tile, g);
+  if (a.resid_part) {
+    __syncthreads();
+    sm.red[tid] = g.rsum;
+    __syncthreads();
+    for (int s = TILE_THREADS / 2; s > 0; s >>= 1) {
+      if (tid < s) sm.red[tid] += sm.red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) a.resid_part[tile] = sm.red[0];
+  }
+}
+
+int tet_tiles(int E) { return (E + TILE_ELEMS - 1) / TILE_ELEMS; }
+
+int launch_tet_stage(const TetArgs& a, hipStream_t s) {
+  dim3 grid(tet_tiles(a.E)), block(TILE_THREADS);
+  switch (a.P) {
+    case 1: hipLaunchKernelGGL(tet_stage_tiled<1>, grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(tet_stage_tiled<2>, grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(tet_stage_tiled<3>, grid, block, 0, s, a); break;
+    default: return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace dgp

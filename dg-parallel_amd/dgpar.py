@@ -160,6 +160,26 @@ def mesh_tri_structured(nx, ny, x0=0.0, y0=0.0, lx=1.0, ly=1.0, periodic_x=True,
     return Mesh(2, vids, xyz, bc)
 
 
+_mesh_tet = _sig("dg_mesh_tet_structured", ctypes.c_int,
+                 [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                  _p_i32, _p_f64])
+
+
+def mesh_tet_structured(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0):
+    """Kuhn subdivision: 6 tetrahedra per box (dg_mesh_tet_structured)."""
+    E = 6 * nx * ny * nz
+    vids = np.zeros((E, 4), np.int32)
+    xyz = np.zeros((E, 4, 3), np.float64)
+    st = _mesh_tet(nx, ny, nz, lx, ly, lz, _ptr(vids, _p_i32), _ptr(xyz, _p_f64))
+    if st != DG_OK:
+        raise DGError(st, "dg_mesh_tet_structured rejected its arguments")
+    return Mesh(3, vids, xyz)
+
+
+def MaxwellLaw(flux=FLUX_UPWIND):
+    return Law(LAW_MAXWELL3D, flux)
+
+
 class Law:
     """Conservation-law selection (dg_law_desc), mirroring the ConservationLaw subclasses."""
 

@@ -215,6 +215,13 @@ dg_status dg_mesh_tri_structured(int nx, int ny, double x0, double y0, double lx
                                  int periodic_x, int periodic_y, double jitter, uint64_t seed,
                                  const int32_t side_tags[4], int32_t* vids, double* xyz, int32_t* bc);
 
+/* Kuhn subdivision of an nx x ny x nz lattice of boxes on [0,lx]x[0,ly]x[0,lz]
+ * into 6 tetrahedra per box sharing its main diagonal (faces match across boxes).
+ * vids[E][4], xyz[E][4][3], E = 6 nx ny nz.  With the Maxwell law every face
+ * without a neighbour is a perfect electric conductor. */
+dg_status dg_mesh_tet_structured(int nx, int ny, int nz, double lx, double ly, double lz, int32_t* vids,
+                                 double* xyz);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@
 
 #include "../../dg-parallel_amd/csrc/dg_adjoint.h"
 #include "../../dg-parallel_amd/csrc/dg_mixed.h"
+#include "../../dg-parallel_amd/csrc/dg_tet.h"
 #include "../../dg-parallel_amd/csrc/dg_setup.h"
 #include "../../dg-parallel_amd/csrc/dg_stage_tiled.h"
 
@@ -524,5 +525,130 @@ extern "C" int emu_sensor(const dg_mesh_desc* mesh, const dg_law_desc* law, int 
   a.orders = m.orders.data();
   for (int i = 0; i < a.E; i++) sensor_element(a, i);
   for (int i = 0; i < m.S.E; i++) orders[m.S.perm[i]] = m.orders[i];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// tetrahedra (Maxwell): the tiled tet kernel's phases
+
+namespace {
+
+template <int P>
+void tet_tiles_run(const TetArgs& a, double* rsum) {
+  using K = TiledTet<P>;
+  std::unique_ptr<typename K::Smem> sm(new typename K::Smem());
+  std::vector<typename K::Regs> g(TILE_THREADS);
+  const int ntiles = (a.E + TILE_ELEMS - 1) / TILE_ELEMS;
+  for (int tile = 0; tile < ntiles; tile++) {
+    std::memset(sm.get(), 0, sizeof(typename K::Smem));
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, *sm, t, tile);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, *sm, t, g[t]);
+    for (int l = 0; l < 4; l++) {
+      for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, *sm, t, tile, l);
+      for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t, l);
+      for (int t = 0; t < TILE_THREADS; t++) K::phaseLift(a, *sm, t, l, g[t]);
+    }
+    for (int t = 0; t < TILE_THREADS; t++) {
+      K::phaseD(a, t, tile, g[t]);
+      *rsum += g[t].rsum;
+    }
+  }
+}
+
+int tet_run(const TetArgs& a, double* rsum) {
+  switch (a.P) {
+    case 1: tet_tiles_run<1>(a, rsum); return 0;
+    case 2: tet_tiles_run<2>(a, rsum); return 0;
+    case 3: tet_tiles_run<3>(a, rsum); return 0;
+  }
+  return -1;
+}
+
+struct TetEmu {
+  TetSetup S;
+  std::vector<double> Un, acc, R, A, B;
+  void to_soa(const double* U, double* dst) const {
+    const int BS = S.NF * S.NP;
+    for (int i = 0; i < S.E; i++)
+      for (int j = 0; j < BS; j++) dst[(size_t)j * S.E + i] = U[(size_t)S.perm[i] * BS + j];
+  }
+  void from_soa(const double* src, double* U) const {
+    const int BS = S.NF * S.NP;
+    for (int i = 0; i < S.E; i++)
+      for (int j = 0; j < BS; j++) U[(size_t)S.perm[i] * BS + j] = src[(size_t)j * S.E + i];
+  }
+  TetArgs args() const {
+    TetArgs a{};
+    a.E = S.E;
+    a.P = S.P;
+    a.tab = S.tab.data();
+    a.mg = S.mg.data();
+    a.detj = S.detj.data();
+    a.link = S.link.data();
+    a.law = S.lp;
+    return a;
+  }
+};
+
+int tet_make(TetEmu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int order) {
+  dg_options opt{};
+  opt.order = order;
+  opt.precision = DG_PREC_FP64;
+  std::string err;
+  if (build_tet_setup(mesh, law, &opt, m.S, err) != DG_OK) return -1;
+  const size_t n = (size_t)m.S.E * m.S.NF * m.S.NP;
+  m.Un.assign(n, 0.0);
+  m.acc.assign(n, 0.0);
+  m.R.assign(n, 0.0);
+  m.A.assign(n, 0.0);
+  m.B.assign(n, 0.0);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int emu_tet_residual(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, const double* U,
+                                double* R) {
+  TetEmu m;
+  if (tet_make(m, mesh, law, order)) return -1;
+  m.to_soa(U, m.Un.data());
+  TetArgs a = m.args();
+  a.Uin = m.Un.data();
+  a.R = m.R.data();
+  a.scheme = MODE_RESIDUAL;
+  double r = 0;
+  if (tet_run(a, &r)) return -2;
+  m.from_soa(m.R.data(), R);
+  return 0;
+}
+
+extern "C" int emu_tet_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int scheme, double dt,
+                             int nsteps, double* U, double* resid) {
+  TetEmu m;
+  if (tet_make(m, mesh, law, order)) return -1;
+  m.to_soa(U, m.Un.data());
+  const int ns = scheme == 0 ? 1 : scheme == 1 ? 2 : scheme == 2 ? 3 : 4;
+  double r = 0;
+  for (int st = 0; st < nsteps; st++) {
+    TetArgs a = m.args();
+    a.Un = m.Un.data();
+    a.acc = m.acc.data();
+    a.dt = dt;
+    a.scheme = scheme;
+    r = 0;
+    for (int s = 0; s < ns; s++) {
+      a.stage = s;
+      const bool first = s == 0, last = s == ns - 1;
+      a.Uin = first ? m.Un.data() : ((s % 2) ? m.A.data() : m.B.data());
+      a.Uout = last ? m.Un.data() : ((s % 2) ? m.B.data() : m.A.data());
+      if (first && last) a.Uout = m.A.data();
+      double rs = 0;
+      if (tet_run(a, &rs)) return -2;
+      if (last) r = rs;
+    }
+    if (ns == 1) m.Un = m.A;
+  }
+  if (resid) *resid = r;
+  m.from_soa(m.Un.data(), U);
   return 0;
 }
