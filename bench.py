@@ -161,6 +161,30 @@ def run_mixed(dgpar, steps, sense_every=20):
     return out
 
 
+def run_maxwell(dgpar, steps, n=48):
+    """Maxwell on tetrahedra: 48^3 boxes x 6 Kuhn tetrahedra on the unit cube, PEC
+    walls, P3, upwind flux, RK4 at CFL 0.1, the (1,1,1) cavity mode; the L2
+    error against the exact mode after the timed steps."""
+    mesh = dgpar.mesh_tet_structured(n, n, n)
+    g = dgpar.DGAnalysis(mesh, dgpar.MaxwellLaw(dgpar.FLUX_UPWIND), 3, precision=dgpar.PREC_FP64, use_graph=True)
+    g.project(dgpar.FIELD_CAVITY)
+    dt = g.computeDt(CFL)
+    g.time_advance(dgpar.RK4, dt, 2)
+    g.project(dgpar.FIELD_CAVITY)
+    dgpar.device_synchronize()
+    ms = g.time_advance(dgpar.RK4, dt, steps)
+    err = g.l2_error(dgpar.FIELD_CAVITY, (), steps * dt)
+    ndof = g.ndof
+    g.close()
+    b = 34.0 + (9 * 8 + 8 + 4 * 4) / (6 * 20.0)
+    ach = b * ndof / (ms / (4 * steps) / 1e3) / 1e9
+    return {"workload": "maxwell3d PEC cavity (1,1,1), %d^3 x 6 Kuhn tets, P3, upwind, RK4 CFL0.1" % n,
+            "elements": 6 * n ** 3, "dof": ndof, "steps": steps, "ms_per_step": ms / steps,
+            "value": ndof * 4 * steps / (ms / 1e3), "unit": "DOF*stage/s",
+            "achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS, "bytes_per_dof_stage": b,
+            "l2_error_E_H": [float(x) for x in err]}
+
+
 def cpu_baseline(mesh, steps=5):
     from oracle import pyoracle as po
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
@@ -184,6 +208,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel", type=int, default=0, help="stage kernel: 0 tiled (default), 1 one element per lane")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="RK4 steps of the adjoint leg (0: skip it)")
+    ap.add_argument("--maxwell-steps", type=int, default=200,
+                    help="RK4 steps of the Maxwell tetrahedra leg (0: skip it)")
     ap.add_argument("--mixed-steps", type=int, default=100,
                     help="SSP-RK3 steps of the mixed-order limited shock-tube leg (0: skip it)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -248,6 +274,8 @@ def main():
         out["adjoint"] = run_adjoint(dgpar, args.adjoint_steps, 20)
     if args.mixed_steps > 0:
         out["mixed_order_limited"] = run_mixed(dgpar, args.mixed_steps)
+    if args.maxwell_steps > 0:
+        out["maxwell3d_tets"] = run_maxwell(dgpar, args.maxwell_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh)
     print(json.dumps(out))

@@ -470,6 +470,11 @@ static LimiterArgs limiter_args(const dg_ctx* c, double* U) {
   l.tvb_m = c->lim_m;
   l.nu = c->lim_nu;
   l.flags = c->flags;
+  l.tab = c->mtab;
+  for (int p = 0; p < 6; p++) {
+    l.moff[p] = c->moff[p];
+    for (int q = 0; q < 6; q++) l.ftoff[p][q] = c->ftoff[p][q];
+  }
   return l;
 }
 

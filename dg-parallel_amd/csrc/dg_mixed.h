@@ -204,7 +204,8 @@ DG_HD void mixed_stage_element(const MixedArgs& a, int i, double& rsum) {
 // ---------------------------------------------------------------------------
 // Triangle TVB limiter (new design), per element, Jacobi semantics: it reads the
 // means (mode 0, never modified) of its neighbours and rewrites only its own
-// modes 1.. in place, so no ordering between elements matters.
+// modes 1.. in place, so no ordering between elements matters.  Euler elements
+// then pass the positivity scaling below (flag bit 4).
 //
 // For field f, with the element mean m = U_0 phi_0 and the face-neighbour means
 // m_l (boundary faces: the law's ghost state of the mean), at the midpoint c_l
@@ -231,12 +232,109 @@ struct LimiterArgs {
   double pinv[2][3];  // linear coefficients from midpoint deviations
   double tvb_m, nu;
   uint8_t* flags;     // internal order
+  // positivity check points (Euler): the blocks of orders 1..4 (volume points,
+  // edge points of every face rule an element of that order meets)
+  const double* tab;
+  int moff[6];
+  int ftoff[6][6];
 };
 
 DG_HD double lim_minmod(double a, double b) {
   if (a > 0.0 && b > 0.0) return a < b ? a : b;
   if (a < 0.0 && b < 0.0) return a > b ? a : b;
   return 0.0;
+}
+
+// Positivity of density and pressure (Zhang & Shu 2010, J. Comput. Phys. 229,
+// 8918-8934), new in this build (the reference has none): after the TVB step,
+// the modes above the mean of an Euler element are scaled towards the mean so
+// that density and then pressure stay >= eps = min(1e-13, mean density, mean
+// pressure) at every point the scheme evaluates: the volume points of its order,
+// the edge points of every face rule it meets and its vertices.  Returns true
+// when it scaled.
+DG_HD double pos_pressure(double gm1, const double* w) {
+  return gm1 * (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
+}
+
+DG_HD bool positivity_element(const LimiterArgs& a, const MixedArgs& ix, int i, int p, int np) {
+  const double gm1 = a.law.gamma - 1.0;
+  const double phi0 = a.mvtx[p * 60];
+  double mean[4];
+  for (int f = 0; f < 4; f++) mean[f] = a.U[mixed_index(ix, p, i, f * np)] * phi0;
+  const double pm = pos_pressure(gm1, mean);
+  double eps = 1e-13;
+  if (mean[0] < eps) eps = mean[0];
+  if (pm < eps) eps = pm;
+  if (!(eps > 0.0)) return false;  // a non-physical mean: nothing to scale towards
+  // the check points: (table row pointer, row stride) per point set
+  const double* tb = a.tab + a.moff[p];
+  const int nqv = p == 1 ? 3 : p == 2 ? 6 : p == 3 ? 12 : 16;
+  double th1 = 1.0, th2 = 1.0;
+  for (int pass = 0; pass < 2; pass++) {
+    // pass 0: density minimum; pass 1: pressure with the density already scaled
+    const double s0 = pass == 0 ? 1.0 : th1;
+    int npts_total = nqv + 3;
+    for (int nq = p + 1; nq <= MAXP_TRI + 1; nq++) npts_total += 3 * nq;
+    for (int pt = 0; pt < npts_total; pt++) {
+      const double* row;
+      if (pt < nqv) {
+        row = tb + TriBlock::off_phi(0) + pt * np;
+      } else if (pt < nqv + 3) {
+        row = a.mvtx + p * 60 + (1 + pt - nqv) * 15;
+      } else {
+        int r = pt - nqv - 3, nq = p + 1;
+        while (r >= 3 * nq) {
+          r -= 3 * nq;
+          nq++;
+        }
+        row = a.tab + a.ftoff[p][nq] + r * np;
+      }
+      double w[4];
+      for (int f = 0; f < 4; f++) {
+        double acc = 0.0;
+        for (int k = 1; k < np; k++) acc += row[k] * a.U[mixed_index(ix, p, i, f * np + k)];
+        w[f] = mean[f] + (f == 0 ? s0 : 1.0) * acc;
+      }
+      if (pass == 0) {
+        if (w[0] < eps) {
+          const double t = (mean[0] - eps) / (mean[0] - w[0]);
+          if (t < th1) th1 = t;
+        }
+      } else {
+        if (pos_pressure(gm1, w) < eps) {
+          // p(mean + s (w - mean)) = eps: rho(s) (E(s) - eps/(g-1)) - |m(s)|^2 / 2 = 0
+          const double dr = w[0] - mean[0], du = w[1] - mean[1], dv = w[2] - mean[2], de = w[3] - mean[3];
+          const double e0 = mean[3] - eps / gm1;
+          const double qa = dr * de - 0.5 * (du * du + dv * dv);
+          const double qb = dr * e0 + mean[0] * de - (mean[1] * du + mean[2] * dv);
+          const double qc = mean[0] * e0 - 0.5 * (mean[1] * mean[1] + mean[2] * mean[2]);
+          double t;
+          if (qa == 0.0) {
+            t = -qc / qb;
+          } else {
+            const double disc = qb * qb - 4.0 * qa * qc;
+            const double sq = dg_sqrt(disc > 0.0 ? disc : 0.0);
+            // the root in (0, 1]: qc > 0 at s = 0, f(1) < 0
+            const double r1 = (-qb - sq) / (2.0 * qa), r2 = (-qb + sq) / (2.0 * qa);
+            t = 1.0;
+            if (r1 > 0.0 && r1 < t) t = r1;
+            if (r2 > 0.0 && r2 < t) t = r2;
+          }
+          if (t < 0.0) t = 0.0;
+          if (t < th2) th2 = t;
+        }
+      }
+    }
+  }
+  if (th1 >= 1.0 && th2 >= 1.0) return false;
+  for (int f = 0; f < 4; f++) {
+    const double sc = (f == 0 ? th1 : 1.0) * th2;
+    for (int k = 1; k < np; k++) {
+      const long idx = mixed_index(ix, p, i, f * np + k);
+      a.U[idx] = sc * a.U[idx];
+    }
+  }
+  return true;
 }
 
 template <class LAW>
@@ -309,6 +407,7 @@ DG_HD void limit_element(const LimiterArgs& a, int i) {
     a.U[mixed_index(ix, p, i, f * np + 2)] = n2;
     for (int k = 3; k < np; k++) a.U[mixed_index(ix, p, i, f * np + k)] = 0.0;
   }
+  if (LAW::NF == 4 && positivity_element(a, ix, i, p, np)) flag |= 16;
   a.flags[i] = flag;
 }
 

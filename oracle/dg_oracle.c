@@ -1022,6 +1022,99 @@ static double tri_size(const or_ctx* c, int e) {
   return sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
 }
 
+/* Positivity of density and pressure for Euler (Zhang & Shu 2010, J. Comput.
+ * Phys. 229, 8918-8934; no counterpart in the reference): the modes above the
+ * mean are scaled towards the mean so that density, then pressure, stay >= eps =
+ * min(1e-13, mean density, mean pressure) at the element's volume points (rule of
+ * degree 2p), its vertices and the Gauss-Legendre points of every edge rule of
+ * p+1..5 points.  Returns 1 when it scaled. */
+static double pos_p(double gm1, const double* w) { return gm1 * (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]); }
+
+static int positivity(or_ctx* c, double* Ue, int p) {
+  int Np = np_of(2, p);
+  double gm1 = c->law.gamma - 1.0, phi0, phi[MAXNP], z[3] = {0, 0, 0};
+  basis_eval(2, 0, z, &phi0, NULL);
+  double mean[4];
+  for (int f = 0; f < 4; f++) mean[f] = Ue[f * Np] * phi0;
+  double pm = pos_p(gm1, mean), eps = 1e-13;
+  if (mean[0] < eps) eps = mean[0];
+  if (pm < eps) eps = pm;
+  if (!(eps > 0.0)) return 0;
+  /* the check points */
+  double pts[128][3];
+  int npt = 0;
+  {
+    double vq[MAXQ][4];
+    int n = vol_rule(2, 2 * p, vq);
+    for (int q = 0; q < n; q++) {
+      pts[npt][0] = vq[q][0];
+      pts[npt][1] = vq[q][1];
+      pts[npt][2] = 0;
+      npt++;
+    }
+    static const double V[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+    for (int v = 0; v < 3; v++) {
+      pts[npt][0] = V[v][0];
+      pts[npt][1] = V[v][1];
+      pts[npt][2] = 0;
+      npt++;
+    }
+    for (int nq = p + 1; nq <= 5; nq++)
+      for (int l = 0; l < 3; l++)
+        for (int q = 0; q < nq; q++) {
+          double sq = dg_gl_pts[dg_gl_off[nq - 1] + q][0];
+          double a = 0.5 * (1.0 + sq), b = 0.5 * (1.0 - sq);
+          pts[npt][0] = a * V[l][0] + b * V[(l + 1) % 3][0];
+          pts[npt][1] = a * V[l][1] + b * V[(l + 1) % 3][1];
+          pts[npt][2] = 0;
+          npt++;
+        }
+  }
+  double th1 = 1.0, th2 = 1.0;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int q = 0; q < npt; q++) {
+      basis_eval(2, p, pts[q], phi, NULL);
+      double w[4];
+      for (int f = 0; f < 4; f++) {
+        double acc = 0;
+        for (int k = 1; k < Np; k++) acc += phi[k] * Ue[f * Np + k];
+        w[f] = mean[f] + (pass == 1 && f == 0 ? th1 : 1.0) * acc;
+      }
+      if (pass == 0) {
+        if (w[0] < eps) {
+          double t = (mean[0] - eps) / (mean[0] - w[0]);
+          if (t < th1) th1 = t;
+        }
+      } else if (pos_p(gm1, w) < eps) {
+        double dr = w[0] - mean[0], du = w[1] - mean[1], dv = w[2] - mean[2], de = w[3] - mean[3];
+        double e0 = mean[3] - eps / gm1;
+        double qa = dr * de - 0.5 * (du * du + dv * dv);
+        double qb = dr * e0 + mean[0] * de - (mean[1] * du + mean[2] * dv);
+        double qc = mean[0] * e0 - 0.5 * (mean[1] * mean[1] + mean[2] * mean[2]);
+        double t;
+        if (qa == 0.0) {
+          t = -qc / qb;
+        } else {
+          double disc = qb * qb - 4.0 * qa * qc;
+          double sqd = sqrt(disc > 0.0 ? disc : 0.0);
+          double r1 = (-qb - sqd) / (2.0 * qa), r2 = (-qb + sqd) / (2.0 * qa);
+          t = 1.0;
+          if (r1 > 0.0 && r1 < t) t = r1;
+          if (r2 > 0.0 && r2 < t) t = r2;
+        }
+        if (t < 0.0) t = 0.0;
+        if (t < th2) th2 = t;
+      }
+    }
+  }
+  if (th1 >= 1.0 && th2 >= 1.0) return 0;
+  for (int f = 0; f < 4; f++) {
+    double sc = (f == 0 ? th1 : 1.0) * th2;
+    for (int k = 1; k < Np; k++) Ue[f * Np + k] = sc * Ue[f * Np + k];
+  }
+  return 1;
+}
+
 /* TVB limiter on triangles, new design (the reference's DGMomentLimiter,
  * DGLimiter.cc:963-1133, indexes quadrilateral tensor modes; its driver
  * TimeIntegrator::limit, TimeIntegrator.cc:1414-1434, limits every cell from
@@ -1033,7 +1126,8 @@ static double tri_size(const or_ctx* c, int e) {
  * 1998, sec. 5.3, for P^k); the linear part's deviations d_l = U1 phi1(c_l) +
  * U2 phi2(c_l) are then limited to minmod(d_l, nu D_l); a troubled field
  * keeps its mean, gets the linear part of the limited deviations (rescaled to
- * sum to zero, Cockburn & Shu 1998, sec. 5.2) and loses its modes of degree >= 2. */
+ * sum to zero, Cockburn & Shu 1998, sec. 5.2) and loses its modes of degree >= 2.
+ * Euler elements then pass the positivity scaling (flag bit 4). */
 int or_limit(or_ctx* c, double* U, double tvb_m, double nu, uint8_t* flags) {
   if (c->dim != 2) return -1;
   int Nf = nfields_of(&c->law);
@@ -1112,6 +1206,7 @@ int or_limit(or_ctx* c, double* U, double tvb_m, double nu, uint8_t* flags) {
       Oe[f * Np + 2] = fma(pinv[1][2], dl[2], fma(pinv[1][1], dl[1], pinv[1][0] * dl[0]));
       for (int k = 3; k < Np; k++) Oe[f * Np + k] = 0.0;
     }
+    if (c->law.law == OR_LAW_EULER2D && positivity(c, out + c->uoff[e], p)) flag |= 16;
     if (flags) flags[e] = flag;
   }
   memcpy(U, out, sizeof(double) * c->ndof);

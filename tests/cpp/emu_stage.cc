@@ -375,6 +375,11 @@ struct MixEmu {
     l.tvb_m = M;
     l.nu = nu;
     l.flags = flags.data();
+    l.tab = S.mtab.data();
+    for (int p = 0; p < 6; p++) {
+      l.moff[p] = S.moff[p];
+      for (int q = 0; q < 6; q++) l.ftoff[p][q] = S.ftoff[p][q];
+    }
     return l;
   }
 };

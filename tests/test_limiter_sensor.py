@@ -240,3 +240,33 @@ def test_emulated_limited_mixed_steps(seed, scheme):
                                   Ue.ctypes.data_as(PD), ctypes.byref(r), fe.ctypes.data_as(PU8)) == 0
     assert np.abs(Ue - Uo).max() <= 1e-12 * np.abs(Uo).max()
     assert abs(r.value - ro) <= 1e-10 * abs(ro)
+
+
+def test_positivity_scaling_keeps_means_and_point_values_positive():
+    """Zhang-Shu scaling: after the limiter, density and pressure are positive at
+    the volume points, the vertices and the edge points of every face rule, and
+    the means are untouched"""
+    vids, xyz, bc, orders, o, U = sod_case(21, p=4)
+    rng = np.random.default_rng(2)
+    B = blocks(U, orders)
+    for b in B:  # overshooting high modes everywhere
+        b[:, 1:] += 0.3 * rng.standard_normal(b[:, 1:].shape) * np.abs(b[:, :1])
+    U = np.concatenate([b.ravel() for b in B])
+    Ul, fl = o.limit(U, 0.0, 1.0)
+    assert (fl & 16).any()
+    B1 = blocks(Ul, orders)
+    for b0, b1 in zip(blocks(U, orders), B1):
+        assert np.array_equal(b0[:, 0], b1[:, 0])
+    gl = {n: np.polynomial.legendre.leggauss(n)[0] for n in range(2, 6)}
+    V = np.array([[0, 0], [1, 0], [0, 1]])
+    for e, b in enumerate(B1):
+        p = orders[e]
+        pts = [V[0], V[1], V[2]]
+        for n in range(p + 1, 6):
+            for l in range(3):
+                for s in gl[n]:
+                    pts.append(0.5 * (1 + s) * V[l] + 0.5 * (1 - s) * V[(l + 1) % 3])
+        for x in pts:
+            w = b @ po.basis(2, p, [x[0], x[1], 0.0])[0]
+            assert w[0] > 0
+            assert 0.4 * (w[3] - 0.5 * (w[1] ** 2 + w[2] ** 2) / w[0]) > -1e-12
