@@ -121,6 +121,92 @@ Mesh Mesh::structuredTri(int nx, int ny, double x0, double y0, double lx, double
   return m;
 }
 
+Mesh Mesh::structuredTet(int nx, int ny, int nz, double lx, double ly, double lz) {
+  Mesh m;
+  m.dim = 3;
+  size_t E = (size_t)6 * nx * ny * nz;
+  m.vids.resize(E * 4);
+  m.xyz.resize(E * 12);
+  dg_status s = dg_mesh_tet_structured(nx, ny, nz, lx, ly, lz, m.vids.data(), m.xyz.data());
+  if (s != DG_OK) throw DGError(s, "dg_mesh_tet_structured rejected its arguments");
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+dg_law_desc Euler2dTwoStates::describe() const {
+  dg_law_desc d = Euler2dRoe::describe();
+  for (int i = 0; i < 4; i++) d.farfield[4 + i] = second_[i];
+  return d;
+}
+
+void Maxwell3d::Fi(const mPoint&, double* Q, mVector* F) const {
+  const double *E = Q, *H = Q + 3;
+  F[0] = mVector(0, -H[2], H[1]);
+  F[1] = mVector(H[2], 0, -H[0]);
+  F[2] = mVector(-H[1], H[0], 0);
+  F[3] = mVector(0, E[2], -E[1]);
+  F[4] = mVector(-E[2], 0, E[0]);
+  F[5] = mVector(E[1], -E[0], 0);
+}
+
+void Maxwell3d::riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* out) const {
+  double Ea[3], Ha[3], dE[3], dH[3];
+  for (int i = 0; i < 3; i++) {
+    Ea[i] = 0.5 * (uL[i] + uR[i]);
+    Ha[i] = 0.5 * (uL[3 + i] + uR[3 + i]);
+    dE[i] = uR[i] - uL[i];
+    dH[i] = uR[3 + i] - uL[3 + i];
+  }
+  out[0] = -(n(1) * Ha[2] - n(2) * Ha[1]);
+  out[1] = -(n(2) * Ha[0] - n(0) * Ha[2]);
+  out[2] = -(n(0) * Ha[1] - n(1) * Ha[0]);
+  out[3] = n(1) * Ea[2] - n(2) * Ea[1];
+  out[4] = n(2) * Ea[0] - n(0) * Ea[2];
+  out[5] = n(0) * Ea[1] - n(1) * Ea[0];
+  if (flux_kind == DG_FLUX_UPWIND) {
+    const double ndE = n(0) * dE[0] + n(1) * dE[1] + n(2) * dE[2];
+    const double ndH = n(0) * dH[0] + n(1) * dH[1] + n(2) * dH[2];
+    for (int i = 0; i < 3; i++) {
+      out[i] += 0.5 * (n(i) * ndE - dE[i]);
+      out[3 + i] += 0.5 * (n(i) * ndH - dH[i]);
+    }
+  }
+}
+
+void Maxwell3d::boundaryFlux(mVector& n, int, const mPoint& x, double* Q, double* flux, double) const {
+  double g[6];
+  for (int i = 0; i < 3; i++) {
+    g[i] = -Q[i];
+    g[3 + i] = Q[3 + i];
+  }
+  riemannSolver(n, x, Q, g, flux);
+}
+
+dg_law_desc Maxwell3d::describe() const {
+  dg_law_desc d{};
+  d.law = DG_LAW_MAXWELL3D;
+  d.flux = flux_kind;
+  return d;
+}
+
+std::vector<uint8_t> DGAnalysis::limit() {
+  std::vector<uint8_t> f(n_elems_);
+  check(dg_limit(ctx_, f.data()));
+  return f;
+}
+
+std::vector<int32_t> DGAnalysis::orders() const {
+  std::vector<int32_t> o(n_elems_);
+  check(dg_get_orders(ctx_, o.data()));
+  return o;
+}
+
+std::vector<int32_t> DGAnalysis::adaptP(const dg_sensor_opts& opts) {
+  std::vector<int32_t> o(n_elems_);
+  check(dg_select_orders(ctx_, &opts, 1, o.data()));
+  return o;
+}
+
 // ---------------------------------------------------------------------------
 double TimeIntegrator::advanceInTime(double t, double dt) {
   double r = 0.0;
@@ -128,12 +214,12 @@ double TimeIntegrator::advanceInTime(double t, double dt) {
   return r;
 }
 
-DGAnalysis::DGAnalysis(const Mesh& mesh, ConservationLaw* law, void* limiter, int order, int precision)
+DGAnalysis::DGAnalysis(const Mesh& mesh, ConservationLaw* law, DGLimiter* limiter, int order, int precision)
     : law_(law), order_(order) {
-  if (limiter) throw DGError(DG_ERR_UNSUPPORTED, "limiters are not built in this entry point");
   dg_mesh_desc md{};
   md.dim = mesh.dim;
   md.n_elems = mesh.n_elems();
+  n_elems_ = md.n_elems;
   md.vids = mesh.vids.data();
   md.xyz = mesh.xyz.data();
   md.bc = mesh.bc.empty() ? nullptr : mesh.bc.data();
@@ -150,6 +236,7 @@ DGAnalysis::DGAnalysis(const Mesh& mesh, ConservationLaw* law, void* limiter, in
     ctx_ = nullptr;
     throw DGError(s, msg);
   }
+  if (limiter) check(dg_set_limiter(ctx_, 1, limiter->tvbM(), limiter->nu()));
   // integrator by order (DGAnalysis.cc:239-242)
   if (order == 0)
     integrator_ = new ForwardEuler(this);

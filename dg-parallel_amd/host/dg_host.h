@@ -120,7 +120,61 @@ class Euler2d : public ConservationLaw {
 
 class Euler2dRoe : public Euler2d {
  public:
-  Euler2dRoe(FieldEvaluator* far, double Gamma = 1.4) : Euler2d(far, Gamma) { flux_kind = DG_FLUX_ROE; }
+  // entropy_fix: Harten's fix (delta = 0.2 c), which the reference's Roe flux lacks
+  Euler2dRoe(FieldEvaluator* far, double Gamma = 1.4, bool entropy_fix = false) : Euler2d(far, Gamma) {
+    flux_kind = entropy_fix ? DG_FLUX_ROE_EF : DG_FLUX_ROE;
+  }
+};
+
+// Euler2d with a second far-field state on faces tagged DG_TAG_FARFIELD2 (the
+// reference evaluates its far-field FieldEvaluator at the boundary point; a shock
+// tube's two ends hold the two initial states)
+class Euler2dTwoStates : public Euler2dRoe {
+  double second_[4];
+
+ public:
+  Euler2dTwoStates(FieldEvaluator* far, const double second[4], double Gamma = 1.4, bool entropy_fix = true)
+      : Euler2dRoe(far, Gamma, entropy_fix) {
+    for (int i = 0; i < 4; i++) second_[i] = second[i];
+  }
+  dg_law_desc describe() const override;
+};
+
+// Maxwell's equations in 3D with eps = mu = 1 (the reference's MaxwellLaw.cc:21-200
+// is the 2D TE form): Q = (E, H), F.n = (-n x H, n x E), upwind or central flux,
+// perfect-conductor walls (E -> -E, H -> H in the ghost)
+class Maxwell3d : public ConservationLaw {
+  int flux_kind;
+
+ public:
+  explicit Maxwell3d(bool upwind = true) : flux_kind(upwind ? DG_FLUX_UPWIND : DG_FLUX_CENTRAL) {}
+  int getNbFields() const override { return 6; }
+  void Fi(const mPoint&, double* Q, mVector* flux) const override;
+  void riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* flux) const override;
+  void boundaryFlux(mVector& n, int BoundaryId, const mPoint&, double* Q, double* flux, double T) const override;
+  double maximumEigenValue(const double*, const mPoint&) const override { return 1.0; }
+  dg_law_desc describe() const override;
+};
+
+// ---------------------------------------------------------------------------
+// DGLimiter (DGLimiter.h:9-14): the reference limits one cell at a time from the
+// integrator; here the GPU limits every cell after every stage, so the plugin
+// carries its parameters to the context.
+class DGLimiter {
+ public:
+  virtual ~DGLimiter() = default;
+  virtual double tvbM() const = 0;
+  virtual double nu() const = 0;
+};
+// triangle TVB limiter with positivity scaling (new design: DGMomentLimiter,
+// DGLimiter.cc:963-1133, indexes quadrilateral modes only)
+class DGTriangleLimiter : public DGLimiter {
+  double m_, nu_;
+
+ public:
+  explicit DGTriangleLimiter(double tvb_m = 0.0, double nu = 1.0) : m_(tvb_m), nu_(nu) {}
+  double tvbM() const override { return m_; }
+  double nu() const override { return nu_; }
 };
 
 // ---------------------------------------------------------------------------
@@ -132,6 +186,8 @@ struct Mesh {
   int n_elems() const { return (int)(vids.size() / (dim + 1)); }
   static Mesh structuredTri(int nx, int ny, double x0, double y0, double lx, double ly, bool periodic_x,
                             bool periodic_y, double jitter, uint64_t seed, const int32_t side_tags[4]);
+  // Kuhn subdivision of an n^3 lattice of boxes (6 tetrahedra per box)
+  static Mesh structuredTet(int nx, int ny, int nz, double lx, double ly, double lz);
 };
 
 class DGAnalysis;
@@ -172,11 +228,13 @@ class DGAnalysis {
   ConservationLaw* law_;
   TimeIntegrator* integrator_ = nullptr;
   int order_;
+  int n_elems_ = 0;
 
  public:
   double TACT = 0.0, TEND = 0.2, CFLMAX = 0.5;
 
-  DGAnalysis(const Mesh& mesh, ConservationLaw* law, void* limiter, int order, int precision = DG_PREC_FP64);
+  // limiter: NULL (no limiting) or a DGLimiter applied after every RK stage
+  DGAnalysis(const Mesh& mesh, ConservationLaw* law, DGLimiter* limiter, int order, int precision = DG_PREC_FP64);
   ~DGAnalysis();
   DGAnalysis(const DGAnalysis&) = delete;
   DGAnalysis& operator=(const DGAnalysis&) = delete;
@@ -192,6 +250,12 @@ class DGAnalysis {
   void project(int field, const double* par, double t = 0.0);
   std::vector<double> residual(double t = 0.0);
   double computeDt(double cfl);
+  // limit every cell once (TimeIntegrator::limit); per-element flags, caller order
+  std::vector<uint8_t> limit();
+  // p-adaptation (DGAnalysis::adaptP, DGPref.cc:602-630) with the modal-decay
+  // sensor; re-buckets the elements by their new orders; returns the orders
+  std::vector<int32_t> adaptP(const dg_sensor_opts& opts);
+  std::vector<int32_t> orders() const;
   // the reference's time loop (commented out in DGAnalysis.cc:480-731): dt from the
   // CFL rule, N = ceil((TEND-TACT)/dt) steps of dt = (TEND-TACT)/N; returns N
   int run();

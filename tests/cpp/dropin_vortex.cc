@@ -1,10 +1,15 @@
 // Drives the library through the host C++ mirror classes the way a reference
 // scenario drives DG-Parallel (Examples.cc / Lilia.cc): build a law and a mesh,
 // construct DGAnalysis, then call residual and TimeIntegrator::advanceInTime.
-//   dropin_vortex OUT_PREFIX
+//   dropin_vortex OUT_PREFIX [vortex|tube|maxwell]
 // writes OUT_PREFIX.U0 (projected state), .R (residual), .U2 (after two steps of
-// the integrator DGAnalysis picked for P3, RK4), raw little-endian doubles, and
+// the integrator DGAnalysis picked for the order), raw little-endian doubles, and
 // prints dt and the residual norms.
+//   vortex   Euler2d (LF), P3, periodic jittered 10x10x2 mesh
+//   tube     Euler2dTwoStates (Roe + entropy fix) with DGTriangleLimiter, seeded
+//            orders P1-P4 on a 16x4x2 tube, the state limited once, two SSP-RK3
+//            steps; then adaptP and .O (new orders as doubles), .U3 (state after it)
+//   maxwell  Maxwell3d (upwind), P3 on a 2x2x2 Kuhn mesh, cavity mode, RK4
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -17,27 +22,83 @@ static void dump(const std::string& path, const std::vector<double>& v) {
   fclose(f);
 }
 
+static int vortex(const std::string& out) {
+  const int32_t tags[4] = {0, 0, 0, 0};
+  dgpar::Mesh mesh = dgpar::Mesh::structuredTri(10, 10, 0.0, 0.0, 10.0, 10.0, true, true, 0.1, 77, tags);
+  dgpar::ConstantField far({1.0, 1.0, 1.0, 3.5});
+  dgpar::Euler2d law(&far, 1.4);
+  dgpar::DGAnalysis an(mesh, &law, nullptr, 3);
+  const double vortex[6] = {5.0, 5.0, 5.0, 1.0, 1.0, 10.0};
+  an.project(DG_FIELD_VORTEX, vortex);
+  dump(out + ".U0", an.getState());
+  dump(out + ".R", an.residual(0.0));
+  double dt = an.computeDt(0.1);
+  double r1 = an.integrator()->advanceInTime(0.0, dt);
+  double r2 = an.integrator()->advanceInTime(dt, dt);
+  dump(out + ".U2", an.getState());
+  printf("dt %.17e\nresid %.17e %.17e\n", dt, r1, r2);
+  return 0;
+}
+
+static int tube(const std::string& out) {
+  const int32_t tags[4] = {DG_TAG_WALL, DG_TAG_FARFIELD2, DG_TAG_WALL, DG_TAG_FARFIELD};
+  dgpar::Mesh mesh = dgpar::Mesh::structuredTri(16, 4, 0.0, 0.0, 1.0, 0.25, false, false, 0.1, 5, tags);
+  // seeded orders 1..4 (splitmix64 of the element index)
+  mesh.order.resize(mesh.n_elems());
+  for (int e = 0; e < mesh.n_elems(); e++) {
+    uint64_t z = (uint64_t)e * 0x9E3779B97F4A7C15ull + 11;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    mesh.order[e] = 1 + (int)((z ^ (z >> 31)) % 4);
+  }
+  dgpar::ConstantField far({1.0, 0.0, 0.0, 2.5});
+  const double right[4] = {0.125, 0.0, 0.0, 0.25};
+  dgpar::Euler2dTwoStates law(&far, right, 1.4, true);
+  dgpar::DGTriangleLimiter lim(0.0, 1.0);
+  dgpar::DGAnalysis an(mesh, &law, &lim, 2);  // order 2: the reference picks SSP-RK3
+  const double sod[1] = {0.53};
+  an.project(DG_FIELD_SOD, sod);
+  an.limit();
+  dump(out + ".U0", an.getState());
+  dump(out + ".R", an.residual(0.0));
+  double dt = an.computeDt(0.1);
+  double r1 = an.integrator()->advanceInTime(0.0, dt);
+  double r2 = an.integrator()->advanceInTime(dt, dt);
+  dump(out + ".U2", an.getState());
+  dg_sensor_opts so{{0.05, 0.02, 0.01, 0.005}, {5e-5, 2e-5, 1e-5, 5e-6}, 1, 4};
+  std::vector<int32_t> o = an.adaptP(so);
+  dump(out + ".O", std::vector<double>(o.begin(), o.end()));
+  dump(out + ".U3", an.getState());
+  std::vector<int32_t> o0 = mesh.order;
+  dump(out + ".O0", std::vector<double>(o0.begin(), o0.end()));
+  printf("dt %.17e\nresid %.17e %.17e\n", dt, r1, r2);
+  return 0;
+}
+
+static int maxwell(const std::string& out) {
+  dgpar::Mesh mesh = dgpar::Mesh::structuredTet(2, 2, 2, 1.0, 1.0, 1.0);
+  dgpar::Maxwell3d law(true);
+  dgpar::DGAnalysis an(mesh, &law, nullptr, 3);
+  an.project(DG_FIELD_CAVITY, nullptr);
+  dump(out + ".U0", an.getState());
+  dump(out + ".R", an.residual(0.0));
+  double dt = an.computeDt(0.1);
+  double r1 = an.integrator()->advanceInTime(0.0, dt);
+  double r2 = an.integrator()->advanceInTime(dt, dt);
+  dump(out + ".U2", an.getState());
+  printf("dt %.17e\nresid %.17e %.17e\n", dt, r1, r2);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
-  std::string out = argv[1];
+  std::string out = argv[1], mode = argc > 2 ? argv[2] : "vortex";
   try {
-    const int32_t tags[4] = {0, 0, 0, 0};
-    dgpar::Mesh mesh = dgpar::Mesh::structuredTri(10, 10, 0.0, 0.0, 10.0, 10.0, true, true, 0.1, 77, tags);
-    dgpar::ConstantField far({1.0, 1.0, 1.0, 3.5});
-    dgpar::Euler2d law(&far, 1.4);
-    dgpar::DGAnalysis an(mesh, &law, nullptr, 3);
-    const double vortex[6] = {5.0, 5.0, 5.0, 1.0, 1.0, 10.0};
-    an.project(DG_FIELD_VORTEX, vortex);
-    dump(out + ".U0", an.getState());
-    dump(out + ".R", an.residual(0.0));
-    double dt = an.computeDt(0.1);
-    double r1 = an.integrator()->advanceInTime(0.0, dt);
-    double r2 = an.integrator()->advanceInTime(dt, dt);
-    dump(out + ".U2", an.getState());
-    printf("dt %.17e\nresid %.17e %.17e\n", dt, r1, r2);
+    if (mode == "tube") return tube(out);
+    if (mode == "maxwell") return maxwell(out);
+    return vortex(out);
   } catch (const dgpar::DGError& e) {
     fprintf(stderr, "DGError %d: %s\n", (int)e.status, e.what());
     return 1;
   }
-  return 0;
 }
