@@ -37,3 +37,64 @@ def test_cpp_dropin_matches_oracle():
     for s in range(2):
         U, _ = o.step(U, po.RK4, s * dt, dt)
     assert np.abs(U2 - U).max() / np.abs(U).max() < 1e-12
+
+
+def run_exe(mode):
+    d = tempfile.mkdtemp()
+    pre = os.path.join(d, "run")
+    r = subprocess.run([EXE, pre, mode], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = dict(l.split(None, 1) for l in r.stdout.splitlines())
+    return pre, float(lines["dt"]), [float(x) for x in lines["resid"].split()]
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="C++ driver not built")
+def test_cpp_dropin_shock_tube_with_limiter_and_adaptation():
+    print("\n[gpu case] C++ drop-in shock tube", flush=True)
+    pre, dt, res = run_exe("tube")
+    U0, R, U2, U3 = (np.fromfile(pre + x) for x in (".U0", ".R", ".U2", ".U3"))
+    O0 = np.fromfile(pre + ".O0").astype(np.int32)
+    O = np.fromfile(pre + ".O").astype(np.int32)
+    vids, xyz, bc = mesh_tri(16, 4, lx=1.0, ly=0.25, periodic_x=False, periodic_y=False, jitter=0.1, seed=5,
+                             side_tags=(40000, 2, 40000, 1))
+    ff = (1.0, 0.0, 0.0, 2.5, 0.125, 0.0, 0.0, 0.25)
+    o = po.Oracle(2, vids, xyz, O0, po.LAW_EULER2D, po.FLUX_ROE_EF, gamma=1.4, farfield=ff, bc=bc)
+    Uo, _ = o.limit(o.project(2, (0.53,)))
+    assert np.abs(U0 - Uo).max() / np.abs(Uo).max() < 1e-13
+    Ro, _ = o.residual(U0)
+    assert np.abs(R - Ro).max() / np.abs(Ro).max() < 1e-12
+    assert abs(dt - o.dt(U0, 0.1)) <= 1e-13 * dt
+    o.set_limiter(True)
+    U = U0
+    for s in range(2):
+        U, r = o.step(U, po.RK_TVD3, s * dt, dt)
+    assert np.abs(U2 - U).max() / np.abs(U).max() < 1e-12
+    assert abs(res[1] - r) <= 1e-10 * abs(r)
+    assert np.array_equal(O, o.sensor(U2, (0.05, 0.02, 0.01, 0.005), (5e-5, 2e-5, 1e-5, 5e-6), 1, 4))
+    out, i = [], 0
+    for p0, p1 in zip(O0, O):
+        b = U2[i:i + 4 * (p0 + 1) * (p0 + 2) // 2].reshape(4, -1)
+        i += b.size
+        c = np.zeros((4, (p1 + 1) * (p1 + 2) // 2))
+        k = min(b.shape[1], c.shape[1])
+        c[:, :k] = b[:, :k]
+        out.append(c.ravel())
+    assert np.array_equal(U3, np.concatenate(out))
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="C++ driver not built")
+def test_cpp_dropin_maxwell_tets():
+    import dgpar
+    print("\n[gpu case] C++ drop-in Maxwell", flush=True)
+    pre, dt, res = run_exe("maxwell")
+    U0, R, U2 = (np.fromfile(pre + x) for x in (".U0", ".R", ".U2"))
+    m = dgpar.mesh_tet_structured(2, 2, 2)
+    o = po.Oracle(3, m.vids, m.xyz, 3, po.LAW_MAXWELL3D, po.FLUX_UPWIND)
+    Uo = o.project(3)
+    assert np.abs(U0 - Uo).max() / np.abs(Uo).max() < 1e-13
+    Ro, _ = o.residual(U0)
+    assert np.abs(R - Ro).max() / np.abs(Ro).max() < 1e-12
+    U = U0
+    for s in range(2):
+        U, r = o.step(U, po.RK4, s * dt, dt)
+    assert np.abs(U2 - U).max() / np.abs(U).max() < 1e-12
