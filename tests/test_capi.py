@@ -61,3 +61,12 @@ def test_mesh_generator_rejects_bad_arguments():
     import dgpar
     with pytest.raises(dgpar.DGError):
         dgpar.mesh_tri_structured(0, 4)
+
+
+def test_autograd_extension_loads():
+    """the libtorch autograd Function module (no GPU call)"""
+    so = os.path.join(ROOT, "dg-parallel_amd", "autograd", "dgpar_torch.so")
+    if not os.path.exists(so):
+        pytest.skip("autograd extension not built")
+    import dgpar_autograd
+    assert callable(dgpar_autograd.rk4) and "ctx_handle" in dgpar_autograd._ext.rk4.__doc__
