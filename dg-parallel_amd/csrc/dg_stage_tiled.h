@@ -40,6 +40,16 @@ DG_HD int wave_uniform(int x) {
 #endif
 }
 
+// true when the predicate holds on every active lane of the wave (on the CPU
+// emulation, which runs one lane at a time, the lane's own predicate)
+DG_HD bool wave_all(bool pred) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __all(pred);
+#else
+  return pred;
+#endif
+}
+
 // Divisions by 3, 6 and 36 are multiplications by the rounded reciprocals (one
 // rounding more than the reference's division, far inside the 1e-12 parity
 // bound; a full FP64 division is a 10-instruction sequence per value).

@@ -7,16 +7,21 @@
 //
 // 64 elements per 256-thread workgroup, lane = element, waves share rows (field,
 // mode) or face points.  Phases:
-//   A   load the tile's coefficients (rows of 64 consecutive doubles), |J| J^-1
-//       and the face links into LDS;
+//   A   load the tile's coefficients (rows of 64 consecutive doubles) and the
+//       face links into LDS;
 //   B   volume term in operator form: the Maxwell flux is linear, F_d(u) = A_d u,
 //       so int F . grad phi_k = sum_r sum_d mg[r][d] sum_j K_r[k][j] (A_d U_j) with
-//       K_r the reference stiffness operators (exact for the degree-2p rule);
-//   per face l (one at a time, so LDS holds one face's traces):
-//   C1  neighbour traces at the 13 face points, in the neighbour's basis through
-//       the trace table of its local face and the face's vertex permutation;
-//   C2  own trace, numerical flux (PEC faces: ghost E -> -E, H -> H), times the
-//       face area and weight, written over the neighbour trace of that point;
+//       K_r the reference stiffness operators (exact for the degree-2p rule) and
+//       mg = |J| J^-1 read per lane;
+//   per face l (one at a time; the two trace arrays of a face reuse the
+//   coefficient rows once B is done, which keeps LDS at 80 KB: two workgroups
+//   per CU):
+//   C1  own and neighbour traces at the face points (12 tasks, field x side,
+//       3 per wave), each from global coefficients (own rows are coalesced,
+//       neighbour rows gathered) through the trace table of the local face, and
+//       for the neighbour of its local face and the face's vertex permutation;
+//   C2  numerical flux (PEC faces: ghost E -> -E, H -> H), times the face area
+//       and weight, written over the own trace of that point;
 //   L   lift into the register rows;
 //   D   inverse mass 1/|J| and the RK stage update.
 #pragma once
@@ -78,15 +83,20 @@ struct TiledTet {
   static constexpr int RPW = (NROW + 3) / 4;
   static constexpr int FW = NQF * NF;
 
+  struct Traces {
+    double WL[FW][T];  // own trace, then flux times area and weight
+    double WR[FW][T];  // neighbour trace
+  };
   struct Smem {
-    double Us[NROW][T];
-    double Ms[9][T];
+    union {
+      double Us[NROW][T];  // phases A-B
+      Traces W;            // phases C1-L
+      double red[TILE_THREADS];
+    };
     int Ls[4][T];
-    double W[FW][T];
-    double red[TILE_THREADS];
   };
   struct Regs {
-    double R[RPW];
+    double R[NF * ((NP + 3) / 4)];
     double rsum;
   };
 
@@ -95,93 +105,157 @@ struct TiledTet {
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
     for (int r = wave; r < NROW; r += 4) s.Us[r][t] = valid ? a.Uin[(long)r * E + e] : 0.0;
-    for (int r = wave; r < 9; r += 4) s.Ms[r][t] = valid ? a.mg[(long)r * E + e] : (r % 4 == 0 ? 1.0 : 0.0);
     s.Ls[wave][t] = valid ? a.link[(long)wave * E + e] : -BC_PEC;
   }
 
-  // volume term, operator form
-  DG_HD static void phaseB(const TetArgs& a, Smem& s, int tid, Regs& g) {
+  // |J| J^-1 of the lane's element (the identity past the end of the mesh)
+  DG_HD static void load_mg(const TetArgs& a, int tile, int t, double m[3][3]) {
+    const int E = a.E, e = tile * T + t;
+    const bool valid = e < E;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) m[r][d] = valid ? a.mg[(long)(r * 3 + d) * E + e] : (r == d ? 1.0 : 0.0);
+  }
+
+  // Rows are owned by k-slot: wave w holds the modes k = w + 4 ks (ks < KS) of
+  // every field, so row (f, k) is register j = f*KS + ks and the table rows a
+  // wave reads depend on ks only (wave-uniform, scalar loads).
+  static constexpr int KS = (NP + 3) / 4;
+  DG_HD static int kmode(int wave, int ks) { return wave + 4 * ks; }
+
+  // volume term, operator form: for each of the wave's modes k, the 3 x 6 products
+  // (K_r U_g)[k] once, then every field's row from its two (d, g) pairs
+  DG_HD static void phaseB(const TetArgs& a, Smem& s, int tid, int tile, Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const double* K = a.tab + TetBlock::off_K(P);
-    for (int j = 0; j < RPW; j++) {
-      const int row = wave + 4 * j;
-      double acc = 0.0;
-      if (row < NROW) {
-        const int f = row / NP, k = row - f * NP;
-        const MaxwellJac J = maxwell_jac(f);
-        for (int m = 0; m < 2; m++) {
-          const int gg = J.g[m], d = J.d[m];
-          double dot[3] = {0.0, 0.0, 0.0};
-#pragma unroll 2
-          for (int jj = 0; jj < NP; jj++) {
-            const double u = s.Us[gg * NP + jj][t];
+    double m[3][3];
+    load_mg(a, tile, t, m);
 #pragma unroll
-            for (int r = 0; r < 3; r++) dot[r] += K[(r * NP + k) * NP + jj] * u;
-          }
-          acc += J.s[m] * (s.Ms[0 * 3 + d][t] * dot[0] + s.Ms[1 * 3 + d][t] * dot[1] + s.Ms[2 * 3 + d][t] * dot[2]);
+    for (int ks = 0; ks < KS; ks++) {
+      const int k = kmode(wave, ks);
+      if (k >= NP) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) g.R[f * KS + ks] = 0.0;
+        continue;
+      }
+      double dot[NF][3];
+#pragma unroll
+      for (int gg = 0; gg < NF; gg++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) dot[gg][r] = 0.0;
+#pragma unroll 4
+      for (int jj = 0; jj < NP; jj++) {
+        const double k0 = K[(0 * NP + k) * NP + jj], k1 = K[(1 * NP + k) * NP + jj], k2 = K[(2 * NP + k) * NP + jj];
+#pragma unroll
+        for (int gg = 0; gg < NF; gg++) {
+          const double u = s.Us[gg * NP + jj][t];
+          dot[gg][0] += k0 * u;
+          dot[gg][1] += k1 * u;
+          dot[gg][2] += k2 * u;
         }
       }
-      g.R[j] = acc;
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        const MaxwellJac J = maxwell_jac(f);
+        double acc = 0.0;
+#pragma unroll
+        for (int mm = 0; mm < 2; mm++) {
+          const int gg = J.g[mm], d = J.d[mm];
+          acc += J.s[mm] * (m[0][d] * dot[gg][0] + m[1][d] * dot[gg][1] + m[2][d] * dot[gg][2]);
+        }
+        g.R[f * KS + ks] = acc;
+      }
     }
   }
 
-  // face l: neighbour traces into W[q*NF + f]
+  // face l: own traces into WL[q*NF + f], neighbour traces into WR[q*NF + f]
   DG_HD static void phaseC1(const TetArgs& a, Smem& s, int tid, int tile, int l) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const bool valid = tile * T + t < a.E;
+    const int E = a.E, e = tile * T + t;
+    const bool valid = e < E;
     const int link = s.Ls[l][t];
-    for (int f = wave; f < NF; f += 4) {
+    for (int task = wave; task < 2 * NF; task += 4) {
       double v[NP];
+      if (task < NF) {
+        const int f = task;
+        for (int k = 0; k < NP; k++) v[k] = valid ? a.Uin[(long)(f * NP + k) * E + e] : 0.0;
+        const double* tn = a.tab + TetBlock::off_phif(P) + l * NQF * NP;
+        for (int q = 0; q < NQF; q++) {
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
+          s.W.WL[q * NF + f][t] = acc;
+        }
+        continue;
+      }
+      const int f = task - NF;
       const bool inner = valid && link >= 0;
       const int nb = inner ? (link >> 5) : 0;
-      for (int k = 0; k < NP; k++) v[k] = inner ? a.Uin[(long)(f * NP + k) * a.E + nb] : 0.0;
-      const int o = (link >> 2) & 7, ln = link & 3;
-      const double* tn = a.tab + TetBlock::off_phin(P) + (inner ? (ln * 6 + o) * NQF * NP : 0);
+      for (int k = 0; k < NP; k++) v[k] = inner ? a.Uin[(long)(f * NP + k) * E + nb] : 0.0;
+      const int key = inner ? ((link >> 2) & 7) * 4 + (link & 3) : 0;  // perm * 4 + neighbour face
+      const int key0 = wave_uniform(key);
+      if (wave_all(key == key0)) {
+        // the usual case (elements are grouped by link signature): one table
+        // for the wave, read through the scalar cache
+        const double* tn = a.tab + TetBlock::off_phin(P) + ((key0 & 3) * 6 + (key0 >> 2)) * NQF * NP;
+        for (int q = 0; q < NQF; q++) {
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
+          s.W.WR[q * NF + f][t] = acc;
+        }
+      } else {
+        const double* tn = a.tab + TetBlock::off_phin(P) + ((key & 3) * 6 + (key >> 2)) * NQF * NP;
 #pragma unroll 1
-      for (int q = 0; q < NQF; q++) {
-        double acc = 0.0;
-        for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
-        s.W[q * NF + f][t] = acc;
+        for (int q = 0; q < NQF; q++) {
+          double acc = 0.0;
+          for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
+          s.W.WR[q * NF + f][t] = acc;
+        }
       }
     }
   }
 
   // face l points: flux times area and weight, over the neighbour traces
-  DG_HD static void phaseC2(const TetArgs& a, Smem& s, int tid, int l) {
+  DG_HD static void phaseC2(const TetArgs& a, Smem& s, int tid, int tile, int l) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const double* tb = a.tab;
     const int link = s.Ls[l][t];
-    double Nr[3], N[3];
+    double m[3][3], Nr[3], N[3];
+    load_mg(a, tile, t, m);
     tet_ref_normal(l, Nr);
-    for (int d = 0; d < 3; d++) N[d] = s.Ms[0 * 3 + d][t] * Nr[0] + s.Ms[1 * 3 + d][t] * Nr[1] + s.Ms[2 * 3 + d][t] * Nr[2];
+    for (int d = 0; d < 3; d++) N[d] = m[0][d] * Nr[0] + m[1][d] * Nr[1] + m[2][d] * Nr[2];
     const double area = dg_sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
     const double n[3] = {N[0] / area, N[1] / area, N[2] / area};
     for (int q = wave; q < NQF; q += 4) {
       double uL[NF], uR[NF], Fh[NF];
       for (int f = 0; f < NF; f++) {
-        double acc = 0.0;
-        for (int k = 0; k < NP; k++) acc += tb[TetBlock::off_phif(P) + (l * NQF + q) * NP + k] * s.Us[f * NP + k][t];
-        uL[f] = acc;
-        uR[f] = s.W[q * NF + f][t];
+        uL[f] = s.W.WL[q * NF + f][t];
+        uR[f] = s.W.WR[q * NF + f][t];
       }
       if (link < 0) LAW::ghost(a.law, -link, n, uL, uR);
       LAW::riemann(a.law, n, uL, uR, Fh);
       const double sw = area * tb[TetBlock::off_wf(P) + q];
-      for (int f = 0; f < NF; f++) s.W[q * NF + f][t] = Fh[f] * sw;
+      for (int f = 0; f < NF; f++) s.W.WL[q * NF + f][t] = Fh[f] * sw;
     }
   }
 
   DG_HD static void phaseLift(const TetArgs& a, Smem& s, int tid, int l, Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const double* tb = a.tab + TetBlock::off_phif(P) + l * NQF * NP;
-    for (int j = 0; j < RPW; j++) {
-      const int row = wave + 4 * j;
-      if (row < NROW) {
-        const int f = row / NP, k = row - f * NP;
-        double acc = g.R[j];
-#pragma unroll 1
-        for (int q = 0; q < NQF; q++) acc -= tb[q * NP + k] * s.W[q * NF + f][t];
-        g.R[j] = acc;
+    for (int q = 0; q < NQF; q++) {
+      double Fq[NF];
+#pragma unroll
+      for (int f = 0; f < NF; f++) Fq[f] = s.W.WL[q * NF + f][t];
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int k = kmode(wave, ks);
+        if (k < NP) {
+          const double phi = tb[q * NP + k];
+#pragma unroll
+          for (int f = 0; f < NF; f++) g.R[f * KS + ks] -= phi * Fq[f];
+        }
       }
     }
   }
@@ -195,19 +269,23 @@ struct TiledTet {
     const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
     const bool reads_acc = sc == 3 * 4 + 1 || sc == 3 * 4 + 2 || sc == 3 * 4 + 3;
     const double dtj = a.dt / a.detj[e];
-    for (int j = 0; j < RPW; j++) {
-      const int row = wave + 4 * j;
-      if (row >= NROW) continue;
-      const long i = (long)row * E + e;
-      if (a.scheme == MODE_RESIDUAL) {
-        a.R[i] = g.R[j];
-        continue;
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int k = kmode(wave, ks);
+        if (k >= NP) continue;
+        const long i = (long)(f * NP + k) * E + e;
+        const double r = g.R[f * KS + ks];
+        if (a.scheme == MODE_RESIDUAL) {
+          a.R[i] = r;
+          continue;
+        }
+        double accv = reads_acc ? a.acc[i] : 0.0;
+        const double o = stage_value_hd(sc, r * dtj, a.Un[i], reads_input ? a.Uin[i] : 0.0, &accv, g.rsum);
+        if (sc >= 3 * 4 && sc < 3 * 4 + 3) a.acc[i] = accv;
+        a.Uout[i] = o;
       }
-      double accv = reads_acc ? a.acc[i] : 0.0;
-      const double o = stage_value_hd(sc, g.R[j] * dtj, a.Un[i], reads_input ? a.Uin[i] : 0.0, &accv, g.rsum);
-      if (sc >= 3 * 4 && sc < 3 * 4 + 3) a.acc[i] = accv;
-      a.Uout[i] = o;
-    }
   }
 };
 

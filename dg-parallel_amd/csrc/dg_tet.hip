@@ -20,13 +20,13 @@ __global__ __launch_bounds__(TILE_THREADS) void tet_stage_tiled(TetArgs a) {
   const int tid = threadIdx.x;
   K::phaseA(a, sm, tid, tile);
   __syncthreads();
-  K::phaseB(a, sm, tid, g);
+  K::phaseB(a, sm, tid, tile, g);
 #pragma unroll 1
   for (int l = 0; l < 4; l++) {
     __syncthreads();
     K::phaseC1(a, sm, tid, tile, l);
     __syncthreads();
-    K::phaseC2(a, sm, tid, l);
+    K::phaseC2(a, sm, tid, tile, l);
     __syncthreads();
     K::phaseLift(a, sm, tid, l, g);
   }

@@ -547,10 +547,10 @@ void tet_tiles_run(const TetArgs& a, double* rsum) {
   for (int tile = 0; tile < ntiles; tile++) {
     std::memset(sm.get(), 0, sizeof(typename K::Smem));
     for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, *sm, t, tile);
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, *sm, t, g[t]);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, *sm, t, tile, g[t]);
     for (int l = 0; l < 4; l++) {
       for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, *sm, t, tile, l);
-      for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t, l);
+      for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t, tile, l);
       for (int t = 0; t < TILE_THREADS; t++) K::phaseLift(a, *sm, t, l, g[t]);
     }
     for (int t = 0; t < TILE_THREADS; t++) {
