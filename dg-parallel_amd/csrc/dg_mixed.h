@@ -55,10 +55,14 @@ struct MixedArgs {
   int nchunks;
 };
 
-DG_HD int mixed_order_of(const MixedArgs& a, int i) {
+// order and coefficient index from the bucket bounds of any argument block
+// (MixedArgs, LimiterArgs, SensorArgs all carry bstart/boff)
+template <class A>
+DG_HD int mixed_order_of(const A& a, int i) {
   return 1 + (i >= a.bstart[2]) + (i >= a.bstart[3]) + (i >= a.bstart[4]);
 }
-DG_HD long mixed_index(const MixedArgs& a, int p, int i, int row) {
+template <class A>
+DG_HD long mixed_index(const A& a, int p, int i, int row) {
   return a.boff[p] + (long)row * (a.bstart[p + 1] - a.bstart[p]) + (i - a.bstart[p]);
 }
 
@@ -256,112 +260,123 @@ DG_HD double pos_pressure(double gm1, const double* w) {
   return gm1 * (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
 }
 
-DG_HD bool positivity_element(const LimiterArgs& a, const MixedArgs& ix, int i, int p, int np) {
+// one check point (basis row without phi_0) of the positivity pass; pass 0
+// tracks the density bound, pass 1 the pressure bound with density scaled by th1
+template <int NP>
+DG_HD void pos_point(const double* row, const double (*U)[NP], const double* mean, int pass, double gm1, double eps,
+                     double& th1, double& th2) {
+  if (pass == 0) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 1; k < NP; k++) acc += row[k] * U[0][k];
+    const double w0 = mean[0] + acc;
+    if (w0 < eps) {
+      const double t = (mean[0] - eps) / (mean[0] - w0);
+      if (t < th1) th1 = t;
+    }
+    return;
+  }
+  double w[4];
+#pragma unroll
+  for (int f = 0; f < 4; f++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 1; k < NP; k++) acc += row[k] * U[f][k];
+    w[f] = mean[f] + (f == 0 ? th1 : 1.0) * acc;
+  }
+  if (pos_pressure(gm1, w) < eps) {
+    // p(mean + s (w - mean)) = eps: rho(s) (E(s) - eps/(g-1)) - |m(s)|^2 / 2 = 0
+    const double dr = w[0] - mean[0], du = w[1] - mean[1], dv = w[2] - mean[2], de = w[3] - mean[3];
+    const double e0 = mean[3] - eps / gm1;
+    const double qa = dr * de - 0.5 * (du * du + dv * dv);
+    const double qb = dr * e0 + mean[0] * de - (mean[1] * du + mean[2] * dv);
+    const double qc = mean[0] * e0 - 0.5 * (mean[1] * mean[1] + mean[2] * mean[2]);
+    double t;
+    if (qa == 0.0) {
+      t = -qc / qb;
+    } else {
+      const double disc = qb * qb - 4.0 * qa * qc;
+      const double sq = dg_sqrt(disc > 0.0 ? disc : 0.0);
+      // the root in (0, 1]: qc > 0 at s = 0, f(1) < 0
+      const double r1 = (-qb - sq) / (2.0 * qa), r2 = (-qb + sq) / (2.0 * qa);
+      t = 1.0;
+      if (r1 > 0.0 && r1 < t) t = r1;
+      if (r2 > 0.0 && r2 < t) t = r2;
+    }
+    if (t < 0.0) t = 0.0;
+    if (t < th2) th2 = t;
+  }
+}
+
+// the scaling on the element's coefficients U (registers); true when it scaled.
+// Check points in order: the volume points of order P, the vertices, then the
+// edge points of the face rules nq = P+1 .. 5 (rows of the order-P trace tables).
+template <int P>
+DG_HD bool positivity_element(const LimiterArgs& a, double (*U)[np_tri(P)]) {
+  constexpr int NP = np_tri(P), NQV = nqv_tri(P);
   const double gm1 = a.law.gamma - 1.0;
-  const double phi0 = a.mvtx[p * 60];
+  const double phi0 = a.mvtx[P * 60];
   double mean[4];
-  for (int f = 0; f < 4; f++) mean[f] = a.U[mixed_index(ix, p, i, f * np)] * phi0;
+#pragma unroll
+  for (int f = 0; f < 4; f++) mean[f] = U[f][0] * phi0;
   const double pm = pos_pressure(gm1, mean);
   double eps = 1e-13;
   if (mean[0] < eps) eps = mean[0];
   if (pm < eps) eps = pm;
   if (!(eps > 0.0)) return false;  // a non-physical mean: nothing to scale towards
-  // the check points: (table row pointer, row stride) per point set
-  const double* tb = a.tab + a.moff[p];
-  const int nqv = p == 1 ? 3 : p == 2 ? 6 : p == 3 ? 12 : 16;
+  const double* tb = a.tab + a.moff[P];
   double th1 = 1.0, th2 = 1.0;
   for (int pass = 0; pass < 2; pass++) {
-    // pass 0: density minimum; pass 1: pressure with the density already scaled
-    const double s0 = pass == 0 ? 1.0 : th1;
-    int npts_total = nqv + 3;
-    for (int nq = p + 1; nq <= MAXP_TRI + 1; nq++) npts_total += 3 * nq;
-    for (int pt = 0; pt < npts_total; pt++) {
-      const double* row;
-      if (pt < nqv) {
-        row = tb + TriBlock::off_phi(0) + pt * np;
-      } else if (pt < nqv + 3) {
-        row = a.mvtx + p * 60 + (1 + pt - nqv) * 15;
-      } else {
-        int r = pt - nqv - 3, nq = p + 1;
-        while (r >= 3 * nq) {
-          r -= 3 * nq;
-          nq++;
-        }
-        row = a.tab + a.ftoff[p][nq] + r * np;
-      }
-      double w[4];
-      for (int f = 0; f < 4; f++) {
-        double acc = 0.0;
-        for (int k = 1; k < np; k++) acc += row[k] * a.U[mixed_index(ix, p, i, f * np + k)];
-        w[f] = mean[f] + (f == 0 ? s0 : 1.0) * acc;
-      }
-      if (pass == 0) {
-        if (w[0] < eps) {
-          const double t = (mean[0] - eps) / (mean[0] - w[0]);
-          if (t < th1) th1 = t;
-        }
-      } else {
-        if (pos_pressure(gm1, w) < eps) {
-          // p(mean + s (w - mean)) = eps: rho(s) (E(s) - eps/(g-1)) - |m(s)|^2 / 2 = 0
-          const double dr = w[0] - mean[0], du = w[1] - mean[1], dv = w[2] - mean[2], de = w[3] - mean[3];
-          const double e0 = mean[3] - eps / gm1;
-          const double qa = dr * de - 0.5 * (du * du + dv * dv);
-          const double qb = dr * e0 + mean[0] * de - (mean[1] * du + mean[2] * dv);
-          const double qc = mean[0] * e0 - 0.5 * (mean[1] * mean[1] + mean[2] * mean[2]);
-          double t;
-          if (qa == 0.0) {
-            t = -qc / qb;
-          } else {
-            const double disc = qb * qb - 4.0 * qa * qc;
-            const double sq = dg_sqrt(disc > 0.0 ? disc : 0.0);
-            // the root in (0, 1]: qc > 0 at s = 0, f(1) < 0
-            const double r1 = (-qb - sq) / (2.0 * qa), r2 = (-qb + sq) / (2.0 * qa);
-            t = 1.0;
-            if (r1 > 0.0 && r1 < t) t = r1;
-            if (r2 > 0.0 && r2 < t) t = r2;
-          }
-          if (t < 0.0) t = 0.0;
-          if (t < th2) th2 = t;
-        }
-      }
+#pragma unroll 1
+    for (int q = 0; q < NQV; q++) pos_point<NP>(tb + TriBlock::off_phi(P) + q * NP, U, mean, pass, gm1, eps, th1, th2);
+#pragma unroll 1
+    for (int v = 0; v < 3; v++) pos_point<NP>(a.mvtx + P * 60 + (1 + v) * 15, U, mean, pass, gm1, eps, th1, th2);
+#pragma unroll
+    for (int nq = P + 1; nq <= MAXP_TRI + 1; nq++) {
+      const double* ft = a.tab + a.ftoff[P][nq];
+#pragma unroll 1
+      for (int r = 0; r < 3 * nq; r++) pos_point<NP>(ft + r * NP, U, mean, pass, gm1, eps, th1, th2);
     }
   }
   if (th1 >= 1.0 && th2 >= 1.0) return false;
+#pragma unroll
   for (int f = 0; f < 4; f++) {
     const double sc = (f == 0 ? th1 : 1.0) * th2;
-    for (int k = 1; k < np; k++) {
-      const long idx = mixed_index(ix, p, i, f * np + k);
-      a.U[idx] = sc * a.U[idx];
-    }
+#pragma unroll
+    for (int k = 1; k < NP; k++) U[f][k] = sc * U[f][k];
   }
   return true;
 }
 
-template <class LAW>
-DG_HD void limit_element(const LimiterArgs& a, int i) {
-  constexpr int NF = LAW::NF;
-  MixedArgs ix{};
-  for (int p = 0; p < 6; p++) {
-    ix.bstart[p] = a.bstart[p];
-    ix.boff[p] = a.boff[p];
-  }
-  const int E = a.E, p = mixed_order_of(ix, i), np = (p + 1) * (p + 2) / 2;
-  const double phi0 = a.mvtx[p * 60];
-  double mean[NF], c1[NF], c2[NF], full[3][NF];
+// the limiter on one element of order P: its coefficients stay in registers and
+// only rewritten fields go back to memory
+template <class LAW, int P>
+DG_HD void limit_element_p(const LimiterArgs& a, int i) {
+  const LimiterArgs& ix = a;
+  constexpr int NF = LAW::NF, NP = np_tri(P);
+  const int E = a.E;
+  const double phi0 = a.mvtx[P * 60];
+  double U[NF][NP];
+#pragma unroll
+  for (int f = 0; f < NF; f++)
+#pragma unroll
+    for (int k = 0; k < NP; k++) U[f][k] = a.U[mixed_index(ix, P, i, f * NP + k)];
+  double mean[NF], full[3][NF];
+#pragma unroll
   for (int f = 0; f < NF; f++) {
-    mean[f] = a.U[mixed_index(ix, p, i, f * np)] * phi0;
-    c1[f] = a.U[mixed_index(ix, p, i, f * np + 1)];
-    c2[f] = a.U[mixed_index(ix, p, i, f * np + 2)];
-    for (int l = 0; l < 3; l++) full[l][f] = c1[f] * a.midv[l][1];
-    for (int k = 2; k < np; k++) {
-      const double u = a.U[mixed_index(ix, p, i, f * np + k)];
-      for (int l = 0; l < 3; l++) full[l][f] = fma(u, a.midv[l][k], full[l][f]);
-    }
+    mean[f] = U[f][0] * phi0;
+#pragma unroll
+    for (int l = 0; l < 3; l++) full[l][f] = U[f][1] * a.midv[l][1];
+#pragma unroll
+    for (int k = 2; k < NP; k++)
+#pragma unroll
+      for (int l = 0; l < 3; l++) full[l][f] = fma(U[f][k], a.midv[l][k], full[l][f]);
   }
   double mg[2][2];
   for (int r = 0; r < 2; r++)
     for (int d = 0; d < 2; d++) mg[r][d] = a.el.mg[(long)(r * 2 + d) * E + i];
   double D[3][NF];
+#pragma unroll
   for (int l = 0; l < 3; l++) {
     const int link = a.el.link[(long)l * E + i];
     double g[NF];
@@ -373,24 +388,28 @@ DG_HD void limit_element(const LimiterArgs& a, int i) {
       edge_normal(mg, l, n, len);
       LAW::ghost(a.law, -link, n, mean, g);
     }
+#pragma unroll
     for (int f = 0; f < NF; f++) D[l][f] = g[f] - mean[f];
   }
   const double h = a.el.size[i];
-  uint8_t flag = 0;
+  unsigned flag = 0;
+#pragma unroll
   for (int f = 0; f < NF; f++) {
     double d[3];
     bool troubled = false;
     double tol = a.tvb_m * h * h;
     const double tr = 1e-10 * (mean[f] < 0.0 ? -mean[f] : mean[f]);
     if (tr > tol) tol = tr;
+#pragma unroll
     for (int l = 0; l < 3; l++) {
       const double u = full[l][f], au = u < 0.0 ? -u : u;
       if (au > tol && lim_minmod(u, a.nu * D[l][f]) != u) troubled = true;
-      d[l] = fma(c2[f], a.midv[l][2], c1[f] * a.midv[l][1]);
+      d[l] = fma(U[f][2], a.midv[l][2], U[f][1] * a.midv[l][1]);
     }
     if (!troubled) continue;
-    flag |= (uint8_t)(1u << f);
+    flag |= 1u << f;
     double dl[3], pos = 0.0, neg = 0.0;
+#pragma unroll
     for (int l = 0; l < 3; l++) {
       dl[l] = lim_minmod(d[l], a.nu * D[l][f]);
       pos += dl[l] > 0.0 ? dl[l] : 0.0;
@@ -399,16 +418,37 @@ DG_HD void limit_element(const LimiterArgs& a, int i) {
     if (pos != neg) {
       const double tp = pos > 0.0 ? (neg < pos ? neg / pos : 1.0) : 0.0;
       const double tn = neg > 0.0 ? (pos < neg ? pos / neg : 1.0) : 0.0;
+#pragma unroll
       for (int l = 0; l < 3; l++) dl[l] = dl[l] > 0.0 ? tp * dl[l] : tn * dl[l];
     }
-    const double n1 = fma(a.pinv[0][2], dl[2], fma(a.pinv[0][1], dl[1], a.pinv[0][0] * dl[0]));
-    const double n2 = fma(a.pinv[1][2], dl[2], fma(a.pinv[1][1], dl[1], a.pinv[1][0] * dl[0]));
-    a.U[mixed_index(ix, p, i, f * np + 1)] = n1;
-    a.U[mixed_index(ix, p, i, f * np + 2)] = n2;
-    for (int k = 3; k < np; k++) a.U[mixed_index(ix, p, i, f * np + k)] = 0.0;
+    U[f][1] = fma(a.pinv[0][2], dl[2], fma(a.pinv[0][1], dl[1], a.pinv[0][0] * dl[0]));
+    U[f][2] = fma(a.pinv[1][2], dl[2], fma(a.pinv[1][1], dl[1], a.pinv[1][0] * dl[0]));
+#pragma unroll
+    for (int k = 3; k < NP; k++) U[f][k] = 0.0;
   }
-  if (LAW::NF == 4 && positivity_element(a, ix, i, p, np)) flag |= 16;
-  a.flags[i] = flag;
+  unsigned dirty = flag;
+  if constexpr (NF == 4) {
+    if (positivity_element<P>(a, U)) {
+      flag |= 16;
+      dirty = 15;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; f++)
+    if (dirty & (1u << f))
+#pragma unroll
+      for (int k = 1; k < NP; k++) a.U[mixed_index(ix, P, i, f * NP + k)] = U[f][k];
+  a.flags[i] = (uint8_t)flag;
+}
+
+template <class LAW>
+DG_HD void limit_element(const LimiterArgs& a, int i) {
+  switch (mixed_order_of(a, i)) {
+    case 1: limit_element_p<LAW, 1>(a, i); break;
+    case 2: limit_element_p<LAW, 2>(a, i); break;
+    case 3: limit_element_p<LAW, 3>(a, i); break;
+    default: limit_element_p<LAW, 4>(a, i); break;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -432,11 +472,7 @@ struct SensorArgs {
 };
 
 DG_HD void sensor_element(const SensorArgs& a, int i) {
-  MixedArgs ix{};
-  for (int p = 0; p < 6; p++) {
-    ix.bstart[p] = a.bstart[p];
-    ix.boff[p] = a.boff[p];
-  }
+  const SensorArgs& ix = a;
   const int p = mixed_order_of(ix, i), np = (p + 1) * (p + 2) / 2, npl = p * (p + 1) / 2;
   double all = 0.0, top = 0.0, high = 0.0;
   for (int k = 0; k < np; k++) {

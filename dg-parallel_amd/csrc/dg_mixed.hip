@@ -56,10 +56,11 @@ __global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_dt(MixedArgs a, double cf
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-template <class LAW>
+// one launch per order bucket: each order's limiter has its own registers
+template <class LAW, int P>
 __global__ __launch_bounds__(256) void tri_limit(LimiterArgs a) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < a.E) limit_element<LAW>(a, i);
+  const int i = a.bstart[P] + blockIdx.x * 256 + threadIdx.x;
+  if (i < a.bstart[P + 1]) limit_element_p<LAW, P>(a, i);
 }
 
 __global__ __launch_bounds__(256) void tri_sensor(SensorArgs a) {
@@ -90,12 +91,20 @@ int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipSt
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+template <class LAW>
+static void launch_limiter_law(const LimiterArgs& a, hipStream_t s) {
+  void (*kern[4])(LimiterArgs) = {tri_limit<LAW, 1>, tri_limit<LAW, 2>, tri_limit<LAW, 3>, tri_limit<LAW, 4>};
+  for (int p = 1; p <= MAXP_TRI; p++) {
+    const int n = a.bstart[p + 1] - a.bstart[p];
+    if (n > 0) hipLaunchKernelGGL(kern[p - 1], dim3((n + 255) / 256), dim3(256), 0, s, a);
+  }
+}
+
 int launch_limiter(const LimiterArgs& a, int law, hipStream_t s) {
-  dim3 grid((a.E + 255) / 256), block(256);
   if (law == LAW_SCALAR)
-    hipLaunchKernelGGL(tri_limit<ScalarLawD>, grid, block, 0, s, a);
+    launch_limiter_law<ScalarLawD>(a, s);
   else if (law == LAW_EULER2D)
-    hipLaunchKernelGGL(tri_limit<EulerLawD>, grid, block, 0, s, a);
+    launch_limiter_law<EulerLawD>(a, s);
   else
     return -1;
   return hipGetLastError() == hipSuccess ? 0 : -2;
