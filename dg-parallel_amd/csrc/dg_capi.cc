@@ -33,6 +33,10 @@ struct dg_ctx {
   int law = 0, prec = 0, use_graph = 0, device = 0;
   double hmin = 0.0;  // tetrahedra: shortest edge (the Maxwell dt needs no state)
   int kernel = 0;  // 0: tiled LDS-staged stage kernel, 1: one element per work-item
+  // launch schedule (dg_set_schedule / dg_autotune): tile shape of the tiled
+  // kernels, workgroups of the persistent mixed-order launch (0: default)
+  int shape = 0, mix_grid = 0;
+  size_t part_cap = 0;  // doubles in part
   int adj_nd = 0;  // adjoint kernel: flux tangent directions per pass (0: default)
   LawParams lp{};
   size_t ndof = 0;
@@ -221,7 +225,7 @@ static dg_status ctx_init_tet(dg_ctx* c, const dg_mesh_desc* m, const dg_law_des
   if ((s = dalloc(c, &c->Un, nd)) || (s = dalloc(c, &c->acc, nd)) || (s = dalloc(c, &c->R, nd)) ||
       (s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd)) || (s = dalloc(c, &c->mg, (size_t)9 * E)) ||
       (s = dalloc(c, &c->detj, E)) || (s = dalloc(c, &c->size, E)) || (s = dalloc(c, &c->link, (size_t)4 * E)) ||
-      (s = dalloc(c, &c->tab, S.tab.size())) || (s = dalloc(c, &c->part, (size_t)tet_tiles(E) + 1)))
+      (s = dalloc(c, &c->tab, S.tab.size())) || (s = dalloc(c, &c->part, c->part_cap = (size_t)tet_tiles(E) + 1)))
     return s;
   if (hipMemcpy(c->mg, S.mg.data(), S.mg.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->detj, S.detj.data(), E * 8, hipMemcpyHostToDevice) != hipSuccess ||
@@ -300,7 +304,7 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
       (s = dalloc(c, &c->mg, (size_t)4 * E)) || (s = dalloc(c, &c->detj, E)) || (s = dalloc(c, &c->size, E)) ||
       (s = dalloc(c, &c->link, (size_t)3 * E)) || (s = dalloc(c, &c->tab, h_tab.size())) ||
       (s = dalloc(c, &c->tabf, h_tabf.size())) || (s = dalloc(c, &c->vtx, h_vtx.size())) ||
-      (s = dalloc(c, &c->part, part_size(S))))
+      (s = dalloc(c, &c->part, c->part_cap = part_size(S))))
     return bail(s);
   if (c->prec == DG_PREC_FP64) {
     if ((s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd))) return bail(s);
@@ -383,6 +387,8 @@ dg_status dg_destroy(dg_ctx* c) {
 }
 
 static bool ready(const dg_ctx* c) { return c && c->stream && c->Un; }
+static bool schedule_ok(const dg_ctx* c, const dg_schedule& s);
+static void apply_schedule(dg_ctx* c, const dg_schedule& s);
 
 // caller layout [e][f][k] (Np of each element's order, elements in caller order)
 // <-> bucketed structure of arrays; visit(i, caller index, soa index)
@@ -500,7 +506,7 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
     t.scheme = a.scheme;
     t.stage = a.stage;
     t.resid_part = a.resid_part;
-    return launch_tet_stage(t, c->stream);
+    return launch_tet_stage(t, c->shape, c->stream);
   }
   if (c->mixed) {
     MixedArgs m = mixed_args(c);
@@ -513,10 +519,10 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
     m.scheme = a.scheme;
     m.stage = a.stage;
     m.resid_part = a.resid_part;
-    r = launch_mixed_stage(m, c->law, c->stream);
+    r = launch_mixed_stage(m, c->law, c->mix_grid > 0 ? c->mix_grid : mixed_grid(c->nchunks), c->stream);
   } else {
     r = c->kernel == 1 ? launch_tri_stage(a, c->law, variant, c->stream)
-                       : launch_tri_stage_tiled(a, c->law, variant, c->stream);
+                       : launch_tri_stage_tiled(a, c->law, variant | c->shape << 2, c->stream);
   }
   if (r == 0 && c->lim_on && a.scheme != MODE_RESIDUAL)
     r = launch_limiter(limiter_args(c, (double*)a.Uout), c->law, c->stream);
@@ -525,7 +531,7 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
 
 static int stage_blocks(const dg_ctx* c) {
   if (c->dim == 3) return tet_tiles(c->E);
-  if (c->mixed) return mixed_grid(c->nchunks);
+  if (c->mixed) return c->mix_grid > 0 ? c->mix_grid : mixed_grid(c->nchunks);
   return c->kernel == 1 ? tri_stage_blocks(c->E) : tri_tiled_blocks(c->E);
 }
 
@@ -957,6 +963,8 @@ static dg_status rebucket(dg_ctx* c, const std::vector<int32_t>& neword) {
   t->lim_m = c->lim_m;
   t->lim_nu = c->lim_nu;
   t->adj_nd = c->adj_nd;
+  const dg_schedule sc{c->shape, c->mix_grid, {0, 0, 0, 0, 0, 0}};
+  if (schedule_ok(t, sc)) apply_schedule(t, sc);  // else the new buckets start from the default
   std::swap(*c, *t);
   free_all(t);
   delete t;
@@ -1221,6 +1229,177 @@ dg_status dg_adjoint_timing(const dg_ctx* c, float* fwd_ms, float* bwd_ms) {
   if (!c || !fwd_ms || !bwd_ms) return DG_ERR_ARG;
   *fwd_ms = c->adj_fwd_ms;
   *bwd_ms = c->adj_bwd_ms;
+  return DG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch schedules and the tuner
+
+static bool schedule_ok(const dg_ctx* c, const dg_schedule& s) {
+  const bool tiled = c->dim == 3 || (!c->mixed && c->kernel == 0);
+  if (s.shape < 0 || s.shape > 1 || (s.shape && !tiled)) return false;
+  if (s.grid < 0 || (s.grid && !c->mixed)) return false;
+  if (s.grid && ((size_t)s.grid + 1 > c->part_cap || s.grid > c->nchunks)) return false;
+  return true;
+}
+
+static void apply_schedule(dg_ctx* c, const dg_schedule& s) {
+  if (c->shape != s.shape || c->mix_grid != s.grid) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured with the old launch
+    c->gexec = nullptr;
+  }
+  c->shape = s.shape;
+  c->mix_grid = s.grid;
+}
+
+dg_status dg_get_schedule(const dg_ctx* c, dg_schedule* out) {
+  if (!c || !out) return DG_ERR_ARG;
+  *out = dg_schedule{};
+  out->shape = c->shape;
+  out->grid = c->mix_grid;
+  return DG_OK;
+}
+
+dg_status dg_set_schedule(dg_ctx* c, const dg_schedule* s) {
+  if (!ready(c) || !s) return fail(c, DG_ERR_ARG, "dg_set_schedule: bad argument");
+  if (!schedule_ok(c, *s)) return fail(c, DG_ERR_ARG, "schedule (shape %d, grid %d) does not fit this context", s->shape, s->grid);
+  apply_schedule(c, *s);
+  return DG_OK;
+}
+
+// the candidates of this context: tile shapes of the tiled kernels, workgroup
+// counts of the persistent mixed-order launch
+static std::vector<dg_schedule> tune_candidates(const dg_ctx* c) {
+  std::vector<dg_schedule> v;
+  if (c->mixed) {
+    // the default (one workgroup per chunk, at most 2048), multiples of the CU
+    // count below it, and a half and a quarter of the full count
+    const int full = std::min<int>(c->nchunks, (int)c->part_cap - 1), def = mixed_grid(c->nchunks);
+    std::vector<int> grids = {0};
+    for (int g : {256, 512, 1024, 2048, 4096, full / 4, full / 2, full})
+      if (g >= 1 && g <= full && g != def && std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
+    for (int g : grids) {
+      dg_schedule s{};
+      s.grid = g;
+      if (schedule_ok(c, s)) v.push_back(s);
+    }
+  } else {
+    for (int sh = 0; sh < 2; sh++) {
+      dg_schedule s{};
+      s.shape = sh;
+      if (schedule_ok(c, s)) v.push_back(s);
+    }
+  }
+  return v;
+}
+
+static void tune_key(const dg_ctx* c, char* key, size_t n) {
+  hipDeviceProp_t prop{};
+  const char* arch = hipGetDeviceProperties(&prop, c->device) == hipSuccess ? prop.gcnArchName : "unknown";
+  char orders[64];
+  if (c->mixed)
+    snprintf(orders, sizeof(orders), "P1-4:%d,%d,%d,%d", c->bstart[2] - c->bstart[1], c->bstart[3] - c->bstart[2],
+             c->bstart[4] - c->bstart[3], c->bstart[5] - c->bstart[4]);
+  else
+    snprintf(orders, sizeof(orders), "P%d", c->P);
+  snprintf(key, n, "%s dim%d law%d flux%d %s prec%d E%d lim%d kernel%d", arch, c->dim, c->m_law.law, c->m_law.flux,
+           orders, c->prec, c->E, c->lim_on, c->kernel);
+  for (char* q = key; *q; q++)
+    if (*q == '\t' || *q == '\n') *q = ' ';
+}
+
+// the last entry of the cache for key, if any
+static bool tune_cache_lookup(const char* path, const char* key, dg_schedule* out) {
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  char line[512];
+  bool found = false;
+  const size_t kl = strlen(key);
+  while (fgets(line, sizeof(line), f)) {
+    if (strncmp(line, key, kl) != 0 || line[kl] != '\t') continue;
+    dg_schedule s{};
+    if (sscanf(line + kl + 1, "%d %d", &s.shape, &s.grid) == 2) {
+      *out = s;
+      found = true;
+    }
+  }
+  fclose(f);
+  return found;
+}
+
+dg_status dg_autotune(dg_ctx* c, int scheme, double dt, int steps, const char* cache, int flags, dg_tune_report* rep) {
+  if (!ready(c) || steps <= 0 || !(dt > 0) || !rep) return fail(c, DG_ERR_ARG, "dg_autotune: bad argument");
+  dg_status st = check_scheme(c, scheme);
+  if (st) return st;
+  HIPCHECK(c, hipSetDevice(c->device));
+  std::memset(rep, 0, sizeof(*rep));
+  tune_key(c, rep->key, sizeof(rep->key));
+  const std::vector<dg_schedule> cand = tune_candidates(c);
+  if (cache && !(flags & DG_TUNE_FORCE)) {
+    dg_schedule s{};
+    if (tune_cache_lookup(cache, rep->key, &s) && schedule_ok(c, s)) {
+      apply_schedule(c, s);
+      rep->best = s;
+      rep->from_cache = 1;
+      return DG_OK;
+    }
+  }
+  // the state (and the limiter flags) before tuning, restored after every run
+  double* save = nullptr;
+  uint8_t* fsave = nullptr;
+  if ((st = dalloc(c, &save, c->ndof))) return st;
+  if (c->flags && (st = dalloc(c, &fsave, (size_t)c->E))) {
+    release(c, save);
+    return st;
+  }
+  auto restore = [&]() -> hipError_t {
+    hipError_t e = hipMemcpyAsync(c->Un, save, c->ndof * 8, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess && fsave) e = hipMemcpyAsync(c->flags, fsave, (size_t)c->E, hipMemcpyDeviceToDevice, c->stream);
+    return e;
+  };
+  hipError_t he = hipMemcpyAsync(save, c->Un, c->ndof * 8, hipMemcpyDeviceToDevice, c->stream);
+  if (he == hipSuccess && fsave) he = hipMemcpyAsync(fsave, c->flags, (size_t)c->E, hipMemcpyDeviceToDevice, c->stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (he == hipSuccess) he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  const dg_schedule before{c->shape, c->mix_grid, {0, 0, 0, 0, 0, 0}};
+  int best = -1;
+  for (size_t i = 0; he == hipSuccess && st == DG_OK && i < cand.size() && i < DG_TUNE_MAX_CAND; i++) {
+    apply_schedule(c, cand[i]);
+    // one untimed step (builds the graph when graphs are on), then the best of 3
+    if ((st = run_steps(c, scheme, dt, 1)) || (he = restore()) != hipSuccess) break;
+    float bestms = 0.f;
+    for (int r = 0; r < 3 && he == hipSuccess && st == DG_OK; r++) {
+      float ms = 0.f;
+      if ((he = hipEventRecord(e0, c->stream)) != hipSuccess || (st = run_steps(c, scheme, dt, steps)) ||
+          (he = hipEventRecord(e1, c->stream)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess ||
+          (he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess || (he = restore()) != hipSuccess)
+        break;
+      if (r == 0 || ms < bestms) bestms = ms;
+    }
+    rep->cand[rep->ncand] = cand[i];
+    rep->ms_per_step[rep->ncand] = bestms / steps;
+    if (best < 0 || rep->ms_per_step[rep->ncand] < rep->ms_per_step[best]) best = rep->ncand;
+    rep->ncand++;
+  }
+  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  release(c, save);
+  if (fsave) release(c, fsave);
+  if (st != DG_OK || he != hipSuccess || best < 0) {
+    apply_schedule(c, before);
+    if (st != DG_OK) return st;
+    return fail(c, DG_ERR_HIP, "dg_autotune: %s", best < 0 ? "no candidate" : hipGetErrorString(he));
+  }
+  apply_schedule(c, rep->cand[best]);
+  rep->best = rep->cand[best];
+  if (cache) {
+    FILE* f = fopen(cache, "a");
+    if (!f) return fail(c, DG_ERR_ARG, "dg_autotune: cannot append to %s", cache);
+    fprintf(f, "%s\t%d %d\n", rep->key, rep->best.shape, rep->best.grid);
+    fclose(f);
+  }
   return DG_OK;
 }
 

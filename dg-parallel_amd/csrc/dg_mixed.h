@@ -516,7 +516,7 @@ DG_HD double dt_element(const MixedArgs& a, int i, double cfl) {
 
 // launches (dg_mixed.hip); part holds one value per workgroup of the launch
 int mixed_grid(int nchunks);
-int launch_mixed_stage(const MixedArgs& a, int law, hipStream_t s);
+int launch_mixed_stage(const MixedArgs& a, int law, int grid, hipStream_t s);
 int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipStream_t s);
 int launch_limiter(const LimiterArgs& a, int law, hipStream_t s);
 int launch_sensor(const SensorArgs& a, hipStream_t s);

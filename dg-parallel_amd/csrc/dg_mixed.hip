@@ -68,9 +68,10 @@ __global__ __launch_bounds__(256) void tri_sensor(SensorArgs a) {
   if (i < a.E) sensor_element(a, i);
 }
 
-int launch_mixed_stage(const MixedArgs& a, int law, hipStream_t s) {
-  if (a.nchunks <= 0) return -1;
-  dim3 grid(mixed_grid(a.nchunks)), block(MIX_BLOCK);
+// grid: workgroups of the persistent launch (1 .. nchunks)
+int launch_mixed_stage(const MixedArgs& a, int law, int ngrid, hipStream_t s) {
+  if (a.nchunks <= 0 || ngrid <= 0 || ngrid > a.nchunks) return -1;
+  dim3 grid(ngrid), block(MIX_BLOCK);
   if (law == LAW_SCALAR)
     hipLaunchKernelGGL(tri_mixed_stage<ScalarLawD>, grid, block, 0, s, a);
   else if (law == LAW_EULER2D)

@@ -106,23 +106,30 @@ DG_HD double stage_value_hd(int sc, double v, double un, double us, double* acc,
   return o;
 }
 
-template <int P, class LAW, typename CT, typename IT, typename OT>
+// NW waves share a tile of 64 elements (NW = 4: 256 threads; NW = 8: 512
+// threads, the same LDS and half the rows per thread); the tile-shape choice of
+// the tuner.
+template <int P, class LAW, typename CT, typename IT, typename OT, int NW = 4>
 struct TiledStage {
   static constexpr int NP = np_tri(P), NQ = nqv_tri(P), NQF = nqf_tri(P), NF = LAW::NF;
   static constexpr int T = TILE_ELEMS;
+  static constexpr int THREADS = NW * T;
   static constexpr int NROW = NF * NP;
   static constexpr int GW = NQ * NF * 2;   // volume-flux rows
   static constexpr int FW = 3 * NQF * NF;  // trace / face-flux rows
   static constexpr int WORK = GW > 2 * FW ? GW : 2 * FW;
-  // output rows of a wave: with four or more fields a wave owns whole fields
-  // (f = wave, wave+4, ..; every mode k), else it owns modes k = wave, wave+4, ..
-  // of the single field -- either way the rows of one field share every LDS read
+  // output rows of a wave: with four or more fields the waves form FG field
+  // groups (f = w % FG, + FG, ..) times KG mode groups (k = w / FG, + KG, ..);
+  // a single field is split by modes only -- either way the rows of one field
+  // share every LDS read
   static constexpr bool BY_FIELD = NF >= 4;
-  static constexpr int FPW = BY_FIELD ? (NF + 3) / 4 : NF;
-  static constexpr int KPW = BY_FIELD ? NP : (NP + 3) / 4;
-  static constexpr int QPW = (NQ + 3) / 4;
-  static constexpr int CPW = (3 * NF + 3) / 4;
-  static constexpr int FPPW = (3 * NQF + 3) / 4;
+  static constexpr int FG = BY_FIELD ? (NW < 4 ? NW : 4) : 1;
+  static constexpr int KG = NW / FG;
+  static constexpr int FPW = (NF + FG - 1) / FG;
+  static constexpr int KPW = (NP + KG - 1) / KG;
+  static constexpr int QPW = (NQ + NW - 1) / NW;
+  static constexpr int CPW = (3 * NF + NW - 1) / NW;
+  static constexpr int FPPW = (3 * NQF + NW - 1) / NW;
 
   struct Smem {
     CT Us[NROW][T];
@@ -130,7 +137,7 @@ struct TiledStage {
     CT Ns[3][3][T];  // per face: unit outward normal (2) and edge length
     int Ls[3][T];
     CT W[WORK][T];
-    double red[TILE_THREADS];
+    double red[THREADS];
   };
 
   // per-thread output rows of its own element
@@ -142,8 +149,8 @@ struct TiledStage {
   DG_HD static const CT* table(const StageArgs& a) {
     return sizeof(CT) == 8 ? (const CT*)(const void*)a.tab : (const CT*)(const void*)a.tabf;
   }
-  DG_HD static int row_f(int wave, int fi) { return BY_FIELD ? wave + 4 * fi : fi; }
-  DG_HD static int row_k(int wave, int ki) { return BY_FIELD ? ki : wave + 4 * ki; }
+  DG_HD static int row_f(int wave, int fi) { return wave % FG + FG * fi; }
+  DG_HD static int row_k(int wave, int ki) { return wave / FG + KG * ki; }
 
   // ---- A: stage-input tile -> LDS (rows split over the waves); own element's
   //      geometry and links -> registers ----------------------------------------
@@ -152,10 +159,9 @@ struct TiledStage {
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
     const IT* Uin = (const IT*)a.Uin;
-    for (int r = wave; r < NROW; r += 4) s.Us[r][t] = valid ? CT(Uin[(long)r * E + e]) : CT(r == 0 ? 1 : 0);
-    if (wave < 4) s.Ms[wave][t] = valid ? CT(a.el.mg[(long)wave * E + e]) : CT(wave == 0 || wave == 3 ? 1 : 0);
-    if (wave < 3) {
-      const int l = wave;
+    for (int r = wave; r < NROW; r += NW) s.Us[r][t] = valid ? CT(Uin[(long)r * E + e]) : CT(r == 0 ? 1 : 0);
+    for (int r = wave; r < 4; r += NW) s.Ms[r][t] = valid ? CT(a.el.mg[(long)r * E + e]) : CT(r == 0 || r == 3 ? 1 : 0);
+    for (int l = wave; l < 3; l += NW) {
       s.Ls[l][t] = valid ? a.el.link[(long)l * E + e] : -BC_FARFIELD;
       // outward normal x edge length = |J| J^-T nhat |fhat|, with nhat |fhat| of
       // the reference edges (0,-1), (1,1), (-1,0)
@@ -186,7 +192,7 @@ struct TiledStage {
     const CT* tb = table(a);
 #pragma unroll 1
     for (int j = 0; j < QPW; j++) {
-      const int q = wave + 4 * j;
+      const int q = wave + NW * j;
       if (q < NQ) {
         CT u[NF];
 #pragma unroll
@@ -263,7 +269,7 @@ struct TiledStage {
     int lk[CPW];
 #pragma unroll
     for (int j = 0; j < CPW; j++) {
-      const int it = wave + 4 * j;
+      const int it = wave + NW * j;
       const int l = it < 3 * NF ? it / NF : 0, f = it < 3 * NF ? it - l * NF : 0;
       lk[j] = s.Ls[l][t];
       const int src = (valid && lk[j] >= 0) ? (lk[j] >> 3) : (valid ? e : 0);
@@ -272,7 +278,7 @@ struct TiledStage {
     }
 #pragma unroll
     for (int j = 0; j < CPW; j++) {
-      const int it = wave + 4 * j;
+      const int it = wave + NW * j;
       if (it < 3 * NF) {
         const int l = it / NF, f = it - l * NF;
         const int link = lk[j];
@@ -311,7 +317,7 @@ struct TiledStage {
     const CT* tb = table(a);
 #pragma unroll 1
     for (int j = 0; j < FPPW; j++) {
-      const int it = wave + 4 * j;
+      const int it = wave + NW * j;
       if (it < 3 * NQF) {
         const int l = it / NQF, q = it - l * NQF;
         const int link = s.Ls[l][t];

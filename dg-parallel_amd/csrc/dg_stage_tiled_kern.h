@@ -1,0 +1,93 @@
+// The tiled triangle stage kernel (phases in dg_stage_tiled.h) and its launcher
+// for one tile shape NW (waves per 64-element tile); each shape's
+// instantiations live in a translation unit of their own.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dg_stage_tiled.h"
+
+namespace dgp {
+
+// blockIdx -> tile: consecutive tiles go to one XCD (workgroups are dealt to the
+// 8 XCDs round robin), so neighbouring tiles share that XCD's L2
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  int full = (nb / 8) * 8;
+  if (b >= full) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
+template <int P, class LAW, typename CT, typename IT, typename OT, int NW>
+__global__ __launch_bounds__(NW * 64) void tri_stage_tiled(StageArgs a) {
+  using K = TiledStage<P, LAW, CT, IT, OT, NW>;
+  __shared__ typename K::Smem sm;
+  typename K::Regs g;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  K::phaseA(a, sm, tid, tile, g);
+  __syncthreads();
+  K::phaseB(a, sm, tid, g);
+  __syncthreads();
+  K::phaseD1(a, sm, tid, g);
+  __syncthreads();
+  K::phaseC1(a, sm, tid, tile, g);
+  __syncthreads();
+  K::phaseC2(a, sm, tid, g);
+  __syncthreads();
+  K::phaseD2(a, sm, tid, tile, g);
+  if (a.resid_part) {
+    // deterministic tree reduction of the step residual norm
+    sm.red[tid] = g.rsum;
+    __syncthreads();
+    for (int s = K::THREADS / 2; s > 0; s >>= 1) {
+      if (tid < s) sm.red[tid] += sm.red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) a.resid_part[tile] = sm.red[0];
+  }
+}
+
+template <int P, class LAW, int NW>
+static int launch_tiled_p(const StageArgs& a, int prec, hipStream_t s) {
+  dim3 grid((a.E + TILE_ELEMS - 1) / TILE_ELEMS), block(NW * 64);
+  switch (prec) {
+    case 0:
+      hipLaunchKernelGGL((tri_stage_tiled<P, LAW, double, double, double, NW>), grid, block, 0, s, a);
+      break;
+    case 1:
+      hipLaunchKernelGGL((tri_stage_tiled<P, LAW, float, double, float, NW>), grid, block, 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((tri_stage_tiled<P, LAW, float, float, float, NW>), grid, block, 0, s, a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((tri_stage_tiled<P, LAW, float, float, double, NW>), grid, block, 0, s, a);
+      break;
+    default:
+      return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <class LAW, int NW>
+static int launch_tiled_law(const StageArgs& a, int prec, hipStream_t s) {
+  switch (a.P) {
+    case 1: return launch_tiled_p<1, LAW, NW>(a, prec, s);
+    case 2: return launch_tiled_p<2, LAW, NW>(a, prec, s);
+    case 3: return launch_tiled_p<3, LAW, NW>(a, prec, s);
+    case 4: return launch_tiled_p<4, LAW, NW>(a, prec, s);
+  }
+  return -1;
+}
+
+template <int NW>
+int launch_tiled_nw(const StageArgs& a, int law, int prec, hipStream_t s) {
+  if (law == LAW_SCALAR) return launch_tiled_law<ScalarLawD, NW>(a, prec, s);
+  if (law == LAW_EULER2D) return launch_tiled_law<EulerLawD, NW>(a, prec, s);
+  return -1;
+}
+
+extern template int launch_tiled_nw<4>(const StageArgs&, int, int, hipStream_t);
+extern template int launch_tiled_nw<8>(const StageArgs&, int, int, hipStream_t);
+
+}  // namespace dgp

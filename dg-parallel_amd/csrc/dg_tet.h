@@ -17,7 +17,7 @@
 //   coefficient rows once B is done, which keeps LDS at 80 KB: two workgroups
 //   per CU):
 //   C1  own and neighbour traces at the face points (12 tasks, field x side,
-//       3 per wave), each from global coefficients (own rows are coalesced,
+//       dealt over the waves), each from global coefficients (own rows are coalesced,
 //       neighbour rows gathered) through the trace table of the local face, and
 //       for the neighbour of its local face and the face's vertex permutation;
 //   C2  numerical flux (PEC faces: ghost E -> -E, H -> H), times the face area
@@ -74,13 +74,15 @@ DG_HD void tet_ref_normal(int l, double N[3]) {
   N[2] = l == 2 ? 0.5 : l == 0 ? -0.5 : 0.0;
 }
 
-template <int P>
+// NW waves share a tile of 64 elements (4: 256 threads, 8: 512 threads with
+// the same LDS and fewer rows per thread)
+template <int P, int NW = 4>
 struct TiledTet {
   using LAW = MaxwellLawD;
   static constexpr int NP = np_tet(P), NQF = nqf_tet(P), NF = 6;
   static constexpr int T = TILE_ELEMS;
+  static constexpr int THREADS = NW * T;
   static constexpr int NROW = NF * NP;
-  static constexpr int RPW = (NROW + 3) / 4;
   static constexpr int FW = NQF * NF;
 
   struct Traces {
@@ -91,12 +93,12 @@ struct TiledTet {
     union {
       double Us[NROW][T];  // phases A-B
       Traces W;            // phases C1-L
-      double red[TILE_THREADS];
+      double red[THREADS];
     };
     int Ls[4][T];
   };
   struct Regs {
-    double R[NF * ((NP + 3) / 4)];
+    double R[NF * ((NP + NW - 1) / NW)];
     double rsum;
   };
 
@@ -104,8 +106,8 @@ struct TiledTet {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
-    for (int r = wave; r < NROW; r += 4) s.Us[r][t] = valid ? a.Uin[(long)r * E + e] : 0.0;
-    s.Ls[wave][t] = valid ? a.link[(long)wave * E + e] : -BC_PEC;
+    for (int r = wave; r < NROW; r += NW) s.Us[r][t] = valid ? a.Uin[(long)r * E + e] : 0.0;
+    for (int l = wave; l < 4; l += NW) s.Ls[l][t] = valid ? a.link[(long)l * E + e] : -BC_PEC;
   }
 
   // |J| J^-1 of the lane's element (the identity past the end of the mesh)
@@ -118,11 +120,11 @@ struct TiledTet {
       for (int d = 0; d < 3; d++) m[r][d] = valid ? a.mg[(long)(r * 3 + d) * E + e] : (r == d ? 1.0 : 0.0);
   }
 
-  // Rows are owned by k-slot: wave w holds the modes k = w + 4 ks (ks < KS) of
+  // Rows are owned by k-slot: wave w holds the modes k = w + NW ks (ks < KS) of
   // every field, so row (f, k) is register j = f*KS + ks and the table rows a
   // wave reads depend on ks only (wave-uniform, scalar loads).
-  static constexpr int KS = (NP + 3) / 4;
-  DG_HD static int kmode(int wave, int ks) { return wave + 4 * ks; }
+  static constexpr int KS = (NP + NW - 1) / NW;
+  DG_HD static int kmode(int wave, int ks) { return wave + NW * ks; }
 
   // volume term, operator form: for each of the wave's modes k, the 3 x 6 products
   // (K_r U_g)[k] once, then every field's row from its two (d, g) pairs
@@ -175,7 +177,7 @@ struct TiledTet {
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
     const int link = s.Ls[l][t];
-    for (int task = wave; task < 2 * NF; task += 4) {
+    for (int task = wave; task < 2 * NF; task += NW) {
       double v[NP];
       if (task < NF) {
         const int f = task;
@@ -228,7 +230,7 @@ struct TiledTet {
     for (int d = 0; d < 3; d++) N[d] = m[0][d] * Nr[0] + m[1][d] * Nr[1] + m[2][d] * Nr[2];
     const double area = dg_sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
     const double n[3] = {N[0] / area, N[1] / area, N[2] / area};
-    for (int q = wave; q < NQF; q += 4) {
+    for (int q = wave; q < NQF; q += NW) {
       double uL[NF], uR[NF], Fh[NF];
       for (int f = 0; f < NF; f++) {
         uL[f] = s.W.WL[q * NF + f][t];
@@ -289,7 +291,8 @@ struct TiledTet {
   }
 };
 
-int launch_tet_stage(const TetArgs& a, hipStream_t s);
+// shape: 0 for 4 waves per tile, 1 for 8
+int launch_tet_stage(const TetArgs& a, int shape, hipStream_t s);
 int tet_tiles(int E);
 
 }  // namespace dgp

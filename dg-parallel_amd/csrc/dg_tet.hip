@@ -11,9 +11,9 @@ __device__ __forceinline__ int xcd_tile_tet(int b, int nb) {
   return (b % 8) * (nb / 8) + b / 8;
 }
 
-template <int P>
-__global__ __launch_bounds__(TILE_THREADS) void tet_stage_tiled(TetArgs a) {
-  using K = TiledTet<P>;
+template <int P, int NW>
+__global__ __launch_bounds__(NW * 64) void tet_stage_tiled(TetArgs a) {
+  using K = TiledTet<P, NW>;
   __shared__ typename K::Smem sm;
   typename K::Regs g;
   const int tile = xcd_tile_tet(blockIdx.x, gridDim.x);
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(TILE_THREADS) void tet_stage_tiled(TetArgs a) {
     __syncthreads();
     sm.red[tid] = g.rsum;
     __syncthreads();
-    for (int s = TILE_THREADS / 2; s > 0; s >>= 1) {
+    for (int s = K::THREADS / 2; s > 0; s >>= 1) {
       if (tid < s) sm.red[tid] += sm.red[tid + s];
       __syncthreads();
     }
@@ -45,15 +45,24 @@ __global__ __launch_bounds__(TILE_THREADS) void tet_stage_tiled(TetArgs a) {
 
 int tet_tiles(int E) { return (E + TILE_ELEMS - 1) / TILE_ELEMS; }
 
-int launch_tet_stage(const TetArgs& a, hipStream_t s) {
-  dim3 grid(tet_tiles(a.E)), block(TILE_THREADS);
+template <int NW>
+static int launch_tet_nw(const TetArgs& a, hipStream_t s) {
+  dim3 grid(tet_tiles(a.E)), block(NW * 64);
   switch (a.P) {
-    case 1: hipLaunchKernelGGL(tet_stage_tiled<1>, grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(tet_stage_tiled<2>, grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(tet_stage_tiled<3>, grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((tet_stage_tiled<1, NW>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((tet_stage_tiled<2, NW>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((tet_stage_tiled<3, NW>), grid, block, 0, s, a); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_tet_stage(const TetArgs& a, int shape, hipStream_t s) {
+  switch (shape) {
+    case 0: return launch_tet_nw<4>(a, s);
+    case 1: return launch_tet_nw<8>(a, s);
+  }
+  return -1;
 }
 
 }  // namespace dgp

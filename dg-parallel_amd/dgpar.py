@@ -108,6 +108,24 @@ _adj_fwd = _sig("dg_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_double, ctype
 _adj_bwd = _sig("dg_adjoint_backward", ctypes.c_int, [_vp, _p_f64, _p_f64])
 _adj_timing = _sig("dg_adjoint_timing", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float),
                                                        ctypes.POINTER(ctypes.c_float)])
+class Schedule(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int), ("grid", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+
+
+TUNE_FORCE = 1
+TUNE_MAX_CAND = 16
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("best", Schedule), ("ncand", ctypes.c_int), ("cand", Schedule * TUNE_MAX_CAND),
+                ("ms_per_step", ctypes.c_float * TUNE_MAX_CAND), ("from_cache", ctypes.c_int),
+                ("key", ctypes.c_char * 256)]
+
+
+_get_schedule = _sig("dg_get_schedule", ctypes.c_int, [_vp, ctypes.POINTER(Schedule)])
+_set_schedule = _sig("dg_set_schedule", ctypes.c_int, [_vp, ctypes.POINTER(Schedule)])
+_autotune = _sig("dg_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_char_p,
+                                                ctypes.c_int, ctypes.POINTER(TuneReport)])
 _copy_probe = _sig("dg_copy_probe", ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_float)])
 _project = _sig("dg_project", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double])
 _l2_error = _sig("dg_l2_error", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64])
@@ -290,6 +308,28 @@ class DGAnalysis:
         ms = ctypes.c_float()
         self._check(_time_advance(self._h, scheme, dt, nsteps, ctypes.byref(ms)))
         return ms.value
+
+    # ---- launch schedule and tuner ----
+    def schedule(self):
+        """(shape, grid): tile shape of the tiled kernels (0: 4 waves per 64-element
+        tile, 1: 8 waves), workgroups of the persistent mixed-order launch (0: default)"""
+        sc = Schedule()
+        self._check(_get_schedule(self._h, ctypes.byref(sc)))
+        return sc.shape, sc.grid
+
+    def set_schedule(self, shape=0, grid=0):
+        self._check(_set_schedule(self._h, ctypes.byref(Schedule(shape, grid))))
+
+    def autotune(self, scheme, dt, steps=5, cache=None, force=False):
+        """time every candidate schedule on this mesh and state (restored after),
+        keep the fastest; returns {"best", "candidates": [(shape, grid, ms/step)],
+        "from_cache", "key"}"""
+        rep = TuneReport()
+        self._check(_autotune(self._h, scheme, dt, steps, cache.encode() if cache else None,
+                              TUNE_FORCE if force else 0, ctypes.byref(rep)))
+        return {"best": (rep.best.shape, rep.best.grid),
+                "candidates": [(rep.cand[i].shape, rep.cand[i].grid, rep.ms_per_step[i]) for i in range(rep.ncand)],
+                "from_cache": bool(rep.from_cache), "key": rep.key.decode()}
 
     # ---- derivatives (FP64) ----
     def residual_jvp(self, U, dU):

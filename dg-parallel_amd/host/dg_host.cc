@@ -207,6 +207,18 @@ std::vector<int32_t> DGAnalysis::adaptP(const dg_sensor_opts& opts) {
   return o;
 }
 
+dg_tune_report DGAnalysis::autotune(double dt, int steps, const char* cache, bool force) {
+  dg_tune_report r{};
+  check(dg_autotune(ctx_, integrator_->scheme(), dt, steps, cache, force ? DG_TUNE_FORCE : 0, &r));
+  return r;
+}
+
+dg_schedule DGAnalysis::schedule() const {
+  dg_schedule s{};
+  check(dg_get_schedule(ctx_, &s));
+  return s;
+}
+
 // ---------------------------------------------------------------------------
 double TimeIntegrator::advanceInTime(double t, double dt) {
   double r = 0.0;

@@ -256,6 +256,12 @@ class DGAnalysis {
   // sensor; re-buckets the elements by their new orders; returns the orders
   std::vector<int32_t> adaptP(const dg_sensor_opts& opts);
   std::vector<int32_t> orders() const;
+  // launch schedule of the stage kernel (no counterpart in the reference): the
+  // tuner times every candidate over `steps` steps of the integrator's scheme
+  // with step dt on the current state (restored after) and keeps the fastest;
+  // cache: optional text file of tuned entries per device and configuration
+  dg_tune_report autotune(double dt, int steps = 5, const char* cache = nullptr, bool force = false);
+  dg_schedule schedule() const;
   // the reference's time loop (commented out in DGAnalysis.cc:480-731): dt from the
   // CFL rule, N = ceil((TEND-TACT)/dt) steps of dt = (TEND-TACT)/N; returns N
   int run();

@@ -186,6 +186,46 @@ dg_status dg_adjoint_backward(dg_ctx* ctx, const double* dJdUT, double* dJdU0);
 /* device times of the last forward and backward calls (HIP events) */
 dg_status dg_adjoint_timing(const dg_ctx* ctx, float* fwd_ms, float* bwd_ms);
 
+/* ---- launch schedules and the tuner (no counterpart in the reference, whose
+ * pthreads split is fixed by parallel.cc) ----------------------------------
+ * A schedule is how the context's stage kernel is launched; it changes no
+ * state (every candidate computes each coefficient with the same operations in
+ * the same order); the step residual norm of dg_advance, a sum over workgroups
+ * and threads, follows the launch to rounding.
+ *   shape: the tiled triangle and tetrahedron kernels, 0 = 4 waves per tile of
+ *          64 elements (256 threads), 1 = 8 waves (512 threads, same LDS);
+ *   grid:  the persistent mixed-order launch, workgroups walking the chunk list
+ *          (0: one per chunk, at most 2048). */
+typedef struct {
+  int shape;
+  int grid;
+  int reserved[6];
+} dg_schedule;
+dg_status dg_get_schedule(const dg_ctx* ctx, dg_schedule* out);
+/* DG_ERR_ARG for a schedule this context cannot run */
+dg_status dg_set_schedule(dg_ctx* ctx, const dg_schedule* sched);
+
+enum { DG_TUNE_FORCE = 1 };  /* time the candidates even when the cache has an entry */
+#define DG_TUNE_MAX_CAND 16
+typedef struct {
+  dg_schedule best;
+  int ncand;
+  dg_schedule cand[DG_TUNE_MAX_CAND];
+  float ms_per_step[DG_TUNE_MAX_CAND];  /* best of the timed repetitions, device time */
+  int from_cache;                       /* 1: best read from the cache, nothing timed */
+  char key[256];                        /* the configuration key of the cache */
+} dg_tune_report;
+/* Times each candidate schedule over `steps` steps of `scheme` with step dt on
+ * this context's mesh and current state (saved before and restored after; the
+ * limiter, when on, is part of every step), keeps the fastest and reports every
+ * candidate.  cache (may be NULL): a text file of "key<TAB>shape grid" lines,
+ * the key naming the device architecture and the configuration (dimension, law,
+ * flux, orders, precision, elements, limiter, kernel); an entry for this key is
+ * applied without timing unless flags has DG_TUNE_FORCE, and a timed result is
+ * appended. */
+dg_status dg_autotune(dg_ctx* ctx, int scheme, double dt, int steps, const char* cache, int flags,
+                      dg_tune_report* report);
+
 /* Measurement helper (no counterpart in the reference): one streaming copy of
  * n_doubles between two fresh device buffers; the reference point that scales
  * the HBM byte counters of the profiler.  Returns the kernel time. */

@@ -19,21 +19,22 @@ using namespace dgp;
 
 namespace {
 
-template <int P, class LAW, typename CT, typename IT, typename OT>
+template <int P, class LAW, typename CT, typename IT, typename OT, int NW>
 void run_tiles(const StageArgs& a) {
-  using K = TiledStage<P, LAW, CT, IT, OT>;
+  using K = TiledStage<P, LAW, CT, IT, OT, NW>;
+  constexpr int TH = K::THREADS;
   std::unique_ptr<typename K::Smem> sm(new typename K::Smem());
-  std::vector<typename K::Regs> g(TILE_THREADS);
+  std::vector<typename K::Regs> g(TH);
   const int ntiles = (a.E + TILE_ELEMS - 1) / TILE_ELEMS;
   for (int tile = 0; tile < ntiles; tile++) {
     std::memset(sm.get(), 0, sizeof(typename K::Smem));
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, *sm, t, tile, g[t]);
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, *sm, t, g[t]);
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseD1(a, *sm, t, g[t]);
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, *sm, t, tile, g[t]);
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t, g[t]);
+    for (int t = 0; t < TH; t++) K::phaseA(a, *sm, t, tile, g[t]);
+    for (int t = 0; t < TH; t++) K::phaseB(a, *sm, t, g[t]);
+    for (int t = 0; t < TH; t++) K::phaseD1(a, *sm, t, g[t]);
+    for (int t = 0; t < TH; t++) K::phaseC1(a, *sm, t, tile, g[t]);
+    for (int t = 0; t < TH; t++) K::phaseC2(a, *sm, t, g[t]);
     double r = 0;
-    for (int t = 0; t < TILE_THREADS; t++) {
+    for (int t = 0; t < TH; t++) {
       K::phaseD2(a, *sm, t, tile, g[t]);
       r += g[t].rsum;
     }
@@ -41,30 +42,40 @@ void run_tiles(const StageArgs& a) {
   }
 }
 
-template <class LAW, typename CT, typename IT, typename OT>
+template <class LAW, typename CT, typename IT, typename OT, int NW>
 int run_law(const StageArgs& a) {
   switch (a.P) {
-    case 1: run_tiles<1, LAW, CT, IT, OT>(a); return 0;
-    case 2: run_tiles<2, LAW, CT, IT, OT>(a); return 0;
-    case 3: run_tiles<3, LAW, CT, IT, OT>(a); return 0;
-    case 4: run_tiles<4, LAW, CT, IT, OT>(a); return 0;
+    case 1: run_tiles<1, LAW, CT, IT, OT, NW>(a); return 0;
+    case 2: run_tiles<2, LAW, CT, IT, OT, NW>(a); return 0;
+    case 3: run_tiles<3, LAW, CT, IT, OT, NW>(a); return 0;
+    case 4: run_tiles<4, LAW, CT, IT, OT, NW>(a); return 0;
   }
   return -1;
 }
 
-template <typename CT, typename IT, typename OT>
+template <typename CT, typename IT, typename OT, int NW>
 int run(const StageArgs& a, int law) {
-  if (law == LAW_SCALAR) return run_law<ScalarLawD, CT, IT, OT>(a);
-  if (law == LAW_EULER2D) return run_law<EulerLawD, CT, IT, OT>(a);
+  if (law == LAW_SCALAR) return run_law<ScalarLawD, CT, IT, OT, NW>(a);
+  if (law == LAW_EULER2D) return run_law<EulerLawD, CT, IT, OT, NW>(a);
   return -1;
 }
 
+template <int NW>
+int run_prec(const StageArgs& a, int law, int prec) {
+  switch (prec) {
+    case 0: return run<double, double, double, NW>(a, law);
+    case 1: return run<float, double, float, NW>(a, law);
+    case 2: return run<float, float, float, NW>(a, law);
+    case 3: return run<float, float, double, NW>(a, law);
+  }
+  return -1;
+}
+
+// variant as the device launcher's: bits 0-1 precision form, bit 2 tile shape
 int run_variant(const StageArgs& a, int law, int variant) {
-  switch (variant) {
-    case 0: return run<double, double, double>(a, law);
-    case 1: return run<float, double, float>(a, law);
-    case 2: return run<float, float, float>(a, law);
-    case 3: return run<float, float, double>(a, law);
+  switch (variant >> 2) {
+    case 0: return run_prec<4>(a, law, variant & 3);
+    case 1: return run_prec<8>(a, law, variant & 3);
   }
   return -1;
 }
@@ -193,8 +204,10 @@ int make(Emu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int order) {
 
 }  // namespace
 
-extern "C" int emu_tri_residual(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int fp32,
+// form: bit 0 the FP32 form, bit 1 the 8-wave tile shape
+extern "C" int emu_tri_residual(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int form,
                                 const double* U, double* R) {
+  const int fp32 = form & 1, shape = (form >> 1) & 1;
   Emu m;
   if (make(m, mesh, law, order)) return -1;
   m.to_soa(U);
@@ -202,15 +215,16 @@ extern "C" int emu_tri_residual(const dg_mesh_desc* mesh, const dg_law_desc* law
   a.Uin = m.Un.data();
   a.R = m.R.data();
   a.scheme = MODE_RESIDUAL;
-  if (run_variant(a, m.S.law, fp32 ? 1 : 0)) return -2;
+  if (run_variant(a, m.S.law, (fp32 ? 1 : 0) | shape << 2)) return -2;
   m.from_soa(m.R.data(), R);
   return 0;
 }
 
 // nsteps steps of the given scheme in place, with the stage / buffer sequence of
 // the library (dg_capi.cc enqueue_step); resid: last step's residual norm
-extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int fp32, int scheme,
+extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int form, int scheme,
                              double dt, int nsteps, double* U, double* resid) {
+  const int fp32 = form & 1, shape = (form >> 1) & 1;
   Emu m;
   if (make(m, mesh, law, order)) return -1;
   m.to_soa(U);
@@ -229,7 +243,7 @@ extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, i
       a.Uout = last ? (void*)m.Un.data() : ((st % 2) ? bufB : bufA);
       if (first && last) a.Uout = bufA;
       a.resid_part = last ? m.part.data() : nullptr;
-      int variant = fp32 ? (first ? 1 : (last ? 3 : 2)) : 0;
+      const int variant = (fp32 ? (first ? 1 : (last ? 3 : 2)) : 0) | shape << 2;
       if (run_variant(a, m.S.law, variant)) return -2;
     }
     if (ns == 1) m.Un = m.A64;
@@ -538,35 +552,42 @@ extern "C" int emu_sensor(const dg_mesh_desc* mesh, const dg_law_desc* law, int 
 
 namespace {
 
-template <int P>
+template <int P, int NW>
 void tet_tiles_run(const TetArgs& a, double* rsum) {
-  using K = TiledTet<P>;
+  using K = TiledTet<P, NW>;
+  constexpr int TH = K::THREADS;
   std::unique_ptr<typename K::Smem> sm(new typename K::Smem());
-  std::vector<typename K::Regs> g(TILE_THREADS);
+  std::vector<typename K::Regs> g(TH);
   const int ntiles = (a.E + TILE_ELEMS - 1) / TILE_ELEMS;
   for (int tile = 0; tile < ntiles; tile++) {
     std::memset(sm.get(), 0, sizeof(typename K::Smem));
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, *sm, t, tile);
-    for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, *sm, t, tile, g[t]);
+    for (int t = 0; t < TH; t++) K::phaseA(a, *sm, t, tile);
+    for (int t = 0; t < TH; t++) K::phaseB(a, *sm, t, tile, g[t]);
     for (int l = 0; l < 4; l++) {
-      for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, *sm, t, tile, l);
-      for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, *sm, t, tile, l);
-      for (int t = 0; t < TILE_THREADS; t++) K::phaseLift(a, *sm, t, l, g[t]);
+      for (int t = 0; t < TH; t++) K::phaseC1(a, *sm, t, tile, l);
+      for (int t = 0; t < TH; t++) K::phaseC2(a, *sm, t, tile, l);
+      for (int t = 0; t < TH; t++) K::phaseLift(a, *sm, t, l, g[t]);
     }
-    for (int t = 0; t < TILE_THREADS; t++) {
+    for (int t = 0; t < TH; t++) {
       K::phaseD(a, t, tile, g[t]);
       *rsum += g[t].rsum;
     }
   }
 }
 
-int tet_run(const TetArgs& a, double* rsum) {
+template <int NW>
+int tet_run_nw(const TetArgs& a, double* rsum) {
   switch (a.P) {
-    case 1: tet_tiles_run<1>(a, rsum); return 0;
-    case 2: tet_tiles_run<2>(a, rsum); return 0;
-    case 3: tet_tiles_run<3>(a, rsum); return 0;
+    case 1: tet_tiles_run<1, NW>(a, rsum); return 0;
+    case 2: tet_tiles_run<2, NW>(a, rsum); return 0;
+    case 3: tet_tiles_run<3, NW>(a, rsum); return 0;
   }
   return -1;
+}
+
+// shape as the device launcher's: 0 for 4 waves per tile, 1 for 8
+int tet_run(const TetArgs& a, int shape, double* rsum) {
+  return shape == 0 ? tet_run_nw<4>(a, rsum) : shape == 1 ? tet_run_nw<8>(a, rsum) : -1;
 }
 
 struct TetEmu {
@@ -612,8 +633,8 @@ int tet_make(TetEmu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int or
 
 }  // namespace
 
-extern "C" int emu_tet_residual(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, const double* U,
-                                double* R) {
+extern "C" int emu_tet_residual(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int shape,
+                                const double* U, double* R) {
   TetEmu m;
   if (tet_make(m, mesh, law, order)) return -1;
   m.to_soa(U, m.Un.data());
@@ -622,13 +643,13 @@ extern "C" int emu_tet_residual(const dg_mesh_desc* mesh, const dg_law_desc* law
   a.R = m.R.data();
   a.scheme = MODE_RESIDUAL;
   double r = 0;
-  if (tet_run(a, &r)) return -2;
+  if (tet_run(a, shape, &r)) return -2;
   m.from_soa(m.R.data(), R);
   return 0;
 }
 
-extern "C" int emu_tet_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int scheme, double dt,
-                             int nsteps, double* U, double* resid) {
+extern "C" int emu_tet_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int shape, int scheme,
+                             double dt, int nsteps, double* U, double* resid) {
   TetEmu m;
   if (tet_make(m, mesh, law, order)) return -1;
   m.to_soa(U, m.Un.data());
@@ -648,7 +669,7 @@ extern "C" int emu_tet_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, i
       a.Uout = last ? m.Un.data() : ((s % 2) ? m.B.data() : m.A.data());
       if (first && last) a.Uout = m.A.data();
       double rs = 0;
-      if (tet_run(a, &rs)) return -2;
+      if (tet_run(a, shape, &rs)) return -2;
       if (last) r = rs;
     }
     if (ns == 1) m.Un = m.A;

@@ -94,6 +94,28 @@ def test_emulated_steps(law, flux, p, scheme):
     assert abs(re - ro) <= 1e-10 * abs(ro)
 
 
+@pytest.mark.parametrize("law,flux,periodic", [("scalar", None, True), ("euler", "lf", True), ("euler", "roe", False)])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_emulated_residual_8_wave_tiles(law, flux, periodic, p):
+    """the 8-wave tile shape (512 threads per 64 elements) computes the same rows"""
+    mesh, lw, o, U = case(law, flux, p, periodic)
+    assert rel(emu_residual(mesh, lw, p, U, fp32=2), o.residual(U)[0]) < 1e-12
+
+
+def test_emulated_8_wave_steps_and_fp32_form():
+    mesh, lw, o, U = case("euler", "roe", 3, periodic=False)
+    dt = o.dt(U, 0.1)
+    Uo, t = U, 0.0
+    for s in range(3):
+        Uo, ro = o.step(Uo, po.RK4, t, dt)
+        t += dt
+    Ue, re = emu_steps(mesh, lw, 3, U, po.RK4, dt, 3, fp32=2)
+    assert rel(Ue, Uo) < 1e-12
+    assert abs(re - ro) <= 1e-10 * abs(ro)
+    U32, _ = emu_steps(mesh, lw, 3, U, po.RK4, dt, 3, fp32=3)
+    assert rel(U32, Uo) < 1e-5
+
+
 def test_emulated_fp32_form():
     mesh, lw, o, U = case("euler", "lf", 3, nx=12)
     R64 = o.residual(U)[0]

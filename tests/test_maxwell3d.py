@@ -98,27 +98,32 @@ if LIBEMU is not None:
     import dgpar
 
     p_md, p_ld = ctypes.POINTER(dgpar.MeshDesc), ctypes.POINTER(dgpar.LawDesc)
-    LIBEMU.emu_tet_residual.argtypes = [p_md, p_ld, ctypes.c_int, PD, PD]
-    LIBEMU.emu_tet_steps.argtypes = [p_md, p_ld, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, PD, PD]
+    LIBEMU.emu_tet_residual.argtypes = [p_md, p_ld, ctypes.c_int, ctypes.c_int, PD, PD]
+    LIBEMU.emu_tet_steps.argtypes = [p_md, p_ld, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                     PD, PD]
 emu = pytest.mark.skipif(LIBEMU is None, reason="emulator not built")
 
 
 @emu
+@pytest.mark.parametrize("shape", [0, 1])
 @pytest.mark.parametrize("flux", [po.FLUX_UPWIND, po.FLUX_CENTRAL])
 @pytest.mark.parametrize("p", [1, 2, 3])
-def test_emulated_tet_residual(flux, p):
+def test_emulated_tet_residual(flux, p, shape):
+    """both tile shapes (4 and 8 waves per 64 tetrahedra)"""
     vids, xyz, o = oracle(2, p, flux)
     U = np.random.default_rng(20 + p).standard_normal(o.ndof)
     md = dgpar.Mesh(3, vids, xyz).desc()
     ld = dgpar.MaxwellLaw(flux).desc()
     R = np.zeros_like(U)
-    assert LIBEMU.emu_tet_residual(ctypes.byref(md), ctypes.byref(ld), p, U.ctypes.data_as(PD), R.ctypes.data_as(PD)) == 0
+    assert LIBEMU.emu_tet_residual(ctypes.byref(md), ctypes.byref(ld), p, shape, U.ctypes.data_as(PD),
+                                   R.ctypes.data_as(PD)) == 0
     Ro, _ = o.residual(U)
     assert np.abs(R - Ro).max() <= 1e-12 * np.abs(Ro).max()
 
 
 @emu
-def test_emulated_tet_rk4_steps():
+@pytest.mark.parametrize("shape", [0, 1])
+def test_emulated_tet_rk4_steps(shape):
     p = 3
     vids, xyz, o = oracle(2, p, po.FLUX_UPWIND)
     U = o.project(3)
@@ -131,7 +136,7 @@ def test_emulated_tet_rk4_steps():
     ld = dgpar.MaxwellLaw(po.FLUX_UPWIND).desc()
     Ue = U.copy()
     r = ctypes.c_double()
-    assert LIBEMU.emu_tet_steps(ctypes.byref(md), ctypes.byref(ld), p, po.RK4, dt, 3, Ue.ctypes.data_as(PD),
+    assert LIBEMU.emu_tet_steps(ctypes.byref(md), ctypes.byref(ld), p, shape, po.RK4, dt, 3, Ue.ctypes.data_as(PD),
                                 ctypes.byref(r)) == 0
     assert np.abs(Ue - Uo).max() <= 1e-12 * np.abs(Uo).max()
     assert abs(r.value - ro) <= 1e-10 * abs(ro)
