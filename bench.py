@@ -65,11 +65,23 @@ def measured_traffic(form):
     return None
 
 
-def run_form(dgpar, mesh, precision, steps, warmup, kernel=0):
+def tuned(g, scheme, dt, tune):
+    """the tuner's choice of launch schedule (timed on the leg's own state,
+    which it restores), or the default schedule"""
+    if not tune:
+        return {"best": g.schedule(), "candidates": []}
+    rep = g.autotune(scheme, dt, steps=20)
+    return {"best": rep["best"], "candidates": [[c[0], c[1], round(c[2], 4)] for c in rep["candidates"]],
+            "fields": "shape (0: 4 waves per 64-element tile, 1: 8 waves), grid (persistent workgroups, 0: default), "
+                      "ms per step"}
+
+
+def run_form(dgpar, mesh, precision, steps, warmup, kernel=0, tune=True):
     law = dgpar.EulerLaw(dgpar.FLUX_LF, 1.4)
     g = dgpar.DGAnalysis(mesh, law, ORDER, precision=precision, use_graph=True, device=0, kernel=kernel)
     g.project(dgpar.FIELD_VORTEX, VORTEX)
     dt = g.computeDt(CFL)
+    sched = tuned(g, dgpar.RK4, dt, tune)
     if warmup > 0:
         g.time_advance(dgpar.RK4, dt, warmup)
     g.project(dgpar.FIELD_VORTEX, VORTEX)
@@ -84,7 +96,7 @@ def run_form(dgpar, mesh, precision, steps, warmup, kernel=0):
     state = g.get_state()
     ndof = g.ndof
     g.close()
-    return dict(ms=ms, wall=wall, dt=dt, err=err, state=state, ndof=ndof)
+    return dict(ms=ms, wall=wall, dt=dt, err=err, state=state, ndof=ndof, schedule=sched)
 
 
 def run_adjoint(dgpar, steps, every):
@@ -117,7 +129,7 @@ def run_adjoint(dgpar, steps, every):
             "fwd_value": ndof * 4 * steps / (fwd_ms / 1e3), "unit": "DOF*stage/s"}
 
 
-def run_mixed(dgpar, steps, sense_every=20):
+def run_mixed(dgpar, steps, sense_every=20, tune=True):
     """Shock tube with mixed orders: 512x128x2 jittered triangles on [0,1]x[0,0.25],
     walls at the bottom and top, the two initial states as far field at the ends,
     a seeded order map uniform on {1,2,3,4}, Roe flux with entropy fix, the
@@ -138,6 +150,7 @@ def run_mixed(dgpar, steps, sense_every=20):
     g.limit()
     dt = g.computeDt(CFL)
     s_hi, s_lo = (0.05, 0.02, 0.01, 0.005), (5e-5, 2e-5, 1e-5, 5e-6)
+    sched = tuned(g, dgpar.RK_TVD3, dt, tune)
     g.time_advance(dgpar.RK_TVD3, dt, 2)  # warm-up (graph capture)
     step_ms, sense_ms, dof_stages = 0.0, 0.0, 0.0
     done = 0
@@ -156,12 +169,12 @@ def run_mixed(dgpar, steps, sense_every=20):
            "steps": steps, "dt": dt, "value": dof_stages / (step_ms / 1e3), "unit": "DOF*stage/s",
            "ms_per_step": step_ms / steps, "sensor_rebucket_ms_per_pass": sense_ms / max(1, -(-steps // sense_every)),
            "orders_histogram_p1_p4": hist, "limited_elements_last_stage": int((flags > 0).sum()),
-           "steps_to_t0.2": int(np.ceil(0.2 / dt))}
+           "steps_to_t0.2": int(np.ceil(0.2 / dt)), "schedule": sched}
     g.close()
     return out
 
 
-def run_maxwell(dgpar, steps, n=48):
+def run_maxwell(dgpar, steps, n=48, tune=True):
     """Maxwell on tetrahedra: 48^3 boxes x 6 Kuhn tetrahedra on the unit cube, PEC
     walls, P3, upwind flux, RK4 at CFL 0.1, the (1,1,1) cavity mode; the L2
     error against the exact mode after the timed steps."""
@@ -169,6 +182,7 @@ def run_maxwell(dgpar, steps, n=48):
     g = dgpar.DGAnalysis(mesh, dgpar.MaxwellLaw(dgpar.FLUX_UPWIND), 3, precision=dgpar.PREC_FP64, use_graph=True)
     g.project(dgpar.FIELD_CAVITY)
     dt = g.computeDt(CFL)
+    sched = tuned(g, dgpar.RK4, dt, tune)
     g.time_advance(dgpar.RK4, dt, 2)
     g.project(dgpar.FIELD_CAVITY)
     dgpar.device_synchronize()
@@ -182,7 +196,7 @@ def run_maxwell(dgpar, steps, n=48):
             "elements": 6 * n ** 3, "dof": ndof, "steps": steps, "ms_per_step": ms / steps,
             "value": ndof * 4 * steps / (ms / 1e3), "unit": "DOF*stage/s",
             "achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS, "bytes_per_dof_stage": b,
-            "l2_error_E_H": [float(x) for x in err]}
+            "l2_error_E_H": [float(x) for x in err], "schedule": sched}
 
 
 def cpu_baseline(mesh, steps=5):
@@ -212,6 +226,7 @@ def main():
                     help="RK4 steps of the Maxwell tetrahedra leg (0: skip it)")
     ap.add_argument("--mixed-steps", type=int, default=100,
                     help="SSP-RK3 steps of the mixed-order limited shock-tube leg (0: skip it)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules (no tuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 256 MiB device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -224,8 +239,9 @@ def main():
         dgpar.copy_probe(64 << 20)
     mesh = dgpar.mesh_tri_structured(NX, NX, lx=10.0, ly=10.0, jitter=0.1, seed=SEED)
     E = mesh.n_elems
-    r32 = run_form(dgpar, mesh, dgpar.PREC_FP32, args.steps, args.warmup, args.kernel)
-    r64 = run_form(dgpar, mesh, dgpar.PREC_FP64, args.steps, args.warmup, args.kernel)
+    tune = not args.no_tune
+    r32 = run_form(dgpar, mesh, dgpar.PREC_FP32, args.steps, args.warmup, args.kernel, tune)
+    r64 = run_form(dgpar, mesh, dgpar.PREC_FP64, args.steps, args.warmup, args.kernel, tune)
     ndof = r32["ndof"]
     np_ = (ORDER + 1) * (ORDER + 2) // 2
     X = mesh.xyz
@@ -260,7 +276,8 @@ def main():
                    "mesh": "256x256x2 periodic [0,10]^2 jitter 0.1h seed %d" % SEED, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": measured_traffic("fp32") if args.kernel == 0 else None,
-                     "kernel": ("tri_stage_tiled" if args.kernel == 0 else "tri_stage_kernel") + "<3,EulerLawD,float,*>",
+                     "kernel": ("tri_stage_tiled<3,EulerLawD,float,*,%d>" % (8 if r32["schedule"]["best"][0] else 4)
+                                if args.kernel == 0 else "tri_stage_kernel<3,EulerLawD,float,*>"),
                      "bytes_per_dof_stage": b32},
         "fp64_plain": {"value": v64, "ms_per_step": r64["ms"] / args.steps, "achieved_GBs": ach64,
                        "frac": ach64 / HBM_PEAK_GBS, "bytes_per_dof_stage": b64,
@@ -268,14 +285,15 @@ def main():
         "ratio_fp32_over_fp64": v32 / v64,
         "l2_rho_vs_exact": {"fp32": float(r32["err"][0]), "fp64": float(r64["err"][0])},
         "l2_rho_fp32_vs_fp64": l2_rho_diff,
+        "schedule": {"fp32": r32["schedule"], "fp64": r64["schedule"]},
         "wall_s_timed": r32["wall"],
     }
     if args.adjoint_steps > 0:
         out["adjoint"] = run_adjoint(dgpar, args.adjoint_steps, 20)
     if args.mixed_steps > 0:
-        out["mixed_order_limited"] = run_mixed(dgpar, args.mixed_steps)
+        out["mixed_order_limited"] = run_mixed(dgpar, args.mixed_steps, tune=tune)
     if args.maxwell_steps > 0:
-        out["maxwell3d_tets"] = run_maxwell(dgpar, args.maxwell_steps)
+        out["maxwell3d_tets"] = run_maxwell(dgpar, args.maxwell_steps, tune=tune)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh)
     print(json.dumps(out))

@@ -1363,25 +1363,34 @@ dg_status dg_autotune(dg_ctx* c, int scheme, double dt, int steps, const char* c
   if (he == hipSuccess) he = hipEventCreate(&e0);
   if (he == hipSuccess) he = hipEventCreate(&e1);
   const dg_schedule before{c->shape, c->mix_grid, {0, 0, 0, 0, 0, 0}};
-  int best = -1;
-  for (size_t i = 0; he == hipSuccess && st == DG_OK && i < cand.size() && i < DG_TUNE_MAX_CAND; i++) {
-    apply_schedule(c, cand[i]);
-    // one untimed step (builds the graph when graphs are on), then the best of 3
-    if ((st = run_steps(c, scheme, dt, 1)) || (he = restore()) != hipSuccess) break;
-    float bestms = 0.f;
-    for (int r = 0; r < 3 && he == hipSuccess && st == DG_OK; r++) {
+  const int ncand = (int)std::min<size_t>(cand.size(), DG_TUNE_MAX_CAND);
+  std::vector<float> bestms(ncand, 0.f);
+  // warm the device (clocks, caches) with the current schedule, untimed, so the
+  // first candidate is not timed cold
+  if (he == hipSuccess && (st = run_steps(c, scheme, dt, 4 * steps)) == DG_OK) he = restore();
+  // rounds over the candidates in turn; per candidate one untimed step after the
+  // switch (the graph of the new launch is built there), then the timed steps;
+  // each candidate keeps its best round
+  constexpr int ROUNDS = 3;
+  for (int r = 0; r < ROUNDS && he == hipSuccess && st == DG_OK; r++)
+    for (int i = 0; i < ncand && he == hipSuccess && st == DG_OK; i++) {
+      apply_schedule(c, cand[i]);
       float ms = 0.f;
-      if ((he = hipEventRecord(e0, c->stream)) != hipSuccess || (st = run_steps(c, scheme, dt, steps)) ||
+      if ((st = run_steps(c, scheme, dt, 1)) || (he = restore()) != hipSuccess ||
+          (he = hipEventRecord(e0, c->stream)) != hipSuccess || (st = run_steps(c, scheme, dt, steps)) ||
           (he = hipEventRecord(e1, c->stream)) != hipSuccess || (he = hipEventSynchronize(e1)) != hipSuccess ||
           (he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess || (he = restore()) != hipSuccess)
         break;
-      if (r == 0 || ms < bestms) bestms = ms;
+      if (r == 0 || ms < bestms[i]) bestms[i] = ms;
     }
-    rep->cand[rep->ncand] = cand[i];
-    rep->ms_per_step[rep->ncand] = bestms / steps;
-    if (best < 0 || rep->ms_per_step[rep->ncand] < rep->ms_per_step[best]) best = rep->ncand;
-    rep->ncand++;
-  }
+  int best = -1;
+  if (he == hipSuccess && st == DG_OK)
+    for (int i = 0; i < ncand; i++) {
+      rep->cand[i] = cand[i];
+      rep->ms_per_step[i] = bestms[i] / steps;
+      if (best < 0 || rep->ms_per_step[i] < rep->ms_per_step[best]) best = i;
+    }
+  rep->ncand = best < 0 ? 0 : ncand;
   if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

@@ -149,8 +149,11 @@ struct TiledStage {
   DG_HD static const CT* table(const StageArgs& a) {
     return sizeof(CT) == 8 ? (const CT*)(const void*)a.tab : (const CT*)(const void*)a.tabf;
   }
-  DG_HD static int row_f(int wave, int fi) { return wave % FG + FG * fi; }
-  DG_HD static int row_k(int wave, int ki) { return wave / FG + KG * ki; }
+  // a group of size one contributes no wave term, so with 4 waves the mode
+  // index of an Euler row (and the field index of a scalar row) is a
+  // compile-time constant and the basis tables are read at fixed offsets
+  DG_HD static int row_f(int wave, int fi) { return (FG == 1 ? 0 : KG == 1 ? wave : wave % FG) + FG * fi; }
+  DG_HD static int row_k(int wave, int ki) { return (KG == 1 ? 0 : FG == 1 ? wave : wave / FG) + KG * ki; }
 
   // ---- A: stage-input tile -> LDS (rows split over the waves); own element's
   //      geometry and links -> registers ----------------------------------------
