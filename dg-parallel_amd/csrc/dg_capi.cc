@@ -67,7 +67,7 @@ struct dg_ctx {
   double *mg = nullptr, *detj = nullptr, *size = nullptr, *tab = nullptr, *vtx = nullptr, *part = nullptr;
   float* tabf = nullptr;
   int32_t* link = nullptr;
-  double *mtab = nullptr, *glw = nullptr, *mvtx = nullptr;
+  double *mtab = nullptr, *glw = nullptr, *mvtx = nullptr, *ntab = nullptr;
   int* chunks = nullptr;
   uint8_t *flags = nullptr, *sens = nullptr;
   std::vector<void*> allocs;
@@ -172,7 +172,7 @@ static size_t part_size(const TriSetup& S) {
   const int E = S.E;
   int n = std::max(tri_stage_blocks(E), tri_tiled_blocks(E));
   n = std::max(n, mixed_grid(mixed_chunks(S)));
-  n = std::max(n, (E + MIX_CHUNK - 1) / MIX_CHUNK);
+  n = std::max(n, mixed_dt_blocks(E));
   return (size_t)n + 1;
 }
 
@@ -323,12 +323,14 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
   if (c->nchunks != mixed_chunks(S)) return bail(fail(c, DG_ERR_ARG, "chunk count mismatch"));
   if ((s = dalloc(c, &c->mtab, S.mtab.size())) || (s = dalloc(c, &c->glw, S.glw.size())) ||
       (s = dalloc(c, &c->mvtx, S.mvtx.size())) || (s = dalloc(c, &c->chunks, chunks.size() + 2)) ||
+      (s = dalloc(c, &c->ntab, S.ntab.size())) ||
       (s = dalloc(c, &c->flags, (size_t)E)) || (s = dalloc(c, &c->sens, (size_t)E)))
     return bail(s);
   if (hipMemcpy(c->mtab, S.mtab.data(), S.mtab.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->glw, S.glw.data(), S.glw.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->mvtx, S.mvtx.data(), S.mvtx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->chunks, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->ntab, S.ntab.data(), S.ntab.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->flags, 0, E) != hipSuccess)
     return bail(fail(c, DG_ERR_HIP, "table upload failed"));
   // limiter constants: phi_1, phi_2 at the edge midpoints and the least-squares
@@ -451,6 +453,7 @@ static MixedArgs mixed_args(const dg_ctx* c) {
   m.el.link = c->link;
   m.law = c->lp;
   m.chunks = c->chunks;
+  m.ntab = c->ntab;
   m.nchunks = c->nchunks;
   return m;
 }
@@ -592,7 +595,7 @@ dg_status dg_compute_dt(dg_ctx* c, double cfl, double* dt_out) {
     MixedArgs m = mixed_args(c);
     m.Uin = c->Un;
     if (launch_mixed_dt(m, c->law, cfl, c->part, c->stream)) return fail(c, DG_ERR_HIP, "dt launch failed");
-    nb = (c->E + MIX_CHUNK - 1) / MIX_CHUNK;
+    nb = mixed_dt_blocks(c->E);
   } else {
     if (launch_tri_dt(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "dt launch failed");
     nb = tri_dt_blocks(c->E);

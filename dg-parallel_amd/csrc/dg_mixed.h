@@ -1,12 +1,7 @@
 // Mixed-order (P1-P4) triangle kernels as per-element __host__ __device__
-// functions, shared by the HIP kernels (dg_mixed.hip) and the CPU emulation:
+// functions, shared by the HIP kernels (dg_mixed.hip) and the CPU emulation (the
+// mixed-order stage itself is the tiled kernel of dg_mixed_tiled.h):
 //
-//   mixed_stage_element  residual + one RK stage of an element of order P whose
-//                        neighbours have any order: each face is integrated with
-//                        the rule of the larger order, nq = max(P, Pnb) + 1
-//                        Gauss-Legendre points (degree 2 max(pL, pR) + 1,
-//                        DGCell.cc:408-423), the neighbour's trace taken with its
-//                        own basis (DGCell.cc:639-732);
 //   limit_element        triangle TVB limiter (new design; the reference's
 //                        DGMomentLimiter::limit, DGLimiter.cc:963-1133, indexes
 //                        quadrilateral tensor modes only);
@@ -29,7 +24,7 @@
 
 namespace dgp {
 
-constexpr int MIX_CHUNK = 256;  // elements per work item of the persistent launch
+constexpr int MIX_CHUNK = 64;   // elements per work item of the persistent launch (one tile)
 constexpr int MIX_MAXNP = 15;   // np_tri(4)
 
 struct MixedArgs {
@@ -53,6 +48,7 @@ struct MixedArgs {
   double* resid_part; // per workgroup of the persistent launch (or null)
   const int* chunks;  // work items: (order, first internal element) pairs
   int nchunks;
+  const double* ntab; // neighbour trace tables of every order and rule size (dg_mixed_tiled.h)
 };
 
 // order and coefficient index from the bucket bounds of any argument block
@@ -82,127 +78,6 @@ DG_HD void edge_normal(const double mg[2][2], int l, double n[2], double& len) {
   len = dg_sqrt(nl0 * nl0 + nl1 * nl1);
   n[0] = nl0 / len;
   n[1] = nl1 / len;
-}
-
-template <int P, class LAW>
-DG_HD void mixed_stage_element(const MixedArgs& a, int i, double& rsum) {
-  constexpr int NP = np_tri(P), NQ = nqv_tri(P), NF = LAW::NF;
-  const int E = a.E;
-  const double* tb = a.tab + a.moff[P];
-  const double* Uin = a.Uin;
-  double U[NF][NP], R[NF][NP];
-  for (int f = 0; f < NF; f++)
-    for (int k = 0; k < NP; k++) {
-      U[f][k] = Uin[mixed_index(a, P, i, f * NP + k)];
-      R[f][k] = 0.0;
-    }
-  double mg[2][2];
-  for (int r = 0; r < 2; r++)
-    for (int d = 0; d < 2; d++) mg[r][d] = a.el.mg[(long)(r * 2 + d) * E + i];
-
-  // volume term (DGCell.cc:184-249)
-  for (int q = 0; q < NQ; q++) {
-    double u[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      double s = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; k++) s += tb[TriBlock::off_phi(P) + q * NP + k] * U[f][k];
-      u[f] = s;
-    }
-    double F[NF][2];
-    LAW::flux(a.law, u, F);
-    const double w = tb[TriBlock::off_wv(P) + q];
-    double G[NF][2];
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      G[f][0] = w * (mg[0][0] * F[f][0] + mg[0][1] * F[f][1]);
-      G[f][1] = w * (mg[1][0] * F[f][0] + mg[1][1] * F[f][1]);
-    }
-#pragma unroll
-    for (int k = 0; k < NP; k++) {
-      const double g0 = tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 0];
-      const double g1 = tb[TriBlock::off_dphi(P) + (q * NP + k) * 2 + 1];
-#pragma unroll
-      for (int f = 0; f < NF; f++) R[f][k] += g0 * G[f][0] + g1 * G[f][1];
-    }
-  }
-
-  // face terms (DGCell.cc:639-732) with the rule of the larger order
-  for (int l = 0; l < 3; l++) {
-    const int link = a.el.link[(long)l * E + i];
-    double n[2], len;
-    edge_normal(mg, l, n, len);
-    int nq = P + 1;
-    double uR[MAXP_TRI + 1][NF];
-    if (link >= 0) {
-      const int nb = link >> 3, lb = link & 3;
-      const bool same = (link >> 2) & 1;
-      const int pn = mixed_order_of(a, nb), npn = (pn + 1) * (pn + 2) / 2;
-      nq = (pn > P ? pn : P) + 1;
-      const double* tn = a.tab + a.ftoff[pn][nq];
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-        double v[MIX_MAXNP];
-#pragma unroll
-        for (int k = 0; k < MIX_MAXNP; k++) v[k] = k < npn ? a.Uin[mixed_index(a, pn, nb, f * npn + k)] : 0.0;
-#pragma unroll
-        for (int q = 0; q <= MAXP_TRI; q++) {
-          double s = 0.0;
-          if (q < nq) {
-            const int qq = same ? q : nq - 1 - q;
-#pragma unroll
-            for (int k = 0; k < MIX_MAXNP; k++)
-              if (k < npn) s += tn[(lb * nq + qq) * npn + k] * v[k];
-          }
-          uR[q][f] = s;
-        }
-      }
-    }
-    const double* to = a.tab + a.ftoff[P][nq];
-    const double* wq = a.glw + GLW_OFF(nq);
-#pragma unroll
-    for (int q = 0; q <= MAXP_TRI; q++) {
-      if (q < nq) {
-        double uL[NF], Fh[NF];
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-          double s = 0.0;
-#pragma unroll
-          for (int k = 0; k < NP; k++) s += to[(l * nq + q) * NP + k] * U[f][k];
-          uL[f] = s;
-        }
-        if (link < 0) LAW::ghost(a.law, -link, n, uL, uR[q]);
-        LAW::riemann(a.law, n, uL, uR[q], Fh);
-        const double sw = len * wq[q];
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-          const double fs = Fh[f] * sw;
-#pragma unroll
-          for (int k = 0; k < NP; k++) R[f][k] -= fs * to[(l * nq + q) * NP + k];
-        }
-      }
-    }
-  }
-
-  // stage update (TimeIntegrator.cc:81-275, 279-438, 838-1048)
-  if (a.scheme == MODE_RESIDUAL) {
-    for (int f = 0; f < NF; f++)
-      for (int k = 0; k < NP; k++) a.R[mixed_index(a, P, i, f * NP + k)] = R[f][k];
-    return;
-  }
-  const double dtj = a.dt / a.el.detj[i];
-  const int sc = a.scheme * 4 + a.stage;
-  const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
-  const bool reads_acc = sc == 3 * 4 + 1 || sc == 3 * 4 + 2 || sc == 3 * 4 + 3;
-  for (int f = 0; f < NF; f++)
-    for (int k = 0; k < NP; k++) {
-      const long idx = mixed_index(a, P, i, f * NP + k);
-      double accv = reads_acc ? a.acc[idx] : 0.0;
-      const double o = stage_value_hd(sc, R[f][k] * dtj, a.Un[idx], reads_input ? Uin[idx] : 0.0, &accv, rsum);
-      if (sc >= 3 * 4 && sc < 3 * 4 + 3) a.acc[idx] = accv;
-      a.Uout[idx] = o;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -517,6 +392,8 @@ DG_HD double dt_element(const MixedArgs& a, int i, double cfl) {
 // launches (dg_mixed.hip); part holds one value per workgroup of the launch
 int mixed_grid(int nchunks);
 int launch_mixed_stage(const MixedArgs& a, int law, int grid, hipStream_t s);
+// part receives one minimum per workgroup: mixed_dt_blocks(E) values
+int mixed_dt_blocks(int E);
 int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipStream_t s);
 int launch_limiter(const LimiterArgs& a, int law, hipStream_t s);
 int launch_sensor(const SensorArgs& a, hipStream_t s);

@@ -1,44 +1,66 @@
-// HIP kernels of the mixed-order path (element functions in dg_mixed.h).
+// HIP kernels of the mixed-order path (tile phases in dg_mixed_tiled.h, element
+// functions in dg_mixed.h).
 //
 // tri_mixed_stage is one persistent launch for every order bucket: the host
-// splits each bucket into chunks of MIX_CHUNK elements, heaviest order first,
-// and each workgroup walks the chunk list with a stride of the grid size.  A
-// chunk never spans buckets, so the order switch is uniform per workgroup and
-// every wave runs one specialised path.  All workgroups reach the end of the
-// list; the residual norm is reduced per workgroup in a fixed order.
+// splits each bucket into chunks of MIX_CHUNK = 64 elements (one tile), heaviest
+// order first, and each workgroup walks the chunk list with a stride of the
+// grid size.  A chunk never spans buckets, so the order switch is uniform per
+// workgroup.  All workgroups reach the end of the list; the residual norm is
+// reduced per workgroup in a fixed order.
 #include <hip/hip_runtime.h>
 
-#include "dg_mixed.h"
+#include "dg_mixed_tiled.h"
 
 namespace dgp {
 
-constexpr int MIX_BLOCK = MIX_CHUNK;
+constexpr int MIX_BLOCK = 256;
 
 int mixed_grid(int nchunks) { return nchunks < 2048 ? nchunks : 2048; }
 
+template <int P, class LAW>
+__device__ __forceinline__ void mixed_tile(const MixedArgs& a, MixSmem<LAW>& s, int tid, int i0, double& rsum) {
+  using K = TiledMixed<P, LAW>;
+  typename K::Regs g;
+  K::phaseA(a, s, tid, i0, g);
+  __syncthreads();
+  for (int q0 = 0; q0 < K::NQ; q0 += MIX_QC) {
+    K::phaseB(a, s, tid, q0);
+    __syncthreads();
+    K::phaseD1(a, s, tid, q0, g);
+    __syncthreads();
+  }
+  K::phaseC1(a, s, tid, i0);
+  __syncthreads();
+  K::phaseC2(a, s, tid);
+  __syncthreads();
+  K::phaseD2(a, s, tid, i0, g);
+  rsum += g.rsum;
+}
+
 template <class LAW>
 __global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_stage(MixedArgs a) {
+  __shared__ MixSmem<LAW> sm;
+  const int tid = threadIdx.x;
   double rsum = 0.0;
+  TiledMixed<1, LAW>::stage_tables(a, sm, tid);  // the same tables for every order
   for (int w = blockIdx.x; w < a.nchunks; w += gridDim.x) {
-    const int p = a.chunks[2 * w], i = a.chunks[2 * w + 1] + threadIdx.x;
-    if (i < a.bstart[p + 1]) {
-      switch (p) {
-        case 1: mixed_stage_element<1, LAW>(a, i, rsum); break;
-        case 2: mixed_stage_element<2, LAW>(a, i, rsum); break;
-        case 3: mixed_stage_element<3, LAW>(a, i, rsum); break;
-        default: mixed_stage_element<4, LAW>(a, i, rsum); break;
-      }
+    const int p = a.chunks[2 * w], i0 = a.chunks[2 * w + 1];
+    switch (p) {
+      case 1: mixed_tile<1, LAW>(a, sm, tid, i0, rsum); break;
+      case 2: mixed_tile<2, LAW>(a, sm, tid, i0, rsum); break;
+      case 3: mixed_tile<3, LAW>(a, sm, tid, i0, rsum); break;
+      default: mixed_tile<4, LAW>(a, sm, tid, i0, rsum); break;
     }
+    __syncthreads();  // the next tile overwrites the LDS
   }
   if (a.resid_part) {
-    __shared__ double red[MIX_BLOCK];
-    red[threadIdx.x] = rsum;
+    sm.red[tid] = rsum;
     __syncthreads();
     for (int s = MIX_BLOCK / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      if (tid < s) sm.red[tid] += sm.red[tid + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) a.resid_part[blockIdx.x] = red[0];
+    if (tid == 0) a.resid_part[blockIdx.x] = sm.red[0];
   }
 }
 
@@ -81,8 +103,10 @@ int launch_mixed_stage(const MixedArgs& a, int law, int ngrid, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+int mixed_dt_blocks(int E) { return (E + MIX_BLOCK - 1) / MIX_BLOCK; }
+
 int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipStream_t s) {
-  dim3 grid((a.E + MIX_BLOCK - 1) / MIX_BLOCK), block(MIX_BLOCK);
+  dim3 grid(mixed_dt_blocks(a.E)), block(MIX_BLOCK);
   if (law == LAW_SCALAR)
     hipLaunchKernelGGL(tri_mixed_dt<ScalarLawD>, grid, block, 0, s, a, cfl, part);
   else if (law == LAW_EULER2D)

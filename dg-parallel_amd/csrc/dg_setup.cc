@@ -213,6 +213,10 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
         S.ftoff[p][nq] = nq < p + 1 ? -1 : S.moff[p] + (nq == p + 1 ? TriBlock::off_phif(p) : TriBlock::off_phifx(p, nq));
     }
     for (int nq = 0; nq < 6; nq++) S.ftoff[0][nq] = -1;
+    S.ntab.assign(MIX_NTAB, 0.0);
+    for (int p = 1; p <= MAXP_TRI; p++)
+      for (int nq = p + 1; nq <= MAXP_TRI + 1; nq++)
+        std::copy_n(S.mtab.begin() + S.ftoff[p][nq], 3 * nq * np_tri(p), S.ntab.begin() + ntab_off(p, nq));
     S.glw.assign(GLW_OFF(MAXP_TRI + 2), 0.0);
     build_glw(S.glw.data());
     S.mvtx.assign((size_t)(MAXP_TRI + 1) * 4 * 15, 0.0);

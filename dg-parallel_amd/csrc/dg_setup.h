@@ -28,6 +28,7 @@ struct TriSetup {
   // glw the Gauss-Legendre half weights (GLW_OFF); mvtx[p][4][15] the mean and
   // vertex basis values of order p
   std::vector<double> mtab, glw, mvtx;
+  std::vector<double> ntab;  // every order's trace tables, MIX_NTAB doubles at ntab_off(p, nq)
   int moff[6] = {0, 0, 0, 0, 0, 0};
   int ftoff[6][6];
   LawParams lp{};

@@ -61,6 +61,26 @@ struct TriBlock {
 // Gauss-Legendre weights (x 1/2) for every rule size used by faces
 constexpr int GLW_OFF(int nq) { return (nq - 1) * nq / 2; }
 
+// The trace tables of every order pn (1..4) and face rule size nq (pn+1 ..
+// MAXP_TRI+1), [3][nq][Np(pn)] each (the phif / phifx rows of the order-pn
+// block), concatenated pn-major: the mixed-order kernel stages them in LDS and
+// reads a neighbour's table per lane.
+constexpr int ntab_off(int pn, int nq) {
+  const int base = pn == 1 ? 0 : pn == 2 ? 126 : pn == 3 ? 342 : 612;  // the blocks of lower orders
+  return base + 3 * ((pn + 1) * (pn + 2) / 2) * ((pn + nq) * (nq - 1 - pn) / 2);
+}
+constexpr int MIX_NTAB = 837;
+constexpr bool ntab_layout_ok() {
+  int o = 0;
+  for (int p = 1; p <= MAXP_TRI; p++)
+    for (int m = p + 1; m <= MAXP_TRI + 1; m++) {
+      if (ntab_off(p, m) != o) return false;
+      o += 3 * m * np_tri(p);
+    }
+  return o == MIX_NTAB;
+}
+static_assert(ntab_layout_ok(), "trace table layout");
+
 // ---------------------------------------------------------------------------
 // Per-order block of the tetrahedron kernels (Maxwell: linear flux, so the
 // volume term is the exact operator form of the degree-2P quadrature).  With

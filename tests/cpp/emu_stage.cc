@@ -11,6 +11,7 @@
 
 #include "../../dg-parallel_amd/csrc/dg_adjoint.h"
 #include "../../dg-parallel_amd/csrc/dg_mixed.h"
+#include "../../dg-parallel_amd/csrc/dg_mixed_tiled.h"
 #include "../../dg-parallel_amd/csrc/dg_tet.h"
 #include "../../dg-parallel_amd/csrc/dg_setup.h"
 #include "../../dg-parallel_amd/csrc/dg_stage_tiled.h"
@@ -366,6 +367,7 @@ struct MixEmu {
     m.el.size = S.size.data();
     m.el.link = S.link.data();
     m.law = S.lp;
+    m.ntab = S.ntab.data();
     return m;
   }
   LimiterArgs lim(double* u, double M, double nu) {
@@ -434,17 +436,39 @@ int mix_make(MixEmu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int or
   return 0;
 }
 
+// one tile of the persistent mixed-order kernel, the phases in the kernel's order
+template <int P, class LAW>
+void mix_tile(const MixedArgs& a, MixSmem<LAW>& sm, int i0, double& rsum) {
+  using K = TiledMixed<P, LAW>;
+  std::vector<typename K::Regs> g(TILE_THREADS);
+  for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, sm, t, i0, g[t]);
+  for (int q0 = 0; q0 < K::NQ; q0 += MIX_QC) {
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, sm, t, q0);
+    for (int t = 0; t < TILE_THREADS; t++) K::phaseD1(a, sm, t, q0, g[t]);
+  }
+  for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, sm, t, i0);
+  for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, sm, t);
+  for (int t = 0; t < TILE_THREADS; t++) {
+    K::phaseD2(a, sm, t, i0, g[t]);
+    rsum += g[t].rsum;
+  }
+}
+
+// every chunk of 64 elements of every bucket, as the persistent launch walks them
 template <class LAW>
 void mix_stage_all(const MixedArgs& a, double& rsum) {
-  for (int i = 0; i < a.E; i++) {
-    const int p = mixed_order_of(a, i);
-    switch (p) {
-      case 1: mixed_stage_element<1, LAW>(a, i, rsum); break;
-      case 2: mixed_stage_element<2, LAW>(a, i, rsum); break;
-      case 3: mixed_stage_element<3, LAW>(a, i, rsum); break;
-      default: mixed_stage_element<4, LAW>(a, i, rsum); break;
+  std::unique_ptr<MixSmem<LAW>> sm(new MixSmem<LAW>());
+  for (int p = MAXP_TRI; p >= 1; p--)
+    for (int i0 = a.bstart[p]; i0 < a.bstart[p + 1]; i0 += MIX_CHUNK) {
+      std::memset(sm.get(), 0, sizeof(MixSmem<LAW>));
+      for (int t = 0; t < TILE_THREADS; t++) TiledMixed<1, LAW>::stage_tables(a, *sm, t);
+      switch (p) {
+        case 1: mix_tile<1, LAW>(a, *sm, i0, rsum); break;
+        case 2: mix_tile<2, LAW>(a, *sm, i0, rsum); break;
+        case 3: mix_tile<3, LAW>(a, *sm, i0, rsum); break;
+        default: mix_tile<4, LAW>(a, *sm, i0, rsum); break;
+      }
     }
-  }
 }
 
 void mix_stage(const MixedArgs& a, int law, double& rsum) {

@@ -184,6 +184,24 @@ def test_emulated_mixed_residual(flux, seed):
 
 
 @pytestmark_emu
+def test_emulated_mixed_residual_many_tiles():
+    """about 180 elements per order bucket: whole tiles of 64 and a partial last
+    tile in every bucket of the persistent launch"""
+    vids, xyz, bc, orders, o, _ = sod_case(3, nx=36, ny=10)
+    assert all(np.bincount(orders, minlength=5)[1:] > 128)
+    rng = np.random.default_rng(3)
+    U = np.concatenate([(0.01 * rng.standard_normal((4, np_tri(p))) +
+                         np.outer([0.6, 0.1, -0.05, 1.4], np.eye(np_tri(p))[0]) / np.sqrt(2.0)).ravel()
+                        for p in orders])
+    mesh, lw = descs(vids, xyz, bc, orders, po.FLUX_ROE_EF)
+    md, ld = mesh.desc(), lw.desc()
+    R = np.zeros_like(U)
+    assert LIBEMU.emu_mixed_residual(ctypes.byref(md), ctypes.byref(ld), U.ctypes.data_as(PD), R.ctypes.data_as(PD)) == 0
+    Ro, _ = o.residual(U)
+    assert np.abs(R - Ro).max() <= 1e-12 * np.abs(Ro).max()
+
+
+@pytestmark_emu
 def test_emulated_mixed_scalar_residual():
     vids, xyz, bc = mesh_tri(6, 6, periodic_x=True, periodic_y=False, jitter=0.1, seed=2, side_tags=(7, 7, 7, 7))
     orders = np.random.default_rng(8).integers(1, 5, vids.shape[0]).astype(np.int32)
