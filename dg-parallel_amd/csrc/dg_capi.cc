@@ -37,6 +37,13 @@ struct dg_ctx {
   // kernels, workgroups of the persistent mixed-order launch (0: default)
   int shape = 0, mix_grid = 0;
   size_t part_cap = 0;  // doubles in part
+  // mixed orders: caller topology and geometry (rebucketing keeps them), the
+  // capacity of the state buffers (every element at the top order) and of the
+  // chunk list, and the old-index map of a rebucketing
+  TriTopo topo;
+  size_t state_cap = 0;
+  int chunk_cap = 0;
+  int32_t* remap = nullptr;
   int adj_nd = 0;  // adjoint kernel: flux tangent directions per pass (0: default)
   LawParams lp{};
   size_t ndof = 0;
@@ -160,18 +167,25 @@ dg_status dg_faces(const dg_ctx* c, int32_t* out4, size_t n) {
   return DG_OK;
 }
 
-// chunks of the persistent mixed-order launch: one per MIX_CHUNK elements of a bucket
-static int mixed_chunks(const TriSetup& S) {
-  int n = 0;
-  for (int p = 1; p <= MAXP_TRI; p++) n += (S.bstart[p + 1] - S.bstart[p] + MIX_CHUNK - 1) / MIX_CHUNK;
-  return n;
+// chunks of the persistent mixed-order launch: one per MIX_CHUNK elements of a
+// bucket, heaviest order first, as (order, first internal element) pairs
+static std::vector<int> mixed_chunk_list(const int* bstart) {
+  std::vector<int> chunks;
+  for (int p = MAXP_TRI; p >= 1; p--)
+    for (int i = bstart[p]; i < bstart[p + 1]; i += MIX_CHUNK) {
+      chunks.push_back(p);
+      chunks.push_back(i);
+    }
+  return chunks;
 }
+// the most chunks any order map of E elements gives
+static int max_chunks(int E) { return (E + MIX_CHUNK - 1) / MIX_CHUNK + MAXP_TRI; }
 
 // per-workgroup partial sums / minima: one slot per workgroup of the largest launch
 static size_t part_size(const TriSetup& S) {
   const int E = S.E;
   int n = std::max(tri_stage_blocks(E), tri_tiled_blocks(E));
-  n = std::max(n, mixed_grid(mixed_chunks(S)));
+  n = std::max(n, mixed_grid(max_chunks(E)));
   n = std::max(n, mixed_dt_blocks(E));
   return (size_t)n + 1;
 }
@@ -279,6 +293,7 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
   c->perm = S.perm;
   c->faces = S.faces;
   c->xyz = S.xyz;
+  if (S.mixed) c->topo = std::move(S.topo);
   const int E = c->E;
   std::vector<double>& h_mg = S.mg;
   std::vector<double>& h_detj = S.detj;
@@ -300,29 +315,30 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
     return bail(fail(c, DG_ERR_HIP, "hipStreamCreate failed"));
   dg_status s;
   const size_t nd = c->ndof;
-  if ((s = dalloc(c, &c->Un, nd)) || (s = dalloc(c, &c->acc, nd)) || (s = dalloc(c, &c->R, nd)) ||
+  // mixed orders: state buffers for every element at the top order, so order
+  // changes rebucket in place
+  c->state_cap = c->mixed ? (size_t)E * c->Nf * np_tri(MAXP_TRI) : nd;
+  const size_t ns = c->state_cap;
+  if ((s = dalloc(c, &c->Un, ns)) || (s = dalloc(c, &c->acc, ns)) || (s = dalloc(c, &c->R, ns)) ||
       (s = dalloc(c, &c->mg, (size_t)4 * E)) || (s = dalloc(c, &c->detj, E)) || (s = dalloc(c, &c->size, E)) ||
       (s = dalloc(c, &c->link, (size_t)3 * E)) || (s = dalloc(c, &c->tab, h_tab.size())) ||
       (s = dalloc(c, &c->tabf, h_tabf.size())) || (s = dalloc(c, &c->vtx, h_vtx.size())) ||
       (s = dalloc(c, &c->part, c->part_cap = part_size(S))))
     return bail(s);
   if (c->prec == DG_PREC_FP64) {
-    if ((s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd))) return bail(s);
+    if ((s = dalloc(c, &c->A64, ns)) || (s = dalloc(c, &c->B64, ns))) return bail(s);
   } else {
     if ((s = dalloc(c, &c->A32, nd)) || (s = dalloc(c, &c->B32, nd))) return bail(s);
   }
   // every order's tables (limiter, sensor, mixed kernels) and the chunk list of
   // the persistent mixed launch (heaviest order first)
-  std::vector<int> chunks;
-  for (int p = MAXP_TRI; p >= 1; p--)
-    for (int i = S.bstart[p]; i < S.bstart[p + 1]; i += MIX_CHUNK) {
-      chunks.push_back(p);
-      chunks.push_back(i);
-    }
+  const std::vector<int> chunks = mixed_chunk_list(S.bstart);
   c->nchunks = (int)chunks.size() / 2;
-  if (c->nchunks != mixed_chunks(S)) return bail(fail(c, DG_ERR_ARG, "chunk count mismatch"));
+  c->chunk_cap = max_chunks(E);
+  if (c->nchunks > c->chunk_cap) return bail(fail(c, DG_ERR_ARG, "chunk count mismatch"));
   if ((s = dalloc(c, &c->mtab, S.mtab.size())) || (s = dalloc(c, &c->glw, S.glw.size())) ||
-      (s = dalloc(c, &c->mvtx, S.mvtx.size())) || (s = dalloc(c, &c->chunks, chunks.size() + 2)) ||
+      (s = dalloc(c, &c->mvtx, S.mvtx.size())) || (s = dalloc(c, &c->chunks, (size_t)2 * c->chunk_cap + 2)) ||
+      (c->mixed && (s = dalloc(c, &c->remap, (size_t)E))) ||
       (s = dalloc(c, &c->ntab, S.ntab.size())) ||
       (s = dalloc(c, &c->flags, (size_t)E)) || (s = dalloc(c, &c->sens, (size_t)E)))
     return bail(s);
@@ -934,7 +950,78 @@ dg_status dg_get_orders(dg_ctx* c, int32_t* orders_out) {
 // rebuild the buckets for new per-element orders, carrying the state over
 // (orthonormal hierarchical modes: raising an order appends zero modes,
 // lowering it drops the top ones)
+// A mixed-order context whose new orders are still mixed: a new internal
+// numbering over the kept topology, its element tables uploaded into the same
+// buffers, and the state carried over on the device (the state buffers hold
+// every element at the top order).  Returns false (nothing changed) when the new
+// layout does not fit.
+static bool rebucket_in_place(dg_ctx* c, const std::vector<int32_t>& neword, dg_status& st) {
+  st = DG_OK;
+  if (!c->mixed || c->topo.E != c->E || !c->remap) return false;
+  bool mixed = false;
+  for (int e = 1; e < c->E; e++)
+    if (neword[e] != neword[0]) mixed = true;
+  if (!mixed) return false;  // one order again: the uniform kernels, built anew
+  const int E = c->E;
+  TriSetup L;
+  L.E = E;
+  L.NF = c->Nf;
+  const std::vector<int> ord(neword.begin(), neword.end());
+  tri_layout(c->topo, ord.data(), L);
+  const std::vector<int> chunks = mixed_chunk_list(L.bstart);
+  if ((size_t)L.boff[5] > c->state_cap || (int)chunks.size() / 2 > c->chunk_cap) return false;
+  // new internal element i was old internal element iold[i]
+  std::vector<int32_t> at_old(E), iold(E);
+  for (int i = 0; i < E; i++) at_old[c->perm[i]] = i;
+  for (int i = 0; i < E; i++) iold[i] = at_old[L.perm[i]];
+  RemapArgs r{};
+  r.E = E;
+  r.NF = c->Nf;
+  for (int p = 0; p < 6; p++) {
+    r.bs0[p] = c->bstart[p];
+    r.bo0[p] = c->boff[p];
+    r.bs1[p] = L.bstart[p];
+    r.bo1[p] = L.boff[p];
+  }
+  r.iold = c->remap;
+  r.src = c->Un;
+  r.dst = c->A64;
+  auto up = [&](void* d, const void* h, size_t bytes) {
+    return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
+  };
+  if (!up(c->remap, iold.data(), (size_t)E * 4) || launch_remap(r, c->stream) ||
+      !up(c->mg, L.mg.data(), L.mg.size() * 8) || !up(c->detj, L.detj.data(), (size_t)E * 8) ||
+      !up(c->size, L.size.data(), (size_t)E * 8) || !up(c->link, L.link.data(), L.link.size() * 4) ||
+      !up(c->chunks, chunks.data(), chunks.size() * 4) ||
+      hipMemsetAsync(c->flags, 0, (size_t)E, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    st = fail(c, DG_ERR_HIP, "rebucketing failed: %s", hipGetErrorString(hipGetLastError()));
+    return true;
+  }
+  std::swap(c->Un, c->A64);
+  c->perm = std::move(L.perm);
+  c->order = std::move(L.order);
+  c->coff = std::move(L.coff);
+  for (int p = 0; p < 6; p++) {
+    c->bstart[p] = L.bstart[p];
+    c->boff[p] = L.boff[p];
+  }
+  c->ndof = (size_t)L.boff[5];
+  c->nchunks = (int)chunks.size() / 2;
+  c->m_order = neword;
+  // the captured step holds the old buffers and chunk count
+  const dg_schedule sc{c->shape, c->mix_grid, {0, 0, 0, 0, 0, 0}};
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  c->shape = 0;
+  c->mix_grid = 0;
+  if (schedule_ok(c, sc)) apply_schedule(c, sc);
+  return true;
+}
+
 static dg_status rebucket(dg_ctx* c, const std::vector<int32_t>& neword) {
+  dg_status st;
+  if (rebucket_in_place(c, neword, st)) return st;
   std::vector<double> U(c->ndof);
   dg_status s = dg_get_state(c, U.data(), U.size());
   if (s) return s;

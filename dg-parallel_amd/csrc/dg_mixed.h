@@ -389,7 +389,22 @@ DG_HD double dt_element(const MixedArgs& a, int i, double cfl) {
   return a.el.size[i] * cfl / (lam * (2. * p + 1.0));
 }
 
+// ---------------------------------------------------------------------------
+// The state carried over to new order buckets (rebucketing in place): element i
+// of the new numbering was element iold[i] of the old one; raised orders get
+// zero modes, lowered orders drop their top modes (DGCell::adaptOrder,
+// DGCell.cc:1542-1602).  Bucket bounds and offsets of both layouts.
+struct RemapArgs {
+  int E, NF;
+  int bs0[6], bs1[6];
+  long bo0[6], bo1[6];
+  const int32_t* iold;
+  const double* src;
+  double* dst;
+};
+
 // launches (dg_mixed.hip); part holds one value per workgroup of the launch
+int launch_remap(const RemapArgs& a, hipStream_t s);
 int mixed_grid(int nchunks);
 int launch_mixed_stage(const MixedArgs& a, int law, int grid, hipStream_t s);
 // part receives one minimum per workgroup: mixed_dt_blocks(E) values

@@ -90,6 +90,36 @@ __global__ __launch_bounds__(256) void tri_sensor(SensorArgs a) {
   if (i < a.E) sensor_element(a, i);
 }
 
+// order p (1..4) of element i and its bucket's first element and offset, by
+// comparisons and select chains (no per-lane index into the argument block)
+__device__ __forceinline__ void remap_bucket(const int* bs, const long* bo, int i, int& p, int& b0, int& b1, long& off) {
+  p = 1 + (i >= bs[2]) + (i >= bs[3]) + (i >= bs[4]);
+  b0 = p == 1 ? bs[1] : p == 2 ? bs[2] : p == 3 ? bs[3] : bs[4];
+  b1 = p == 1 ? bs[2] : p == 2 ? bs[3] : p == 3 ? bs[4] : bs[5];
+  off = p == 1 ? bo[1] : p == 2 ? bo[2] : p == 3 ? bo[3] : bo[4];
+}
+
+__global__ __launch_bounds__(256) void tri_remap(RemapArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.E) return;
+  const int j = a.iold[i];
+  int p1, s1, e1, p0, s0, e0;
+  long o1, o0;
+  remap_bucket(a.bs1, a.bo1, i, p1, s1, e1, o1);
+  remap_bucket(a.bs0, a.bo0, j, p0, s0, e0, o0);
+  const int np1 = (p1 + 1) * (p1 + 2) / 2, np0 = (p0 + 1) * (p0 + 2) / 2;
+  const long n1 = e1 - s1, n0 = e0 - s0;
+  const long d = o1 + (i - s1), r = o0 + (j - s0);
+  for (int f = 0; f < a.NF; f++)
+    for (int k = 0; k < np1; k++) a.dst[d + (long)(f * np1 + k) * n1] = k < np0 ? a.src[r + (long)(f * np0 + k) * n0] : 0.0;
+}
+
+int launch_remap(const RemapArgs& a, hipStream_t s) {
+  if (a.E <= 0) return -1;
+  hipLaunchKernelGGL(tri_remap, dim3((a.E + 255) / 256), dim3(256), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 // grid: workgroups of the persistent launch (1 .. nchunks)
 int launch_mixed_stage(const MixedArgs& a, int law, int ngrid, hipStream_t s) {
   if (a.nchunks <= 0 || ngrid <= 0 || ngrid > a.nchunks) return -1;

@@ -21,18 +21,148 @@ static dg_status failS(std::string& err, dg_status s, const char* fmt, ...) {
   return s;
 }
 
-namespace {
-struct FaceKey {
-  int32_t k[2];
-  int32_t e, l;
-  bool operator<(const FaceKey& o) const {
-    if (k[0] != o.k[0]) return k[0] < o.k[0];
-    if (k[1] != o.k[1]) return k[1] < o.k[1];
-    if (e != o.e) return e < o.e;
-    return l < o.l;
+dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err) {
+  const int E = m->n_elems;
+  T.E = E;
+  // ---- connectivity (mMesh.cc:321-378, translation-periodic by vertex ids): edges
+  //      keyed by their sorted vertex ids, ties in (element, local edge) order
+  std::vector<std::pair<uint64_t, int32_t>> keys((size_t)E * 3);
+  for (int e = 0; e < E; e++)
+    for (int l = 0; l < 3; l++) {
+      const uint32_t a = (uint32_t)m->vids[e * 3 + l], b = (uint32_t)m->vids[e * 3 + (l + 1) % 3];
+      if (a == b) return failS(err, DG_ERR_ARG, "degenerate edge in element %d", e);
+      keys[(size_t)e * 3 + l] = {(uint64_t)std::min(a, b) << 32 | std::max(a, b), e * 3 + l};
+    }
+  std::sort(keys.begin(), keys.end());
+  T.nbe.assign((size_t)E * 3, -1);
+  T.nbl.assign((size_t)E * 3, -1);
+  T.same.assign((size_t)E * 3, 0);
+  T.faces.clear();
+  T.faces.reserve((size_t)E * 3 * 4 / 2 + 16);
+  for (size_t i = 0; i < keys.size();) {
+    size_t j = i + 1;
+    while (j < keys.size() && keys[j].first == keys[i].first) j++;
+    if (j - i > 2)
+      return failS(err, DG_ERR_ARG, "edge (%u,%u) shared by %zu elements", (unsigned)(keys[i].first >> 32),
+                   (unsigned)(keys[i].first & 0xffffffffu), j - i);
+    const int Le = keys[i].second / 3, Ll = keys[i].second % 3;
+    if (j - i == 2) {
+      const int Re = keys[i + 1].second / 3, Rl = keys[i + 1].second % 3;
+      T.nbe[Le * 3 + Ll] = Re;
+      T.nbl[Le * 3 + Ll] = Rl;
+      T.nbe[Re * 3 + Rl] = Le;
+      T.nbl[Re * 3 + Rl] = Ll;
+      const int sm = m->vids[Le * 3 + Ll] == m->vids[Re * 3 + Rl];
+      T.same[Le * 3 + Ll] = T.same[Re * 3 + Rl] = sm;
+      T.faces.insert(T.faces.end(), {Le, Ll, Re, Rl});
+    } else {
+      T.faces.insert(T.faces.end(), {Le, Ll, -1, -1});
+    }
+    i = j;
   }
-};
-}  // namespace
+  T.bcode.assign((size_t)E * 3, 0);
+  T.sig.assign(E, 0);
+  for (int e = 0; e < E; e++) {
+    int sg = 0;
+    for (int l = 0; l < 3; l++) {
+      int v;
+      if (T.nbe[e * 3 + l] >= 0) {
+        v = T.nbl[e * 3 + l] | (T.same[e * 3 + l] << 2);
+      } else {
+        const int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
+        if (tag == DG_TAG_CHARACTERISTIC)
+          return failS(err, DG_ERR_UNSUPPORTED, "characteristic far-field boundary (tag 30000) is not built");
+        const int code = tag == DG_TAG_WALL ? BC_WALL : tag == DG_TAG_FARFIELD2 ? BC_FARFIELD2 : BC_FARFIELD;
+        T.bcode[e * 3 + l] = (int8_t)code;
+        v = 8 + code;
+      }
+      sg = sg * 16 + v;
+    }
+    T.sig[e] = sg;
+  }
+
+  // ---- geometry (Mapping.cc:442-566, 687-712; DGCell.cc:1209-1229) ---------------
+  T.mg.assign((size_t)4 * E, 0.0);
+  T.detj.assign(E, 0.0);
+  T.size.assign(E, 0.0);
+  for (int e = 0; e < E; e++) {
+    const double* X = m->xyz + (size_t)e * 6;
+    double J00 = X[2] - X[0], J01 = X[4] - X[0];  // dx/du, dx/dv
+    double J10 = X[3] - X[1], J11 = X[5] - X[1];  // dy/du, dy/dv
+    double det = J00 * J11 - J01 * J10;
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return failS(err, DG_ERR_ARG, "degenerate element %d", e);
+    double ad = std::fabs(det);
+    // |J| * (J^-1)[r][d]
+    double inv[2][2] = {{J11 / det, -J01 / det}, {-J10 / det, J00 / det}};
+    for (int r = 0; r < 2; r++)
+      for (int d = 0; d < 2; d++) T.mg[(size_t)e * 4 + r * 2 + d] = ad * inv[r][d];
+    T.detj[e] = ad;
+    double a = std::sqrt((X[2] - X[0]) * (X[2] - X[0]) + (X[3] - X[1]) * (X[3] - X[1]));
+    double b = std::sqrt((X[4] - X[2]) * (X[4] - X[2]) + (X[5] - X[3]) * (X[5] - X[3]));
+    double cc = std::sqrt((X[0] - X[4]) * (X[0] - X[4]) + (X[1] - X[5]) * (X[1] - X[5]));
+    double hp = 0.5 * (a + b + cc);
+    T.size[e] = std::sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
+  }
+  return DG_OK;
+}
+
+void tri_layout(const TriTopo& T, const int* ord, TriSetup& S) {
+  const int E = T.E, NF = S.NF;
+  S.mixed = false;
+  for (int e = 1; e < E; e++)
+    if (ord[e] != ord[0]) S.mixed = true;
+  S.P = S.mixed ? 0 : ord[0];
+  S.NP = S.mixed ? 0 : np_tri(ord[0]);
+  S.coff.assign(E + 1, 0);
+  for (int e = 0; e < E; e++) S.coff[e + 1] = S.coff[e] + (long)NF * np_tri(ord[e]);
+  // order buckets, blocks of BLK caller elements, signatures: each block's
+  // elements of one order are few, so a stable sort per block is cheap
+  const int BLK = 2 * 64;
+  int cnt[6] = {0, 0, 0, 0, 0, 0};
+  for (int e = 0; e < E; e++) cnt[ord[e]]++;
+  int at[6] = {0, 0, 0, 0, 0, 0};  // next free slot of each order's run in perm
+  for (int p = 2; p <= MAXP_TRI; p++) at[p] = at[p - 1] + cnt[p - 1];
+  S.perm.resize(E);
+  for (int b0 = 0; b0 < E; b0 += BLK) {
+    int run[6];  // where this block's elements of each order begin
+    for (int p = 1; p <= MAXP_TRI; p++) run[p] = at[p];
+    const int b1 = std::min(E, b0 + BLK);
+    for (int e = b0; e < b1; e++) {
+      // stable insertion by signature into this block's elements of order ord[e]
+      const int p = ord[e], sg = T.sig[e];
+      int k = at[p]++;
+      for (; k > run[p] && T.sig[S.perm[k - 1]] > sg; k--) S.perm[k] = S.perm[k - 1];
+      S.perm[k] = e;
+    }
+  }
+  std::vector<int32_t> iperm(E);
+  for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
+  S.order.resize(E);
+  for (int i = 0; i < E; i++) S.order[i] = (int8_t)ord[S.perm[i]];
+  S.bstart[0] = 0;
+  S.boff[0] = 0;
+  S.bstart[1] = 0;
+  S.boff[1] = 0;
+  for (int p = 1; p <= MAXP_TRI; p++) {
+    S.bstart[p + 1] = S.bstart[p] + cnt[p];
+    S.boff[p + 1] = S.boff[p] + (long)NF * np_tri(p) * cnt[p];
+  }
+  S.mg.resize((size_t)4 * E);
+  S.detj.resize(E);
+  S.size.resize(E);
+  S.link.resize((size_t)3 * E);
+  for (int i = 0; i < E; i++) {
+    const int e = S.perm[i];
+    for (int r = 0; r < 4; r++) S.mg[(size_t)r * E + i] = T.mg[(size_t)e * 4 + r];
+    S.detj[i] = T.detj[e];
+    S.size[i] = T.size[e];
+    for (int l = 0; l < 3; l++) {
+      const int nb = T.nbe[e * 3 + l];
+      S.link[(size_t)l * E + i] = nb >= 0 ? (iperm[nb] << 3) | (T.same[e * 3 + l] << 2) | T.nbl[e * 3 + l]
+                                           : -(int32_t)T.bcode[e * 3 + l];
+    }
+  }
+}
 
 dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TriSetup& S,
                           std::string& err) {
@@ -53,146 +183,22 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   for (int e = 0; e < S.E; e++)
     if (ord[e] < 1 || ord[e] > MAXP_TRI)
       return failS(err, DG_ERR_UNSUPPORTED, "order %d of element %d outside 1..%d", ord[e], e, MAXP_TRI);
-  int P = ord[0];
+  bool mixed = false;
   for (int e = 1; e < S.E; e++)
-    if (ord[e] != P) S.mixed = true;
+    if (ord[e] != ord[0]) mixed = true;
   if (opt->precision != DG_PREC_FP64 && opt->precision != DG_PREC_FP32) return failS(err, DG_ERR_ARG, "bad precision");
-  if (S.mixed && opt->precision != DG_PREC_FP64)
+  if (mixed && opt->precision != DG_PREC_FP64)
     return failS(err, DG_ERR_UNSUPPORTED, "mixed orders run in the FP64 form only");
-  S.P = S.mixed ? 0 : P;
-  S.NP = S.mixed ? 0 : np_tri(P);
   S.lp.gamma = law->gamma;
   for (int i = 0; i < 3; i++) S.lp.vel[i] = law->vel[i];
   for (int i = 0; i < 8; i++) S.lp.farfield[i] = law->farfield[i];
   S.lp.flux = law->flux;
-  const int E = S.E;
-  S.xyz.assign(m->xyz, m->xyz + (size_t)E * 6);
-  S.coff.assign(E + 1, 0);
-  for (int e = 0; e < E; e++) S.coff[e + 1] = S.coff[e] + (long)S.NF * np_tri(ord[e]);
-
-  // ---- connectivity (mMesh.cc:321-378, translation-periodic by vertex ids) ----
-  std::vector<FaceKey> keys((size_t)E * 3);
-  for (int e = 0; e < E; e++)
-    for (int l = 0; l < 3; l++) {
-      int32_t a = m->vids[e * 3 + l], b = m->vids[e * 3 + (l + 1) % 3];
-      if (a == b) return failS(err, DG_ERR_ARG, "degenerate edge in element %d", e);
-      FaceKey& k = keys[(size_t)e * 3 + l];
-      k.k[0] = std::min(a, b);
-      k.k[1] = std::max(a, b);
-      k.e = e;
-      k.l = l;
-    }
-  std::sort(keys.begin(), keys.end());
-  std::vector<int32_t> nbe((size_t)E * 3, -1), nbl((size_t)E * 3, -1), same((size_t)E * 3, 0);
-  for (size_t i = 0; i < keys.size();) {
-    size_t j = i + 1;
-    while (j < keys.size() && keys[j].k[0] == keys[i].k[0] && keys[j].k[1] == keys[i].k[1]) j++;
-    if (j - i > 2) return failS(err, DG_ERR_ARG, "edge (%d,%d) shared by %zu elements", keys[i].k[0], keys[i].k[1], j - i);
-    const FaceKey& L = keys[i];
-    if (j - i == 2) {
-      const FaceKey& Rk = keys[i + 1];
-      nbe[L.e * 3 + L.l] = Rk.e;
-      nbl[L.e * 3 + L.l] = Rk.l;
-      nbe[Rk.e * 3 + Rk.l] = L.e;
-      nbl[Rk.e * 3 + Rk.l] = L.l;
-      int s = m->vids[L.e * 3 + L.l] == m->vids[Rk.e * 3 + Rk.l];
-      same[L.e * 3 + L.l] = same[Rk.e * 3 + Rk.l] = s;
-      S.faces.insert(S.faces.end(), {L.e, L.l, Rk.e, Rk.l});
-    } else {
-      S.faces.insert(S.faces.end(), {L.e, L.l, -1, -1});
-    }
-    i = j;
-  }
-
-  if (m->bc)
-    for (int e = 0; e < E; e++)
-      for (int l = 0; l < 3; l++)
-        if (nbe[e * 3 + l] < 0 && m->bc[e * 3 + l] == DG_TAG_CHARACTERISTIC)
-          return failS(err, DG_ERR_UNSUPPORTED, "characteristic far-field boundary (tag 30000) is not built");
-
-  // ---- internal order: within consecutive blocks of 2 x 64 caller elements,
-  //      group elements whose face links share (nb local face, orientation,
-  //      boundary code) so a wavefront takes one path.  Blocks keep face
-  //      neighbours in nearby tiles (the same XCD's L2); on the structured
-  //      meshes each block holds the 64 lower and 64 upper triangles of 64 cells,
-  //      one tile each. ---------------------------------------------------------
-  auto sig = [&](int e) {
-    int s = 0;
-    for (int l = 0; l < 3; l++) {
-      int v;
-      if (nbe[e * 3 + l] >= 0)
-        v = nbl[e * 3 + l] | (same[e * 3 + l] << 2);
-      else {
-        int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
-        v = 8 + (tag == DG_TAG_WALL ? BC_WALL : tag == DG_TAG_FARFIELD2 ? BC_FARFIELD2 : BC_FARFIELD);
-      }
-      s = s * 16 + v;
-    }
-    return s;
-  };
-  S.perm.resize(E);
-  for (int e = 0; e < E; e++) S.perm[e] = e;
-  std::vector<int> sg(E);
-  for (int e = 0; e < E; e++) sg[e] = sig(e);
-  const int BLK = 2 * 64;
-  std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) {
-    if (ord[a] != ord[b]) return ord[a] < ord[b];  // order buckets
-    if (a / BLK != b / BLK) return a / BLK < b / BLK;
-    return sg[a] < sg[b];
-  });
-  std::vector<int32_t> iperm(E);
-  for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
-  S.order.resize(E);
-  for (int i = 0; i < E; i++) S.order[i] = (int8_t)ord[S.perm[i]];
-  {
-    int cnt[6] = {0, 0, 0, 0, 0, 0};
-    for (int e = 0; e < E; e++) cnt[ord[e]]++;
-    S.bstart[1] = 0;
-    S.boff[1] = 0;
-    for (int p = 1; p <= MAXP_TRI; p++) {
-      S.bstart[p + 1] = S.bstart[p] + cnt[p];
-      S.boff[p + 1] = S.boff[p] + (long)S.NF * np_tri(p) * cnt[p];
-    }
-  }
-
-  // ---- geometry (Mapping.cc:442-566, 687-712; DGCell.cc:1209-1229) ---------------
-  S.mg.assign((size_t)4 * E, 0.0);
-  S.detj.assign(E, 0.0);
-  S.size.assign(E, 0.0);
-  S.link.assign((size_t)3 * E, 0);
-  std::vector<double>& h_mg = S.mg;
-  std::vector<double>& h_detj = S.detj;
-  std::vector<double>& h_size = S.size;
-  std::vector<int32_t>& h_link = S.link;
-  for (int i = 0; i < E; i++) {
-    int e = S.perm[i];
-    const double* X = m->xyz + (size_t)e * 6;
-    double J00 = X[2] - X[0], J01 = X[4] - X[0];  // dx/du, dx/dv
-    double J10 = X[3] - X[1], J11 = X[5] - X[1];  // dy/du, dy/dv
-    double det = J00 * J11 - J01 * J10;
-    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return failS(err, DG_ERR_ARG, "degenerate element %d", e);
-    double ad = std::fabs(det);
-    // |J| * (J^-1)[r][d]
-    double inv[2][2] = {{J11 / det, -J01 / det}, {-J10 / det, J00 / det}};
-    for (int r = 0; r < 2; r++)
-      for (int d = 0; d < 2; d++) h_mg[(size_t)(r * 2 + d) * E + i] = ad * inv[r][d];
-    h_detj[i] = ad;
-    double a = std::sqrt((X[2] - X[0]) * (X[2] - X[0]) + (X[3] - X[1]) * (X[3] - X[1]));
-    double b = std::sqrt((X[4] - X[2]) * (X[4] - X[2]) + (X[5] - X[3]) * (X[5] - X[3]));
-    double cc = std::sqrt((X[0] - X[4]) * (X[0] - X[4]) + (X[1] - X[5]) * (X[1] - X[5]));
-    double hp = 0.5 * (a + b + cc);
-    h_size[i] = std::sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
-    for (int l = 0; l < 3; l++) {
-      int32_t v;
-      if (nbe[e * 3 + l] >= 0)
-        v = (iperm[nbe[e * 3 + l]] << 3) | (same[e * 3 + l] << 2) | nbl[e * 3 + l];
-      else {
-        int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
-        v = -(tag == DG_TAG_WALL ? BC_WALL : tag == DG_TAG_FARFIELD2 ? BC_FARFIELD2 : BC_FARFIELD);
-      }
-      h_link[(size_t)l * E + i] = v;
-    }
-  }
+  S.xyz.assign(m->xyz, m->xyz + (size_t)S.E * 6);
+  dg_status st = build_tri_topo(m, S.topo, err);
+  if (st != DG_OK) return st;
+  S.faces = S.topo.faces;
+  tri_layout(S.topo, ord.data(), S);
+  const int P = S.P;
 
   // ---- tables ---------------------------------------------------------------------
   {

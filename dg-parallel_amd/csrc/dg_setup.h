@@ -12,8 +12,24 @@
 
 namespace dgp {
 
+// What order changes keep: face connectivity, boundary codes and geometry of
+// every element, in the caller's numbering.
+struct TriTopo {
+  int E = 0;
+  std::vector<int32_t> nbe, nbl, same;  // [E*3]: neighbour, its local face, equal first vertex (-1: none)
+  std::vector<int8_t> bcode;            // [E*3]: BC_* of a face without neighbour
+  std::vector<int32_t> sig;             // [E]: link signature (faces' neighbour face, orientation, boundary)
+  std::vector<double> mg, detj, size;   // [E*4] |J| J^-1 (r*2+d), [E], [E] (2 x inradius)
+  std::vector<int32_t> faces;           // (eL, lL, eR, lR)
+};
+
+// Caller topology and geometry (mMesh.cc:321-378, Mapping.cc:442-566,
+// DGCell.cc:1209-1229); validates the mesh.
+dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err);
+
 struct TriSetup {
   int E = 0, P = 0, NP = 0, NF = 0, law = 0;
+  TriTopo topo;
   // Elements are bucketed by order: internal elements [bstart[p], bstart[p+1])
   // have order p (p = 1..4) and their coefficients start at double boff[p] of
   // the state arrays, stored [row][element] within the bucket.  A uniform-order
@@ -45,6 +61,14 @@ struct TriSetup {
 
 dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TriSetup& S,
                           std::string& err);
+
+// The internal numbering and element tables of S for per-element orders ord
+// (caller numbering, 1..4) over topology T: perm, order, bucket bounds, coff,
+// mg, detj, size, link and the mixed flag (S.E, S.NF set by the caller).
+// Internal order: order buckets, then consecutive blocks of 128 caller elements,
+// then link signature (stable), so a wavefront takes one path and face
+// neighbours sit in nearby tiles.
+void tri_layout(const TriTopo& T, const int* ord, TriSetup& S);
 
 // Tetrahedra (Maxwell, one order): the same roles as TriSetup.  mg holds
 // |J| J^-1 as [9][E] (row-major r*3+d), size the shortest edge (DGCell.cc:1230-1262),

@@ -194,3 +194,60 @@ def test_shock_tube_sequence():
         t += dt
     g.advanceInTime(0.0, dt, scheme=po.RK_TVD3, nsteps=5)
     assert rel(g.get_state(), U2) < 1e-11
+
+
+def test_rebucket_in_place_twice_and_back_to_uniform():
+    """mixed -> mixed twice (in place on the device), then every element at one
+    order (the uniform path, built anew): the state carries over each time and the
+    residual matches an oracle built on the new orders"""
+    announce("rebucket in place twice, then uniform")
+    vids, xyz, bc, orders, o, g = tube(17, nx=24, ny=8)
+    cur = orders
+    U = smooth_state(cur, 3)
+    g.set_state(U)
+    for step in range(2):
+        # sensor thresholds that raise every element (step 0: top < 2 all always,
+        # top > 2 all never) and then lower every element (step 1: top > -all)
+        want = np.clip(cur + (1 if step == 0 else -1), 1, 4).astype(np.int32)
+        s_hi = (2.0,) * 4 if step == 0 else (-1.0,) * 4
+        s_lo = (2.0,) * 4 if step == 0 else (0.0,) * 4
+        got = g.select_orders(s_hi, s_lo, 1, 4, apply=True)
+        assert np.array_equal(got, want) and np.array_equal(g.orders(), want)
+        V = reorder(U, cur, want)
+        assert g.ndof == V.size and np.array_equal(g.get_state(), V)
+        o2 = po.Oracle(2, vids, xyz, want, po.LAW_EULER2D, po.FLUX_ROE_EF, gamma=1.4, farfield=SOD_FF, bc=bc)
+        W = smooth_state(want, 10 + step)
+        g.set_state(W)
+        assert rel(g.residual(), o2.residual(W)[0]) < 1e-12
+        assert abs(g.computeDt(0.1) - o2.dt(W, 0.1)) <= 1e-13 * o2.dt(W, 0.1)
+        U, cur = W, want
+    # every element down to order 1, one order per sensor pass (the sensor moves
+    # an order by one): the last pass leaves a uniform context
+    for _ in range(3):
+        g.select_orders((-1.0,) * 4, (0.0,) * 4, 1, 4, apply=True)
+    ones = np.ones_like(cur)
+    assert np.array_equal(g.orders(), ones)
+    o3 = po.Oracle(2, vids, xyz, ones, po.LAW_EULER2D, po.FLUX_ROE_EF, gamma=1.4, farfield=SOD_FF, bc=bc)
+    W = smooth_state(ones, 20)
+    g.set_state(W)
+    assert rel(g.residual(), o3.residual(W)[0]) < 1e-12
+
+
+def test_rebucket_keeps_schedule_and_steps_match_oracle():
+    announce("rebucket keeps schedule")
+    vids, xyz, bc, orders, o, g = tube(19, nx=32, ny=8)
+    U, _ = o.limit(o.project(2, (0.53,)))
+    g.set_state(U)
+    g.set_schedule(grid=2)
+    new = g.select_orders(S_HI, S_LO, 1, 4, apply=True)
+    assert g.schedule() == (0, 2)
+    o2 = po.Oracle(2, vids, xyz, new, po.LAW_EULER2D, po.FLUX_ROE_EF, gamma=1.4, farfield=SOD_FF, bc=bc)
+    V = reorder(U, orders, new)
+    dt = 0.5 * o2.dt(V, 0.1)
+    Vo, t = V.copy(), 0.0
+    for _ in range(3):
+        Vo, ro = o2.step(Vo, po.RK_TVD3, t, dt)
+        t += dt
+    r = g.advanceInTime(0.0, dt, scheme=po.RK_TVD3, nsteps=3)
+    assert rel(g.get_state(), Vo) < 1e-12
+    assert abs(r - ro) <= 1e-10 * abs(ro)
