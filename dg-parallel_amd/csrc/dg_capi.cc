@@ -168,14 +168,24 @@ dg_status dg_faces(const dg_ctx* c, int32_t* out4, size_t n) {
 }
 
 // chunks of the persistent mixed-order launch: one per MIX_CHUNK elements of a
-// bucket, heaviest order first, as (order, first internal element) pairs
-static std::vector<int> mixed_chunk_list(const int* bstart) {
-  std::vector<int> chunks;
+// bucket, as (order, first internal element) pairs, in the caller position of
+// their first element (each bucket runs in caller-block order, so this
+// interleaves the buckets region by region: the tiles in flight together
+// gather neighbours of every order from the same part of the mesh, which stays
+// in L2), ties heaviest order first
+static std::vector<int> mixed_chunk_list(const int* bstart, const std::vector<int32_t>& perm) {
+  std::vector<std::pair<int, int>> ch;  // (order, first)
   for (int p = MAXP_TRI; p >= 1; p--)
-    for (int i = bstart[p]; i < bstart[p + 1]; i += MIX_CHUNK) {
-      chunks.push_back(p);
-      chunks.push_back(i);
-    }
+    for (int i = bstart[p]; i < bstart[p + 1]; i += MIX_CHUNK) ch.push_back({p, i});
+  std::stable_sort(ch.begin(), ch.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+    return perm[a.second] < perm[b.second];
+  });
+  std::vector<int> chunks;
+  chunks.reserve(ch.size() * 2);
+  for (auto& c : ch) {
+    chunks.push_back(c.first);
+    chunks.push_back(c.second);
+  }
   return chunks;
 }
 // the most chunks any order map of E elements gives
@@ -332,7 +342,7 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
   }
   // every order's tables (limiter, sensor, mixed kernels) and the chunk list of
   // the persistent mixed launch (heaviest order first)
-  const std::vector<int> chunks = mixed_chunk_list(S.bstart);
+  const std::vector<int> chunks = mixed_chunk_list(S.bstart, S.perm);
   c->nchunks = (int)chunks.size() / 2;
   c->chunk_cap = max_chunks(E);
   if (c->nchunks > c->chunk_cap) return bail(fail(c, DG_ERR_ARG, "chunk count mismatch"));
@@ -968,7 +978,7 @@ static bool rebucket_in_place(dg_ctx* c, const std::vector<int32_t>& neword, dg_
   L.NF = c->Nf;
   const std::vector<int> ord(neword.begin(), neword.end());
   tri_layout(c->topo, ord.data(), L);
-  const std::vector<int> chunks = mixed_chunk_list(L.bstart);
+  const std::vector<int> chunks = mixed_chunk_list(L.bstart, L.perm);
   if ((size_t)L.boff[5] > c->state_cap || (int)chunks.size() / 2 > c->chunk_cap) return false;
   // new internal element i was old internal element iold[i]
   std::vector<int32_t> at_old(E), iold(E);
@@ -1327,7 +1337,8 @@ dg_status dg_adjoint_timing(const dg_ctx* c, float* fwd_ms, float* bwd_ms) {
 
 static bool schedule_ok(const dg_ctx* c, const dg_schedule& s) {
   const bool tiled = c->dim == 3 || (!c->mixed && c->kernel == 0);
-  if (s.shape < 0 || s.shape > 1 || (s.shape && !tiled)) return false;
+  const int nshape = c->dim == 3 ? TET_SHAPES : 2;
+  if (s.shape < 0 || s.shape >= nshape || (s.shape && !tiled)) return false;
   if (s.grid < 0 || (s.grid && !c->mixed)) return false;
   if (s.grid && ((size_t)s.grid + 1 > c->part_cap || s.grid > c->nchunks)) return false;
   return true;
@@ -1374,7 +1385,7 @@ static std::vector<dg_schedule> tune_candidates(const dg_ctx* c) {
       if (schedule_ok(c, s)) v.push_back(s);
     }
   } else {
-    for (int sh = 0; sh < 2; sh++) {
+    for (int sh = 0; sh < (c->dim == 3 ? TET_SHAPES : 2); sh++) {
       dg_schedule s{};
       s.shape = sh;
       if (schedule_ok(c, s)) v.push_back(s);

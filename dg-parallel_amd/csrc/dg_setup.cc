@@ -349,15 +349,52 @@ dg_status build_tet_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
     for (int l = 0; l < 4; l++) s = s * 32 + (nbe[e * 4 + l] >= 0 ? (nbo[e * 4 + l] * 4 + nbl[e * 4 + l]) : 31);
     return s;
   };
-  S.perm.resize(E);
-  for (int e = 0; e < E; e++) S.perm[e] = e;
+  // Locality first: elements along a Morton curve of their centroids, quantised
+  // to cells of about one box of 6 tetrahedra (the cube root of 6 |bbox| / E),
+  // so a block of 384 elements is a compact brick and face neighbours across
+  // blocks sit in nearby blocks (the tiles of a workgroup window share L2 lines
+  // instead of each face of a lexicographic order reaching a whole slab away).
+  std::vector<uint64_t> key(E);
+  {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t i = 0; i < (size_t)E * 4; i++)
+      for (int d = 0; d < 3; d++) {
+        lo[d] = std::min(lo[d], m->xyz[i * 3 + d]);
+        hi[d] = std::max(hi[d], m->xyz[i * 3 + d]);
+      }
+    const double vol = std::max(1e-300, (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]));
+    const double h = std::cbrt(6.0 * vol / E);
+    auto spread = [](uint64_t x) {  // 21 bits -> every third bit
+      x &= 0x1fffff;
+      x = (x | x << 32) & 0x1f00000000ffffULL;
+      x = (x | x << 16) & 0x1f0000ff0000ffULL;
+      x = (x | x << 8) & 0x100f00f00f00f00fULL;
+      x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+      x = (x | x << 2) & 0x1249249249249249ULL;
+      return x;
+    };
+    for (int e = 0; e < E; e++) {
+      uint64_t q[3];
+      for (int d = 0; d < 3; d++) {
+        double c = 0.0;
+        for (int v = 0; v < 4; v++) c += m->xyz[((size_t)e * 4 + v) * 3 + d];
+        const double t = (0.25 * c - lo[d]) / h;
+        q[d] = (uint64_t)std::min(std::max(t, 0.0), (double)0x1fffff);
+      }
+      key[e] = spread(q[0]) | spread(q[1]) << 1 | spread(q[2]) << 2;
+    }
+  }
+  std::vector<int32_t> loc(E);
+  for (int e = 0; e < E; e++) loc[e] = e;
+  std::stable_sort(loc.begin(), loc.end(), [&](int a, int b) { return key[a] < key[b]; });
+  // then block-local grouping by link signature (as for triangles)
   std::vector<int> sg(E);
   for (int e = 0; e < E; e++) sg[e] = sig(e);
   const int BLK = 6 * 64;
-  std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) {
-    if (a / BLK != b / BLK) return a / BLK < b / BLK;
-    return sg[a] < sg[b];
-  });
+  S.perm = loc;
+  for (int b0 = 0; b0 < E; b0 += BLK)
+    std::stable_sort(S.perm.begin() + b0, S.perm.begin() + std::min(E, b0 + BLK),
+                     [&](int a, int b) { return sg[a] < sg[b]; });
   std::vector<int32_t> iperm(E);
   for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
   if ((long)E >= (1L << 26)) return failS(err, DG_ERR_ARG, "too many elements for the link encoding");

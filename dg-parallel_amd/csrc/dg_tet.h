@@ -26,6 +26,8 @@
 //   D   inverse mass 1/|J| and the RK stage update.
 #pragma once
 
+#include <type_traits>
+
 #include "dg_kernels.h"
 #include "dg_laws.h"
 #include "dg_stage_tiled.h"
@@ -74,9 +76,12 @@ DG_HD void tet_ref_normal(int l, double N[3]) {
   N[2] = l == 2 ? 0.5 : l == 0 ? -0.5 : 0.0;
 }
 
-// NW waves share a tile of 64 elements (4: 256 threads, 8: 512 threads with
-// the same LDS and fewer rows per thread)
-template <int P, int NW = 4>
+// NW waves share a tile of 64 elements (4: 256 threads, 8: 512, 16: 1024, with
+// fewer rows per thread).  ALIAS: the face traces reuse the coefficient rows in
+// LDS (80 KB at P3: two workgroups per CU; own traces are then taken from the
+// coefficients in global memory, one pass per face), else the coefficient rows
+// stay and own traces come from LDS (101 KB at P3: one workgroup per CU).
+template <int P, int NW = 4, bool ALIAS = true>
 struct TiledTet {
   using LAW = MaxwellLawD;
   static constexpr int NP = np_tet(P), NQF = nqf_tet(P), NF = 6;
@@ -89,7 +94,7 @@ struct TiledTet {
     double WL[FW][T];  // own trace, then flux times area and weight
     double WR[FW][T];  // neighbour trace
   };
-  struct Smem {
+  struct SmemAlias {
     union {
       double Us[NROW][T];  // phases A-B
       Traces W;            // phases C1-L
@@ -97,6 +102,23 @@ struct TiledTet {
     };
     int Ls[4][T];
   };
+  struct SmemKeep {
+    double Us[NROW][T];
+    union {
+      double WR[FW][T];  // neighbour trace, then flux times area and weight
+      double red[THREADS];
+    };
+    int Ls[4][T];
+  };
+  static_assert(ALIAS || FW * T >= THREADS, "reduction buffer");
+  using Smem = std::conditional_t<ALIAS, SmemAlias, SmemKeep>;
+  // the rows that hold the neighbour trace and the flux of face point row r
+  DG_HD static double* nb_row(Smem& s, int r) {
+    if constexpr (ALIAS) return s.W.WR[r]; else return s.WR[r];
+  }
+  DG_HD static double* flux_row(Smem& s, int r) {
+    if constexpr (ALIAS) return s.W.WL[r]; else return s.WR[r];
+  }
   struct Regs {
     double R[NF * ((NP + NW - 1) / NW)];
     double rsum;
@@ -177,7 +199,9 @@ struct TiledTet {
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
     const int link = s.Ls[l][t];
-    for (int task = wave; task < 2 * NF; task += NW) {
+    // ALIAS: 12 tasks, own then neighbour traces; else the 6 neighbour traces
+    constexpr int T0 = ALIAS ? 0 : NF;
+    for (int task = T0 + wave; task < 2 * NF; task += NW) {
       double v[NP];
       if (task < NF) {
         const int f = task;
@@ -187,7 +211,7 @@ struct TiledTet {
           double acc = 0.0;
 #pragma unroll
           for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
-          s.W.WL[q * NF + f][t] = acc;
+          flux_row(s, q * NF + f)[t] = acc;
         }
         continue;
       }
@@ -205,7 +229,7 @@ struct TiledTet {
           double acc = 0.0;
 #pragma unroll
           for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
-          s.W.WR[q * NF + f][t] = acc;
+          nb_row(s, q * NF + f)[t] = acc;
         }
       } else {
         const double* tn = a.tab + TetBlock::off_phin(P) + ((key & 3) * 6 + (key >> 2)) * NQF * NP;
@@ -213,7 +237,7 @@ struct TiledTet {
         for (int q = 0; q < NQF; q++) {
           double acc = 0.0;
           for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
-          s.W.WR[q * NF + f][t] = acc;
+          nb_row(s, q * NF + f)[t] = acc;
         }
       }
     }
@@ -232,14 +256,22 @@ struct TiledTet {
     const double n[3] = {N[0] / area, N[1] / area, N[2] / area};
     for (int q = wave; q < NQF; q += NW) {
       double uL[NF], uR[NF], Fh[NF];
-      for (int f = 0; f < NF; f++) {
-        uL[f] = s.W.WL[q * NF + f][t];
-        uR[f] = s.W.WR[q * NF + f][t];
+      if constexpr (ALIAS) {
+        for (int f = 0; f < NF; f++) uL[f] = s.W.WL[q * NF + f][t];
+      } else {
+        const double* to = tb + TetBlock::off_phif(P) + (l * NQF + q) * NP;
+        for (int f = 0; f < NF; f++) {
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < NP; k++) acc += to[k] * s.Us[f * NP + k][t];
+          uL[f] = acc;
+        }
       }
+      for (int f = 0; f < NF; f++) uR[f] = nb_row(s, q * NF + f)[t];
       if (link < 0) LAW::ghost(a.law, -link, n, uL, uR);
       LAW::riemann(a.law, n, uL, uR, Fh);
       const double sw = area * tb[TetBlock::off_wf(P) + q];
-      for (int f = 0; f < NF; f++) s.W.WL[q * NF + f][t] = Fh[f] * sw;
+      for (int f = 0; f < NF; f++) flux_row(s, q * NF + f)[t] = Fh[f] * sw;
     }
   }
 
@@ -249,7 +281,7 @@ struct TiledTet {
     for (int q = 0; q < NQF; q++) {
       double Fq[NF];
 #pragma unroll
-      for (int f = 0; f < NF; f++) Fq[f] = s.W.WL[q * NF + f][t];
+      for (int f = 0; f < NF; f++) Fq[f] = flux_row(s, q * NF + f)[t];
 #pragma unroll
       for (int ks = 0; ks < KS; ks++) {
         const int k = kmode(wave, ks);
@@ -291,7 +323,9 @@ struct TiledTet {
   }
 };
 
-// shape: 0 for 4 waves per tile, 1 for 8
+// shape: 0 for 4 waves per tile, 1 for 8 (traces over the coefficient rows);
+// 2 for 8, 3 for 16 (coefficient rows kept)
+constexpr int TET_SHAPES = 4;
 int launch_tet_stage(const TetArgs& a, int shape, hipStream_t s);
 int tet_tiles(int E);
 

@@ -11,9 +11,9 @@ __device__ __forceinline__ int xcd_tile_tet(int b, int nb) {
   return (b % 8) * (nb / 8) + b / 8;
 }
 
-template <int P, int NW>
+template <int P, int NW, bool ALIAS>
 __global__ __launch_bounds__(NW * 64) void tet_stage_tiled(TetArgs a) {
-  using K = TiledTet<P, NW>;
+  using K = TiledTet<P, NW, ALIAS>;
   __shared__ typename K::Smem sm;
   typename K::Regs g;
   const int tile = xcd_tile_tet(blockIdx.x, gridDim.x);
@@ -45,13 +45,13 @@ __global__ __launch_bounds__(NW * 64) void tet_stage_tiled(TetArgs a) {
 
 int tet_tiles(int E) { return (E + TILE_ELEMS - 1) / TILE_ELEMS; }
 
-template <int NW>
+template <int NW, bool ALIAS>
 static int launch_tet_nw(const TetArgs& a, hipStream_t s) {
   dim3 grid(tet_tiles(a.E)), block(NW * 64);
   switch (a.P) {
-    case 1: hipLaunchKernelGGL((tet_stage_tiled<1, NW>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((tet_stage_tiled<2, NW>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((tet_stage_tiled<3, NW>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((tet_stage_tiled<1, NW, ALIAS>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((tet_stage_tiled<2, NW, ALIAS>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((tet_stage_tiled<3, NW, ALIAS>), grid, block, 0, s, a); break;
     default: return -1;
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -59,8 +59,10 @@ static int launch_tet_nw(const TetArgs& a, hipStream_t s) {
 
 int launch_tet_stage(const TetArgs& a, int shape, hipStream_t s) {
   switch (shape) {
-    case 0: return launch_tet_nw<4>(a, s);
-    case 1: return launch_tet_nw<8>(a, s);
+    case 0: return launch_tet_nw<4, true>(a, s);
+    case 1: return launch_tet_nw<8, true>(a, s);
+    case 2: return launch_tet_nw<8, false>(a, s);
+    case 3: return launch_tet_nw<16, false>(a, s);
   }
   return -1;
 }

@@ -576,9 +576,9 @@ extern "C" int emu_sensor(const dg_mesh_desc* mesh, const dg_law_desc* law, int 
 
 namespace {
 
-template <int P, int NW>
+template <int P, int NW, bool ALIAS>
 void tet_tiles_run(const TetArgs& a, double* rsum) {
-  using K = TiledTet<P, NW>;
+  using K = TiledTet<P, NW, ALIAS>;
   constexpr int TH = K::THREADS;
   std::unique_ptr<typename K::Smem> sm(new typename K::Smem());
   std::vector<typename K::Regs> g(TH);
@@ -599,19 +599,25 @@ void tet_tiles_run(const TetArgs& a, double* rsum) {
   }
 }
 
-template <int NW>
+template <int NW, bool ALIAS>
 int tet_run_nw(const TetArgs& a, double* rsum) {
   switch (a.P) {
-    case 1: tet_tiles_run<1, NW>(a, rsum); return 0;
-    case 2: tet_tiles_run<2, NW>(a, rsum); return 0;
-    case 3: tet_tiles_run<3, NW>(a, rsum); return 0;
+    case 1: tet_tiles_run<1, NW, ALIAS>(a, rsum); return 0;
+    case 2: tet_tiles_run<2, NW, ALIAS>(a, rsum); return 0;
+    case 3: tet_tiles_run<3, NW, ALIAS>(a, rsum); return 0;
   }
   return -1;
 }
 
-// shape as the device launcher's: 0 for 4 waves per tile, 1 for 8
+// shape as the device launcher's
 int tet_run(const TetArgs& a, int shape, double* rsum) {
-  return shape == 0 ? tet_run_nw<4>(a, rsum) : shape == 1 ? tet_run_nw<8>(a, rsum) : -1;
+  switch (shape) {
+    case 0: return tet_run_nw<4, true>(a, rsum);
+    case 1: return tet_run_nw<8, true>(a, rsum);
+    case 2: return tet_run_nw<8, false>(a, rsum);
+    case 3: return tet_run_nw<16, false>(a, rsum);
+  }
+  return -1;
 }
 
 struct TetEmu {

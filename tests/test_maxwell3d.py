@@ -105,11 +105,12 @@ emu = pytest.mark.skipif(LIBEMU is None, reason="emulator not built")
 
 
 @emu
-@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
 @pytest.mark.parametrize("flux", [po.FLUX_UPWIND, po.FLUX_CENTRAL])
 @pytest.mark.parametrize("p", [1, 2, 3])
 def test_emulated_tet_residual(flux, p, shape):
-    """both tile shapes (4 and 8 waves per 64 tetrahedra)"""
+    """every tile shape: 4 / 8 waves per 64 tetrahedra with traces over the
+    coefficient rows, 8 / 16 waves with the rows kept"""
     vids, xyz, o = oracle(2, p, flux)
     U = np.random.default_rng(20 + p).standard_normal(o.ndof)
     md = dgpar.Mesh(3, vids, xyz).desc()
@@ -122,7 +123,7 @@ def test_emulated_tet_residual(flux, p, shape):
 
 
 @emu
-@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
 def test_emulated_tet_rk4_steps(shape):
     p = 3
     vids, xyz, o = oracle(2, p, po.FLUX_UPWIND)
