@@ -164,10 +164,18 @@ def run_mixed(dgpar, steps, sense_every=20, tune=True):
         sense_ms += (time.perf_counter() - t0) * 1e3
     flags = g.limiter_flags()
     hist = np.bincount(g.orders(), minlength=5)[1:].tolist()
+    # algorithmic bytes per DOF*stage: SSP-RK3 stage traffic (stage 1 reads U^n
+    # and writes A: 16 B; stages 2, 3 read the stage input and U^n and write:
+    # 24 B each), the element table (|J| J^-1, |J|, 3 links: 52 B per element
+    # over Nf * mean Np DOF) and the limiter's read of the stage output (8 B)
+    mean_np = sum(c * (p + 1) * (p + 2) // 2 for p, c in zip((1, 2, 3, 4), hist)) / E
+    b = (16 + 24 + 24) / 3.0 + 52.0 / (4 * mean_np) + 8.0
+    ach = b * dof_stages / (step_ms / 1e3) / 1e9
     out = {"workload": "euler2d Sod tube 512x128x2 [0,1]x[0,0.25] jitter 0.1h, orders P1-P4 seeded, Roe+entropy fix, "
                        "SSP-RK3 CFL0.1, limiter every stage, sensor every %d steps" % sense_every,
            "steps": steps, "dt": dt, "value": dof_stages / (step_ms / 1e3), "unit": "DOF*stage/s",
-           "ms_per_step": step_ms / steps, "sensor_rebucket_ms_per_pass": sense_ms / max(1, -(-steps // sense_every)),
+           "ms_per_step": step_ms / steps, "achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS, "bytes_per_dof_stage": b,
+           "sensor_rebucket_ms_per_pass": sense_ms / max(1, -(-steps // sense_every)),
            "orders_histogram_p1_p4": hist, "limited_elements_last_stage": int((flags > 0).sum()),
            "steps_to_t0.2": int(np.ceil(0.2 / dt)), "schedule": sched}
     g.close()
@@ -199,16 +207,20 @@ def run_maxwell(dgpar, steps, n=48, tune=True):
             "l2_error_E_H": [float(x) for x in err], "schedule": sched}
 
 
-def cpu_baseline(mesh, steps=5):
+def cpu_baseline(mesh, budget_s=12.0):
+    """RK4 steps of the FP64 CPU oracle on the same mesh until about budget_s of
+    CPU work (a bounded sample of the workload)"""
     from oracle import pyoracle as po
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     o = po.Oracle(2, mesh.vids, mesh.xyz, ORDER, po.LAW_EULER2D, po.FLUX_LF, gamma=1.4)
     U = o.project(1, VORTEX)
     dt = o.dt(U, CFL)
     t0 = time.perf_counter()
-    for s in range(steps):
-        U, _ = o.step(U, po.RK4, s * dt, dt, nthreads=threads)
-    el = time.perf_counter() - t0
+    steps, el = 0, 0.0
+    while el < budget_s:
+        U, _ = o.step(U, po.RK4, steps * dt, dt, nthreads=threads)
+        steps += 1
+        el = time.perf_counter() - t0
     return dict(value=o.ndof * 4 * steps / el, unit="DOF*stage/s", cores=threads, kind="port",
                 sample="%d RK4 steps of the full 131072-triangle P3 vortex mesh (FP64 CPU oracle, %d threads, %.1f s)"
                 % (steps, threads, el))
