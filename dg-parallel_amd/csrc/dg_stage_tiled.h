@@ -140,10 +140,14 @@ struct TiledStage {
     double red[THREADS];
   };
 
-  // per-thread output rows of its own element
+  // per-thread output rows of its own element, and the neighbour coefficients of
+  // its face tasks, loaded in phase A so their latency hides behind B and D1
+  // (a workgroup barrier waits for LDS and scalar loads only)
   struct Regs {
     CT R[FPW * KPW];
     double rsum;
+    CT nv[CPW][NP];
+    int lk[CPW];
   };
 
   DG_HD static const CT* table(const StageArgs& a) {
@@ -161,16 +165,50 @@ struct TiledStage {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
+    const int es = valid ? e : 0;  // padding lanes load element 0 (in bounds) and discard it
     const IT* Uin = (const IT*)a.Uin;
-    for (int r = wave; r < NROW; r += NW) s.Us[r][t] = valid ? CT(Uin[(long)r * E + e]) : CT(r == 0 ? 1 : 0);
-    for (int r = wave; r < 4; r += NW) s.Ms[r][t] = valid ? CT(a.el.mg[(long)r * E + e]) : CT(r == 0 || r == 3 ? 1 : 0);
-    for (int l = wave; l < 3; l += NW) {
-      s.Ls[l][t] = valid ? a.el.link[(long)l * E + e] : -BC_FARFIELD;
+    // Every global load of the phase is issued, unconditionally and in this
+    // order, before any of them is used: the links (the neighbour addresses need
+    // only them; the memory counter is in order), the tile's rows and |J| J^-1,
+    // then C1's neighbour rows.  The LDS writes come last.
+    int lk3[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) lk3[l] = a.el.link[(long)l * E + es];
+    constexpr int RA = (NROW + NW - 1) / NW;
+    CT u[RA];
+#pragma unroll
+    for (int j = 0; j < RA; j++) {
+      const int r = wave + NW * j;
+      u[j] = r < NROW ? CT(Uin[(long)r * E + es]) : CT(0);
+    }
+    CT m[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) m[r] = CT(a.el.mg[(long)r * E + es]);
+#pragma unroll
+    for (int j = 0; j < CPW; j++) {
+      const int it = wave + NW * j;
+      const int l = it < 3 * NF ? it / NF : 0, f = it < 3 * NF ? it - l * NF : 0;
+      const int lk = valid ? lk3[l] : -BC_FARFIELD;
+      g.lk[j] = lk;
+      const int src = lk >= 0 ? (lk >> 3) : es;  // boundary faces read their own element and discard it
+#pragma unroll
+      for (int k = 0; k < NP; k++) g.nv[j][k] = CT(Uin[(long)(f * NP + k) * E + src]);
+    }
+#pragma unroll
+    for (int j = 0; j < RA; j++) {
+      const int r = wave + NW * j;
+      if (r < NROW) s.Us[r][t] = valid ? u[j] : CT(r == 0 ? 1 : 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (!valid) m[r] = CT(r == 0 || r == 3 ? 1 : 0);
+      if (r == wave) s.Ms[r][t] = m[r];
+    }
+    if (wave < 3) {
+      const int l = wave;
+      s.Ls[l][t] = valid ? lk3[l] : -BC_FARFIELD;
       // outward normal x edge length = |J| J^-T nhat |fhat|, with nhat |fhat| of
       // the reference edges (0,-1), (1,1), (-1,0)
-      CT m[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) m[r] = valid ? CT(a.el.mg[(long)r * E + e]) : CT(r == 0 || r == 3 ? 1 : 0);
       CT nl0, nl1;
       if (l == 0) {
         nl0 = -m[2];
@@ -260,31 +298,18 @@ struct TiledStage {
     }
   }
 
-  DG_HD static void phaseC1(const StageArgs& a, Smem& s, int tid, int tile, const Regs&) {
+  DG_HD static void phaseC1(const StageArgs& a, Smem& s, int tid, int tile, const Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int e = tile * T + t;
     const bool valid = e < a.E;
     const CT* tb = table(a);
-    const IT* Uin = (const IT*)a.Uin;
-    // issue every neighbour load of this thread before any arithmetic (boundary and
-    // padding lanes read their own element, in bounds, and discard it)
-    CT v[CPW][NP];
-    int lk[CPW];
-#pragma unroll
-    for (int j = 0; j < CPW; j++) {
-      const int it = wave + NW * j;
-      const int l = it < 3 * NF ? it / NF : 0, f = it < 3 * NF ? it - l * NF : 0;
-      lk[j] = s.Ls[l][t];
-      const int src = (valid && lk[j] >= 0) ? (lk[j] >> 3) : (valid ? e : 0);
-#pragma unroll
-      for (int k = 0; k < NP; k++) v[j][k] = CT(Uin[(long)(f * NP + k) * a.E + src]);
-    }
+    const auto& v = g.nv;  // loaded in phase A
 #pragma unroll
     for (int j = 0; j < CPW; j++) {
       const int it = wave + NW * j;
       if (it < 3 * NF) {
         const int l = it / NF, f = it - l * NF;
-        const int link = lk[j];
+        const int link = g.lk[j];
         CT out[NQF];
         if (valid && link >= 0) {
           const int lb = link & 3;
