@@ -72,12 +72,27 @@ struct TiledVjp {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
-    for (int r = wave; r < NROW; r += 4) {
-      s.Us[r][t] = valid ? a.U[(long)r * E + e] : (r == 0 ? 1.0 : 0.0);
-      s.Zs[r][t] = valid ? a.Z[(long)r * E + e] : 0.0;
+    const int es = valid ? e : 0;  // padding lanes load element 0 (in bounds) and discard it
+    // every load is issued, unconditionally, before the LDS writes use any of them
+    double u[RPW], z[RPW];
+#pragma unroll
+    for (int j = 0; j < RPW; j++) {
+      const int r = wave + 4 * j;
+      u[j] = r < NROW ? a.U[(long)r * E + es] : 0.0;
+      z[j] = r < NROW ? a.Z[(long)r * E + es] : 0.0;
     }
-    if (wave < 4) s.Ms[wave][t] = valid ? a.el.mg[(long)wave * E + e] : (wave == 0 || wave == 3 ? 1.0 : 0.0);
-    if (wave < 3) s.Ls[wave][t] = valid ? a.el.link[(long)wave * E + e] : -BC_FARFIELD;
+    const double mg = a.el.mg[(long)(wave & 3) * E + es];
+    const int lk = a.el.link[(long)(wave < 3 ? wave : 0) * E + es];
+#pragma unroll
+    for (int j = 0; j < RPW; j++) {
+      const int r = wave + 4 * j;
+      if (r < NROW) {
+        s.Us[r][t] = valid ? u[j] : (r == 0 ? 1.0 : 0.0);
+        s.Zs[r][t] = valid ? z[j] : 0.0;
+      }
+    }
+    if (wave < 4) s.Ms[wave][t] = valid ? mg : (wave == 0 || wave == 3 ? 1.0 : 0.0);
+    if (wave < 3) s.Ls[wave][t] = valid ? lk : -BC_FARFIELD;
   }
 
   // volume: ubar_q = w_q sum_d (dF_d/du)^T Y_d, Y_d = sum_r mg[r][d] sum_k dphi_k^r z_k
@@ -285,30 +300,40 @@ struct TiledVjp {
     const int E = a.E, e = tile * T + t;
     if (e >= E) return;
     const double dtj = a.dt / a.el.detj[e];
+    // the rows' lam and mu are loaded before any is used
+    const bool rk = a.mode != ADJ_PLAIN;
+    double lamv[RPW], muv[RPW];
+#pragma unroll
+    for (int j = 0; j < RPW; j++) {
+      const int row = wave + 4 * j;
+      const long i = (long)(row < NROW ? row : 0) * E + e;
+      lamv[j] = rk ? a.lam[i] : 0.0;
+      muv[j] = rk && a.stage < 3 ? a.mu[i] : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < RPW; j++) {
       const int row = wave + 4 * j;
       if (row < NROW) {
         const double acc = g.R[j];
         const long i = (long)row * E + e;
-        if (a.mode == ADJ_PLAIN) {
+        if (!rk) {
           a.W[i] = acc;
         } else {
           // reverse RK4 (forward stages TimeIntegrator.cc:81-275): with ubar_s the
           // adjoint through stage s, mu = lam + ubar_3 + ubar_2 + ubar_1 + ubar_0 and
           // kbar_2 = lam/3 + ubar_3, kbar_1 = lam/3 + ubar_2/2, kbar_0 = lam/6 + ubar_1/2
-          const double lam = a.lam[i];
+          const double lam = lamv[j];
           if (a.stage == 3) {
             a.mu[i] = lam + acc;
             a.znext[i] = dtj * (lam * (1.0 / 3.0) + acc);
           } else if (a.stage == 2) {
-            a.mu[i] += acc;
+            a.mu[i] = muv[j] + acc;
             a.znext[i] = dtj * (lam * (1.0 / 3.0) + 0.5 * acc);
           } else if (a.stage == 1) {
-            a.mu[i] += acc;
+            a.mu[i] = muv[j] + acc;
             a.znext[i] = dtj * (lam * (1.0 / 6.0) + 0.5 * acc);
           } else {
-            a.mu[i] += acc;
+            a.mu[i] = muv[j] + acc;
           }
         }
       }

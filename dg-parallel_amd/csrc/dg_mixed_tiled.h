@@ -109,15 +109,35 @@ struct TiledMixed {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int E = a.E, i = i0 + t;
     const bool valid = i < a.bstart[P + 1];
-    for (int r = wave; r < NROW; r += 4) s.Us[r][t] = valid ? a.Uin[idx(a, i, r)] : (r == 0 ? 1.0 : 0.0);
-    s.Ms[wave][t] = valid ? a.el.mg[(long)wave * E + i] : (wave == 0 || wave == 3 ? 1.0 : 0.0);
+    const int is = valid ? i : a.bstart[P];  // padding lanes load the bucket's first element and discard it
+    // every load is issued, unconditionally, before any is used
+    constexpr int RA = (NROW + 3) / 4;
+    double u[RA];
+#pragma unroll
+    for (int j = 0; j < RA; j++) {
+      const int r = wave + 4 * j;
+      u[j] = r < NROW ? a.Uin[idx(a, is, r)] : 0.0;
+    }
+    double m[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) m[r] = a.el.mg[(long)r * E + is];
+    const int lk = a.el.link[(long)(wave < 3 ? wave : 0) * E + is];
+#pragma unroll
+    for (int j = 0; j < RA; j++) {
+      const int r = wave + 4 * j;
+      if (r < NROW) s.Us[r][t] = valid ? u[j] : (r == 0 ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (!valid) m[r] = r == 0 || r == 3 ? 1.0 : 0.0;
+    s.Ms[wave][t] = m[wave];
     if (wave < 3) {
       const int l = wave;
-      const int link = valid ? a.el.link[(long)l * E + i] : -BC_FARFIELD;
+      const int link = valid ? lk : -BC_FARFIELD;
       s.Ls[l][t] = link;
       double mg[2][2];
       for (int r = 0; r < 2; r++)
-        for (int d = 0; d < 2; d++) mg[r][d] = valid ? a.el.mg[(long)(r * 2 + d) * E + i] : (r == d ? 1.0 : 0.0);
+        for (int d = 0; d < 2; d++) mg[r][d] = m[r * 2 + d];
       double n[2], len;
       edge_normal(mg, l, n, len);
       s.Ns[l][0][t] = n[0];

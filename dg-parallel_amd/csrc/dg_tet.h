@@ -128,8 +128,22 @@ struct TiledTet {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
-    for (int r = wave; r < NROW; r += NW) s.Us[r][t] = valid ? a.Uin[(long)r * E + e] : 0.0;
-    for (int l = wave; l < 4; l += NW) s.Ls[l][t] = valid ? a.link[(long)l * E + e] : -BC_PEC;
+    const int es = valid ? e : 0;  // padding lanes load element 0 (in bounds) and discard it
+    // every row load is issued, unconditionally, before the LDS writes use any
+    constexpr int RA = (NROW + NW - 1) / NW;
+    double u[RA];
+#pragma unroll
+    for (int j = 0; j < RA; j++) {
+      const int r = wave + NW * j;
+      u[j] = r < NROW ? a.Uin[(long)r * E + es] : 0.0;
+    }
+    const int lk = a.link[(long)(wave & 3) * E + es];
+#pragma unroll
+    for (int j = 0; j < RA; j++) {
+      const int r = wave + NW * j;
+      if (r < NROW) s.Us[r][t] = valid ? u[j] : 0.0;
+    }
+    if (wave < 4) s.Ls[wave][t] = valid ? lk : -BC_PEC;
   }
 
   // |J| J^-1 of the lane's element (the identity past the end of the mesh)
