@@ -68,6 +68,10 @@ struct dg_ctx {
   double lim_m = 0.0, lim_nu = 1.0;
   double lim_midv[3][15], lim_pinv[2][3];
   hipStream_t stream = nullptr;
+  // mixed orders with the limiter: three side streams and fork/join events that
+  // let the order buckets' limiter launches run side by side (also when captured)
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
   // device memory
   double *Un = nullptr, *acc = nullptr, *A64 = nullptr, *B64 = nullptr, *R = nullptr;
   float *A32 = nullptr, *B32 = nullptr;
@@ -139,6 +143,14 @@ static void release(dg_ctx* c, void* p) {
 static void free_all(dg_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   c->gexec = nullptr;
+  for (int i = 0; i < 3; i++) {
+    if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
+    if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
+    c->side[i] = nullptr;
+    c->ev_join[i] = nullptr;
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  c->ev_fork = nullptr;
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -323,6 +335,14 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
   if (hipSetDevice(c->device) != hipSuccess) return bail(fail(c, DG_ERR_HIP, "hipSetDevice failed"));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(c, DG_ERR_HIP, "hipStreamCreate failed"));
+  if (c->mixed) {
+    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(c, DG_ERR_HIP, "hipEventCreate failed"));
+    for (int j = 0; j < 3; j++)
+      if (hipStreamCreateWithFlags(&c->side[j], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ev_join[j], hipEventDisableTiming) != hipSuccess)
+        return bail(fail(c, DG_ERR_HIP, "side stream creation failed"));
+  }
   dg_status s;
   const size_t nd = c->ndof;
   // mixed orders: state buffers for every element at the top order, so order
@@ -553,8 +573,24 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
     r = c->kernel == 1 ? launch_tri_stage(a, c->law, variant, c->stream)
                        : launch_tri_stage_tiled(a, c->law, variant | c->shape << 2, c->stream);
   }
-  if (r == 0 && c->lim_on && a.scheme != MODE_RESIDUAL)
-    r = launch_limiter(limiter_args(c, (double*)a.Uout), c->law, c->stream);
+  if (r == 0 && c->lim_on && a.scheme != MODE_RESIDUAL) {
+    const LimiterArgs la = limiter_args(c, (double*)a.Uout);
+    if (c->mixed && c->side[0]) {
+      // orders 2..4 on the side streams, order 1 on the context stream, joined
+      // before anything that follows on the context stream
+      if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return -2;
+      for (int j = 0; j < 3; j++)
+        if (hipStreamWaitEvent(c->side[j], c->ev_fork, 0) != hipSuccess ||
+            launch_limiter_bucket(la, c->law, 2 + j, c->side[j]) ||
+            hipEventRecord(c->ev_join[j], c->side[j]) != hipSuccess)
+          return -2;
+      if (launch_limiter_bucket(la, c->law, 1, c->stream)) return -2;
+      for (int j = 0; j < 3; j++)
+        if (hipStreamWaitEvent(c->stream, c->ev_join[j], 0) != hipSuccess) return -2;
+    } else {
+      r = launch_limiter(la, c->law, c->stream);
+    }
+  }
   return r;
 }
 

@@ -411,6 +411,9 @@ int launch_mixed_stage(const MixedArgs& a, int law, int grid, hipStream_t s);
 int mixed_dt_blocks(int E);
 int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipStream_t s);
 int launch_limiter(const LimiterArgs& a, int law, hipStream_t s);
+// the limiter of one order bucket p; buckets are independent (each rewrites only
+// its own elements and reads only neighbour means, which limiting keeps)
+int launch_limiter_bucket(const LimiterArgs& a, int law, int p, hipStream_t s);
 int launch_sensor(const SensorArgs& a, hipStream_t s);
 
 }  // namespace dgp

@@ -147,22 +147,27 @@ int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipSt
 }
 
 template <class LAW>
-static void launch_limiter_law(const LimiterArgs& a, hipStream_t s) {
+static void launch_limiter_law(const LimiterArgs& a, int p, hipStream_t s) {
   void (*kern[4])(LimiterArgs) = {tri_limit<LAW, 1>, tri_limit<LAW, 2>, tri_limit<LAW, 3>, tri_limit<LAW, 4>};
-  for (int p = 1; p <= MAXP_TRI; p++) {
-    const int n = a.bstart[p + 1] - a.bstart[p];
-    if (n > 0) hipLaunchKernelGGL(kern[p - 1], dim3((n + 255) / 256), dim3(256), 0, s, a);
-  }
+  const int n = a.bstart[p + 1] - a.bstart[p];
+  if (n > 0) hipLaunchKernelGGL(kern[p - 1], dim3((n + 255) / 256), dim3(256), 0, s, a);
 }
 
-int launch_limiter(const LimiterArgs& a, int law, hipStream_t s) {
+int launch_limiter_bucket(const LimiterArgs& a, int law, int p, hipStream_t s) {
+  if (p < 1 || p > MAXP_TRI) return -1;
   if (law == LAW_SCALAR)
-    launch_limiter_law<ScalarLawD>(a, s);
+    launch_limiter_law<ScalarLawD>(a, p, s);
   else if (law == LAW_EULER2D)
-    launch_limiter_law<EulerLawD>(a, s);
+    launch_limiter_law<EulerLawD>(a, p, s);
   else
     return -1;
   return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_limiter(const LimiterArgs& a, int law, hipStream_t s) {
+  for (int p = 1; p <= MAXP_TRI; p++)
+    if (launch_limiter_bucket(a, law, p, s)) return -2;
+  return 0;
 }
 
 int launch_sensor(const SensorArgs& a, hipStream_t s) {
