@@ -115,9 +115,12 @@ struct TiledStage {
   static constexpr int T = TILE_ELEMS;
   static constexpr int THREADS = NW * T;
   static constexpr int NROW = NF * NP;
-  static constexpr int GW = NQ * NF * 2;   // volume-flux rows
-  static constexpr int FW = 3 * NQF * NF;  // trace / face-flux rows
-  static constexpr int WORK = GW > 2 * FW ? GW : 2 * FW;
+  // LDS work rows: the volume points run in two passes of QH points, and each
+  // face flux is written over the neighbour trace of its point
+  static constexpr int QH = (NQ + 1) / 2;
+  static constexpr int GW = QH * NF * 2;   // volume-flux rows of one pass
+  static constexpr int FW = 3 * NQF * NF;  // trace, then face-flux rows
+  static constexpr int WORK = GW > FW ? GW : FW;
   // output rows of a wave: with four or more fields the waves form FG field
   // groups (f = w % FG, + FG, ..) times KG mode groups (k = w / FG, + KG, ..);
   // a single field is split by modes only -- either way the rows of one field
@@ -127,7 +130,7 @@ struct TiledStage {
   static constexpr int KG = NW / FG;
   static constexpr int FPW = (NF + FG - 1) / FG;
   static constexpr int KPW = (NP + KG - 1) / KG;
-  static constexpr int QPW = (NQ + NW - 1) / NW;
+  static constexpr int QPW = (QH + NW - 1) / NW;
   static constexpr int CPW = (3 * NF + NW - 1) / NW;
   static constexpr int FPPW = (3 * NQF + NW - 1) / NW;
 
@@ -136,8 +139,10 @@ struct TiledStage {
     CT Ms[4][T];
     CT Ns[3][3][T];  // per face: unit outward normal (2) and edge length
     int Ls[3][T];
-    CT W[WORK][T];
-    double red[THREADS];
+    union {
+      CT W[WORK][T];
+      double red[THREADS];  // the tile's residual-norm reduction, after D2
+    };
   };
 
   // per-thread output rows of its own element, and the neighbour coefficients of
@@ -171,6 +176,8 @@ struct TiledStage {
     // order, before any of them is used: the links (the neighbour addresses need
     // only them; the memory counter is in order), the tile's rows and |J| J^-1,
     // then C1's neighbour rows.  The LDS writes come last.
+#pragma unroll
+    for (int i = 0; i < FPW * KPW; i++) g.R[i] = CT(0);
     int lk3[3];
 #pragma unroll
     for (int l = 0; l < 3; l++) lk3[l] = a.el.link[(long)l * E + es];
@@ -227,14 +234,14 @@ struct TiledStage {
     }
   }
 
-  // ---- B: volume flux at the quadrature points ---------------------------------
-  DG_HD static void phaseB(const StageArgs& a, Smem& s, int tid, const Regs& g) {
+  // ---- B: volume flux at the quadrature points [q0, q0 + QH) ----------------------
+  DG_HD static void phaseB(const StageArgs& a, Smem& s, int tid, const Regs& g, int q0) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const CT* tb = table(a);
 #pragma unroll 1
     for (int j = 0; j < QPW; j++) {
-      const int q = wave + NW * j;
-      if (q < NQ) {
+      const int qi = wave + NW * j, q = q0 + qi;
+      if (qi < QH && q < NQ) {
         CT u[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
@@ -249,26 +256,25 @@ struct TiledStage {
         const CT m00 = s.Ms[0][t], m01 = s.Ms[1][t], m10 = s.Ms[2][t], m11 = s.Ms[3][t];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-          s.W[(q * NF + f) * 2 + 0][t] = w * (m00 * F[f][0] + m01 * F[f][1]);
-          s.W[(q * NF + f) * 2 + 1][t] = w * (m10 * F[f][0] + m11 * F[f][1]);
+          s.W[(qi * NF + f) * 2 + 0][t] = w * (m00 * F[f][0] + m01 * F[f][1]);
+          s.W[(qi * NF + f) * 2 + 1][t] = w * (m10 * F[f][0] + m11 * F[f][1]);
         }
       }
     }
   }
 
-  // ---- D1: contraction with the basis gradients ----------------------------
-  DG_HD static void phaseD1(const StageArgs& a, Smem& s, int tid, Regs& g) {
+  // ---- D1: contraction of points [q0, q0 + QH) with the basis gradients ------
+  DG_HD static void phaseD1(const StageArgs& a, Smem& s, int tid, Regs& g, int q0) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const CT* tb = table(a);
-#pragma unroll
-    for (int i = 0; i < FPW * KPW; i++) g.R[i] = CT(0);
+    const int q1 = q0 + QH < NQ ? q0 + QH : NQ;
 #pragma unroll
     for (int fi = 0; fi < FPW; fi++) {
       const int f = row_f(wave, fi);
       if (f < NF) {
-        for (int q = 0; q < NQ; q++) {
-          const CT G0 = s.W[(q * NF + f) * 2 + 0][t];
-          const CT G1 = s.W[(q * NF + f) * 2 + 1][t];
+        for (int q = q0; q < q1; q++) {
+          const CT G0 = s.W[((q - q0) * NF + f) * 2 + 0][t];
+          const CT G1 = s.W[((q - q0) * NF + f) * 2 + 1][t];
 #pragma unroll
           for (int ki = 0; ki < KPW; ki++) {
             const int k = row_k(wave, ki);
@@ -367,7 +373,7 @@ struct TiledStage {
         // |J_face| w_q = (len / 2) w_q; the table holds w_q / 2
         const CT sw = len * tb[TriBlock::off_wf(P) + q];
 #pragma unroll
-        for (int f = 0; f < NF; f++) s.W[FW + (l * NQF + q) * NF + f][t] = Fh[f] * sw;
+        for (int f = 0; f < NF; f++) s.W[(l * NQF + q) * NF + f][t] = Fh[f] * sw;  // over the neighbour trace
       }
     }
   }
@@ -404,7 +410,7 @@ struct TiledStage {
         // every table row into SGPRs and spills them to VGPR lanes
 #pragma unroll NQF
         for (int lq = 0; lq < 3 * NQF; lq++) {
-          const CT Fq = s.W[FW + lq * NF + f][t];
+          const CT Fq = s.W[lq * NF + f][t];
 #pragma unroll
           for (int ki = 0; ki < KPW; ki++) {
             const int k = row_k(wave, ki);

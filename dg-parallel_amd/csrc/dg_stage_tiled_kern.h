@@ -26,17 +26,22 @@ __global__ __launch_bounds__(NW * 64) void tri_stage_tiled(StageArgs a) {
   const int tid = threadIdx.x;
   K::phaseA(a, sm, tid, tile, g);
   __syncthreads();
-  K::phaseB(a, sm, tid, g);
-  __syncthreads();
-  K::phaseD1(a, sm, tid, g);
-  __syncthreads();
+#pragma unroll
+  for (int q0 = 0; q0 < K::NQ; q0 += K::QH) {
+    K::phaseB(a, sm, tid, g, q0);
+    __syncthreads();
+    K::phaseD1(a, sm, tid, g, q0);
+    __syncthreads();
+  }
   K::phaseC1(a, sm, tid, tile, g);
   __syncthreads();
   K::phaseC2(a, sm, tid, g);
   __syncthreads();
   K::phaseD2(a, sm, tid, tile, g);
   if (a.resid_part) {
-    // deterministic tree reduction of the step residual norm
+    // deterministic tree reduction of the step residual norm (red overlays the
+    // face fluxes D2 reads)
+    __syncthreads();
     sm.red[tid] = g.rsum;
     __syncthreads();
     for (int s = K::THREADS / 2; s > 0; s >>= 1) {

@@ -30,8 +30,10 @@ void run_tiles(const StageArgs& a) {
   for (int tile = 0; tile < ntiles; tile++) {
     std::memset(sm.get(), 0, sizeof(typename K::Smem));
     for (int t = 0; t < TH; t++) K::phaseA(a, *sm, t, tile, g[t]);
-    for (int t = 0; t < TH; t++) K::phaseB(a, *sm, t, g[t]);
-    for (int t = 0; t < TH; t++) K::phaseD1(a, *sm, t, g[t]);
+    for (int q0 = 0; q0 < K::NQ; q0 += K::QH) {
+      for (int t = 0; t < TH; t++) K::phaseB(a, *sm, t, g[t], q0);
+      for (int t = 0; t < TH; t++) K::phaseD1(a, *sm, t, g[t], q0);
+    }
     for (int t = 0; t < TH; t++) K::phaseC1(a, *sm, t, tile, g[t]);
     for (int t = 0; t < TH; t++) K::phaseC2(a, *sm, t, g[t]);
     double r = 0;
