@@ -212,14 +212,33 @@ struct TiledTet {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
+    const int es = valid ? e : 0;  // padding lanes read element 0 (in bounds) and discard it
     const int link = s.Ls[l][t];
-    // ALIAS: 12 tasks, own then neighbour traces; else the 6 neighbour traces
+    const bool inner = valid && link >= 0;
+    const int nb = inner ? (link >> 5) : es;
+    // ALIAS: 12 tasks, own then neighbour traces; else the 6 neighbour traces.
+    // Software-pipelined: the next task's coefficient rows are in flight while
+    // the current task's traces are formed.
     constexpr int T0 = ALIAS ? 0 : NF;
-    for (int task = T0 + wave; task < 2 * NF; task += NW) {
-      double v[NP];
+    constexpr int NT = (2 * NF - T0 + NW - 1) / NW;  // tasks of a wave, at most
+    auto load = [&](int j, double* v) {
+      const int task = T0 + wave + NW * j;
+      if (task >= 2 * NF) return;
+      const int f = task < NF ? task : task - NF;
+      const int src = task < NF ? es : nb;
+#pragma unroll
+      for (int k = 0; k < NP; k++) v[k] = a.Uin[(long)(f * NP + k) * E + src];
+    };
+    double vb[2][NP];
+    load(0, vb[0]);
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+      if (j + 1 < NT) load(j + 1, vb[(j + 1) & 1]);
+      const int task = T0 + wave + NW * j;
+      if (task >= 2 * NF) break;
+      double* v = vb[j & 1];
       if (task < NF) {
         const int f = task;
-        for (int k = 0; k < NP; k++) v[k] = valid ? a.Uin[(long)(f * NP + k) * E + e] : 0.0;
         const double* tn = a.tab + TetBlock::off_phif(P) + l * NQF * NP;
         for (int q = 0; q < NQF; q++) {
           double acc = 0.0;
@@ -230,9 +249,8 @@ struct TiledTet {
         continue;
       }
       const int f = task - NF;
-      const bool inner = valid && link >= 0;
-      const int nb = inner ? (link >> 5) : 0;
-      for (int k = 0; k < NP; k++) v[k] = inner ? a.Uin[(long)(f * NP + k) * E + nb] : 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; k++) v[k] = inner ? v[k] : 0.0;
       const int key = inner ? ((link >> 2) & 7) * 4 + (link & 3) : 0;  // perm * 4 + neighbour face
       const int key0 = wave_uniform(key);
       if (wave_all(key == key0)) {
