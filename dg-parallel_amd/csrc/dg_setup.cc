@@ -72,7 +72,10 @@ dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err) {
         const int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
         if (tag == DG_TAG_CHARACTERISTIC)
           return failS(err, DG_ERR_UNSUPPORTED, "characteristic far-field boundary (tag 30000) is not built");
-        const int code = tag == DG_TAG_WALL ? BC_WALL : tag == DG_TAG_FARFIELD2 ? BC_FARFIELD2 : BC_FARFIELD;
+        // 20000, the reference's curved wall, is the 40000 wall on straight faces
+        const int code = (tag == DG_TAG_WALL || tag == DG_TAG_CURVED_WALL) ? BC_WALL
+                         : tag == DG_TAG_FARFIELD2                          ? BC_FARFIELD2
+                                                                            : BC_FARFIELD;
         T.bcode[e * 3 + l] = (int8_t)code;
         v = 8 + code;
       }

@@ -38,7 +38,7 @@ LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
 FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL, FLUX_ROE_EF = 0, 1, 2, 3, 4
 RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
-TAG_FARFIELD, TAG_FARFIELD2, TAG_CHARACTERISTIC, TAG_WALL = 1, 2, 30000, 40000
+TAG_FARFIELD, TAG_FARFIELD2, TAG_CURVED_WALL, TAG_CHARACTERISTIC, TAG_WALL = 1, 2, 20000, 30000, 40000
 FIELD_GAUSSIAN, FIELD_VORTEX, FIELD_SOD, FIELD_CAVITY = 0, 1, 2, 3
 
 STATUS_NAMES = {0: "DG_OK", 1: "DG_ERR_ARG", 2: "DG_ERR_OOM", 3: "DG_ERR_HIP", 4: "DG_ERR_NONFINITE",
@@ -192,6 +192,35 @@ def mesh_tet_structured(nx, ny, nz, lx=1.0, ly=1.0, lz=1.0):
     if st != DG_OK:
         raise DGError(st, "dg_mesh_tet_structured rejected its arguments")
     return Mesh(3, vids, xyz)
+
+
+class GmshInfo(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("n_elems", ctypes.c_int), ("n_nodes", ctypes.c_int),
+                ("n_quads_split", ctypes.c_int), ("n_periodic_pairs", ctypes.c_int), ("n_tagged", ctypes.c_int)]
+
+
+_read_gmsh = _sig("dg_mesh_read_gmsh", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(GmshInfo), _p_i32,
+                                                       _p_f64, _p_i32, ctypes.c_char_p, ctypes.c_size_t])
+
+
+def mesh_read_gmsh(path, split_quads=False):
+    """A Gmsh version-1 file as a Mesh (dg_mesh_read_gmsh): straight triangles or
+    tetrahedra, boundary tags in bc, 510/610 periodic sides joined.  Returns
+    (mesh, info dict)."""
+    info = GmshInfo()
+    err = ctypes.create_string_buffer(512)
+    st = _read_gmsh(str(path).encode(), 1 if split_quads else 0, ctypes.byref(info), None, None, None, err, 512)
+    if st != DG_OK:
+        raise DGError(st, err.value.decode())
+    nv = info.dim + 1
+    vids = np.zeros((info.n_elems, nv), np.int32)
+    xyz = np.zeros((info.n_elems, nv, info.dim), np.float64)
+    bc = np.zeros((info.n_elems, nv), np.int32)
+    st = _read_gmsh(str(path).encode(), 1 if split_quads else 0, ctypes.byref(info), _ptr(vids, _p_i32),
+                    _ptr(xyz, _p_f64), _ptr(bc, _p_i32), err, 512)
+    if st != DG_OK:
+        raise DGError(st, err.value.decode())
+    return Mesh(info.dim, vids, xyz, bc), {f: getattr(info, f) for f, _ in GmshInfo._fields_}
 
 
 def MaxwellLaw(flux=FLUX_UPWIND):

@@ -132,6 +132,22 @@ Mesh Mesh::structuredTet(int nx, int ny, int nz, double lx, double ly, double lz
   return m;
 }
 
+Mesh Mesh::readGmsh(const char* path, bool split_quads) {
+  dg_gmsh_info info{};
+  char err[512] = {0};
+  dg_status s = dg_mesh_read_gmsh(path, split_quads ? 1 : 0, &info, nullptr, nullptr, nullptr, err, sizeof(err));
+  if (s != DG_OK) throw DGError(s, err);
+  Mesh m;
+  m.dim = info.dim;
+  const size_t nv = (size_t)info.dim + 1;
+  m.vids.resize(info.n_elems * nv);
+  m.bc.resize(info.n_elems * nv);
+  m.xyz.resize(info.n_elems * nv * info.dim);
+  s = dg_mesh_read_gmsh(path, split_quads ? 1 : 0, &info, m.vids.data(), m.xyz.data(), m.bc.data(), err, sizeof(err));
+  if (s != DG_OK) throw DGError(s, err);
+  return m;
+}
+
 // ---------------------------------------------------------------------------
 dg_law_desc Euler2dTwoStates::describe() const {
   dg_law_desc d = Euler2dRoe::describe();

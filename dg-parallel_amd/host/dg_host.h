@@ -188,6 +188,9 @@ struct Mesh {
                             bool periodic_y, double jitter, uint64_t seed, const int32_t side_tags[4]);
   // Kuhn subdivision of an n^3 lattice of boxes (6 tetrahedra per box)
   static Mesh structuredTet(int nx, int ny, int nz, double lx, double ly, double lz);
+  // a Gmsh version-1 file (mImportExport::importGmshFile): straight triangles or
+  // tetrahedra, boundary tags, 510/610 periodic sides joined
+  static Mesh readGmsh(const char* path, bool split_quads = false);
 };
 
 class DGAnalysis;

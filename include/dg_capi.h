@@ -46,13 +46,17 @@ enum { DG_FLUX_UPWIND = 0, DG_FLUX_LF = 1, DG_FLUX_ROE = 2, DG_FLUX_CENTRAL = 3,
 enum { DG_RK_FE = 0, DG_RK_TVD2 = 1, DG_RK_TVD3 = 2, DG_RK4 = 3 };
 enum { DG_PREC_FP64 = 0, DG_PREC_FP32 = 1 };
 /* Boundary tags (the reference's classification ids, EulerLaw.cc:182-296):
- * 40000 reflective wall; 30000 characteristic far field (not built: rejected);
+ * 40000 reflective wall; 20000 the reference's curved wall (EulerLaw.cc:109-179,
+ * its ghost state reflects the velocity about the exact surface normal N; on the
+ * straight-sided faces built here N is the face normal and the ghost is the
+ * 40000 one); 30000 characteristic far field (not built: rejected);
  * any other tag on a face without neighbour: Riemann flux against the law's
  * far-field state (the reference's default branch, EulerLaw.cc:293-295).  The
  * reference evaluates its far-field FieldEvaluator at the boundary point; tag
  * DG_TAG_FARFIELD2 selects a second constant state (Euler: farfield[4..8)), so
  * two boundaries can hold different states (the two ends of a shock tube). */
-enum { DG_TAG_FARFIELD = 1, DG_TAG_FARFIELD2 = 2, DG_TAG_CHARACTERISTIC = 30000, DG_TAG_WALL = 40000 };
+enum { DG_TAG_FARFIELD = 1, DG_TAG_FARFIELD2 = 2, DG_TAG_CURVED_WALL = 20000, DG_TAG_CHARACTERISTIC = 30000,
+       DG_TAG_WALL = 40000 };
 
 /* Mesh handed over by the caller (replaces mDGMesh + mImportExport for the path:
  * mMesh.h:52-139).  Faces are matched by their sorted topological vertex ids;
@@ -254,6 +258,28 @@ dg_status dg_l2_error(dg_ctx* ctx, int field, const double* par, double t, doubl
 dg_status dg_mesh_tri_structured(int nx, int ny, double x0, double y0, double lx, double ly,
                                  int periodic_x, int periodic_y, double jitter, uint64_t seed,
                                  const int32_t side_tags[4], int32_t* vids, double* xyz, int32_t* bc);
+
+/* Gmsh version-1 files (mImportExport::importGmshFile, mImportExport.cc:77-170;
+ * periodic pairing as mMesh::setPeriodicBC, mMesh.cc:457-523): straight-sided
+ * triangles (2D) or tetrahedra (3D, when the file holds any).  Boundary lines
+ * (2D) or triangles (3D) carry physical tags onto the element faces they cover
+ * (bc; untagged faces without neighbour: DG_TAG_FARFIELD; tag 100 interior).
+ * Edges tagged 510 pair by equal y, 610 by equal x: the paired vertices get one
+ * topological id, so the sides become face neighbours.  Quadrilaterals are
+ * DG_ERR_UNSUPPORTED unless split_quads (two triangles along the v0-v2 diagonal).
+ * Call with vids = xyz = bc = NULL for the sizes in info, then with buffers of
+ * [E][dim+1] ids, [E][dim+1][dim] coordinates, [E][dim+1] tags.  err_out (may be
+ * NULL) receives a message on failure. */
+typedef struct {
+  int dim;
+  int n_elems;
+  int n_nodes;
+  int n_quads_split;
+  int n_periodic_pairs;
+  int n_tagged;          /* boundary entities carrying a tag other than 100 / 510 / 610 */
+} dg_gmsh_info;
+dg_status dg_mesh_read_gmsh(const char* path, int split_quads, dg_gmsh_info* info, int32_t* vids, double* xyz,
+                            int32_t* bc, char* err_out, size_t err_len);
 
 /* Kuhn subdivision of an nx x ny x nz lattice of boxes on [0,lx]x[0,ly]x[0,lz]
  * into 6 tetrahedra per box sharing its main diagonal (faces match across boxes).

@@ -805,6 +805,18 @@ static void law_riemann(const or_law* L, const double* n, const double* uL, cons
 }
 
 /* boundary flux for faces without a neighbour */
+/* Euler2d::boundary, tag 20000, ALG II (EulerLaw.cc:133-147) with N = n */
+static void curved_wall_ghost(const double* n, const double* Q, double* g) {
+  double urN = -(n[0] * Q[1] + n[1] * Q[2]) / Q[0];
+  double urT = (n[1] * Q[1] - n[0] * Q[2]) / Q[0];
+  double u = urN * n[0] + urT * n[1];
+  double v = urN * n[1] - urT * n[0];
+  g[0] = Q[0];
+  g[1] = u * Q[0];
+  g[2] = v * Q[0];
+  g[3] = Q[3];
+}
+
 static void law_boundary(const or_law* L, int tag, const double* n, const double* x, double T, const double* Q,
                          double* out) {
   int Nf = nfields_of(L);
@@ -816,7 +828,12 @@ static void law_boundary(const or_law* L, int tag, const double* n, const double
     return;
   }
   if (L->law == OR_LAW_EULER2D) {
-    if (tag == 40000) {
+    if (tag == 20000) {
+      /* curved wall, the reference's ALG II (EulerLaw.cc:109-179): the velocity
+       * reflected about the surface normal N, here the face normal (straight
+       * faces), in velocity form */
+      curved_wall_ghost(n, Q, ghost);
+    } else if (tag == 40000) {
       /* reflective wall ghost (EulerLaw.cc:280-291) */
       double vn = -(Q[1] * n[0] + Q[2] * n[1]);
       double vt = Q[1] * n[1] - Q[2] * n[0];
@@ -995,6 +1012,10 @@ int or_residual(or_ctx* c, const double* U, double t, double* R, int nthreads) {
  * (ScalarLaw.cc:39-59). */
 static void law_ghost(const or_law* L, int tag, const double* n, const double* Q, double* g) {
   int Nf = nfields_of(L);
+  if (L->law == OR_LAW_EULER2D && tag == 20000) {
+    curved_wall_ghost(n, Q, g);
+    return;
+  }
   if (L->law == OR_LAW_EULER2D && tag == 40000) {
     double vn = -(Q[1] * n[0] + Q[2] * n[1]);
     double vt = Q[1] * n[1] - Q[2] * n[0];

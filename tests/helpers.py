@@ -123,3 +123,37 @@ def direction(U, seed):
     """a random direction scaled per entry to the state (keeps the Euler states physical)"""
     rng = np.random.default_rng(seed)
     return rng.standard_normal(U.shape) * (0.05 * np.abs(U).max())
+
+
+def write_msh(path, nodes, records):
+    """A Gmsh version-1 file: nodes [(x, y, z)] numbered from 1, records
+    [(type, physical tag, [node numbers])]."""
+    with open(path, "w") as f:
+        f.write("$NOD\n%d\n" % len(nodes))
+        for i, (x, y, z) in enumerate(nodes):
+            f.write("%d %.17g %.17g %.17g\n" % (i + 1, x, y, z))
+        f.write("$ENDNOD\n$ELM\n%d\n" % len(records))
+        for i, (typ, tag, nv) in enumerate(records):
+            f.write("%d %d %d %d %d %s\n" % (i + 1, typ, tag, 1, len(nv), " ".join(str(v) for v in nv)))
+        f.write("$ENDELM\n")
+
+
+def lattice_msh(path, nx, ny, lx=1.0, ly=1.0, side_tags=(40000, 1, 40000, 1)):
+    """The structured triangle mesh of dg_mesh_tri_structured (no jitter, cell
+    (i,j) split by its (i,j)-(i+1,j+1) diagonal into (v00, v10, v11) and
+    (v00, v11, v01)) as a Gmsh file with every lattice vertex its own node and the
+    bottom, right, top, left sides tagged side_tags."""
+    nid = lambda i, j: j * (nx + 1) + i + 1
+    nodes = [(lx * i / nx, ly * j / ny, 0.0) for j in range(ny + 1) for i in range(nx + 1)]
+    recs = []
+    for j in range(ny):
+        for i in range(nx):
+            recs.append((2, 100, [nid(i, j), nid(i + 1, j), nid(i + 1, j + 1)]))
+            recs.append((2, 100, [nid(i, j), nid(i + 1, j + 1), nid(i, j + 1)]))
+    for i in range(nx):
+        recs.append((1, side_tags[0], [nid(i, 0), nid(i + 1, 0)]))
+        recs.append((1, side_tags[2], [nid(i + 1, ny), nid(i, ny)]))
+    for j in range(ny):
+        recs.append((1, side_tags[1], [nid(nx, j), nid(nx, j + 1)]))
+        recs.append((1, side_tags[3], [nid(0, j + 1), nid(0, j)]))
+    write_msh(path, nodes, recs)
