@@ -18,6 +18,7 @@
 
 #include "../../include/dg_capi.h"
 #include "dg_adjoint.h"
+#include "dg_ctx.h"
 #include "dg_fields.h"
 #include "dg_kernels.h"
 #include "dg_mixed.h"
@@ -27,94 +28,6 @@
 
 using namespace dgp;
 
-struct dg_ctx {
-  std::string err;
-  int dim = 2, E = 0, Nf = 0, P = 0, NP = 0;
-  int law = 0, prec = 0, use_graph = 0, device = 0;
-  double hmin = 0.0;  // tetrahedra: shortest edge (the Maxwell dt needs no state)
-  int kernel = 0;  // 0: tiled LDS-staged stage kernel, 1: one element per work-item
-  // launch schedule (dg_set_schedule / dg_autotune): tile shape of the tiled
-  // kernels, workgroups of the persistent mixed-order launch (0: default)
-  int shape = 0, mix_grid = 0;
-  size_t part_cap = 0;  // doubles in part
-  // mixed orders: caller topology and geometry (rebucketing keeps them), the
-  // capacity of the state buffers (every element at the top order) and of the
-  // chunk list, and the old-index map of a rebucketing
-  TriTopo topo;
-  size_t state_cap = 0;
-  int chunk_cap = 0;
-  int32_t* remap = nullptr;
-  int adj_nd = 0;  // adjoint kernel: flux tangent directions per pass (0: default)
-  LawParams lp{};
-  size_t ndof = 0;
-  std::vector<int32_t> perm;  // internal index -> caller element
-  std::vector<int32_t> faces; // (eL, lL, eR, lR) caller numbering
-  std::vector<double> xyz;    // caller corner coordinates [E][3][2]
-  // order buckets (a uniform mesh is one bucket at offset 0)
-  bool mixed = false;
-  int bstart[6] = {0, 0, 0, 0, 0, 0};
-  long boff[6] = {0, 0, 0, 0, 0, 0};
-  std::vector<int8_t> order;  // internal order per element
-  std::vector<long> coff;     // caller element -> first coefficient (E+1)
-  int moff[6] = {0, 0, 0, 0, 0, 0};
-  int ftoff[6][6];
-  int nchunks = 0;
-  // what dg_create was given, kept to rebuild the buckets when orders change
-  std::vector<int32_t> m_vids, m_bc, m_order;
-  dg_law_desc m_law{};
-  dg_options m_opt{};
-  // limiter (applied after every RK stage when on)
-  int lim_on = 0;
-  double lim_m = 0.0, lim_nu = 1.0;
-  double lim_midv[3][15], lim_pinv[2][3];
-  hipStream_t stream = nullptr;
-  // mixed orders with the limiter: three side streams and fork/join events that
-  // let the order buckets' limiter launches run side by side (also when captured)
-  hipStream_t side[3] = {nullptr, nullptr, nullptr};
-  hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
-  // device memory
-  double *Un = nullptr, *acc = nullptr, *A64 = nullptr, *B64 = nullptr, *R = nullptr;
-  float *A32 = nullptr, *B32 = nullptr;
-  double *mg = nullptr, *detj = nullptr, *size = nullptr, *tab = nullptr, *vtx = nullptr, *part = nullptr;
-  float* tabf = nullptr;
-  int32_t* link = nullptr;
-  double *mtab = nullptr, *glw = nullptr, *mvtx = nullptr, *ntab = nullptr;
-  int* chunks = nullptr;
-  uint8_t *flags = nullptr, *sens = nullptr;
-  std::vector<void*> allocs;
-  // graph cache
-  hipGraphExec_t gexec = nullptr;
-  int g_scheme = -1;
-  double g_dt = 0.0;
-  // discrete adjoint (FP64, RK4): checkpoints every adj_every steps, the stage
-  // states of one segment, and the adjoint work vectors
-  double *ckpt = nullptr, *seg = nullptr, *lam = nullptr, *mu = nullptr, *za = nullptr, *zb = nullptr,
-         *wv = nullptr;
-  int adj_nsteps = 0, adj_every = 0, adj_nck = 0;
-  size_t adj_ck_cap = 0, adj_seg_cap = 0;  // doubles allocated for ckpt and seg
-  double adj_dt = 0.0;
-  bool adj_ready = false;
-  std::vector<double> adj_seed;  // dJ/dU(T) of the density-misfit objective, caller layout
-  float adj_fwd_ms = 0.f, adj_bwd_ms = 0.f;
-};
-
-static dg_status fail(dg_ctx* c, dg_status s, const char* fmt, ...) {
-  if (c) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    c->err = buf;
-  }
-  return s;
-}
-
-#define HIPCHECK(c, call)                                                                   \
-  do {                                                                                      \
-    hipError_t e_ = (call);                                                                 \
-    if (e_ != hipSuccess) return fail((c), DG_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
 
 template <typename T>
 static dg_status dalloc(dg_ctx* c, T** p, size_t n) {
@@ -434,7 +347,6 @@ dg_status dg_destroy(dg_ctx* c) {
   return DG_OK;
 }
 
-static bool ready(const dg_ctx* c) { return c && c->stream && c->Un; }
 static bool schedule_ok(const dg_ctx* c, const dg_schedule& s);
 static void apply_schedule(dg_ctx* c, const dg_schedule& s);
 
@@ -1065,7 +977,7 @@ static bool rebucket_in_place(dg_ctx* c, const std::vector<int32_t>& neword, dg_
   return true;
 }
 
-static dg_status rebucket(dg_ctx* c, const std::vector<int32_t>& neword) {
+extern "C++" dg_status ctx_rebucket(dg_ctx* c, const std::vector<int32_t>& neword) {
   dg_status st;
   if (rebucket_in_place(c, neword, st)) return st;
   std::vector<double> U(c->ndof);
@@ -1131,7 +1043,7 @@ dg_status dg_select_orders(dg_ctx* c, const dg_sensor_opts* o, int apply, int32_
   if (s) return s;
   std::vector<int32_t> neword(h.begin(), h.end());
   if (orders_out) std::memcpy(orders_out, neword.data(), neword.size() * 4);
-  if (apply && neword != c->m_order) return rebucket(c, neword);
+  if (apply && neword != c->m_order) return ctx_rebucket(c, neword);
   return DG_OK;
 }
 

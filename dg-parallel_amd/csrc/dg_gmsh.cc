@@ -17,6 +17,7 @@
 // periodicity).  Every other tag goes to the bc array of the element face it
 // covers; untagged faces without neighbour get DG_TAG_FARFIELD.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -34,6 +35,7 @@ struct GmshMesh {
   int dim = 0;
   std::vector<int32_t> elems;         // [E][dim+1] node indices
   std::vector<double> nodes;          // [N][3]
+  std::vector<long> ids;              // [N] node numbers of the file
   std::vector<std::vector<int>> bnd;  // tagged boundary entities: node indices, tag last
   int split = 0;
 };
@@ -50,6 +52,7 @@ dg_status parse(const char* path, int split_quads, GmshMesh& M, std::string& err
     return DG_ERR_ARG;
   }
   std::unordered_map<long, int> node_at;  // node id -> index
+  M.ids.clear();
   std::vector<std::vector<long>> tris, tets, lines, quads;
   std::vector<int> tri_tag, line_tag;
   char w[256];
@@ -73,6 +76,7 @@ dg_status parse(const char* path, int split_quads, GmshMesh& M, std::string& err
           break;
         }
         node_at[id] = (int)i;
+        M.ids.push_back(id);
         M.nodes[i * 3] = x;
         M.nodes[i * 3 + 1] = y;
         M.nodes[i * 3 + 2] = z;
@@ -267,6 +271,18 @@ extern "C" dg_status dg_mesh_read_gmsh(const char* path, int split_quads, dg_gms
     std::sort(key.begin(), key.end());
     tag_of[key] = tag;
   }
+  // the vertex id of a class: its smallest node number, minus one (node numbers
+  // of a file written by dg_mesh_write_gmsh are vertex id + 1)
+  std::vector<long> cls(N, LONG_MAX);
+  for (int i = 0; i < N; i++) {
+    const int r = find_root(root, i);
+    cls[r] = std::min(cls[r], M.ids[i]);
+  }
+  for (int i = 0; i < N; i++)
+    if (root[i] == i && (cls[i] - 1 < 0 || cls[i] - 1 > INT32_MAX - 1)) {
+      err = "node numbers must lie in 1..2^31-1";
+      return fail(DG_ERR_ARG);
+    }
   info->dim = M.dim;
   info->n_elems = E;
   info->n_nodes = N;
@@ -280,7 +296,7 @@ extern "C" dg_status dg_mesh_read_gmsh(const char* path, int split_quads, dg_gms
   for (int e = 0; e < E; e++) {
     for (int v = 0; v < nv; v++) {
       const int k = M.elems[(size_t)e * nv + v];
-      if (vids) vids[(size_t)e * nv + v] = find_root(root, k);
+      if (vids) vids[(size_t)e * nv + v] = (int32_t)(cls[find_root(root, k)] - 1);
       if (xyz)
         for (int d = 0; d < M.dim; d++) xyz[((size_t)e * nv + v) * M.dim + d] = M.nodes[(size_t)k * 3 + d];
     }

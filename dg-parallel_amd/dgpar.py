@@ -102,6 +102,12 @@ _limit = _sig("dg_limit", ctypes.c_int, [_vp, _p_u8])
 _limiter_flags = _sig("dg_limiter_flags", ctypes.c_int, [_vp, _p_u8])
 _select_orders = _sig("dg_select_orders", ctypes.c_int, [_vp, ctypes.POINTER(SensorOpts), ctypes.c_int, _p_i32])
 _get_orders = _sig("dg_get_orders", ctypes.c_int, [_vp, _p_i32])
+_write_restart = _sig("dg_write_restart", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_int])
+_read_restart = _sig("dg_read_restart", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                                       ctypes.POINTER(ctypes.c_int)])
+_sample_lattice = _sig("dg_sample_lattice", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_size_t])
+_export_pos = _sig("dg_export_pos", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                   ctypes.c_int, _p_f64])
 _jvp = _sig("dg_residual_jvp", ctypes.c_int, [_vp, _p_f64, _p_f64, _p_f64])
 _vjp = _sig("dg_residual_vjp", ctypes.c_int, [_vp, _p_f64, _p_f64, _p_f64])
 _adj_fwd = _sig("dg_adjoint_forward", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _p_f64, _p_f64])
@@ -221,6 +227,28 @@ def mesh_read_gmsh(path, split_quads=False):
     if st != DG_OK:
         raise DGError(st, err.value.decode())
     return Mesh(info.dim, vids, xyz, bc), {f: getattr(info, f) for f, _ in GmshInfo._fields_}
+
+
+_write_gmsh = _sig("dg_mesh_write_gmsh", ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_char_p, ctypes.c_char_p,
+                                                         ctypes.c_size_t])
+
+
+def mesh_write_gmsh(mesh, path):
+    """The mesh as a Gmsh version-1 file that mesh_read_gmsh reads back
+    (dg_mesh_write_gmsh: boundary tags, periodic sides as 510/610 lines)."""
+    err = ctypes.create_string_buffer(512)
+    md = mesh.desc()
+    st = _write_gmsh(ctypes.byref(md), str(path).encode(), err, 512)
+    if st != DG_OK:
+        raise DGError(st, err.value.decode())
+
+
+_lattice_points = _sig("dg_lattice_points", ctypes.c_size_t, [ctypes.c_int, ctypes.c_int])
+
+
+def lattice_points(dim, nsub):
+    """points of the lattice (i, j[, k]) / nsub of a reference simplex"""
+    return int(_lattice_points(dim, nsub))
 
 
 def MaxwellLaw(flux=FLUX_UPWIND):
@@ -425,3 +453,32 @@ class DGAnalysis:
         out = np.zeros(self.mesh_E, np.int32)
         self._check(_get_orders(self._h, _ptr(out, _p_i32)))
         return out
+
+    # ---- checkpoint / restart and state export ----
+    def write_restart(self, path, t=0.0, iteration=0):
+        """the data stream of DGAnalysis::write (DGPref.cc:712-740)"""
+        self._check(_write_restart(self._h, str(path).encode(), t, iteration))
+
+    def read_restart(self, path):
+        """DGAnalysis::read (DGPref.cc:742-768); returns (t, iteration)"""
+        t, it = ctypes.c_double(), ctypes.c_int()
+        self._check(_read_restart(self._h, str(path).encode(), ctypes.byref(t), ctypes.byref(it)))
+        self.ndof = _ndof(self._h)
+        return t.value, it.value
+
+    def write(self, mesh_path, data_path, t=0.0, iteration=0):
+        """DGAnalysis::write(mesh, data): the mesh as a Gmsh file and the data stream"""
+        mesh_write_gmsh(self.mesh, mesh_path)
+        self.write_restart(data_path, t, iteration)
+
+    def sample_lattice(self, nsub):
+        """every field at the lattice points of every element: [E][Nf][points]"""
+        npts = lattice_points(self.mesh.dim, nsub)
+        out = np.zeros((self.mesh_E, self.law.nfields, max(npts, 1)), np.float64)
+        self._check(_sample_lattice(self._h, nsub, _ptr(out, _p_f64), out.size if npts else 0))
+        return out
+
+    def export_pos(self, prefix, iteration=0, nsub=0, t=0.0, exact_field=-1, exact_par=()):
+        """Gmsh .pos views of the fields of interest (DGAnalysis::exportGmsh)"""
+        par = np.ascontiguousarray(list(exact_par) + [0.0] * (8 - len(exact_par)), np.float64)
+        self._check(_export_pos(self._h, str(prefix).encode(), iteration, nsub, t, exact_field, _ptr(par, _p_f64)))

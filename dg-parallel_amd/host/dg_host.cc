@@ -148,6 +148,24 @@ Mesh Mesh::readGmsh(const char* path, bool split_quads) {
   return m;
 }
 
+dg_mesh_desc Mesh::describe() const {
+  dg_mesh_desc md{};
+  md.dim = dim;
+  md.n_elems = n_elems();
+  md.vids = vids.data();
+  md.xyz = xyz.data();
+  md.bc = bc.empty() ? nullptr : bc.data();
+  md.order = order.empty() ? nullptr : order.data();
+  return md;
+}
+
+void Mesh::writeGmsh(const char* path) const {
+  const dg_mesh_desc md = describe();
+  char err[512] = {0};
+  dg_status s = dg_mesh_write_gmsh(&md, path, err, sizeof(err));
+  if (s != DG_OK) throw DGError(s, err);
+}
+
 // ---------------------------------------------------------------------------
 dg_law_desc Euler2dTwoStates::describe() const {
   dg_law_desc d = Euler2dRoe::describe();
@@ -243,15 +261,9 @@ double TimeIntegrator::advanceInTime(double t, double dt) {
 }
 
 DGAnalysis::DGAnalysis(const Mesh& mesh, ConservationLaw* law, DGLimiter* limiter, int order, int precision)
-    : law_(law), order_(order) {
-  dg_mesh_desc md{};
-  md.dim = mesh.dim;
-  md.n_elems = mesh.n_elems();
+    : mesh_(mesh), law_(law), order_(order) {
+  const dg_mesh_desc md = mesh.describe();
   n_elems_ = md.n_elems;
-  md.vids = mesh.vids.data();
-  md.xyz = mesh.xyz.data();
-  md.bc = mesh.bc.empty() ? nullptr : mesh.bc.data();
-  md.order = mesh.order.empty() ? nullptr : mesh.order.data();
   dg_law_desc ld = law->describe();
   dg_options opt{};
   opt.order = order;
@@ -307,12 +319,24 @@ double DGAnalysis::computeDt(double cfl) {
   return dt;
 }
 
+void DGAnalysis::write(const char* mesh_path, const char* data_path) const {
+  mesh_.writeGmsh(mesh_path);
+  check(dg_write_restart(ctx_, data_path, TACT, ITER));
+}
+
+void DGAnalysis::read(const char* data_path) { check(dg_read_restart(ctx_, data_path, &TACT, &ITER)); }
+
+void DGAnalysis::exportGmsh(const char* fileName, int iteration) const {
+  check(dg_export_pos(ctx_, fileName, iteration, 0, TACT, -1, nullptr));
+}
+
 int DGAnalysis::run() {
   double dt = computeDt(CFLMAX);
   int N = (int)std::ceil((TEND - TACT) / dt);
   dt = (TEND - TACT) / N;
   check(dg_advance(ctx_, integrator_->scheme(), TACT, dt, N, nullptr));
   TACT = TEND;
+  ITER += N;
   return N;
 }
 

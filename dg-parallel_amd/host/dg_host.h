@@ -191,6 +191,10 @@ struct Mesh {
   // a Gmsh version-1 file (mImportExport::importGmshFile): straight triangles or
   // tetrahedra, boundary tags, 510/610 periodic sides joined
   static Mesh readGmsh(const char* path, bool split_quads = false);
+  // the mesh as a Gmsh version-1 file that readGmsh reads back
+  // (mImportExport::exportDGFile, mImportExport.cc:611-642)
+  void writeGmsh(const char* path) const;
+  dg_mesh_desc describe() const;
 };
 
 class DGAnalysis;
@@ -228,6 +232,7 @@ class RungeKutta4 : public TimeIntegrator {
 // chosen from the order as the reference does (DGAnalysis.cc:239-242).
 class DGAnalysis {
   dg_ctx* ctx_ = nullptr;
+  Mesh mesh_;
   ConservationLaw* law_;
   TimeIntegrator* integrator_ = nullptr;
   int order_;
@@ -235,6 +240,7 @@ class DGAnalysis {
 
  public:
   double TACT = 0.0, TEND = 0.2, CFLMAX = 0.5;
+  int ITER = 0;
 
   // limiter: NULL (no limiting) or a DGLimiter applied after every RK stage
   DGAnalysis(const Mesh& mesh, ConservationLaw* law, DGLimiter* limiter, int order, int precision = DG_PREC_FP64);
@@ -265,6 +271,14 @@ class DGAnalysis {
   // cache: optional text file of tuned entries per device and configuration
   dg_tune_report autotune(double dt, int steps = 5, const char* cache = nullptr, bool force = false);
   dg_schedule schedule() const;
+  // checkpoint / restart (DGAnalysis::write / read, DGPref.cc:712-768): the mesh
+  // as a Gmsh file and the data stream (TACT, ITER, orders, coefficients); read
+  // takes the data stream of a context over the same mesh and sets TACT, ITER
+  void write(const char* mesh_path, const char* data_path) const;
+  void read(const char* data_path);
+  // Gmsh post-processing views (DGAnalysis::exportGmsh, DGSensor.cc:337-525):
+  // one <fileName>-sample<iteration><field>.pos file per field of interest
+  void exportGmsh(const char* fileName, int iteration) const;
   // the reference's time loop (commented out in DGAnalysis.cc:480-731): dt from the
   // CFL rule, N = ceil((TEND-TACT)/dt) steps of dt = (TEND-TACT)/N; returns N
   int run();

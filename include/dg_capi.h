@@ -281,6 +281,59 @@ typedef struct {
 dg_status dg_mesh_read_gmsh(const char* path, int split_quads, dg_gmsh_info* info, int32_t* vids, double* xyz,
                             int32_t* bc, char* err_out, size_t err_len);
 
+/* The mesh half of DGAnalysis::write (mImportExport::exportDGFile,
+ * mImportExport.cc:611-642): a Gmsh version-1 file of the descriptor that
+ * dg_mesh_read_gmsh reads back to the same elements, faces and tags.  One node
+ * per (vertex id, coordinates); faces without neighbour become lines (2D) or
+ * triangles (3D) carrying their bc tag; the two sides of a periodic face pair
+ * (one vertex id at two positions, a translation along x or y) become lines
+ * tagged 510 (translated along x: paired by equal y) or 610 (along y).
+ * Periodic tetrahedral meshes are DG_ERR_UNSUPPORTED. */
+dg_status dg_mesh_write_gmsh(const dg_mesh_desc* mesh, const char* path, char* err_out, size_t err_len);
+
+/* ---- checkpoint / restart (DGAnalysis::write / read, DGPref.cc:712-768) ----
+ * The data stream of DGAnalysis::write: t, the iteration, "dim E F", then per
+ * element in caller order its mMesh::writeMEntity record (entity type 2 or 4
+ * and the vertex ids, mMesh.cc:401-412) followed by DGCell::write (the order and
+ * the Nf*Np coefficients field by field, DGCell.cc:1523-1530), then one record
+ * per face (type 1 or 2 and its vertex ids).  Coefficients are written with 17
+ * significant digits (the reference: 16), so a restart holds the same doubles. */
+dg_status dg_write_restart(dg_ctx* ctx, const char* path, double t, int iter);
+/* Reads such a stream (DGAnalysis::read, DGPref.cc:742-768, which expects a
+ * geometric order after each entity type that write never writes; this reader
+ * takes the records as write produces them).  Cell records are matched to the
+ * context's elements by their vertex ids (any record order; each element's
+ * vertices must come in the context's order).  A triangle context in the FP64
+ * form takes the orders of the file (re-bucketing as dg_select_orders does);
+ * otherwise they must equal the context's.  Face records are not needed. */
+dg_status dg_read_restart(dg_ctx* ctx, const char* path, double* t_out, int* iter_out);
+
+/* ---- state export (DGAnalysis::exportGmsh, DGSensor.cc:337-525, and
+ * exportGmsh_Volume, :800-) ----------------------------------------------------
+ * Lattice of a reference simplex split nsub times: points (i, j[, k]) / nsub
+ * with i + j [+ k] <= nsub, k outer, i inner.  dg_lattice_points gives their
+ * number (0 for nsub outside 1..8).  dg_sample_lattice evaluates every field
+ * at every element's lattice points on the device: out[E][Nf][points], caller
+ * element order, n = E * Nf * points. */
+size_t dg_lattice_points(int dim, int nsub);
+dg_status dg_sample_lattice(dg_ctx* ctx, int nsub, double* out, size_t n);
+/* Gmsh post-processing views, one file "<prefix>-sample<iteration><name>.pos"
+ * per field of interest (ST/VT triangles, SS/VS tetrahedra):
+ *   Euler:   velocity, pressure, mach, density, error (rho u - far field rho u),
+ *            entropy (against the far-field state), limit (Euler::getNameAndSize,
+ *            Euler2d::getIthFieldOfInterest, EulerLaw.cc:1240-1340);
+ *   Scalar:  error and exact (with an exact field), numEr, function, limit
+ *            (ScalarLaw.cc:111-165);
+ *   Maxwell (3D): E and H vectors, energy density, error |E - E_exact| (with an
+ *            exact field);
+ *   all:     order.
+ * "limit" is the element's flag byte of the last limiter pass.  nsub <= 0: each
+ * triangle split by the highest element order (the reference: its own order),
+ * each tetrahedron into 1 (order 1) or 8 (red refinement).  exact_field:
+ * DG_FIELD_* at time t with exact_par, or -1. */
+dg_status dg_export_pos(dg_ctx* ctx, const char* prefix, int iteration, int nsub, double t, int exact_field,
+                        const double* exact_par);
+
 /* Kuhn subdivision of an nx x ny x nz lattice of boxes on [0,lx]x[0,ly]x[0,lz]
  * into 6 tetrahedra per box sharing its main diagonal (faces match across boxes).
  * vids[E][4], xyz[E][4][3], E = 6 nx ny nz.  With the Maxwell law every face
