@@ -265,7 +265,8 @@ dg_status dg_mesh_tri_structured(int nx, int ny, double x0, double y0, double lx
  * (2D) or triangles (3D) carry physical tags onto the element faces they cover
  * (bc; untagged faces without neighbour: DG_TAG_FARFIELD; tag 100 interior).
  * Edges tagged 510 pair by equal y, 610 by equal x: the paired vertices get one
- * topological id, so the sides become face neighbours.  Quadrilaterals are
+ * topological id, so the sides become face neighbours.  The vertex id of a
+ * node (or of a class of paired nodes) is its smallest node number minus one.  Quadrilaterals are
  * DG_ERR_UNSUPPORTED unless split_quads (two triangles along the v0-v2 diagonal).
  * Call with vids = xyz = bc = NULL for the sizes in info, then with buffers of
  * [E][dim+1] ids, [E][dim+1][dim] coordinates, [E][dim+1] tags.  err_out (may be
