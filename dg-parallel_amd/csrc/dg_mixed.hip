@@ -43,7 +43,13 @@ __global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_stage(MixedArgs a) {
   const int tid = threadIdx.x;
   double rsum = 0.0;
   TiledMixed<1, LAW>::stage_tables(a, sm, tid);  // the same tables for every order
-  for (int w = blockIdx.x; w < a.nchunks; w += gridDim.x) {
+  // the hardware hands workgroup b to XCD b % 8: numbering the workgroups of
+  // one XCD consecutively gives each XCD a contiguous run of chunks (in mesh
+  // order) at every pass, so the neighbour rows the tiles in flight gather are
+  // in that XCD's L2
+  const int nb = gridDim.x, full = (nb / 8) * 8, b = blockIdx.x;
+  const int lw = b >= full ? b : (b % 8) * (nb / 8) + b / 8;
+  for (int w = lw; w < a.nchunks; w += nb) {
     const int p = a.chunks[2 * w], i0 = a.chunks[2 * w + 1];
     switch (p) {
       case 1: mixed_tile<1, LAW>(a, sm, tid, i0, rsum); break;

@@ -1,7 +1,7 @@
 // Drives the library through the host C++ mirror classes the way a reference
 // scenario drives DG-Parallel (Examples.cc / Lilia.cc): build a law and a mesh,
 // construct DGAnalysis, then call residual and TimeIntegrator::advanceInTime.
-//   dropin_vortex OUT_PREFIX [vortex|tube|maxwell]
+//   dropin_vortex OUT_PREFIX [vortex|tube|maxwell|restart]
 // writes OUT_PREFIX.U0 (projected state), .R (residual), .U2 (after two steps of
 // the integrator DGAnalysis picked for the order), raw little-endian doubles, and
 // prints dt and the residual norms.
@@ -10,6 +10,7 @@
 //            orders P1-P4 on a 16x4x2 tube, the state limited once, two SSP-RK3
 //            steps; then adaptP and .O (new orders as doubles), .U3 (state after it)
 //   maxwell  Maxwell3d (upwind), P3 on a 2x2x2 Kuhn mesh, cavity mode, RK4
+//   restart  the vortex written, plotted, read back and continued (see restart())
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -90,12 +91,44 @@ static int maxwell(const std::string& out) {
   return 0;
 }
 
+// checkpoint, plot and restart through the mirror (DGAnalysis::write / read /
+// exportGmsh): after two steps the run writes OUT_PREFIX.msh / .dat and the .pos
+// views, a second DGAnalysis built from the written mesh file reads the data,
+// and both take two more steps: .U4a (first) and .U4b (restarted)
+static int restart(const std::string& out) {
+  const int32_t tags[4] = {0, 0, 0, 0};
+  dgpar::Mesh mesh = dgpar::Mesh::structuredTri(10, 10, 0.0, 0.0, 10.0, 10.0, true, true, 0.1, 77, tags);
+  dgpar::ConstantField far({1.0, 1.0, 1.0, 3.5});
+  dgpar::Euler2d law(&far, 1.4);
+  dgpar::DGAnalysis an(mesh, &law, nullptr, 3);
+  const double vortex[6] = {5.0, 5.0, 5.0, 1.0, 1.0, 10.0};
+  an.project(DG_FIELD_VORTEX, vortex);
+  double dt = an.computeDt(0.1);
+  an.integrator()->advanceInTime(0.0, dt);
+  an.integrator()->advanceInTime(dt, dt);
+  an.TACT = 2 * dt;
+  an.ITER = 2;
+  an.write((out + ".msh").c_str(), (out + ".dat").c_str());
+  an.exportGmsh(out.c_str(), an.ITER);
+  dgpar::DGAnalysis again(dgpar::Mesh::readGmsh((out + ".msh").c_str()), &law, nullptr, 3);
+  again.read((out + ".dat").c_str());
+  printf("dt %.17e\nrestart %.17e %d\n", dt, again.TACT, again.ITER);
+  for (int n = 2; n < 4; n++) {
+    an.integrator()->advanceInTime(n * dt, dt);
+    again.integrator()->advanceInTime(n * dt, dt);
+  }
+  dump(out + ".U4a", an.getState());
+  dump(out + ".U4b", again.getState());
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
   std::string out = argv[1], mode = argc > 2 ? argv[2] : "vortex";
   try {
     if (mode == "tube") return tube(out);
     if (mode == "maxwell") return maxwell(out);
+    if (mode == "restart") return restart(out);
     return vortex(out);
   } catch (const dgpar::DGError& e) {
     fprintf(stderr, "DGError %d: %s\n", (int)e.status, e.what());

@@ -98,3 +98,22 @@ def test_cpp_dropin_maxwell_tets():
     for s in range(2):
         U, r = o.step(U, po.RK4, s * dt, dt)
     assert np.abs(U2 - U).max() / np.abs(U).max() < 1e-12
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="C++ driver not built")
+def test_cpp_dropin_write_read_export():
+    print("\n[gpu case] C++ drop-in checkpoint, plot and restart", flush=True)
+    d = tempfile.mkdtemp()
+    pre = os.path.join(d, "run")
+    r = subprocess.run([EXE, pre, "restart"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = dict(l.split(None, 1) for l in r.stdout.splitlines())
+    dt = float(lines["dt"])
+    t, it = lines["restart"].split()
+    assert float(t) == 2 * dt and int(it) == 2
+    Ua, Ub = np.fromfile(pre + ".U4a"), np.fromfile(pre + ".U4b")
+    assert Ua.size == 200 * 40 and (Ua == Ub).all()
+    for f in ("velocity", "pressure", "mach", "density", "error", "entropy", "limit", "order"):
+        with open("%s-sample2%s.pos" % (pre, f)) as fh:
+            body = fh.read().splitlines()
+        assert body[0] == 'View "Exported field" {' and len(body) == 2 + 200 * 9
