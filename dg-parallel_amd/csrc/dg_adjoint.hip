@@ -70,6 +70,15 @@ static int launch_vjp_nd(const AdjArgs& a, hipStream_t s) {
 // one wave) on the P3 vortex.
 int launch_tri_vjp(const AdjArgs& a, int law, int nd, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_vjp_nd<ScalarLawD, 1>(a, s);
+  if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT) {
+    switch (nd) {
+      case 0:
+      case 1: return launch_vjp_nd<EulerExactLawD, 1>(a, s);
+      case 2: return launch_vjp_nd<EulerExactLawD, 2>(a, s);
+      case 4: return launch_vjp_nd<EulerExactLawD, 4>(a, s);
+    }
+    return -1;
+  }
   if (law == LAW_EULER2D) {
     switch (nd) {
       case 0:
@@ -96,6 +105,7 @@ static int launch_jvp_law(const AdjArgs& a, hipStream_t s) {
 
 int launch_tri_jvp(const AdjArgs& a, int law, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_jvp_law<ScalarLawD>(a, s);
+  if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT) return launch_jvp_law<EulerExactLawD>(a, s);
   if (law == LAW_EULER2D) return launch_jvp_law<EulerLawD>(a, s);
   return -1;
 }

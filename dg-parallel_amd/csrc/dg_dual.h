@@ -95,6 +95,27 @@ DG_HD bool operator>(const Dual<T, N>& a, const Dual<T, N>& b) {
 }
 
 template <typename T, int N>
+DG_HD bool operator<=(const Dual<T, N>& a, const Dual<T, N>& b) {
+  return a.v <= b.v;
+}
+template <typename T, int N>
+DG_HD bool operator>=(const Dual<T, N>& a, const Dual<T, N>& b) {
+  return a.v >= b.v;
+}
+
+// a^b: d(a^b) = b a^(b-1) da + a^b ln(a) db (the second term only where b
+// carries a tangent: the exponents of the Riemann solver are constants)
+template <typename T, int N>
+DG_HD Dual<T, N> dg_pow(Dual<T, N> a, Dual<T, N> b) {
+  Dual<T, N> r;
+  r.v = dg_pow(a.v, b.v);
+  const T da = b.v * dg_pow(a.v, b.v - T(1));
+#pragma unroll
+  for (int i = 0; i < N; i++) r.d[i] = da * a.d[i] + (b.d[i] != T(0) ? r.v * log(a.v) * b.d[i] : T(0));
+  return r;
+}
+
+template <typename T, int N>
 DG_HD Dual<T, N> dg_sqrt(Dual<T, N> a) {
   Dual<T, N> r;
   r.v = dg_sqrt(a.v);

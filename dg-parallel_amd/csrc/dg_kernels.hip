@@ -307,6 +307,7 @@ static int launch_law(const StageArgs& a, int variant, hipStream_t s) {
 
 int launch_tri_stage(const StageArgs& a, int law, int variant, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_law<ScalarLawD>(a, variant, s);
+  if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT) return launch_law<EulerExactLawD>(a, variant, s);
   if (law == LAW_EULER2D) return launch_law<EulerLawD>(a, variant, s);
   return -1;
 }

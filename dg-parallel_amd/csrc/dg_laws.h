@@ -8,7 +8,9 @@
 //   EulerLawD    Euler2d::Fi (EulerLaw.cc:47-69), Euler2dRoe::riemannSolver
 //                (EulerLaw.cc:913-1010), wall ghost (EulerLaw.cc:280-291), far-field
 //                ghost (EulerLaw.cc:293-295), maximumEigenValue (EulerLaw.cc:1012-1034),
-//                isPhysical (EulerLaw.cc:1398-1404); local Lax-Friedrichs flux is new.
+//                isPhysical (EulerLaw.cc:1398-1404), the exact Godunov flux of
+//                Euler2d::riemannSolver (EulerLaw.cc:602-887); local Lax-Friedrichs
+//                flux is new.
 //   MaxwellLawD  3D Maxwell, eps = mu = 1, upwind/central flux, PEC ghost (new).
 #pragma once
 
@@ -24,7 +26,7 @@
 namespace dgp {
 
 enum { LAW_SCALAR = 0, LAW_EULER2D = 1, LAW_MAXWELL3D = 2 };
-enum { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2, FLUX_CENTRAL = 3, FLUX_ROE_EF = 4 };
+enum { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2, FLUX_CENTRAL = 3, FLUX_ROE_EF = 4, FLUX_EXACT = 5 };
 // boundary codes: far field (farfield[0..NF)), wall, PEC, second far field
 // (farfield[4..8), Euler: a far-field state that differs by boundary, as the
 // reference's far-field FieldEvaluator evaluated at the boundary point)
@@ -44,6 +46,10 @@ template <>
 DG_HD float dg_sqrt<float>(float x) { return sqrtf(x); }
 template <typename T>
 DG_HD T dg_abs(T x) { return x < T(0) ? -x : x; }
+template <typename T>
+DG_HD T dg_pow(T x, T y) { return pow(x, y); }
+template <>
+DG_HD float dg_pow<float>(float x, float y) { return powf(x, y); }
 // reciprocal: exact division in FP64; the hardware reciprocal (1 ulp) in the
 // FP32 form
 template <typename T>
@@ -201,6 +207,104 @@ struct EulerLawD {
     for (int i = 0; i < 4; i++)
       flux_out[i] = T(0.5) * ((f1[i][0] + f2[i][0]) * n[0] + (f1[i][1] + f2[i][1]) * n[1] - roe[i]);
   }
+  // Exact Riemann problem in the face frame (u normal, v tangential velocity),
+  // sampled at x/t = 0, in the steps of Euler::riemann (EulerLaw.cc:602-851):
+  // both states supersonic one way -> that state; else Newton for p* from the
+  // two-shock guess (at most 12 iterations while the relative change is
+  // >= 1e-14, p* floored at 1e-14), u* from the final p*, the side by the sign
+  // of u*, and on that side the star state for a shock (no shock-speed test),
+  // the star state when the rarefaction tail and u* have opposite signs, else
+  // the fan state at x/t = 0 (no fan-head test).
+  template <typename T>
+  DG_HD static void exact_state(T g, T rl, T ul, T vl, T pl, T rr, T ur, T vr, T pr, T& rho, T& u, T& v, T& p) {
+    const T TOL = T(1e-14), gm1 = g - T(1), gp1 = g + T(1);
+    const T ril = T(1) / rl, rir = T(1) / rr;
+    const T cl = dg_sqrt(dg_abs(g * pl * ril)), cr = dg_sqrt(dg_abs(g * pr * rir));
+    if (ul >= cl && ur >= cr) {
+      rho = rl, u = ul, v = vl, p = pl;
+      return;
+    }
+    if (ur <= -cr && ul <= -cl) {
+      rho = rr, u = ur, v = vr, p = pr;
+      return;
+    }
+    const T ig = T(1) / g, C3 = T(0.5) * gm1 * ig, C4 = T(2) / gm1;
+    const T du = ur - ul;
+    T p0 = T(0.5) * (pl + pr) - T(0.125) * du * (rl + rr) * (cl + cr);
+    p0 = p0 < TOL ? TOL : p0;
+    const T gl = dg_sqrt(T(2) / (rl * (gm1 * pl + gp1 * p0)));
+    const T gr = dg_sqrt(T(2) / (rr * (gm1 * pr + gp1 * p0)));
+    T ps = (gl * pl + gr * pr - du) / (gl + gr);
+    ps = ps < TOL ? TOL : ps;
+    T prev = ps, rel = TOL;
+    // f_K(p) and its derivative: shock branch above p_K, rarefaction below
+    auto branch = [&](T pk, T rik, T ck, T& f, T& df) {
+      if (ps > pk) {
+        const T a = T(1) / (gp1 * ps + gm1 * pk);
+        const T b = dg_sqrt(T(2) * a * rik);
+        f = (ps - pk) * b;
+        df = b * (T(1) - T(0.5) * gp1 * (ps - pk) * a);
+      } else {
+        const T b = dg_pow(ps / pk, C3);
+        f = C4 * ck * (b - T(1));
+        df = b * ck * ig / ps;
+      }
+    };
+    for (int it = 0; it < 12 && rel >= TOL; it++) {
+      T fl, dfl, fr, dfr;
+      branch(pl, ril, cl, fl, dfl);
+      branch(pr, rir, cr, fr, dfr);
+      ps -= (fl + fr + du) / (dfl + dfr);
+      ps = ps < TOL ? TOL : ps;
+      rel = dg_abs(ps - prev) / (T(0.5) * (prev + ps));
+      prev = ps;
+    }
+    const T fl = ps > pl ? (ps - pl) * dg_sqrt(T(2) * ril / (gp1 * ps + gm1 * pl)) : C4 * cl * (dg_pow(ps / pl, C3) - T(1));
+    const T fr = ps > pr ? (ps - pr) * dg_sqrt(T(2) * rir / (gp1 * ps + gm1 * pr)) : C4 * cr * (dg_pow(ps / pr, C3) - T(1));
+    const T us = (ul + ur + fr - fl) * T(0.5);
+    const bool left = us >= T(0);
+    const T pK = left ? pl : pr, uK = left ? ul : ur, rK = left ? rl : rr, cK = left ? cl : cr;
+    const T sg = left ? T(1) : T(-1);
+    v = left ? vl : vr;
+    if (ps >= pK) {
+      const T C = gm1 / gp1, q = ps / pK;
+      rho = rK * (q + C) / (C * q + T(1));
+      u = us;
+      p = ps;
+      return;
+    }
+    const T rstar = rK * dg_pow(ps / pK, ig);
+    const T cstar = dg_sqrt(g * ps / rstar);
+    if ((us - sg * cstar) * us <= T(0)) {
+      rho = rstar, u = us, p = ps;
+      return;
+    }
+    const T C = T(2) / gp1, q = C + sg * gm1 * uK / (cK * gp1);
+    rho = rK * dg_pow(q, C4);
+    u = C * (gm1 * T(0.5) * uK + sg * cK);
+    p = pK * dg_pow(q, C4 * g);
+  }
+  // exact Godunov flux (Euler2d::riemannSolver, EulerLaw.cc:853-887, the
+  // reference's default Euler flux): the exact state turned back to (x, y), its
+  // physical flux dotted with n
+  template <typename T>
+  DG_HD static void godunov(const LawParams& L, const T* n, const T* u0, const T* u1, T* out) {
+    const T g = T(L.gamma), gm1 = g - T(1);
+    const T vn1 = n[0] * (u0[1] / u0[0]) + n[1] * (u0[2] / u0[0]);
+    const T vt1 = n[1] * (u0[1] / u0[0]) - n[0] * (u0[2] / u0[0]);
+    const T vn2 = n[0] * (u1[1] / u1[0]) + n[1] * (u1[2] / u1[0]);
+    const T vt2 = n[1] * (u1[1] / u1[0]) - n[0] * (u1[2] / u1[0]);
+    const T p1 = gm1 * u0[3] - T(0.5) * gm1 * (u0[1] * u0[1] + u0[2] * u0[2]) / u0[0];
+    const T p2 = gm1 * u1[3] - T(0.5) * gm1 * (u1[1] * u1[1] + u1[2] * u1[2]) / u1[0];
+    T r, vn, vt, p;
+    exact_state(g, u0[0], vn1, vt1, p1, u1[0], vn2, vt2, p2, r, vn, vt, p);
+    const T q[4] = {r, r * (n[0] * vn + n[1] * vt), r * (n[1] * vn - n[0] * vt),
+                    (T(1) / gm1) * (p + T(0.5) * gm1 * r * (vn * vn + vt * vt))};
+    T F[4][DIM];
+    flux(L, q, F);
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = F[i][0] * n[0] + F[i][1] * n[1];
+  }
   template <typename T>
   DG_HD static void riemann(const LawParams& L, const T* n, const T* uL, const T* uR, T* out) {
     if (L.flux == FLUX_ROE || L.flux == FLUX_ROE_EF)
@@ -247,6 +351,22 @@ struct EulerLawD {
     v2 *= rho_inv;
     T c = dg_sqrt(T(L.gamma) * p * rho_inv);
     return c + dg_sqrt(v2);
+  }
+};
+
+// Euler2d with its own numerical flux, the exact Godunov flux: a law type of
+// its own, so the kernels of the other Euler fluxes keep their register budget
+// (the Newton solver would otherwise set it for all of them)
+struct EulerExactLawD : EulerLawD {
+  template <typename T>
+  DG_HD static void riemann(const LawParams& L, const T* n, const T* uL, const T* uR, T* out) {
+    godunov(L, n, uL, uR, out);
+  }
+  template <typename T>
+  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* Q, T* out) {
+    T g[4];
+    ghost(L, code, n, Q, g);
+    godunov(L, n, Q, g, out);
   }
 };
 

@@ -132,6 +132,8 @@ int launch_mixed_stage(const MixedArgs& a, int law, int ngrid, hipStream_t s) {
   dim3 grid(ngrid), block(MIX_BLOCK);
   if (law == LAW_SCALAR)
     hipLaunchKernelGGL(tri_mixed_stage<ScalarLawD>, grid, block, 0, s, a);
+  else if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT)
+    hipLaunchKernelGGL(tri_mixed_stage<EulerExactLawD>, grid, block, 0, s, a);
   else if (law == LAW_EULER2D)
     hipLaunchKernelGGL(tri_mixed_stage<EulerLawD>, grid, block, 0, s, a);
   else

@@ -175,7 +175,8 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   S.law = law->law;
   S.NF = law->law == DG_LAW_SCALAR ? 1 : law->law == DG_LAW_EULER2D ? 4 : -1;
   if (S.NF < 0) return failS(err, DG_ERR_UNSUPPORTED, "law %d not built for triangles", law->law);
-  if (S.law == DG_LAW_EULER2D && law->flux != DG_FLUX_LF && law->flux != DG_FLUX_ROE && law->flux != DG_FLUX_ROE_EF)
+  if (S.law == DG_LAW_EULER2D && law->flux != DG_FLUX_LF && law->flux != DG_FLUX_ROE && law->flux != DG_FLUX_ROE_EF &&
+      law->flux != DG_FLUX_EXACT)
     return failS(err, DG_ERR_UNSUPPORTED, "Euler flux %d not built", law->flux);
   if (S.law == DG_LAW_SCALAR && law->flux != DG_FLUX_UPWIND)
     return failS(err, DG_ERR_UNSUPPORTED, "scalar flux %d not built", law->flux);

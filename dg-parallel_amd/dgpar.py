@@ -35,7 +35,7 @@ def device_synchronize():
 
 DG_OK, DG_ERR_ARG, DG_ERR_OOM, DG_ERR_HIP, DG_ERR_NONFINITE, DG_ERR_NONPHYSICAL, DG_ERR_UNSUPPORTED = range(7)
 LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
-FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL, FLUX_ROE_EF = 0, 1, 2, 3, 4
+FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL, FLUX_ROE_EF, FLUX_EXACT = 0, 1, 2, 3, 4, 5
 RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
 TAG_FARFIELD, TAG_FARFIELD2, TAG_CURVED_WALL, TAG_CHARACTERISTIC, TAG_WALL = 1, 2, 20000, 30000, 40000

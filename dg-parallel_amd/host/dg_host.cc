@@ -84,7 +84,10 @@ void Euler2d::riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, d
 void Euler2d::boundaryFlux(mVector& n, int BoundaryId, const mPoint&, double* Q, double* flux, double) const {
   if (BoundaryId == DG_TAG_CHARACTERISTIC) throw DGError(DG_ERR_UNSUPPORTED, "characteristic far field not built");
   dgp::LawParams L = params_of(describe());
-  dgp::EulerLawD::boundary<double>(L, BoundaryId == DG_TAG_WALL ? dgp::BC_WALL : dgp::BC_FARFIELD, n.v, Q, flux);
+  const int code = BoundaryId == DG_TAG_WALL || BoundaryId == DG_TAG_CURVED_WALL ? dgp::BC_WALL
+                   : BoundaryId == DG_TAG_FARFIELD2                                ? dgp::BC_FARFIELD2
+                                                                                  : dgp::BC_FARFIELD;
+  dgp::EulerLawD::boundary<double>(L, code, n.v, Q, flux);
 }
 
 double Euler2d::maximumEigenValue(const double* Q, const mPoint&) const {

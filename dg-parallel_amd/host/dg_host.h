@@ -100,12 +100,13 @@ class ScalarLaw : public ConservationLaw {
   dg_law_desc describe() const override;
 };
 
-// Euler2d (EulerLaw.cc:47-69, 182-296, 1012-1034, 1398-1404); the numerical flux
-// is local Lax-Friedrichs here, Roe in Euler2dRoe
+// Euler2d (EulerLaw.cc:47-69, 182-296, 1012-1034, 1398-1404) with its own
+// numerical flux, the exact Godunov flux (EulerLaw.cc:602-887); Roe in
+// Euler2dRoe, local Lax-Friedrichs (new) in Euler2dLF
 class Euler2d : public ConservationLaw {
  protected:
   double gamma;
-  int flux_kind = DG_FLUX_LF;
+  int flux_kind = DG_FLUX_EXACT;
 
  public:
   Euler2d(FieldEvaluator* far, double Gamma = 1.4);
@@ -116,6 +117,11 @@ class Euler2d : public ConservationLaw {
   double maximumEigenValue(const double* Q, const mPoint&) const override;
   bool isPhysical(double* Q) const override;
   dg_law_desc describe() const override;
+};
+
+class Euler2dLF : public Euler2d {
+ public:
+  Euler2dLF(FieldEvaluator* far, double Gamma = 1.4) : Euler2d(far, Gamma) { flux_kind = DG_FLUX_LF; }
 };
 
 class Euler2dRoe : public Euler2d {

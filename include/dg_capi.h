@@ -41,8 +41,11 @@ typedef enum {
 
 enum { DG_LAW_SCALAR = 0, DG_LAW_EULER2D = 1, DG_LAW_MAXWELL3D = 2 };
 /* DG_FLUX_ROE is Euler2dRoe::riemannSolver (EulerLaw.cc:913-1010); DG_FLUX_ROE_EF
- * adds Harten's entropy fix (delta = 0.2 c), which the reference lacks */
-enum { DG_FLUX_UPWIND = 0, DG_FLUX_LF = 1, DG_FLUX_ROE = 2, DG_FLUX_CENTRAL = 3, DG_FLUX_ROE_EF = 4 };
+ * adds Harten's entropy fix (delta = 0.2 c), which the reference lacks;
+ * DG_FLUX_EXACT is Euler2d's own riemannSolver, the exact Godunov flux
+ * (EulerLaw.cc:602-887: Newton for p*, at most 12 iterations to 1e-14) */
+enum { DG_FLUX_UPWIND = 0, DG_FLUX_LF = 1, DG_FLUX_ROE = 2, DG_FLUX_CENTRAL = 3, DG_FLUX_ROE_EF = 4,
+       DG_FLUX_EXACT = 5 };
 enum { DG_RK_FE = 0, DG_RK_TVD2 = 1, DG_RK_TVD3 = 2, DG_RK4 = 3 };
 enum { DG_PREC_FP64 = 0, DG_PREC_FP32 = 1 };
 /* Boundary tags (the reference's classification ids, EulerLaw.cc:182-296):

@@ -757,6 +757,112 @@ static void euler_roe(const or_law* L, const double* n, const double* u0, const 
                      (f1[i][2] + f2[i][2]) * n[2] - roe[i]);
 }
 
+/* Euler::riemann (EulerLaw.cc:602-851): the exact Riemann problem in the face
+ * frame (normal velocity u, tangential v), sampled at x/t = 0, in the
+ * reference's steps:
+ *  - both states supersonic the same way: that state;
+ *  - p* from Newton on f_L(p) + f_R(p) + (u_R - u_L) = 0 (Toro ch. 4), started
+ *    from the two-shock guess around p0 = max(1e-14, PVRS estimate), at most
+ *    12 iterations while |dp|/mean(p) >= 1e-14, p* floored at 1e-14;
+ *  - u* from f_L, f_R recomputed at the final p*;
+ *  - the side of the contact by the sign of u* (u* >= 0: left);
+ *  - on that side: p* >= p_K gives the star state (the reference does not test
+ *    the shock speed); otherwise the star state when the rarefaction tail and
+ *    u* have opposite signs ((u* -+ c*) u* <= 0), else the fan state at x/t = 0
+ *    (no test of the fan head either).
+ * Sound speeds use |gamma p / rho| as the reference does for unphysical states. */
+static void exact_riemann(double gamma, double rhol, double ulft, double vlft, double plft, double rhor,
+                          double urght, double vrght, double prght, double* rho, double* u, double* v,
+                          double* p) {
+  const double TOL = 1e-14, PMIN = 1e-14;
+  const double ril = 1. / rhol, rir = 1. / rhor, gm1 = gamma - 1.;
+  const double cl = sqrt(fabs(gamma * plft * ril)), cr = sqrt(fabs(gamma * prght * rir));
+  if (ulft >= cl && urght >= cr) {
+    *rho = rhol, *u = ulft, *v = vlft, *p = plft;
+    return;
+  }
+  if (urght <= -cr && ulft <= -cl) {
+    *rho = rhor, *u = urght, *v = vrght, *p = prght;
+    return;
+  }
+  const double ig = 1. / gamma, gp1 = gamma + 1.;
+  const double C3 = 0.5 * gm1 * ig, C4 = 2. / gm1;
+  const double du = urght - ulft;
+  double p0 = 0.5 * (plft + prght) - 0.125 * du * (rhol + rhor) * (cl + cr);
+  if (p0 < PMIN) p0 = PMIN;
+  const double gl = sqrt(2. / (rhol * (gm1 * plft + gp1 * p0)));
+  const double gr = sqrt(2. / (rhor * (gm1 * prght + gp1 * p0)));
+  double ps = (gl * plft + gr * prght - du) / (gl + gr);
+  if (ps < TOL) ps = TOL;
+  double prev = ps, rel = TOL;
+  for (int it = 0; it < 12 && rel >= TOL; it++) {
+    double fk[2], dfk[2];
+    const double pk[2] = {plft, prght}, rik[2] = {ril, rir}, ck[2] = {cl, cr};
+    for (int k = 0; k < 2; k++) {
+      if (ps > pk[k]) { /* shock branch */
+        const double a = 1. / (gp1 * ps + gm1 * pk[k]);
+        const double b = sqrt(2. * a * rik[k]);
+        fk[k] = (ps - pk[k]) * b;
+        dfk[k] = b * (1. - 0.5 * gp1 * (ps - pk[k]) * a);
+      } else { /* rarefaction branch */
+        const double b = pow(ps / pk[k], C3);
+        fk[k] = C4 * ck[k] * (b - 1.);
+        dfk[k] = b * ck[k] * ig / ps;
+      }
+    }
+    ps -= (fk[0] + fk[1] + du) / (dfk[0] + dfk[1]);
+    if (ps < PMIN) ps = PMIN;
+    rel = fabs(ps - prev) / (.5 * (prev + ps));
+    prev = ps;
+  }
+  const double fl = ps > plft ? (ps - plft) * sqrt(2. * ril / (gp1 * ps + gm1 * plft)) : C4 * cl * (pow(ps / plft, C3) - 1.);
+  const double fr = ps > prght ? (ps - prght) * sqrt(2. * rir / (gp1 * ps + gm1 * prght)) : C4 * cr * (pow(ps / prght, C3) - 1.);
+  const double us = (ulft + urght + fr - fl) * 0.5;
+  const int left = us >= 0;
+  const double pK = left ? plft : prght, uK = left ? ulft : urght, rK = left ? rhol : rhor, cK = left ? cl : cr;
+  const double sg = left ? 1. : -1.;
+  *v = left ? vlft : vrght;
+  if (ps >= pK) {
+    const double C = gm1 / gp1, q = ps / pK;
+    *rho = rK * (q + C) / (C * q + 1.);
+    *u = us;
+    *p = ps;
+    return;
+  }
+  const double rstar = rK * pow(ps / pK, ig);
+  const double cstar = sqrt(gamma * ps / rstar);
+  if ((us - sg * cstar) * us <= 0.0) {
+    *rho = rstar, *u = us, *p = ps;
+    return;
+  }
+  const double C = 2. / gp1, q = C + sg * gm1 * uK / (cK * gp1);
+  *rho = rK * pow(q, C4);
+  *u = C * (gm1 * 0.5 * uK + sg * cK);
+  *p = pK * pow(q, C4 * gamma);
+}
+
+/* Euler2d::riemannSolver (EulerLaw.cc:853-887): the exact state of the face
+ * frame turned back to (x, y), then its physical flux dotted with n */
+static void euler_godunov(const or_law* L, const double* n, const double* u0, const double* u1, double* flux) {
+  const double g = L->gamma;
+  const double vn1 = n[0] * (u0[1] / u0[0]) + n[1] * (u0[2] / u0[0]);
+  const double vt1 = n[1] * (u0[1] / u0[0]) - n[0] * (u0[2] / u0[0]);
+  const double vn2 = n[0] * (u1[1] / u1[0]) + n[1] * (u1[2] / u1[0]);
+  const double vt2 = n[1] * (u1[1] / u1[0]) - n[0] * (u1[2] / u1[0]);
+  const double p1 = (g - 1) * u0[3] - 0.5 * (g - 1) * (u0[1] * u0[1] + u0[2] * u0[2]) / u0[0];
+  const double p2 = (g - 1) * u1[3] - 0.5 * (g - 1) * (u1[1] * u1[1] + u1[2] * u1[2]) / u1[0];
+  double r, vn, vt, p;
+  exact_riemann(g, u0[0], vn1, vt1, p1, u1[0], vn2, vt2, p2, &r, &vn, &vt, &p);
+  double q[4];
+  q[0] = r;
+  q[1] = r * (n[0] * vn + n[1] * vt);
+  q[2] = r * (n[1] * vn - n[0] * vt);
+  q[3] = (1. / (g - 1)) * (p + 0.5 * (g - 1) * r * (vn * vn + vt * vt));
+  double F[MAXNF][3];
+  law_flux(L, q, F);
+  for (int i = 0; i < 4; i++) flux[i] = F[i][0] * n[0] + F[i][1] * n[1];
+}
+
 /* Maxwell upwind / central flux for eps = mu = 1 (Hesthaven & Warburton 2008,
  * sec. 6.5): F*.n = Fc.n - alpha/2 * jump terms */
 static void maxwell_flux(const or_law* L, const double* n, const double* uL, const double* uR, double* out) {
@@ -797,6 +903,8 @@ static void law_riemann(const or_law* L, const double* n, const double* uL, cons
   } else if (L->law == OR_LAW_EULER2D) {
     if (L->flux == OR_FLUX_ROE || L->flux == OR_FLUX_ROE_EF)
       euler_roe(L, n, uL, uR, out);
+    else if (L->flux == OR_FLUX_EXACT)
+      euler_godunov(L, n, uL, uR, out);
     else
       euler_lf(L, n, uL, uR, out);
   } else {

@@ -39,10 +39,9 @@ def make(law, flux, p, periodic, nx=6, adj_nd=0):
     else:
         vids, xyz, bc = mesh_tri(nx, nx, lx=10, ly=10, periodic_x=periodic, periodic_y=periodic, jitter=0.1, seed=3,
                                  side_tags=(40000, 7, 40000, 7))
-        o = po.Oracle(2, vids, xyz, p, po.LAW_EULER2D, po.FLUX_LF if flux == "lf" else po.FLUX_ROE, gamma=1.4,
-                      farfield=FF_OFF, bc=bc)
-        lw = dgpar.Law(dgpar.LAW_EULER2D, dgpar.FLUX_LF if flux == "lf" else dgpar.FLUX_ROE, gamma=1.4,
-                       farfield=FF_OFF)
+        fl = {"lf": po.FLUX_LF, "roe": po.FLUX_ROE, "exact": po.FLUX_EXACT}[flux]
+        o = po.Oracle(2, vids, xyz, p, po.LAW_EULER2D, fl, gamma=1.4, farfield=FF_OFF, bc=bc)
+        lw = dgpar.Law(dgpar.LAW_EULER2D, fl, gamma=1.4, farfield=FF_OFF)
         U = o.project(1, VORTEX)
     g = dgpar.DGAnalysis(dgpar.Mesh(2, vids, xyz, bc), lw, p, adj_nd=adj_nd)
     return g, o, roughen(U, p)
@@ -57,7 +56,7 @@ def rk4_run(o, U, dt, n):
 
 
 CASES = [("scalar", None, False), ("euler", "lf", True), ("euler", "roe", True), ("euler", "lf", False),
-         ("euler", "roe", False)]
+         ("euler", "roe", False), ("euler", "exact", True), ("euler", "exact", False)]
 
 
 @pytest.mark.parametrize("law,flux,periodic", CASES)
@@ -78,7 +77,8 @@ def test_gpu_jvp_finite_differences_and_dot_product(law, flux, periodic, p):
 
 
 @pytest.mark.parametrize("law,flux,periodic,p", [("scalar", None, False, 2), ("euler", "lf", True, 3),
-                                                 ("euler", "roe", False, 2), ("euler", "roe", False, 4)])
+                                                 ("euler", "roe", False, 2), ("euler", "roe", False, 4),
+                                                 ("euler", "exact", False, 3)])
 def test_gpu_adjoint_gradient_finite_differences(law, flux, periodic, p):
     announce("adjoint FD %s %s periodic=%s P%d" % (law, flux, periodic, p))
     g, o, U0 = make(law, flux, p, periodic)

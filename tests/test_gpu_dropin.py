@@ -27,7 +27,8 @@ def test_cpp_dropin_matches_oracle():
         lines = dict(l.split(None, 1) for l in r.stdout.splitlines())
         dt = float(lines["dt"])
     vids, xyz, bc = mesh_tri(10, 10, lx=10.0, ly=10.0, jitter=0.1, seed=77)
-    o = po.Oracle(2, vids, xyz, 3, po.LAW_EULER2D, po.FLUX_LF, gamma=1.4, farfield=(1.0, 1.0, 1.0, 3.5))
+    # dgpar::Euler2d carries the reference's own flux, the exact Godunov flux
+    o = po.Oracle(2, vids, xyz, 3, po.LAW_EULER2D, po.FLUX_EXACT, gamma=1.4, farfield=(1.0, 1.0, 1.0, 3.5))
     Uo = o.project(1, VORTEX)
     assert np.abs(U0 - Uo).max() / np.abs(Uo).max() < 1e-13
     Ro, _ = o.residual(U0)

@@ -69,7 +69,7 @@ def rel(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
-@pytest.mark.parametrize("flux", [po.FLUX_ROE, po.FLUX_LF, po.FLUX_ROE_EF])
+@pytest.mark.parametrize("flux", [po.FLUX_ROE, po.FLUX_LF, po.FLUX_ROE_EF, po.FLUX_EXACT])
 @pytest.mark.parametrize("seed", [1, 2])
 def test_mixed_residual_and_dt(flux, seed):
     announce("mixed residual flux %d seed %d" % (flux, seed))

@@ -36,10 +36,9 @@ def make_pair(law, flux, p, nx=8, periodic=True, jitter=0.1, seed=3, precision=0
         tags = (40000, 7, 40000, 7)
         vids, xyz, bc = mesh_tri(nx, nx, lx=10, ly=10, periodic_x=periodic, periodic_y=periodic, jitter=jitter,
                                  seed=seed, side_tags=tags)
-        ofl = po.FLUX_LF if flux == "lf" else po.FLUX_ROE
+        ofl = {"lf": po.FLUX_LF, "roe": po.FLUX_ROE, "exact": po.FLUX_EXACT}[flux]
         o = po.Oracle(2, vids, xyz, p, po.LAW_EULER2D, ofl, gamma=1.4, farfield=FREESTREAM, bc=bc)
-        lw = dgpar.Law(dgpar.LAW_EULER2D, dgpar.FLUX_LF if flux == "lf" else dgpar.FLUX_ROE, gamma=1.4,
-                       farfield=FREESTREAM)
+        lw = dgpar.Law(dgpar.LAW_EULER2D, ofl, gamma=1.4, farfield=FREESTREAM)
         U = o.project(1, VORTEX)
     mesh = dgpar.Mesh(2, vids, xyz, bc)
     g = dgpar.DGAnalysis(mesh, lw, p, precision=precision, use_graph=use_graph, kernel=kernel)
@@ -61,7 +60,7 @@ def test_scalar_residual(p, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("flux", ["lf", "roe"])
+@pytest.mark.parametrize("flux", ["lf", "roe", "exact"])
 @pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_euler_residual_periodic(p, flux, kernel):
     announce("euler %s residual P%d periodic kernel %d" % (flux, p, kernel))
@@ -70,7 +69,7 @@ def test_euler_residual_periodic(p, flux, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("flux", ["lf", "roe"])
+@pytest.mark.parametrize("flux", ["lf", "roe", "exact"])
 @pytest.mark.parametrize("p", [1, 3])
 def test_euler_residual_walls_and_farfield(p, flux, kernel):
     announce("euler %s residual P%d walls kernel %d" % (flux, p, kernel))
@@ -89,7 +88,7 @@ def test_state_roundtrip_and_connectivity():
 
 @pytest.mark.parametrize("law,flux,p,scheme", [("scalar", None, 2, po.RK_TVD3), ("scalar", None, 1, po.RK_FE),
                                                ("euler", "lf", 3, po.RK4), ("euler", "roe", 2, po.RK_TVD2),
-                                               ("euler", "roe", 3, po.RK_TVD3)])
+                                               ("euler", "roe", 3, po.RK_TVD3), ("euler", "exact", 3, po.RK4)])
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_steps(law, flux, p, scheme, kernel):
     announce("steps %s %s P%d scheme %d kernel %d" % (law, flux, p, scheme, kernel))
@@ -167,6 +166,14 @@ def test_fp32_form_error_budget(kernel):
     l2 = np.sqrt((dd ** 2 * o.detj()[:, None]).sum())
     assert l2 < 1e-5
     assert abs(e32[0] - e64[0]) < 1e-5
+
+
+def test_fp32_form_exact_flux():
+    announce("fp32 form, exact Godunov flux")
+    import dgpar
+    g64, o, U = make_pair("euler", "exact", 3, nx=16)
+    g32, _, _ = make_pair("euler", "exact", 3, nx=16, precision=dgpar.PREC_FP32)
+    assert rel(g32.residual(), g64.residual()) < 5e-5
 
 
 def test_error_paths():

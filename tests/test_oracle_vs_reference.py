@@ -56,7 +56,8 @@ def euler_state(o, p, seed):
 
 
 CASES = [("scalar", "upwind", True), ("euler", "lf", True), ("euler", "roe", True), ("euler", "roe", False),
-         ("euler", "lf", False)]
+         ("euler", "lf", False), ("euler", "exact", True), ("euler", "exact", False)]
+EULER_FLUX = {"lf": po.FLUX_LF, "roe": po.FLUX_ROE, "exact": po.FLUX_EXACT}
 
 
 def build(law, flux, periodic, p, n=5):
@@ -66,7 +67,7 @@ def build(law, flux, periodic, p, n=5):
                       bc=m["bc"])
         U = np.random.default_rng(p).standard_normal(o.ndof)
     else:
-        o = po.Oracle(2, m["top"], m["xyz"], p, po.LAW_EULER2D, po.FLUX_LF if flux == "lf" else po.FLUX_ROE,
+        o = po.Oracle(2, m["top"], m["xyz"], p, po.LAW_EULER2D, EULER_FLUX[flux],
                       gamma=1.4, farfield=FF, bc=m["bc"])
         U = euler_state(o, p, p)
     return m, o, U
@@ -99,7 +100,8 @@ def test_residual_reference_tables(law, flux, periodic, p, reference_tables):
 @pytest.mark.parametrize("law,flux,periodic,p,op,scheme", [
     ("scalar", "upwind", True, 2, "tvd3", po.RK_TVD3), ("scalar", "upwind", True, 1, "fe", po.RK_FE),
     ("euler", "lf", True, 3, "rk4", po.RK4), ("euler", "roe", False, 2, "tvd2", po.RK_TVD2),
-    ("euler", "roe", False, 3, "tvd3", po.RK_TVD3)])
+    ("euler", "roe", False, 3, "tvd3", po.RK_TVD3), ("euler", "exact", False, 2, "rk4", po.RK4),
+    ("euler", "exact", True, 3, "tvd3", po.RK_TVD3)])
 def test_steps_reference_tables(law, flux, periodic, p, op, scheme, reference_tables):
     m, o, U = build(law, flux, periodic, p)
     dt = 0.2 * o.dt(U, 0.1)
@@ -122,7 +124,7 @@ def build_mixed(law, flux, periodic, seed, n=6):
                       farfield=(0.3,), bc=m["bc"])
         U = np.random.default_rng(seed).standard_normal(o.ndof)
     else:
-        o = po.Oracle(2, m["top"], m["xyz"], orders, po.LAW_EULER2D, po.FLUX_LF if flux == "lf" else po.FLUX_ROE,
+        o = po.Oracle(2, m["top"], m["xyz"], orders, po.LAW_EULER2D, EULER_FLUX[flux],
                       gamma=1.4, farfield=FF, bc=m["bc"])
         rng = np.random.default_rng(seed)
         blocks = []
@@ -138,7 +140,8 @@ def build_mixed(law, flux, periodic, seed, n=6):
 # cells hold different orders (adaptOrder re-initialises the faces, the periodic
 # partner faces do not survive it), so mixed orders are compared on bounded
 # meshes: walls at the bottom and top, far field at the sides.
-MIXED_CASES = [("scalar", "upwind", False), ("euler", "lf", False), ("euler", "roe", False)]
+MIXED_CASES = [("scalar", "upwind", False), ("euler", "lf", False), ("euler", "roe", False),
+               ("euler", "exact", False)]
 
 
 @pytest.mark.parametrize("law,flux,periodic", MIXED_CASES)
