@@ -264,8 +264,7 @@ struct TiledVjp {
           for (int i = 0; i < ND; i++)
             if (f == j0 + i) uL[f].d[i] = 1.0;
         }
-        if (link < 0) LAW::ghost(a.law, -link, n, uL, uR);
-        LAW::riemann(a.law, n, uL, uR, Fh);
+        face_flux<LAW>(a.law, link, n, uL, uR, Fh);
 #pragma unroll
         for (int i = 0; i < ND; i++) {
           double acc = 0.0;
@@ -414,10 +413,8 @@ DG_HD void jvp_element(const AdjArgs& a, int e) {
           }
           uR[f] = s;
         }
-      } else {
-        LAW::ghost(a.law, -link, n, uL, uR);
       }
-      LAW::riemann(a.law, n, uL, uR, Fh);
+      face_flux<LAW>(a.law, link, n, uL, uR, Fh);
       const double sw = len * tb[TriBlock::off_wf(P) + q];
       for (int f = 0; f < NF; f++)
         for (int k = 0; k < NP; k++) R[f][k] -= (sw * tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k]) * Fh[f];

@@ -172,8 +172,7 @@ __global__ __launch_bounds__(BLOCK) void tri_stage_kernel(StageArgs a) {
           uL[f] = s;
         }
         CT Fh[NF];
-        if (link < 0) LAW::ghost(law, -link, n, uL, uR[q]);
-        LAW::riemann(law, n, uL, uR[q], Fh);
+        face_flux<LAW>(law, link, n, uL, uR[q], Fh);
         const CT sw = len * tb[TriBlock::off_wf(P) + q];  // |J_face| w = (len/2) w
 #pragma unroll
         for (int f = 0; f < NF; f++) {
@@ -309,6 +308,7 @@ int launch_tri_stage(const StageArgs& a, int law, int variant, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_law<ScalarLawD>(a, variant, s);
   if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT) return launch_law<EulerExactLawD>(a, variant, s);
   if (law == LAW_EULER2D) return launch_law<EulerLawD>(a, variant, s);
+
   return -1;
 }
 
@@ -373,6 +373,8 @@ static int launch_dt_law(const DtArgs& a, hipStream_t s) {
 int launch_tri_dt(const DtArgs& a, int law, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_dt_law<ScalarLawD>(a, s);
   if (law == LAW_EULER2D) return launch_dt_law<EulerLawD>(a, s);
+  if (law == LAW_MAXWELL2D) return launch_dt_law<Maxwell2dLawD>(a, s);
+  if (law == LAW_BURGERS2D) return launch_dt_law<Burgers2DLawD>(a, s);
   return -1;
 }
 

@@ -12,6 +12,8 @@
 //                Euler2d::riemannSolver (EulerLaw.cc:602-887); local Lax-Friedrichs
 //                flux is new.
 //   MaxwellLawD  3D Maxwell, eps = mu = 1, upwind/central flux, PEC ghost (new).
+//   Maxwell2dLawD Maxwell2d, the TE fields (MaxwellLaw.cc:21-200).
+//   Burgers2DLawD Burgers2D with its central flux (Burgers2D.cc:22-92).
 #pragma once
 
 #ifdef __HIPCC__
@@ -25,7 +27,7 @@
 
 namespace dgp {
 
-enum { LAW_SCALAR = 0, LAW_EULER2D = 1, LAW_MAXWELL3D = 2 };
+enum { LAW_SCALAR = 0, LAW_EULER2D = 1, LAW_MAXWELL3D = 2, LAW_MAXWELL2D = 3, LAW_BURGERS2D = 4 };
 enum { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2, FLUX_CENTRAL = 3, FLUX_ROE_EF = 4, FLUX_EXACT = 5 };
 // boundary codes: far field (farfield[0..NF)), wall, PEC, second far field
 // (farfield[4..8), Euler: a far-field state that differs by boundary, as the
@@ -67,6 +69,7 @@ DG_HD float dg_rcp<float>(float x) {
 struct ScalarLawD {
   static constexpr int NF = 1;
   static constexpr int DIM = 2;
+  static constexpr bool GHOST = true;  // boundary faces: ghost state + two-point flux
   template <typename T>
   DG_HD static void flux(const LawParams& L, const T* q, T F[][DIM]) {
     F[0][0] = T(L.vel[0]) * q[0];
@@ -107,6 +110,7 @@ struct ScalarLawD {
 struct EulerLawD {
   static constexpr int NF = 4;
   static constexpr int DIM = 2;
+  static constexpr bool GHOST = true;  // boundary faces: ghost state + two-point flux
   template <typename T>
   DG_HD static void flux(const LawParams& L, const T* Q, T F[][DIM]) {
     T g = T(L.gamma);
@@ -370,12 +374,118 @@ struct EulerExactLawD : EulerLawD {
   }
 };
 
+// Maxwell2d (MaxwellLaw.cc:21-37, 115-200): the TE fields (Hx, Hy, Ez), eps = mu
+// = 1.  F_Hx = (0, Ez), F_Hy = (-Ez, 0), F_Ez = (-Hy, Hx).  The numerical flux is
+// the physical flux of the reference's interface state, in its condensed form
+// (MaxwellLaw.cc:170-175).  Walls (40000, and 20000 on straight faces) mirror H
+// about the normal (MaxwellLaw.cc:37-56, 115-126); other boundary faces take
+// the far-field state; the wave speed is 1.  The reference's RHS holds a pulse
+// source of one scenario (MaxwellLaw.cc:362-367); this law has none.
+struct Maxwell2dLawD {
+  static constexpr int NF = 3;
+  static constexpr int DIM = 2;
+  static constexpr bool GHOST = true;
+  template <typename T>
+  DG_HD static void flux(const LawParams&, const T* Q, T F[][DIM]) {
+    F[0][0] = T(0);  F[0][1] = Q[2];
+    F[1][0] = -Q[2]; F[1][1] = T(0);
+    F[2][0] = -Q[1]; F[2][1] = Q[0];
+  }
+  template <typename T>
+  DG_HD static void riemann(const LawParams&, const T* n, const T* u0, const T* u1, T* out) {
+    const T nx = n[0], ny = n[1], h = T(2);
+    const T s0 = (u0[0] + u1[0]) * (nx * ny / h + ny * ny / h) + (u0[1] + u1[1]) * (nx * nx / h - nx * ny / h) +
+                 (u0[2] - u1[2]) * ny / h;
+    const T s1 = (u0[0] + u1[0]) * (-nx * ny / h + ny * ny / h) + (u0[1] + u1[1]) * (nx * nx / h + nx * ny / h) +
+                 (u0[2] - u1[2]) * (-nx) / h;
+    const T s2 = ny * (u0[0] - u1[0]) / h + nx * (u1[1] - u0[1]) / h + (u0[2] + u1[2]) / h;
+    out[0] = s2 * ny;
+    out[1] = -s2 * nx;
+    out[2] = -s1 * nx + s0 * ny;
+  }
+  template <typename T>
+  DG_HD static void ghost(const LawParams& L, int code, const T* n, const T* Q, T* g) {
+    if (code == BC_WALL) {
+      const T hn = Q[0] * n[0] + Q[1] * n[1];
+      g[0] = -Q[0] + T(2) * n[0] * hn;
+      g[1] = -Q[1] + T(2) * n[1] * hn;
+      g[2] = Q[2];
+    } else {
+      for (int i = 0; i < 3; i++) g[i] = T(L.farfield[i]);
+    }
+  }
+  template <typename T>
+  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* Q, T* out) {
+    T g[3];
+    ghost(L, code, n, Q, g);
+    riemann(L, n, Q, g, out);
+  }
+  template <typename T>
+  DG_HD static bool physical(const LawParams&, const T*) { return true; }
+  template <typename T>
+  DG_HD static T maxev(const LawParams&, const T*) { return T(1); }
+};
+
+// Burgers2D (Burgers2D.cc:22-92): F = (u^2/2, u^2/2).  The numerical flux is the
+// reference's central flux (its dissipation coefficient maxIV is 0, :74-77).  A
+// boundary face takes the interior flux on outflow and the far-field flux on
+// inflow (boundaryFlux, :31-49); no ghost state gives that through a central
+// flux, so the law supplies its boundary flux (GHOST = false).  The wave speed
+// is 1 (maximumEigenValue, :88-92).
+struct Burgers2DLawD {
+  static constexpr int NF = 1;
+  static constexpr int DIM = 2;
+  static constexpr bool GHOST = false;
+  template <typename T>
+  DG_HD static void flux(const LawParams&, const T* Q, T F[][DIM]) {
+    F[0][0] = Q[0] * Q[0] * T(0.5);
+    F[0][1] = Q[0] * Q[0] * T(0.5);
+  }
+  template <typename T>
+  DG_HD static void riemann(const LawParams&, const T* n, const T* u0, const T* u1, T* out) {
+    out[0] = T(0.5) * ((u0[0] * u0[0] + u1[0] * u1[0]) * (n[0] + n[1]) * T(0.5));
+  }
+  // the state seen across a boundary face by the limiter: the far field
+  template <typename T>
+  DG_HD static void ghost(const LawParams& L, int, const T*, const T*, T* g) {
+    g[0] = T(L.farfield[0]);
+  }
+  template <typename T>
+  DG_HD static void boundary(const LawParams& L, int, const T* n, const T* Q, T* out) {
+    const T vn = Q[0] * Q[0] * T(0.5) * (n[0] + n[1]);
+    const T ff = T(L.farfield[0]);
+    out[0] = vn < T(0) ? ff * ff * T(0.5) * (n[0] + n[1]) : vn;
+  }
+  template <typename T>
+  DG_HD static bool physical(const LawParams&, const T*) { return true; }
+  template <typename T>
+  DG_HD static T maxev(const LawParams&, const T*) { return T(1); }
+};
+
+// Numerical flux on a face.  Interior faces (link >= 0): the law's two-point
+// flux of the two traces.  Boundary faces (link < 0, code -link): the two-point
+// flux against the law's ghost state, written over uR, or the law's own
+// boundary flux when no ghost state expresses it.
+template <class LAW, typename T>
+DG_HD void face_flux(const LawParams& L, int link, const T* n, const T* uL, T* uR, T* out) {
+  if constexpr (LAW::GHOST) {
+    if (link < 0) LAW::ghost(L, -link, n, uL, uR);
+    LAW::riemann(L, n, uL, uR, out);
+  } else {
+    if (link < 0)
+      LAW::boundary(L, -link, n, uL, out);
+    else
+      LAW::riemann(L, n, uL, uR, out);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Maxwell in 3D with eps = mu = 1: dE/dt = curl H, dH/dt = -curl E, written
 // as dq/dt + div F(q) = 0 with (F.n)_E = -n x H and (F.n)_H = n x E.
 struct MaxwellLawD {
   static constexpr int NF = 6;
   static constexpr int DIM = 3;
+  static constexpr bool GHOST = true;  // boundary faces: ghost state + two-point flux
   template <typename T>
   DG_HD static void flux(const LawParams&, const T* Q, T F[][DIM]) {
     const T *E = Q, *H = Q + 3;

@@ -136,6 +136,10 @@ int launch_mixed_stage(const MixedArgs& a, int law, int ngrid, hipStream_t s) {
     hipLaunchKernelGGL(tri_mixed_stage<EulerExactLawD>, grid, block, 0, s, a);
   else if (law == LAW_EULER2D)
     hipLaunchKernelGGL(tri_mixed_stage<EulerLawD>, grid, block, 0, s, a);
+  else if (law == LAW_MAXWELL2D)
+    hipLaunchKernelGGL(tri_mixed_stage<Maxwell2dLawD>, grid, block, 0, s, a);
+  else if (law == LAW_BURGERS2D)
+    hipLaunchKernelGGL(tri_mixed_stage<Burgers2DLawD>, grid, block, 0, s, a);
   else
     return -1;
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -149,6 +153,10 @@ int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipSt
     hipLaunchKernelGGL(tri_mixed_dt<ScalarLawD>, grid, block, 0, s, a, cfl, part);
   else if (law == LAW_EULER2D)
     hipLaunchKernelGGL(tri_mixed_dt<EulerLawD>, grid, block, 0, s, a, cfl, part);
+  else if (law == LAW_MAXWELL2D)
+    hipLaunchKernelGGL(tri_mixed_dt<Maxwell2dLawD>, grid, block, 0, s, a, cfl, part);
+  else if (law == LAW_BURGERS2D)
+    hipLaunchKernelGGL(tri_mixed_dt<Burgers2DLawD>, grid, block, 0, s, a, cfl, part);
   else
     return -1;
   return hipGetLastError() == hipSuccess ? 0 : -2;

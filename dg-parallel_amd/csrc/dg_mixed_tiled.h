@@ -283,8 +283,7 @@ struct TiledMixed {
         double uR[NF], Fh[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) uR[f] = s.W[(l * MIX_NQF + q) * NF + f][t];
-        if (link < 0) LAW::ghost(a.law, -link, n, uL, uR);
-        LAW::riemann(a.law, n, uL, uR, Fh);
+        face_flux<LAW>(a.law, link, n, uL, uR, Fh);
         const double sw = s.Ns[l][2][t] * a.glw[GLW_OFF(nq) + q];
 #pragma unroll
         for (int f = 0; f < NF; f++) s.W[(l * MIX_NQF + q) * NF + f][t] = Fh[f] * sw;

@@ -173,13 +173,21 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   if (m->dim != 2) return failS(err, DG_ERR_UNSUPPORTED, "only triangles (dim 2) are built");
   S.E = m->n_elems;
   S.law = law->law;
-  S.NF = law->law == DG_LAW_SCALAR ? 1 : law->law == DG_LAW_EULER2D ? 4 : -1;
+  S.NF = law->law == DG_LAW_SCALAR      ? 1
+         : law->law == DG_LAW_EULER2D   ? 4
+         : law->law == DG_LAW_MAXWELL2D ? 3
+         : law->law == DG_LAW_BURGERS2D ? 1
+                                        : -1;
   if (S.NF < 0) return failS(err, DG_ERR_UNSUPPORTED, "law %d not built for triangles", law->law);
   if (S.law == DG_LAW_EULER2D && law->flux != DG_FLUX_LF && law->flux != DG_FLUX_ROE && law->flux != DG_FLUX_ROE_EF &&
       law->flux != DG_FLUX_EXACT)
     return failS(err, DG_ERR_UNSUPPORTED, "Euler flux %d not built", law->flux);
   if (S.law == DG_LAW_SCALAR && law->flux != DG_FLUX_UPWIND)
     return failS(err, DG_ERR_UNSUPPORTED, "scalar flux %d not built", law->flux);
+  if (S.law == DG_LAW_MAXWELL2D && law->flux != DG_FLUX_UPWIND)
+    return failS(err, DG_ERR_UNSUPPORTED, "Maxwell2d flux %d not built (its own upwind state: DG_FLUX_UPWIND)", law->flux);
+  if (S.law == DG_LAW_BURGERS2D && law->flux != DG_FLUX_CENTRAL)
+    return failS(err, DG_ERR_UNSUPPORTED, "Burgers2D flux %d not built (its central flux: DG_FLUX_CENTRAL)", law->flux);
   // per-element orders (mesh->order) or one order for all (opt->order)
   std::vector<int> ord(S.E, opt->order);
   if (m->order)

@@ -368,8 +368,7 @@ struct TiledStage {
         }
         // boundary faces: the law's ghost state, then the same two-point flux (one
         // inlined flux for interior and boundary lanes alike)
-        if (link < 0) LAW::ghost(a.law, -link, n, uL, uR);
-        LAW::riemann(a.law, n, uL, uR, Fh);
+        face_flux<LAW>(a.law, link, n, uL, uR, Fh);
         // |J_face| w_q = (len / 2) w_q; the table holds w_q / 2
         const CT sw = len * tb[TriBlock::off_wf(P) + q];
 #pragma unroll

@@ -90,6 +90,8 @@ int launch_tiled_nw(const StageArgs& a, int law, int prec, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_tiled_law<ScalarLawD, NW>(a, prec, s);
   if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT) return launch_tiled_law<EulerExactLawD, NW>(a, prec, s);
   if (law == LAW_EULER2D) return launch_tiled_law<EulerLawD, NW>(a, prec, s);
+  if (law == LAW_MAXWELL2D) return launch_tiled_law<Maxwell2dLawD, NW>(a, prec, s);
+  if (law == LAW_BURGERS2D) return launch_tiled_law<Burgers2DLawD, NW>(a, prec, s);
   return -1;
 }
 

@@ -34,7 +34,7 @@ def device_synchronize():
         raise RuntimeError("hipDeviceSynchronize failed with %d" % st)
 
 DG_OK, DG_ERR_ARG, DG_ERR_OOM, DG_ERR_HIP, DG_ERR_NONFINITE, DG_ERR_NONPHYSICAL, DG_ERR_UNSUPPORTED = range(7)
-LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
+LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D, LAW_MAXWELL2D, LAW_BURGERS2D = 0, 1, 2, 3, 4
 FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL, FLUX_ROE_EF, FLUX_EXACT = 0, 1, 2, 3, 4, 5
 RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
@@ -270,7 +270,17 @@ class Law:
 
     @property
     def nfields(self):
-        return {LAW_SCALAR: 1, LAW_EULER2D: 4, LAW_MAXWELL3D: 6}[self.law]
+        return {LAW_SCALAR: 1, LAW_EULER2D: 4, LAW_MAXWELL3D: 6, LAW_MAXWELL2D: 3, LAW_BURGERS2D: 1}[self.law]
+
+
+def Maxwell2dLaw(farfield=(0.0, 0.0, 0.0)):
+    """Maxwell2d (MaxwellLaw.cc:21-200): (Hx, Hy, Ez), its own upwind interface state"""
+    return Law(LAW_MAXWELL2D, FLUX_UPWIND, farfield=farfield)
+
+
+def Burgers2DLaw(farfield=(0.0,)):
+    """Burgers2D (Burgers2D.cc:22-92): F = (u^2/2, u^2/2), central flux"""
+    return Law(LAW_BURGERS2D, FLUX_CENTRAL, farfield=farfield)
 
 
 def ScalarLaw(vel):

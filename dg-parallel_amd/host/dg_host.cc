@@ -60,6 +60,55 @@ dg_law_desc ScalarLaw::describe() const {
 }
 
 // ---------------------------------------------------------------------------
+template <class LAW, int NF>
+static void law_fi(const dg_law_desc& d, double* Q, mVector* flux) {
+  dgp::LawParams L = params_of(d);
+  double F[NF][2];
+  LAW::template flux<double>(L, Q, F);
+  for (int f = 0; f < NF; f++) {
+    flux[f](0) = F[f][0];
+    flux[f](1) = F[f][1];
+    flux[f](2) = 0.0;
+  }
+}
+
+static int bc_code(int BoundaryId) {
+  return BoundaryId == DG_TAG_WALL || BoundaryId == DG_TAG_CURVED_WALL ? dgp::BC_WALL
+         : BoundaryId == DG_TAG_FARFIELD2                             ? dgp::BC_FARFIELD2
+                                                                     : dgp::BC_FARFIELD;
+}
+
+void Maxwell2d::Fi(const mPoint&, double* Q, mVector* flux) const { law_fi<dgp::Maxwell2dLawD, 3>(describe(), Q, flux); }
+void Maxwell2d::riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* flux) const {
+  dgp::Maxwell2dLawD::riemann<double>(params_of(describe()), n.v, uL, uR, flux);
+}
+void Maxwell2d::boundaryFlux(mVector& n, int BoundaryId, const mPoint&, double* Q, double* flux, double) const {
+  dgp::Maxwell2dLawD::boundary<double>(params_of(describe()), bc_code(BoundaryId), n.v, Q, flux);
+}
+dg_law_desc Maxwell2d::describe() const {
+  dg_law_desc d{};
+  d.law = DG_LAW_MAXWELL2D;
+  d.flux = DG_FLUX_UPWIND;
+  far_state(farField, 3, d.farfield);
+  return d;
+}
+
+void Burgers2D::Fi(const mPoint&, double* Q, mVector* flux) const { law_fi<dgp::Burgers2DLawD, 1>(describe(), Q, flux); }
+void Burgers2D::riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* flux) const {
+  dgp::Burgers2DLawD::riemann<double>(params_of(describe()), n.v, uL, uR, flux);
+}
+void Burgers2D::boundaryFlux(mVector& n, int BoundaryId, const mPoint&, double* Q, double* flux, double) const {
+  dgp::Burgers2DLawD::boundary<double>(params_of(describe()), bc_code(BoundaryId), n.v, Q, flux);
+}
+dg_law_desc Burgers2D::describe() const {
+  dg_law_desc d{};
+  d.law = DG_LAW_BURGERS2D;
+  d.flux = DG_FLUX_CENTRAL;
+  far_state(farField, 1, d.farfield);
+  return d;
+}
+
+// ---------------------------------------------------------------------------
 Euler2d::Euler2d(FieldEvaluator* far, double Gamma) : gamma(Gamma) {
   farField = far;
   exactField = far;

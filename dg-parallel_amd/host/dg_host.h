@@ -100,6 +100,32 @@ class ScalarLaw : public ConservationLaw {
   dg_law_desc describe() const override;
 };
 
+// Maxwell2d (MaxwellLaw.cc:21-200): (Hx, Hy, Ez), its own upwind interface
+// state, walls (40000, 20000) mirror H, other boundaries take the far field
+class Maxwell2d : public ConservationLaw {
+ public:
+  explicit Maxwell2d(FieldEvaluator* far) { farField = far; exactField = far; }
+  int getNbFields() const override { return 3; }
+  void Fi(const mPoint&, double* Q, mVector* flux) const override;
+  void riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* flux) const override;
+  void boundaryFlux(mVector& n, int BoundaryId, const mPoint&, double* Q, double* flux, double T) const override;
+  double maximumEigenValue(const double*, const mPoint&) const override { return 1.0; }
+  dg_law_desc describe() const override;
+};
+
+// Burgers2D (Burgers2D.cc:22-92): F = (u^2/2, u^2/2), central flux, far field
+// upwind on boundaries
+class Burgers2D : public ConservationLaw {
+ public:
+  explicit Burgers2D(FieldEvaluator* far) { farField = far; exactField = far; }
+  int getNbFields() const override { return 1; }
+  void Fi(const mPoint&, double* Q, mVector* flux) const override;
+  void riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* flux) const override;
+  void boundaryFlux(mVector& n, int BoundaryId, const mPoint&, double* Q, double* flux, double T) const override;
+  double maximumEigenValue(const double*, const mPoint&) const override { return 1.0; }
+  dg_law_desc describe() const override;
+};
+
 // Euler2d (EulerLaw.cc:47-69, 182-296, 1012-1034, 1398-1404) with its own
 // numerical flux, the exact Godunov flux (EulerLaw.cc:602-887); Roe in
 // Euler2dRoe, local Lax-Friedrichs (new) in Euler2dLF

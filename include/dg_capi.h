@@ -39,7 +39,11 @@ typedef enum {
   DG_ERR_UNSUPPORTED = 6   /* law / order / element combination not built */
 } dg_status;
 
-enum { DG_LAW_SCALAR = 0, DG_LAW_EULER2D = 1, DG_LAW_MAXWELL3D = 2 };
+/* DG_LAW_MAXWELL2D: Maxwell2d, the TE fields (Hx, Hy, Ez) with its own upwind
+ * interface state (MaxwellLaw.cc:21-200; flux DG_FLUX_UPWIND; walls mirror H);
+ * DG_LAW_BURGERS2D: Burgers2D, F = (u^2/2, u^2/2) with its central flux
+ * (Burgers2D.cc:22-92; flux DG_FLUX_CENTRAL; far field upwind on boundaries) */
+enum { DG_LAW_SCALAR = 0, DG_LAW_EULER2D = 1, DG_LAW_MAXWELL3D = 2, DG_LAW_MAXWELL2D = 3, DG_LAW_BURGERS2D = 4 };
 /* DG_FLUX_ROE is Euler2dRoe::riemannSolver (EulerLaw.cc:913-1010); DG_FLUX_ROE_EF
  * adds Harten's entropy fix (delta = 0.2 c), which the reference lacks;
  * DG_FLUX_EXACT is Euler2d's own riemannSolver, the exact Godunov flux

@@ -284,6 +284,8 @@ static int nfields_of(const or_law* l) {
     case OR_LAW_SCALAR: return 1;
     case OR_LAW_EULER2D: return 4;
     case OR_LAW_MAXWELL3D: return 6;
+    case OR_LAW_MAXWELL2D: return 3;
+    case OR_LAW_BURGERS2D: return 1;
   }
   return -1;
 }
@@ -645,6 +647,16 @@ void or_detj(const or_ctx* c, double* out) { memcpy(out, c->detj, sizeof(double)
 static void law_flux(const or_law* L, const double* Q, double F[MAXNF][3]) {
   if (L->law == OR_LAW_SCALAR) {
     for (int d = 0; d < 3; d++) F[0][d] = L->vel[d] * Q[0];
+  } else if (L->law == OR_LAW_MAXWELL2D) {
+    /* Maxwell2d::Fi (MaxwellLaw.cc:21-37), Q = (Hx, Hy, Ez) */
+    F[0][0] = 0.0; F[0][1] = Q[2]; F[0][2] = 0.0;
+    F[1][0] = -1 * Q[2]; F[1][1] = 0.0; F[1][2] = 0.0;
+    F[2][0] = -1 * Q[1]; F[2][1] = Q[0]; F[2][2] = 0.0;
+  } else if (L->law == OR_LAW_BURGERS2D) {
+    /* Burgers2D::Fi (Burgers2D.cc:22-28) */
+    F[0][0] = Q[0] * Q[0] * 0.5;
+    F[0][1] = Q[0] * Q[0] * 0.5;
+    F[0][2] = 0.0;
   } else if (L->law == OR_LAW_EULER2D) {
     double g = L->gamma;
     double invrho = 1. / Q[0];
@@ -893,9 +905,28 @@ static void maxwell_flux(const or_law* L, const double* n, const double* uL, con
   }
 }
 
+/* Maxwell2d::riemannSolver (MaxwellLaw.cc:137-195): the physical flux of the
+ * interface state in the reference's condensed form */
+static void maxwell2d_flux(const or_law* L, const double* n, const double* u0, const double* u1, double* flux) {
+  double nx = n[0], ny = n[1], s[3], F[MAXNF][3];
+  s[0] = (u0[0] + u1[0]) * (nx * ny / 2 + ny * ny / 2) + (u0[1] + u1[1]) * (nx * nx / 2 - nx * ny / 2) +
+         (u0[2] - u1[2]) * ny / 2;
+  s[1] = (u0[0] + u1[0]) * (-nx * ny / 2 + ny * ny / 2) + (u0[1] + u1[1]) * (nx * nx / 2 + nx * ny / 2) +
+         (u0[2] - u1[2]) * (-nx) / 2;
+  s[2] = ny * (u0[0] - u1[0]) / 2 + nx * (u1[1] - u0[1]) / 2 + (u0[2] + u1[2]) / 2;
+  law_flux(L, s, F);
+  for (int i = 0; i < 3; i++) flux[i] = F[i][0] * n[0] + F[i][1] * n[1] + F[i][2] * n[2];
+}
+
 /* numerical flux Fhat(uL, uR).n (riemannSolver) */
 static void law_riemann(const or_law* L, const double* n, const double* uL, const double* uR, double* out) {
-  if (L->law == OR_LAW_SCALAR) {
+  if (L->law == OR_LAW_MAXWELL2D) {
+    maxwell2d_flux(L, n, uL, uR, out);
+  } else if (L->law == OR_LAW_BURGERS2D) {
+    /* Burgers2D::riemannSolver (Burgers2D.cc:58-78): central, maxIV = 0 */
+    double maxIV = 0;
+    out[0] = 0.5 * ((uL[0] * uL[0] + uR[0] * uR[0]) * (n[0] + n[1]) * 0.5 - maxIV * (uR[0] - uL[0]));
+  } else if (L->law == OR_LAW_SCALAR) {
     /* ScalarLaw::riemannSolver (ScalarLaw.cc:69-88) */
     double vn = n[0] * L->vel[0] + n[1] * L->vel[1] + n[2] * L->vel[2];
     double u = vn > 0 ? uL[0] : uR[0];
@@ -929,6 +960,26 @@ static void law_boundary(const or_law* L, int tag, const double* n, const double
                          double* out) {
   int Nf = nfields_of(L);
   double ghost[MAXNF];
+  if (L->law == OR_LAW_BURGERS2D) {
+    /* Burgers2D::boundaryFlux (Burgers2D.cc:31-49): every tag */
+    double nv = Q[0] * Q[0] * 0.5 * (n[0] + n[1]);
+    out[0] = nv < 0 ? L->farfield[0] * L->farfield[0] * 0.5 * (n[0] + n[1]) : nv;
+    return;
+  }
+  if (L->law == OR_LAW_MAXWELL2D) {
+    if (tag == 40000 || tag == 20000) {
+      /* Maxwell2d::boundaryFlux reflection (MaxwellLaw.cc:115-126); the curved
+       * wall's boundary() (:37-56) with N = n on straight faces */
+      double hn = Q[0] * n[0] + Q[1] * n[1];
+      ghost[0] = -Q[0] + 2 * n[0] * hn;
+      ghost[1] = -Q[1] + 2 * n[1] * hn;
+      ghost[2] = Q[2];
+    } else {
+      for (int i = 0; i < 3; i++) ghost[i] = L->farfield[i];
+    }
+    law_riemann(L, n, Q, ghost, out);
+    return;
+  }
   if (L->law == OR_LAW_SCALAR) {
     /* ScalarLaw::boundaryFlux (ScalarLaw.cc:39-59) */
     double vn = n[0] * L->vel[0] + n[1] * L->vel[1] + n[2] * L->vel[2];

@@ -19,7 +19,8 @@
 extern "C" {
 #endif
 
-enum { OR_LAW_SCALAR = 0, OR_LAW_EULER2D = 1, OR_LAW_MAXWELL3D = 2 };
+/* OR_LAW_MAXWELL2D: Maxwell2d (MaxwellLaw.cc:21-200); OR_LAW_BURGERS2D: Burgers2D (Burgers2D.cc:22-92) */
+enum { OR_LAW_SCALAR = 0, OR_LAW_EULER2D = 1, OR_LAW_MAXWELL3D = 2, OR_LAW_MAXWELL2D = 3, OR_LAW_BURGERS2D = 4 };
 /* OR_FLUX_EXACT: Euler2d's own riemannSolver, the exact Godunov flux (EulerLaw.cc:602-887) */
 enum { OR_FLUX_UPWIND = 0, OR_FLUX_LF = 1, OR_FLUX_ROE = 2, OR_FLUX_CENTRAL = 3, OR_FLUX_ROE_EF = 4, OR_FLUX_EXACT = 5 };
 enum { OR_RK_FE = 0, OR_RK_TVD2 = 1, OR_RK_TVD3 = 2, OR_RK4 = 3 };

@@ -9,7 +9,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "liboracle.so")
 
-LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D = 0, 1, 2
+LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D, LAW_MAXWELL2D, LAW_BURGERS2D = 0, 1, 2, 3, 4
 FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL, FLUX_ROE_EF, FLUX_EXACT = 0, 1, 2, 3, 4, 5
 RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
 

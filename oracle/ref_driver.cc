@@ -5,7 +5,8 @@
 // reference's main.cc (main.cc:55-159) and scenario functions (Lilia.cc:84-108):
 //
 //   ref_driver MESH.msh LAW FLUX ORDER OP STATE_IN OUT [DT NSTEPS]
-//     LAW   scalar | euler      FLUX  upwind | exact | roe | lf
+//     LAW   scalar | euler | maxwell2d | burgers
+//     FLUX  upwind | exact | roe | lf (Euler; the other laws have one flux each)
 //     OP    residual | fe | tvd2 | tvd3 | rk4
 //     STATE_IN: one line per cell: 3 vertex ids (any order), then Nf*Np coefficients;
 //          with ORDER 0 each line carries the cell's order after the ids, and the
@@ -15,7 +16,8 @@
 //     OUT: one line per cell in the reference's iteration order: its 3 vertex ids
 //          in the reference's own vertex order, then the residual or new state
 //   scalar velocity (VX VY) and far-field state are read from the environment
-//   (REF_VEL="vx vy", REF_FARFIELD="q0 q1 q2 q3").
+//   (REF_VEL="vx vy", REF_FARFIELD="q0 q1 q2 q3"); time steps start at REF_T0
+//   (default 0; Maxwell2d's RHS holds a pulse source for t <= 0.06).
 //
 // Only the serial code paths run (numThreads = 1): the threaded face assembly
 // races (parallel.cc:46-76), the serial one does not.
@@ -31,7 +33,9 @@
 #include "DGAnalysis.h"
 #include "DGCell.h"
 #include "EulerLaw.h"
+#include "Burgers.h"
 #include "FieldEvaluator.h"
+#include "MaxwellLaw.h"
 #include "ScalarLaw.h"
 #include "TimeIntegrator.h"
 #include "Vertex.h"
@@ -108,13 +112,22 @@ int main(int argc, char** argv) {
   mesh->setPeriodicBC(dim);
   for (int i = 0; i < 3; i++) threadInfo[i].push_back(infoWrapper(mesh->begin(i), mesh->end(i), 0, 0));
 
+  double t0 = getenv("REF_T0") ? atof(getenv("REF_T0")) : 0.0;
   ConservationLaw* law = 0;
-  ConstField vel(g_vel, 3), ff(g_ff, 4);
+  // far-field evaluators of each law's field count (the laws evaluate into
+  // arrays of that size)
+  ConstField vel(g_vel, 3), ff(g_ff, 4), ff3(g_ff, 3), ff1(g_ff, 1);
   set<int> walls;
   walls.insert(40000);
   int nf;
   if (!strcmp(law_s, "scalar")) {
     law = new ScalarLaw(&ff, &vel, &ff);
+    nf = 1;
+  } else if (!strcmp(law_s, "maxwell2d")) {
+    law = new Maxwell2d(walls, &ff3);
+    nf = 3;
+  } else if (!strcmp(law_s, "burgers")) {
+    law = new Burgers2D(&ff1, &ff1);
     nf = 1;
   } else {
     if (!strcmp(flux_s, "roe"))
@@ -192,7 +205,7 @@ int main(int argc, char** argv) {
       ti = new RungeKuttaTVD2(mesh, 0, nf, DOF);
     else
       ti = new ForwardEuler(mesh, 0, nf, DOF);
-    double t = 0.0;
+    double t = t0;
     for (int s = 0; s < nsteps; s++) {
       double r = ti->advanceInTime(t, dt);
       printf("resid %.17e\n", r);

@@ -60,6 +60,8 @@ template <typename CT, typename IT, typename OT, int NW>
 int run(const StageArgs& a, int law) {
   if (law == LAW_SCALAR) return run_law<ScalarLawD, CT, IT, OT, NW>(a);
   if (law == LAW_EULER2D) return run_law<EulerLawD, CT, IT, OT, NW>(a);
+  if (law == LAW_MAXWELL2D) return run_law<Maxwell2dLawD, CT, IT, OT, NW>(a);
+  if (law == LAW_BURGERS2D) return run_law<Burgers2DLawD, CT, IT, OT, NW>(a);
   return -1;
 }
 
@@ -162,6 +164,8 @@ int run_vjp_law(const AdjArgs& a) {
 // nd: flux tangent directions per pass (0: all fields at once)
 int run_vjp(const AdjArgs& a, int law, int nd = 0) {
   if (law == LAW_SCALAR) return run_vjp_law<ScalarLawD, 1>(a);
+  if (law == LAW_MAXWELL2D) return run_vjp_law<Maxwell2dLawD, 1>(a);
+  if (law == LAW_BURGERS2D) return run_vjp_law<Burgers2DLawD, 1>(a);
   if (law == LAW_EULER2D) {
     if (nd == 1) return run_vjp_law<EulerLawD, 1>(a);
     if (nd == 2) return run_vjp_law<EulerLawD, 2>(a);
@@ -185,6 +189,8 @@ int run_jvp_law(const AdjArgs& a) {
 int run_jvp(const AdjArgs& a, int law) {
   if (law == LAW_SCALAR) return run_jvp_law<ScalarLawD>(a);
   if (law == LAW_EULER2D) return run_jvp_law<EulerLawD>(a);
+  if (law == LAW_MAXWELL2D) return run_jvp_law<Maxwell2dLawD>(a);
+  if (law == LAW_BURGERS2D) return run_jvp_law<Burgers2DLawD>(a);
   return -1;
 }
 
@@ -476,6 +482,10 @@ void mix_stage_all(const MixedArgs& a, double& rsum) {
 void mix_stage(const MixedArgs& a, int law, double& rsum) {
   if (law == LAW_SCALAR)
     mix_stage_all<ScalarLawD>(a, rsum);
+  else if (law == LAW_MAXWELL2D)
+    mix_stage_all<Maxwell2dLawD>(a, rsum);
+  else if (law == LAW_BURGERS2D)
+    mix_stage_all<Burgers2DLawD>(a, rsum);
   else
     mix_stage_all<EulerLawD>(a, rsum);
 }

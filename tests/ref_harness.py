@@ -84,7 +84,10 @@ def write_gmsh(mesh, path):
         f.write("$ENDELM\n")
 
 
-def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfield=(0, 0, 0, 0)):
+NFIELDS = {"scalar": 1, "euler": 4, "maxwell2d": 3, "burgers": 1}
+
+
+def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfield=(0, 0, 0, 0), t0=0.0):
     """Returns (values in the caller's element order, reference vertex order per element, resid list).
     order: one order for every cell, or a per-element array (mixed orders; U is then
     the concatenation of the per-element blocks of Nf*Np(p_e) coefficients)."""
@@ -92,7 +95,7 @@ def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfiel
     U = np.asarray(U, np.float64).ravel()
     mixed = not np.isscalar(order)
     orders = np.asarray(order, np.int64) if mixed else np.full(E, int(order))
-    nf = 1 if law == "scalar" else 4
+    nf = NFIELDS[law]
     sizes = nf * (orders + 1) * (orders + 2) // 2
     offs = np.concatenate([[0], np.cumsum(sizes)])
     with tempfile.TemporaryDirectory() as d:
@@ -102,7 +105,8 @@ def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfiel
             for e in range(E):
                 f.write(" ".join(str(v + 1) for v in mesh["lat"][e]) + (" %d" % orders[e] if mixed else "") + " " +
                         " ".join("%.17e" % x for x in U[offs[e]:offs[e + 1]]) + "\n")
-        env = dict(os.environ, REF_VEL="%r %r" % tuple(vel[:2]), REF_FARFIELD=" ".join("%r" % x for x in farfield))
+        env = dict(os.environ, REF_VEL="%r %r" % tuple(vel[:2]), REF_FARFIELD=" ".join("%r" % x for x in farfield),
+                   REF_T0="%r" % t0)
         r = subprocess.run([DRIVER, mp, law, flux, "0" if mixed else str(order), op, sp, op_, "%r" % dt,
                             str(nsteps)], cwd=d, env=env, capture_output=True, text=True, timeout=600)
         if r.returncode != 0:
