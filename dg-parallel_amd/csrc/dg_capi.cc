@@ -758,35 +758,6 @@ static dg_status project_tet(dg_ctx* c, int field, const double* par, double t) 
   return dg_set_state(c, U.data(), U.size());
 }
 
-static dg_status l2_error_tet(dg_ctx* c, int field, const double* par, double t, double* err) {
-  const int P = c->P, NP = c->NP, NF = c->Nf;
-  std::vector<double> U(c->ndof);
-  dg_status s = dg_get_state(c, U.data(), U.size());
-  if (s) return s;
-  auto rule = tet_rule(std::min(2 * (P + 1), 6));
-  std::vector<double> phi(rule.size() * NP);
-  for (size_t q = 0; q < rule.size(); q++) basis(3, P, rule[q].x, &phi[q * NP], nullptr);
-  std::vector<double> acc(NF, 0.0);
-  for (int e = 0; e < c->E; e++) {
-    const double* X = &c->xyz[(size_t)e * 12];
-    const double ad = tet_detj(X);
-    const double* Ue = &U[c->coff[e]];
-    for (size_t q = 0; q < rule.size(); q++) {
-      double x[3], val[8];
-      tet_map(X, rule[q].x, x);
-      if (exact_field(field, par, c->lp, x, t, val)) return fail(c, DG_ERR_ARG, "unknown field %d", field);
-      for (int f = 0; f < NF; f++) {
-        double s2 = 0;
-        for (int k = 0; k < NP; k++) s2 += Ue[f * NP + k] * phi[q * NP + k];
-        const double d = s2 - val[f];
-        acc[f] += d * d * ad * rule[q].w;
-      }
-    }
-  }
-  for (int f = 0; f < NF; f++) err[f] = std::sqrt(acc[f]);
-  return DG_OK;
-}
-
 dg_status dg_project(dg_ctx* c, int field, const double* par, double t) {
   if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
   double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -825,46 +796,65 @@ dg_status dg_project(dg_ctx* c, int field, const double* par, double t) {
   return dg_set_state(c, U.data(), U.size());
 }
 
-dg_status dg_l2_error(dg_ctx* c, int field, const double* par, double t, double* err) {
-  if (!ready(c) || !err) return fail(c, DG_ERR_ARG, "bad argument");
+// Per-field L1, L2 and Linf norms of (state - exact field at t) at the points
+// of the rule of degree 2(p+1) (DGCell::L1exact, densityErrorL2, LinfError:
+// DGError.cc:549-580, 805-831, which take field 0); any output may be NULL
+extern "C++" dg_status ctx_error_norms(dg_ctx* c, int field, const double* par, double t, double* l1, double* l2,
+                                       double* linf) {
   double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (!par) par = zero;
-  if (c->dim == 3) return l2_error_tet(c, field, par, t, err);
-  const int E = c->E, NF = c->Nf;
+  const int E = c->E, NF = c->Nf, dim = c->dim, nv = dim + 1;
   std::vector<double> U(c->ndof);
   dg_status s = dg_get_state(c, U.data(), U.size());
   if (s) return s;
-  // per order: the rule of degree 2(p+1) (DGError.cc:805-831)
+  const int pmax = dim == 2 ? MAXP_TRI : 3;
   std::vector<QPoint> rules[MAXP_TRI + 1];
   std::vector<double> phis[MAXP_TRI + 1];
-  for (int p = 1; p <= MAXP_TRI; p++) {
-    rules[p] = tri_rule(std::min(2 * (p + 1), 10));
-    phis[p].resize(rules[p].size() * np_tri(p));
-    for (size_t q = 0; q < rules[p].size(); q++) basis(2, p, rules[p][q].x, &phis[p][q * np_tri(p)], nullptr);
+  for (int p = 1; p <= pmax; p++) {
+    rules[p] = dim == 2 ? tri_rule(std::min(2 * (p + 1), 10)) : tet_rule(std::min(2 * (p + 1), 6));
+    const int np = dim == 2 ? np_tri(p) : np_tet(p);
+    phis[p].resize(rules[p].size() * np);
+    for (size_t q = 0; q < rules[p].size(); q++) basis(dim, p, rules[p][q].x, &phis[p][q * np], nullptr);
   }
-  std::vector<double> acc(NF, 0.0);
+  std::vector<double> a1(NF, 0.0), a2(NF, 0.0), ai(NF, 0.0);
   for (int e = 0; e < E; e++) {
-    const int P = c->m_order[e], NP = np_tri(P);
+    const int P = c->m_order[e], NP = dim == 2 ? np_tri(P) : np_tet(P);
     const auto& rule = rules[P];
     const auto& phi = phis[P];
-    const double* X = &c->xyz[(size_t)e * 6];
-    double ad = std::fabs((X[2] - X[0]) * (X[5] - X[1]) - (X[4] - X[0]) * (X[3] - X[1]));
+    const double* X = &c->xyz[(size_t)e * nv * dim];
+    const double ad = dim == 2 ? std::fabs((X[2] - X[0]) * (X[5] - X[1]) - (X[4] - X[0]) * (X[3] - X[1])) : tet_detj(X);
     const double* Ue = &U[c->coff[e]];
     for (size_t q = 0; q < rule.size(); q++) {
-      const double u = rule[q].x[0], v = rule[q].x[1];
-      double x[3] = {X[0] + (X[2] - X[0]) * u + (X[4] - X[0]) * v, X[1] + (X[3] - X[1]) * u + (X[5] - X[1]) * v, 0.0};
-      double val[8];
+      double x[3] = {0.0, 0.0, 0.0}, val[8];
+      if (dim == 2) {
+        const double u = rule[q].x[0], v = rule[q].x[1];
+        x[0] = X[0] + (X[2] - X[0]) * u + (X[4] - X[0]) * v;
+        x[1] = X[1] + (X[3] - X[1]) * u + (X[5] - X[1]) * v;
+      } else {
+        tet_map(X, rule[q].x, x);
+      }
       if (exact_field(field, par, c->lp, x, t, val)) return fail(c, DG_ERR_ARG, "unknown field %d", field);
       for (int f = 0; f < NF; f++) {
         double s2 = 0;
         for (int k = 0; k < NP; k++) s2 += Ue[f * NP + k] * phi[q * NP + k];
         const double d = s2 - val[f];
-        acc[f] += d * d * ad * rule[q].w;
+        a1[f] += std::fabs(d) * ad * rule[q].w;
+        a2[f] += d * d * ad * rule[q].w;
+        ai[f] = std::max(ai[f], std::fabs(d));
       }
     }
   }
-  for (int f = 0; f < NF; f++) err[f] = std::sqrt(acc[f]);
+  for (int f = 0; f < NF; f++) {
+    if (l1) l1[f] = a1[f];
+    if (l2) l2[f] = std::sqrt(a2[f]);
+    if (linf) linf[f] = ai[f];
+  }
   return DG_OK;
+}
+
+dg_status dg_l2_error(dg_ctx* c, int field, const double* par, double t, double* err) {
+  if (!ready(c) || !err) return fail(c, DG_ERR_ARG, "bad argument");
+  return ctx_error_norms(c, field, par, t, nullptr, err, nullptr);
 }
 
 // ---------------------------------------------------------------------------

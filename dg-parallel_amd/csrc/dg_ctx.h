@@ -109,3 +109,6 @@ inline bool ready(const dg_ctx* c) { return c && c->stream && c->Un; }
 // new per-element orders (caller numbering) for a triangle context, carrying
 // the state over (raised orders get zero modes, lowered orders drop the top ones)
 dg_status ctx_rebucket(dg_ctx* c, const std::vector<int32_t>& neword);
+
+// per-field L1, L2, Linf of (state - exact field) (outputs may be NULL)
+dg_status ctx_error_norms(dg_ctx* c, int field, const double* par, double t, double* l1, double* l2, double* linf);

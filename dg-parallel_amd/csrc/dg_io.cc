@@ -323,6 +323,51 @@ dg_status dg_mesh_write_gmsh(const dg_mesh_desc* m, const char* path, char* err_
 // ---------------------------------------------------------------------------
 // state export
 
+dg_status dg_norms(dg_ctx* c, double t, int exact_field_id, const double* exact_par, double* l1, double* l2,
+                   double* linf, double* totals, double* energy) {
+  if (!ready(c)) return fail(c, DG_ERR_ARG, "dg_norms: bad context");
+  if (totals || energy) {
+    TotalsArgs a{};
+    a.E = c->E;
+    a.NF = c->Nf;
+    for (int p = 0; p < 6; p++) {
+      a.bstart[p] = c->bstart[p];
+      a.boff[p] = c->boff[p];
+      a.np[p] = p >= 1 && p <= 4 ? np_of(c->dim, p) : 0;
+    }
+    // int phi_0 over the reference simplex of measure 1/2 or 1/6, phi_0 = 1/sqrt(measure)
+    a.phi0_int = c->dim == 2 ? std::sqrt(0.5) : std::sqrt(1.0 / 6.0);
+    a.U = c->Un;
+    a.detj = c->detj;
+    const int nb = (c->E + TOTALS_BLOCK - 1) / TOTALS_BLOCK;
+    const size_t n = (size_t)nb * 2 * c->Nf;
+    HIPCHECK(c, hipSetDevice(c->device));
+    double* dpart = nullptr;
+    if (hipMalloc(&dpart, n * 8) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, DG_ERR_OOM, "dg_norms: out of device memory");
+    }
+    a.part = dpart;
+    std::vector<double> h(n);
+    hipError_t e = launch_totals(a, c->stream) ? hipErrorLaunchFailure : hipSuccess;
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), dpart, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dpart);
+    if (e != hipSuccess) return fail(c, DG_ERR_HIP, "dg_norms: %s", hipGetErrorString(e));
+    for (int j = 0; j < 2 * c->Nf; j++) {
+      double acc = 0.0;
+      for (int b = 0; b < nb; b++) acc += h[(size_t)b * 2 * c->Nf + j];
+      if (j < c->Nf && totals) totals[j] = acc;
+      if (j >= c->Nf && energy) energy[j - c->Nf] = acc;
+    }
+  }
+  if (l1 || l2 || linf) {
+    if (exact_field_id < 0) return fail(c, DG_ERR_ARG, "dg_norms: error norms need an exact field");
+    return ctx_error_norms(c, exact_field_id, exact_par, t, l1, l2, linf);
+  }
+  return DG_OK;
+}
+
 size_t dg_lattice_points(int dim, int nsub) {
   if ((dim != 2 && dim != 3) || nsub < 1 || nsub > SAMPLE_MAX_NSUB) return 0;
   return (size_t)lattice_points(dim, nsub);

@@ -30,4 +30,21 @@ struct SampleArgs {
 
 int launch_sample(const SampleArgs& a, hipStream_t s);
 
+// Per-field totals int_K u_f = |J| U_f0 int phi_0 (the other modes integrate to
+// zero) and energies int_K u_f^2 = |J| sum_k U_fk^2 (orthonormal modes), summed
+// over the elements of each block of TOTALS_BLOCK; part[block][2 NF] (totals,
+// then energies), added by the host in block order.
+constexpr int TOTALS_BLOCK = 256;
+struct TotalsArgs {
+  int E, NF;
+  int bstart[6];
+  long boff[6];
+  int np[6];
+  double phi0_int;      // int over the reference element of phi_0
+  const double* U;
+  const double* detj;   // |J| per internal element
+  double* part;
+};
+int launch_totals(const TotalsArgs& a, hipStream_t s);
+
 }  // namespace dgp

@@ -49,6 +49,52 @@ __global__ void __launch_bounds__(256) sample_lattice(SampleArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(TOTALS_BLOCK) state_totals(TotalsArgs a) {
+  __shared__ double red[TOTALS_BLOCK];
+  const int i = blockIdx.x * TOTALS_BLOCK + threadIdx.x;
+  double tot[6] = {0, 0, 0, 0, 0, 0}, en[6] = {0, 0, 0, 0, 0, 0};
+  if (i < a.E) {
+    int p = 1;
+#pragma unroll
+    for (int q = 2; q <= 4; q++)
+      if (i >= a.bstart[q]) p = q;
+    const int Ep = a.bstart[p + 1] - a.bstart[p], np = a.np[p];
+    const double* U = a.U + a.boff[p] + (i - a.bstart[p]);
+    const double dj = a.detj[i];
+    for (int f = 0; f < a.NF && f < 6; f++) {
+      double s2 = 0.0;
+      for (int k = 0; k < np; k++) {
+        const double v = U[(size_t)(f * np + k) * Ep];
+        s2 = fma(v, v, s2);
+      }
+      tot[f] = dj * a.phi0_int * U[(size_t)(f * np) * Ep];
+      en[f] = dj * s2;
+    }
+  }
+  // one fixed-order tree per quantity
+  for (int j = 0; j < 2 * a.NF; j++) {
+    const int f = j < a.NF ? j : j - a.NF;
+    double v = 0.0;
+#pragma unroll
+    for (int g = 0; g < 6; g++)
+      if (g == f) v = j < a.NF ? tot[g] : en[g];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = TOTALS_BLOCK / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.part[(size_t)blockIdx.x * 2 * a.NF + j] = red[0];
+    __syncthreads();
+  }
+}
+
+int launch_totals(const TotalsArgs& a, hipStream_t s) {
+  if (a.E <= 0 || a.NF < 1 || a.NF > 6) return -1;
+  hipLaunchKernelGGL(state_totals, dim3((a.E + TOTALS_BLOCK - 1) / TOTALS_BLOCK), dim3(TOTALS_BLOCK), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_sample(const SampleArgs& a, hipStream_t s) {
   if (a.E <= 0) return 0;
   for (int p = 1; p <= 4; p++) {

@@ -106,6 +106,8 @@ _write_restart = _sig("dg_write_restart", ctypes.c_int, [_vp, ctypes.c_char_p, c
 _read_restart = _sig("dg_read_restart", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                                        ctypes.POINTER(ctypes.c_int)])
 _sample_lattice = _sig("dg_sample_lattice", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_size_t])
+_norms = _sig("dg_norms", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _p_f64, _p_f64, _p_f64, _p_f64, _p_f64,
+                                         _p_f64])
 _export_pos = _sig("dg_export_pos", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                    ctypes.c_int, _p_f64])
 _jvp = _sig("dg_residual_jvp", ctypes.c_int, [_vp, _p_f64, _p_f64, _p_f64])
@@ -462,6 +464,19 @@ class DGAnalysis:
     def orders(self):
         out = np.zeros(self.mesh_E, np.int32)
         self._check(_get_orders(self._h, _ptr(out, _p_i32)))
+        return out
+
+    def norms(self, t=0.0, exact_field=-1, par=()):
+        """dg_norms: per-field totals (int u) and energies (int u^2), and with an
+        exact field the L1, L2 and Linf error norms; a dict of arrays"""
+        nf = self.law.nfields
+        out = {k: np.zeros(nf) for k in ("totals", "energy")}
+        if exact_field >= 0:
+            out.update({k: np.zeros(nf) for k in ("l1", "l2", "linf")})
+        pv = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)
+        g = lambda k: _ptr(out[k], _p_f64) if k in out else None  # noqa: E731
+        self._check(_norms(self._h, t, exact_field, _ptr(pv, _p_f64), g("l1"), g("l2"), g("linf"), g("totals"),
+                           g("energy")))
         return out
 
     # ---- checkpoint / restart and state export ----

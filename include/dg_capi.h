@@ -316,6 +316,18 @@ dg_status dg_write_restart(dg_ctx* ctx, const char* path, double t, int iter);
  * otherwise they must equal the context's.  Face records are not needed. */
 dg_status dg_read_restart(dg_ctx* ctx, const char* path, double* t_out, int* iter_out);
 
+/* Norms and conserved totals (the sums of DGPref.cc:249-394 and the total mass
+ * of DGAnalysis.cc:554, DGCell::computeMass, DGError.cc:833-845), per field:
+ *   totals[f] = sum_e int_K u_f and energy[f] = sum_e int_K u_f^2 (device
+ *   reduction over the state, summed in a fixed order);
+ *   l1, l2, linf of (u_f - exact_f) at t against a built-in exact field
+ *   (DG_FIELD_*, parameters exact_par; DGCell::L1exact, densityErrorL2,
+ *   LinfError, DGError.cc:549-580, 805-831, which take field 0), at the points
+ *   of the rule of degree 2(p+1).
+ * Every output (Nf doubles) may be NULL; error norms need exact_field >= 0. */
+dg_status dg_norms(dg_ctx* ctx, double t, int exact_field, const double* exact_par, double* l1, double* l2,
+                   double* linf, double* totals, double* energy);
+
 /* ---- state export (DGAnalysis::exportGmsh, DGSensor.cc:337-525, and
  * exportGmsh_Volume, :800-) ----------------------------------------------------
  * Lattice of a reference simplex split nsub times: points (i, j[, k]) / nsub

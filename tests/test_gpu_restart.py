@@ -18,6 +18,7 @@ dg_export_pos: DGAnalysis::exportGmsh, DGSensor.cc:337-525, 800-).
 - the .pos views carry the sampled fields at the corners of the plot
   sub-simplices: density, pressure, velocity and Mach from the sampled
   conserved values, E*nsub^2 triangles or E*8 tetrahedra per view."""
+import ctypes
 import os
 import re
 import sys
@@ -304,3 +305,52 @@ def test_pos_views_scalar_and_tetrahedra(tmp_path):
     assert np.abs(E[0][2].reshape(4, 3) - S[0][:3, [0, 1, 3, 6]].T).max() <= 1e-9 * max(1.0, np.abs(S).max())
     e = 0.5 * (S[0][:, [0, 1, 3, 6]] ** 2).sum(axis=0)
     assert np.abs(en[0][2] - e).max() <= 1e-9
+
+
+def test_norms_totals_and_errors():
+    """dg_norms: totals and energies against closed forms and invariants, error
+    norms against closed forms of the Gaussian pulse"""
+    import dgpar
+    announce("norms and totals")
+    # totals of a projected constant: c * area; energy: c^2 * area
+    vids, xyz, bc = mesh_tri(8, 8, lx=2.0, ly=3.0, jitter=0.1, seed=3)
+    g = dgpar.DGAnalysis(dgpar.Mesh(2, vids, xyz, bc), dgpar.EulerLaw(dgpar.FLUX_LF, farfield=(1.0, 1.0, 1.0, 3.5)), 3)
+    U = np.zeros(g.ndof).reshape(-1, 4, 10)
+    c = np.array([1.3, -0.2, 0.7, 2.9])
+    U[:, :, 0] = c / np.sqrt(2.0)
+    g.set_state(U.ravel())
+    n = g.norms()
+    assert np.abs(n["totals"] - 6.0 * c).max() <= 1e-13 * 6.0 * np.abs(c).max()
+    assert np.abs(n["energy"] - 6.0 * c * c).max() <= 1e-13 * 6.0 * (c * c).max()
+    # the energy of a general state is its L2 norm squared, measured with an
+    # independent collapsed Gauss rule on the oracle's basis
+    m, h = vortex(p=2, n=4)
+    U = h.get_state().reshape(-1, 4, 6)
+    from helpers import collapsed_rule
+    pts, wts = collapsed_rule(5)
+    B = np.array([po.basis(2, 2, x)[0] for x in pts])  # [points][modes]
+    X = m.xyz
+    dj = np.abs((X[:, 1, 0] - X[:, 0, 0]) * (X[:, 2, 1] - X[:, 0, 1]) - (X[:, 2, 0] - X[:, 0, 0]) * (X[:, 1, 1] - X[:, 0, 1]))
+    vals = np.einsum("efk,qk->efq", U, B)
+    ref = np.einsum("efq,q,e->f", vals ** 2, wts, dj)
+    assert np.abs(h.norms()["energy"] - ref).max() <= 1e-12 * ref.max()
+    # conservation: periodic mesh, totals unchanged by RK4 steps
+    t0 = h.norms()["totals"]
+    dt = h.computeDt(0.1)
+    h.advanceInTime(0.0, dt, scheme=dgpar.RK4, nsteps=10)
+    assert np.abs(h.norms()["totals"] - t0).max() <= 1e-12 * np.abs(t0).max()
+    # error norms of the zero state against the Gaussian pulse exp(-r^2/0.01) on
+    # the unit square: int g = 0.01 pi, int g^2 = 0.005 pi (tails ~e^-25)
+    vids, xyz, bc = mesh_tri(24, 24)
+    s = dgpar.DGAnalysis(dgpar.Mesh(2, vids, xyz, bc), dgpar.ScalarLaw((1.0, 0.5)), 4)
+    s.set_state(np.zeros(s.ndof))
+    e = s.norms(0.0, dgpar.FIELD_GAUSSIAN)
+    assert abs(e["l1"][0] - 0.01 * np.pi) <= 1e-5
+    assert abs(e["l2"][0] - np.sqrt(0.005 * np.pi)) <= 1e-5
+    assert 0.9 < e["linf"][0] <= 1.0
+    s.project(dgpar.FIELD_GAUSSIAN)
+    e = s.norms(0.0, dgpar.FIELD_GAUSSIAN)
+    assert e["l2"][0] == s.l2_error(dgpar.FIELD_GAUSSIAN)[0]
+    assert e["l1"][0] <= e["l2"][0] and e["l2"][0] <= e["linf"][0]  # unit square: |.|_1 <= |.|_2 <= |.|_inf
+    with pytest.raises(dgpar.DGError, match="exact field"):
+        s._check(dgpar._norms(s._h, 0.0, -1, None, (ctypes.c_double * 1)(), None, None, None, None))
