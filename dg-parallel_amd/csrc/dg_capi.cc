@@ -1283,9 +1283,13 @@ dg_status dg_adjoint_timing(const dg_ctx* c, float* fwd_ms, float* bwd_ms) {
 // ---------------------------------------------------------------------------
 // launch schedules and the tuner
 
+// triangle tile shapes: 0 (4 waves), 1 (8 waves), and 2 (4 waves with the
+// contractions on the matrix cores) for the FP32 form of the Euler law
+static int tri_shapes(const dg_ctx* c) { return c->law == DG_LAW_EULER2D && c->prec == DG_PREC_FP32 ? 3 : 2; }
+
 static bool schedule_ok(const dg_ctx* c, const dg_schedule& s) {
   const bool tiled = c->dim == 3 || (!c->mixed && c->kernel == 0);
-  const int nshape = c->dim == 3 ? TET_SHAPES : 2;
+  const int nshape = c->dim == 3 ? TET_SHAPES : tri_shapes(c);
   if (s.shape < 0 || s.shape >= nshape || (s.shape && !tiled)) return false;
   if (s.grid < 0 || (s.grid && !c->mixed)) return false;
   if (s.grid && ((size_t)s.grid + 1 > c->part_cap || s.grid > c->nchunks)) return false;
@@ -1333,7 +1337,7 @@ static std::vector<dg_schedule> tune_candidates(const dg_ctx* c) {
       if (schedule_ok(c, s)) v.push_back(s);
     }
   } else {
-    for (int sh = 0; sh < (c->dim == 3 ? TET_SHAPES : 2); sh++) {
+    for (int sh = 0; sh < (c->dim == 3 ? TET_SHAPES : tri_shapes(c)); sh++) {
       dg_schedule s{};
       s.shape = sh;
       if (schedule_ok(c, s)) v.push_back(s);

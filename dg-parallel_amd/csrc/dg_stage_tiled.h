@@ -22,6 +22,7 @@
 // Each face flux is computed by both of its elements (no face buffer, no
 // atomics; results independent of scheduling).
 #pragma once
+#include <type_traits>
 
 #include "dg_kernels.h"
 #include "dg_laws.h"
@@ -106,10 +107,31 @@ DG_HD double stage_value_hd(int sc, double v, double un, double us, double* acc,
   return o;
 }
 
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+// D += A B for one wave: v_mfma_f32_16x16x4_f32.  Lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] and holds D[4 (l / 16) + r][l % 16], r = 0..3.
+DG_HD mfma_f4 mfma_16x16x4(float a, float b, mfma_f4 c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+#else
+  // the MFMA tile shape runs on the device only (no host instantiation calls it)
+  __builtin_trap();
+  return c;
+#endif
+}
+
+struct MfmaAccOn {
+  mfma_f4 a[4];  // column blocks of 16 elements of the wave's field
+};
+struct MfmaAccOff {};
+
 // NW waves share a tile of 64 elements (NW = 4: 256 threads; NW = 8: 512
 // threads, the same LDS and half the rows per thread); the tile-shape choice of
-// the tuner.
-template <int P, class LAW, typename CT, typename IT, typename OT, int NW = 4>
+// the tuner.  MF (FP32 form, four fields, four waves): the test contraction
+// (D1) and the face lift (D2) run on the matrix cores, each wave one field,
+// rows = modes and columns = the tile's elements in four blocks of 16.
+template <int P, class LAW, typename CT, typename IT, typename OT, int NW = 4, bool MF = false>
 struct TiledStage {
   static constexpr int NP = np_tri(P), NQ = nqv_tri(P), NQF = nqf_tri(P), NF = LAW::NF;
   static constexpr int T = TILE_ELEMS;
@@ -148,11 +170,13 @@ struct TiledStage {
   // per-thread output rows of its own element, and the neighbour coefficients of
   // its face tasks, loaded in phase A so their latency hides behind B and D1
   // (a workgroup barrier waits for LDS and scalar loads only)
+  static_assert(!MF || (NF == 4 && NW == 4 && sizeof(CT) == 4 && NP <= 16), "MFMA tile: FP32, 4 fields, 4 waves");
   struct Regs {
     CT R[FPW * KPW];
     double rsum;
     CT nv[CPW][NP];
     int lk[CPW];
+    typename std::conditional<MF, MfmaAccOn, MfmaAccOff>::type m;
   };
 
   DG_HD static const CT* table(const StageArgs& a) {
@@ -178,6 +202,10 @@ struct TiledStage {
     // then C1's neighbour rows.  The LDS writes come last.
 #pragma unroll
     for (int i = 0; i < FPW * KPW; i++) g.R[i] = CT(0);
+    if constexpr (MF) {
+#pragma unroll
+      for (int cb = 0; cb < 4; cb++) g.m.a[cb] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    }
     int lk3[3];
 #pragma unroll
     for (int l = 0; l < 3; l++) lk3[l] = a.el.link[(long)l * E + es];
@@ -267,6 +295,24 @@ struct TiledStage {
   DG_HD static void phaseD1(const StageArgs& a, Smem& s, int tid, Regs& g, int q0) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const CT* tb = table(a);
+    if constexpr (MF) {
+      // R[i][e] += sum over (point, direction) kk of dphi[q][i][d] G[q][f][d][e],
+      // four kk per MFMA step; lane: column (element) l % 16, step row l / 16
+      constexpr int KK = 2 * QH, KS = (KK + 3) / 4;
+      const int col = t & 15, kr = t >> 4, f = wave;
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int kk = 4 * ks + kr, qi = kk >> 1, d = kk & 1, q = q0 + qi;
+        const bool ok = kk < KK && q < NQ;
+        const float A = ok && col < NP ? tb[TriBlock::off_dphi(P) + (q * NP + col) * 2 + d] : 0.f;
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+          const float B = ok ? s.W[(qi * NF + f) * 2 + d][16 * cb + col] : 0.f;
+          g.m.a[cb] = mfma_16x16x4(A, B, g.m.a[cb]);
+        }
+      }
+      return;
+    }
     const int q1 = q0 + QH < NQ ? q0 + QH : NQ;
 #pragma unroll
     for (int fi = 0; fi < FPW; fi++) {
@@ -377,6 +423,41 @@ struct TiledStage {
     }
   }
 
+  // ---- MF: the face lift on the matrix cores, then the rows back to their
+  //      elements through LDS (after a barrier: W holds the face fluxes) ------
+  DG_HD static void phaseLiftMF(const StageArgs& a, const Smem& s, int tid, Regs& g) {
+    if constexpr (MF) {
+      const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+      const CT* tb = table(a);
+      constexpr int KK = 3 * NQF, KS = (KK + 3) / 4;
+      const int col = t & 15, kr = t >> 4, f = wave;
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int kk = 4 * ks + kr;
+        const bool ok = kk < KK;
+        const float A = ok && col < NP ? -tb[TriBlock::off_phif(P) + kk * NP + col] : 0.f;
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+          const float B = ok ? s.W[kk * NF + f][16 * cb + col] : 0.f;
+          g.m.a[cb] = mfma_16x16x4(A, B, g.m.a[cb]);
+        }
+      }
+    }
+  }
+  DG_HD static void phaseStoreMF(Smem& s, int tid, const Regs& g) {
+    if constexpr (MF) {
+      const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
+      const int col = t & 15, kr = t >> 4, f = wave;
+#pragma unroll
+      for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = 4 * kr + r;
+          if (i < NP) s.W[f * NP + i][16 * cb + col] = g.m.a[cb][r];
+        }
+    }
+  }
+
   // ---- D2 + stage update ----------------------------------------------------
   DG_HD static void phaseD2(const StageArgs& a, Smem& s, int tid, int tile, Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
@@ -405,15 +486,24 @@ struct TiledStage {
           ac[ki] = reads_acc ? a.acc[i] : 0.0;
           us[ki] = reads_input ? double(Uin[i]) : 0.0;
         }
-        // one face's points per unrolled block: unrolling all 3*NQF points hoists
-        // every table row into SGPRs and spills them to VGPR lanes
-#pragma unroll NQF
-        for (int lq = 0; lq < 3 * NQF; lq++) {
-          const CT Fq = s.W[lq * NF + f][t];
+        if constexpr (MF) {
+          // the rows of volume and face terms, stored by phaseStoreMF
 #pragma unroll
           for (int ki = 0; ki < KPW; ki++) {
             const int k = row_k(wave, ki);
-            if (k < NP) g.R[fi * KPW + ki] -= tb[TriBlock::off_phif(P) + lq * NP + k] * Fq;
+            if (k < NP) g.R[fi * KPW + ki] = s.W[f * NP + k][t];
+          }
+        } else {
+          // one face's points per unrolled block: unrolling all 3*NQF points hoists
+          // every table row into SGPRs and spills them to VGPR lanes
+#pragma unroll NQF
+          for (int lq = 0; lq < 3 * NQF; lq++) {
+            const CT Fq = s.W[lq * NF + f][t];
+#pragma unroll
+            for (int ki = 0; ki < KPW; ki++) {
+              const int k = row_k(wave, ki);
+              if (k < NP) g.R[fi * KPW + ki] -= tb[TriBlock::off_phif(P) + lq * NP + k] * Fq;
+            }
           }
         }
 #pragma unroll

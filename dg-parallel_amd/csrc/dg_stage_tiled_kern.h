@@ -17,9 +17,9 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return (b % 8) * (nb / 8) + b / 8;
 }
 
-template <int P, class LAW, typename CT, typename IT, typename OT, int NW>
+template <int P, class LAW, typename CT, typename IT, typename OT, int NW, bool MF = false>
 __global__ __launch_bounds__(NW * 64) void tri_stage_tiled(StageArgs a) {
-  using K = TiledStage<P, LAW, CT, IT, OT, NW>;
+  using K = TiledStage<P, LAW, CT, IT, OT, NW, MF>;
   __shared__ typename K::Smem sm;
   typename K::Regs g;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -37,6 +37,12 @@ __global__ __launch_bounds__(NW * 64) void tri_stage_tiled(StageArgs a) {
   __syncthreads();
   K::phaseC2(a, sm, tid, g);
   __syncthreads();
+  if constexpr (MF) {
+    K::phaseLiftMF(a, sm, tid, g);
+    __syncthreads();  // every wave has read the face fluxes
+    K::phaseStoreMF(sm, tid, g);
+    __syncthreads();
+  }
   K::phaseD2(a, sm, tid, tile, g);
   if (a.resid_part) {
     // deterministic tree reduction of the step residual norm (red overlays the
@@ -85,6 +91,37 @@ static int launch_tiled_law(const StageArgs& a, int prec, hipStream_t s) {
   return -1;
 }
 
+// the MFMA tile shape: FP32 forms (prec 1..3) of the four-field law
+template <int P, class LAW>
+static int launch_tiled_mf_p(const StageArgs& a, int prec, hipStream_t s) {
+  dim3 grid((a.E + TILE_ELEMS - 1) / TILE_ELEMS), block(4 * 64);
+  switch (prec) {
+    case 1:
+      hipLaunchKernelGGL((tri_stage_tiled<P, LAW, float, double, float, 4, true>), grid, block, 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((tri_stage_tiled<P, LAW, float, float, float, 4, true>), grid, block, 0, s, a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((tri_stage_tiled<P, LAW, float, float, double, 4, true>), grid, block, 0, s, a);
+      break;
+    default:
+      return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <class LAW>
+static int launch_tiled_mf_law(const StageArgs& a, int prec, hipStream_t s) {
+  switch (a.P) {
+    case 1: return launch_tiled_mf_p<1, LAW>(a, prec, s);
+    case 2: return launch_tiled_mf_p<2, LAW>(a, prec, s);
+    case 3: return launch_tiled_mf_p<3, LAW>(a, prec, s);
+    case 4: return launch_tiled_mf_p<4, LAW>(a, prec, s);
+  }
+  return -1;
+}
+
 template <int NW>
 int launch_tiled_nw(const StageArgs& a, int law, int prec, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_tiled_law<ScalarLawD, NW>(a, prec, s);
@@ -96,6 +133,8 @@ int launch_tiled_nw(const StageArgs& a, int law, int prec, hipStream_t s) {
 }
 
 extern template int launch_tiled_nw<4>(const StageArgs&, int, int, hipStream_t);
+// the MFMA tile shape (dg_stage_tiled_mf.hip): the Euler law's FP32 forms
+int launch_tiled_mfma(const StageArgs& a, int law, int prec, hipStream_t s);
 extern template int launch_tiled_nw<8>(const StageArgs&, int, int, hipStream_t);
 
 }  // namespace dgp

@@ -199,12 +199,17 @@ dg_status dg_adjoint_timing(const dg_ctx* ctx, float* fwd_ms, float* bwd_ms);
 
 /* ---- launch schedules and the tuner (no counterpart in the reference, whose
  * pthreads split is fixed by parallel.cc) ----------------------------------
- * A schedule is how the context's stage kernel is launched; it changes no
- * state (every candidate computes each coefficient with the same operations in
- * the same order); the step residual norm of dg_advance, a sum over workgroups
- * and threads, follows the launch to rounding.
+ * A schedule is how the context's stage kernel is launched; except for the
+ * matrix-core shape below it changes no state (every candidate computes each
+ * coefficient with the same operations in the same order); the step residual
+ * norm of dg_advance, a sum over workgroups and threads, follows the launch to
+ * rounding.
  *   shape: the tiled triangle and tetrahedron kernels, 0 = 4 waves per tile of
  *          64 elements (256 threads), 1 = 8 waves (512 threads, same LDS);
+ *          2 (the FP32 form of the Euler law) = 4 waves with the test
+ *          contraction and the face lift on the matrix cores
+ *          (v_mfma_f32_16x16x4_f32): its sums run in the matrix cores' order,
+ *          so the state follows shape 0 to FP32 rounding, not bit for bit;
  *   grid:  the persistent mixed-order launch, workgroups walking the chunk list
  *          (0: one per chunk, at most 2048). */
 typedef struct {

@@ -197,3 +197,38 @@ def test_bad_schedules_refused():
     with pytest.raises(dgpar.DGError):
         gm.set_schedule(grid=10 ** 6)     # more workgroups than chunks
     assert g.schedule() == (0, 0) and gm.schedule() == (0, 0)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_tri_mfma_tiles_fp32_form(p):
+    """shape 2 (FP32 form of the Euler law): the test contraction and the face
+    lift on the matrix cores; the residual follows the FP64 oracle to the FP32
+    form's budget and shape 0 to FP32 rounding, and RK4 steps follow shape 0"""
+    import dgpar
+    announce("mfma tiles P%d" % p)
+    o, g = tri_pair(p, n=9, precision=dgpar.PREC_FP32)  # 162 elements: a partial last tile
+    U = smooth_euler(o, p, o.ndof // (4 * np_tri(p)), 13)
+    g.set_state(U)
+    R0 = g.residual()
+    g.set_schedule(shape=2)
+    assert g.schedule() == (2, 0)
+    R2 = g.residual()
+    Ro = o.residual(U)[0]
+    assert np.abs(R2 - Ro).max() <= 5e-5 * np.abs(Ro).max()
+    assert np.abs(R2 - R0).max() <= 2e-6 * np.abs(Ro).max()
+    dt = o.dt(U, 0.1)
+    U0, r0 = steps_under(g, U, dt, po.RK4, 5, shape=0)
+    U2, r2 = steps_under(g, U, dt, po.RK4, 5, shape=2)
+    assert np.abs(U2 - U0).max() <= 1e-6 * np.abs(U0).max()
+    assert abs(r2 - r0) <= 1e-4 * abs(r0)
+
+
+def test_autotune_fp32_offers_the_mfma_shape():
+    import dgpar
+    announce("autotune fp32 candidates")
+    o, g = tri_pair(3, precision=dgpar.PREC_FP32)
+    U = smooth_euler(o, 3, o.ndof // (4 * np_tri(3)), 9)
+    g.set_state(U)
+    rep = g.autotune(po.RK4, o.dt(U, 0.1), steps=3)
+    assert [c[:2] for c in rep["candidates"]] == [(0, 0), (1, 0), (2, 0)]
+    assert np.array_equal(g.get_state(), U)
