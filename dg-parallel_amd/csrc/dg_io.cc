@@ -502,6 +502,18 @@ dg_status dg_export_pos(dg_ctx* c, const char* prefix, int iteration, int nsub, 
     views.push_back({"numEr", 1});
     views.push_back({"function", 1});
     views.push_back({"limit", 1});
+  } else if (c->law == DG_LAW_MAXWELL2D) {
+    // Maxwell::getNameAndSize / Maxwell2d::getIthFieldOfInterest
+    // (MaxwellLaw.cc:256-330): Ez^2, Hx, Hy, Ez (its fifth view is the order)
+    views = {{"Intensity_RefCond", 1}, {"HxField_RefCond", 1}, {"HyField_RefCond", 1}, {"EField_RefCond", 1}};
+  } else if (c->law == DG_LAW_BURGERS2D) {
+    // Burgers2D::getNameAndSize / getIthFieldOfInterest (Burgers2D.cc:121-200)
+    views = {{"function", 1}};
+    if (have_exact) {
+      views.push_back({"error", 1});
+      views.push_back({"exact", 1});
+    }
+    views.push_back({"numEr", 1});
   } else {
     // 3D Maxwell: the field vectors and the energy density (no 3D counterpart
     // in the reference, whose 2D list is MaxwellLaw.cc:256-330)
@@ -534,10 +546,15 @@ dg_status dg_export_pos(dg_ctx* c, const char* prefix, int iteration, int nsub, 
         const double pff = (g - 1.0) * (ff[3] - 0.5 * (ff[1] * ff[1] + ff[2] * ff[2]) / ff[0]);
         r[0] = p / pff / std::pow(u[0] / ff[0], g) - 1.0;
       }
-    } else if (c->law == DG_LAW_SCALAR) {
+    } else if (c->law == DG_LAW_SCALAR || c->law == DG_LAW_BURGERS2D) {
       if (nm == "error") r[0] = exact[0] - u[0];
       else if (nm == "exact") r[0] = exact[0];
       else r[0] = u[0];
+    } else if (c->law == DG_LAW_MAXWELL2D) {
+      if (nm == "Intensity_RefCond") r[0] = u[2] * u[2];
+      else if (nm == "HxField_RefCond") r[0] = u[0];
+      else if (nm == "HyField_RefCond") r[0] = u[1];
+      else r[0] = u[2];
     } else {
       if (nm == "E") r[0] = u[0], r[1] = u[1], r[2] = u[2];
       else if (nm == "H") r[0] = u[3], r[1] = u[4], r[2] = u[5];

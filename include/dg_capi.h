@@ -349,6 +349,10 @@ dg_status dg_sample_lattice(dg_ctx* ctx, int nsub, double* out, size_t n);
  *            Euler2d::getIthFieldOfInterest, EulerLaw.cc:1240-1340);
  *   Scalar:  error and exact (with an exact field), numEr, function, limit
  *            (ScalarLaw.cc:111-165);
+ *   Maxwell2d: Intensity_RefCond (Ez^2), HxField_RefCond, HyField_RefCond,
+ *            EField_RefCond (MaxwellLaw.cc:256-330);
+ *   Burgers2D: function, error and exact (with an exact field), numEr
+ *            (Burgers2D.cc:121-200);
  *   Maxwell (3D): E and H vectors, energy density, error |E - E_exact| (with an
  *            exact field);
  *   all:     order.

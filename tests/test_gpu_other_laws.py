@@ -121,3 +121,24 @@ def test_limiter_refused():
     g, _, _ = pair("maxwell2d", 2)
     with pytest.raises(dgpar.DGError, match="limiter"):
         g.set_limiter(True)
+
+
+def test_pos_views(tmp_path):
+    """the .pos views of both laws carry the sampled fields (Maxwell2d: Ez^2,
+    Hx, Hy, Ez, MaxwellLaw.cc:256-330; Burgers2D: the function, Burgers2D.cc:121-200)"""
+    announce("pos views of the other laws")
+    g, o, U = pair("maxwell2d", 2)
+    g.set_state(U)
+    g.export_pos(str(tmp_path / "m"), 1, nsub=1)
+    S = g.sample_lattice(1)  # the corners, one sub-triangle per element
+    for name, val in (("Intensity_RefCond", S[:, 2] ** 2), ("HxField_RefCond", S[:, 0]),
+                      ("HyField_RefCond", S[:, 1]), ("EField_RefCond", S[:, 2])):
+        lines = open(tmp_path / ("m-sample1%s.pos" % name)).read().splitlines()[1:-1]
+        got = np.array([[float(x) for x in ln.split("{")[1].rstrip("};").split(",")] for ln in lines])
+        assert got.shape == val.shape and np.abs(got - val).max() <= 1e-8 * max(1.0, np.abs(val).max())
+    b, _, Ub = pair("burgers", 2)
+    b.set_state(Ub)
+    b.export_pos(str(tmp_path / "b"), 0, nsub=1)
+    lines = open(tmp_path / "b-sample0function.pos").read().splitlines()[1:-1]
+    got = np.array([[float(x) for x in ln.split("{")[1].rstrip("};").split(",")] for ln in lines])
+    assert np.abs(got - b.sample_lattice(1)[:, 0]).max() <= 1e-8
