@@ -11,9 +11,11 @@ exact vortex and against each other.  The discrete adjoint is timed on the
 128x128x2 vortex mesh: 200 RK4 steps forward with a checkpoint every 20 steps,
 then the backward sweep (segment recomputation + fused reverse stages) for the
 gradient of the density misfit against the solution started from the vortex
-shifted by (0.5, 0.3); the ratio backward/forward is reported.  The CPU baseline is the FP64 CPU oracle
-(a restatement of the reference algorithm) run on a bounded sample on this
-host's cores.
+shifted by (0.5, 0.3); the ratio backward/forward is reported.  The CPU
+baseline is the reference's own serial code (its DGAnalysis and TimeIntegrator,
+built from its sources into oracle/_ref) running RK4 steps on the same mesh;
+the FP64 CPU oracle (a restatement of the reference algorithm) on a bounded
+sample on this host's cores is reported beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -207,6 +209,78 @@ def run_maxwell(dgpar, steps, n=48, tune=True):
             "l2_error_E_H": [float(x) for x in err], "schedule": sched}
 
 
+REF_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "ref_driver")
+
+
+def reference_cpu_baseline(mesh, U, dt, nsteps=1):
+    """The reference's own serial code (oracle/_ref/ref_driver, built from the
+    reference's sources with its Euler2d law and a local Lax-Friedrichs
+    riemannSolver) on the full mesh: nsteps RK4 steps, timed by the driver
+    around its step loop (mesh import and DGAnalysis construction excluded).
+    The mesh goes over as a Gmsh file: the (NX+1)^2 lattice nodes, the
+    triangles in the generator's order, and 510 / 610 lines on the periodic
+    sides.  None when the reference build is absent or fails."""
+    if not os.path.exists(REF_DRIVER):
+        return None
+    import subprocess
+    import tempfile
+    n, N1 = NX, NX + 1
+    X = mesh.xyz.reshape(n, n, 2, 3, 2)  # [j][i][triangle][corner][x, y]
+    c = np.zeros((N1, N1, 2))
+    c[:n, :n] = X[:, :, 0, 0]
+    c[:n, n] = X[:, n - 1, 0, 1]
+    c[n, :n] = X[n - 1, :, 1, 2]
+    c[n, n] = X[n - 1, n - 1, 0, 2]
+    # the reference joins periodic sides through the reflected partner point
+    # (DGCell.cc:557-577, 687-693), which is the translation image only on a
+    # domain centred at the origin: the mesh goes over shifted to [-5, 5]^2
+    # (a translation leaves every element's modal state and residual unchanged)
+    c -= 0.5 * (c.reshape(-1, 2).min(axis=0) + c.reshape(-1, 2).max(axis=0))
+    nid = lambda i, j: j * N1 + i + 1  # noqa: E731
+    with tempfile.TemporaryDirectory() as d:
+        mp, sp, op_ = os.path.join(d, "m.msh"), os.path.join(d, "s.txt"), os.path.join(d, "o.txt")
+        with open(mp, "w") as f:
+            f.write("$NOD\n%d\n" % (N1 * N1))
+            for j in range(N1):
+                for i in range(N1):
+                    f.write("%d %.17e %.17e 0\n" % (nid(i, j), c[j, i, 0], c[j, i, 1]))
+            recs = []
+            for j in range(n):
+                recs += [(510, nid(0, j), nid(0, j + 1)), (510, nid(n, j), nid(n, j + 1))]
+            for i in range(n):
+                recs += [(610, nid(i, 0), nid(i + 1, 0)), (610, nid(i, n), nid(i + 1, n))]
+            f.write("$ENDNOD\n$ELM\n%d\n" % (len(recs) + 2 * n * n))
+            k = 1
+            for tag, a, b in recs:
+                f.write("%d 1 %d 1 2 %d %d\n" % (k, tag, a, b))
+                k += 1
+            tris = []
+            for j in range(n):
+                for i in range(n):
+                    tris.append((nid(i, j), nid(i + 1, j), nid(i + 1, j + 1)))
+                    tris.append((nid(i, j), nid(i + 1, j + 1), nid(i, j + 1)))
+            for t in tris:
+                f.write("%d 2 100 1 3 %d %d %d\n" % (k, t[0], t[1], t[2]))
+                k += 1
+            f.write("$ENDELM\n")
+        Ue = np.asarray(U).reshape(len(tris), -1)
+        with open(sp, "w") as f:
+            for t, row in zip(tris, Ue):
+                f.write("%d %d %d " % t + " ".join("%.17e" % x for x in row) + "\n")
+        env = dict(os.environ, REF_FARFIELD="1 1 1 3.5")
+        r = subprocess.run([REF_DRIVER, mp, "euler", "lf", str(ORDER), "rk4", sp, op_, "%r" % dt, str(nsteps)],
+                           cwd=d, env=env, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            return None
+        sec = [float(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("step_seconds ")]
+    if not sec or sec[0] <= 0:
+        return None
+    return dict(value=Ue.size * 4 * nsteps / sec[0], unit="DOF*stage/s", cores=1, kind="reference",
+                sample="%d RK4 step(s) of the full %d-triangle P3 vortex mesh by the reference's own serial "
+                       "code (its DGAnalysis and TimeIntegrator built from its sources, LF flux), %.2f s of steps"
+                % (nsteps, len(tris), sec[0]))
+
+
 def cpu_baseline(mesh, budget_s=12.0):
     """RK4 steps of the FP64 CPU oracle on the same mesh until about budget_s of
     CPU work (a bounded sample of the workload)"""
@@ -307,7 +381,20 @@ def main():
     if args.maxwell_steps > 0:
         out["maxwell3d_tets"] = run_maxwell(dgpar, args.maxwell_steps, tune=tune)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mesh)
+        # the reference's own serial code on the same mesh (its build travels with
+        # the repository as oracle/_ref); the FP64 oracle port on this host's
+        # cores beside it
+        port = cpu_baseline(mesh)
+        from oracle import pyoracle as po
+        o = po.Oracle(2, mesh.vids, mesh.xyz, ORDER, po.LAW_EULER2D, po.FLUX_LF, gamma=1.4)
+        U0 = o.project(1, VORTEX)
+        ref = reference_cpu_baseline(mesh, U0, o.dt(U0, CFL), nsteps=3)
+        del o
+        if ref is not None:
+            out["cpu_baseline"] = ref
+            out["cpu_baseline_port"] = port
+        else:
+            out["cpu_baseline"] = port
     print(json.dumps(out))
     return 0
 

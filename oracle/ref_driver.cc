@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <map>
 #include <set>
 #include <vector>
@@ -206,11 +207,16 @@ int main(int argc, char** argv) {
     else
       ti = new ForwardEuler(mesh, 0, nf, DOF);
     double t = t0;
+    timespec c0, c1;
+    clock_gettime(CLOCK_MONOTONIC, &c0);
     for (int s = 0; s < nsteps; s++) {
       double r = ti->advanceInTime(t, dt);
       printf("resid %.17e\n", r);
       t += dt;
     }
+    clock_gettime(CLOCK_MONOTONIC, &c1);
+    // the step loop alone (mesh import and DGAnalysis construction excluded)
+    printf("step_seconds %.6f\n", (c1.tv_sec - c0.tv_sec) + 1e-9 * (c1.tv_nsec - c0.tv_nsec));
   }
 
   FILE* fo = fopen(argv[7], "w");
