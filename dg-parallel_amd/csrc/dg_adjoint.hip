@@ -72,6 +72,10 @@ int launch_tri_vjp(const AdjArgs& a, int law, int nd, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_vjp_nd<ScalarLawD, 1>(a, s);
   if (law == LAW_BURGERS2D) return launch_vjp_nd<Burgers2DLawD, 1>(a, s);
   if (law == LAW_MAXWELL2D) return launch_vjp_nd<Maxwell2dLawD, 1>(a, s);
+  if (law == LAW_EULER2D && a.law.char_bc) {
+    if (a.law.flux == FLUX_EXACT) return launch_vjp_nd<EulerExactCharLawD, 1>(a, s);
+    return launch_vjp_nd<EulerCharLawD, 1>(a, s);
+  }
   if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT) {
     switch (nd) {
       case 0:
@@ -107,6 +111,8 @@ static int launch_jvp_law(const AdjArgs& a, hipStream_t s) {
 
 int launch_tri_jvp(const AdjArgs& a, int law, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_jvp_law<ScalarLawD>(a, s);
+  if (law == LAW_EULER2D && a.law.char_bc)
+    return a.law.flux == FLUX_EXACT ? launch_jvp_law<EulerExactCharLawD>(a, s) : launch_jvp_law<EulerCharLawD>(a, s);
   if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT) return launch_jvp_law<EulerExactLawD>(a, s);
   if (law == LAW_EULER2D) return launch_jvp_law<EulerLawD>(a, s);
   if (law == LAW_MAXWELL2D) return launch_jvp_law<Maxwell2dLawD>(a, s);

@@ -224,6 +224,8 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
   // laws; for Maxwell2d at orders 3 and 4 it returned wrong values on the GPU
   // (its instantiation spills thousands of scalar registers into vector lanes),
   // so the other laws run the tiled kernel only
+  if (c->kernel == 1 && c->lp.char_bc)
+    return bail(fail(c, DG_ERR_UNSUPPORTED, "the characteristic far field runs the tiled kernel only"));
   if (c->kernel == 1 && c->law != DG_LAW_SCALAR && c->law != DG_LAW_EULER2D)
     return bail(fail(c, DG_ERR_UNSUPPORTED, "the one-element-per-work-item kernel is built for the scalar and Euler laws"));
   c->m_vids.assign(m->vids, m->vids + (size_t)S.E * 3);

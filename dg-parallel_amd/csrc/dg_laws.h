@@ -25,6 +25,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace dgp {
 
 enum { LAW_SCALAR = 0, LAW_EULER2D = 1, LAW_MAXWELL3D = 2, LAW_MAXWELL2D = 3, LAW_BURGERS2D = 4 };
@@ -32,14 +34,17 @@ enum { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2, FLUX_CENTRAL = 3, FLUX_ROE_EF
 // boundary codes: far field (farfield[0..NF)), wall, PEC, second far field
 // (farfield[4..8), Euler: a far-field state that differs by boundary, as the
 // reference's far-field FieldEvaluator evaluated at the boundary point)
-enum { BC_FARFIELD = 1, BC_WALL = 2, BC_PEC = 3, BC_FARFIELD2 = 4 };
+// BC_CHAR: the Euler characteristic far field (tag 30000), whose flux is the
+// physical flux of a boundary state; the two-point flux takes that state on
+// both sides (FP64 form)
+enum { BC_FARFIELD = 1, BC_WALL = 2, BC_PEC = 3, BC_FARFIELD2 = 4, BC_CHAR = 5 };
 
 struct LawParams {
   double gamma;
   double vel[3];
   double farfield[8];
   int flux;
-  int pad;
+  int char_bc;  // 1: the mesh has characteristic far-field faces (BC_CHAR)
 };
 
 template <typename T>
@@ -70,6 +75,7 @@ struct ScalarLawD {
   static constexpr int NF = 1;
   static constexpr int DIM = 2;
   static constexpr bool GHOST = true;  // boundary faces: ghost state + two-point flux
+  DG_HD static constexpr bool both_sides(int) { return false; }
   template <typename T>
   DG_HD static void flux(const LawParams& L, const T* q, T F[][DIM]) {
     F[0][0] = T(L.vel[0]) * q[0];
@@ -341,6 +347,47 @@ struct EulerLawD {
     ghost(L, code, n, Q, g);
     riemann(L, n, Q, g, out);
   }
+  DG_HD static constexpr bool both_sides(int) { return false; }
+  // the boundary state of the characteristic far field (Euler2d::boundaryFlux,
+  // tag 30000, EulerLaw.cc:187-278): supersonic inflow the far field, supersonic
+  // outflow the interior state, subsonic the state of the Riemann invariants
+  // J+ and J- taken from the far field and the interior
+  template <typename T>
+  DG_HD static void char_state(const LawParams& L, const T* n, const T* Q, T* g) {
+    const T gamma = T(L.gamma), gm1 = gamma - T(1), ogm1 = T(1) / gm1;
+    const T qi[4] = {T(L.farfield[0]), T(L.farfield[1]), T(L.farfield[2]), T(L.farfield[3])};
+    const T inv_rho = T(1) / Q[0], inv_rho_inf = T(1) / qi[0];
+    const T p = gm1 * (Q[3] - T(0.5) * (Q[1] * Q[1] + Q[2] * Q[2]) * inv_rho);
+    const T pinf = gm1 * (qi[3] - T(0.5) * (qi[1] * qi[1] + qi[2] * qi[2]) * inv_rho_inf);
+    const T vn = (n[0] * Q[1] + n[1] * Q[2]) * inv_rho;
+    const T vn_inf = (n[0] * qi[1] + n[1] * qi[2]) * inv_rho_inf;
+    const T c = dg_sqrt(gamma * p * inv_rho);
+    const bool inflow = vn < T(0);
+    if (dg_abs(vn) >= c) {
+      for (int i = 0; i < 4; i++) g[i] = inflow ? qi[i] : Q[i];
+      return;
+    }
+    const T c_inf = dg_sqrt(gamma * pinf * inv_rho_inf);
+    T vn_b, c_b, vt, r_b;
+    if (inflow) {
+      const T jp = -vn_inf + T(2) * c_inf * ogm1, jm = -vn - c * T(2) * ogm1;
+      vn_b = T(-0.5) * (jp + jm);
+      c_b = T(0.25) * (jp - jm) * gm1;
+      vt = (n[1] * qi[1] - n[0] * qi[2]) * inv_rho_inf;
+      r_b = qi[0] * dg_pow(c_b / c_inf, T(2) * ogm1);
+    } else {
+      const T jm = vn_inf - T(2) * c_inf * ogm1, jp = vn + T(2) * c * ogm1;
+      vn_b = T(0.5) * (jp + jm);
+      c_b = T(0.25) * (jp - jm) * gm1;
+      vt = (n[1] * Q[1] - n[0] * Q[2]) * inv_rho;
+      r_b = Q[0] * dg_pow(c_b / c, T(2) * ogm1);
+    }
+    g[0] = r_b;
+    g[1] = r_b * (n[0] * vn_b + n[1] * vt);
+    g[2] = r_b * (n[1] * vn_b - n[0] * vt);
+    const T p_b = r_b * c_b * c_b / gamma;
+    g[3] = p_b * ogm1 + T(0.5) * (g[1] * g[1] + g[2] * g[2]) / r_b;
+  }
   template <typename T>
   DG_HD static bool physical(const LawParams& L, const T* Q) {
     if (Q[0] < T(0)) return false;
@@ -374,6 +421,31 @@ struct EulerExactLawD : EulerLawD {
   }
 };
 
+// The Euler laws on meshes with characteristic far-field faces (tag 30000,
+// BC_CHAR; FP64 form): their ghost on those faces is the boundary state, taken
+// on both sides of the two-point flux, so the flux is that state's physical
+// flux (consistency of every Euler flux).  Types of their own: the other Euler
+// kernels do not carry the branch.
+template <class BASE>
+struct EulerCharLaw : BASE {
+  DG_HD static constexpr bool both_sides(int code) { return code == BC_CHAR; }
+  template <typename T>
+  DG_HD static void ghost(const LawParams& L, int code, const T* n, const T* Q, T* g) {
+    if (code == BC_CHAR)
+      BASE::char_state(L, n, Q, g);
+    else
+      BASE::ghost(L, code, n, Q, g);
+  }
+  template <typename T>
+  DG_HD static void boundary(const LawParams& L, int code, const T* n, const T* Q, T* out) {
+    T g[4];
+    ghost(L, code, n, Q, g);
+    BASE::riemann(L, n, code == BC_CHAR ? g : Q, g, out);
+  }
+};
+using EulerCharLawD = EulerCharLaw<EulerLawD>;
+using EulerExactCharLawD = EulerCharLaw<EulerExactLawD>;
+
 // Maxwell2d (MaxwellLaw.cc:21-37, 115-200): the TE fields (Hx, Hy, Ez), eps = mu
 // = 1.  F_Hx = (0, Ez), F_Hy = (-Ez, 0), F_Ez = (-Hy, Hx).  The numerical flux is
 // the physical flux of the reference's interface state, in its condensed form
@@ -385,6 +457,7 @@ struct Maxwell2dLawD {
   static constexpr int NF = 3;
   static constexpr int DIM = 2;
   static constexpr bool GHOST = true;
+  DG_HD static constexpr bool both_sides(int) { return false; }
   template <typename T>
   DG_HD static void flux(const LawParams&, const T* Q, T F[][DIM]) {
     F[0][0] = T(0);  F[0][1] = Q[2];
@@ -436,6 +509,7 @@ struct Burgers2DLawD {
   static constexpr int NF = 1;
   static constexpr int DIM = 2;
   static constexpr bool GHOST = false;
+  DG_HD static constexpr bool both_sides(int) { return false; }
   template <typename T>
   DG_HD static void flux(const LawParams&, const T* Q, T F[][DIM]) {
     F[0][0] = Q[0] * Q[0] * T(0.5);
@@ -470,7 +544,16 @@ template <class LAW, typename T>
 DG_HD void face_flux(const LawParams& L, int link, const T* n, const T* uL, T* uR, T* out) {
   if constexpr (LAW::GHOST) {
     if (link < 0) LAW::ghost(L, -link, n, uL, uR);
-    LAW::riemann(L, n, uL, uR, out);
+    if constexpr (LAW::both_sides(BC_CHAR)) {
+      // a boundary state for both sides (the Euler characteristic far field)
+      const bool both = link < 0 && LAW::both_sides(-link);
+      T ul[LAW::NF];
+#pragma unroll
+      for (int f = 0; f < LAW::NF; f++) ul[f] = both ? uR[f] : uL[f];
+      LAW::riemann(L, n, ul, uR, out);
+    } else {
+      LAW::riemann(L, n, uL, uR, out);
+    }
   } else {
     if (link < 0)
       LAW::boundary(L, -link, n, uL, out);
@@ -486,6 +569,7 @@ struct MaxwellLawD {
   static constexpr int NF = 6;
   static constexpr int DIM = 3;
   static constexpr bool GHOST = true;  // boundary faces: ghost state + two-point flux
+  DG_HD static constexpr bool both_sides(int) { return false; }
   template <typename T>
   DG_HD static void flux(const LawParams&, const T* Q, T F[][DIM]) {
     const T *E = Q, *H = Q + 3;

@@ -132,6 +132,10 @@ int launch_mixed_stage(const MixedArgs& a, int law, int ngrid, hipStream_t s) {
   dim3 grid(ngrid), block(MIX_BLOCK);
   if (law == LAW_SCALAR)
     hipLaunchKernelGGL(tri_mixed_stage<ScalarLawD>, grid, block, 0, s, a);
+  else if (law == LAW_EULER2D && a.law.char_bc && a.law.flux == FLUX_EXACT)
+    hipLaunchKernelGGL(tri_mixed_stage<EulerExactCharLawD>, grid, block, 0, s, a);
+  else if (law == LAW_EULER2D && a.law.char_bc)
+    hipLaunchKernelGGL(tri_mixed_stage<EulerCharLawD>, grid, block, 0, s, a);
   else if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT)
     hipLaunchKernelGGL(tri_mixed_stage<EulerExactLawD>, grid, block, 0, s, a);
   else if (law == LAW_EULER2D)
@@ -173,6 +177,8 @@ int launch_limiter_bucket(const LimiterArgs& a, int law, int p, hipStream_t s) {
   if (p < 1 || p > MAXP_TRI) return -1;
   if (law == LAW_SCALAR)
     launch_limiter_law<ScalarLawD>(a, p, s);
+  else if (law == LAW_EULER2D && a.law.char_bc)
+    launch_limiter_law<EulerCharLawD>(a, p, s);
   else if (law == LAW_EULER2D)
     launch_limiter_law<EulerLawD>(a, p, s);
   else

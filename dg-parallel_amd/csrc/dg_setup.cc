@@ -70,11 +70,10 @@ dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err) {
         v = T.nbl[e * 3 + l] | (T.same[e * 3 + l] << 2);
       } else {
         const int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
-        if (tag == DG_TAG_CHARACTERISTIC)
-          return failS(err, DG_ERR_UNSUPPORTED, "characteristic far-field boundary (tag 30000) is not built");
         // 20000, the reference's curved wall, is the 40000 wall on straight faces
         const int code = (tag == DG_TAG_WALL || tag == DG_TAG_CURVED_WALL) ? BC_WALL
                          : tag == DG_TAG_FARFIELD2                          ? BC_FARFIELD2
+                         : tag == DG_TAG_CHARACTERISTIC                     ? BC_CHAR
                                                                             : BC_FARFIELD;
         T.bcode[e * 3 + l] = (int8_t)code;
         v = 8 + code;
@@ -208,6 +207,20 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   S.xyz.assign(m->xyz, m->xyz + (size_t)S.E * 6);
   dg_status st = build_tri_topo(m, S.topo, err);
   if (st != DG_OK) return st;
+  // the characteristic far field (tag 30000) is an Euler boundary (EulerLaw.cc:
+  // 185-278), run in the FP64 form; the other laws' boundary fluxes give 30000
+  // no meaning of its own and treat it as any far-field tag (ScalarLaw.cc:39-59,
+  // MaxwellLaw.cc:115-130, Burgers2D.cc:31-49)
+  S.lp.char_bc = 0;
+  for (int8_t& b : S.topo.bcode)
+    if (b == BC_CHAR) {
+      if (S.law == DG_LAW_EULER2D)
+        S.lp.char_bc = 1;
+      else
+        b = BC_FARFIELD;
+    }
+  if (S.lp.char_bc && opt->precision != DG_PREC_FP64)
+    return failS(err, DG_ERR_UNSUPPORTED, "the characteristic far field (tag 30000) runs in the FP64 form only");
   S.faces = S.topo.faces;
   tri_layout(S.topo, ord.data(), S);
   const int P = S.P;

@@ -131,8 +131,12 @@ void Euler2d::riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, d
 }
 
 void Euler2d::boundaryFlux(mVector& n, int BoundaryId, const mPoint&, double* Q, double* flux, double) const {
-  if (BoundaryId == DG_TAG_CHARACTERISTIC) throw DGError(DG_ERR_UNSUPPORTED, "characteristic far field not built");
   dgp::LawParams L = params_of(describe());
+  if (BoundaryId == DG_TAG_CHARACTERISTIC) {
+    // the physical flux of the characteristic boundary state (EulerLaw.cc:185-278)
+    dgp::EulerCharLawD::boundary<double>(L, dgp::BC_CHAR, n.v, Q, flux);
+    return;
+  }
   const int code = BoundaryId == DG_TAG_WALL || BoundaryId == DG_TAG_CURVED_WALL ? dgp::BC_WALL
                    : BoundaryId == DG_TAG_FARFIELD2                                ? dgp::BC_FARFIELD2
                                                                                   : dgp::BC_FARFIELD;

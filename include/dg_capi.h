@@ -56,7 +56,10 @@ enum { DG_PREC_FP64 = 0, DG_PREC_FP32 = 1 };
  * 40000 reflective wall; 20000 the reference's curved wall (EulerLaw.cc:109-179,
  * its ghost state reflects the velocity about the exact surface normal N; on the
  * straight-sided faces built here N is the face normal and the ghost is the
- * 40000 one); 30000 characteristic far field (not built: rejected);
+ * 40000 one); 30000 characteristic far field (EulerLaw.cc:185-278: the
+ * free stream or the interior state at supersonic normal speed, else the state
+ * of the Riemann invariants J+ from outside and J- from inside; Euler, FP64 form,
+ * tiled kernel);
  * any other tag on a face without neighbour: Riemann flux against the law's
  * far-field state (the reference's default branch, EulerLaw.cc:293-295).  The
  * reference evaluates its far-field FieldEvaluator at the boundary point; tag

@@ -956,6 +956,66 @@ static void curved_wall_ghost(const double* n, const double* Q, double* g) {
   g[3] = Q[3];
 }
 
+/* The characteristic far field's boundary state (EulerLaw.cc:185-270): the free
+ * stream or the interior state when the normal speed is supersonic, else the
+ * state of the Riemann invariants J+ and J- taken from outside and inside.  The
+ * flux of that state is written by euler_char_flux below, the way the reference
+ * writes it; this state alone serves the limiter's neighbour mean. */
+static void euler_char_state(const or_law* L, const double* n, const double* Q, double* Qb, double* vnb,
+                             double* pb) {
+  const double* Qi = L->farfield;
+  const double gamma = L->gamma, gm1 = gamma - 1.0, ogm1 = 1.0 / gm1;
+  double p = gm1 * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]);
+  double pinf = gm1 * (Qi[3] - 0.5 * (Qi[1] * Qi[1] + Qi[2] * Qi[2]) / Qi[0]);
+  double vn = (n[0] * Q[1] + n[1] * Q[2]) / Q[0];
+  double vninf = (n[0] * Qi[1] + n[1] * Qi[2]) / Qi[0];
+  double c = sqrt(gamma * p / Q[0]), cinf = sqrt(gamma * pinf / Qi[0]);
+  int in = vn < 0;
+  if (fabs(vn) >= c) {
+    for (int i = 0; i < 4; i++) Qb[i] = in ? Qi[i] : Q[i];
+    *vnb = in ? vninf : vn;
+    *pb = in ? pinf : p;
+    return;
+  }
+  /* J- travels outward from inside, J+ inward from the free stream (inflow);
+   * the reverse at an outflow face.  In the inflow branch the reference counts
+   * the normal speed inward (the minus signs), so vn_b is negated back. */
+  double jp, jm, vb, cb, vt, rb;
+  if (in) {
+    jp = -vninf + 2.0 * cinf * ogm1;
+    jm = -vn - 2.0 * c * ogm1;
+    vb = -0.5 * (jp + jm);
+    cb = 0.25 * (jp - jm) * gm1;
+    vt = (n[1] * Qi[1] - n[0] * Qi[2]) / Qi[0];
+    rb = Qi[0] * pow(cb / cinf, 2.0 * ogm1);
+  } else {
+    jm = vninf - 2.0 * cinf * ogm1;
+    jp = vn + 2.0 * c * ogm1;
+    vb = 0.5 * (jp + jm);
+    cb = 0.25 * (jp - jm) * gm1;
+    vt = (n[1] * Q[1] - n[0] * Q[2]) / Q[0];
+    rb = Q[0] * pow(cb / c, 2.0 * ogm1);
+  }
+  Qb[0] = rb;
+  Qb[1] = rb * (n[0] * vb + n[1] * vt);
+  Qb[2] = rb * (n[1] * vb - n[0] * vt);
+  *pb = rb * cb * cb / gamma;
+  Qb[3] = *pb * ogm1 + 0.5 * (Qb[1] * Qb[1] + Qb[2] * Qb[2]) / rb;
+  *vnb = vb;
+}
+
+/* The flux through a characteristic far-field face (EulerLaw.cc:185-278): the
+ * physical normal flux of the boundary state, written with the boundary normal
+ * speed and pressure directly, no Riemann solver. */
+static void euler_char_flux(const or_law* L, const double* n, const double* Q, double* out) {
+  double Qb[4], vb, pb;
+  euler_char_state(L, n, Q, Qb, &vb, &pb);
+  out[0] = vb * Qb[0];
+  out[1] = vb * Qb[1] + pb * n[0];
+  out[2] = vb * Qb[2] + pb * n[1];
+  out[3] = vb * (Qb[3] + pb);
+}
+
 static void law_boundary(const or_law* L, int tag, const double* n, const double* x, double T, const double* Q,
                          double* out) {
   int Nf = nfields_of(L);
@@ -984,6 +1044,10 @@ static void law_boundary(const or_law* L, int tag, const double* n, const double
     /* ScalarLaw::boundaryFlux (ScalarLaw.cc:39-59) */
     double vn = n[0] * L->vel[0] + n[1] * L->vel[1] + n[2] * L->vel[2];
     out[0] = vn < 0 ? vn * L->farfield[0] : vn * Q[0];
+    return;
+  }
+  if (L->law == OR_LAW_EULER2D && tag == 30000) {
+    euler_char_flux(L, n, Q, out);
     return;
   }
   if (L->law == OR_LAW_EULER2D) {
@@ -1167,12 +1231,18 @@ int or_residual(or_ctx* c, const double* U, double t, double* R, int nthreads) {
 /* limiter and sensor (triangles)                                             */
 /* ------------------------------------------------------------------------- */
 /* Ghost state of a boundary face for the limiter's neighbour mean: Euler wall
- * mirror (EulerLaw.cc:280-291), far field (EulerLaw.cc:293-295), scalar far field
- * (ScalarLaw.cc:39-59). */
+ * mirror (EulerLaw.cc:280-291), far field (EulerLaw.cc:293-295), the
+ * characteristic far field's boundary state (EulerLaw.cc:185-270), scalar far
+ * field (ScalarLaw.cc:39-59). */
 static void law_ghost(const or_law* L, int tag, const double* n, const double* Q, double* g) {
   int Nf = nfields_of(L);
   if (L->law == OR_LAW_EULER2D && tag == 20000) {
     curved_wall_ghost(n, Q, g);
+    return;
+  }
+  if (L->law == OR_LAW_EULER2D && tag == 30000) {
+    double vb, pb;
+    euler_char_state(L, n, Q, g, &vb, &pb);
     return;
   }
   if (L->law == OR_LAW_EULER2D && tag == 40000) {

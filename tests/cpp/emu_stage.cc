@@ -59,6 +59,7 @@ int run_law(const StageArgs& a) {
 template <typename CT, typename IT, typename OT, int NW>
 int run(const StageArgs& a, int law) {
   if (law == LAW_SCALAR) return run_law<ScalarLawD, CT, IT, OT, NW>(a);
+  if (law == LAW_EULER2D && a.law.char_bc) return run_law<EulerCharLawD, CT, IT, OT, NW>(a);
   if (law == LAW_EULER2D) return run_law<EulerLawD, CT, IT, OT, NW>(a);
   if (law == LAW_MAXWELL2D) return run_law<Maxwell2dLawD, CT, IT, OT, NW>(a);
   if (law == LAW_BURGERS2D) return run_law<Burgers2DLawD, CT, IT, OT, NW>(a);
@@ -166,6 +167,7 @@ int run_vjp(const AdjArgs& a, int law, int nd = 0) {
   if (law == LAW_SCALAR) return run_vjp_law<ScalarLawD, 1>(a);
   if (law == LAW_MAXWELL2D) return run_vjp_law<Maxwell2dLawD, 1>(a);
   if (law == LAW_BURGERS2D) return run_vjp_law<Burgers2DLawD, 1>(a);
+  if (law == LAW_EULER2D && a.law.char_bc) return run_vjp_law<EulerCharLawD, 1>(a);
   if (law == LAW_EULER2D) {
     if (nd == 1) return run_vjp_law<EulerLawD, 1>(a);
     if (nd == 2) return run_vjp_law<EulerLawD, 2>(a);
@@ -188,6 +190,7 @@ int run_jvp_law(const AdjArgs& a) {
 
 int run_jvp(const AdjArgs& a, int law) {
   if (law == LAW_SCALAR) return run_jvp_law<ScalarLawD>(a);
+  if (law == LAW_EULER2D && a.law.char_bc) return run_jvp_law<EulerCharLawD>(a);
   if (law == LAW_EULER2D) return run_jvp_law<EulerLawD>(a);
   if (law == LAW_MAXWELL2D) return run_jvp_law<Maxwell2dLawD>(a);
   if (law == LAW_BURGERS2D) return run_jvp_law<Burgers2DLawD>(a);
@@ -486,6 +489,8 @@ void mix_stage(const MixedArgs& a, int law, double& rsum) {
     mix_stage_all<Maxwell2dLawD>(a, rsum);
   else if (law == LAW_BURGERS2D)
     mix_stage_all<Burgers2DLawD>(a, rsum);
+  else if (a.law.char_bc)
+    mix_stage_all<EulerCharLawD>(a, rsum);
   else
     mix_stage_all<EulerLawD>(a, rsum);
 }
@@ -494,6 +499,8 @@ void mix_limit(const LimiterArgs& l, int law) {
   for (int i = 0; i < l.E; i++) {
     if (law == LAW_SCALAR)
       limit_element<ScalarLawD>(l, i);
+    else if (l.law.char_bc)
+      limit_element<EulerCharLawD>(l, i);
     else
       limit_element<EulerLawD>(l, i);
   }

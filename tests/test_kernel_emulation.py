@@ -31,8 +31,7 @@ LIBEMU = _lib() if os.path.exists(LIB) else None
 FF = (1.0, 1.0, 1.0, 3.5)
 
 
-def case(law, flux, p, periodic=True, nx=8, ff=FF):
-    tags = (40000, 7, 40000, 7)
+def case(law, flux, p, periodic=True, nx=8, ff=FF, tags=(40000, 7, 40000, 7)):
     if law == "scalar":
         vids, xyz, bc = mesh_tri(nx, nx, periodic_x=periodic, periodic_y=periodic, jitter=0.1, seed=3, side_tags=tags)
         o = po.Oracle(2, vids, xyz, p, po.LAW_SCALAR, po.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.2,), bc=bc)
@@ -78,6 +77,23 @@ def rel(a, b):
 def test_emulated_residual(law, flux, periodic, p):
     mesh, lw, o, U = case(law, flux, p, periodic)
     assert rel(emu_residual(mesh, lw, p, U), o.residual(U)[0]) < 1e-12
+
+
+@pytest.mark.parametrize("ff", [FF, (1.0, 2.0, 0.5, 4.625)])
+@pytest.mark.parametrize("flux", ["lf", "roe"])
+@pytest.mark.parametrize("p", [1, 3])
+def test_emulated_characteristic_farfield(ff, flux, p):
+    """tag 30000 (EulerLaw.cc:185-278) on three sides, a wall on the fourth:
+    the kernel's boundary state under the law's own flux against the oracle's
+    direct flux formulas; residual, then RK4 steps"""
+    mesh, lw, o, U = case("euler", flux, p, periodic=False, ff=ff, tags=(30000, 30000, 40000, 30000))
+    assert rel(emu_residual(mesh, lw, p, U), o.residual(U)[0]) < 1e-12
+    dt = o.dt(U, 0.1)
+    Uo, t = U, 0.0
+    for s in range(2):
+        Uo, _ = o.step(Uo, po.RK4, t, dt)
+        t += dt
+    assert rel(emu_steps(mesh, lw, p, U, po.RK4, dt, 2)[0], Uo) < 1e-12
 
 
 @pytest.mark.parametrize("law,flux,p,scheme", [("scalar", None, 2, po.RK_TVD3), ("scalar", None, 1, po.RK_FE),

@@ -215,3 +215,54 @@ def test_other_laws_steps_reference_tables(law, op, scheme, p, reference_tables)
         t += dt
     assert np.abs(Uo - Ur).max() / np.abs(Ur).max() < 1e-13
     assert abs(r - resid[-1]) <= 1e-9 * abs(resid[-1])
+
+
+# The characteristic far field (tag 30000, EulerLaw.cc:185-278).  A subsonic
+# free stream gives inflow and outflow faces of the Riemann-invariant branch; a
+# supersonic one (Mach ~1.8 along x) adds the free-stream and extrapolation
+# branches on the x sides, with subsonic y sides.
+CHAR_FF = {"subsonic": FF, "supersonic": (1.0, 2.0, 0.5, 4.625)}
+CHAR_TAGS = {"subsonic": (40000, 30000, 40000, 30000), "supersonic": (30000, 30000, 30000, 30000)}
+
+
+def build_char(regime, flux, p, n=5, seed=0, tags=None):
+    m = rh.square_mesh(n, jitter=0.12, seed=60 + p + seed, periodic=False, side_tags=tags or CHAR_TAGS[regime])
+    ff = CHAR_FF[regime]
+    o = po.Oracle(2, m["top"], m["xyz"], p, po.LAW_EULER2D, EULER_FLUX[flux], gamma=1.4, farfield=ff, bc=m["bc"])
+    U = np.zeros((o.E, 4, np_tri(p)))
+    U[:, :, 0] = np.array(ff) / np.sqrt(2.0)
+    U += 0.01 * np.random.default_rng(p + seed).standard_normal(U.shape)
+    return m, o, U.ravel(), ff
+
+
+@pytest.mark.parametrize("regime", ["subsonic", "supersonic"])
+@pytest.mark.parametrize("flux", ["lf", "roe", "exact"])
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_characteristic_farfield_reference_tables(regime, flux, p, reference_tables):
+    m, o, U, ff = build_char(regime, flux, p)
+    Rr, _, _ = rh.run(m, "euler", flux, p, "residual", U, farfield=ff)
+    Ro, _ = o.residual(U)
+    assert np.abs(Ro - Rr).max() / np.abs(Rr).max() < 2e-13
+
+
+@pytest.mark.parametrize("regime", ["subsonic", "supersonic"])
+def test_characteristic_farfield_steps_reference_tables(regime, reference_tables):
+    m, o, U, ff = build_char(regime, "roe", 2, seed=3)
+    dt = 0.2 * o.dt(U, 0.1)
+    Ur, _, _ = rh.run(m, "euler", "roe", 2, "rk4", U, dt=dt, nsteps=3, farfield=ff)
+    Uo, t = U, 0.0
+    for _ in range(3):
+        Uo, _ = o.step(Uo, po.RK4, t, dt)
+        t += dt
+    assert np.abs(Uo - Ur).max() / np.abs(Ur).max() < 1e-13
+
+
+@pytest.mark.parametrize("regime", ["subsonic", "supersonic"])
+def test_characteristic_farfield_keeps_free_stream(regime):
+    """with the interior equal to the free stream both invariants come from the
+    same state, the boundary state is that state, and the residual vanishes
+    (every side characteristic: a wall would turn the normal velocity)"""
+    m, o, _, ff = build_char(regime, "lf", 2, tags=(30000,) * 4)
+    U = np.zeros((o.E, 4, np_tri(2)))
+    U[:, :, 0] = np.array(ff) / np.sqrt(2.0)
+    assert np.abs(o.residual(U.ravel())[0]).max() < 1e-12
