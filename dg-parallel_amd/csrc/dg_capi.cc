@@ -1287,7 +1287,11 @@ dg_status dg_adjoint_timing(const dg_ctx* c, float* fwd_ms, float* bwd_ms) {
 
 // triangle tile shapes: 0 (4 waves), 1 (8 waves), and 2 (4 waves with the
 // contractions on the matrix cores) for the FP32 form of the Euler law
-static int tri_shapes(const dg_ctx* c) { return c->law == DG_LAW_EULER2D && c->prec == DG_PREC_FP32 ? 3 : 2; }
+// tile shapes of the triangle stage: 4 and 8 waves, and the matrix-core shape of
+// the FP32 Euler form (not built with the characteristic far field)
+static int tri_shapes(const dg_ctx* c) {
+  return c->law == DG_LAW_EULER2D && c->prec == DG_PREC_FP32 && !c->lp.char_bc ? 3 : 2;
+}
 
 static bool schedule_ok(const dg_ctx* c, const dg_schedule& s) {
   const bool tiled = c->dim == 3 || (!c->mixed && c->kernel == 0);

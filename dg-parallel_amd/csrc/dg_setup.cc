@@ -208,7 +208,7 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   dg_status st = build_tri_topo(m, S.topo, err);
   if (st != DG_OK) return st;
   // the characteristic far field (tag 30000) is an Euler boundary (EulerLaw.cc:
-  // 185-278), run in the FP64 form; the other laws' boundary fluxes give 30000
+  // 185-278); the other laws' boundary fluxes give 30000
   // no meaning of its own and treat it as any far-field tag (ScalarLaw.cc:39-59,
   // MaxwellLaw.cc:115-130, Burgers2D.cc:31-49)
   S.lp.char_bc = 0;
@@ -219,8 +219,6 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
       else
         b = BC_FARFIELD;
     }
-  if (S.lp.char_bc && opt->precision != DG_PREC_FP64)
-    return failS(err, DG_ERR_UNSUPPORTED, "the characteristic far field (tag 30000) runs in the FP64 form only");
   S.faces = S.topo.faces;
   tri_layout(S.topo, ord.data(), S);
   const int P = S.P;

@@ -122,27 +122,12 @@ static int launch_tiled_mf_law(const StageArgs& a, int prec, hipStream_t s) {
   return -1;
 }
 
-// the FP64 form only (the characteristic far-field law types)
-template <class LAW, int NW>
-static int launch_tiled_law64(const StageArgs& a, int prec, hipStream_t s) {
-  if (prec != 0) return -1;
-  dim3 grid((a.E + TILE_ELEMS - 1) / TILE_ELEMS), block(NW * 64);
-  switch (a.P) {
-    case 1: hipLaunchKernelGGL((tri_stage_tiled<1, LAW, double, double, double, NW>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((tri_stage_tiled<2, LAW, double, double, double, NW>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((tri_stage_tiled<3, LAW, double, double, double, NW>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((tri_stage_tiled<4, LAW, double, double, double, NW>), grid, block, 0, s, a); break;
-    default: return -1;
-  }
-  return hipGetLastError() == hipSuccess ? 0 : -2;
-}
-
 template <int NW>
 int launch_tiled_nw(const StageArgs& a, int law, int prec, hipStream_t s) {
   if (law == LAW_SCALAR) return launch_tiled_law<ScalarLawD, NW>(a, prec, s);
   if (law == LAW_EULER2D && a.law.char_bc)
-    return a.law.flux == FLUX_EXACT ? launch_tiled_law64<EulerExactCharLawD, NW>(a, prec, s)
-                                    : launch_tiled_law64<EulerCharLawD, NW>(a, prec, s);
+    return a.law.flux == FLUX_EXACT ? launch_tiled_law<EulerExactCharLawD, NW>(a, prec, s)
+                                    : launch_tiled_law<EulerCharLawD, NW>(a, prec, s);
   if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT) return launch_tiled_law<EulerExactLawD, NW>(a, prec, s);
   if (law == LAW_EULER2D) return launch_tiled_law<EulerLawD, NW>(a, prec, s);
   if (law == LAW_MAXWELL2D) return launch_tiled_law<Maxwell2dLawD, NW>(a, prec, s);

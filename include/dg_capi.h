@@ -58,8 +58,8 @@ enum { DG_PREC_FP64 = 0, DG_PREC_FP32 = 1 };
  * straight-sided faces built here N is the face normal and the ghost is the
  * 40000 one); 30000 characteristic far field (EulerLaw.cc:185-278: the
  * free stream or the interior state at supersonic normal speed, else the state
- * of the Riemann invariants J+ from outside and J- from inside; Euler, FP64 form,
- * tiled kernel);
+ * of the Riemann invariants J+ from outside and J- from inside; Euler, FP64 and
+ * FP32 forms on the 4- and 8-wave tiles);
  * any other tag on a face without neighbour: Riemann flux against the law's
  * far-field state (the reference's default branch, EulerLaw.cc:293-295).  The
  * reference evaluates its far-field FieldEvaluator at the boundary point; tag

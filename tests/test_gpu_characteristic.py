@@ -7,8 +7,10 @@ formulas and is itself pinned against the reference build:
 - RK steps and dt, mixed orders P1-P4, limited steps of a shock tube whose ends
   are characteristic (limiter flags bit for bit), within 1e-12;
 - the derivative kernels against finite differences and each other;
-- the FP32 form and the one-element-per-work-item kernel refuse the tag; the
-  other laws read it as a plain far-field tag, as the reference's do."""
+- the FP32 form (FP32 arithmetic, FP64 stage accumulation) within 5e-5;
+- the matrix-core tile shape and the one-element-per-work-item kernel refuse
+  the tag; the other laws read it as a plain far-field tag, as the reference's
+  do."""
 import sys
 
 import numpy as np
@@ -65,6 +67,10 @@ def test_residual(regime, flux, p):
     g.set_schedule(shape=1)  # 8-wave tiles
     g.set_state(U)
     assert rel(g.residual(), Ro) < 1e-12
+    import dgpar
+    g32, _, _ = pair(regime, flux, p, precision=dgpar.PREC_FP32)
+    g32.set_state(U)
+    assert rel(g32.residual(), Ro) < 5e-5
 
 
 @pytest.mark.parametrize("regime,flux,p,scheme", [("subsonic", "roe", 3, po.RK4), ("supersonic", "lf", 2, po.RK_TVD3),
@@ -146,11 +152,26 @@ def test_derivatives(regime, flux):
     assert abs(float(np.dot(Av, W)) - float(np.dot(V, ATw))) <= 1e-12 * np.abs(Av).sum() * np.abs(W).max()
 
 
+def test_fp32_steps():
+    import dgpar
+    announce("characteristic FP32 RK4 steps")
+    g, o, U = pair("supersonic", "roe", 3, precision=dgpar.PREC_FP32)
+    g.set_state(U)
+    dt = o.dt(U, 0.1)
+    Uo, t = U, 0.0
+    for _ in range(3):
+        Uo, _ = o.step(Uo, po.RK4, t, dt)
+        t += dt
+    g.advanceInTime(0.0, dt, po.RK4, 3)
+    assert rel(g.get_state(), Uo) < 1e-5
+
+
 def test_refusals_and_other_laws():
     import dgpar
     announce("characteristic refusals; other laws read 30000 as far field")
-    with pytest.raises(dgpar.DGError, match="FP64"):
-        pair("subsonic", "roe", 2, precision=dgpar.PREC_FP32)
+    g32, _, _ = pair("subsonic", "roe", 2, precision=dgpar.PREC_FP32)
+    with pytest.raises(dgpar.DGError):
+        g32.set_schedule(shape=2)  # the matrix-core shape is not built for tag 30000
     with pytest.raises(dgpar.DGError, match="tiled"):
         pair("subsonic", "roe", 2, kernel=1)
     vids, xyz, bc = mesh_tri(8, 8, periodic_x=False, periodic_y=False, jitter=0.1, seed=6, side_tags=TAGS)
