@@ -131,13 +131,40 @@ def run_adjoint(dgpar, steps, every):
             "fwd_value": ndof * 4 * steps / (fwd_ms / 1e3), "unit": "DOF*stage/s"}
 
 
-def run_mixed(dgpar, steps, sense_every=20, tune=True):
+def run_scalar(dgpar, steps=100, tune=True):
+    """Scalar advection, velocity (1, 0.5), periodic unit square as 32x32 squares
+    split into 2048 triangles, P2, upwind flux, Gaussian pulse, SSP-RK3 at CFL
+    0.1, 100 steps, FP64; the L2 error against the translated pulse after the
+    timed steps.  12 288 DOF: the step is bound by launch latency, not HBM."""
+    mesh = dgpar.mesh_tri_structured(32, 32)
+    law = dgpar.Law(dgpar.LAW_SCALAR, dgpar.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.0,))
+    g = dgpar.DGAnalysis(mesh, law, 2, precision=dgpar.PREC_FP64, use_graph=True, device=0)
+    g.project(dgpar.FIELD_GAUSSIAN)
+    dt = g.computeDt(CFL)
+    sched = tuned(g, dgpar.RK_TVD3, dt, tune)
+    g.time_advance(dgpar.RK_TVD3, dt, 10)
+    g.project(dgpar.FIELD_GAUSSIAN)
+    dgpar.device_synchronize()
+    ms = g.time_advance(dgpar.RK_TVD3, dt, steps)
+    err = g.l2_error(dgpar.FIELD_GAUSSIAN, (), steps * dt)
+    ndof = g.ndof
+    g.close()
+    return {"workload": "scalar advection v=(1,0.5), 32x32x2 periodic unit square, P2, upwind, Gaussian pulse, "
+                        "SSP-RK3 CFL0.1", "dof": ndof, "steps": steps, "dt": dt, "ms_per_step": ms / steps,
+            "value": ndof * 3 * steps / (ms / 1e3), "unit": "DOF*stage/s", "l2_error_vs_exact": float(err[0]),
+            "schedule": sched}
+
+
+def run_mixed(dgpar, steps, sense_every=20, tune=True, t_end=0.2):
     """Shock tube with mixed orders: 512x128x2 jittered triangles on [0,1]x[0,0.25],
     walls at the bottom and top, the two initial states as far field at the ends,
     a seeded order map uniform on {1,2,3,4}, Roe flux with entropy fix, the
     triangle limiter after every SSP-RK3 stage, and the order sensor with
-    re-bucketing every sense_every steps.  Times steps (one persistent launch per
-    stage + the limiter) and the sensor passes separately."""
+    re-bucketing every sense_every steps.  steps < 0: the whole run to t_end
+    (the last step lands on it), with the density error against the exact Sod
+    solution.  The timed region holds the steps (one persistent launch per stage
+    + the limiter) and the sensor passes between them; both are also reported
+    apart."""
     nx, ny = 512, 128
     mesh0 = dgpar.mesh_tri_structured(nx, ny, lx=1.0, ly=0.25, jitter=0.1, seed=SEED, periodic_x=False,
                                       periodic_y=False, side_tags=(dgpar.TAG_WALL, dgpar.TAG_FARFIELD2,
@@ -154,18 +181,50 @@ def run_mixed(dgpar, steps, sense_every=20, tune=True):
     s_hi, s_lo = (0.05, 0.02, 0.01, 0.005), (5e-5, 2e-5, 1e-5, 5e-6)
     sched = tuned(g, dgpar.RK_TVD3, dt, tune)
     g.time_advance(dgpar.RK_TVD3, dt, 2)  # warm-up (graph capture)
+    # restart from the initial state after the tuner and the warm-up (neither
+    # changes the orders)
+    g.project(dgpar.FIELD_SOD, (0.5,))
+    g.set_limiter(True, 0.0, 1.0)
+    g.limit()
+    n0 = g.norms()
+    full = steps < 0
+    if full:
+        steps = int(np.ceil(t_end / dt - 1e-9))
     step_ms, sense_ms, dof_stages = 0.0, 0.0, 0.0
-    done = 0
+    done, t = 0, 0.0
+    dgpar.device_synchronize()
+    w0 = time.perf_counter()
     while done < steps:
         n = min(sense_every, steps - done)
-        step_ms += g.time_advance(dgpar.RK_TVD3, dt, n)
+        last = done + n == steps
+        if full and last:
+            h = t_end - (t + (n - 1) * dt)  # the last step lands on t_end
+            if n > 1:
+                step_ms += g.time_advance(dgpar.RK_TVD3, dt, n - 1)
+            step_ms += g.time_advance(dgpar.RK_TVD3, h, 1)
+            t += (n - 1) * dt + h
+        else:
+            step_ms += g.time_advance(dgpar.RK_TVD3, dt, n)
+            t += n * dt
         dof_stages += g.ndof * 3 * n
         done += n
-        t0 = time.perf_counter()
-        g.select_orders(s_hi, s_lo, 1, 4, apply=True)
-        sense_ms += (time.perf_counter() - t0) * 1e3
-    flags = g.limiter_flags()
+        if not last:
+            t0 = time.perf_counter()
+            g.select_orders(s_hi, s_lo, 1, 4, apply=True)
+            sense_ms += (time.perf_counter() - t0) * 1e3
+    dgpar.device_synchronize()
+    wall_ms = (time.perf_counter() - w0) * 1e3
+    flags = g.limiter_flags()  # the last stage's limiter pass (no sensor pass since)
     hist = np.bincount(g.orders(), minlength=5)[1:].tolist()
+    exact = None
+    if full:
+        n1 = g.norms(t, dgpar.FIELD_SOD, (0.5,))
+        area = 0.25
+        exact = {"t": t, "density_l1_per_area": float(n1["l1"][0] / area),
+                 "density_l2_per_sqrt_area": float(n1["l2"][0] / np.sqrt(area)),
+                 "density_linf": float(n1["linf"][0]),
+                 "mass_drift": float((n1["totals"][0] - n0["totals"][0]) / n0["totals"][0]),
+                 "energy_drift": float((n1["totals"][3] - n0["totals"][3]) / n0["totals"][3])}
     # algorithmic bytes per DOF*stage: SSP-RK3 stage traffic (stage 1 reads U^n
     # and writes A: 16 B; stages 2, 3 read the stage input and U^n and write:
     # 24 B each), the element table (|J| J^-1, |J|, 3 links: 52 B per element
@@ -173,13 +232,19 @@ def run_mixed(dgpar, steps, sense_every=20, tune=True):
     mean_np = sum(c * (p + 1) * (p + 2) // 2 for p, c in zip((1, 2, 3, 4), hist)) / E
     b = (16 + 24 + 24) / 3.0 + 52.0 / (4 * mean_np) + 8.0
     ach = b * dof_stages / (step_ms / 1e3) / 1e9
+    passes = -(-steps // sense_every) - 1
     out = {"workload": "euler2d Sod tube 512x128x2 [0,1]x[0,0.25] jitter 0.1h, orders P1-P4 seeded, Roe+entropy fix, "
-                       "SSP-RK3 CFL0.1, limiter every stage, sensor every %d steps" % sense_every,
-           "steps": steps, "dt": dt, "value": dof_stages / (step_ms / 1e3), "unit": "DOF*stage/s",
-           "ms_per_step": step_ms / steps, "achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS, "bytes_per_dof_stage": b,
-           "sensor_rebucket_ms_per_pass": sense_ms / max(1, -(-steps // sense_every)),
+                       "SSP-RK3 CFL0.1, limiter every stage, sensor every %d steps%s"
+                       % (sense_every, ", to t=%g" % t_end if full else ""),
+           "steps": steps, "dt": dt, "value": dof_stages / (wall_ms / 1e3), "unit": "DOF*stage/s",
+           "value_includes": "steps, limiter and sensor + re-bucketing passes (wall clock of the whole run)",
+           "value_steps_only": dof_stages / (step_ms / 1e3),
+           "ms_per_step": wall_ms / steps, "ms_per_step_steps_only": step_ms / steps,
+           "achieved_GBs_steps_only": ach, "frac_steps_only": ach / HBM_PEAK_GBS, "bytes_per_dof_stage": b,
+           "sensor_passes": passes, "sensor_rebucket_ms_per_pass": sense_ms / max(1, passes),
            "orders_histogram_p1_p4": hist, "limited_elements_last_stage": int((flags > 0).sum()),
-           "steps_to_t0.2": int(np.ceil(0.2 / dt)), "schedule": sched}
+           "positivity_scaled_last_stage": int(((flags & 16) > 0).sum()),
+           "steps_to_t0.2": int(np.ceil(0.2 / dt)), "exact_sod": exact, "schedule": sched}
     g.close()
     return out
 
@@ -281,11 +346,24 @@ def reference_cpu_baseline(mesh, U, dt, nsteps=1):
                 % (nsteps, len(tris), sec[0]))
 
 
-def cpu_baseline(mesh, budget_s=12.0):
-    """RK4 steps of the FP64 CPU oracle on the same mesh until about budget_s of
-    CPU work (a bounded sample of the workload)"""
+def cpu_model():
+    """the host CPU's model name and the cores this process may use"""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": name, "cores_available": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(mesh, threads, budget_s=10.0):
+    """RK4 steps of the FP64 CPU oracle on the same mesh on `threads` threads
+    until about budget_s of CPU work (a bounded sample of the workload)"""
     from oracle import pyoracle as po
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     o = po.Oracle(2, mesh.vids, mesh.xyz, ORDER, po.LAW_EULER2D, po.FLUX_LF, gamma=1.4)
     U = o.project(1, VORTEX)
     dt = o.dt(U, CFL)
@@ -310,8 +388,11 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200, help="RK4 steps of the adjoint leg (0: skip it)")
     ap.add_argument("--maxwell-steps", type=int, default=200,
                     help="RK4 steps of the Maxwell tetrahedra leg (0: skip it)")
-    ap.add_argument("--mixed-steps", type=int, default=100,
-                    help="SSP-RK3 steps of the mixed-order limited shock-tube leg (0: skip it)")
+    ap.add_argument("--mixed-steps", type=int, default=-1,
+                    help="SSP-RK3 steps of the mixed-order limited shock-tube leg (0: skip it; -1: the whole run "
+                         "to t = 0.2)")
+    ap.add_argument("--scalar-steps", type=int, default=100,
+                    help="SSP-RK3 steps of the scalar advection leg (0: skip it)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules (no tuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 256 MiB device copy (reference point for HBM byte counters)")
@@ -376,7 +457,9 @@ def main():
     }
     if args.adjoint_steps > 0:
         out["adjoint"] = run_adjoint(dgpar, args.adjoint_steps, 20)
-    if args.mixed_steps > 0:
+    if args.scalar_steps > 0:
+        out["scalar_advection"] = run_scalar(dgpar, args.scalar_steps, tune=tune)
+    if args.mixed_steps != 0:
         out["mixed_order_limited"] = run_mixed(dgpar, args.mixed_steps, tune=tune)
     if args.maxwell_steps > 0:
         out["maxwell3d_tets"] = run_maxwell(dgpar, args.maxwell_steps, tune=tune)
@@ -384,17 +467,22 @@ def main():
         # the reference's own serial code on the same mesh (its build travels with
         # the repository as oracle/_ref); the FP64 oracle port on this host's
         # cores beside it
-        port = cpu_baseline(mesh)
+        # the box's CPU share: 16 cores per GPU
+        allc = max(1, min(16, len(os.sched_getaffinity(0))))
+        port = {"threads_1": cpu_baseline(mesh, 1), "threads_%d" % allc: cpu_baseline(mesh, allc),
+                "cpu": cpu_model()}
         from oracle import pyoracle as po
         o = po.Oracle(2, mesh.vids, mesh.xyz, ORDER, po.LAW_EULER2D, po.FLUX_LF, gamma=1.4)
         U0 = o.project(1, VORTEX)
         ref = reference_cpu_baseline(mesh, U0, o.dt(U0, CFL), nsteps=3)
         del o
         if ref is not None:
+            ref["cpu"] = port["cpu"]
             out["cpu_baseline"] = ref
             out["cpu_baseline_port"] = port
         else:
-            out["cpu_baseline"] = port
+            out["cpu_baseline"] = dict(port["threads_%d" % allc], cpu=port["cpu"])
+            out["cpu_baseline_port"] = port
     print(json.dumps(out))
     return 0
 

@@ -252,7 +252,8 @@ dg_status dg_copy_probe(int device, size_t n_doubles, float* ms_out);
 
 /* Built-in exact fields: 0 Gaussian pulse carried by the scalar velocity on the
  * periodic unit square; 1 isentropic vortex, par = {beta, xc, yc, u0, v0, box
- * length}; 2 Sod tube (Examples.cc:62-83), par = {interface x}; 3 PEC cavity mode. */
+ * length}; 2 Sod tube (Examples.cc:62-83), par = {interface x}: the initial step
+ * at t = 0 and the exact Riemann solution (Toro ch. 4) at t > 0; 3 PEC cavity mode. */
 enum { DG_FIELD_GAUSSIAN = 0, DG_FIELD_VORTEX = 1, DG_FIELD_SOD = 2, DG_FIELD_CAVITY = 3 };
 
 /* DGCell::L2ProjInitial of an exact field at time t over all cells

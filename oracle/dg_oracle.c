@@ -1668,6 +1668,87 @@ static double wrap(double x, double L) {
   return r;
 }
 
+/* The exact solution of a 1D Riemann problem (Toro, "Riemann Solvers and
+ * Numerical Methods for Fluid Dynamics", ch. 4): the pressure function
+ * f_L(p) + f_R(p) + u_R - u_L = 0 solved by Newton from the PVRS guess to
+ * 1e-15 relative change, u* from it, then the solution sampled at s = x/t
+ * through the shock, rarefaction fan and contact of each side (Toro 4.5-4.7).
+ * wl, wr, out: primitive (rho, u, p). */
+static void riemann_fk(double g, double p, double rk, double pk, double ck, double* f, double* df) {
+  if (p > pk) {
+    double A = 2.0 / ((g + 1.0) * rk), B = (g - 1.0) / (g + 1.0) * pk, q = sqrt(A / (B + p));
+    *f = (p - pk) * q;
+    *df = q * (1.0 - 0.5 * (p - pk) / (B + p));
+  } else {
+    double pr = p / pk;
+    *f = 2.0 * ck / (g - 1.0) * (pow(pr, (g - 1.0) / (2.0 * g)) - 1.0);
+    *df = 1.0 / (rk * ck) * pow(pr, -(g + 1.0) / (2.0 * g));
+  }
+}
+
+static void riemann_exact_sample(double g, const double* wl, const double* wr, double s, double* out) {
+  double cl = sqrt(g * wl[2] / wl[0]), cr = sqrt(g * wr[2] / wr[0]);
+  double du = wr[1] - wl[1];
+  double p = 0.5 * (wl[2] + wr[2]) - 0.125 * du * (wl[0] + wr[0]) * (cl + cr);
+  if (p < 1e-12) p = 1e-12;
+  for (int it = 0; it < 100; it++) {
+    double fl, dfl, fr, dfr;
+    riemann_fk(g, p, wl[0], wl[2], cl, &fl, &dfl);
+    riemann_fk(g, p, wr[0], wr[2], cr, &fr, &dfr);
+    double pn = p - (fl + fr + du) / (dfl + dfr);
+    if (pn < 1e-12) pn = 1e-12;
+    double ch = 2.0 * fabs(pn - p) / (pn + p);
+    p = pn;
+    if (ch < 1e-15) break;
+  }
+  double fl, fr, d;
+  riemann_fk(g, p, wl[0], wl[2], cl, &fl, &d);
+  riemann_fk(g, p, wr[0], wr[2], cr, &fr, &d);
+  double us = 0.5 * (wl[1] + wr[1]) + 0.5 * (fr - fl);
+  double g1 = (g - 1.0) / (2.0 * g), g2 = (g + 1.0) / (2.0 * g), g6 = (g - 1.0) / (g + 1.0);
+  int left = s <= us;
+  const double* w = left ? wl : wr;
+  double c = left ? cl : cr, sg = left ? 1.0 : -1.0; /* sg: the side's wave runs towards -sg */
+  double pr = p / w[2];
+  if (p > w[2]) { /* shock */
+    double S = w[1] - sg * c * sqrt(g2 * pr + g1);
+    if (sg * (s - S) <= 0.0) {
+      out[0] = w[0], out[1] = w[1], out[2] = w[2];
+    } else {
+      out[0] = w[0] * (pr + g6) / (g6 * pr + 1.0), out[1] = us, out[2] = p;
+    }
+    return;
+  }
+  /* rarefaction: head w_u - sg c, tail u* - sg c* */
+  double sh = w[1] - sg * c, cs = c * pow(pr, g1), st = us - sg * cs;
+  if (sg * (s - sh) <= 0.0) {
+    out[0] = w[0], out[1] = w[1], out[2] = w[2];
+  } else if (sg * (s - st) >= 0.0) {
+    out[0] = w[0] * pow(pr, 1.0 / g), out[1] = us, out[2] = p;
+  } else {
+    double cf = 2.0 / (g + 1.0) * (c + sg * 0.5 * (g - 1.0) * (w[1] - s));
+    out[0] = w[0] * pow(cf / c, 2.0 / (g - 1.0));
+    out[1] = 2.0 / (g + 1.0) * (sg * c + 0.5 * (g - 1.0) * w[1] + s);
+    out[2] = w[2] * pow(cf / c, 2.0 * g / (g - 1.0));
+  }
+}
+
+/* Sod's states (Examples.cc:62-83): (1, 0, 1) | (0.125, 0, 0.1) */
+static void sod_exact(double g, double s, double* out) {
+  static const double wl[3] = {1.0, 0.0, 1.0}, wr[3] = {0.125, 0.0, 0.1};
+  riemann_exact_sample(g, wl, wr, s, out);
+}
+
+int or_riemann_exact(double gamma, const double* wl, const double* wr, double s, double* out) {
+  riemann_exact_sample(gamma, wl, wr, s, out);
+  return 0;
+}
+
+int or_riemann(const or_law* law, const double* n, const double* uL, const double* uR, double* out) {
+  law_riemann(law, n, uL, uR, out);
+  return 0;
+}
+
 int or_exact(int field, const double* par, const or_law* law, const double* x, double t, double* val) {
   if (field == 0) {
     /* Gaussian exp(-((x-.5)^2+(y-.5)^2)/0.01) advected with vel, unit square periodic */
@@ -1695,14 +1776,21 @@ int or_exact(int field, const double* par, const or_law* law, const double* x, d
     return 0;
   }
   if (field == 2) {
-    /* Sod (Examples.cc:62-83) */
-    double g = law->gamma;
-    int left = x[0] < par[0];
-    double rho = left ? 1.0 : 0.125, p = left ? 1.0 : 0.1;
-    val[0] = rho;
-    val[1] = 0.0;
+    /* Sod (Examples.cc:62-83): the initial step at t = 0, the exact solution of
+     * the Riemann problem (Toro ch. 4) at t > 0 */
+    double g = law->gamma, w[3];
+    if (t > 0.0) {
+      sod_exact(g, (x[0] - par[0]) / t, w);
+    } else {
+      int left = x[0] < par[0];
+      w[0] = left ? 1.0 : 0.125;
+      w[1] = 0.0;
+      w[2] = left ? 1.0 : 0.1;
+    }
+    val[0] = w[0];
+    val[1] = w[0] * w[1];
     val[2] = 0.0;
-    val[3] = p / (g - 1.0);
+    val[3] = w[2] / (g - 1.0) + 0.5 * w[0] * w[1] * w[1];
     return 0;
   }
   if (field == 3) {

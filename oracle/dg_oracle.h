@@ -72,11 +72,17 @@ int or_basis(int dim, int p, const double* uvw, double* phi, double* dphi);
  * field 0: Gaussian pulse translated by vel*t (periodic unit square)
  * field 1: isentropic vortex (gamma, beta=par[0], centre par[1],par[2], free stream
  *          (1, par[3], par[4], 1), periodic box [0,par[5]]^2) at time t
- * field 2: Sod tube, interface x = par[0]
+ * field 2: Sod tube, interface x = par[0]: the initial step at t = 0, the exact
+ *          Riemann solution (Toro ch. 4) at t > 0
  * field 3: PEC cavity mode (1,1,1) with E amplitude (1,-1,0) at time t */
 int or_project(or_ctx* c, int field, const double* par, double t, double* U);
 /* evaluates the same exact fields at a point */
 int or_exact(int field, const double* par, const or_law* law, const double* x, double t, double* val);
+/* test hooks: the law's numerical flux Fhat(uL, uR).n through one face point
+ * (riemannSolver); the exact solution of a 1D Riemann problem with primitive
+ * states wl, wr = (rho, u, p) sampled at s = x/t */
+int or_riemann(const or_law* law, const double* n, const double* uL, const double* uR, double* out);
+int or_riemann_exact(double gamma, const double* wl, const double* wr, double s, double* out);
 /* per-field L2 error of U against an exact field, rule of degree 2(p+1)
  * (DGError.cc:805-831) */
 int or_l2_error(or_ctx* c, const double* U, int field, const double* par, double t, double* err);

@@ -57,6 +57,8 @@ def _load():
     lib.or_limit.argtypes = [vp, _p_f64, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_uint8)]
     lib.or_set_limiter.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     lib.or_sensor.argtypes = [vp, _p_f64, _p_f64, _p_f64, ctypes.c_int, ctypes.c_int, _p_i32]
+    lib.or_riemann.argtypes = [ctypes.POINTER(_Law), _p_f64, _p_f64, _p_f64, _p_f64]
+    lib.or_riemann_exact.argtypes = [ctypes.c_double, _p_f64, _p_f64, ctypes.c_double, _p_f64]
     lib.or_set_tri_basis.argtypes = [ctypes.c_int, _p_f64]
     lib.or_set_tri_rule.argtypes = [ctypes.c_int, ctypes.c_int, _p_f64]
     return lib
@@ -77,6 +79,41 @@ def set_tri_rule(degree, pts):
 
 def reset_tables():
     _lib.or_reset_tables()
+
+
+def make_law(law, flux, gamma=1.4, vel=(0, 0, 0), farfield=()):
+    ff = list(farfield) + [0.0] * (8 - len(farfield))
+    v = list(vel) + [0.0] * (3 - len(vel))
+    return _Law(law, flux, gamma, (ctypes.c_double * 3)(*v), (ctypes.c_double * 8)(*ff[:8]))
+
+
+def riemann(law, n, uL, uR):
+    """the oracle's numerical flux Fhat(uL, uR).n at one face point; law from make_law"""
+    n = np.ascontiguousarray(np.resize(np.asarray(n, np.float64), 3))
+    uL = np.ascontiguousarray(uL, np.float64)
+    uR = np.ascontiguousarray(uR, np.float64)
+    out = np.zeros(max(len(uL), 6))
+    _lib.or_riemann(ctypes.byref(law), _p(n, _p_f64), _p(uL, _p_f64), _p(uR, _p_f64), _p(out, _p_f64))
+    return out[:len(uL)]
+
+
+def riemann_exact(gamma, wl, wr, s):
+    """the exact 1D Riemann solution (rho, u, p) at x/t = s for primitive states wl, wr"""
+    wl = np.ascontiguousarray(wl, np.float64)
+    wr = np.ascontiguousarray(wr, np.float64)
+    out = np.zeros(3)
+    _lib.or_riemann_exact(gamma, _p(wl, _p_f64), _p(wr, _p_f64), s, _p(out, _p_f64))
+    return out
+
+
+def exact(field, x, t, par=(), law=None):
+    """an exact field's conserved values at point x and time t"""
+    law = law if law is not None else make_law(LAW_EULER2D, FLUX_LF)
+    par = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)
+    x = np.ascontiguousarray(np.resize(np.asarray(x, np.float64), 3))
+    out = np.zeros(6)
+    _lib.or_exact(field, _p(par, _p_f64), ctypes.byref(law), _p(x, _p_f64), t, _p(out, _p_f64))
+    return out
 
 
 def _p(a, t):
