@@ -176,7 +176,8 @@ static dg_status ctx_init_tet(dg_ctx* c, const dg_mesh_desc* m, const dg_law_des
   if ((s = dalloc(c, &c->Un, nd)) || (s = dalloc(c, &c->acc, nd)) || (s = dalloc(c, &c->R, nd)) ||
       (s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd)) || (s = dalloc(c, &c->mg, (size_t)9 * E)) ||
       (s = dalloc(c, &c->detj, E)) || (s = dalloc(c, &c->size, E)) || (s = dalloc(c, &c->link, (size_t)4 * E)) ||
-      (s = dalloc(c, &c->tab, S.tab.size())) || (s = dalloc(c, &c->part, c->part_cap = (size_t)tet_tiles(E) + 1)))
+      (s = dalloc(c, &c->tab, S.tab.size())) || (s = dalloc(c, &c->part, c->part_cap = (size_t)tet_tiles(E) + 1)) ||
+      (s = dalloc(c, &c->status, 2)))
     return s;
   if (hipMemcpy(c->mg, S.mg.data(), S.mg.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->detj, S.detj.data(), E * 8, hipMemcpyHostToDevice) != hipSuccess ||
@@ -293,14 +294,15 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
       (s = dalloc(c, &c->mvtx, S.mvtx.size())) || (s = dalloc(c, &c->chunks, (size_t)2 * c->chunk_cap + 2)) ||
       (c->mixed && (s = dalloc(c, &c->remap, (size_t)E))) ||
       (s = dalloc(c, &c->ntab, S.ntab.size())) ||
-      (s = dalloc(c, &c->flags, (size_t)E)) || (s = dalloc(c, &c->sens, (size_t)E)))
+      (s = dalloc(c, &c->flags, (size_t)E)) || (s = dalloc(c, &c->sens, (size_t)E)) ||
+      (s = dalloc(c, &c->npflags, (size_t)E)) || (s = dalloc(c, &c->status, 2)))
     return bail(s);
   if (hipMemcpy(c->mtab, S.mtab.data(), S.mtab.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->glw, S.glw.data(), S.glw.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->mvtx, S.mvtx.data(), S.mvtx.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->chunks, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->ntab, S.ntab.data(), S.ntab.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(c->flags, 0, E) != hipSuccess)
+      hipMemset(c->flags, 0, E) != hipSuccess || hipMemset(c->npflags, 0, E) != hipSuccess)
     return bail(fail(c, DG_ERR_HIP, "table upload failed"));
   // limiter constants: phi_1, phi_2 at the edge midpoints and the least-squares
   // inverse that rebuilds (U_1, U_2) from midpoint deviations
@@ -455,8 +457,18 @@ static LimiterArgs limiter_args(const dg_ctx* c, double* U) {
   return l;
 }
 
+static RecoverArgs recover_args(const dg_ctx* c, double* U, int apply, uint8_t* flags, unsigned* status) {
+  RecoverArgs r{};
+  r.l = limiter_args(c, U);
+  r.l.flags = flags;
+  r.status = status;
+  r.apply = apply;
+  return r;
+}
+
 // one stage kernel (uniform tiled / per-element, or the persistent mixed-order
-// launch), then the limiter on its output when it is on
+// launch), then the limiter on its output when it is on, then the non-physical
+// recovery of that output (the next residual's input) when it is on
 static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
   int r;
   if (c->dim == 3) {
@@ -513,6 +525,8 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
       r = launch_limiter(la, c->law, c->stream);
     }
   }
+  if (r == 0 && c->rec_on && a.scheme != MODE_RESIDUAL)
+    r = launch_recover(recover_args(c, (double*)a.Uout, 1, c->npflags, c->status), c->law, c->Nf, c->stream);
   return r;
 }
 
@@ -553,15 +567,15 @@ dg_status dg_residual(dg_ctx* c, double t, double* R_out) {
   return DG_OK;
 }
 
-dg_status dg_compute_dt(dg_ctx* c, double cfl, double* dt_out) {
-  if (!ready(c) || !dt_out || !(cfl > 0)) return fail(c, DG_ERR_ARG, "dg_compute_dt: bad argument");
-  if (c->dim == 3) {
-    // Maxwell's largest eigenvalue is the light speed 1 everywhere (DGCell.cc:1155-1207
-    // with the TET size: the shortest edge, DGCell.cc:1230-1262)
-    *dt_out = cfl * c->hmin / (2. * c->P + 1.0);
-    return DG_OK;
+// per-workgroup dt minima of the triangle kernels into part; returns their
+// number, or -1 when a launch fails
+static int enqueue_dt(dg_ctx* c, double cfl, double* part) {
+  if (c->mixed) {
+    MixedArgs m = mixed_args(c);
+    m.Uin = c->Un;
+    if (launch_mixed_dt(m, c->law, cfl, part, c->stream)) return -1;
+    return mixed_dt_blocks(c->E);
   }
-  HIPCHECK(c, hipSetDevice(c->device));
   DtArgs a{};
   a.E = c->E;
   a.P = c->P;
@@ -573,17 +587,22 @@ dg_status dg_compute_dt(dg_ctx* c, double cfl, double* dt_out) {
   a.el.link = c->link;
   a.law = c->lp;
   a.cfl = cfl;
-  a.part = c->part;
-  int nb;
-  if (c->mixed) {
-    MixedArgs m = mixed_args(c);
-    m.Uin = c->Un;
-    if (launch_mixed_dt(m, c->law, cfl, c->part, c->stream)) return fail(c, DG_ERR_HIP, "dt launch failed");
-    nb = mixed_dt_blocks(c->E);
-  } else {
-    if (launch_tri_dt(a, c->law, c->stream)) return fail(c, DG_ERR_HIP, "dt launch failed");
-    nb = tri_dt_blocks(c->E);
+  a.part = part;
+  if (launch_tri_dt(a, c->law, c->stream)) return -1;
+  return tri_dt_blocks(c->E);
+}
+
+dg_status dg_compute_dt(dg_ctx* c, double cfl, double* dt_out) {
+  if (!ready(c) || !dt_out || !(cfl > 0)) return fail(c, DG_ERR_ARG, "dg_compute_dt: bad argument");
+  if (c->dim == 3) {
+    // Maxwell's largest eigenvalue is the light speed 1 everywhere (DGCell.cc:1155-1207
+    // with the TET size: the shortest edge, DGCell.cc:1230-1262)
+    *dt_out = cfl * c->hmin / (2. * c->P + 1.0);
+    return DG_OK;
   }
+  HIPCHECK(c, hipSetDevice(c->device));
+  const int nb = enqueue_dt(c, cfl, c->part);
+  if (nb < 0) return fail(c, DG_ERR_HIP, "dt launch failed");
   std::vector<double> h(nb);
   HIPCHECK(c, hipMemcpyAsync(h.data(), c->part, nb * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -640,28 +659,60 @@ static dg_status check_scheme(dg_ctx* c, int scheme) {
   return DG_OK;
 }
 
+// the captured step graph of (scheme, dt), captured, instantiated and uploaded
+// to the device when missing; advances nothing
+static dg_status ensure_graph(dg_ctx* c, int scheme, double dt) {
+  if (c->gexec && c->g_scheme == scheme && c->g_dt == dt) return DG_OK;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  hipGraph_t g;
+  HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(c, scheme, dt, false);
+  hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  if (rc || ec != hipSuccess) return fail(c, DG_ERR_HIP, "graph capture failed");
+  hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(c, DG_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+  c->g_scheme = scheme;
+  c->g_dt = dt;
+  HIPCHECK(c, hipGraphUpload(c->gexec, c->stream));
+  return DG_OK;
+}
+
 static dg_status run_steps(dg_ctx* c, int scheme, double dt, int nsteps) {
   if (nsteps <= 0) return DG_OK;
   if (c->use_graph && scheme != DG_RK_FE) {
-    if (!c->gexec || c->g_scheme != scheme || c->g_dt != dt) {
-      if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-      c->gexec = nullptr;
-      hipGraph_t g;
-      HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_step(c, scheme, dt, false);
-      hipError_t ec = hipStreamEndCapture(c->stream, &g);
-      if (rc || ec != hipSuccess) return fail(c, DG_ERR_HIP, "graph capture failed");
-      hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (ei != hipSuccess) return fail(c, DG_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
-      c->g_scheme = scheme;
-      c->g_dt = dt;
-    }
-    for (int s = 0; s < nsteps; s++) HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
+    dg_status s = ensure_graph(c, scheme, dt);
+    if (s) return s;
+    for (int i = 0; i < nsteps; i++) HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
   } else {
     for (int s = 0; s < nsteps; s++)
       if (enqueue_step(c, scheme, dt, false)) return fail(c, DG_ERR_HIP, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
   }
+  return DG_OK;
+}
+
+// The state after dg_advance's steps: every coefficient finite and, for the
+// Euler law on triangles, no element non-physical at a point its next residual
+// evaluates (checked, not changed); a recovery pass that met a non-physical
+// mean during the steps is reported too.
+static dg_status check_after_steps(dg_ctx* c) {
+  int r;
+  if (c->dim == 2)
+    r = launch_recover(recover_args(c, c->Un, 0, nullptr, c->status + 1), c->law, c->Nf, c->stream);
+  else
+    r = launch_nonfinite_scan(c->Un, (long)c->ndof, c->status + 1, c->stream);
+  if (r) return fail(c, DG_ERR_HIP, "state check launch failed");
+  unsigned st[2] = {0, 0};
+  HIPCHECK(c, hipMemcpyAsync(st, c->status, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (st[1] & NP_NONFINITE) return fail(c, DG_ERR_NONFINITE, "the state holds NaN or Inf after the steps");
+  if ((st[0] | st[1]) & NP_MEAN)
+    return fail(c, DG_ERR_NONPHYSICAL, "an element's mean state is non-physical (negative density or pressure)");
+  if (st[1] & NP_POINT)
+    return fail(c, DG_ERR_NONPHYSICAL,
+                "the state is non-physical (negative density or pressure) at quadrature points of some elements; "
+                "dg_recover names them, dg_set_recovery reduces them during the steps");
   return DG_OK;
 }
 
@@ -671,6 +722,12 @@ dg_status dg_advance(dg_ctx* c, int scheme, double t, double dt, int nsteps, dou
   dg_status s = check_scheme(c, scheme);
   if (s) return s;
   HIPCHECK(c, hipSetDevice(c->device));
+  // status words and the recovery flags of this call (outside any captured graph)
+  HIPCHECK(c, hipMemsetAsync(c->status, 0, 2 * sizeof(unsigned), c->stream));
+  if (c->dim == 2) HIPCHECK(c, hipMemsetAsync(c->npflags, 0, c->E, c->stream));
+  // the first residual's input
+  if (c->rec_on && launch_recover(recover_args(c, c->Un, 1, c->npflags, c->status), c->law, c->Nf, c->stream))
+    return fail(c, DG_ERR_HIP, "recovery launch failed");
   int body = resid_out ? nsteps - 1 : nsteps;
   if ((s = run_steps(c, scheme, dt, body))) return s;
   if (resid_out && nsteps > 0) {
@@ -683,8 +740,7 @@ dg_status dg_advance(dg_ctx* c, int scheme, double t, double dt, int nsteps, dou
     for (double v : h) r += v;
     *resid_out = r;
   }
-  HIPCHECK(c, hipStreamSynchronize(c->stream));
-  return DG_OK;
+  return check_after_steps(c);
 }
 
 dg_status dg_time_advance(dg_ctx* c, int scheme, double dt, int nsteps, float* ms_out) {
@@ -695,10 +751,8 @@ dg_status dg_time_advance(dg_ctx* c, int scheme, double dt, int nsteps, float* m
   hipEvent_t e0, e1;
   HIPCHECK(c, hipEventCreate(&e0));
   HIPCHECK(c, hipEventCreate(&e1));
-  // build the graph (if any) outside the timed region
-  if (c->use_graph && scheme != DG_RK_FE && (!c->gexec || c->g_scheme != scheme || c->g_dt != dt)) {
-    if ((s = run_steps(c, scheme, dt, 1))) return s;
-  }
+  // build the graph (if any) outside the timed region, without taking a step
+  if (c->use_graph && scheme != DG_RK_FE && (s = ensure_graph(c, scheme, dt))) return s;
   HIPCHECK(c, hipEventRecord(e0, c->stream));
   s = run_steps(c, scheme, dt, nsteps);
   HIPCHECK(c, hipEventRecord(e1, c->stream));
@@ -882,6 +936,37 @@ static dg_status fetch_flags(dg_ctx* c, const uint8_t* dsrc, uint8_t* out) {
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < c->E; i++) out[c->perm[i]] = h[i];
   return DG_OK;
+}
+
+dg_status dg_set_recovery(dg_ctx* c, int enabled) {
+  if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
+  if (enabled && (c->dim != 2 || c->law != DG_LAW_EULER2D))
+    return fail(c, DG_ERR_UNSUPPORTED, "recovery serves the Euler law on triangles (other laws are always physical)");
+  if (enabled && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "recovery runs in the FP64 form only");
+  c->rec_on = enabled ? 1 : 0;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured steps change
+  c->gexec = nullptr;
+  return DG_OK;
+}
+
+dg_status dg_recover(dg_ctx* c, int apply, uint8_t* flags_out) {
+  if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
+  if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "built for triangles");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipMemsetAsync(c->npflags, 0, c->E, c->stream));
+  HIPCHECK(c, hipMemsetAsync(c->status, 0, sizeof(unsigned), c->stream));
+  if (launch_recover(recover_args(c, c->Un, apply ? 1 : 0, c->npflags, c->status), c->law, c->Nf, c->stream))
+    return fail(c, DG_ERR_HIP, "recovery launch failed");
+  if (flags_out) return fetch_flags(c, c->npflags, flags_out);
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return DG_OK;
+}
+
+dg_status dg_recovery_flags(dg_ctx* c, uint8_t* flags_out) {
+  if (!ready(c) || !flags_out) return fail(c, DG_ERR_ARG, "bad argument");
+  if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "built for triangles");
+  HIPCHECK(c, hipSetDevice(c->device));
+  return fetch_flags(c, c->npflags, flags_out);
 }
 
 dg_status dg_limit(dg_ctx* c, uint8_t* flags_out) {

@@ -69,6 +69,12 @@ struct dg_ctx {
   double *mtab = nullptr, *glw = nullptr, *mvtx = nullptr, *ntab = nullptr;
   int* chunks = nullptr;
   uint8_t *flags = nullptr, *sens = nullptr;
+  // non-physical recovery (dg_set_recovery) after every stage when on; per-element
+  // flags of the recovery passes of the last dg_advance (or of dg_recover), and
+  // two status words: [0] the recovery passes, [1] the check after the steps
+  int rec_on = 0;
+  uint8_t* npflags = nullptr;
+  unsigned* status = nullptr;
   std::vector<void*> allocs;
   // graph cache
   hipGraphExec_t gexec = nullptr;

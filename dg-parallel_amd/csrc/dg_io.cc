@@ -110,7 +110,9 @@ dg_status dg_write_restart(dg_ctx* c, const char* path, double t, int iter) {
   std::fprintf(out.f, "%.17g\n%d\n%d %d %d\n", t, iter, c->dim, c->E, NF);
   for (int e = 0; e < c->E; e++) {
     std::fprintf(out.f, "%d ", c->dim == 2 ? T_TRI : T_TET);
-    for (int v = 0; v < nv; v++) std::fprintf(out.f, "%d ", c->m_vids[(size_t)e * nv + v]);
+    // vertex ids at the file boundary are node numbers, id + 1 (Vertex::getId() of a
+    // Gmsh-read reference mesh is the node number, mMesh.cc:401-412)
+    for (int v = 0; v < nv; v++) std::fprintf(out.f, "%d ", c->m_vids[(size_t)e * nv + v] + 1);
     const int p = c->m_order[e];
     std::fprintf(out.f, "\n%d ", p);
     for (long i = c->coff[e]; i < c->coff[e + 1]; i++) std::fprintf(out.f, "%.17g ", U[i]);
@@ -121,7 +123,7 @@ dg_status dg_write_restart(dg_ctx* c, const char* path, double t, int iter) {
     int fv[3];
     const int n = face_vertices(c->dim, l, fv);
     std::fprintf(out.f, "%d ", c->dim == 2 ? T_EDGE : T_TRI);
-    for (int k = 0; k < n; k++) std::fprintf(out.f, "%d ", c->m_vids[(size_t)e * nv + fv[k]]);
+    for (int k = 0; k < n; k++) std::fprintf(out.f, "%d ", c->m_vids[(size_t)e * nv + fv[k]] + 1);
     std::fputc('\n', out.f);
   }
   if (std::ferror(out.f)) return fail(c, DG_ERR_ARG, "write error on %s", path);
@@ -150,8 +152,10 @@ dg_status dg_read_restart(dg_ctx* c, const char* path, double* t_out, int* iter_
     int32_t ids[4];
     if (std::fscanf(in.f, "%d", &type) != 1 || type != (dim == 2 ? T_TRI : T_TET))
       return fail(c, DG_ERR_ARG, "%s: cell record %d: bad entity type", path, r);
-    for (int v = 0; v < nv; v++)
+    for (int v = 0; v < nv; v++) {
       if (std::fscanf(in.f, "%d", &ids[v]) != 1) return fail(c, DG_ERR_ARG, "%s: cell record %d truncated", path, r);
+      ids[v] -= 1;  // node number -> vertex id
+    }
     auto it = elem_of.find(sorted_key(ids, nv));
     if (it == elem_of.end()) return fail(c, DG_ERR_ARG, "%s: cell record %d: no element with these vertices", path, r);
     const int e = it->second;

@@ -68,4 +68,7 @@ struct DtArgs {
 int launch_tri_dt(const DtArgs& a, int law, hipStream_t s);
 int tri_dt_blocks(int E);
 
+// *status |= 4 when any of U[0 .. n) is NaN or Inf (one atomic per wave that meets one)
+int launch_nonfinite_scan(const double* U, long n, unsigned* status, hipStream_t s);
+
 }  // namespace dgp

@@ -21,6 +21,8 @@
 // local-face index on each face; the switch below then runs one path per wave.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dg_kernels.h"
 #include "dg_tables.h"
 
@@ -376,6 +378,25 @@ int launch_tri_dt(const DtArgs& a, int law, hipStream_t s) {
   if (law == LAW_MAXWELL2D) return launch_dt_law<Maxwell2dLawD>(a, s);
   if (law == LAW_BURGERS2D) return launch_dt_law<Burgers2DLawD>(a, s);
   return -1;
+}
+
+// ---------------------------------------------------------------------------
+// NaN / Inf check of a state array
+__global__ __launch_bounds__(BLOCK) void nonfinite_scan_kernel(const double* __restrict__ U, long n,
+                                                               unsigned* __restrict__ status) {
+  bool bad = false;
+  for (long i = blockIdx.x * (long)BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * BLOCK) {
+    const double x = U[i];
+    bad |= !(x - x == 0.0);
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 4u);
+}
+
+int launch_nonfinite_scan(const double* U, long n, unsigned* status, hipStream_t s) {
+  const long blocks = std::min<long>((n + BLOCK - 1) / BLOCK, 4096);
+  hipLaunchKernelGGL(nonfinite_scan_kernel, dim3((unsigned)std::max<long>(blocks, 1)), dim3(BLOCK), 0, s, U, n,
+                     status);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 }  // namespace dgp

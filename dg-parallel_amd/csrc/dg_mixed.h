@@ -92,9 +92,12 @@ DG_HD void edge_normal(const double mg[2][2], int l, double n[2], double& len) {
 //   u_l   = u(c_l) - m                          the whole polynomial's deviation
 //   d_l   = U_1 phi_1(c_l) + U_2 phi_2(c_l)    the linear part's deviation
 //   D_l   = m_l - m
-//   the field is troubled when some |u_l| > tol and minmod(u_l, nu D_l) != u_l,
-//   tol = max(M h^2, 1e-10 |m|) (h = 2 x inradius; M = 0 is the TVD limiter;
-//   Cockburn & Shu 1998, sec. 5.3, for P^k);
+//   the field is troubled when some |u_l| > tol and the minmod moves u_l:
+//   |u_l - minmod(u_l, nu D_l)| > 1e-10 (|u_l| + |nu D_l|) (a relative margin:
+//   a slope the limiter rebuilt equals the neighbour difference exactly, a tie
+//   that the next stage keeps to rounding); tol = max(M h^2, 1e-10 s) with the
+//   field's scale s = |m|, sqrt(|rho E|) for the Euler momenta (h = 2 x
+//   inradius; M = 0 is the TVD limiter; Cockburn & Shu 1998, sec. 5.3, for P^k);
 // a troubled field keeps its mean, takes the linear part rebuilt from the
 // limited deviations dl_l = minmod(d_l, nu D_l) (after the Cockburn-Shu
 // positive/negative rescaling that makes them sum to zero) and drops every
@@ -272,13 +275,22 @@ DG_HD void limit_element_p(const LimiterArgs& a, int i) {
   for (int f = 0; f < NF; f++) {
     double d[3];
     bool troubled = false;
+    // the field's scale: its mean, or for the Euler momenta sqrt(rho E) (a
+    // momentum that does not vanish in a gas at rest)
+    double sc = fabs(mean[f]);
+    if constexpr (NF == 4)
+      if (f == 1 || f == 2) sc = sqrt(fabs(mean[0] * mean[3]));
     double tol = a.tvb_m * h * h;
-    const double tr = 1e-10 * (mean[f] < 0.0 ? -mean[f] : mean[f]);
+    const double tr = 1e-10 * sc;
     if (tr > tol) tol = tr;
 #pragma unroll
     for (int l = 0; l < 3; l++) {
-      const double u = full[l][f], au = u < 0.0 ? -u : u;
-      if (au > tol && lim_minmod(u, a.nu * D[l][f]) != u) troubled = true;
+      // troubled when the minmod moves the deviation by more than 1e-10 of the
+      // compared values: a limited slope equals the neighbour difference exactly,
+      // a tie that the next stage keeps to rounding
+      const double u = full[l][f], nd = a.nu * D[l][f];
+      const double gap = fabs(u - lim_minmod(u, nd));
+      if (fabs(u) > tol && gap > 1e-10 * (fabs(u) + fabs(nd))) troubled = true;
       d[l] = fma(U[f][2], a.midv[l][2], U[f][1] * a.midv[l][1]);
     }
     if (!troubled) continue;
@@ -403,8 +415,98 @@ struct RemapArgs {
   double* dst;
 };
 
+// ---------------------------------------------------------------------------
+// Non-physical recovery and the state check (DGCell::setToZero, DGCell.cc:251-279,
+// triggered from the volume pass, DGCell.cc:198-223, and from the face pass,
+// DGCell.cc:639-732 with parallel.cc:57-72).  An Euler element whose state is
+// non-physical (Euler2d::isPhysical, EulerLaw.cc:1398-1404: rho < 0 or p < 0)
+// at a point the residual evaluates -- its volume points and its own traces at
+// the points of each face rule (max(p, p_nb) + 1 Gauss-Legendre points; p + 1
+// on boundary faces) -- keeps its means and drops every mode >= 1 (apply), or
+// is only reported (check).  Every element is checked on the input state and
+// rewrites only its own modes, so the pass is independent of scheduling; the
+// reference instead reduces cells one at a time inside its assembly.
+// Flag bits OR-ed into flags[i] and *status: NP_POINT a non-physical point
+// (reduced when applying), NP_MEAN a non-physical mean (unrecoverable; the
+// reference exits, DGCell.cc:218), NP_NONFINITE a NaN or Inf coefficient.
+enum { NP_POINT = 1, NP_MEAN = 2, NP_NONFINITE = 4 };
+
+struct RecoverArgs {
+  LimiterArgs l;      // the state (in place), tables, links, flags (OR-ed; may be null: report only)
+  unsigned* status;   // OR of every element's bits
+  int apply;          // 1: reduce; 0: check only
+};
+
+DG_HD bool finite_hd(double x) { return x - x == 0.0; }
+
+template <int NF, bool EULER, int P>
+DG_HD unsigned recover_element_p(const RecoverArgs& r, int i) {
+  const LimiterArgs& a = r.l;
+  constexpr int NP = np_tri(P), NQV = nqv_tri(P);
+  const int E = a.E;
+  double U[NF][NP];
+  unsigned bits = 0;
+#pragma unroll
+  for (int f = 0; f < NF; f++)
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      U[f][k] = a.U[mixed_index(a, P, i, f * NP + k)];
+      if (!finite_hd(U[f][k])) bits |= NP_NONFINITE;
+    }
+  if constexpr (EULER) {
+    const double gm1 = a.law.gamma - 1.0;
+    const double* tb = a.tab + a.moff[P];
+    auto bad = [&](const double* row) {
+      double w[4];
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP; k++) acc += U[f][k] * row[k];
+        w[f] = acc;
+      }
+      return w[0] < 0.0 || gm1 * (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]) < 0.0;
+    };
+    bool np = false;
+#pragma unroll 1
+    for (int q = 0; q < NQV && !np; q++) np = bad(tb + TriBlock::off_phi(P) + q * NP);
+#pragma unroll 1
+    for (int l = 0; l < 3 && !np; l++) {
+      const int link = a.el.link[(long)l * E + i];
+      int nq = P + 1;
+      if (link >= 0) {
+        const int pn = mixed_order_of(a, link >> 3);
+        if (pn + 1 > nq) nq = pn + 1;
+      }
+      const double* ft = a.tab + (nq == 2 ? a.ftoff[P][2] : nq == 3 ? a.ftoff[P][3] : nq == 4 ? a.ftoff[P][4]
+                                                                                              : a.ftoff[P][5]);
+#pragma unroll 1
+      for (int q = 0; q < nq && !np; q++) np = bad(ft + (l * nq + q) * NP);
+    }
+    if (np) {
+      bits |= NP_POINT;
+      const double phi0 = a.mvtx[P * 60];
+      double mean[4];
+#pragma unroll
+      for (int f = 0; f < 4; f++) mean[f] = U[f][0] * phi0;
+      if (mean[0] < 0.0 || gm1 * (mean[3] - 0.5 * (mean[1] * mean[1] + mean[2] * mean[2]) / mean[0]) < 0.0)
+        bits |= NP_MEAN;
+      if (r.apply)
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+#pragma unroll
+          for (int k = 1; k < NP; k++) a.U[mixed_index(a, P, i, f * NP + k)] = 0.0;
+    }
+  }
+  if (bits && a.flags) a.flags[i] = (uint8_t)(a.flags[i] | bits);  // checks without a flag array only report
+  return bits;
+}
+
 // launches (dg_mixed.hip); part holds one value per workgroup of the launch
 int launch_remap(const RemapArgs& a, hipStream_t s);
+// the recovery pass or check over every order bucket (triangles; Euler checks
+// points, every law checks finiteness)
+int launch_recover(const RecoverArgs& a, int law, int nf, hipStream_t s);
 int mixed_grid(int nchunks);
 int launch_mixed_stage(const MixedArgs& a, int law, int grid, hipStream_t s);
 // part receives one minimum per workgroup: mixed_dt_blocks(E) values

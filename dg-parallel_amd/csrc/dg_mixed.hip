@@ -91,6 +91,34 @@ __global__ __launch_bounds__(256) void tri_limit(LimiterArgs a) {
   if (i < a.bstart[P + 1]) limit_element_p<LAW, P>(a, i);
 }
 
+// one launch per order bucket; a wave with any flagged lane sets the status once
+template <int NF, bool EULER, int P>
+__global__ __launch_bounds__(256) void tri_recover(RecoverArgs a) {
+  const int i = a.l.bstart[P] + blockIdx.x * 256 + threadIdx.x;
+  unsigned bits = 0;
+  if (i < a.l.bstart[P + 1]) bits = recover_element_p<NF, EULER, P>(a, i);
+  const unsigned b3 = (__any(bits & 1) ? 1u : 0u) | (__any(bits & 2) ? 2u : 0u) | (__any(bits & 4) ? 4u : 0u);
+  if (b3 && (threadIdx.x & 63) == 0) atomicOr(a.status, b3);
+}
+
+template <int NF, bool EULER>
+static int launch_recover_t(const RecoverArgs& a, hipStream_t s) {
+  void (*kern[4])(RecoverArgs) = {tri_recover<NF, EULER, 1>, tri_recover<NF, EULER, 2>, tri_recover<NF, EULER, 3>,
+                                  tri_recover<NF, EULER, 4>};
+  for (int p = 1; p <= MAXP_TRI; p++) {
+    const int n = a.l.bstart[p + 1] - a.l.bstart[p];
+    if (n > 0) hipLaunchKernelGGL(kern[p - 1], dim3((n + 255) / 256), dim3(256), 0, s, a);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_recover(const RecoverArgs& a, int law, int nf, hipStream_t s) {
+  if (law == LAW_EULER2D) return launch_recover_t<4, true>(a, s);
+  if (nf == 1) return launch_recover_t<1, false>(a, s);
+  if (nf == 3) return launch_recover_t<3, false>(a, s);
+  return -1;
+}
+
 __global__ __launch_bounds__(256) void tri_sensor(SensorArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < a.E) sensor_element(a, i);

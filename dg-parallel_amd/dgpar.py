@@ -100,6 +100,9 @@ _p_u8 = ctypes.POINTER(ctypes.c_uint8)
 _set_limiter = _sig("dg_set_limiter", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double])
 _limit = _sig("dg_limit", ctypes.c_int, [_vp, _p_u8])
 _limiter_flags = _sig("dg_limiter_flags", ctypes.c_int, [_vp, _p_u8])
+_set_recovery = _sig("dg_set_recovery", ctypes.c_int, [_vp, ctypes.c_int])
+_recover = _sig("dg_recover", ctypes.c_int, [_vp, ctypes.c_int, _p_u8])
+_recovery_flags = _sig("dg_recovery_flags", ctypes.c_int, [_vp, _p_u8])
 _select_orders = _sig("dg_select_orders", ctypes.c_int, [_vp, ctypes.POINTER(SensorOpts), ctypes.c_int, _p_i32])
 _get_orders = _sig("dg_get_orders", ctypes.c_int, [_vp, _p_i32])
 _write_restart = _sig("dg_write_restart", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_double, ctypes.c_int])
@@ -450,6 +453,23 @@ class DGAnalysis:
     def limiter_flags(self):
         out = np.zeros(self.mesh_E, np.uint8)
         self._check(_limiter_flags(self._h, out.ctypes.data_as(_p_u8)))
+        return out
+
+    # ---- non-physical states (Euler, triangles) ----
+    def set_recovery(self, enabled):
+        """reduce non-physical elements to their means before every residual"""
+        self._check(_set_recovery(self._h, 1 if enabled else 0))
+
+    def recover(self, apply=True):
+        """one recovery pass (apply) or check over the current state; per-element
+        flags (bit 0 non-physical point, bit 1 non-physical mean, bit 2 NaN/Inf)"""
+        out = np.zeros(self.mesh_E, np.uint8)
+        self._check(_recover(self._h, 1 if apply else 0, out.ctypes.data_as(_p_u8)))
+        return out
+
+    def recovery_flags(self):
+        out = np.zeros(self.mesh_E, np.uint8)
+        self._check(_recovery_flags(self._h, out.ctypes.data_as(_p_u8)))
         return out
 
     def select_orders(self, s_hi, s_lo, pmin=1, pmax=4, apply=False):

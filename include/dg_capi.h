@@ -163,6 +163,30 @@ dg_status dg_limit(dg_ctx* ctx, uint8_t* flags_out);
 /* flags of the last limiter pass (dg_limit or the last stage of dg_advance) */
 dg_status dg_limiter_flags(dg_ctx* ctx, uint8_t* flags_out);
 
+/* ---- non-physical states (DGCell::setToZero, DGCell.cc:251-279, which the
+ * reference's assembly calls for a cell whose Euler state has rho < 0 or p < 0,
+ * Euler2d::isPhysical, EulerLaw.cc:1398-1404, at a volume point,
+ * DGCell.cc:198-223, or on one of its faces, DGCell.cc:639-732 with
+ * parallel.cc:57-72 and TimeIntegrator.cc:1390-1405) --------------------------
+ * dg_advance checks the state after its steps: DG_ERR_NONFINITE when a
+ * coefficient is NaN or Inf; DG_ERR_NONPHYSICAL when (Euler, triangles) an
+ * element is non-physical at a point its next residual evaluates (its volume
+ * points and its traces at the points of each face rule), or when a recovery
+ * pass met a non-physical mean.  With recovery enabled (Euler, triangles, FP64
+ * form), every stage's output and the first stage's input pass through the
+ * recovery: a non-physical element keeps its means and drops every mode >= 1,
+ * as setToZero does.  All elements are checked on the same input state and each
+ * rewrites only itself, so the result does not depend on any order (the
+ * reference reduces cells one at a time inside its assembly). */
+dg_status dg_set_recovery(dg_ctx* ctx, int enabled);
+/* One pass over the current state: apply = 1 reduces the non-physical elements,
+ * 0 only reports them.  flags_out (caller order, may be NULL): bit 0 a
+ * non-physical point, bit 1 a non-physical mean, bit 2 a NaN or Inf coefficient. */
+dg_status dg_recover(dg_ctx* ctx, int apply, uint8_t* flags_out);
+/* the flags (as dg_recover's) OR-ed over the recovery passes of the last
+ * dg_advance, or of the last dg_recover */
+dg_status dg_recovery_flags(dg_ctx* ctx, uint8_t* flags_out);
+
 /* Order selection (replaces DGAnalysis::adaptP / modifyP, DGPref.cc:24-60,
  * 602-630, whose face-jump bins choose orders 1..3): a modal-decay sensor on
  * field 0.  With top = energy of the degree-p modes and all = energy of every
@@ -311,7 +335,8 @@ dg_status dg_mesh_write_gmsh(const dg_mesh_desc* mesh, const char* path, char* e
 /* ---- checkpoint / restart (DGAnalysis::write / read, DGPref.cc:712-768) ----
  * The data stream of DGAnalysis::write: t, the iteration, "dim E F", then per
  * element in caller order its mMesh::writeMEntity record (entity type 2 or 4
- * and the vertex ids, mMesh.cc:401-412) followed by DGCell::write (the order and
+ * and the vertex ids, mMesh.cc:401-412, written as node numbers: vertex id + 1,
+ * the reference's Vertex::getId() of a Gmsh node) followed by DGCell::write (the order and
  * the Nf*Np coefficients field by field, DGCell.cc:1523-1530), then one record
  * per face (type 1 or 2 and its vertex ids).  Coefficients are written with 17
  * significant digits (the reference: 16), so a restart holds the same doubles. */
@@ -319,8 +344,12 @@ dg_status dg_write_restart(dg_ctx* ctx, const char* path, double t, int iter);
 /* Reads such a stream (DGAnalysis::read, DGPref.cc:742-768, which expects a
  * geometric order after each entity type that write never writes; this reader
  * takes the records as write produces them).  Cell records are matched to the
- * context's elements by their vertex ids (any record order; each element's
- * vertices must come in the context's order).  A triangle context in the FP64
+ * context's elements by their vertex ids (node number - 1; any record order;
+ * each element's vertices must come in the context's order), so a file the
+ * reference wrote over a Gmsh mesh without periodic sides reads into a context
+ * built from that mesh file.  On periodic sides the reference records the node
+ * number of each image while dg_mesh_read_gmsh gives a pair one id: those
+ * records find no element (DG_ERR_ARG names the record).  A triangle context in the FP64
  * form takes the orders of the file (re-bucketing as dg_select_orders does);
  * otherwise they must equal the context's.  Face records are not needed. */
 dg_status dg_read_restart(dg_ctx* ctx, const char* path, double* t_out, int* iter_out);

@@ -277,6 +277,10 @@ struct or_ctx {
   /* limiter applied after every RK stage when on (or_set_limiter) */
   int lim_on;
   double lim_m, lim_nu;
+  /* non-physical recovery before every residual when on (or_set_recovery) */
+  int rec_on;
+  /* or_residual_abs: accumulate |term| instead of the signed terms */
+  int absmode;
 };
 
 static int nfields_of(const or_law* l) {
@@ -1121,7 +1125,7 @@ static void volume_elem(or_ctx* c, const double* Ue, double* Re, int e, double t
       for (int f = 0; f < Nf; f++) {
         double s = 0;
         for (int d = 0; d < dim; d++) s += F[f][d] * g[d];
-        Re[f * Np + k] += s; /* source terms are zero for every law here */
+        Re[f * Np + k] += c->absmode ? fabs(s) : s; /* source terms are zero for every law here */
       }
     }
   }
@@ -1163,6 +1167,11 @@ static int face_flux(or_ctx* c, const double* U, int fi, double t) {
     double jw = c->fjw[P];
     /* DGCell.cc:709-728 */
     for (int k = 0; k < Nf; k++) {
+      if (c->absmode) {
+        for (int j = 0; j < NpL; j++) cL[k * NpL + j] += fabs(Fh[k] * phL[j] * jw);
+        for (int j = 0; j < NpR; j++) cR[k * NpR + j] += fabs(Fh[k] * phR[j] * jw);
+        continue;
+      }
       for (int j = 0; j < NpL; j++) cL[k * NpL + j] -= Fh[k] * phL[j] * jw;
       for (int j = 0; j < NpR; j++) cR[k * NpR + j] += Fh[k] * phR[j] * jw;
     }
@@ -1215,6 +1224,13 @@ static int run_phase(or_ctx* c, const double* U, double* R, double t, int phase,
     np |= w[i].nonphys;
   }
   return np;
+}
+
+int or_residual_abs(or_ctx* c, const double* U, double t, double* Rabs, int nthreads) {
+  c->absmode = 1;
+  int r = or_residual(c, U, t, Rabs, nthreads);
+  c->absmode = 0;
+  return r;
 }
 
 int or_residual(or_ctx* c, const double* U, double t, double* R, int nthreads) {
@@ -1428,14 +1444,23 @@ int or_limit(or_ctx* c, double* U, double tvb_m, double nu, uint8_t* flags) {
     double h = tri_size(c, e);
     uint8_t flag = 0;
     for (int f = 0; f < Nf; f++) {
-      double d[3], tol = tvb_m * h * h, tr = 1e-10 * fabs(mean[f]);
+      /* the field's scale: its mean, or for the Euler momenta sqrt(rho E) (a
+       * momentum that does not vanish in a gas at rest) */
+      double sc = fabs(mean[f]);
+      if (c->law.law == OR_LAW_EULER2D && (f == 1 || f == 2)) sc = sqrt(fabs(mean[0] * mean[3]));
+      double d[3], tol = tvb_m * h * h, tr = 1e-10 * sc;
       if (tr > tol) tol = tr;
       int troubled = 0;
       for (int l = 0; l < 3; l++) {
-        /* detection on the whole polynomial: u(c_l) - mean */
+        /* detection on the whole polynomial: u(c_l) - mean, troubled when the
+         * minmod moves it by more than 1e-10 of the compared values (a limited
+         * slope equals the neighbour difference exactly, a tie the next stage
+         * keeps to rounding) */
         double full = Ue[f * Np + 1] * midv[l][1];
         for (int k = 2; k < Np; k++) full = fma(Ue[f * Np + k], midv[l][k], full);
-        if (fabs(full) > tol && mm2(full, nu * D[l][f]) != full) troubled = 1;
+        const double nd = nu * D[l][f];
+        const double gap = fabs(full - mm2(full, nd));
+        if (fabs(full) > tol && gap > 1e-10 * (fabs(full) + fabs(nd))) troubled = 1;
         d[l] = fma(Ue[f * Np + 2], m2[l], Ue[f * Np + 1] * m1[l]);
       }
       if (!troubled) continue;
@@ -1497,15 +1522,89 @@ void or_set_limiter(or_ctx* c, int on, double tvb_m, double nu) {
   c->lim_nu = nu;
 }
 
+/* Non-physical recovery (DGCell::setToZero, DGCell.cc:251-279, triggered from
+ * the volume pass, DGCell.cc:198-223, and from the face pass through
+ * computeBoundaryContributions' return code, DGCell.cc:639-732 with
+ * parallel.cc:57-72): a cell whose state is non-physical (Euler2d::isPhysical,
+ * EulerLaw.cc:1398-1404: rho < 0 or p < 0) at a point the residual evaluates
+ * keeps its means and drops every mode >= 1, and its contributions are
+ * computed from the reduced state.  The points are its volume points and its
+ * traces at the points of each of its faces (rule of max(pL, pR)).  The
+ * reference reduces cells one at a time inside its assembly, in cell and face
+ * order, reversing and re-adding contributions; here every element is checked
+ * on the input state first and the flagged ones are reduced before the
+ * residual, which is the same result whenever the reduced cells do not touch
+ * each other and independent of any order.  flags[e] (may be NULL): bit 0
+ * reduced, bit 1 the mean itself is non-physical (the reference exits there,
+ * DGCell.cc:218), bit 2 a NaN or Inf coefficient.
+ * Returns the number of reduced elements; laws other than Euler are never
+ * non-physical. */
+static int elem_nonphysical(or_ctx* c, const double* Ue, int e) {
+  const or_law* L = &c->law;
+  int dim = c->dim, p = c->order[e], Np = np_of(dim, p), Nf = nfields_of(L);
+  double val[MAXNF];
+  for (int q = 0; q < c->nqv[p]; q++) {
+    for (int f = 0; f < Nf; f++) {
+      double s = 0;
+      for (int k = 0; k < Np; k++) s += Ue[f * Np + k] * c->vphi[p][q][k];
+      val[f] = s;
+    }
+    if (!law_physical(L, val)) return 1;
+  }
+  for (int l = 0; l < c->nfl; l++) {
+    const or_face* fc = &c->faces[c->ef[e * c->nfl + l]];
+    const double* tab = c->eside[e * c->nfl + l] ? c->fphiR : c->fphiL;
+    for (int q = 0; q < fc->nq; q++) {
+      const double* ph = tab + (long)(fc->off_pts + q) * MAXNP;
+      for (int f = 0; f < Nf; f++) {
+        double s = 0;
+        for (int k = 0; k < Np; k++) s += Ue[f * Np + k] * ph[k];
+        val[f] = s;
+      }
+      if (!law_physical(L, val)) return 1;
+    }
+  }
+  return 0;
+}
+
+int or_recover(or_ctx* c, double* U, uint8_t* flags) {
+  int Nf = nfields_of(&c->law), count = 0;
+  uint8_t* fl = (uint8_t*)calloc(c->E, 1);
+  for (int e = 0; e < c->E; e++) {
+    if (c->law.law == OR_LAW_EULER2D) fl[e] = (uint8_t)elem_nonphysical(c, U + c->uoff[e], e);
+    for (long i = c->uoff[e]; i < c->uoff[e + 1]; i++)
+      if (!isfinite(U[i])) fl[e] |= 4;
+  }
+  for (int e = 0; e < c->E; e++) {
+    if (!(fl[e] & 1)) continue;
+    int Np = np_of(c->dim, c->order[e]);
+    double* Ue = U + c->uoff[e];
+    for (int f = 0; f < Nf; f++)
+      for (int k = 1; k < Np; k++) Ue[f * Np + k] = 0.0;
+    double phi0 = c->vphi[c->order[e]][0][0], mean[MAXNF];
+    for (int f = 0; f < Nf; f++) mean[f] = Ue[f * Np] * phi0;
+    if (!law_physical(&c->law, mean)) fl[e] |= 2;
+    count++;
+  }
+  if (flags) memcpy(flags, fl, c->E);
+  free(fl);
+  return count;
+}
+
+void or_set_recovery(or_ctx* c, int on) { c->rec_on = on ? 1 : 0; }
+
 /* the limiter after every stage, as TimeIntegrator's integrators call
- * TimeIntegrator::limit (TimeIntegrator.cc:139-272, 359-429, 850-987) */
+ * TimeIntegrator::limit (TimeIntegrator.cc:139-272, 359-429, 850-987), then the
+ * non-physical recovery of the next residual's input */
 #define LIMIT()                                                          \
   do {                                                                   \
     if (c->lim_on) or_limit(c, U, c->lim_m, c->lim_nu, NULL);            \
+    if (c->rec_on) or_recover(c, U, NULL);                               \
   } while (0)
 
 double or_step(or_ctx* c, int scheme, double* U, double t, double dt, int nthreads) {
   long n = c->ndof;
+  if (c->rec_on) or_recover(c, U, NULL);
   double* R = (double*)malloc(sizeof(double) * n);
   double* s0 = (double*)malloc(sizeof(double) * n);
   double* acc = (double*)malloc(sizeof(double) * n);

@@ -56,6 +56,10 @@ void or_faces(const or_ctx* c, int32_t* out4);
 
 /* R = int F.grad(phi) + S phi - oint Fhat phi  (DGCell.cc:184-249, 639-732) */
 int or_residual(or_ctx* c, const double* U, double t, double* R, int nthreads);
+/* the same loops summing the magnitudes of the terms, |volume term| per point
+ * and |face term| per face point: the scale of the residual's summation, which
+ * bounds its rounding error (test hook) */
+int or_residual_abs(or_ctx* c, const double* U, double t, double* Rabs, int nthreads);
 /* one step of the chosen scheme in place; returns the residual norm the
  * reference's advanceInTime returns (TimeIntegrator.cc:81-275, 279-438, 838-990, 993-1048) */
 double or_step(or_ctx* c, int scheme, double* U, double t, double dt, int nthreads);
@@ -92,6 +96,14 @@ int or_l2_error(or_ctx* c, const double* U, int field, const double* par, double
  * (flags may be NULL).  or_set_limiter makes or_step limit after every stage. */
 int or_limit(or_ctx* c, double* U, double tvb_m, double nu, uint8_t* flags);
 void or_set_limiter(or_ctx* c, int on, double tvb_m, double nu);
+/* non-physical recovery (DGCell::setToZero, DGCell.cc:251-279): elements whose
+ * Euler state has rho < 0 or p < 0 at a volume point or at a point of one of
+ * their faces keep their means and drop modes >= 1, in place; flags[e] (may be
+ * NULL): bit 0 reduced, bit 1 with a non-physical mean, bit 2 a NaN or Inf
+ * coefficient; returns the number reduced.
+ * or_set_recovery makes or_step apply it to every stage's residual input. */
+int or_recover(or_ctx* c, double* U, uint8_t* flags);
+void or_set_recovery(or_ctx* c, int on);
 /* modal-decay sensor: next order per element (s_hi, s_lo indexed by order-1) */
 int or_sensor(or_ctx* c, const double* U, const double* s_hi, const double* s_lo, int pmin, int pmax,
               int32_t* orders);

@@ -45,6 +45,9 @@ def _load():
     lib.or_nfaces.argtypes = [vp]
     lib.or_faces.argtypes = [vp, _p_i32]
     lib.or_residual.argtypes = [vp, _p_f64, ctypes.c_double, _p_f64, ctypes.c_int]
+    lib.or_residual_abs.argtypes = [vp, _p_f64, ctypes.c_double, _p_f64, ctypes.c_int]
+    lib.or_recover.argtypes = [vp, _p_f64, ctypes.POINTER(ctypes.c_uint8)]
+    lib.or_set_recovery.argtypes = [vp, ctypes.c_int]
     lib.or_step.restype = ctypes.c_double
     lib.or_step.argtypes = [vp, ctypes.c_int, _p_f64, ctypes.c_double, ctypes.c_double, ctypes.c_int]
     lib.or_dt.restype = ctypes.c_double
@@ -171,6 +174,25 @@ class Oracle:
         R = np.zeros(self.ndof)
         flag = _lib.or_residual(self._h, _p(U, _p_f64), t, _p(R, _p_f64), nthreads)
         return R, flag
+
+    def residual_scale(self, U, t=0.0, nthreads=1):
+        """per DOF the sum of the magnitudes of the residual's terms"""
+        U = np.ascontiguousarray(U, np.float64)
+        R = np.zeros(self.ndof)
+        _lib.or_residual_abs(self._h, _p(U, _p_f64), t, _p(R, _p_f64), nthreads)
+        return R
+
+    def recover(self, U):
+        """non-physical recovery: reduced copy of U and per-element flags
+        (1 reduced, 2 reduced with a non-physical mean)"""
+        U = np.array(U, np.float64, copy=True)
+        flags = np.zeros(self.E, np.uint8)
+        _lib.or_recover(self._h, _p(U, _p_f64), flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        return U, flags
+
+    def set_recovery(self, on):
+        """recovery before every residual of step()"""
+        _lib.or_set_recovery(self._h, 1 if on else 0)
 
     def step(self, U, scheme, t, dt, nthreads=1):
         U = np.array(U, np.float64, copy=True)

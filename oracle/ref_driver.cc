@@ -7,7 +7,8 @@
 //   ref_driver MESH.msh LAW FLUX ORDER OP STATE_IN OUT [DT NSTEPS]
 //     LAW   scalar | euler | maxwell2d | burgers
 //     FLUX  upwind | exact | roe | lf (Euler; the other laws have one flux each)
-//     OP    residual | fe | tvd2 | tvd3 | rk4
+//     OP    residual | fe | tvd2 | tvd3 | rk4 | write (DGAnalysis::write of the
+//           state: the data stream to OUT, the mesh to OUT.msh)
 //     STATE_IN: one line per cell: 3 vertex ids (any order), then Nf*Np coefficients;
 //          with ORDER 0 each line carries the cell's order after the ids, and the
 //          driver gives every cell that order through DGCell::adaptOrder
@@ -26,7 +27,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <fstream>
 #include <map>
+#include <string>
 #include <set>
 #include <vector>
 
@@ -189,6 +192,13 @@ int main(int argc, char** argv) {
     DOF += nf * NP;
   }
 
+  if (!strcmp(op, "write")) {
+    // DGAnalysis::write (DGPref.cc:712-740): the mesh to OUT.msh, the data stream to OUT
+    std::string mp = std::string(argv[7]) + ".msh";
+    std::ofstream ms(mp.c_str()), ds(argv[7]);
+    an->write(ms, ds);
+    return ms.good() && ds.good() ? 0 : 6;
+  }
   const bool residual = !strcmp(op, "residual");
   if (residual) {
     // serial assembly as TimeIntegrator::assembleVolume/assembleBoundary do with one thread

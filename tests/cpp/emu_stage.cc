@@ -568,6 +568,47 @@ extern "C" int emu_limit(const dg_mesh_desc* mesh, const dg_law_desc* law, int o
   return 0;
 }
 
+// the non-physical recovery pass (apply) or check of dg_mixed.h over every
+// element; flags (caller order) may be NULL, as in dg_advance's check after
+// its steps (the element function then records bits in the status only)
+extern "C" int emu_recover(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int apply, double* U,
+                           uint8_t* flags, unsigned* status) {
+  MixEmu m;
+  if (mix_make(m, mesh, law, order)) return -1;
+  m.to_soa(U, m.U.data());
+  std::fill(m.flags.begin(), m.flags.end(), 0);
+  RecoverArgs r{};
+  r.l = m.lim(m.U.data(), 0.0, 1.0);
+  r.l.flags = flags ? m.flags.data() : nullptr;
+  r.apply = apply;
+  unsigned st = 0;
+  for (int i = 0; i < m.S.E; i++) {
+    const int p = mixed_order_of(r.l, i);
+    unsigned b = 0;
+    if (m.S.law == LAW_EULER2D) {
+      switch (p) {
+        case 1: b = recover_element_p<4, true, 1>(r, i); break;
+        case 2: b = recover_element_p<4, true, 2>(r, i); break;
+        case 3: b = recover_element_p<4, true, 3>(r, i); break;
+        default: b = recover_element_p<4, true, 4>(r, i); break;
+      }
+    } else {
+      switch (p) {
+        case 1: b = recover_element_p<1, false, 1>(r, i); break;
+        case 2: b = recover_element_p<1, false, 2>(r, i); break;
+        case 3: b = recover_element_p<1, false, 3>(r, i); break;
+        default: b = recover_element_p<1, false, 4>(r, i); break;
+      }
+    }
+    st |= b;
+  }
+  m.from_soa(m.U.data(), U);
+  if (flags)
+    for (int i = 0; i < m.S.E; i++) flags[m.S.perm[i]] = m.flags[i];
+  if (status) *status = st;
+  return 0;
+}
+
 extern "C" int emu_sensor(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, const double* U,
                           const double* s_hi, const double* s_lo, int pmin, int pmax, int32_t* orders) {
   MixEmu m;

@@ -63,8 +63,9 @@ def test_restart_file_layout_and_round_trip(tmp_path):
     tk = tokens(p)
     E, F = m.n_elems, g.faces().shape[0]
     assert float(tk[0]) == 2 * dt and tk[1:5] == ["2", "2", str(E), str(F)]
-    # first cell: type TRI = 2, its vertex ids, order 3, 4 x 10 coefficients
-    assert tk[5] == "2" and [int(v) for v in tk[6:9]] == m.vids[0].tolist() and tk[9] == "3"
+    # first cell: type TRI = 2, its vertex ids as node numbers (id + 1), order 3,
+    # 4 x 10 coefficients
+    assert tk[5] == "2" and [int(v) for v in tk[6:9]] == (m.vids[0] + 1).tolist() and tk[9] == "3"
     assert np.array([float(v) for v in tk[10:50]]).tolist() == U[:40].tolist()
     # after the cells, F edge records (type 1 and two vertex ids)
     cells = 5 + E * (1 + 3 + 1 + 40)
@@ -139,6 +140,32 @@ def test_restart_matches_cells_by_vertex_ids(tmp_path):
         b.read_restart(tmp_path / "t.dat")
     with pytest.raises(dgpar.DGError, match="cannot open"):
         b.read_restart(tmp_path / "missing.dat")
+
+
+def test_reads_a_restart_the_reference_wrote():
+    """tests/golden/ref_restart_tri.dat: DGAnalysis::write of the reference build
+    (P2 Euler, a 6x4 lattice mesh file with walls and far field, node numbers as
+    vertex ids) into a context built from that mesh file"""
+    import dgpar
+    announce("restart written by the reference")
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    m, info = dgpar.mesh_read_gmsh(os.path.join(gold, "ref_restart_tri.msh"))
+    g = dgpar.DGAnalysis(m, dgpar.EulerLaw(dgpar.FLUX_ROE, farfield=(1.0, 0.2, -0.1, 2.6)), 2)
+    t, it = g.read_restart(os.path.join(gold, "ref_restart_tri.dat"))
+    assert t == 0.0 and it == 0
+    # the records parsed here: (type 2, three node numbers, order 2, 24 coefficients)
+    tk = tokens(os.path.join(gold, "ref_restart_tri.dat"))
+    E = int(tk[3])
+    assert E == m.n_elems
+    want = {}
+    i = 5
+    for _ in range(E):
+        assert tk[i] == "2" and tk[i + 4] == "2"
+        want[tuple(int(v) for v in tk[i + 1:i + 4])] = [float(x) for x in tk[i + 5:i + 29]]
+        i += 29
+    U = g.get_state().reshape(E, 24)
+    for e in range(E):
+        assert U[e].tolist() == want[tuple((m.vids[e] + 1).tolist())]
 
 
 def test_write_mesh_and_data_then_restart_from_files(tmp_path):
