@@ -347,6 +347,9 @@ dg_status dg_create(const dg_mesh_desc* m, const dg_law_desc* law, const dg_opti
   dg_status s = ctx_init(c, m, law, opt);
   // on failure the context is kept (device memory released) to report the error
   if (s != DG_OK) free_all(c);
+  // the allocation-free entry point's scratch address on this device (looked up
+  // once per device; that entry point reports a failed lookup)
+  if (s == DG_OK && hipSetDevice(c->device) == hipSuccess) (void)noalloc_scratch_init(c->device);
   *out = c;
   return s;
 }
@@ -610,6 +613,30 @@ dg_status dg_compute_dt(dg_ctx* c, double cfl, double* dt_out) {
   for (double v : h) dt = std::min(dt, v);
   if (!std::isfinite(dt) || !(dt > 0)) return fail(c, DG_ERR_NONFINITE, "time step is %g", dt);
   *dt_out = dt;
+  return DG_OK;
+}
+
+dg_status dg_compute_dt_noalloc(dg_ctx* c, double cfl, double* dt_dev) {
+  if (!ready(c) || !dt_dev || !(cfl > 0)) return fail(c, DG_ERR_ARG, "dg_compute_dt_noalloc: bad argument");
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (c->dim == 3) {
+    // the shortest edge and the light speed 1: no reduction (as dg_compute_dt)
+    if (launch_fill_one(dt_dev, cfl * c->hmin / (2. * c->P + 1.0), c->stream))
+      return fail(c, DG_ERR_HIP, "dt launch failed");
+    return DG_OK;
+  }
+  const int nb = c->mixed ? mixed_dt_blocks(c->E) : tri_dt_blocks(c->E);
+  if (nb > NOALLOC_SCRATCH_DOUBLES)
+    return fail(c, DG_ERR_ARG,
+                "dg_compute_dt_noalloc: %d elements need %d workgroup minima; its scratch holds %d (at most %d "
+                "elements); use dg_compute_dt",
+                c->E, nb, NOALLOC_SCRATCH_DOUBLES, NOALLOC_SCRATCH_DOUBLES * 256);
+  double* part = noalloc_scratch(c->device);
+  if (!part) return fail(c, DG_ERR_HIP, "dg_compute_dt_noalloc: no scratch address for device %d", c->device);
+  // every one of the nb minima is rewritten before the reduction reads it, so
+  // the previous call's contents need no clearing
+  if (enqueue_dt(c, cfl, part) != nb || launch_min_reduce(part, nb, dt_dev, c->stream))
+    return fail(c, DG_ERR_HIP, "dt launch failed");
   return DG_OK;
 }
 

@@ -68,7 +68,18 @@ struct DtArgs {
 int launch_tri_dt(const DtArgs& a, int law, hipStream_t s);
 int tri_dt_blocks(int E);
 
+// The allocation-free time step: per-workgroup minima go to a file-scope
+// device array of NOALLOC_SCRATCH_DOUBLES doubles (one copy per device, kept
+// until the process ends) instead of a context buffer.  noalloc_scratch_init
+// looks up that array's device address once per device (call with the device
+// current); noalloc_scratch returns it (nullptr before the init).
+constexpr int NOALLOC_SCRATCH_DOUBLES = 1 << 16;
+int noalloc_scratch_init(int device);
+double* noalloc_scratch(int device);
+// out[0] = min(part[0 .. n)), one workgroup; out[0] = v (one work-item)
+int launch_min_reduce(const double* part, int n, double* out, hipStream_t s);
 // *status |= 4 when any of U[0 .. n) is NaN or Inf (one atomic per wave that meets one)
 int launch_nonfinite_scan(const double* U, long n, unsigned* status, hipStream_t s);
+int launch_fill_one(double* out, double v, hipStream_t s);
 
 }  // namespace dgp

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "dg_kernels.h"
 #include "dg_tables.h"
@@ -381,7 +382,51 @@ int launch_tri_dt(const DtArgs& a, int law, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// NaN / Inf check of a state array
+// Scratch of the allocation-free time step: the per-workgroup minima of the dt
+// kernels.  Host code never passes `scratch` itself to a kernel (on the host it
+// names a shadow variable, not device memory); it passes the address that
+// hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double* scratch_addr[NOALLOC_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return -1;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_addr[device]) return 0;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return -2;
+  }
+  scratch_addr[device] = (double*)p;
+  return 0;
+}
+
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  return scratch_addr[device];
+}
+
+__global__ __launch_bounds__(BLOCK) void min_reduce_kernel(const double* __restrict__ part, int n,
+                                                           double* __restrict__ out) {
+  double v = 1e300;
+  for (int i = threadIdx.x; i < n; i += BLOCK) v = fmin(v, part[i]);
+  __shared__ double red[BLOCK];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ void fill_one_kernel(double* out, double v) { out[0] = v; }
+
 __global__ __launch_bounds__(BLOCK) void nonfinite_scan_kernel(const double* __restrict__ U, long n,
                                                                unsigned* __restrict__ status) {
   bool bad = false;
@@ -396,6 +441,16 @@ int launch_nonfinite_scan(const double* U, long n, unsigned* status, hipStream_t
   const long blocks = std::min<long>((n + BLOCK - 1) / BLOCK, 4096);
   hipLaunchKernelGGL(nonfinite_scan_kernel, dim3((unsigned)std::max<long>(blocks, 1)), dim3(BLOCK), 0, s, U, n,
                      status);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_min_reduce(const double* part, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(min_reduce_kernel, dim3(1), dim3(BLOCK), 0, s, part, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_fill_one(double* out, double v, hipStream_t s) {
+  hipLaunchKernelGGL(fill_one_kernel, dim3(1), dim3(1), 0, s, out, v);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -131,6 +131,17 @@ dg_status dg_residual(dg_ctx* ctx, double t, double* R_out);
 /* DGCell::adaptTimeStep min over cells (DGCell.cc:1155-1262, DGAnalysis.cc:510-540):
  * dt = min_e CFL*size_e / ((2p_e+1) lambda_e), without the TEND snapping. */
 dg_status dg_compute_dt(dg_ctx* ctx, double cfl, double* dt_out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same global
+ * minimum as dg_compute_dt, written to dt_dev (a device pointer to one double)
+ * on the context's stream, with no allocation, no host copy and no
+ * synchronisation (the caller synchronises before reading it).  The
+ * per-workgroup minima go to a fixed device array of 65 536 doubles (512 KiB,
+ * one per device, held until the process ends), so triangle meshes are limited
+ * to 16 777 216 elements (DG_ERR_ARG beyond); tetrahedra need no scratch.
+ * Calls that overlap on one device (from several host threads or streams) race
+ * on that array. */
+dg_status dg_compute_dt_noalloc(dg_ctx* ctx, double cfl, double* dt_dev);
 
 /* nsteps calls of TimeIntegrator::advanceInTime(t, dt) (TimeIntegrator.h:28) with
  * the scheme chosen by rk_scheme (ForwardEuler, RungeKuttaTVD2, RungeKuttaTVD3,
