@@ -249,29 +249,60 @@ def run_mixed(dgpar, steps, sense_every=20, tune=True, t_end=0.2):
     return out
 
 
-def run_maxwell(dgpar, steps, n=48, tune=True):
-    """Maxwell on tetrahedra: 48^3 boxes x 6 Kuhn tetrahedra on the unit cube, PEC
-    walls, P3, upwind flux, RK4 at CFL 0.1, the (1,1,1) cavity mode; the L2
-    error against the exact mode after the timed steps."""
-    mesh = dgpar.mesh_tet_structured(n, n, n)
-    g = dgpar.DGAnalysis(mesh, dgpar.MaxwellLaw(dgpar.FLUX_UPWIND), 3, precision=dgpar.PREC_FP64, use_graph=True)
+def run_maxwell_form(dgpar, mesh, precision, steps, tune):
+    g = dgpar.DGAnalysis(mesh, dgpar.MaxwellLaw(dgpar.FLUX_UPWIND), 3, precision=precision, use_graph=True)
     g.project(dgpar.FIELD_CAVITY)
     dt = g.computeDt(CFL)
     sched = tuned(g, dgpar.RK4, dt, tune)
     g.time_advance(dgpar.RK4, dt, 2)
     g.project(dgpar.FIELD_CAVITY)
+    e0 = float(g.norms()["energy"].sum())
     dgpar.device_synchronize()
     ms = g.time_advance(dgpar.RK4, dt, steps)
-    err = g.l2_error(dgpar.FIELD_CAVITY, (), steps * dt)
+    nrm = g.norms(steps * dt, dgpar.FIELD_CAVITY)
+    state = g.get_state()
     ndof = g.ndof
     g.close()
-    b = 34.0 + (9 * 8 + 8 + 4 * 4) / (6 * 20.0)
-    ach = b * ndof / (ms / (4 * steps) / 1e3) / 1e9
+    return dict(ms=ms, ndof=ndof, err=[float(x) for x in nrm["l2"]],
+                drift=(float(nrm["energy"].sum()) - e0) / e0, state=state, sched=sched)
+
+
+def run_maxwell(dgpar, steps, n=48, tune=True):
+    """Maxwell on tetrahedra: 48^3 boxes x 6 Kuhn tetrahedra on the unit cube, PEC
+    walls, P3, upwind flux, RK4 at CFL 0.1, the (1,1,1) cavity mode, in the FP64
+    form and in the FP32 form (FP32 arithmetic and stage copies, FP64 step-start
+    state, accumulator and update); per form the L2 error of E and H against the
+    exact mode after the timed steps and the drift of the total energy (FP64
+    sums of the FP64 state), and the FP32 form's L2 distance to the FP64 form."""
+    mesh = dgpar.mesh_tet_structured(n, n, n)
+    r64 = run_maxwell_form(dgpar, mesh, dgpar.PREC_FP64, steps, tune)
+    r32 = run_maxwell_form(dgpar, mesh, dgpar.PREC_FP32, steps, tune)
+    ndof = r64["ndof"]
+    # algorithmic bytes per DOF*stage: RK4 state traffic and the element table
+    # (|J| J^-1, |J|, 4 links per element over 6 x 20 DOF); FP32 stage copies
+    table = (9 * 8 + 8 + 4 * 4) / (6 * 20.0)
+    b64 = 34.0 + table
+    b32 = (20 + 32 + 32 + 28) / 4.0 + table
+
+    def leg(r, b):
+        ach = b * ndof / (r["ms"] / (4 * steps) / 1e3) / 1e9
+        return {"ms_per_step": r["ms"] / steps, "value": ndof * 4 * steps / (r["ms"] / 1e3), "achieved_GBs": ach,
+                "frac": ach / HBM_PEAK_GBS, "bytes_per_dof_stage": b, "l2_error_E_H": r["err"],
+                "energy_drift": r["drift"], "schedule": r["sched"]}
+    d = (r32["state"] - r64["state"]).reshape(6 * n ** 3, 6, 20)
+    X = mesh.xyz
+    J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]], axis=2)
+    detj = np.abs(np.linalg.det(J))
+    l2diff = np.sqrt((d ** 2 * detj[:, None, None]).sum(axis=(0, 2)))
+    f64, f32 = leg(r64, b64), leg(r32, b32)
     return {"workload": "maxwell3d PEC cavity (1,1,1), %d^3 x 6 Kuhn tets, P3, upwind, RK4 CFL0.1" % n,
-            "elements": 6 * n ** 3, "dof": ndof, "steps": steps, "ms_per_step": ms / steps,
-            "value": ndof * 4 * steps / (ms / 1e3), "unit": "DOF*stage/s",
-            "achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS, "bytes_per_dof_stage": b,
-            "l2_error_E_H": [float(x) for x in err], "schedule": sched}
+            "elements": 6 * n ** 3, "dof": ndof, "steps": steps, "unit": "DOF*stage/s",
+            "fp64": f64, "fp32": f32, "ratio_fp32_over_fp64": f32["value"] / f64["value"],
+            "l2_fp32_vs_fp64_E_H": [float(x) for x in l2diff],
+            # the FP64 form's figures at the top level, as the earlier lines had them
+            "ms_per_step": f64["ms_per_step"], "value": f64["value"], "achieved_GBs": f64["achieved_GBs"],
+            "frac": f64["frac"], "bytes_per_dof_stage": b64, "l2_error_E_H": f64["l2_error_E_H"],
+            "schedule": f64["schedule"]}
 
 
 REF_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "ref_driver")

@@ -139,7 +139,8 @@ static dg_status ctx_init_tet(dg_ctx* c, const dg_mesh_desc* m, const dg_law_des
   c->Nf = S.NF;
   c->P = S.P;
   c->NP = S.NP;
-  c->prec = DG_PREC_FP64;
+  c->prec = opt->precision;
+  if (c->prec != DG_PREC_FP64 && c->prec != DG_PREC_FP32) return fail(c, DG_ERR_ARG, "unknown precision %d", c->prec);
   c->use_graph = opt->use_graph;
   c->device = opt->device;
   c->lp = S.lp;
@@ -174,16 +175,24 @@ static dg_status ctx_init_tet(dg_ctx* c, const dg_mesh_desc* m, const dg_law_des
   const size_t nd = c->ndof;
   const int E = c->E;
   if ((s = dalloc(c, &c->Un, nd)) || (s = dalloc(c, &c->acc, nd)) || (s = dalloc(c, &c->R, nd)) ||
-      (s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd)) || (s = dalloc(c, &c->mg, (size_t)9 * E)) ||
+      (s = dalloc(c, &c->mg, (size_t)9 * E)) ||
       (s = dalloc(c, &c->detj, E)) || (s = dalloc(c, &c->size, E)) || (s = dalloc(c, &c->link, (size_t)4 * E)) ||
-      (s = dalloc(c, &c->tab, S.tab.size())) || (s = dalloc(c, &c->part, c->part_cap = (size_t)tet_tiles(E) + 1)) ||
-      (s = dalloc(c, &c->status, 2)))
+      (s = dalloc(c, &c->tab, S.tab.size())) || (s = dalloc(c, &c->tabf, S.tab.size())) ||
+      (s = dalloc(c, &c->part, c->part_cap = (size_t)tet_tiles(E) + 1)) || (s = dalloc(c, &c->status, 2)))
     return s;
+  // stage copies: FP64, or FP32 in the FP32 form (step-start state and accumulator stay FP64)
+  if (c->prec == DG_PREC_FP64) {
+    if ((s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd))) return s;
+  } else {
+    if ((s = dalloc(c, &c->A32, nd)) || (s = dalloc(c, &c->B32, nd))) return s;
+  }
+  const std::vector<float> tabf(S.tab.begin(), S.tab.end());
   if (hipMemcpy(c->mg, S.mg.data(), S.mg.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->detj, S.detj.data(), E * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->size, S.size.data(), E * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->link, S.link.data(), S.link.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->tab, S.tab.data(), S.tab.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->tabf, tabf.data(), tabf.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(c->Un, 0, nd * 8) != hipSuccess || hipMemset(c->acc, 0, nd * 8) != hipSuccess)
     return fail(c, DG_ERR_HIP, "table upload failed");
   return DG_OK;
@@ -479,20 +488,21 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
     t.E = c->E;
     t.P = c->P;
     t.tab = c->tab;
+    t.tabf = c->tabf;
     t.mg = c->mg;
     t.detj = c->detj;
     t.link = c->link;
     t.law = c->lp;
-    t.Uin = (const double*)a.Uin;
+    t.Uin = a.Uin;
     t.Un = a.Un;
     t.acc = a.acc;
-    t.Uout = (double*)a.Uout;
+    t.Uout = a.Uout;
     t.R = a.R;
     t.dt = a.dt;
     t.scheme = a.scheme;
     t.stage = a.stage;
     t.resid_part = a.resid_part;
-    return launch_tet_stage(t, c->shape, c->stream);
+    return launch_tet_stage(t, c->shape, variant & 3, c->stream);
   }
   if (c->mixed) {
     MixedArgs m = mixed_args(c);

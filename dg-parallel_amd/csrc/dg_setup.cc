@@ -294,7 +294,7 @@ dg_status build_tet_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   if (law->law != DG_LAW_MAXWELL3D) return failS(err, DG_ERR_UNSUPPORTED, "tetrahedra run the Maxwell law only");
   if (law->flux != DG_FLUX_UPWIND && law->flux != DG_FLUX_CENTRAL)
     return failS(err, DG_ERR_UNSUPPORTED, "Maxwell flux %d not built", law->flux);
-  if (opt->precision != DG_PREC_FP64) return failS(err, DG_ERR_UNSUPPORTED, "tetrahedra run the FP64 form only");
+  if (opt->precision != DG_PREC_FP64 && opt->precision != DG_PREC_FP32) return failS(err, DG_ERR_ARG, "bad precision");
   int P = opt->order;
   if (m->order)
     for (int e = 0; e < m->n_elems; e++)

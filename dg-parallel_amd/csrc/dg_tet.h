@@ -38,14 +38,15 @@ namespace dgp {
 struct TetArgs {
   int E, P;
   const double* tab;     // TetBlock of order P
+  const float* tabf;     // its FP32 copy (the FP32 form)
   const double* mg;      // [9][E] |J| J^-1 (r*3+d)
   const double* detj;    // [E]
   const int32_t* link;   // [4][E]
   LawParams law;
-  const double* Uin;
+  const void* Uin;       // stage input: FP64, or FP32 in the FP32 form's stages 2..4
   const double* Un;
   double* acc;
-  double* Uout;
+  void* Uout;            // stage output: FP64, or FP32 in the FP32 form's stages 1..3
   double* R;
   double dt;
   int scheme, stage;     // scheme MODE_RESIDUAL: write R only
@@ -78,10 +79,14 @@ DG_HD void tet_ref_normal(int l, double N[3]) {
 
 // NW waves share a tile of 64 elements (4: 256 threads, 8: 512, 16: 1024, with
 // fewer rows per thread).  ALIAS: the face traces reuse the coefficient rows in
-// LDS (80 KB at P3: two workgroups per CU; own traces are then taken from the
-// coefficients in global memory, one pass per face), else the coefficient rows
-// stay and own traces come from LDS (101 KB at P3: one workgroup per CU).
-template <int P, int NW = 4, bool ALIAS = true>
+// LDS (80 KB at P3 in FP64: two workgroups per CU; own traces are then taken
+// from the coefficients in global memory, one pass per face), else the
+// coefficient rows stay and own traces come from LDS (101 KB at P3 in FP64: one
+// workgroup per CU; 50 KB in FP32: three).
+// CT: arithmetic type (the FP32 form: float, with the FP32 tables); IT / OT:
+// stage input / output types; the step-start state, the accumulator, dt/|J| and
+// the stage update stay FP64.
+template <int P, int NW = 4, bool ALIAS = true, typename CT = double, typename IT = double, typename OT = double>
 struct TiledTet {
   using LAW = MaxwellLawD;
   static constexpr int NP = np_tet(P), NQF = nqf_tet(P), NF = 6;
@@ -91,36 +96,41 @@ struct TiledTet {
   static constexpr int FW = NQF * NF;
 
   struct Traces {
-    double WL[FW][T];  // own trace, then flux times area and weight
-    double WR[FW][T];  // neighbour trace
+    CT WL[FW][T];  // own trace, then flux times area and weight
+    CT WR[FW][T];  // neighbour trace
   };
   struct SmemAlias {
     union {
-      double Us[NROW][T];  // phases A-B
-      Traces W;            // phases C1-L
+      CT Us[NROW][T];  // phases A-B
+      Traces W;        // phases C1-L
       double red[THREADS];
     };
     int Ls[4][T];
   };
   struct SmemKeep {
-    double Us[NROW][T];
     union {
-      double WR[FW][T];  // neighbour trace, then flux times area and weight
-      double red[THREADS];
+      struct {
+        CT Us[NROW][T];
+        CT WR[FW][T];  // neighbour trace, then flux times area and weight
+      };
+      double red[THREADS];  // after the stage update
     };
     int Ls[4][T];
   };
-  static_assert(ALIAS || FW * T >= THREADS, "reduction buffer");
+  static_assert(ALIAS || sizeof(CT) * (NROW + FW) * T >= sizeof(double) * THREADS, "reduction buffer");
   using Smem = std::conditional_t<ALIAS, SmemAlias, SmemKeep>;
   // the rows that hold the neighbour trace and the flux of face point row r
-  DG_HD static double* nb_row(Smem& s, int r) {
+  DG_HD static CT* nb_row(Smem& s, int r) {
     if constexpr (ALIAS) return s.W.WR[r]; else return s.WR[r];
   }
-  DG_HD static double* flux_row(Smem& s, int r) {
+  DG_HD static CT* flux_row(Smem& s, int r) {
     if constexpr (ALIAS) return s.W.WL[r]; else return s.WR[r];
   }
+  DG_HD static const CT* table(const TetArgs& a) {
+    if constexpr (sizeof(CT) == 8) return (const CT*)(const void*)a.tab; else return (const CT*)(const void*)a.tabf;
+  }
   struct Regs {
-    double R[NF * ((NP + NW - 1) / NW)];
+    CT R[NF * ((NP + NW - 1) / NW)];
     double rsum;
   };
 
@@ -131,29 +141,30 @@ struct TiledTet {
     const int es = valid ? e : 0;  // padding lanes load element 0 (in bounds) and discard it
     // every row load is issued, unconditionally, before the LDS writes use any
     constexpr int RA = (NROW + NW - 1) / NW;
-    double u[RA];
+    const IT* Uin = (const IT*)a.Uin;
+    CT u[RA];
 #pragma unroll
     for (int j = 0; j < RA; j++) {
       const int r = wave + NW * j;
-      u[j] = r < NROW ? a.Uin[(long)r * E + es] : 0.0;
+      u[j] = r < NROW ? CT(Uin[(long)r * E + es]) : CT(0);
     }
     const int lk = a.link[(long)(wave & 3) * E + es];
 #pragma unroll
     for (int j = 0; j < RA; j++) {
       const int r = wave + NW * j;
-      if (r < NROW) s.Us[r][t] = valid ? u[j] : 0.0;
+      if (r < NROW) s.Us[r][t] = valid ? u[j] : CT(0);
     }
     if (wave < 4) s.Ls[wave][t] = valid ? lk : -BC_PEC;
   }
 
   // |J| J^-1 of the lane's element (the identity past the end of the mesh)
-  DG_HD static void load_mg(const TetArgs& a, int tile, int t, double m[3][3]) {
+  DG_HD static void load_mg(const TetArgs& a, int tile, int t, CT m[3][3]) {
     const int E = a.E, e = tile * T + t;
     const bool valid = e < E;
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
-      for (int d = 0; d < 3; d++) m[r][d] = valid ? a.mg[(long)(r * 3 + d) * E + e] : (r == d ? 1.0 : 0.0);
+      for (int d = 0; d < 3; d++) m[r][d] = valid ? CT(a.mg[(long)(r * 3 + d) * E + e]) : CT(r == d ? 1 : 0);
   }
 
   // Rows are owned by k-slot: wave w holds the modes k = w + NW ks (ks < KS) of
@@ -166,28 +177,28 @@ struct TiledTet {
   // (K_r U_g)[k] once, then every field's row from its two (d, g) pairs
   DG_HD static void phaseB(const TetArgs& a, Smem& s, int tid, int tile, Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const double* K = a.tab + TetBlock::off_K(P);
-    double m[3][3];
+    const CT* K = table(a) + TetBlock::off_K(P);
+    CT m[3][3];
     load_mg(a, tile, t, m);
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
       const int k = kmode(wave, ks);
       if (k >= NP) {
 #pragma unroll
-        for (int f = 0; f < NF; f++) g.R[f * KS + ks] = 0.0;
+        for (int f = 0; f < NF; f++) g.R[f * KS + ks] = CT(0);
         continue;
       }
-      double dot[NF][3];
+      CT dot[NF][3];
 #pragma unroll
       for (int gg = 0; gg < NF; gg++)
 #pragma unroll
-        for (int r = 0; r < 3; r++) dot[gg][r] = 0.0;
+        for (int r = 0; r < 3; r++) dot[gg][r] = CT(0);
 #pragma unroll 4
       for (int jj = 0; jj < NP; jj++) {
-        const double k0 = K[(0 * NP + k) * NP + jj], k1 = K[(1 * NP + k) * NP + jj], k2 = K[(2 * NP + k) * NP + jj];
+        const CT k0 = K[(0 * NP + k) * NP + jj], k1 = K[(1 * NP + k) * NP + jj], k2 = K[(2 * NP + k) * NP + jj];
 #pragma unroll
         for (int gg = 0; gg < NF; gg++) {
-          const double u = s.Us[gg * NP + jj][t];
+          const CT u = s.Us[gg * NP + jj][t];
           dot[gg][0] += k0 * u;
           dot[gg][1] += k1 * u;
           dot[gg][2] += k2 * u;
@@ -196,11 +207,11 @@ struct TiledTet {
 #pragma unroll
       for (int f = 0; f < NF; f++) {
         const MaxwellJac J = maxwell_jac(f);
-        double acc = 0.0;
+        CT acc = CT(0);
 #pragma unroll
         for (int mm = 0; mm < 2; mm++) {
           const int gg = J.g[mm], d = J.d[mm];
-          acc += J.s[mm] * (m[0][d] * dot[gg][0] + m[1][d] * dot[gg][1] + m[2][d] * dot[gg][2]);
+          acc += CT(J.s[mm]) * (m[0][d] * dot[gg][0] + m[1][d] * dot[gg][1] + m[2][d] * dot[gg][2]);
         }
         g.R[f * KS + ks] = acc;
       }
@@ -221,27 +232,29 @@ struct TiledTet {
     // the current task's traces are formed.
     constexpr int T0 = ALIAS ? 0 : NF;
     constexpr int NT = (2 * NF - T0 + NW - 1) / NW;  // tasks of a wave, at most
-    auto load = [&](int j, double* v) {
+    const IT* Uin = (const IT*)a.Uin;
+    const CT* tab = table(a);
+    auto load = [&](int j, CT* v) {
       const int task = T0 + wave + NW * j;
       if (task >= 2 * NF) return;
       const int f = task < NF ? task : task - NF;
       const int src = task < NF ? es : nb;
 #pragma unroll
-      for (int k = 0; k < NP; k++) v[k] = a.Uin[(long)(f * NP + k) * E + src];
+      for (int k = 0; k < NP; k++) v[k] = CT(Uin[(long)(f * NP + k) * E + src]);
     };
-    double vb[2][NP];
+    CT vb[2][NP];
     load(0, vb[0]);
 #pragma unroll
     for (int j = 0; j < NT; j++) {
       if (j + 1 < NT) load(j + 1, vb[(j + 1) & 1]);
       const int task = T0 + wave + NW * j;
       if (task >= 2 * NF) break;
-      double* v = vb[j & 1];
+      CT* v = vb[j & 1];
       if (task < NF) {
         const int f = task;
-        const double* tn = a.tab + TetBlock::off_phif(P) + l * NQF * NP;
+        const CT* tn = tab + TetBlock::off_phif(P) + l * NQF * NP;
         for (int q = 0; q < NQF; q++) {
-          double acc = 0.0;
+          CT acc = CT(0);
 #pragma unroll
           for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
           flux_row(s, q * NF + f)[t] = acc;
@@ -250,24 +263,24 @@ struct TiledTet {
       }
       const int f = task - NF;
 #pragma unroll
-      for (int k = 0; k < NP; k++) v[k] = inner ? v[k] : 0.0;
+      for (int k = 0; k < NP; k++) v[k] = inner ? v[k] : CT(0);
       const int key = inner ? ((link >> 2) & 7) * 4 + (link & 3) : 0;  // perm * 4 + neighbour face
       const int key0 = wave_uniform(key);
       if (wave_all(key == key0)) {
         // the usual case (elements are grouped by link signature): one table
         // for the wave, read through the scalar cache
-        const double* tn = a.tab + TetBlock::off_phin(P) + ((key0 & 3) * 6 + (key0 >> 2)) * NQF * NP;
+        const CT* tn = tab + TetBlock::off_phin(P) + ((key0 & 3) * 6 + (key0 >> 2)) * NQF * NP;
         for (int q = 0; q < NQF; q++) {
-          double acc = 0.0;
+          CT acc = CT(0);
 #pragma unroll
           for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
           nb_row(s, q * NF + f)[t] = acc;
         }
       } else {
-        const double* tn = a.tab + TetBlock::off_phin(P) + ((key & 3) * 6 + (key >> 2)) * NQF * NP;
+        const CT* tn = tab + TetBlock::off_phin(P) + ((key & 3) * 6 + (key >> 2)) * NQF * NP;
 #pragma unroll 1
         for (int q = 0; q < NQF; q++) {
-          double acc = 0.0;
+          CT acc = CT(0);
           for (int k = 0; k < NP; k++) acc += tn[q * NP + k] * v[k];
           nb_row(s, q * NF + f)[t] = acc;
         }
@@ -278,22 +291,23 @@ struct TiledTet {
   // face l points: flux times area and weight, over the neighbour traces
   DG_HD static void phaseC2(const TetArgs& a, Smem& s, int tid, int tile, int l) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const double* tb = a.tab;
+    const CT* tb = table(a);
     const int link = s.Ls[l][t];
-    double m[3][3], Nr[3], N[3];
+    CT m[3][3], N[3];
+    double Nr[3];
     load_mg(a, tile, t, m);
     tet_ref_normal(l, Nr);
-    for (int d = 0; d < 3; d++) N[d] = m[0][d] * Nr[0] + m[1][d] * Nr[1] + m[2][d] * Nr[2];
-    const double area = dg_sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
-    const double n[3] = {N[0] / area, N[1] / area, N[2] / area};
+    for (int d = 0; d < 3; d++) N[d] = m[0][d] * CT(Nr[0]) + m[1][d] * CT(Nr[1]) + m[2][d] * CT(Nr[2]);
+    const CT area = dg_sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
+    const CT n[3] = {N[0] / area, N[1] / area, N[2] / area};
     for (int q = wave; q < NQF; q += NW) {
-      double uL[NF], uR[NF], Fh[NF];
+      CT uL[NF], uR[NF], Fh[NF];
       if constexpr (ALIAS) {
         for (int f = 0; f < NF; f++) uL[f] = s.W.WL[q * NF + f][t];
       } else {
-        const double* to = tb + TetBlock::off_phif(P) + (l * NQF + q) * NP;
+        const CT* to = tb + TetBlock::off_phif(P) + (l * NQF + q) * NP;
         for (int f = 0; f < NF; f++) {
-          double acc = 0.0;
+          CT acc = CT(0);
 #pragma unroll
           for (int k = 0; k < NP; k++) acc += to[k] * s.Us[f * NP + k][t];
           uL[f] = acc;
@@ -302,23 +316,23 @@ struct TiledTet {
       for (int f = 0; f < NF; f++) uR[f] = nb_row(s, q * NF + f)[t];
       if (link < 0) LAW::ghost(a.law, -link, n, uL, uR);
       LAW::riemann(a.law, n, uL, uR, Fh);
-      const double sw = area * tb[TetBlock::off_wf(P) + q];
+      const CT sw = area * tb[TetBlock::off_wf(P) + q];
       for (int f = 0; f < NF; f++) flux_row(s, q * NF + f)[t] = Fh[f] * sw;
     }
   }
 
   DG_HD static void phaseLift(const TetArgs& a, Smem& s, int tid, int l, Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const double* tb = a.tab + TetBlock::off_phif(P) + l * NQF * NP;
+    const CT* tb = table(a) + TetBlock::off_phif(P) + l * NQF * NP;
     for (int q = 0; q < NQF; q++) {
-      double Fq[NF];
+      CT Fq[NF];
 #pragma unroll
       for (int f = 0; f < NF; f++) Fq[f] = flux_row(s, q * NF + f)[t];
 #pragma unroll
       for (int ks = 0; ks < KS; ks++) {
         const int k = kmode(wave, ks);
         if (k < NP) {
-          const double phi = tb[q * NP + k];
+          const CT phi = tb[q * NP + k];
 #pragma unroll
           for (int f = 0; f < NF; f++) g.R[f * KS + ks] -= phi * Fq[f];
         }
@@ -342,23 +356,26 @@ struct TiledTet {
         const int k = kmode(wave, ks);
         if (k >= NP) continue;
         const long i = (long)(f * NP + k) * E + e;
-        const double r = g.R[f * KS + ks];
+        const double r = double(g.R[f * KS + ks]);
         if (a.scheme == MODE_RESIDUAL) {
           a.R[i] = r;
           continue;
         }
         double accv = reads_acc ? a.acc[i] : 0.0;
-        const double o = stage_value_hd(sc, r * dtj, a.Un[i], reads_input ? a.Uin[i] : 0.0, &accv, g.rsum);
+        const double o = stage_value_hd(sc, r * dtj, a.Un[i], reads_input ? double(((const IT*)a.Uin)[i]) : 0.0,
+                                        &accv, g.rsum);
         if (sc >= 3 * 4 && sc < 3 * 4 + 3) a.acc[i] = accv;
-        a.Uout[i] = o;
+        ((OT*)a.Uout)[i] = OT(o);
       }
   }
 };
 
 // shape: 0 for 4 waves per tile, 1 for 8 (traces over the coefficient rows);
-// 2 for 8, 3 for 16 (coefficient rows kept)
+// 2 for 8, 3 for 16 (coefficient rows kept).  prec: 0 FP64; 1..3 the FP32 form
+// (1: FP64 input, FP32 output; 2: FP32 both; 3: FP32 input, FP64 output).
 constexpr int TET_SHAPES = 4;
-int launch_tet_stage(const TetArgs& a, int shape, hipStream_t s);
+int launch_tet_stage(const TetArgs& a, int shape, int prec, hipStream_t s);
+int launch_tet_stage32(const TetArgs& a, int shape, int prec, hipStream_t s);
 int tet_tiles(int E);
 
 }  // namespace dgp

@@ -1,70 +1,14 @@
-// HIP kernel wrapper of the tetrahedron stage (phases in dg_tet.h).
-#include <hip/hip_runtime.h>
-
-#include "dg_tet.h"
+// HIP kernel of the tetrahedron stage, FP64 form (phases in dg_tet.h, kernel
+// and launcher in dg_tet_kern.h); dispatch of every precision form.
+#include "dg_tet_kern.h"
 
 namespace dgp {
 
-__device__ __forceinline__ int xcd_tile_tet(int b, int nb) {
-  int full = (nb / 8) * 8;
-  if (b >= full) return b;
-  return (b % 8) * (nb / 8) + b / 8;
-}
-
-template <int P, int NW, bool ALIAS>
-__global__ __launch_bounds__(NW * 64) void tet_stage_tiled(TetArgs a) {
-  using K = TiledTet<P, NW, ALIAS>;
-  __shared__ typename K::Smem sm;
-  typename K::Regs g;
-  const int tile = xcd_tile_tet(blockIdx.x, gridDim.x);
-  const int tid = threadIdx.x;
-  K::phaseA(a, sm, tid, tile);
-  __syncthreads();
-  K::phaseB(a, sm, tid, tile, g);
-#pragma unroll 1
-  for (int l = 0; l < 4; l++) {
-    __syncthreads();
-    K::phaseC1(a, sm, tid, tile, l);
-    __syncthreads();
-    K::phaseC2(a, sm, tid, tile, l);
-    __syncthreads();
-    K::phaseLift(a, sm, tid, l, g);
-  }
-  K::phaseD(a, tid, tile, g);
-  if (a.resid_part) {
-    __syncthreads();
-    sm.red[tid] = g.rsum;
-    __syncthreads();
-    for (int s = K::THREADS / 2; s > 0; s >>= 1) {
-      if (tid < s) sm.red[tid] += sm.red[tid + s];
-      __syncthreads();
-    }
-    if (tid == 0) a.resid_part[tile] = sm.red[0];
-  }
-}
-
 int tet_tiles(int E) { return (E + TILE_ELEMS - 1) / TILE_ELEMS; }
 
-template <int NW, bool ALIAS>
-static int launch_tet_nw(const TetArgs& a, hipStream_t s) {
-  dim3 grid(tet_tiles(a.E)), block(NW * 64);
-  switch (a.P) {
-    case 1: hipLaunchKernelGGL((tet_stage_tiled<1, NW, ALIAS>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((tet_stage_tiled<2, NW, ALIAS>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((tet_stage_tiled<3, NW, ALIAS>), grid, block, 0, s, a); break;
-    default: return -1;
-  }
-  return hipGetLastError() == hipSuccess ? 0 : -2;
-}
-
-int launch_tet_stage(const TetArgs& a, int shape, hipStream_t s) {
-  switch (shape) {
-    case 0: return launch_tet_nw<4, true>(a, s);
-    case 1: return launch_tet_nw<8, true>(a, s);
-    case 2: return launch_tet_nw<8, false>(a, s);
-    case 3: return launch_tet_nw<16, false>(a, s);
-  }
-  return -1;
+int launch_tet_stage(const TetArgs& a, int shape, int prec, hipStream_t s) {
+  if (prec == 0) return launch_tet_shape<double, double, double>(a, shape, s);
+  return launch_tet_stage32(a, shape, prec, s);
 }
 
 }  // namespace dgp

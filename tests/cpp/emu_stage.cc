@@ -636,9 +636,9 @@ extern "C" int emu_sensor(const dg_mesh_desc* mesh, const dg_law_desc* law, int 
 
 namespace {
 
-template <int P, int NW, bool ALIAS>
+template <int P, int NW, bool ALIAS, typename CT = double, typename IT = double, typename OT = double>
 void tet_tiles_run(const TetArgs& a, double* rsum) {
-  using K = TiledTet<P, NW, ALIAS>;
+  using K = TiledTet<P, NW, ALIAS, CT, IT, OT>;
   constexpr int TH = K::THREADS;
   std::unique_ptr<typename K::Smem> sm(new typename K::Smem());
   std::vector<typename K::Regs> g(TH);
@@ -669,6 +669,36 @@ int tet_run_nw(const TetArgs& a, double* rsum) {
   return -1;
 }
 
+template <int NW, bool ALIAS, typename CT, typename IT, typename OT>
+int tet_run_nw_form(const TetArgs& a, double* rsum) {
+  switch (a.P) {
+    case 1: tet_tiles_run<1, NW, ALIAS, CT, IT, OT>(a, rsum); return 0;
+    case 2: tet_tiles_run<2, NW, ALIAS, CT, IT, OT>(a, rsum); return 0;
+    case 3: tet_tiles_run<3, NW, ALIAS, CT, IT, OT>(a, rsum); return 0;
+  }
+  return -1;
+}
+
+// the FP32 form (prec 1..3 as the device launcher's) for one shape
+template <typename IT, typename OT>
+int tet_run32_io(const TetArgs& a, int shape, double* rsum) {
+  switch (shape) {
+    case 0: return tet_run_nw_form<4, true, float, IT, OT>(a, rsum);
+    case 1: return tet_run_nw_form<8, true, float, IT, OT>(a, rsum);
+    case 2: return tet_run_nw_form<8, false, float, IT, OT>(a, rsum);
+    case 3: return tet_run_nw_form<16, false, float, IT, OT>(a, rsum);
+  }
+  return -1;
+}
+int tet_run32(const TetArgs& a, int shape, int prec, double* rsum) {
+  switch (prec) {
+    case 1: return tet_run32_io<double, float>(a, shape, rsum);
+    case 2: return tet_run32_io<float, float>(a, shape, rsum);
+    case 3: return tet_run32_io<float, double>(a, shape, rsum);
+  }
+  return -1;
+}
+
 // shape as the device launcher's
 int tet_run(const TetArgs& a, int shape, double* rsum) {
   switch (shape) {
@@ -683,6 +713,7 @@ int tet_run(const TetArgs& a, int shape, double* rsum) {
 struct TetEmu {
   TetSetup S;
   std::vector<double> Un, acc, R, A, B;
+  std::vector<float> tabf, A32, B32;
   void to_soa(const double* U, double* dst) const {
     const int BS = S.NF * S.NP;
     for (int i = 0; i < S.E; i++)
@@ -698,6 +729,7 @@ struct TetEmu {
     a.E = S.E;
     a.P = S.P;
     a.tab = S.tab.data();
+    a.tabf = tabf.data();
     a.mg = S.mg.data();
     a.detj = S.detj.data();
     a.link = S.link.data();
@@ -718,6 +750,9 @@ int tet_make(TetEmu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int or
   m.R.assign(n, 0.0);
   m.A.assign(n, 0.0);
   m.B.assign(n, 0.0);
+  m.A32.assign(n, 0.0f);
+  m.B32.assign(n, 0.0f);
+  m.tabf.assign(m.S.tab.begin(), m.S.tab.end());
   return 0;
 }
 
@@ -765,6 +800,35 @@ extern "C" int emu_tet_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, i
     if (ns == 1) m.Un = m.A;
   }
   if (resid) *resid = r;
+  m.from_soa(m.Un.data(), U);
+  return 0;
+}
+
+// RK steps of the tetrahedron kernel's FP32 form with the library's buffer
+// sequence: stage 1 reads the FP64 step-start state and writes an FP32 copy,
+// the middle stages run FP32 to FP32, the last writes the FP64 state
+extern "C" int emu_tet_steps32(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int shape, int scheme,
+                               double dt, int nsteps, double* U) {
+  TetEmu m;
+  if (tet_make(m, mesh, law, order)) return -1;
+  m.to_soa(U, m.Un.data());
+  const int ns = scheme == 1 ? 2 : scheme == 2 ? 3 : 4;
+  if (scheme == 0) return -3;
+  for (int st = 0; st < nsteps; st++) {
+    TetArgs a = m.args();
+    a.Un = m.Un.data();
+    a.acc = m.acc.data();
+    a.dt = dt;
+    a.scheme = scheme;
+    for (int s = 0; s < ns; s++) {
+      a.stage = s;
+      const bool first = s == 0, last = s == ns - 1;
+      a.Uin = first ? (const void*)m.Un.data() : (const void*)((s % 2) ? m.A32.data() : m.B32.data());
+      a.Uout = last ? (void*)m.Un.data() : (void*)((s % 2) ? m.B32.data() : m.A32.data());
+      double rs = 0;
+      if (tet_run32(a, shape, first ? 1 : (last ? 3 : 2), &rs)) return -2;
+    }
+  }
   m.from_soa(m.Un.data(), U);
   return 0;
 }

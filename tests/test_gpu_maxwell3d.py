@@ -71,3 +71,29 @@ def test_tet_rejects_what_it_does_not_build():
     g = dgpar.DGAnalysis(m, dgpar.MaxwellLaw(), 2)
     with pytest.raises(dgpar.DGError):
         g.set_limiter(True)
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_tet_fp32_form(p, shape):
+    """the FP32 form on tetrahedra (FP32 arithmetic, tables and stage copies;
+    FP64 step-start state, accumulator and update) against the FP64 oracle:
+    the residual within 5e-5 and 3 RK4 steps within 2e-6 (FP32 rounding)"""
+    import dgpar
+    announce("tet FP32 form P%d shape %d" % (p, shape))
+    m = dgpar.mesh_tet_structured(3, 3, 3)
+    o = po.Oracle(3, m.vids, m.xyz, p, po.LAW_MAXWELL3D, po.FLUX_UPWIND)
+    g = dgpar.DGAnalysis(m, dgpar.MaxwellLaw(po.FLUX_UPWIND), p, precision=dgpar.PREC_FP32)
+    g.set_schedule(shape=shape)
+    U = np.random.default_rng(p).standard_normal(o.ndof)
+    g.set_state(U)
+    assert rel(g.residual(), o.residual(U)[0]) < 5e-5
+    g.project(3)
+    U = o.project(3)
+    dt = o.dt(U, 0.1)
+    t = 0.0
+    for _ in range(3):
+        U, _ = o.step(U, po.RK4, t, dt)
+        t += dt
+    g.advanceInTime(0.0, dt, scheme=po.RK4, nsteps=3)
+    assert rel(g.get_state(), U) < 2e-6

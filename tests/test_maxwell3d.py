@@ -141,3 +141,33 @@ def test_emulated_tet_rk4_steps(shape):
                                 ctypes.byref(r)) == 0
     assert np.abs(Ue - Uo).max() <= 1e-12 * np.abs(Uo).max()
     assert abs(r.value - ro) <= 1e-10 * abs(ro)
+
+
+if LIBEMU is not None:
+    LIBEMU.emu_tet_steps32.argtypes = [p_md, p_ld, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int, PD]
+
+
+@emu
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_emulated_tet_fp32_form_rk4_steps(p, shape):
+    """the FP32 form (FP32 arithmetic, tables and stage copies; FP64 step-start
+    state, accumulator and update): 3 RK4 steps of the cavity mode against the
+    FP64 oracle within FP32 rounding (2e-6 of the field scale), and closer to
+    the oracle than the step itself moves the state"""
+    vids, xyz, o = oracle(2, p, po.FLUX_UPWIND)
+    U = o.project(3)
+    dt = o.dt(U, 0.1)
+    Uo, t = U, 0.0
+    for _ in range(3):
+        Uo, _ = o.step(Uo, po.RK4, t, dt)
+        t += dt
+    md = dgpar.Mesh(3, vids, xyz).desc()
+    ld = dgpar.MaxwellLaw(po.FLUX_UPWIND).desc()
+    Ue = U.copy()
+    assert LIBEMU.emu_tet_steps32(ctypes.byref(md), ctypes.byref(ld), p, shape, po.RK4, dt, 3,
+                                  Ue.ctypes.data_as(PD)) == 0
+    err = np.abs(Ue - Uo).max() / np.abs(Uo).max()
+    assert err <= 2e-6
+    assert err < 1e-2 * np.abs(Uo - U).max() / np.abs(Uo).max()
