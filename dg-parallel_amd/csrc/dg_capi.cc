@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -562,6 +563,7 @@ static StageArgs base_args(dg_ctx* c) {
   a.law = c->lp;
   a.Un = c->Un;
   a.acc = c->acc;
+  a.pvel = c->pvel;
   return a;
 }
 
@@ -601,8 +603,33 @@ static int enqueue_dt(dg_ctx* c, double cfl, double* part) {
   a.law = c->lp;
   a.cfl = cfl;
   a.part = part;
+  a.pvel = c->pvel;
   if (launch_tri_dt(a, c->law, c->stream)) return -1;
   return tri_dt_blocks(c->E);
+}
+
+// ScalarLaw's velocity FieldEvaluator (ScalarLaw.cc:29-97): the caller's field
+// evaluated once, in FP64 on the host, at every volume point and every face
+// point of every element (each element at its own image of a periodic face's
+// point), uploaded as [NQV + 3 NQF][2][E] in the internal element order
+dg_status dg_set_velocity_field(dg_ctx* c, dg_field_fn fn, void* user) {
+  if (!ready(c) || !fn) return fail(c, DG_ERR_ARG, "dg_set_velocity_field: bad argument");
+  if (c->dim != 2 || c->law != DG_LAW_SCALAR)
+    return fail(c, DG_ERR_UNSUPPORTED, "a velocity field serves the scalar law on triangles");
+  if (c->mixed || c->kernel != 0)
+    return fail(c, DG_ERR_UNSUPPORTED, "a velocity field runs the tiled kernel of uniform orders only");
+  std::vector<double> h;
+  if (!tri_point_field(c->xyz, c->perm, c->P, 2, fn, user, h))
+    return fail(c, DG_ERR_NONFINITE, "the velocity field is not finite at a quadrature point");
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (!c->pvel) {
+    dg_status s = dalloc(c, &c->pvel, h.size());
+    if (s) return s;
+  }
+  HIPCHECK(c, hipMemcpy(c->pvel, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured steps change
+  c->gexec = nullptr;
+  return DG_OK;
 }
 
 dg_status dg_compute_dt(dg_ctx* c, double cfl, double* dt_out) {
@@ -1148,6 +1175,7 @@ dg_status dg_select_orders(dg_ctx* c, const dg_sensor_opts* o, int apply, int32_
   if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "built for triangles");
   if (o->pmin < 1 || o->pmax > MAXP_TRI || o->pmin > o->pmax) return fail(c, DG_ERR_ARG, "orders must lie in 1..%d", MAXP_TRI);
   if (apply && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "order changes run in the FP64 form only");
+  if (apply && c->pvel) return fail(c, DG_ERR_UNSUPPORTED, "a velocity field runs uniform orders only");
   HIPCHECK(c, hipSetDevice(c->device));
   SensorArgs a{};
   a.E = c->E;
@@ -1214,7 +1242,8 @@ static AdjArgs adj_args(dg_ctx* c) {
 
 dg_status dg_residual_jvp(dg_ctx* c, const double* U, const double* dU, double* dR) {
   if (!ready(c) || !U || !dU || !dR) return fail(c, DG_ERR_ARG, "dg_residual_jvp: bad argument");
-  if (c->mixed || c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform-order triangles");
+  if (c->mixed || c->dim != 2 || c->pvel)
+    return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform-order triangles (constant velocity)");
   HIPCHECK(c, hipSetDevice(c->device));
   dg_status s = adj_buffers(c);
   if (s || (s = upload_soa(c, U, c->lam)) || (s = upload_soa(c, dU, c->za))) return s;
@@ -1228,7 +1257,8 @@ dg_status dg_residual_jvp(dg_ctx* c, const double* U, const double* dU, double* 
 
 dg_status dg_residual_vjp(dg_ctx* c, const double* U, const double* z, double* w) {
   if (!ready(c) || !U || !z || !w) return fail(c, DG_ERR_ARG, "dg_residual_vjp: bad argument");
-  if (c->mixed || c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform-order triangles");
+  if (c->mixed || c->dim != 2 || c->pvel)
+    return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform-order triangles (constant velocity)");
   HIPCHECK(c, hipSetDevice(c->device));
   dg_status s = adj_buffers(c);
   if (s || (s = upload_soa(c, U, c->lam)) || (s = upload_soa(c, z, c->za))) return s;
@@ -1283,7 +1313,7 @@ dg_status dg_adjoint_forward(dg_ctx* c, double dt, int nsteps, int ckpt_every, c
   if (!ready(c) || !(dt > 0) || nsteps <= 0 || ckpt_every <= 0)
     return fail(c, DG_ERR_ARG, "dg_adjoint_forward: bad argument");
   if (c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the FP64 form");
-  if (c->mixed || c->lim_on || c->dim != 2)
+  if (c->mixed || c->lim_on || c->dim != 2 || c->pvel)
     return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the unlimited uniform-order triangle residual");
   HIPCHECK(c, hipSetDevice(c->device));
   c->adj_ready = false;

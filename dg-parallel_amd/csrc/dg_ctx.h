@@ -73,6 +73,8 @@ struct dg_ctx {
   // flags of the recovery passes of the last dg_advance (or of dg_recover), and
   // two status words: [0] the recovery passes, [1] the check after the steps
   int rec_on = 0;
+  // the scalar law's velocity at every quadrature point (dg_set_velocity_field), or null
+  double* pvel = nullptr;
   uint8_t* npflags = nullptr;
   unsigned* status = nullptr;
   std::vector<void*> allocs;

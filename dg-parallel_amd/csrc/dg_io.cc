@@ -171,6 +171,7 @@ dg_status dg_read_restart(dg_ctx* c, const char* path, double* t_out, int* iter_
   }
   // the face records that follow are the mesh's, which the context already has
   if (ord != c->m_order) {
+    if (c->pvel) return fail(c, DG_ERR_UNSUPPORTED, "%s: orders differ and a velocity field runs one order", path);
     if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "%s: orders differ from the context's (tetrahedra: one order)", path);
     if (c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "%s: order changes run in the FP64 form only", path);
     dg_status s = ctx_rebucket(c, ord);

@@ -38,6 +38,9 @@ struct StageArgs {
   int scheme;            // DG_RK_* or MODE_RESIDUAL
   int stage;
   double* resid_part;    // optional per-block partial sums of the step residual norm
+  // the scalar law's velocity at the quadrature points (a velocity field), FP64,
+  // [NQV + 3 NQF][2][E]: volume points, then face l point q at NQV + l NQF + q
+  const double* pvel;
 };
 
 // variant: 0 = FP64 throughout; 1 = FP32 arithmetic, FP64 input, FP32 output;
@@ -64,6 +67,7 @@ struct DtArgs {
   LawParams law;
   double cfl;
   double* part;          // per-block minima
+  const double* pvel;    // the velocity table of StageArgs (or null)
 };
 int launch_tri_dt(const DtArgs& a, int law, hipStream_t s);
 int tri_dt_blocks(int E);

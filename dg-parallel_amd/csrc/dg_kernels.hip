@@ -332,19 +332,25 @@ __global__ __launch_bounds__(BLOCK) void tri_dt_kernel(DtArgs a) {
     double mean[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) mean[f] = U[f][0] * a.vtx[0];
-    double lam = LAW::maxev(a.law, mean);
+    double lam;
+    if constexpr (point_vel<LAW>::value) {
+      // ScalarLaw::maximumEigenValue at the cell's first volume point (DGCell.cc:1165-1186)
+      lam = fabs(a.pvel[e]) + fabs(a.pvel[(long)a.E + e]);
+    } else {
+      lam = LAW::maxev(a.law, mean);
 #pragma unroll
-    for (int v = 0; v < 3; v++) {
-      double val[NF];
+      for (int v = 0; v < 3; v++) {
+        double val[NF];
 #pragma unroll
-      for (int f = 0; f < NF; f++) {
-        double s = 0;
+        for (int f = 0; f < NF; f++) {
+          double s = 0;
 #pragma unroll
-        for (int k = 0; k < NP; k++) s += U[f][k] * a.vtx[(1 + v) * NP + k];
-        val[f] = s;
+          for (int k = 0; k < NP; k++) s += U[f][k] * a.vtx[(1 + v) * NP + k];
+          val[f] = s;
+        }
+        double b = LAW::maxev(a.law, val);
+        lam = b > lam ? b : lam;
       }
-      double b = LAW::maxev(a.law, val);
-      lam = b > lam ? b : lam;
     }
     dt = a.el.size[e] * a.cfl / (lam * (2. * P + 1.0));
   }
@@ -374,6 +380,7 @@ static int launch_dt_law(const DtArgs& a, hipStream_t s) {
 }
 
 int launch_tri_dt(const DtArgs& a, int law, hipStream_t s) {
+  if (law == LAW_SCALAR && a.pvel) return launch_dt_law<ScalarFieldLawD>(a, s);
   if (law == LAW_SCALAR) return launch_dt_law<ScalarLawD>(a, s);
   if (law == LAW_EULER2D) return launch_dt_law<EulerLawD>(a, s);
   if (law == LAW_MAXWELL2D) return launch_dt_law<Maxwell2dLawD>(a, s);

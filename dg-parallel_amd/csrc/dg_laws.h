@@ -112,6 +112,34 @@ struct ScalarLawD {
   }
 };
 
+// ScalarLaw with a velocity FieldEvaluator (ScalarLaw.cc:29-97, velocityField->eval
+// at every point): the kernels read the velocity of each quadrature point from a
+// table the host filled by evaluating the caller's field there (POINT_VEL), and
+// pass it to flux_at / riemann_at; the far field and the ghost are ScalarLawD's
+struct ScalarFieldLawD : ScalarLawD {
+  static constexpr bool POINT_VEL = true;
+  template <typename T>
+  DG_HD static void flux_at(const T* v, const T* q, T F[][DIM]) {
+    F[0][0] = v[0] * q[0];
+    F[0][1] = v[1] * q[0];
+  }
+  // upwind by v.n at the point: ScalarLaw::riemannSolver; with the far-field
+  // ghost on boundary faces it is ScalarLaw::boundaryFlux (inflow takes the far
+  // field, outflow the interior)
+  template <typename T>
+  DG_HD static void riemann_at(const T* v, const T* n, const T* uL, const T* uR, T* out) {
+    T vn = n[0] * v[0] + n[1] * v[1];
+    T u = vn > T(0) ? uL[0] : uR[0];
+    out[0] = (v[0] * u) * n[0] + (v[1] * u) * n[1];
+  }
+};
+
+// true for laws whose flux needs the velocity of the point (ScalarFieldLawD)
+template <class L, class = void>
+struct point_vel : std::false_type {};
+template <class L>
+struct point_vel<L, std::void_t<decltype(L::POINT_VEL)>> : std::bool_constant<L::POINT_VEL> {};
+
 // ---------------------------------------------------------------------------
 struct EulerLawD {
   static constexpr int NF = 4;

@@ -479,4 +479,39 @@ dg_status build_tet_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
   return DG_OK;
 }
 
+bool tri_point_field(const std::vector<double>& xyz, const std::vector<int32_t>& perm, int P, int nval,
+                     dg_field_fn fn, void* user, std::vector<double>& out) {
+  const int E = (int)perm.size(), NQ = nqv_tri(P), NQF = nqf_tri(P), NPT = NQ + 3 * NQF;
+  auto vr = tri_rule(2 * P);
+  auto gl = gl_rule(NQF);
+  std::vector<double> xi((size_t)NPT * 2);
+  for (int q = 0; q < NQ; q++) {
+    xi[2 * q] = vr[q].x[0];
+    xi[2 * q + 1] = vr[q].x[1];
+  }
+  for (int l = 0; l < 3; l++)
+    for (int q = 0; q < NQF; q++) {
+      double r[3];
+      edge_point(l, gl[q].x[0], r);
+      xi[2 * (NQ + l * NQF + q)] = r[0];
+      xi[2 * (NQ + l * NQF + q) + 1] = r[1];
+    }
+  out.assign((size_t)NPT * nval * E, 0.0);
+  for (int i = 0; i < E; i++) {
+    const double* X = &xyz[(size_t)perm[i] * 6];
+    for (int q = 0; q < NPT; q++) {
+      const double a = xi[2 * q], b = xi[2 * q + 1];
+      const double x[3] = {X[0] + a * (X[2] - X[0]) + b * (X[4] - X[0]), X[1] + a * (X[3] - X[1]) + b * (X[5] - X[1]),
+                           0.0};
+      double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      fn(x, 0.0, v, user);
+      for (int k = 0; k < nval; k++) {
+        if (!std::isfinite(v[k])) return false;
+        out[((size_t)q * nval + k) * E + i] = v[k];
+      }
+    }
+  }
+  return true;
+}
+
 }  // namespace dgp

@@ -85,6 +85,14 @@ struct TetSetup {
   std::vector<double> tab;
 };
 
+// a field (dg_field_fn, time 0) at every volume point and every face point of
+// every element: [NQV + 3 NQF][nval][E] in the internal order (element i is
+// caller element perm[i], corners xyz [E][3][2] in caller order), face l point q
+// at row NQV + l NQF + q (the points of the order-P trace tables); false when a
+// value is not finite
+bool tri_point_field(const std::vector<double>& xyz, const std::vector<int32_t>& perm, int P, int nval,
+                     dg_field_fn fn, void* user, std::vector<double>& out);
+
 dg_status build_tet_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TetSetup& S,
                           std::string& err);
 

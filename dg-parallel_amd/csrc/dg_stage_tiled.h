@@ -174,6 +174,7 @@ struct TiledStage {
   struct Regs {
     CT R[FPW * KPW];
     double rsum;
+    int es;  // the lane's element (0 on padding lanes)
     CT nv[CPW][NP];
     int lk[CPW];
     typename std::conditional<MF, MfmaAccOn, MfmaAccOff>::type m;
@@ -196,6 +197,7 @@ struct TiledStage {
     const bool valid = e < E;
     const int es = valid ? e : 0;  // padding lanes load element 0 (in bounds) and discard it
     const IT* Uin = (const IT*)a.Uin;
+    g.es = es;
     // Every global load of the phase is issued, unconditionally and in this
     // order, before any of them is used: the links (the neighbour addresses need
     // only them; the memory counter is in order), the tile's rows and |J| J^-1,
@@ -279,7 +281,12 @@ struct TiledStage {
           u[f] = acc;
         }
         CT F[NF][2];
-        LAW::flux(a.law, u, F);
+        if constexpr (point_vel<LAW>::value) {
+          const CT v[2] = {CT(a.pvel[(long)(q * 2 + 0) * a.E + g.es]), CT(a.pvel[(long)(q * 2 + 1) * a.E + g.es])};
+          LAW::flux_at(v, u, F);
+        } else {
+          LAW::flux(a.law, u, F);
+        }
         const CT w = tb[TriBlock::off_wv(P) + q];
         const CT m00 = s.Ms[0][t], m01 = s.Ms[1][t], m10 = s.Ms[2][t], m11 = s.Ms[3][t];
 #pragma unroll
@@ -392,7 +399,7 @@ struct TiledStage {
   }
 
   // ---- C2: numerical flux at the face points ----------------------------------
-  DG_HD static void phaseC2(const StageArgs& a, Smem& s, int tid, const Regs&) {
+  DG_HD static void phaseC2(const StageArgs& a, Smem& s, int tid, const Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const CT* tb = table(a);
 #pragma unroll 1
@@ -414,7 +421,14 @@ struct TiledStage {
         }
         // boundary faces: the law's ghost state, then the same two-point flux (one
         // inlined flux for interior and boundary lanes alike)
-        face_flux<LAW>(a.law, link, n, uL, uR, Fh);
+        if constexpr (point_vel<LAW>::value) {
+          const long r = (long)(NQ + l * NQF + q) * 2;
+          const CT v[2] = {CT(a.pvel[r * a.E + g.es]), CT(a.pvel[(r + 1) * a.E + g.es])};
+          if (link < 0) LAW::ghost(a.law, -link, n, uL, uR);
+          LAW::riemann_at(v, n, uL, uR, Fh);
+        } else {
+          face_flux<LAW>(a.law, link, n, uL, uR, Fh);
+        }
         // |J_face| w_q = (len / 2) w_q; the table holds w_q / 2
         const CT sw = len * tb[TriBlock::off_wf(P) + q];
 #pragma unroll

@@ -124,6 +124,7 @@ static int launch_tiled_mf_law(const StageArgs& a, int prec, hipStream_t s) {
 
 template <int NW>
 int launch_tiled_nw(const StageArgs& a, int law, int prec, hipStream_t s) {
+  if (law == LAW_SCALAR && a.pvel) return launch_tiled_law<ScalarFieldLawD, NW>(a, prec, s);
   if (law == LAW_SCALAR) return launch_tiled_law<ScalarLawD, NW>(a, prec, s);
   if (law == LAW_EULER2D && a.law.char_bc)
     return a.law.flux == FLUX_EXACT ? launch_tiled_law<EulerExactCharLawD, NW>(a, prec, s)

@@ -80,7 +80,7 @@ void basis(int dim, int p, const double* xi, double* phi, double* dphi) {
   }
 }
 
-static void edge_point(int l, double s, double* xi) {
+void edge_point(int l, double s, double* xi) {
   static const double V[3][2] = {{0, 0}, {1, 0}, {0, 1}};
   double a = 0.5 * (1.0 + s), b = 0.5 * (1.0 - s);
   const double* A = V[l];

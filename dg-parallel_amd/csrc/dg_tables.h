@@ -29,6 +29,10 @@ std::vector<QPoint> tri_rule(int degree);
 std::vector<QPoint> tet_rule(int degree);
 std::vector<QPoint> gl_rule(int npts);
 
+// reference coordinates of the point s in [-1, 1] of local edge l (v_l, v_{l+1}):
+// (1+s)/2 v_l + (1-s)/2 v_{l+1}, the face points of the trace tables
+void edge_point(int l, double s, double* xi);
+
 // phi[k], dphi[k*dim + d] of the orthonormal basis of order p
 void basis(int dim, int p, const double* xi, double* phi, double* dphi);
 

@@ -100,6 +100,10 @@ _p_u8 = ctypes.POINTER(ctypes.c_uint8)
 _set_limiter = _sig("dg_set_limiter", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double])
 _limit = _sig("dg_limit", ctypes.c_int, [_vp, _p_u8])
 _limiter_flags = _sig("dg_limiter_flags", ctypes.c_int, [_vp, _p_u8])
+# a FieldEvaluator (dg_field_fn): fn(x[3], t, val, user)
+FIELD_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                            ctypes.c_void_p)
+_set_velocity_field = _sig("dg_set_velocity_field", ctypes.c_int, [_vp, FIELD_FN, _vp])
 _set_recovery = _sig("dg_set_recovery", ctypes.c_int, [_vp, ctypes.c_int])
 _recover = _sig("dg_recover", ctypes.c_int, [_vp, ctypes.c_int, _p_u8])
 _recovery_flags = _sig("dg_recovery_flags", ctypes.c_int, [_vp, _p_u8])
@@ -454,6 +458,15 @@ class DGAnalysis:
         out = np.zeros(self.mesh_E, np.uint8)
         self._check(_limiter_flags(self._h, out.ctypes.data_as(_p_u8)))
         return out
+
+    def set_velocity_field(self, fn):
+        """ScalarLaw's velocity FieldEvaluator (ScalarLaw.cc:29-97): fn(x, y) ->
+        (vx, vy), evaluated once at every quadrature point (dg_set_velocity_field)"""
+        def cb(x, t, val, user):
+            v = fn(x[0], x[1])
+            val[0], val[1], val[2] = v[0], v[1], 0.0
+        cfn = FIELD_FN(cb)
+        self._check(_set_velocity_field(self._h, cfn, None))
 
     # ---- non-physical states (Euler, triangles) ----
     def set_recovery(self, enabled):

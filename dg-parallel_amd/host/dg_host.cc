@@ -32,22 +32,47 @@ ScalarLaw::ScalarLaw(FieldEvaluator* far, const double vel[3], FieldEvaluator* e
   for (int i = 0; i < 3; i++) vel_[i] = vel[i];
 }
 
-void ScalarLaw::Fi(const mPoint&, double* Q, mVector* flux) const {
-  for (int d = 0; d < 3; d++) flux[0](d) = vel_[d] * Q[0];
+ScalarLaw::ScalarLaw(FieldEvaluator* far, FieldEvaluator* velocity, FieldEvaluator* exact) {
+  farField = far;
+  exactField = exact;
+  velField_ = velocity;
+  for (int i = 0; i < 3; i++) vel_[i] = 0.0;
 }
 
-void ScalarLaw::riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* flux) const {
+// the velocity at a point: the field's at time 0 (ScalarLaw.cc:33, 73), or the constant
+void ScalarLaw::velocity(const mPoint& p, double* v) const {
+  if (velField_) {
+    v[0] = v[1] = v[2] = 0.0;
+    velField_->eval(p, 0.0, v);
+  } else {
+    for (int d = 0; d < 3; d++) v[d] = vel_[d];
+  }
+}
+
+void ScalarLaw::Fi(const mPoint& p, double* Q, mVector* flux) const {
+  double v[3];
+  velocity(p, v);
+  for (int d = 0; d < 3; d++) flux[0](d) = v[d] * Q[0];
+}
+
+void ScalarLaw::riemannSolver(mVector& n, const mPoint& p, double* uL, double* uR, double* flux) const {
+  double v[3];
+  velocity(p, v);
+  dgp::ScalarFieldLawD::riemann_at<double>(v, n.v, uL, uR, flux);
+}
+
+void ScalarLaw::boundaryFlux(mVector& n, int, const mPoint& p, double* Q, double* flux, double) const {
+  double v[3], g[1];
+  velocity(p, v);
   dgp::LawParams L = params_of(describe());
-  dgp::ScalarLawD::riemann<double>(L, n.v, uL, uR, flux);
+  dgp::ScalarLawD::ghost<double>(L, dgp::BC_FARFIELD, n.v, Q, g);
+  dgp::ScalarFieldLawD::riemann_at<double>(v, n.v, Q, g, flux);
 }
 
-void ScalarLaw::boundaryFlux(mVector& n, int, const mPoint&, double* Q, double* flux, double) const {
-  dgp::LawParams L = params_of(describe());
-  dgp::ScalarLawD::boundary<double>(L, dgp::BC_FARFIELD, n.v, Q, flux);
-}
-
-double ScalarLaw::maximumEigenValue(const double*, const mPoint&) const {
-  return std::fabs(vel_[0]) + std::fabs(vel_[1]) + std::fabs(vel_[2]);
+double ScalarLaw::maximumEigenValue(const double*, const mPoint& p) const {
+  double v[3];
+  velocity(p, v);
+  return std::fabs(v[0]) + std::fabs(v[1]) + std::fabs(v[2]);
 }
 
 dg_law_desc ScalarLaw::describe() const {
@@ -316,6 +341,11 @@ double TimeIntegrator::advanceInTime(double t, double dt) {
   return r;
 }
 
+// a FieldEvaluator behind the C ABI's dg_field_fn
+static void eval_field(const double* x, double t, double* val, void* user) {
+  static_cast<const FieldEvaluator*>(user)->eval(mPoint(x[0], x[1], x[2]), t, val);
+}
+
 DGAnalysis::DGAnalysis(const Mesh& mesh, ConservationLaw* law, DGLimiter* limiter, int order, int precision)
     : mesh_(mesh), law_(law), order_(order) {
   const dg_mesh_desc md = mesh.describe();
@@ -333,6 +363,7 @@ DGAnalysis::DGAnalysis(const Mesh& mesh, ConservationLaw* law, DGLimiter* limite
     throw DGError(s, msg);
   }
   if (limiter) check(dg_set_limiter(ctx_, 1, limiter->tvbM(), limiter->nu()));
+  if (FieldEvaluator* vf = law->velocityField()) check(dg_set_velocity_field(ctx_, eval_field, vf));
   // integrator by order (DGAnalysis.cc:239-242)
   if (order == 0)
     integrator_ = new ForwardEuler(this);

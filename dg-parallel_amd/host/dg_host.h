@@ -84,14 +84,23 @@ class ConservationLaw {
   virtual bool isPhysical(double* val) const { return true; }
   // the device-side description of this law (no counterpart in the reference)
   virtual dg_law_desc describe() const = 0;
+  // a velocity FieldEvaluator the device evaluates at its quadrature points
+  // (ScalarLaw's velocityField), or null
+  virtual FieldEvaluator* velocityField() const { return nullptr; }
 };
 
-// ScalarLaw (ScalarLaw.cc:19-97) with a constant velocity field
+// ScalarLaw (ScalarLaw.cc:19-97): the reference's constructor takes the velocity
+// as a FieldEvaluator (ScalarLaw(farField, velocityField, exactField)); the
+// second constructor takes a constant velocity
 class ScalarLaw : public ConservationLaw {
   double vel_[3];
+  FieldEvaluator* velField_ = nullptr;
+  void velocity(const mPoint& p, double* v) const;
 
  public:
+  ScalarLaw(FieldEvaluator* far, FieldEvaluator* velocity, FieldEvaluator* exact);
   ScalarLaw(FieldEvaluator* far, const double vel[3], FieldEvaluator* exact = nullptr);
+  FieldEvaluator* velocityField() const override { return velField_; }
   int getNbFields() const override { return 1; }
   void Fi(const mPoint&, double* Q, mVector* flux) const override;
   void riemannSolver(mVector& n, const mPoint&, double* uL, double* uR, double* flux) const override;

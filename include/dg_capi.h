@@ -117,6 +117,24 @@ int dg_nfaces(const dg_ctx* ctx);
  * elemR; boundary faces carry elemR = -1.  Order: sorted by sorted vertex ids. */
 dg_status dg_faces(const dg_ctx* ctx, int32_t* out4, size_t n);
 
+/* A FieldEvaluator (FieldEvaluator.h:12, eval(const mPoint&, double t, double* val)):
+ * val receives the field's components at the physical point x (x[2] = 0 in 2D). */
+typedef void (*dg_field_fn)(const double* x, double t, double* val, void* user);
+/* ScalarLaw's velocity FieldEvaluator (ScalarLaw(farField, velocityField, exact),
+ * ScalarLaw.cc:20-97: Fi, riemannSolver, boundaryFlux and maximumEigenValue
+ * evaluate velocityField at the point they work on).  fn is called once per
+ * volume and face quadrature point of every element, at time 0 as the
+ * reference's Fi and riemannSolver call it, and the velocities are kept on the
+ * device; the dt rule takes |vx| + |vy| at each element's first volume point
+ * (DGCell::computeMAXEV, DGCell.cc:1165-1186).  Each element evaluates a face
+ * point at its own position, so on a periodic face the two images must give the
+ * same normal velocity (a periodic field, or a rotation about the centre of an
+ * origin-centred square, as the reference's ScalarPb, Lilia.cc:66-108); the
+ * reference takes one side's point.  Scalar law on triangles, uniform order, the
+ * tiled kernel; derivatives, the adjoint and order changes refuse such a
+ * context (DG_ERR_UNSUPPORTED).  Replaces the constant dg_law_desc.vel. */
+dg_status dg_set_velocity_field(dg_ctx* ctx, dg_field_fn fn, void* user);
+
 /* DGCell::theFieldsCoefficients write/read for all cells (DGCell.h:22-40) */
 dg_status dg_set_state(dg_ctx* ctx, const double* U, size_t n);
 dg_status dg_get_state(dg_ctx* ctx, double* U, size_t n);

@@ -281,6 +281,9 @@ struct or_ctx {
   int rec_on;
   /* or_residual_abs: accumulate |term| instead of the signed terms */
   int absmode;
+  /* ScalarLaw's velocity FieldEvaluator (or_set_velocity_field), or NULL */
+  or_field_fn velfn;
+  void* veluser;
 };
 
 static int nfields_of(const or_law* l) {
@@ -1112,7 +1115,15 @@ static void volume_elem(or_ctx* c, const double* Ue, double* Re, int e, double t
       val[f] = s;
     }
     double F[MAXNF][3];
-    law_flux(&c->law, val, F);
+    if (c->velfn) {
+      /* ScalarLaw::Fi (ScalarLaw.cc:29-37): the velocity at the point */
+      double x[3], v[3] = {0, 0, 0};
+      elem_map(c, e, c->vq[p][q], x);
+      c->velfn(x, 0.0, v, c->veluser);
+      for (int d = 0; d < 3; d++) F[0][d] = v[d] * val[0];
+    } else {
+      law_flux(&c->law, val, F);
+    }
     double jw = dj * c->vq[p][q][3];
     /* grads = (grad_ref phi . J^-1) |J| w (DGCell.cc:156-158, FunctionSpace.cc:7-16) */
     for (int k = 0; k < Np; k++) {
@@ -1160,7 +1171,23 @@ static int face_flux(or_ctx* c, const double* U, int fi, double t) {
         uR[k] = s;
       }
       if (!law_physical(&c->law, uR)) nonphys |= 2;
-      law_riemann(&c->law, f->n, uL, uR, Fh);
+      if (c->velfn) {
+        /* ScalarLaw::riemannSolver (ScalarLaw.cc:69-88) with the velocity at the
+         * face's point (the left cell's frame) */
+        double v[3] = {0, 0, 0};
+        c->velfn(c->fx + (long)P * 3, 0.0, v, c->veluser);
+        double vn = f->n[0] * v[0] + f->n[1] * v[1] + f->n[2] * v[2];
+        double u = vn > 0 ? uL[0] : uR[0];
+        Fh[0] = (v[0] * u) * f->n[0] + (v[1] * u) * f->n[1] + (v[2] * u) * f->n[2];
+      } else {
+        law_riemann(&c->law, f->n, uL, uR, Fh);
+      }
+    } else if (c->velfn) {
+      /* ScalarLaw::boundaryFlux (ScalarLaw.cc:39-59) with the velocity at the point */
+      double v[3] = {0, 0, 0};
+      c->velfn(c->fx + (long)P * 3, t, v, c->veluser);
+      double vn = f->n[0] * v[0] + f->n[1] * v[1] + f->n[2] * v[2];
+      Fh[0] = vn < 0 ? vn * c->law.farfield[0] : vn * uL[0];
     } else {
       law_boundary(&c->law, f->tag, f->n, c->fx + (long)P * 3, t, uL, Fh);
     }
@@ -1224,6 +1251,13 @@ static int run_phase(or_ctx* c, const double* U, double* R, double t, int phase,
     np |= w[i].nonphys;
   }
   return np;
+}
+
+int or_set_velocity_field(or_ctx* c, or_field_fn fn, void* user) {
+  if (c->law.law != OR_LAW_SCALAR) return -1;
+  c->velfn = fn;
+  c->veluser = user;
+  return 0;
 }
 
 int or_residual_abs(or_ctx* c, const double* U, double t, double* Rabs, int nthreads) {
@@ -1737,6 +1771,17 @@ double or_dt(or_ctx* c, const double* U, double cfl) {
       basis_eval(dim, 0, z, &phi0, NULL);
     }
     for (int f = 0; f < Nf; f++) mean[f] = Ue[f * Np] * phi0; /* computeMean (DGCell.cc:1130-1153) */
+    if (c->velfn && dim == 2) {
+      /* ScalarLaw::maximumEigenValue |vx| + |vy| + |vz| at the cell's first volume
+       * point (DGCell.cc:1165-1186: every evaluation takes pt(0)) */
+      double x[3], v[3] = {0, 0, 0};
+      elem_map(c, e, c->vq[p][0], x);
+      c->velfn(x, 0.0, v, c->veluser);
+      double a = fabs(v[0]) + fabs(v[1]) + fabs(v[2]);
+      double d = cfl * size / (a * (2. * p + 1.0));
+      if (d < dt) dt = d;
+      continue;
+    }
     double a = law_maxev(&c->law, mean);
     if (dim == 2) {
       static const double corner[3][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}};

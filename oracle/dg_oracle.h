@@ -44,6 +44,9 @@ typedef struct {
 
 typedef struct or_ctx or_ctx;
 
+/* a FieldEvaluator (FieldEvaluator.h:12): val at the physical point x and time t */
+typedef void (*or_field_fn)(const double* x, double t, double* val, void* user);
+
 /* builds connectivity, geometry and quadrature tables; NULL on bad input */
 or_ctx* or_create(const or_mesh* m, const or_law* law);
 void or_destroy(or_ctx* c);
@@ -56,6 +59,10 @@ void or_faces(const or_ctx* c, int32_t* out4);
 
 /* R = int F.grad(phi) + S phi - oint Fhat phi  (DGCell.cc:184-249, 639-732) */
 int or_residual(or_ctx* c, const double* U, double t, double* R, int nthreads);
+/* ScalarLaw's velocity FieldEvaluator (ScalarLaw.cc:29-97): the flux, the
+ * Riemann and boundary fluxes and the time step evaluate it at their point;
+ * face fluxes at the face's point in the left cell's frame (0: scalar law only) */
+int or_set_velocity_field(or_ctx* c, or_field_fn fn, void* user);
 /* the same loops summing the magnitudes of the terms, |volume term| per point
  * and |face term| per face point: the scale of the residual's summation, which
  * bounds its rounding error (test hook) */

@@ -27,6 +27,19 @@ class _Law(ctypes.Structure):
                 ("vel", ctypes.c_double * 3), ("farfield", ctypes.c_double * 8)]
 
 
+# a FieldEvaluator: fn(x[3], t, val[3], user)
+FIELD_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                            ctypes.c_void_p)
+
+
+def field_fn(py):
+    """a FieldEvaluator from a Python function (x, y) -> (vx, vy)"""
+    def cb(x, t, val, user):
+        v = py(x[0], x[1])
+        val[0], val[1], val[2] = v[0], v[1], 0.0
+    return FIELD_FN(cb)
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -60,6 +73,7 @@ def _load():
     lib.or_limit.argtypes = [vp, _p_f64, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_uint8)]
     lib.or_set_limiter.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     lib.or_sensor.argtypes = [vp, _p_f64, _p_f64, _p_f64, ctypes.c_int, ctypes.c_int, _p_i32]
+    lib.or_set_velocity_field.argtypes = [vp, FIELD_FN, vp]
     lib.or_riemann.argtypes = [ctypes.POINTER(_Law), _p_f64, _p_f64, _p_f64, _p_f64]
     lib.or_riemann_exact.argtypes = [ctypes.c_double, _p_f64, _p_f64, ctypes.c_double, _p_f64]
     lib.or_set_tri_basis.argtypes = [ctypes.c_int, _p_f64]
@@ -189,6 +203,12 @@ class Oracle:
         flags = np.zeros(self.E, np.uint8)
         _lib.or_recover(self._h, _p(U, _p_f64), flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
         return U, flags
+
+    def set_velocity_field(self, py):
+        """ScalarLaw's velocity FieldEvaluator: py(x, y) -> (vx, vy)"""
+        self._velfn = field_fn(py)  # kept alive with the context
+        if _lib.or_set_velocity_field(self._h, self._velfn, None) != 0:
+            raise ValueError("a velocity field serves the scalar law")
 
     def set_recovery(self, on):
         """recovery before every residual of step()"""

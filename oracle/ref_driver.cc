@@ -17,7 +17,8 @@
 //     OUT: one line per cell in the reference's iteration order: its 3 vertex ids
 //          in the reference's own vertex order, then the residual or new state
 //   scalar velocity (VX VY) and far-field state are read from the environment
-//   (REF_VEL="vx vy", REF_FARFIELD="q0 q1 q2 q3"); time steps start at REF_T0
+//   (REF_VEL="vx vy", or REF_VEL=rotate for ScalarPb's (2 pi y, -2 pi x);
+//   REF_FARFIELD="q0 q1 q2 q3"); time steps start at REF_T0
 //   (default 0; Maxwell2d's RHS holds a pulse source for t <= 0.06).
 //
 // Only the serial code paths run (numThreads = 1): the threaded face assembly
@@ -73,6 +74,19 @@ class ConstField : public FieldEvaluator {
   virtual int order() const { return 0; }
 };
 
+// the rotating velocity of the reference's ScalarPb scenario (VelocityOne,
+// Lilia.cc:66-82): (2 pi y, -2 pi x), selected by REF_VEL=rotate
+class RotatingField : public FieldEvaluator {
+ public:
+  virtual void eval(const mPoint& x, double, double* val) const {
+    const double pi = 3.14159265358979323846;
+    val[0] = 2. * pi * x(1);
+    val[1] = -2. * pi * x(0);
+    val[2] = 0.0;
+  }
+  virtual int order() const { return 1; }
+};
+
 // Local Lax-Friedrichs flux plugged into the reference's Euler2d law through
 // the riemannSolver virtual (ConservationLaw.h:29-33): the only part of the
 // path the reference does not provide.  Wave speed as maximumEigenValue(n,.,.)
@@ -102,7 +116,8 @@ int main(int argc, char** argv) {
   const char* op = argv[5];
   double dt = argc > 8 ? atof(argv[8]) : 0.0;
   int nsteps = argc > 9 ? atoi(argv[9]) : 1;
-  if (getenv("REF_VEL")) sscanf(getenv("REF_VEL"), "%lf %lf", &g_vel[0], &g_vel[1]);
+  const bool rotate = getenv("REF_VEL") && !strcmp(getenv("REF_VEL"), "rotate");
+  if (getenv("REF_VEL") && !rotate) sscanf(getenv("REF_VEL"), "%lf %lf", &g_vel[0], &g_vel[1]);
   if (getenv("REF_FARFIELD"))
     sscanf(getenv("REF_FARFIELD"), "%lf %lf %lf %lf", &g_ff[0], &g_ff[1], &g_ff[2], &g_ff[3]);
 
@@ -124,8 +139,9 @@ int main(int argc, char** argv) {
   set<int> walls;
   walls.insert(40000);
   int nf;
+  RotatingField rot;
   if (!strcmp(law_s, "scalar")) {
-    law = new ScalarLaw(&ff, &vel, &ff);
+    law = new ScalarLaw(&ff, rotate ? (FieldEvaluator*)&rot : (FieldEvaluator*)&vel, &ff);
     nf = 1;
   } else if (!strcmp(law_s, "maxwell2d")) {
     law = new Maxwell2d(walls, &ff3);

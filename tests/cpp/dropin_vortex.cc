@@ -1,7 +1,7 @@
 // Drives the library through the host C++ mirror classes the way a reference
 // scenario drives DG-Parallel (Examples.cc / Lilia.cc): build a law and a mesh,
 // construct DGAnalysis, then call residual and TimeIntegrator::advanceInTime.
-//   dropin_vortex OUT_PREFIX [vortex|tube|maxwell|restart]
+//   dropin_vortex OUT_PREFIX [vortex|tube|maxwell|restart|scalarpb]
 // writes OUT_PREFIX.U0 (projected state), .R (residual), .U2 (after two steps of
 // the integrator DGAnalysis picked for the order), raw little-endian doubles, and
 // prints dt and the residual norms.
@@ -10,7 +10,10 @@
 //            orders P1-P4 on a 16x4x2 tube, the state limited once, two SSP-RK3
 //            steps; then adaptP and .O (new orders as doubles), .U3 (state after it)
 //   maxwell  Maxwell3d (upwind), P3 on a 2x2x2 Kuhn mesh, cavity mode, RK4
+//   scalarpb the reference's ScalarPb: ScalarLaw with the rotating velocity
+//            FieldEvaluator, P1 on a periodic jittered 12x12x2 mesh of [-1,1]^2
 //   restart  the vortex written, plotted, read back and continued (see restart())
+#include <cmath>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -76,6 +79,41 @@ static int tube(const std::string& out) {
   return 0;
 }
 
+// the reference's ScalarPb scenario (Lilia.cc:84-108): ScalarLaw with the
+// rotating VelocityOne field (Lilia.cc:66-82) as its velocity FieldEvaluator, on
+// a jittered periodic 12x12x2 mesh of [-1,1]^2, P1 (the reference's benchmark
+// order), the integrator DGAnalysis picks for it
+class VelocityOne : public dgpar::FieldEvaluator {
+ public:
+  void eval(const dgpar::mPoint& space, double, double* val) const override {
+    const double pi = 3.14159265358979323846;
+    val[0] = 2. * pi * space(1);
+    val[1] = -2. * pi * space(0);
+    val[2] = 0.0;
+  }
+  int order() const override { return 1; }
+};
+
+static int scalarpb(const std::string& out) {
+  const int32_t tags[4] = {0, 0, 0, 0};
+  dgpar::Mesh mesh = dgpar::Mesh::structuredTri(12, 12, -1.0, -1.0, 2.0, 2.0, true, true, 0.1, 31, tags);
+  dgpar::ConstantField i({0.0});
+  VelocityOne v;
+  dgpar::ScalarLaw theLaw(&i, &v, &i);
+  dgpar::DGAnalysis analysis(mesh, &theLaw, nullptr, 1);
+  std::vector<double> U(analysis.ndof());
+  for (size_t k = 0; k < U.size(); k++) U[k] = std::sin(0.37 * (double)k) + 0.2 * std::cos(1.3 * (double)k);
+  analysis.setState(U);
+  dump(out + ".U0", analysis.getState());
+  dump(out + ".R", analysis.residual(0.0));
+  double dt = analysis.computeDt(0.1);
+  double r1 = analysis.integrator()->advanceInTime(0.0, dt);
+  double r2 = analysis.integrator()->advanceInTime(dt, dt);
+  dump(out + ".U2", analysis.getState());
+  printf("dt %.17e\nresid %.17e %.17e\n", dt, r1, r2);
+  return 0;
+}
+
 static int maxwell(const std::string& out) {
   dgpar::Mesh mesh = dgpar::Mesh::structuredTet(2, 2, 2, 1.0, 1.0, 1.0);
   dgpar::Maxwell3d law(true);
@@ -129,6 +167,7 @@ int main(int argc, char** argv) {
     if (mode == "tube") return tube(out);
     if (mode == "maxwell") return maxwell(out);
     if (mode == "restart") return restart(out);
+    if (mode == "scalarpb") return scalarpb(out);
     return vortex(out);
   } catch (const dgpar::DGError& e) {
     fprintf(stderr, "DGError %d: %s\n", (int)e.status, e.what());

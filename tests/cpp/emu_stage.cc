@@ -58,6 +58,7 @@ int run_law(const StageArgs& a) {
 
 template <typename CT, typename IT, typename OT, int NW>
 int run(const StageArgs& a, int law) {
+  if (law == LAW_SCALAR && a.pvel) return run_law<ScalarFieldLawD, CT, IT, OT, NW>(a);
   if (law == LAW_SCALAR) return run_law<ScalarLawD, CT, IT, OT, NW>(a);
   if (law == LAW_EULER2D && a.law.char_bc) return run_law<EulerCharLawD, CT, IT, OT, NW>(a);
   if (law == LAW_EULER2D) return run_law<EulerLawD, CT, IT, OT, NW>(a);
@@ -224,6 +225,34 @@ extern "C" int emu_tri_residual(const dg_mesh_desc* mesh, const dg_law_desc* law
   if (make(m, mesh, law, order)) return -1;
   m.to_soa(U);
   StageArgs a = m.base();
+  a.Uin = m.Un.data();
+  a.R = m.R.data();
+  a.scheme = MODE_RESIDUAL;
+  if (run_variant(a, m.S.law, (fp32 ? 1 : 0) | shape << 2)) return -2;
+  m.from_soa(m.R.data(), R);
+  return 0;
+}
+
+// ScalarPb's rotating velocity (2 pi y, -2 pi x) (Lilia.cc:66-82)
+static void rotating_field(const double* x, double, double* v, void*) {
+  const double pi = 3.14159265358979323846;
+  v[0] = 2. * pi * x[1];
+  v[1] = -2. * pi * x[0];
+  v[2] = 0.0;
+}
+
+// the residual of the scalar law with the rotating velocity field: the point
+// velocity table built as dg_set_velocity_field builds it
+extern "C" int emu_tri_residual_rotating(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int form,
+                                         const double* U, double* R) {
+  const int fp32 = form & 1, shape = (form >> 1) & 1;
+  Emu m;
+  if (make(m, mesh, law, order)) return -1;
+  std::vector<double> pv;
+  if (!tri_point_field(m.S.xyz, m.S.perm, order, 2, rotating_field, nullptr, pv)) return -3;
+  m.to_soa(U);
+  StageArgs a = m.base();
+  a.pvel = pv.data();
   a.Uin = m.Un.data();
   a.R = m.R.data();
   a.scheme = MODE_RESIDUAL;

@@ -89,6 +89,7 @@ NFIELDS = {"scalar": 1, "euler": 4, "maxwell2d": 3, "burgers": 1}
 
 def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfield=(0, 0, 0, 0), t0=0.0):
     """Returns (values in the caller's element order, reference vertex order per element, resid list).
+    vel: the scalar law's constant velocity, or "rotate" for ScalarPb's (2 pi y, -2 pi x).
     order: one order for every cell, or a per-element array (mixed orders; U is then
     the concatenation of the per-element blocks of Nf*Np(p_e) coefficients)."""
     E = len(mesh["lat"])
@@ -105,7 +106,8 @@ def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfiel
             for e in range(E):
                 f.write(" ".join(str(v + 1) for v in mesh["lat"][e]) + (" %d" % orders[e] if mixed else "") + " " +
                         " ".join("%.17e" % x for x in U[offs[e]:offs[e + 1]]) + "\n")
-        env = dict(os.environ, REF_VEL="%r %r" % tuple(vel[:2]), REF_FARFIELD=" ".join("%r" % x for x in farfield),
+        env = dict(os.environ, REF_VEL=vel if isinstance(vel, str) else "%r %r" % tuple(vel[:2]),
+                   REF_FARFIELD=" ".join("%r" % x for x in farfield),
                    REF_T0="%r" % t0)
         r = subprocess.run([DRIVER, mp, law, flux, "0" if mixed else str(order), op, sp, op_, "%r" % dt,
                             str(nsteps)], cwd=d, env=env, capture_output=True, text=True, timeout=600)

@@ -118,3 +118,24 @@ def test_cpp_dropin_write_read_export():
         with open("%s-sample2%s.pos" % (pre, f)) as fh:
             body = fh.read().splitlines()
         assert body[0] == 'View "Exported field" {' and len(body) == 2 + 200 * 9
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="C++ driver not built")
+def test_cpp_dropin_scalarpb_with_a_velocity_field():
+    """the reference's ScalarPb written as its scenario is: ScalarLaw(&i, &v, &i)
+    with the rotating VelocityOne FieldEvaluator, P1 (SSP-RK2, the integrator the
+    reference picks for order 1)"""
+    print("\n[gpu case] C++ drop-in ScalarPb", flush=True)
+    pre, dt, res = run_exe("scalarpb")
+    U0, R, U2 = np.fromfile(pre + ".U0"), np.fromfile(pre + ".R"), np.fromfile(pre + ".U2")
+    vids, xyz, bc = mesh_tri(12, 12, x0=-1.0, y0=-1.0, lx=2.0, ly=2.0, jitter=0.1, seed=31)
+    o = po.Oracle(2, vids, xyz, 1, po.LAW_SCALAR, po.FLUX_UPWIND, farfield=(0.0,))
+    o.set_velocity_field(lambda x, y: (2 * np.pi * y, -2 * np.pi * x))
+    Ro, _ = o.residual(U0)
+    assert np.abs(R - Ro).max() / np.abs(Ro).max() < 1e-12
+    assert abs(dt - o.dt(U0, 0.1)) <= 1e-14 * dt
+    U = U0
+    for s in range(2):
+        U, r = o.step(U, po.RK_TVD2, s * dt, dt)
+    assert np.abs(U2 - U).max() / np.abs(U).max() < 1e-12
+    assert abs(res[1] - r) <= 1e-10 * abs(r)
