@@ -308,19 +308,12 @@ def run_maxwell(dgpar, steps, n=48, tune=True):
 REF_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "ref_driver")
 
 
-def reference_cpu_baseline(mesh, U, dt, nsteps=1):
-    """The reference's own serial code (oracle/_ref/ref_driver, built from the
-    reference's sources with its Euler2d law and a local Lax-Friedrichs
-    riemannSolver) on the full mesh: nsteps RK4 steps, timed by the driver
-    around its step loop (mesh import and DGAnalysis construction excluded).
-    The mesh goes over as a Gmsh file: the (NX+1)^2 lattice nodes, the
-    triangles in the generator's order, and 510 / 610 lines on the periodic
-    sides.  None when the reference build is absent or fails."""
-    if not os.path.exists(REF_DRIVER):
-        return None
-    import subprocess
-    import tempfile
-    n, N1 = NX, NX + 1
+def write_ref_lattice(mesh, n, d, U):
+    """The structured n x n x 2 mesh as a Gmsh file for the reference's driver (the
+    (n+1)^2 lattice nodes, the triangles in the generator's order, 510 / 610 lines
+    on the periodic sides) and the state U as its state file, in directory d.
+    Returns (mesh path, state path, number of triangles)."""
+    N1 = n + 1
     X = mesh.xyz.reshape(n, n, 2, 3, 2)  # [j][i][triangle][corner][x, y]
     c = np.zeros((N1, N1, 2))
     c[:n, :n] = X[:, :, 0, 0]
@@ -329,52 +322,152 @@ def reference_cpu_baseline(mesh, U, dt, nsteps=1):
     c[n, n] = X[n - 1, n - 1, 0, 2]
     # the reference joins periodic sides through the reflected partner point
     # (DGCell.cc:557-577, 687-693), which is the translation image only on a
-    # domain centred at the origin: the mesh goes over shifted to [-5, 5]^2
+    # domain centred at the origin: the mesh goes over shifted to be centred there
     # (a translation leaves every element's modal state and residual unchanged)
     c -= 0.5 * (c.reshape(-1, 2).min(axis=0) + c.reshape(-1, 2).max(axis=0))
     nid = lambda i, j: j * N1 + i + 1  # noqa: E731
-    with tempfile.TemporaryDirectory() as d:
-        mp, sp, op_ = os.path.join(d, "m.msh"), os.path.join(d, "s.txt"), os.path.join(d, "o.txt")
-        with open(mp, "w") as f:
-            f.write("$NOD\n%d\n" % (N1 * N1))
-            for j in range(N1):
-                for i in range(N1):
-                    f.write("%d %.17e %.17e 0\n" % (nid(i, j), c[j, i, 0], c[j, i, 1]))
-            recs = []
-            for j in range(n):
-                recs += [(510, nid(0, j), nid(0, j + 1)), (510, nid(n, j), nid(n, j + 1))]
+    mp, sp = os.path.join(d, "m.msh"), os.path.join(d, "s.txt")
+    with open(mp, "w") as f:
+        f.write("$NOD\n%d\n" % (N1 * N1))
+        for j in range(N1):
+            for i in range(N1):
+                f.write("%d %.17e %.17e 0\n" % (nid(i, j), c[j, i, 0], c[j, i, 1]))
+        recs = []
+        for j in range(n):
+            recs += [(510, nid(0, j), nid(0, j + 1)), (510, nid(n, j), nid(n, j + 1))]
+        for i in range(n):
+            recs += [(610, nid(i, 0), nid(i + 1, 0)), (610, nid(i, n), nid(i + 1, n))]
+        f.write("$ENDNOD\n$ELM\n%d\n" % (len(recs) + 2 * n * n))
+        k = 1
+        for tag, a, b in recs:
+            f.write("%d 1 %d 1 2 %d %d\n" % (k, tag, a, b))
+            k += 1
+        tris = []
+        for j in range(n):
             for i in range(n):
-                recs += [(610, nid(i, 0), nid(i + 1, 0)), (610, nid(i, n), nid(i + 1, n))]
-            f.write("$ENDNOD\n$ELM\n%d\n" % (len(recs) + 2 * n * n))
-            k = 1
-            for tag, a, b in recs:
-                f.write("%d 1 %d 1 2 %d %d\n" % (k, tag, a, b))
-                k += 1
-            tris = []
-            for j in range(n):
-                for i in range(n):
-                    tris.append((nid(i, j), nid(i + 1, j), nid(i + 1, j + 1)))
-                    tris.append((nid(i, j), nid(i + 1, j + 1), nid(i, j + 1)))
-            for t in tris:
-                f.write("%d 2 100 1 3 %d %d %d\n" % (k, t[0], t[1], t[2]))
-                k += 1
-            f.write("$ENDELM\n")
-        Ue = np.asarray(U).reshape(len(tris), -1)
-        with open(sp, "w") as f:
-            for t, row in zip(tris, Ue):
-                f.write("%d %d %d " % t + " ".join("%.17e" % x for x in row) + "\n")
-        env = dict(os.environ, REF_FARFIELD="1 1 1 3.5")
-        r = subprocess.run([REF_DRIVER, mp, "euler", "lf", str(ORDER), "rk4", sp, op_, "%r" % dt, str(nsteps)],
-                           cwd=d, env=env, capture_output=True, text=True, timeout=900)
-        if r.returncode != 0:
-            return None
-        sec = [float(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("step_seconds ")]
+                tris.append((nid(i, j), nid(i + 1, j), nid(i + 1, j + 1)))
+                tris.append((nid(i, j), nid(i + 1, j + 1), nid(i, j + 1)))
+        for t in tris:
+            f.write("%d 2 100 1 3 %d %d %d\n" % (k, t[0], t[1], t[2]))
+            k += 1
+        f.write("$ENDELM\n")
+    Ue = np.asarray(U).reshape(len(tris), -1)
+    with open(sp, "w") as f:
+        for t, row in zip(tris, Ue):
+            f.write("%d %d %d " % t + " ".join("%.17e" % x for x in row) + "\n")
+    return mp, sp, len(tris)
+
+
+def run_ref_driver(args, env):
+    """oracle/_ref/ref_driver's stdout, or None when the build is absent or fails"""
+    if not os.path.exists(REF_DRIVER):
+        return None
+    import subprocess
+    r = subprocess.run([REF_DRIVER] + args, cwd=os.path.dirname(args[0]), env=dict(os.environ, **env),
+                       capture_output=True, text=True, timeout=900)
+    return r.stdout if r.returncode == 0 else None
+
+
+def reference_cpu_baseline(mesh, U, dt, nsteps=1):
+    """The reference's own serial code (oracle/_ref/ref_driver, built from the
+    reference's sources with its Euler2d law and a local Lax-Friedrichs
+    riemannSolver) on the full mesh: nsteps RK4 steps, timed by the driver
+    around its step loop (mesh import and DGAnalysis construction excluded).
+    None when the reference build is absent or fails."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        mp, sp, ntri = write_ref_lattice(mesh, NX, d, U)
+        out = run_ref_driver([mp, "euler", "lf", str(ORDER), "rk4", sp, os.path.join(d, "o.txt"), "%r" % dt,
+                              str(nsteps)], {"REF_FARFIELD": "1 1 1 3.5"})
+    sec = [float(l.split()[1]) for l in (out or "").splitlines() if l.startswith("step_seconds ")]
     if not sec or sec[0] <= 0:
         return None
-    return dict(value=Ue.size * 4 * nsteps / sec[0], unit="DOF*stage/s", cores=1, kind="reference",
+    return dict(value=np.asarray(U).size * 4 * nsteps / sec[0], unit="DOF*stage/s", cores=1, kind="reference",
                 sample="%d RK4 step(s) of the full %d-triangle P3 vortex mesh by the reference's own serial "
                        "code (its DGAnalysis and TimeIntegrator built from its sources, LF flux), %.2f s of steps"
-                % (nsteps, len(tris), sec[0]))
+                % (nsteps, ntri, sec[0]))
+
+
+TWO_PI = 2.0 * np.pi
+
+
+def scalarpb_bump(x, y, z, t):
+    """ScalarPb's initial condition (CaseOne, Lilia.cc:38-58): cos^2(2 pi r)
+    within r <= 0.25 of (-0.5, 0)"""
+    r = np.hypot(x + 0.5, y)
+    return (np.cos(TWO_PI * r) ** 2 if r <= 0.25 else 0.0,)
+
+
+def bytes_per_dof_stage_scalarpb():
+    """Algorithmic HBM bytes per DOF and SSP-RK2 stage of the P1 scalar stage
+    kernel with a velocity field (FP64): stage 1 reads U^n and writes U1, stage 2
+    reads U1 and U^n and writes U^{n+1} (20 B per DOF on average); per element
+    the table (|J| J^-1: 4 doubles, |J|, 3 int32 links) and the velocity at the
+    3 volume and 3 x 2 face points (2 doubles each), over 3 DOF"""
+    return (16 + 24) / 2.0 + (4 * 8 + 8 + 3 * 4) / 3.0 + (3 + 3 * 2) * 16 / 3.0
+
+
+def run_scalarpb(dgpar, n=512, n_ref=128, steps=200, sweeps=40, tune=True, ref=True):
+    """The reference's only benchmarked workload, ScalarPb (Lilia.cc:66-108):
+    scalar advection with the rotating velocity FieldEvaluator (2 pi y, -2 pi x),
+    CaseOne's bump as the initial state, P1 (SSP-RK2, the integrator the reference
+    picks for order 1) at its default CFL 0.5 (DGAnalysis.cc:144), upwind flux,
+    on jittered periodic triangle meshes of [-1,1]^2.  GPU: `steps` steps on
+    n x n x 2 triangles (throughput), and the same on n_ref x n_ref x 2 (as many
+    elements as the reference's advection16384 quad mesh, split); beside it the
+    reference's own serial residual sweeps (volume + boundary contributions, as
+    its profiling harness, DGAnalysis.cc:321-477) on that same mesh, timed on
+    this host by oracle/_ref/ref_driver."""
+    import tempfile
+    out = {"workload": "ScalarPb: scalar advection, velocity FieldEvaluator (2 pi y, -2 pi x), CaseOne bump, P1, "
+                       "upwind, SSP-RK2 CFL0.5, periodic [-1,1]^2 jittered triangles"}
+
+    def velocity(x, y):
+        return (TWO_PI * y, -TWO_PI * x)
+    for key, m in (("gpu", n), ("gpu_reference_mesh", n_ref)):
+        mesh = dgpar.mesh_tri_structured(m, m, x0=-1.0, y0=-1.0, lx=2.0, ly=2.0, jitter=0.1, seed=SEED)
+        law = dgpar.Law(dgpar.LAW_SCALAR, dgpar.FLUX_UPWIND, farfield=(0.0,))
+        g = dgpar.DGAnalysis(mesh, law, 1, precision=dgpar.PREC_FP64, use_graph=True, device=0)
+        t0 = time.perf_counter()
+        g.set_velocity_field(velocity)
+        g.project_fn(scalarpb_bump)
+        setup_s = time.perf_counter() - t0
+        U0 = g.get_state()
+        dt = g.computeDt(0.5)
+        sched = tuned(g, dgpar.RK_TVD2, dt, tune)
+        g.set_state(U0)
+        g.time_advance(dgpar.RK_TVD2, dt, 10)
+        g.set_state(U0)
+        dgpar.device_synchronize()
+        ms = g.time_advance(dgpar.RK_TVD2, dt, steps)
+        total = float(g.norms()["totals"][0])
+        g.set_state(U0)
+        total0 = float(g.norms()["totals"][0])
+        ndof = g.ndof
+        g.close()
+        bpd = bytes_per_dof_stage_scalarpb()
+        ach = bpd * ndof / (ms / (2 * steps) / 1e3) / 1e9
+        out[key] = {"mesh": "%dx%dx2 = %d triangles" % (m, m, 2 * m * m), "dof": ndof, "steps": steps, "dt": dt,
+                    "ms_per_step": ms / steps, "value": ndof * 2 * steps / (ms / 1e3), "unit": "DOF*stage/s",
+                    "bytes_per_dof_stage": bpd, "achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS,
+                    "field_tables_setup_s": setup_s, "schedule": sched,
+                    # the periodic rotation conserves the total (int u)
+                    "total_rel_change": abs(total - total0) / abs(total0)}
+        if key == "gpu_reference_mesh" and ref:
+            with tempfile.TemporaryDirectory() as d:
+                mp, sp, ntri = write_ref_lattice(mesh, m, d, U0)
+                so = run_ref_driver([mp, "scalar", "upwind", "1", "residual", sp, os.path.join(d, "o.txt")],
+                                    {"REF_VEL": "rotate", "REF_FAR": "bump", "REF_SWEEPS": str(sweeps)})
+            sec = [float(l.split()[1]) for l in (so or "").splitlines() if l.startswith("residual_seconds ")]
+            if sec and sec[0] > 0:
+                out["reference_cpu"] = {
+                    "value": ndof * sweeps / sec[0], "unit": "DOF/s (residual sweeps: volume + boundary)",
+                    "cores": 1, "kind": "reference", "sweeps": sweeps, "seconds": sec[0],
+                    "sample": "%d serial residual sweeps of the reference's own code (ScalarLaw with the rotating "
+                              "FieldEvaluator) on the same %d-triangle mesh" % (sweeps, ntri)}
+                # a GPU stage is one residual plus the stage update per DOF
+                out["gpu_stage_over_reference_sweep"] = out[key]["value"] / out["reference_cpu"]["value"]
+    return out
 
 
 def cpu_model():
@@ -424,6 +517,8 @@ def main():
                          "to t = 0.2)")
     ap.add_argument("--scalar-steps", type=int, default=100,
                     help="SSP-RK3 steps of the scalar advection leg (0: skip it)")
+    ap.add_argument("--scalarpb-steps", type=int, default=200,
+                    help="SSP-RK2 steps of the ScalarPb leg (rotating velocity field; 0: skip it)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules (no tuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 256 MiB device copy (reference point for HBM byte counters)")
@@ -490,6 +585,8 @@ def main():
         out["adjoint"] = run_adjoint(dgpar, args.adjoint_steps, 20)
     if args.scalar_steps > 0:
         out["scalar_advection"] = run_scalar(dgpar, args.scalar_steps, tune=tune)
+    if args.scalarpb_steps > 0:
+        out["scalarpb"] = run_scalarpb(dgpar, steps=args.scalarpb_steps, tune=tune, ref=not args.no_cpu)
     if args.mixed_steps != 0:
         out["mixed_order_limited"] = run_mixed(dgpar, args.mixed_steps, tune=tune)
     if args.maxwell_steps > 0:

@@ -858,62 +858,74 @@ static double tet_detj(const double* X) {
                    J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]));
 }
 
-static dg_status project_tet(dg_ctx* c, int field, const double* par, double t) {
-  const int P = c->P, NP = c->NP, NF = c->Nf;
-  auto rule = tet_rule(2 * P);
-  std::vector<double> phi(rule.size() * NP);
-  for (size_t q = 0; q < rule.size(); q++) basis(3, P, rule[q].x, &phi[q * NP], nullptr);
+// L2 projection of a field onto every element's basis with the rule of degree
+// 2p (DGCell::L2ProjInitial, DGCell.cc:1458-1483; the orthonormal basis makes
+// the mass matrix |J| I, so |J| cancels).  ev(x, t, val) fills the field at a
+// physical point and returns 0, or an error status.
+extern "C++" {
+template <class Eval>
+static dg_status project_with(dg_ctx* c, double t, Eval ev) {
+  const int E = c->E, NF = c->Nf, dim = c->dim;
+  const int pmax = dim == 2 ? MAXP_TRI : c->P;
+  std::vector<QPoint> rules[MAXP_TRI + 1];
+  std::vector<double> phis[MAXP_TRI + 1];
+  for (int p = dim == 2 ? 1 : c->P; p <= pmax; p++) {
+    rules[p] = dim == 2 ? tri_rule(2 * p) : tet_rule(2 * p);
+    const int np = dim == 2 ? np_tri(p) : np_tet(p);
+    phis[p].resize(rules[p].size() * np);
+    for (size_t q = 0; q < rules[p].size(); q++) basis(dim, p, rules[p][q].x, &phis[p][q * np], nullptr);
+  }
   std::vector<double> U(c->ndof, 0.0);
-  for (int e = 0; e < c->E; e++) {
-    const double* X = &c->xyz[(size_t)e * 12];
+  for (int e = 0; e < E; e++) {
+    const int P = dim == 2 ? c->m_order[e] : c->P, NP = dim == 2 ? np_tri(P) : np_tet(P);
+    const auto& rule = rules[P];
+    const auto& phi = phis[P];
     double* Ue = &U[c->coff[e]];
+    // triangles weigh by |J| and divide it out after the sum, as L2ProjInitial does
+    const double* X = &c->xyz[(size_t)e * (dim == 2 ? 6 : 12)];
+    const double ad = dim == 2 ? std::fabs((X[2] - X[0]) * (X[5] - X[1]) - (X[4] - X[0]) * (X[3] - X[1])) : 1.0;
     for (size_t q = 0; q < rule.size(); q++) {
-      double x[3], val[8];
-      tet_map(X, rule[q].x, x);
-      if (exact_field(field, par, c->lp, x, t, val)) return fail(c, DG_ERR_ARG, "unknown field %d", field);
+      double x[3] = {0.0, 0.0, 0.0}, val[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (dim == 2) {
+        const double u = rule[q].x[0], v = rule[q].x[1];
+        x[0] = X[0] + (X[2] - X[0]) * u + (X[4] - X[0]) * v;
+        x[1] = X[1] + (X[3] - X[1]) * u + (X[5] - X[1]) * v;
+      } else {
+        tet_map(X, rule[q].x, x);
+      }
+      if (dg_status s = ev(x, t, val)) return s;
+      const double jw = ad * rule[q].w;
       for (int f = 0; f < NF; f++)
-        for (int k = 0; k < NP; k++) Ue[f * NP + k] += val[f] * phi[q * NP + k] * rule[q].w;  // |J| cancels
+        for (int k = 0; k < NP; k++) Ue[f * NP + k] += val[f] * phi[q * NP + k] * jw;
+    }
+    if (dim == 2) {
+      const double inv = 1. / ad;
+      for (int i = 0; i < NF * NP; i++) Ue[i] *= inv;
     }
   }
   return dg_set_state(c, U.data(), U.size());
 }
+}  // extern "C++"
 
 dg_status dg_project(dg_ctx* c, int field, const double* par, double t) {
   if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
   double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (!par) par = zero;
-  if (c->dim == 3) return project_tet(c, field, par, t);
-  const int E = c->E, NF = c->Nf;
-  // per order: the rule of degree 2p and the basis at its points (DGCell.cc:1458-1483)
-  std::vector<QPoint> rules[MAXP_TRI + 1];
-  std::vector<double> phis[MAXP_TRI + 1];
-  for (int p = 1; p <= MAXP_TRI; p++) {
-    rules[p] = tri_rule(2 * p);
-    phis[p].resize(rules[p].size() * np_tri(p));
-    for (size_t q = 0; q < rules[p].size(); q++) basis(2, p, rules[p][q].x, &phis[p][q * np_tri(p)], nullptr);
-  }
-  std::vector<double> U(c->ndof, 0.0);
-  for (int e = 0; e < E; e++) {
-    const int P = c->m_order[e], NP = np_tri(P);
-    const auto& rule = rules[P];
-    const auto& phi = phis[P];
-    const double* X = &c->xyz[(size_t)e * 6];
-    double det = (X[2] - X[0]) * (X[5] - X[1]) - (X[4] - X[0]) * (X[3] - X[1]);
-    double ad = std::fabs(det);
-    double* Ue = &U[c->coff[e]];
-    for (size_t q = 0; q < rule.size(); q++) {
-      const double u = rule[q].x[0], v = rule[q].x[1];
-      double x[3] = {X[0] + (X[2] - X[0]) * u + (X[4] - X[0]) * v, X[1] + (X[3] - X[1]) * u + (X[5] - X[1]) * v, 0.0};
-      double val[8];
-      if (exact_field(field, par, c->lp, x, t, val)) return fail(c, DG_ERR_ARG, "unknown field %d", field);
-      const double jw = ad * rule[q].w;
-      for (int f = 0; f < NF; f++)
-        for (int k = 0; k < NP; k++) Ue[k + NP * f] += val[f] * phi[q * NP + k] * jw;
-    }
-    const double inv = 1. / ad;
-    for (int i = 0; i < NF * NP; i++) Ue[i] *= inv;
-  }
-  return dg_set_state(c, U.data(), U.size());
+  return project_with(c, t, [&](const double* x, double tt, double* val) -> dg_status {
+    if (exact_field(field, par, c->lp, x, tt, val)) return fail(c, DG_ERR_ARG, "unknown field %d", field);
+    return DG_OK;
+  });
+}
+
+dg_status dg_project_fn(dg_ctx* c, dg_field_fn fn, void* user, double t) {
+  if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
+  if (!fn) return fail(c, DG_ERR_ARG, "no field function");
+  return project_with(c, t, [&](const double* x, double tt, double* val) -> dg_status {
+    fn(x, tt, val, user);
+    for (int f = 0; f < c->Nf; f++)
+      if (!std::isfinite(val[f])) return fail(c, DG_ERR_ARG, "field function returned a non-finite value");
+    return DG_OK;
+  });
 }
 
 // Per-field L1, L2 and Linf norms of (state - exact field at t) at the points

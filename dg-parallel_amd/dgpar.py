@@ -143,6 +143,7 @@ _autotune = _sig("dg_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_doubl
                                                 ctypes.c_int, ctypes.POINTER(TuneReport)])
 _copy_probe = _sig("dg_copy_probe", ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_float)])
 _project = _sig("dg_project", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double])
+_project_fn = _sig("dg_project_fn", ctypes.c_int, [_vp, FIELD_FN, _vp, ctypes.c_double])
 _l2_error = _sig("dg_l2_error", ctypes.c_int, [_vp, ctypes.c_int, _p_f64, ctypes.c_double, _p_f64])
 _mesh_tri = _sig("dg_mesh_tri_structured", ctypes.c_int,
                  [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -373,6 +374,18 @@ class DGAnalysis:
     def project(self, field, par=(), t=0.0):
         par = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)
         self._check(_project(self._h, field, _ptr(par, _p_f64), t))
+
+    def project_fn(self, fn, t=0.0):
+        """L2 projection of fn(x, y, z, t) -> Nf values (dg_project_fn; the
+        reference's DGAnalysis projecting its far-field FieldEvaluator)"""
+        nf = self.law.nfields
+
+        def cb(x, tt, val, user):
+            v = fn(x[0], x[1], x[2], tt)
+            for k in range(nf):
+                val[k] = v[k]
+        cfn = FIELD_FN(cb)
+        self._check(_project_fn(self._h, cfn, None, t))
 
     def l2_error(self, field, par=(), t=0.0):
         par = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)

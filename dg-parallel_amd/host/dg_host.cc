@@ -364,6 +364,8 @@ DGAnalysis::DGAnalysis(const Mesh& mesh, ConservationLaw* law, DGLimiter* limite
   }
   if (limiter) check(dg_set_limiter(ctx_, 1, limiter->tvbM(), limiter->nu()));
   if (FieldEvaluator* vf = law->velocityField()) check(dg_set_velocity_field(ctx_, eval_field, vf));
+  // every cell starts as the L2 projection of the law's far field (DGAnalysis.cc:200)
+  if (FieldEvaluator* ff = law->getFarField()) check(dg_project_fn(ctx_, eval_field, ff, 0.0));
   // integrator by order (DGAnalysis.cc:239-242)
   if (order == 0)
     integrator_ = new ForwardEuler(this);

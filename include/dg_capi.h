@@ -313,6 +313,12 @@ enum { DG_FIELD_GAUSSIAN = 0, DG_FIELD_VORTEX = 1, DG_FIELD_SOD = 2, DG_FIELD_CA
  * (DGCell.cc:1458-1483; volume rule of degree 2p); becomes the current state. */
 dg_status dg_project(dg_ctx* ctx, int field, const double* par, double t);
 
+/* The same projection of a caller's field: DGAnalysis's constructor projecting
+ * the law's far-field FieldEvaluator into every cell (DGAnalysis.cc:200,
+ * DGCell::L2ProjInitial).  fn(x, t, val, user) fills the Nf values at physical
+ * point x on the host; a non-finite value is DG_ERR_ARG. */
+dg_status dg_project_fn(dg_ctx* ctx, dg_field_fn fn, void* user, double t);
+
 /* Per-field L2 norm of (state - exact field at t), rule of degree 2(p+1)
  * (DGCell::densityErrorL2, DGError.cc:805-831); err holds Nf doubles. */
 dg_status dg_l2_error(dg_ctx* ctx, int field, const double* par, double t, double* err);

@@ -1960,16 +1960,21 @@ int or_exact(int field, const double* par, const or_law* law, const double* x, d
   return -1;
 }
 
-int or_project(or_ctx* c, int field, const double* par, double t, double* U) {
+/* DGCell::L2ProjInitial (DGCell.cc:1458-1483): the field at the points of the
+ * degree-2p volume rule, weighed by |J| w, then divided by |J| (the orthonormal
+ * basis' mass matrix); ev fills the field at a physical point, 0 on success */
+typedef int (*point_eval)(void* env, const double* x, double t, double* val);
+
+static int project_with(or_ctx* c, double t, double* U, point_eval ev, void* env) {
   int dim = c->dim, Nf = nfields_of(&c->law);
   for (int e = 0; e < c->E; e++) {
     int p = c->order[e], Np = np_of(dim, p);
     double* Ue = U + c->uoff[e];
     for (int i = 0; i < Nf * Np; i++) Ue[i] = 0.0;
     for (int q = 0; q < c->nqv[p]; q++) {
-      double x[3], val[MAXNF];
+      double x[3], val[MAXNF] = {0};
       elem_map(c, e, c->vq[p][q], x);
-      if (or_exact(field, par, &c->law, x, t, val)) return -1;
+      if (ev(env, x, t, val)) return -1;
       double jw = c->detj[e] * c->vq[p][q][3];
       for (int f = 0; f < Nf; f++)
         for (int k = 0; k < Np; k++) Ue[k + Np * f] += val[f] * c->vphi[p][q][k] * jw;
@@ -1978,6 +1983,28 @@ int or_project(or_ctx* c, int field, const double* par, double t, double* U) {
     for (int i = 0; i < Nf * Np; i++) Ue[i] *= inv;
   }
   return 0;
+}
+
+struct exact_env { int field; const double* par; const or_law* law; };
+static int eval_exact(void* env, const double* x, double t, double* val) {
+  struct exact_env* a = (struct exact_env*)env;
+  return or_exact(a->field, a->par, a->law, x, t, val);
+}
+struct fn_env { or_field_fn fn; void* user; };
+static int eval_fn(void* env, const double* x, double t, double* val) {
+  struct fn_env* a = (struct fn_env*)env;
+  a->fn(x, t, val, a->user);
+  return 0;
+}
+
+int or_project(or_ctx* c, int field, const double* par, double t, double* U) {
+  struct exact_env a = {field, par, &c->law};
+  return project_with(c, t, U, eval_exact, &a);
+}
+
+int or_project_fn(or_ctx* c, or_field_fn fn, void* user, double t, double* U) {
+  struct fn_env a = {fn, user};
+  return project_with(c, t, U, eval_fn, &a);
 }
 
 int or_l2_error(or_ctx* c, const double* U, int field, const double* par, double t, double* err) {

@@ -87,6 +87,9 @@ int or_basis(int dim, int p, const double* uvw, double* phi, double* dphi);
  *          Riemann solution (Toro ch. 4) at t > 0
  * field 3: PEC cavity mode (1,1,1) with E amplitude (1,-1,0) at time t */
 int or_project(or_ctx* c, int field, const double* par, double t, double* U);
+/* the same projection of a caller's field fn(x, t, val, user) (the reference's
+ * DGAnalysis projecting its far-field FieldEvaluator, DGAnalysis.cc:200) */
+int or_project_fn(or_ctx* c, or_field_fn fn, void* user, double t, double* U);
 /* evaluates the same exact fields at a point */
 int or_exact(int field, const double* par, const or_law* law, const double* x, double t, double* val);
 /* test hooks: the law's numerical flux Fhat(uL, uR).n through one face point

@@ -74,6 +74,7 @@ def _load():
     lib.or_set_limiter.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
     lib.or_sensor.argtypes = [vp, _p_f64, _p_f64, _p_f64, ctypes.c_int, ctypes.c_int, _p_i32]
     lib.or_set_velocity_field.argtypes = [vp, FIELD_FN, vp]
+    lib.or_project_fn.argtypes = [vp, FIELD_FN, vp, ctypes.c_double, _p_f64]
     lib.or_riemann.argtypes = [ctypes.POINTER(_Law), _p_f64, _p_f64, _p_f64, _p_f64]
     lib.or_riemann_exact.argtypes = [ctypes.c_double, _p_f64, _p_f64, ctypes.c_double, _p_f64]
     lib.or_set_tri_basis.argtypes = [ctypes.c_int, _p_f64]
@@ -228,6 +229,17 @@ class Oracle:
         U = np.zeros(self.ndof)
         if _lib.or_project(self._h, field, _p(par, _p_f64), t, _p(U, _p_f64)) != 0:
             raise ValueError("bad field")
+        return U
+
+    def project_fn(self, py, t=0.0):
+        """L2 projection of py(x, y, z, t) -> Nf values"""
+        def cb(x, tt, val, user):
+            v = py(x[0], x[1], x[2], tt)
+            for k in range(len(v)):
+                val[k] = v[k]
+        f = FIELD_FN(cb)
+        U = np.zeros(self.ndof)
+        _lib.or_project_fn(self._h, f, None, t, _p(U, _p_f64))
         return U
 
     def limit(self, U, tvb_m=0.0, nu=1.0):

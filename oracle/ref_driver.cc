@@ -8,7 +8,8 @@
 //     LAW   scalar | euler | maxwell2d | burgers
 //     FLUX  upwind | exact | roe | lf (Euler; the other laws have one flux each)
 //     OP    residual | fe | tvd2 | tvd3 | rk4 | write (DGAnalysis::write of the
-//           state: the data stream to OUT, the mesh to OUT.msh)
+//           state: the data stream to OUT, the mesh to OUT.msh) | project (the
+//           coefficients the constructor projected from the far field)
 //     STATE_IN: one line per cell: 3 vertex ids (any order), then Nf*Np coefficients;
 //          with ORDER 0 each line carries the cell's order after the ids, and the
 //          driver gives every cell that order through DGCell::adaptOrder
@@ -20,6 +21,9 @@
 //   (REF_VEL="vx vy", or REF_VEL=rotate for ScalarPb's (2 pi y, -2 pi x);
 //   REF_FARFIELD="q0 q1 q2 q3"); time steps start at REF_T0
 //   (default 0; Maxwell2d's RHS holds a pulse source for t <= 0.06).
+//   REF_FAR=bump: the scalar law's far field is ScalarPb's initial bump.
+//   REF_SWEEPS=N (residual): N timed sweeps before the written residual,
+//   "residual_seconds S N" on stdout.
 //
 // Only the serial code paths run (numThreads = 1): the threaded face assembly
 // races (parallel.cc:46-76), the serial one does not.
@@ -87,6 +91,19 @@ class RotatingField : public FieldEvaluator {
   virtual int order() const { return 1; }
 };
 
+// the initial condition of the reference's ScalarPb scenario (CaseOne,
+// Lilia.cc:38-58): cos^2(2 pi r) within r <= 0.25 of (-0.5, 0), zero outside;
+// the scalar law's far field with REF_FAR=bump
+class BumpField : public FieldEvaluator {
+ public:
+  virtual void eval(const mPoint& x, double, double* val) const {
+    const double pi = 3.14159265358979323846;
+    const double r = sqrt((x(0) + 0.5) * (x(0) + 0.5) + x(1) * x(1));
+    val[0] = r <= 0.25 ? cos(2 * pi * r) * cos(2 * pi * r) : 0.0;
+  }
+  virtual int order() const { return 2; }
+};
+
 // Local Lax-Friedrichs flux plugged into the reference's Euler2d law through
 // the riemannSolver virtual (ConservationLaw.h:29-33): the only part of the
 // path the reference does not provide.  Wave speed as maximumEigenValue(n,.,.)
@@ -140,8 +157,11 @@ int main(int argc, char** argv) {
   walls.insert(40000);
   int nf;
   RotatingField rot;
+  BumpField bump;
+  const bool bump_far = getenv("REF_FAR") && !strcmp(getenv("REF_FAR"), "bump");
   if (!strcmp(law_s, "scalar")) {
-    law = new ScalarLaw(&ff, rotate ? (FieldEvaluator*)&rot : (FieldEvaluator*)&vel, &ff);
+    FieldEvaluator* far = bump_far ? (FieldEvaluator*)&bump : (FieldEvaluator*)&ff;
+    law = new ScalarLaw(far, rotate ? (FieldEvaluator*)&rot : (FieldEvaluator*)&vel, far);
     nf = 1;
   } else if (!strcmp(law_s, "maxwell2d")) {
     law = new Maxwell2d(walls, &ff3);
@@ -192,7 +212,10 @@ int main(int argc, char** argv) {
     }
     if (mixed) ((DGCell*)m->getCell())->adaptOrder(orders[key]);
   }
-  for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it) {
+  // op project: the coefficients DGAnalysis's constructor projected from the far
+  // field (DGAnalysis.cc:200) are the output; the state file only gives the order
+  const bool project = !strcmp(op, "project");
+  for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim) && !project; ++it) {
     mEntity* m = *it;
     DGCell* cell = (DGCell*)m->getCell();
     std::vector<int> key;
@@ -216,8 +239,26 @@ int main(int argc, char** argv) {
     return ms.good() && ds.good() ? 0 : 6;
   }
   const bool residual = !strcmp(op, "residual");
-  if (residual) {
+  if (project) {
+  } else if (residual) {
     // serial assembly as TimeIntegrator::assembleVolume/assembleBoundary do with one thread
+    const int sweeps = getenv("REF_SWEEPS") ? atoi(getenv("REF_SWEEPS")) : 0;
+    if (sweeps > 0) {
+      // timed sweeps of both passes, as the reference's profiling harness runs them
+      // (DGAnalysis.cc:321-477, serial); the right-hand sides accumulate and are
+      // cleared before the residual that is written
+      timespec c0, c1;
+      clock_gettime(CLOCK_MONOTONIC, &c0);
+      for (int k = 0; k < sweeps; k++) {
+        for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it)
+          ((DGCell*)(*it)->getCell())->computeVolumeContribution(0.0);
+        for (mMesh::iter it = mesh->begin(dim - 1); it != mesh->end(dim - 1); ++it)
+          ((DGBoundaryCell*)(*it)->getCell())->computeBoundaryContributions(0.0);
+      }
+      clock_gettime(CLOCK_MONOTONIC, &c1);
+      printf("residual_seconds %.6f %d\n", (c1.tv_sec - c0.tv_sec) + 1e-9 * (c1.tv_nsec - c0.tv_nsec), sweeps);
+      for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it) ((DGCell*)(*it)->getCell())->ZeroRHS();
+    }
     for (mMesh::iter it = mesh->begin(dim); it != mesh->end(dim); ++it)
       ((DGCell*)(*it)->getCell())->computeVolumeContribution(0.0);
     for (mMesh::iter it = mesh->begin(dim - 1); it != mesh->end(dim - 1); ++it)

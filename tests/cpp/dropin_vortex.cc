@@ -94,13 +94,26 @@ class VelocityOne : public dgpar::FieldEvaluator {
   int order() const override { return 1; }
 };
 
+// its initial condition (CaseOne, Lilia.cc:38-58): the far field the
+// constructor projects into every cell
+class CaseOne : public dgpar::FieldEvaluator {
+ public:
+  void eval(const dgpar::mPoint& space, double, double* val) const override {
+    const double pi = 3.14159265358979323846;
+    const double r = std::sqrt((space(0) + 0.5) * (space(0) + 0.5) + space(1) * space(1));
+    val[0] = r <= 0.25 ? std::pow(std::cos(2 * pi * r), 2) : 0.0;
+  }
+  int order() const override { return 2; }
+};
+
 static int scalarpb(const std::string& out) {
   const int32_t tags[4] = {0, 0, 0, 0};
   dgpar::Mesh mesh = dgpar::Mesh::structuredTri(12, 12, -1.0, -1.0, 2.0, 2.0, true, true, 0.1, 31, tags);
-  dgpar::ConstantField i({0.0});
+  CaseOne i;
   VelocityOne v;
   dgpar::ScalarLaw theLaw(&i, &v, &i);
   dgpar::DGAnalysis analysis(mesh, &theLaw, nullptr, 1);
+  dump(out + ".UP", analysis.getState());  // the constructor's projection of CaseOne
   std::vector<double> U(analysis.ndof());
   for (size_t k = 0; k < U.size(); k++) U[k] = std::sin(0.37 * (double)k) + 0.2 * std::cos(1.3 * (double)k);
   analysis.setState(U);

@@ -87,11 +87,15 @@ def write_gmsh(mesh, path):
 NFIELDS = {"scalar": 1, "euler": 4, "maxwell2d": 3, "burgers": 1}
 
 
-def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfield=(0, 0, 0, 0), t0=0.0):
+def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfield=(0, 0, 0, 0), t0=0.0, far=None,
+        stats=None):
     """Returns (values in the caller's element order, reference vertex order per element, resid list).
     vel: the scalar law's constant velocity, or "rotate" for ScalarPb's (2 pi y, -2 pi x).
     order: one order for every cell, or a per-element array (mixed orders; U is then
-    the concatenation of the per-element blocks of Nf*Np(p_e) coefficients)."""
+    the concatenation of the per-element blocks of Nf*Np(p_e) coefficients).
+    far: "bump" for ScalarPb's initial condition as the scalar law's far field
+    (op "project" returns the constructor's projection of it).
+    stats: a dict that receives what the driver timed (REF_SWEEPS in os.environ)."""
     E = len(mesh["lat"])
     U = np.asarray(U, np.float64).ravel()
     mixed = not np.isscalar(order)
@@ -109,11 +113,17 @@ def run(mesh, law, flux, order, op, U, dt=0.0, nsteps=1, vel=(1.0, 0.5), farfiel
         env = dict(os.environ, REF_VEL=vel if isinstance(vel, str) else "%r %r" % tuple(vel[:2]),
                    REF_FARFIELD=" ".join("%r" % x for x in farfield),
                    REF_T0="%r" % t0)
+        if far:
+            env["REF_FAR"] = far
         r = subprocess.run([DRIVER, mp, law, flux, "0" if mixed else str(order), op, sp, op_, "%r" % dt,
                             str(nsteps)], cwd=d, env=env, capture_output=True, text=True, timeout=600)
         if r.returncode != 0:
             raise RuntimeError("ref_driver failed: %s\n%s" % (r.stdout[-2000:], r.stderr[-2000:]))
         resid = [float(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("resid ")]
+        if stats is not None:
+            for l in r.stdout.splitlines():
+                if l.startswith("residual_seconds ") or l.startswith("step_seconds "):
+                    stats[l.split()[0]] = float(l.split()[1])
         rows = [l.split() for l in open(op_).read().splitlines() if l.strip()]
     key = {tuple(sorted(mesh["lat"][e] + 1)): e for e in range(E)}
     out = np.zeros_like(U)

@@ -123,14 +123,21 @@ def test_cpp_dropin_write_read_export():
 @pytest.mark.skipif(not os.path.exists(EXE), reason="C++ driver not built")
 def test_cpp_dropin_scalarpb_with_a_velocity_field():
     """the reference's ScalarPb written as its scenario is: ScalarLaw(&i, &v, &i)
-    with the rotating VelocityOne FieldEvaluator, P1 (SSP-RK2, the integrator the
-    reference picks for order 1)"""
+    with the rotating VelocityOne FieldEvaluator and CaseOne's bump as the far
+    field the constructor projects, P1 (SSP-RK2, the integrator the reference
+    picks for order 1)"""
     print("\n[gpu case] C++ drop-in ScalarPb", flush=True)
     pre, dt, res = run_exe("scalarpb")
     U0, R, U2 = np.fromfile(pre + ".U0"), np.fromfile(pre + ".R"), np.fromfile(pre + ".U2")
     vids, xyz, bc = mesh_tri(12, 12, x0=-1.0, y0=-1.0, lx=2.0, ly=2.0, jitter=0.1, seed=31)
     o = po.Oracle(2, vids, xyz, 1, po.LAW_SCALAR, po.FLUX_UPWIND, farfield=(0.0,))
     o.set_velocity_field(lambda x, y: (2 * np.pi * y, -2 * np.pi * x))
+
+    def bump(x, y, z, t):
+        r = np.hypot(x + 0.5, y)
+        return (np.cos(2 * np.pi * r) ** 2 if r <= 0.25 else 0.0,)
+    UP, UPo = np.fromfile(pre + ".UP"), o.project_fn(bump)
+    assert np.abs(UPo).max() > 0.1 and np.abs(UP - UPo).max() <= 1e-13 * np.abs(UPo).max()
     Ro, _ = o.residual(U0)
     assert np.abs(R - Ro).max() / np.abs(Ro).max() < 1e-12
     assert abs(dt - o.dt(U0, 0.1)) <= 1e-14 * dt

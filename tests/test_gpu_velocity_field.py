@@ -6,7 +6,10 @@ ScalarPb velocity (2 pi y, -2 pi x) (Lilia.cc:66-82):
   dt within 1e-14 and SSP-RK3 steps within 1e-12, periodic and walled squares;
 - the reference's own mesh files (advection meshes of [-1,1]^2, quadrilaterals
   split) with their 510/610 periodic sides;
-- what the velocity field does not serve is refused."""
+- what the velocity field does not serve is refused;
+- dg_project_fn (a caller's field projected as DGAnalysis's constructor projects
+  the law's far field) within 1e-13 of the oracle's projection, triangles P1-P4
+  and tetrahedra (the cavity mode through the callback equals dg_project's)."""
 import os
 import sys
 
@@ -112,3 +115,43 @@ def test_velocity_field_refuses_what_it_does_not_serve():
         s.residual_jvp(U, U)
     with pytest.raises(dgpar.DGError):
         s.select_orders((0.05,) * 4, (1e-5,) * 4, apply=True)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_project_fn_matches_the_oracle(p):
+    """dg_project_fn of ScalarPb's bump (triangles) and of a smooth Euler state
+    against the oracle's projection of the same Python field"""
+    import dgpar
+    announce("project_fn P%d" % p)
+    vids, xyz, bc = mesh_tri(10, 10, x0=-1, y0=-1, lx=2, ly=2, jitter=0.1, seed=p)
+
+    def bump(x, y, z, t):
+        r = np.hypot(x + 0.5, y)
+        return (np.cos(2 * np.pi * r) ** 2 if r <= 0.25 else 0.0,)
+
+    def smooth(x, y, z, t):
+        return (1 + 0.2 * np.sin(x + t), 0.3 * np.cos(y), -0.1 * x * y, 2.5 + 0.1 * np.sin(x * y))
+    for law, fn, po_law in ((dgpar.Law(dgpar.LAW_SCALAR, dgpar.FLUX_UPWIND), bump, po.LAW_SCALAR),
+                            (dgpar.EulerLaw(dgpar.FLUX_LF, 1.4), smooth, po.LAW_EULER2D)):
+        g = dgpar.DGAnalysis(dgpar.Mesh(2, vids, xyz, bc), law, p)
+        g.project_fn(fn, t=0.25)
+        o = po.Oracle(2, vids, xyz, p, po_law, po.FLUX_LF if po_law == po.LAW_EULER2D else po.FLUX_UPWIND,
+                      gamma=1.4, bc=bc)
+        Uo = o.project_fn(fn, t=0.25)
+        assert np.abs(g.get_state() - Uo).max() <= 1e-13 * np.abs(Uo).max()  # as dg_project
+
+
+def test_project_fn_on_tetrahedra_and_refusals():
+    import dgpar
+    announce("project_fn tets and argument errors")
+    m = dgpar.mesh_tet_structured(2, 2, 2)
+    g = dgpar.DGAnalysis(m, dgpar.MaxwellLaw(), 2)
+    g.project(dgpar.FIELD_CAVITY, t=0.1)
+    ref = g.get_state()
+    # the cavity mode through the callback equals the built-in field's projection
+    h = dgpar.DGAnalysis(m, dgpar.MaxwellLaw(), 2)
+    law = po.make_law(po.LAW_MAXWELL3D, po.FLUX_UPWIND)
+    h.project_fn(lambda x, y, z, t: po.exact(3, (x, y, z), t, law=law), t=0.1)
+    assert np.abs(h.get_state() - ref).max() <= 1e-13 * np.abs(ref).max()
+    with pytest.raises(dgpar.DGError):
+        h.project_fn(lambda x, y, z, t: (np.nan,) * 6)

@@ -115,3 +115,63 @@ def test_emulated_kernel_with_the_rotating_field(p, form):
     Ro, _ = o.residual(U)
     tol = 5e-5 if form & 1 else 1e-12
     assert np.abs(R - Ro).max() <= tol * np.abs(Ro).max()
+
+
+def bump(x, y, z=0.0, t=0.0):
+    """ScalarPb's initial condition (CaseOne, Lilia.cc:38-58)"""
+    r = math.sqrt((x + 0.5) ** 2 + y * y)
+    return (math.cos(TWO_PI * r) ** 2 if r <= 0.25 else 0.0,)
+
+
+@need_ref
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_projection_of_a_field_matches_the_reference(p):
+    """the oracle's projection of a caller's field against what the reference's
+    DGAnalysis constructor projects from its far field (DGAnalysis.cc:200) with
+    ScalarPb's bump; the mesh is fine enough that several elements cut the bump"""
+    m = rh.square_mesh(8, jitter=0.12, seed=40 + p, periodic=True)
+    o = po.Oracle(2, m["top"], m["xyz"], p, po.LAW_SCALAR, po.FLUX_UPWIND, farfield=(0.0,), bc=m["bc"])
+    Uo = o.project_fn(bump)
+    Ur, _, _ = rh.run(m, "scalar", "upwind", p, "project", np.zeros(o.ndof), vel="rotate", far="bump")
+    assert np.abs(Uo).max() > 0.1 and np.count_nonzero(np.abs(Uo) > 1e-3) > 20
+    # the reference's printed quadrature tables set the agreement, as for residuals
+    assert np.abs(Uo - Ur).max() / np.abs(Ur).max() < OWN_TOL[p]
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_projection_reproduces_polynomials_of_the_order(p):
+    """a field of degree p lies in the element's space: its projection evaluated
+    at any point of the element is the field there"""
+    vids, xyz, bc = mesh_tri(4, 3, jitter=0.15, seed=2)
+    o = po.Oracle(2, vids, xyz, p, po.LAW_EULER2D, po.FLUX_LF, gamma=1.4)
+    cs = np.random.default_rng(p).standard_normal((4, p + 1, p + 1))
+
+    def poly(x, y, z=0.0, t=0.0):
+        return [sum(cs[f, i, j] * x ** i * y ** j for i in range(p + 1) for j in range(p + 1 - i)) for f in range(4)]
+    U = o.project_fn(poly).reshape(o.E, 4, -1)
+    rng = np.random.default_rng(9)
+    for e in range(0, o.E, 5):
+        for _ in range(3):
+            a, b = rng.uniform(0, 0.5, 2)
+            phi, _ = po.basis(2, p, (a, b, 0))
+            X = xyz[e]
+            x, y = X[0] + a * (X[1] - X[0]) + b * (X[2] - X[0])
+            assert np.allclose(U[e] @ phi, poly(x, y), rtol=0, atol=1e-12)
+
+
+@need_ref
+def test_the_reference_times_its_residual_sweeps():
+    """REF_SWEEPS: the driver times its sweeps and still writes one residual"""
+    m = rh.square_mesh(4, jitter=0.1, seed=3, periodic=True)
+    o = po.Oracle(2, m["top"], m["xyz"], 1, po.LAW_SCALAR, po.FLUX_UPWIND, farfield=(0.0,), bc=m["bc"])
+    o.set_velocity_field(rotating)
+    U = np.random.default_rng(1).standard_normal(o.ndof)
+    stats = {}
+    os.environ["REF_SWEEPS"] = "3"
+    try:
+        Rr, _, _ = rh.run(m, "scalar", "upwind", 1, "residual", U, vel="rotate", stats=stats)
+    finally:
+        del os.environ["REF_SWEEPS"]
+    assert stats["residual_seconds"] > 0
+    Ro, _ = o.residual(U)
+    assert np.abs(Ro - Rr).max() / np.abs(Rr).max() < OWN_TOL[1]
