@@ -204,7 +204,7 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
   if (m && m->dim == 3) return ctx_init_tet(c, m, law, opt);
   TriSetup S;
   std::string err;
-  dg_status st = build_tri_setup(m, law, opt, S, err);
+  dg_status st = build_tri_setup(m, law, opt, S, err, c->lts_grp.empty() ? nullptr : c->lts_grp.data());
   if (st != DG_OK) return bail(fail(c, st, "%s", err.c_str()));
   c->dim = 2;
   c->E = S.E;
@@ -604,6 +604,7 @@ static int enqueue_dt(dg_ctx* c, double cfl, double* part) {
   a.cfl = cfl;
   a.part = part;
   a.pvel = c->pvel;
+  a.dtscale = c->lts_nlev > 0 ? c->dtscale : nullptr;
   if (launch_tri_dt(a, c->law, c->stream)) return -1;
   return tri_dt_blocks(c->E);
 }
@@ -688,7 +689,10 @@ static int nstages(int scheme) {
 }
 
 // enqueue one full step on the stream; returns 0 on success
+static int enqueue_lts_step(dg_ctx* c, double h);
+
 static int enqueue_step(dg_ctx* c, int scheme, double dt, bool want_resid) {
+  if (scheme == DG_RK_TVD2_LTS) return enqueue_lts_step(c, dt);
   const int ns = nstages(scheme);
   StageArgs a = base_args(c);
   a.dt = dt;
@@ -717,6 +721,12 @@ static int enqueue_step(dg_ctx* c, int scheme, double dt, bool want_resid) {
 }
 
 static dg_status check_scheme(dg_ctx* c, int scheme) {
+  if (scheme == DG_RK_TVD2_LTS) {
+    if (c->lts_nlev <= 0) return fail(c, DG_ERR_ARG, "DG_RK_TVD2_LTS needs dg_lts_setup first");
+    if (c->lim_on || c->rec_on)
+      return fail(c, DG_ERR_UNSUPPORTED, "DG_RK_TVD2_LTS steps apply neither the limiter nor the recovery pass");
+    return DG_OK;
+  }
   if (nstages(scheme) < 0) return fail(c, DG_ERR_ARG, "unknown RK scheme %d", scheme);
   if (scheme == DG_RK_FE && c->prec != DG_PREC_FP64)
     return fail(c, DG_ERR_UNSUPPORTED, "forward Euler is built for the FP64 form only");
@@ -792,6 +802,11 @@ dg_status dg_advance(dg_ctx* c, int scheme, double t, double dt, int nsteps, dou
   // the first residual's input
   if (c->rec_on && launch_recover(recover_args(c, c->Un, 1, c->npflags, c->status), c->law, c->Nf, c->stream))
     return fail(c, DG_ERR_HIP, "recovery launch failed");
+  if (scheme == DG_RK_TVD2_LTS && resid_out) {
+    // RungeKuttaTVD2adaptive::advanceInTime returns 0 (TimeIntegrator.cc:831)
+    *resid_out = 0.0;
+    resid_out = nullptr;
+  }
   int body = resid_out ? nsteps - 1 : nsteps;
   if ((s = run_steps(c, scheme, dt, body))) return s;
   if (resid_out && nsteps > 0) {
@@ -1140,9 +1155,11 @@ static bool rebucket_in_place(dg_ctx* c, const std::vector<int32_t>& neword, dg_
   return true;
 }
 
-extern "C++" dg_status ctx_rebucket(dg_ctx* c, const std::vector<int32_t>& neword) {
-  dg_status st;
-  if (rebucket_in_place(c, neword, st)) return st;
+// a context built anew for per-element orders neword and layout groups grp
+// (both caller numbering; grp may be empty), the state carried over (raised
+// orders get zero modes, lowered orders drop the top ones) with the limiter,
+// recovery and schedule settings
+static dg_status ctx_relayout(dg_ctx* c, const std::vector<int32_t>& neword, const std::vector<int32_t>& grp) {
   std::vector<double> U(c->ndof);
   dg_status s = dg_get_state(c, U.data(), U.size());
   if (s) return s;
@@ -1156,6 +1173,7 @@ extern "C++" dg_status ctx_rebucket(dg_ctx* c, const std::vector<int32_t>& newor
   }
   dg_ctx* t = new (std::nothrow) dg_ctx();
   if (!t) return fail(c, DG_ERR_OOM, "out of memory");
+  t->lts_grp = grp;
   dg_mesh_desc m{};
   m.dim = 2;
   m.n_elems = c->E;
@@ -1173,6 +1191,7 @@ extern "C++" dg_status ctx_rebucket(dg_ctx* c, const std::vector<int32_t>& newor
   t->lim_on = c->lim_on;
   t->lim_m = c->lim_m;
   t->lim_nu = c->lim_nu;
+  t->rec_on = c->rec_on;
   t->adj_nd = c->adj_nd;
   const dg_schedule sc{c->shape, c->mix_grid, {0, 0, 0, 0, 0, 0}};
   if (schedule_ok(t, sc)) apply_schedule(t, sc);  // else the new buckets start from the default
@@ -1180,6 +1199,82 @@ extern "C++" dg_status ctx_rebucket(dg_ctx* c, const std::vector<int32_t>& newor
   free_all(t);
   delete t;
   return dg_set_state(c, V.data(), V.size());
+}
+
+extern "C++" dg_status ctx_rebucket(dg_ctx* c, const std::vector<int32_t>& neword) {
+  dg_status st;
+  if (rebucket_in_place(c, neword, st)) return st;
+  // new orders end local time stepping (uniform orders only)
+  return ctx_relayout(c, neword, {});
+}
+
+// ---------------------------------------------------------------------------
+// local time stepping (DGAnalysis::sortCellsBySize + RungeKuttaTVD2adaptive;
+// the method in dg_lts.h)
+
+dg_status dg_lts_setup(dg_ctx* c, int max_levels, int32_t* levels_out) {
+  if (!ready(c) || max_levels < 1 || max_levels > LTS_MAX_LEVELS)
+    return fail(c, DG_ERR_ARG, "dg_lts_setup: bad argument (max_levels 1..%d)", LTS_MAX_LEVELS);
+  if (c->dim != 2 || c->mixed) return fail(c, DG_ERR_UNSUPPORTED, "local time stepping runs triangles of one order");
+  if (c->prec != DG_PREC_FP64 || c->kernel != 0)
+    return fail(c, DG_ERR_UNSUPPORTED, "local time stepping runs the FP64 form of the tiled kernel");
+  if (c->pvel)
+    return fail(c, DG_ERR_UNSUPPORTED, "set the velocity field after dg_lts_setup (the elements are re-laid out)");
+  HIPCHECK(c, hipSetDevice(c->device));
+  const int E = c->E;
+  std::vector<double> size(E);
+  for (int e = 0; e < E; e++) size[e] = tri_cell_size(c->xyz.data() + (size_t)e * 6);
+  std::vector<int32_t> lev, m1, m2, cid;
+  std::vector<LtsClass> cls;
+  const int nlev = lts_levels(E, size.data(), c->faces, max_levels, lev);
+  lts_reach(E, c->faces, lev, m1, m2);
+  lts_classify(E, lev, m1, m2, cid, cls);
+  const std::vector<int32_t> ord(E, c->P);
+  dg_status s = ctx_relayout(c, ord, cid);
+  if (s) return s;
+  // each class is one internal range
+  for (auto& k : cls) k.e0 = k.e1 = -1;
+  for (int i = 0; i < E; i++) {
+    LtsClass& k = cls[cid[c->perm[i]]];
+    if (k.e0 < 0) k.e0 = i;
+    else if (k.e1 != i) return fail(c, DG_ERR_ARG, "local time stepping: class %d not contiguous", cid[c->perm[i]]);
+    k.e1 = i + 1;
+  }
+  std::vector<double> scale(E);
+  for (int i = 0; i < E; i++) scale[i] = (double)(1 << lev[c->perm[i]]);
+  if ((s = dalloc(c, &c->dtscale, (size_t)E))) return s;
+  HIPCHECK(c, hipMemcpy(c->dtscale, scale.data(), (size_t)E * 8, hipMemcpyHostToDevice));
+  c->lts_nlev = nlev;
+  c->lts_level = lev;
+  c->lts_cls = cls;
+  if (levels_out) std::memcpy(levels_out, lev.data(), (size_t)E * 4);
+  return DG_OK;
+}
+
+// one DG_RK_TVD2_LTS step of size h on the stream: the stage input of even
+// stages is Un (the span-start states), of odd stages A64 (the predictors);
+// acc holds each element's span increment
+static int enqueue_lts_step(dg_ctx* c, double h) {
+  std::vector<LtsLaunch> plan;
+  lts_schedule(c->lts_nlev, c->lts_cls, h, plan);
+  StageArgs a = base_args(c);
+  a.scheme = SCHEME_LTS;
+  const int shape = c->shape == 1 ? 1 : 0;  // the FP64 tile shapes
+  for (const LtsLaunch& L : plan) {
+    if (L.kind == LTS_COMMIT) {
+      if (launch_lts_commit(c->E, c->Nf * c->NP, L.e0, L.e1, c->acc, c->Un, c->stream)) return -1;
+      continue;
+    }
+    a.e0 = L.e0;
+    a.e1 = L.e1;
+    a.stage = L.flags;
+    a.dt = L.hb;
+    a.lts_w = L.w;
+    a.Uin = L.input ? (const void*)c->A64 : (const void*)c->Un;
+    a.Uout = (L.flags & LTS_ODD) ? (void*)c->Un : (void*)c->A64;
+    if (launch_tri_stage_tiled(a, c->law, shape << 2, c->stream)) return -1;
+  }
+  return 0;
 }
 
 dg_status dg_select_orders(dg_ctx* c, const dg_sensor_opts* o, int apply, int32_t* orders_out) {

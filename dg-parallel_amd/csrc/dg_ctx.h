@@ -11,6 +11,7 @@
 
 #include "../../include/dg_capi.h"
 #include "dg_laws.h"
+#include "dg_lts.h"
 #include "dg_setup.h"
 
 using namespace dgp;
@@ -75,6 +76,14 @@ struct dg_ctx {
   int rec_on = 0;
   // the scalar law's velocity at every quadrature point (dg_set_velocity_field), or null
   double* pvel = nullptr;
+  // local time stepping (dg_lts_setup): levels (caller numbering), the element
+  // classes' internal ranges (the layout groups each class: lts_grp, caller
+  // numbering), and 2^level per internal element on the device (the time step
+  // of the coarsest level)
+  int lts_nlev = 0;
+  std::vector<int32_t> lts_grp, lts_level;
+  std::vector<LtsClass> lts_cls;
+  double* dtscale = nullptr;
   uint8_t* npflags = nullptr;
   unsigned* status = nullptr;
   std::vector<void*> allocs;

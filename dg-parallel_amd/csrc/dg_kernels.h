@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "dg_laws.h"
+#include "dg_lts.h"
 
 namespace dgp {
 
@@ -21,6 +22,9 @@ struct TriElemTable {
 
 // stage modes
 enum { MODE_RESIDUAL = -1 };
+// the stage of local time stepping (dg_lts.h): stage holds the LTS_* flags, dt
+// the level's substep h_b and lts_w the weight of h_b k in the span increment
+enum { SCHEME_LTS = 4 };
 
 struct StageArgs {
   int E;
@@ -41,6 +45,10 @@ struct StageArgs {
   // the scalar law's velocity at the quadrature points (a velocity field), FP64,
   // [NQV + 3 NQF][2][E]: volume points, then face l point q at NQV + l NQF + q
   const double* pvel;
+  // the tiled kernels update internal elements [e0, e1) only (e1 = 0: all E);
+  // every element's rows keep the stride E
+  int e0, e1;
+  double lts_w;
 };
 
 // variant: 0 = FP64 throughout; 1 = FP32 arithmetic, FP64 input, FP32 output;
@@ -68,6 +76,7 @@ struct DtArgs {
   double cfl;
   double* part;          // per-block minima
   const double* pvel;    // the velocity table of StageArgs (or null)
+  const double* dtscale; // per-element factor of dt (local time stepping: 2^level), or null
 };
 int launch_tri_dt(const DtArgs& a, int law, hipStream_t s);
 int tri_dt_blocks(int E);
@@ -85,5 +94,7 @@ int launch_min_reduce(const double* part, int n, double* out, hipStream_t s);
 // *status |= 4 when any of U[0 .. n) is NaN or Inf (one atomic per wave that meets one)
 int launch_nonfinite_scan(const double* U, long n, unsigned* status, hipStream_t s);
 int launch_fill_one(double* out, double v, hipStream_t s);
+// local time stepping: Un += acc on rows [0, nrow) of internal elements [e0, e1)
+int launch_lts_commit(int E, int nrow, int e0, int e1, const double* acc, double* Un, hipStream_t s);
 
 }  // namespace dgp

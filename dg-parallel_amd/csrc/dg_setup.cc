@@ -99,16 +99,20 @@ dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err) {
     for (int r = 0; r < 2; r++)
       for (int d = 0; d < 2; d++) T.mg[(size_t)e * 4 + r * 2 + d] = ad * inv[r][d];
     T.detj[e] = ad;
-    double a = std::sqrt((X[2] - X[0]) * (X[2] - X[0]) + (X[3] - X[1]) * (X[3] - X[1]));
-    double b = std::sqrt((X[4] - X[2]) * (X[4] - X[2]) + (X[5] - X[3]) * (X[5] - X[3]));
-    double cc = std::sqrt((X[0] - X[4]) * (X[0] - X[4]) + (X[1] - X[5]) * (X[1] - X[5]));
-    double hp = 0.5 * (a + b + cc);
-    T.size[e] = std::sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
+    T.size[e] = tri_cell_size(X);
   }
   return DG_OK;
 }
 
-void tri_layout(const TriTopo& T, const int* ord, TriSetup& S) {
+double tri_cell_size(const double* X) {
+  const double a = std::sqrt((X[2] - X[0]) * (X[2] - X[0]) + (X[3] - X[1]) * (X[3] - X[1]));
+  const double b = std::sqrt((X[4] - X[2]) * (X[4] - X[2]) + (X[5] - X[3]) * (X[5] - X[3]));
+  const double cc = std::sqrt((X[0] - X[4]) * (X[0] - X[4]) + (X[1] - X[5]) * (X[1] - X[5]));
+  const double hp = 0.5 * (a + b + cc);
+  return std::sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
+}
+
+void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* grp) {
   const int E = T.E, NF = S.NF;
   S.mixed = false;
   for (int e = 1; e < E; e++)
@@ -136,6 +140,15 @@ void tri_layout(const TriTopo& T, const int* ord, TriSetup& S) {
       for (; k > run[p] && T.sig[S.perm[k - 1]] > sg; k--) S.perm[k] = S.perm[k - 1];
       S.perm[k] = e;
     }
+  }
+  if (grp && !S.mixed) {
+    // one order: groups first, then the same blocks and signatures
+    for (int e = 0; e < E; e++) S.perm[e] = e;
+    std::stable_sort(S.perm.begin(), S.perm.end(), [&](int a, int b) {
+      if (grp[a] != grp[b]) return grp[a] < grp[b];
+      if (a / BLK != b / BLK) return a / BLK < b / BLK;
+      return T.sig[a] < T.sig[b];
+    });
   }
   std::vector<int32_t> iperm(E);
   for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
@@ -167,7 +180,7 @@ void tri_layout(const TriTopo& T, const int* ord, TriSetup& S) {
 }
 
 dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TriSetup& S,
-                          std::string& err) {
+                          std::string& err, const int32_t* grp) {
   if (!m || !law || !opt || !m->vids || !m->xyz || m->n_elems <= 0) return failS(err, DG_ERR_ARG, "null mesh/law/options");
   if (m->dim != 2) return failS(err, DG_ERR_UNSUPPORTED, "only triangles (dim 2) are built");
   S.E = m->n_elems;
@@ -220,7 +233,7 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
         b = BC_FARFIELD;
     }
   S.faces = S.topo.faces;
-  tri_layout(S.topo, ord.data(), S);
+  tri_layout(S.topo, ord.data(), S, grp);
   const int P = S.P;
 
   // ---- tables ---------------------------------------------------------------------

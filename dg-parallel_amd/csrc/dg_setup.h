@@ -23,6 +23,10 @@ struct TriTopo {
   std::vector<int32_t> faces;           // (eL, lL, eR, lR)
 };
 
+// a triangle's size, 2 x inradius (DGCell::getSize, DGCell.cc:1209-1229), from its
+// corners [3][2]
+double tri_cell_size(const double* X);
+
 // Caller topology and geometry (mMesh.cc:321-378, Mapping.cc:442-566,
 // DGCell.cc:1209-1229); validates the mesh.
 dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err);
@@ -60,15 +64,17 @@ struct TriSetup {
 };
 
 dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TriSetup& S,
-                          std::string& err);
+                          std::string& err, const int32_t* grp = nullptr);
 
 // The internal numbering and element tables of S for per-element orders ord
 // (caller numbering, 1..4) over topology T: perm, order, bucket bounds, coff,
 // mg, detj, size, link and the mixed flag (S.E, S.NF set by the caller).
 // Internal order: order buckets, then consecutive blocks of 128 caller elements,
 // then link signature (stable), so a wavefront takes one path and face
-// neighbours sit in nearby tiles.
-void tri_layout(const TriTopo& T, const int* ord, TriSetup& S);
+// neighbours sit in nearby tiles.  grp (caller numbering, uniform orders only):
+// a group key ahead of the blocks (the classes of local time stepping), so each
+// group is one internal range.
+void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* grp = nullptr);
 
 // Tetrahedra (Maxwell, one order): the same roles as TriSetup.  mg holds
 // |J| J^-1 as [9][E] (row-major r*3+d), size the shortest edge (DGCell.cc:1230-1262),

@@ -180,6 +180,9 @@ struct TiledStage {
     typename std::conditional<MF, MfmaAccOn, MfmaAccOff>::type m;
   };
 
+  // the tile's first element and the end of the launch's element range
+  DG_HD static int range_end(const StageArgs& a) { return a.e1 > 0 ? a.e1 : a.E; }
+
   DG_HD static const CT* table(const StageArgs& a) {
     return sizeof(CT) == 8 ? (const CT*)(const void*)a.tab : (const CT*)(const void*)a.tabf;
   }
@@ -193,9 +196,9 @@ struct TiledStage {
   //      geometry and links -> registers ----------------------------------------
   DG_HD static void phaseA(const StageArgs& a, Smem& s, int tid, int tile, Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const int E = a.E, e = tile * T + t;
-    const bool valid = e < E;
-    const int es = valid ? e : 0;  // padding lanes load element 0 (in bounds) and discard it
+    const int E = a.E, e = a.e0 + tile * T + t;
+    const bool valid = e < range_end(a);
+    const int es = valid ? e : a.e0;  // padding lanes load the range's first element (in bounds) and discard it
     const IT* Uin = (const IT*)a.Uin;
     g.es = es;
     // Every global load of the phase is issued, unconditionally and in this
@@ -359,8 +362,8 @@ struct TiledStage {
 
   DG_HD static void phaseC1(const StageArgs& a, Smem& s, int tid, int tile, const Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const int e = tile * T + t;
-    const bool valid = e < a.E;
+    const int e = a.e0 + tile * T + t;
+    const bool valid = e < range_end(a);
     const CT* tb = table(a);
     const auto& v = g.nv;  // loaded in phase A
 #pragma unroll
@@ -475,15 +478,16 @@ struct TiledStage {
   // ---- D2 + stage update ----------------------------------------------------
   DG_HD static void phaseD2(const StageArgs& a, Smem& s, int tid, int tile, Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const int E = a.E, e = tile * T + t;
-    const bool valid = e < E;
-    const int es = valid ? e : 0;
+    const int E = a.E, e = a.e0 + tile * T + t;
+    const bool valid = e < range_end(a);
+    const int es = valid ? e : a.e0;
     const CT* tb = table(a);
     const IT* Uin = (const IT*)a.Uin;
     const int sc = a.scheme * 4 + a.stage;
     const bool residual_only = a.scheme == MODE_RESIDUAL;
-    const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
-    const bool reads_acc = sc == 3 * 4 + 1 || sc == 3 * 4 + 2 || sc == 3 * 4 + 3;
+    const bool lts = a.scheme == SCHEME_LTS;
+    const bool reads_input = !lts && (sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1);
+    const bool reads_acc = lts ? !(a.stage & LTS_RESET) : (sc == 3 * 4 + 1 || sc == 3 * 4 + 2 || sc == 3 * 4 + 3);
     const double dtj = valid ? a.dt / a.el.detj[e] : 0.0;
     g.rsum = 0.0;
 #pragma unroll
@@ -528,6 +532,16 @@ struct TiledStage {
             const CT r = g.R[fi * KPW + ki];
             if (residual_only) {
               a.R[i] = double(r);
+            } else if (lts) {
+              // span increment += w h_b k; even stages write the predictor
+              // y_m + h_b k, the span's last stage writes y_m + increment
+              const double v = double(r) * dtj, inc = ac[ki] + a.lts_w * v;
+              if (!(a.stage & LTS_ODD))
+                ((OT*)a.Uout)[i] = OT(un[ki] + v);
+              if (a.stage & LTS_END)
+                ((OT*)a.Uout)[i] = OT(un[ki] + inc);
+              else
+                a.acc[i] = inc;
             } else {
               double accv = ac[ki];
               const double o = stage_value_hd(sc, double(r) * dtj, un[ki], us[ki], &accv, g.rsum);

@@ -58,9 +58,14 @@ __global__ __launch_bounds__(NW * 64) void tri_stage_tiled(StageArgs a) {
   }
 }
 
+// tiles of the launch's element range [e0, e1) (e1 = 0: the whole mesh)
+static inline int tiled_grid(const StageArgs& a) {
+  return ((a.e1 > 0 ? a.e1 : a.E) - a.e0 + TILE_ELEMS - 1) / TILE_ELEMS;
+}
+
 template <int P, class LAW, int NW>
 static int launch_tiled_p(const StageArgs& a, int prec, hipStream_t s) {
-  dim3 grid((a.E + TILE_ELEMS - 1) / TILE_ELEMS), block(NW * 64);
+  dim3 grid(tiled_grid(a)), block(NW * 64);
   switch (prec) {
     case 0:
       hipLaunchKernelGGL((tri_stage_tiled<P, LAW, double, double, double, NW>), grid, block, 0, s, a);
@@ -94,7 +99,7 @@ static int launch_tiled_law(const StageArgs& a, int prec, hipStream_t s) {
 // the MFMA tile shape: FP32 forms (prec 1..3) of the four-field law
 template <int P, class LAW>
 static int launch_tiled_mf_p(const StageArgs& a, int prec, hipStream_t s) {
-  dim3 grid((a.E + TILE_ELEMS - 1) / TILE_ELEMS), block(4 * 64);
+  dim3 grid(tiled_grid(a)), block(4 * 64);
   switch (prec) {
     case 1:
       hipLaunchKernelGGL((tri_stage_tiled<P, LAW, float, double, float, 4, true>), grid, block, 0, s, a);

@@ -36,7 +36,7 @@ def device_synchronize():
 DG_OK, DG_ERR_ARG, DG_ERR_OOM, DG_ERR_HIP, DG_ERR_NONFINITE, DG_ERR_NONPHYSICAL, DG_ERR_UNSUPPORTED = range(7)
 LAW_SCALAR, LAW_EULER2D, LAW_MAXWELL3D, LAW_MAXWELL2D, LAW_BURGERS2D = 0, 1, 2, 3, 4
 FLUX_UPWIND, FLUX_LF, FLUX_ROE, FLUX_CENTRAL, FLUX_ROE_EF, FLUX_EXACT = 0, 1, 2, 3, 4, 5
-RK_FE, RK_TVD2, RK_TVD3, RK4 = 0, 1, 2, 3
+RK_FE, RK_TVD2, RK_TVD3, RK4, RK_TVD2_LTS = 0, 1, 2, 3, 4
 PREC_FP64, PREC_FP32 = 0, 1
 TAG_FARFIELD, TAG_FARFIELD2, TAG_CURVED_WALL, TAG_CHARACTERISTIC, TAG_WALL = 1, 2, 20000, 30000, 40000
 FIELD_GAUSSIAN, FIELD_VORTEX, FIELD_SOD, FIELD_CAVITY = 0, 1, 2, 3
@@ -91,6 +91,7 @@ _compute_dt = _sig("dg_compute_dt", ctypes.c_int, [_vp, ctypes.c_double, _p_f64]
 _advance = _sig("dg_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _p_f64])
 _time_advance = _sig("dg_time_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                        ctypes.POINTER(ctypes.c_float)])
+_lts_setup = _sig("dg_lts_setup", ctypes.c_int, [_vp, ctypes.c_int, _p_i32])
 class SensorOpts(ctypes.Structure):
     _fields_ = [("s_hi", ctypes.c_double * 4), ("s_lo", ctypes.c_double * 4), ("pmin", ctypes.c_int),
                 ("pmax", ctypes.c_int)]
@@ -392,6 +393,16 @@ class DGAnalysis:
         err = np.zeros(self.law.nfields)
         self._check(_l2_error(self._h, field, _ptr(par, _p_f64), t, _ptr(err, _p_f64)))
         return err
+
+    def sortCellsBySize(self, max_levels):
+        """local time stepping (DGAnalysis::sortCellsBySize + RungeKuttaTVD2adaptive):
+        levels by cell size; afterwards computeDt returns the coarsest level's step
+        and RK_TVD2_LTS steps advance every level.  Returns the levels (caller order)."""
+        lev = np.zeros(self.mesh_E, np.int32)
+        self._check(_lts_setup(self._h, max_levels, _ptr(lev, _p_i32)))
+        return lev
+
+    lts_setup = sortCellsBySize
 
     def time_advance(self, scheme, dt, nsteps):
         ms = ctypes.c_float()

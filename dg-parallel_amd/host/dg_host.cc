@@ -377,6 +377,14 @@ DGAnalysis::DGAnalysis(const Mesh& mesh, ConservationLaw* law, DGLimiter* limite
     integrator_ = new RungeKutta4(this);
 }
 
+std::vector<int32_t> DGAnalysis::sortCellsBySize(int max_levels) {
+  std::vector<int32_t> lev(n_elems_);
+  check(dg_lts_setup(ctx_, max_levels, lev.data()));
+  delete integrator_;
+  integrator_ = new RungeKuttaTVD2adaptive(this);
+  return lev;
+}
+
 DGAnalysis::~DGAnalysis() {
   delete integrator_;
   if (ctx_) dg_destroy(ctx_);

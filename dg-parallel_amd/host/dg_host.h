@@ -268,6 +268,13 @@ class RungeKutta4 : public TimeIntegrator {
  public:
   explicit RungeKutta4(DGAnalysis* a) : TimeIntegrator(a, DG_RK4) {}
 };
+// SSP-RK2 with local time steps per size level (TimeIntegrator.h:45-70,
+// TimeIntegrator.cc:442-833): dt is the coarsest level's step; advanceInTime
+// returns 0 as the reference's does
+class RungeKuttaTVD2adaptive : public TimeIntegrator {
+ public:
+  explicit RungeKuttaTVD2adaptive(DGAnalysis* a) : TimeIntegrator(a, DG_RK_TVD2_LTS) {}
+};
 
 // DGAnalysis (DGAnalysis.h:110-135): owns the GPU context and the integrator,
 // chosen from the order as the reference does (DGAnalysis.cc:239-242).
@@ -306,6 +313,12 @@ class DGAnalysis {
   // sensor; re-buckets the elements by their new orders; returns the orders
   std::vector<int32_t> adaptP(const dg_sensor_opts& opts);
   std::vector<int32_t> orders() const;
+  // local time stepping (DGAnalysis::sortCellsBySize, DGAnalysis.cc:749-890, and
+  // the integrator the reference selects with it, DGAnalysis.cc:232-234): cells
+  // binned by size into at most max_levels levels, the integrator replaced by
+  // RungeKuttaTVD2adaptive; computeDt then returns the coarsest level's step.
+  // Returns each cell's level (caller order).
+  std::vector<int32_t> sortCellsBySize(int max_levels = 6);
   // launch schedule of the stage kernel (no counterpart in the reference): the
   // tuner times every candidate over `steps` steps of the integrator's scheme
   // with step dt on the current state (restored after) and keeps the fastest;

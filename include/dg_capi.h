@@ -50,7 +50,9 @@ enum { DG_LAW_SCALAR = 0, DG_LAW_EULER2D = 1, DG_LAW_MAXWELL3D = 2, DG_LAW_MAXWE
  * (EulerLaw.cc:602-887: Newton for p*, at most 12 iterations to 1e-14) */
 enum { DG_FLUX_UPWIND = 0, DG_FLUX_LF = 1, DG_FLUX_ROE = 2, DG_FLUX_CENTRAL = 3, DG_FLUX_ROE_EF = 4,
        DG_FLUX_EXACT = 5 };
-enum { DG_RK_FE = 0, DG_RK_TVD2 = 1, DG_RK_TVD3 = 2, DG_RK4 = 3 };
+/* DG_RK_TVD2_LTS: RungeKuttaTVD2adaptive, SSP-RK2 with local time steps per
+ * size level (TimeIntegrator.cc:442-833); needs dg_lts_setup */
+enum { DG_RK_FE = 0, DG_RK_TVD2 = 1, DG_RK_TVD3 = 2, DG_RK4 = 3, DG_RK_TVD2_LTS = 4 };
 enum { DG_PREC_FP64 = 0, DG_PREC_FP32 = 1 };
 /* Boundary tags (the reference's classification ids, EulerLaw.cc:182-296):
  * 40000 reflective wall; 20000 the reference's curved wall (EulerLaw.cc:109-179,
@@ -166,6 +168,31 @@ dg_status dg_compute_dt_noalloc(dg_ctx* ctx, double cfl, double* dt_dev);
  * RungeKutta4: TimeIntegrator.cc:81-275,279-438,838-990,993-1048).  resid_out
  * (optional) receives the last step's residual norm as the reference returns it. */
 dg_status dg_advance(dg_ctx* ctx, int rk_scheme, double t, double dt, int nsteps, double* resid_out);
+
+/* ---- local time stepping (triangles, FP64 form, uniform order) ------------
+ * DGAnalysis::sortCellsBySize (DGAnalysis.cc:749-890) with the integrator it
+ * selects, RungeKuttaTVD2adaptive (DGAnalysis.cc:232-234; TimeIntegrator.cc:
+ * 442-833).  Cells are binned by size (DGCell::getSize, 2 x inradius): level b
+ * holds the cells with size <= 1.0001 smax / 2^b (the largest b below
+ * max_levels, 1..6) and takes 2^b SSP-RK2 substeps per step; face neighbours
+ * are kept within one level of each other by moving the coarser one a level
+ * down (the reference stops with "Too big a difference between elements").
+ * The reference's range assembly is empty (TimeIntegrator.cc:1438-1487), so its
+ * adaptive step advances nothing; this build's multirate step is a new design:
+ * a partitioned Runge-Kutta method of 2^levels stages whose levels share the
+ * stage weights, so it is conservative and second order, and on one level it is
+ * SSP-RK2.  Elements away from finer ones skip the stages whose inputs repeat.
+ * After this call (which re-lays out the elements by level, keeping the state),
+ * dg_compute_dt returns the step of the coarsest level, min over cells of their
+ * CFL step x 2^level (the reference's dt x 2^(bins-1), DGAnalysis.cc:562-589),
+ * and dg_advance / dg_time_advance take DG_RK_TVD2_LTS steps of that size.
+ * levels_out (caller order, may be NULL) receives each cell's level; the return
+ * is DG_OK, or DG_ERR_UNSUPPORTED for tetrahedra, mixed orders, the FP32 form,
+ * the one-element kernel or a context with a velocity field (set the field after
+ * this call).  The limiter and the recovery pass are not applied by
+ * DG_RK_TVD2_LTS steps (DG_ERR_UNSUPPORTED when enabled).  max_levels = 1 puts
+ * every cell on level 0. */
+dg_status dg_lts_setup(dg_ctx* ctx, int max_levels, int32_t* levels_out);
 
 /* Device-resident variants for callers that keep state on the GPU (torch tensors):
  * pointers are device pointers holding dg_ndof() doubles in the boundary layout. */

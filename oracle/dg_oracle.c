@@ -25,6 +25,12 @@
  *   RK stages        TimeIntegrator.cc:81-275 (RK4), 279-438 (TVD2), 838-990 (TVD3),
  *                    993-1048 (FE).  Pinned by order of accuracy on linear ODE modes.
  *   dt               DGCell.cc:1155-1262.  Closed form.
+ *   local time steps New design over the bins of DGAnalysis.cc:749-890
+ *                    (RungeKuttaTVD2adaptive's shape, TimeIntegrator.cc:442-833).
+ *                    Pinned by: equality with SSP-RK2 on one level, exact
+ *                    conservation, free streams, second-order convergence, and a
+ *                    textbook partitioned Runge-Kutta step built from its Butcher
+ *                    tableaux (tests/test_lts.py).
  *
  * Face accumulation is race-free: each face writes its own contribution block,
  * then every element gathers its faces in local-face order.  Results do not
@@ -1737,10 +1743,11 @@ static double law_maxev(const or_law* L, const double* Q) {
   return 1.0; /* Maxwell with eps = mu = 1: light speed */
 }
 
-double or_dt(or_ctx* c, const double* U, double cfl) {
+/* per-element time step of or_dt, each element's candidate into dte[E] */
+static void elem_dts(or_ctx* c, const double* U, double cfl, double* dte) {
   int dim = c->dim, Nf = nfields_of(&c->law);
-  double dt = 1e300;
   for (int e = 0; e < c->E; e++) {
+    double dt = 1e300;
     int p = c->order[e], Np = np_of(dim, p);
     const double* Ue = U + c->uoff[e];
     const double* X = c->xyz + (long)e * c->nv * dim;
@@ -1780,6 +1787,7 @@ double or_dt(or_ctx* c, const double* U, double cfl) {
       double a = fabs(v[0]) + fabs(v[1]) + fabs(v[2]);
       double d = cfl * size / (a * (2. * p + 1.0));
       if (d < dt) dt = d;
+      dte[e] = dt;
       continue;
     }
     double a = law_maxev(&c->law, mean);
@@ -1799,8 +1807,116 @@ double or_dt(or_ctx* c, const double* U, double cfl) {
     }
     double d = size * cfl / (a * (2. * p + 1.0));
     if (dt > d) dt = d;
+    dte[e] = dt;
   }
+}
+
+double or_dt(or_ctx* c, const double* U, double cfl) {
+  double* dte = (double*)malloc(sizeof(double) * c->E);
+  elem_dts(c, U, cfl, dte);
+  double dt = 1e300;
+  for (int e = 0; e < c->E; e++)
+    if (dte[e] < dt) dt = dte[e];
+  free(dte);
   return dt;
+}
+
+/* ------------------------------------------------------------------------- */
+/* local time stepping (new design; the reference's shape: sortCellsBySize,   */
+/* DGAnalysis.cc:749-890, with RungeKuttaTVD2adaptive, TimeIntegrator.cc:      */
+/* 442-833, whose range assembly is empty, TimeIntegrator.cc:1438-1487)        */
+/* ------------------------------------------------------------------------- */
+int or_lts_levels(or_ctx* c, int max_levels, int32_t* level) {
+  if (c->dim != 2 || max_levels < 1) return -1;
+  double smax = 0.0;
+  for (int e = 0; e < c->E; e++) {
+    double s = tri_size(c, e);
+    if (s > smax) smax = s;
+  }
+  for (int e = 0; e < c->E; e++) {
+    /* bin b: size <= BB[b] * 1.0001, BB[b] = smax / 2^b (DGAnalysis.cc:772-803) */
+    double s = tri_size(c, e), bb = smax;
+    int b = 0;
+    while (b + 1 < max_levels && s <= 0.5 * bb * 1.0001) {
+      bb *= 0.5;
+      b++;
+    }
+    level[e] = b;
+  }
+  /* neighbours at most one level apart (the reference exits instead,
+   * DGAnalysis.cc:836): the coarser side moves a level down */
+  int changed = 1;
+  while (changed) {
+    changed = 0;
+    for (int f = 0; f < c->nf; f++) {
+      int a = c->faces[f].eL, r = c->faces[f].eR;
+      if (r < 0) continue;
+      if (level[a] + 1 < level[r]) {
+        level[a] = level[r] - 1;
+        changed = 1;
+      }
+      if (level[r] + 1 < level[a]) {
+        level[r] = level[a] - 1;
+        changed = 1;
+      }
+    }
+  }
+  int top = 0;
+  for (int e = 0; e < c->E; e++)
+    if (level[e] > top) top = level[e];
+  return top + 1;
+}
+
+double or_lts_dt(or_ctx* c, const double* U, double cfl, const int32_t* level) {
+  double* dte = (double*)malloc(sizeof(double) * c->E);
+  elem_dts(c, U, cfl, dte);
+  double dt = 1e300;
+  for (int e = 0; e < c->E; e++) {
+    double d = dte[e] * (double)(1 << level[e]);
+    if (d < dt) dt = d;
+  }
+  free(dte);
+  return dt;
+}
+
+/* One step of size h: every level-b element takes 2^b Heun substeps of h / 2^b
+ * inside one sequence of S = 2^levels stages shared by all elements.  Stage i
+ * is stage j = i mod L (L = S / 2^b) of the element's current substep; its
+ * stage state is y (j even) or y + (h/2^b) k_{j-1} (j odd), every element's
+ * residual of stage i is evaluated on all elements' stage-i states, and at
+ * j = L - 1 the substep ends: y += (h/2^b) (1/L) sum_j k_j.  Every stage is
+ * evaluated for every element (the product skips stages whose inputs repeat). */
+int or_lts_step(or_ctx* c, const int32_t* level, double* U, double t, double h, int nthreads) {
+  if (c->dim != 2) return -1;
+  int top = 0;
+  for (int e = 0; e < c->E; e++)
+    if (level[e] > top) top = level[e];
+  const int S = 1 << (top + 1);
+  const long n = c->ndof;
+  double* Y = (double*)malloc(sizeof(double) * n);
+  double* yp = (double*)malloc(sizeof(double) * n);
+  double* acc = (double*)calloc(n, sizeof(double));
+  double* R = (double*)malloc(sizeof(double) * n);
+  for (int i = 0; i < S; i++) {
+    memcpy(Y, (i & 1) ? yp : U, sizeof(double) * n);
+    or_residual(c, Y, t, R, nthreads);
+    for (int e = 0; e < c->E; e++) {
+      const int b = level[e], L = S >> b, j = i % L;
+      const double hb = h / (double)(1 << b);
+      for (long q = c->uoff[e]; q < c->uoff[e + 1]; q++) {
+        const double v = hb * R[q] / c->detj[e];
+        if (j == 0) acc[q] = 0.0;
+        acc[q] += v;
+        if (!(j & 1)) yp[q] = U[q] + v;
+        if (j == L - 1) U[q] += acc[q] / L;
+      }
+    }
+  }
+  free(Y);
+  free(yp);
+  free(acc);
+  free(R);
+  return 0;
 }
 
 /* ------------------------------------------------------------------------- */

@@ -72,6 +72,16 @@ int or_residual_abs(or_ctx* c, const double* U, double t, double* Rabs, int nthr
 double or_step(or_ctx* c, int scheme, double* U, double t, double dt, int nthreads);
 /* global min of CFL*size/((2p+1) lambda) (DGCell.cc:1155-1262) */
 double or_dt(or_ctx* c, const double* U, double cfl);
+/* local time stepping (triangles; new design over the size bins of
+ * DGAnalysis::sortCellsBySize, DGAnalysis.cc:749-890): level of each element
+ * (bin b: size <= 1.0001 smax / 2^b, the largest b < max_levels; face
+ * neighbours raised to within one level), returns the number of levels;
+ * or_lts_dt: min over elements of their or_dt candidate x 2^level; or_lts_step:
+ * one step of size h in place, level-b elements taking 2^b Heun substeps inside
+ * one shared sequence of 2^levels stages (every stage evaluated everywhere) */
+int or_lts_levels(or_ctx* c, int max_levels, int32_t* level);
+double or_lts_dt(or_ctx* c, const double* U, double cfl, const int32_t* level);
+int or_lts_step(or_ctx* c, const int32_t* level, double* U, double t, double h, int nthreads);
 /* per-element detJ (reference-to-physical) and mean of field f */
 void or_detj(const or_ctx* c, double* out);
 

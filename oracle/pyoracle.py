@@ -77,6 +77,10 @@ def _load():
     lib.or_project_fn.argtypes = [vp, FIELD_FN, vp, ctypes.c_double, _p_f64]
     lib.or_riemann.argtypes = [ctypes.POINTER(_Law), _p_f64, _p_f64, _p_f64, _p_f64]
     lib.or_riemann_exact.argtypes = [ctypes.c_double, _p_f64, _p_f64, ctypes.c_double, _p_f64]
+    lib.or_lts_levels.argtypes = [vp, ctypes.c_int, _p_i32]
+    lib.or_lts_dt.restype = ctypes.c_double
+    lib.or_lts_dt.argtypes = [vp, _p_f64, ctypes.c_double, _p_i32]
+    lib.or_lts_step.argtypes = [vp, _p_i32, _p_f64, ctypes.c_double, ctypes.c_double, ctypes.c_int]
     lib.or_set_tri_basis.argtypes = [ctypes.c_int, _p_f64]
     lib.or_set_tri_rule.argtypes = [ctypes.c_int, ctypes.c_int, _p_f64]
     return lib
@@ -223,6 +227,27 @@ class Oracle:
     def dt(self, U, cfl):
         U = np.ascontiguousarray(U, np.float64)
         return _lib.or_dt(self._h, _p(U, _p_f64), cfl)
+
+    def lts_levels(self, max_levels):
+        """local time stepping: (levels per element, number of levels)"""
+        lev = np.zeros(self.E, np.int32)
+        n = _lib.or_lts_levels(self._h, max_levels, _p(lev, _p_i32))
+        if n < 1:
+            raise ValueError("local time stepping serves triangles")
+        return lev, n
+
+    def lts_dt(self, U, cfl, levels):
+        U = np.ascontiguousarray(U, np.float64)
+        levels = np.ascontiguousarray(levels, np.int32)
+        return _lib.or_lts_dt(self._h, _p(U, _p_f64), cfl, _p(levels, _p_i32))
+
+    def lts_step(self, U, levels, h, t=0.0, nthreads=1):
+        """one local-time-stepping step of size h (a new copy of U)"""
+        U = np.array(U, np.float64, copy=True)
+        levels = np.ascontiguousarray(levels, np.int32)
+        if _lib.or_lts_step(self._h, _p(levels, _p_i32), _p(U, _p_f64), t, h, nthreads) != 0:
+            raise ValueError("local time stepping serves triangles")
+        return U
 
     def project(self, field, par=(), t=0.0):
         par = np.ascontiguousarray(list(par) + [0.0] * (8 - len(par)), np.float64)

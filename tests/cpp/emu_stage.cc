@@ -26,7 +26,7 @@ void run_tiles(const StageArgs& a) {
   constexpr int TH = K::THREADS;
   std::unique_ptr<typename K::Smem> sm(new typename K::Smem());
   std::vector<typename K::Regs> g(TH);
-  const int ntiles = (a.E + TILE_ELEMS - 1) / TILE_ELEMS;
+  const int ntiles = ((a.e1 > 0 ? a.e1 : a.E) - a.e0 + TILE_ELEMS - 1) / TILE_ELEMS;
   for (int tile = 0; tile < ntiles; tile++) {
     std::memset(sm.get(), 0, sizeof(typename K::Smem));
     for (int t = 0; t < TH; t++) K::phaseA(a, *sm, t, tile, g[t]);
@@ -198,11 +198,11 @@ int run_jvp(const AdjArgs& a, int law) {
   return -1;
 }
 
-int make(Emu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int order) {
+int make(Emu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int order, const int32_t* grp = nullptr) {
   dg_options opt{};
   opt.order = order;
   std::string err;
-  if (build_tri_setup(mesh, law, &opt, m.S, err) != DG_OK) return -1;
+  if (build_tri_setup(mesh, law, &opt, m.S, err, grp) != DG_OK) return -1;
   size_t n = (size_t)m.S.E * m.S.NF * m.S.NP;
   m.Un.assign(n, 0.0);
   m.acc.assign(n, 0.0);
@@ -292,6 +292,59 @@ extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, i
   double r = 0;
   for (int i = 0; i < (m.S.E + TILE_ELEMS - 1) / TILE_ELEMS; i++) r += m.part[i];
   if (resid) *resid = r;
+  m.from_soa(m.Un.data(), U);
+  return 0;
+}
+
+// nsteps local-time-stepping steps of size h in place, with the planning and
+// launch sequence of the library (dg_lts_setup, enqueue_lts_step): levels by
+// cell size (max_levels), elements grouped by class, the stage kernel over each
+// class's range; levels (caller order, may be NULL) receives the levels
+extern "C" int emu_tri_lts_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int max_levels,
+                                 double h, int nsteps, double* U, int32_t* levels) {
+  const int E = mesh->n_elems;
+  std::vector<double> size(E);
+  for (int e = 0; e < E; e++) size[e] = tri_cell_size(mesh->xyz + (size_t)e * 6);
+  Emu m0;
+  if (make(m0, mesh, law, order)) return -1;
+  std::vector<int32_t> lev, m1, m2, cid;
+  std::vector<LtsClass> cls;
+  const int nlev = lts_levels(E, size.data(), m0.S.faces, max_levels, lev);
+  lts_reach(E, m0.S.faces, lev, m1, m2);
+  lts_classify(E, lev, m1, m2, cid, cls);
+  Emu m;
+  if (make(m, mesh, law, order, cid.data())) return -1;
+  for (auto& k : cls) k.e0 = k.e1 = -1;
+  for (int i = 0; i < E; i++) {
+    LtsClass& k = cls[cid[m.S.perm[i]]];
+    if (k.e0 < 0) k.e0 = i;
+    else if (k.e1 != i) return -4;
+    k.e1 = i + 1;
+  }
+  if (levels) std::memcpy(levels, lev.data(), (size_t)E * 4);
+  m.to_soa(U);
+  std::vector<LtsLaunch> plan;
+  lts_schedule(nlev, cls, h, plan);
+  const int nrow = m.S.NF * m.S.NP;
+  for (int s = 0; s < nsteps; s++) {
+    StageArgs a = m.base();
+    a.scheme = SCHEME_LTS;
+    for (const LtsLaunch& L : plan) {
+      if (L.kind == LTS_COMMIT) {
+        for (int r = 0; r < nrow; r++)
+          for (int e = L.e0; e < L.e1; e++) m.Un[(size_t)r * E + e] += m.acc[(size_t)r * E + e];
+        continue;
+      }
+      a.e0 = L.e0;
+      a.e1 = L.e1;
+      a.stage = L.flags;
+      a.dt = L.hb;
+      a.lts_w = L.w;
+      a.Uin = L.input ? (const void*)m.A64.data() : (const void*)m.Un.data();
+      a.Uout = (L.flags & LTS_ODD) ? (void*)m.Un.data() : (void*)m.A64.data();
+      if (run_variant(a, m.S.law, 0)) return -2;
+    }
+  }
   m.from_soa(m.Un.data(), U);
   return 0;
 }

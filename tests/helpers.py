@@ -157,3 +157,41 @@ def lattice_msh(path, nx, ny, lx=1.0, ly=1.0, side_tags=(40000, 1, 40000, 1)):
         recs.append((1, side_tags[1], [nid(nx, j), nid(nx, j + 1)]))
         recs.append((1, side_tags[3], [nid(0, j + 1), nid(0, j)]))
     write_msh(path, nodes, recs)
+
+
+def mesh_tri_graded(xs, ys, periodic=True):
+    """A tensor grid of nodes xs x ys, each cell split into two triangles as
+    mesh_tri splits them, periodic in x and y: cells of different sizes for
+    local time stepping.  Returns (vids, xyz)."""
+    xs, ys = np.asarray(xs, float), np.asarray(ys, float)
+    nx, ny = len(xs) - 1, len(ys) - 1
+
+    def topo(i, j):
+        if periodic and i == nx:
+            i = 0
+        if periodic and j == ny:
+            j = 0
+        return j * (nx + 1) + i
+
+    E = 2 * nx * ny
+    vids = np.zeros((E, 3), np.int32)
+    xyz = np.zeros((E, 3, 2))
+    e = 0
+    for j in range(ny):
+        for i in range(nx):
+            for tri in (((i, j), (i + 1, j), (i + 1, j + 1)), ((i, j), (i + 1, j + 1), (i, j + 1))):
+                for v in range(3):
+                    vids[e, v] = topo(*tri[v])
+                    xyz[e, v] = (xs[tri[v][0]], ys[tri[v][1]])
+                e += 1
+    return vids, xyz
+
+
+def graded_nodes(n, lo, hi, ratios):
+    """n cells over [lo, hi] with widths proportional to ratios (cycled over the
+    cells in blocks), e.g. ratios (4, 2, 1, 1, 2, 4) over 6 blocks"""
+    nb = len(ratios)
+    w = np.repeat(np.asarray(ratios, float), n // nb)
+    w = np.concatenate([w, np.full(n - len(w), ratios[-1], float)])
+    x = np.concatenate([[0.0], np.cumsum(w)])
+    return lo + (hi - lo) * x / x[-1]
