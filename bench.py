@@ -470,6 +470,86 @@ def run_scalarpb(dgpar, n=512, n_ref=128, steps=200, sweeps=40, tune=True, ref=T
     return out
 
 
+def graded_mesh(dgpar, n, L, grade=(4, 2, 1, 1, 2, 4)):
+    """n x n x 2 periodic triangles on [0, L]^2 over a tensor grid whose cell
+    widths follow grade in blocks (the same in x and y): cells of several sizes
+    for local time stepping"""
+    w = np.repeat(np.asarray(grade, float), n // len(grade))
+    w = np.concatenate([w, np.full(n - len(w), float(grade[-1]))])
+    xs = np.concatenate([[0.0], np.cumsum(w)]) * (L / w.sum())
+    vids = np.zeros((2 * n * n, 3), np.int32)
+    xyz = np.zeros((2 * n * n, 3, 2))
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    topo = lambda a, b: (b % n) * (n + 1) + (a % n)  # noqa: E731
+    for k, corners in enumerate((((0, 0), (1, 0), (1, 1)), ((0, 0), (1, 1), (0, 1)))):
+        for v, (di, dj) in enumerate(corners):
+            vids[k::2, v] = topo(i + di, j + dj)
+            xyz[k::2, v, 0] = xs[i + di]
+            xyz[k::2, v, 1] = xs[j + dj]
+    return dgpar.Mesh(2, vids, xyz)
+
+
+LTS_GRADE = (8,) * 10 + (4, 2, 1, 1, 2, 4) + (8,) * 10
+
+
+def run_lts(dgpar, steps=40, n=256, levels=4):
+    """Local time stepping (dg_lts_setup, DG_RK_TVD2_LTS): the isentropic vortex,
+    P3, LF, on n x n x 2 periodic triangles of [0,10]^2 whose cell widths are
+    8 outside a band through the centre graded 4:2:1:1:2:4, in both directions
+    (a refined cross; four size levels).  LTS steps of the
+    coarsest level's CFL 0.1 step h against global SSP-RK2 at the smallest
+    element's CFL step over the same simulated time; the speedup is the ratio of
+    wall time per simulated time, and the density L2 distance between the two
+    end states is reported with the L2 error of each against the exact vortex."""
+    mesh = graded_mesh(dgpar, n, 10.0, LTS_GRADE)
+    law = dgpar.EulerLaw(dgpar.FLUX_LF, 1.4)
+    out = {"workload": "euler2d-isentropic-vortex P3 LF, %dx%dx2 periodic [0,10]^2, cell widths 8 with a band graded "
+                       "4:2:1:1:2:4 through the centre in x and y, local time stepping (at most %d size levels) vs "
+                       "global SSP-RK2, CFL 0.1" % (n, n, levels)}
+    with dgpar.DGAnalysis(mesh, law, ORDER, use_graph=True) as g:
+        g.project(dgpar.FIELD_VORTEX, VORTEX)
+        dt_min = g.computeDt(CFL)
+        lev = g.sortCellsBySize(levels)
+        h = g.computeDt(CFL)
+        nlev = int(lev.max()) + 1
+        g.time_advance(dgpar.RK_TVD2_LTS, h, 2)  # warm-up (graph capture)
+        g.project(dgpar.FIELD_VORTEX, VORTEX)
+        dgpar.device_synchronize()
+        ms_lts = g.time_advance(dgpar.RK_TVD2_LTS, h, steps)
+        T = steps * h
+        err_lts = g.l2_error(dgpar.FIELD_VORTEX, VORTEX, T)
+        U_lts = g.get_state()
+        ndof = g.ndof
+    nglob = int(np.ceil(T / dt_min - 1e-9))
+    dtg = T / nglob
+    with dgpar.DGAnalysis(mesh, law, ORDER, use_graph=True) as g:
+        g.project(dgpar.FIELD_VORTEX, VORTEX)
+        g.time_advance(dgpar.RK_TVD2, dtg, 2)
+        g.project(dgpar.FIELD_VORTEX, VORTEX)
+        dgpar.device_synchronize()
+        ms_glob = g.time_advance(dgpar.RK_TVD2, dtg, nglob)
+        err_glob = g.l2_error(dgpar.FIELD_VORTEX, VORTEX, T)
+        U_glob = g.get_state()
+    E = mesh.n_elems
+    X = mesh.xyz
+    detj = np.abs((X[:, 1, 0] - X[:, 0, 0]) * (X[:, 2, 1] - X[:, 0, 1]) -
+                  (X[:, 2, 0] - X[:, 0, 0]) * (X[:, 1, 1] - X[:, 0, 1]))
+    d = (U_lts - U_glob).reshape(E, 4, -1)[:, 0, :]
+    counts = np.bincount(lev, minlength=nlev).tolist()
+    # residual evaluations per element and step h: a level-b element needs at
+    # least 2^(b+1) (its Heun substeps); global SSP-RK2 takes 2 h / dt per element
+    out.update({"elements": E, "dof": ndof, "levels_histogram": counts, "h": h, "dt_min": dt_min,
+                "lts_steps": steps, "lts_ms_per_step": ms_lts / steps, "simulated_time": T,
+                "global_steps": nglob, "global_ms_per_step": ms_glob / nglob,
+                "speedup_vs_global_ssp_rk2": ms_glob / ms_lts,
+                "min_residuals_per_step": sum(c * 2 ** (b + 1) for b, c in enumerate(counts)) / E,
+                "global_residuals_per_h": 2 * h / dtg,
+                "l2_rho_vs_exact": {"lts": float(err_lts[0]), "global": float(err_glob[0])},
+                "l2_rho_lts_vs_global": float(np.sqrt((d ** 2 * detj[:, None]).sum()))})
+    return out
+
+
 def cpu_model():
     """the host CPU's model name and the cores this process may use"""
     name = "unknown"
@@ -519,6 +599,8 @@ def main():
                     help="SSP-RK3 steps of the scalar advection leg (0: skip it)")
     ap.add_argument("--scalarpb-steps", type=int, default=200,
                     help="SSP-RK2 steps of the ScalarPb leg (rotating velocity field; 0: skip it)")
+    ap.add_argument("--lts-steps", type=int, default=40,
+                    help="local-time-stepping steps of the graded-mesh leg (0: skip it)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules (no tuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 256 MiB device copy (reference point for HBM byte counters)")
@@ -591,6 +673,8 @@ def main():
         out["mixed_order_limited"] = run_mixed(dgpar, args.mixed_steps, tune=tune)
     if args.maxwell_steps > 0:
         out["maxwell3d_tets"] = run_maxwell(dgpar, args.maxwell_steps, tune=tune)
+    if args.lts_steps > 0:
+        out["local_time_stepping"] = run_lts(dgpar, args.lts_steps)
     if not args.no_cpu:
         # the reference's own serial code on the same mesh (its build travels with
         # the repository as oracle/_ref); the FP64 oracle port on this host's

@@ -1242,8 +1242,17 @@ dg_status dg_lts_setup(dg_ctx* c, int max_levels, int32_t* levels_out) {
   }
   std::vector<double> scale(E);
   for (int i = 0; i < E; i++) scale[i] = (double)(1 << lev[c->perm[i]]);
-  if ((s = dalloc(c, &c->dtscale, (size_t)E))) return s;
+  LtsPlan plan;
+  lts_plan(nlev, cls, plan);
+  if ((s = dalloc(c, &c->dtscale, (size_t)E)) || (s = dalloc(c, &c->lts_tiles, plan.tiles.size())) ||
+      (s = dalloc(c, &c->lts_ctiles, plan.ctiles.size() + 1)))
+    return s;
   HIPCHECK(c, hipMemcpy(c->dtscale, scale.data(), (size_t)E * 8, hipMemcpyHostToDevice));
+  HIPCHECK(c, hipMemcpy(c->lts_tiles, plan.tiles.data(), plan.tiles.size() * sizeof(LtsTile), hipMemcpyHostToDevice));
+  if (!plan.ctiles.empty())
+    HIPCHECK(c, hipMemcpy(c->lts_ctiles, plan.ctiles.data(), plan.ctiles.size() * sizeof(LtsTile),
+                          hipMemcpyHostToDevice));
+  c->lts_plan = std::move(plan);
   c->lts_nlev = nlev;
   c->lts_level = lev;
   c->lts_cls = cls;
@@ -1251,28 +1260,24 @@ dg_status dg_lts_setup(dg_ctx* c, int max_levels, int32_t* levels_out) {
   return DG_OK;
 }
 
-// one DG_RK_TVD2_LTS step of size h on the stream: the stage input of even
-// stages is Un (the span-start states), of odd stages A64 (the predictors);
-// acc holds each element's span increment
+// one DG_RK_TVD2_LTS step of size h on the stream: per stage one launch over
+// the stage's tiles (input Un, the span-start states, at even stages; A64, the
+// predictors, at odd stages) and, at odd stages, one commit launch; acc holds
+// each element's span increment
 static int enqueue_lts_step(dg_ctx* c, double h) {
-  std::vector<LtsLaunch> plan;
-  lts_schedule(c->lts_nlev, c->lts_cls, h, plan);
+  const LtsPlan& P = c->lts_plan;
   StageArgs a = base_args(c);
   a.scheme = SCHEME_LTS;
-  const int shape = c->shape == 1 ? 1 : 0;  // the FP64 tile shapes
-  for (const LtsLaunch& L : plan) {
-    if (L.kind == LTS_COMMIT) {
-      if (launch_lts_commit(c->E, c->Nf * c->NP, L.e0, L.e1, c->acc, c->Un, c->stream)) return -1;
-      continue;
-    }
-    a.e0 = L.e0;
-    a.e1 = L.e1;
-    a.stage = L.flags;
-    a.dt = L.hb;
-    a.lts_w = L.w;
-    a.Uin = L.input ? (const void*)c->A64 : (const void*)c->Un;
-    a.Uout = (L.flags & LTS_ODD) ? (void*)c->Un : (void*)c->A64;
-    if (launch_tri_stage_tiled(a, c->law, shape << 2, c->stream)) return -1;
+  a.dt = h;
+  for (int i = 0; i < P.S; i++) {
+    const bool odd = i & 1;
+    a.lts_tiles = c->lts_tiles + P.off[i];
+    a.Uin = odd ? (const void*)c->A64 : (const void*)c->Un;
+    a.Uout = odd ? (void*)c->Un : (void*)c->A64;
+    if (launch_tri_stage_lts(a, c->law, P.off[i + 1] - P.off[i], c->stream)) return -1;
+    if (launch_lts_commit(c->E, c->Nf * c->NP, c->lts_ctiles + P.coff[i], P.coff[i + 1] - P.coff[i], c->acc, c->Un,
+                          c->stream))
+      return -1;
   }
   return 0;
 }

@@ -83,6 +83,8 @@ struct dg_ctx {
   int lts_nlev = 0;
   std::vector<int32_t> lts_grp, lts_level;
   std::vector<LtsClass> lts_cls;
+  LtsPlan lts_plan;
+  LtsTile *lts_tiles = nullptr, *lts_ctiles = nullptr;  // the plan's tile lists (device)
   double* dtscale = nullptr;
   uint8_t* npflags = nullptr;
   unsigned* status = nullptr;

@@ -49,6 +49,9 @@ struct StageArgs {
   // every element's rows keep the stride E
   int e0, e1;
   double lts_w;
+  // local time stepping: the launch's tiles (launch_tri_stage_lts), each with
+  // its own range, flags (stage), level (dt / 2^b) and weight (lts_w)
+  const LtsTile* lts_tiles;
 };
 
 // variant: 0 = FP64 throughout; 1 = FP32 arithmetic, FP64 input, FP32 output;
@@ -94,7 +97,11 @@ int launch_min_reduce(const double* part, int n, double* out, hipStream_t s);
 // *status |= 4 when any of U[0 .. n) is NaN or Inf (one atomic per wave that meets one)
 int launch_nonfinite_scan(const double* U, long n, unsigned* status, hipStream_t s);
 int launch_fill_one(double* out, double v, hipStream_t s);
-// local time stepping: Un += acc on rows [0, nrow) of internal elements [e0, e1)
-int launch_lts_commit(int E, int nrow, int e0, int e1, const double* acc, double* Un, hipStream_t s);
+// local time stepping: Un += acc on rows [0, nrow) of the tiles' elements
+int launch_lts_commit(int E, int nrow, const LtsTile* tiles, int ntiles, const double* acc, double* Un,
+                      hipStream_t s);
+// local time stepping: the FP64 tiled stage kernel (4- or 8-wave tiles) over a tile list
+// (a.lts_tiles, ntiles workgroups), step size a.dt
+int launch_tri_stage_lts(const StageArgs& a, int law, int ntiles, hipStream_t s);
 
 }  // namespace dgp

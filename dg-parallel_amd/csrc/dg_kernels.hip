@@ -462,24 +462,6 @@ int launch_fill_one(double* out, double v, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-// local time stepping: the span's increment added to y_m (one work-item per
-// element and row of the range)
-__global__ __launch_bounds__(BLOCK) void lts_commit_kernel(int E, int n, int e0, int nrow, const double* __restrict__ acc,
-                                                           double* __restrict__ Un) {
-  const long i = (long)blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= (long)n * nrow) return;
-  const long r = i / n, e = e0 + (i - r * n);
-  Un[r * E + e] += acc[r * E + e];
-}
-
-int launch_lts_commit(int E, int nrow, int e0, int e1, const double* acc, double* Un, hipStream_t s) {
-  const int n = e1 - e0;
-  if (n <= 0) return 0;
-  const long tot = (long)n * nrow;
-  hipLaunchKernelGGL(lts_commit_kernel, dim3((unsigned)((tot + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, s, E, n, e0, nrow,
-                     acc, Un);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
-}
 
 }  // namespace dgp
 

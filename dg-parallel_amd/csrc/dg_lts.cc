@@ -79,27 +79,35 @@ void lts_classify(int E, const std::vector<int32_t>& level, const std::vector<in
     cid[e] = (int32_t)(std::lower_bound(uniq.begin(), uniq.end(), keys[e]) - uniq.begin());
 }
 
-void lts_schedule(int nlev, const std::vector<LtsClass>& cls, double h, std::vector<LtsLaunch>& out) {
-  out.clear();
+static void push_tiles(const LtsClass& c, int flags, double w, std::vector<LtsTile>& out) {
+  for (int e = c.e0; e < c.e1; e += 64) out.push_back({e, std::min(c.e1, e + 64), flags, c.b, w});
+}
+
+void lts_plan(int nlev, const std::vector<LtsClass>& cls, LtsPlan& plan) {
   const int S = 1 << nlev;
+  plan.S = S;
+  plan.tiles.clear();
+  plan.ctiles.clear();
+  plan.off.assign(1, 0);
+  plan.coff.assign(1, 0);
   for (int i = 0; i < S; i++) {
     const bool odd = i & 1;
     for (const LtsClass& c : cls) {
       if (c.e1 <= c.e0) continue;
       const int Lb = S >> c.b, L1 = S >> c.m1, L2 = S >> c.m2, j = i % Lb;
-      const double hb = h / (double)(1 << c.b);
       if (!odd && i % L1 == 0)
-        out.push_back({LTS_STAGE, c.e0, c.e1, j == 0 ? LTS_RESET : 0, 0, hb, (double)L1 / (2.0 * Lb)});
+        push_tiles(c, j == 0 ? LTS_RESET : 0, (double)L1 / (2.0 * Lb), plan.tiles);
       else if (odd && (i - 1) % L2 == 0)
-        out.push_back({LTS_STAGE, c.e0, c.e1, LTS_ODD | (j == Lb - 1 ? LTS_END : 0), 1, hb, (double)L2 / (2.0 * Lb)});
+        // with L2 = Lb the span's only evaluated odd stage is its second one, and
+        // no element reads this element's y_m at an even stage before the span
+        // ends (that would need a finer element within two faces): the span's
+        // increment is complete there and goes to y_m at once
+        push_tiles(c, LTS_ODD | (j == Lb - 1 || L2 == Lb ? LTS_END : 0), (double)L2 / (2.0 * Lb), plan.tiles);
+      // a span that ends at a stage its elements skip
+      if (odd && j == Lb - 1 && (i - 1) % L2 != 0 && L2 != Lb) push_tiles(c, 0, 0.0, plan.ctiles);
     }
-    if (!odd) continue;
-    // spans that end at a stage their elements skip
-    for (const LtsClass& c : cls) {
-      if (c.e1 <= c.e0) continue;
-      const int Lb = S >> c.b, L2 = S >> c.m2;
-      if (i % Lb == Lb - 1 && (i - 1) % L2 != 0) out.push_back({LTS_COMMIT, c.e0, c.e1, 0, 0, 0.0, 0.0});
-    }
+    plan.off.push_back((int)plan.tiles.size());
+    plan.coff.push_back((int)plan.ctiles.size());
   }
 }
 

@@ -30,7 +30,9 @@
 // stages i = 1 mod L2; a skipped stage's k equals the last evaluated k of the
 // same parity exactly (same inputs, same arithmetic), and enters the sum with
 // multiplicity L1/2 (even) or L2/2 (odd).  Far from finer elements that is one
-// Heun substep per h_b, as on a uniform mesh of that level.
+// Heun substep per h_b, as on a uniform mesh of that level.  Such an element
+// (L2 = L_b) writes its new y_m at the span's second stage: until the span
+// ends, only odd stages read it, and they read its predictor.
 #pragma once
 #include <stdint.h>
 
@@ -65,20 +67,29 @@ struct LtsClass {
 void lts_classify(int E, const std::vector<int32_t>& level, const std::vector<int32_t>& m1,
                   const std::vector<int32_t>& m2, std::vector<int32_t>& cid, std::vector<LtsClass>& cls);
 
-// one launch of a step: kind LTS_STAGE evaluates the residual of the class's
-// elements on the stage input (input 0: the span-start states y_m, 1: the
-// predictors) and updates them; kind LTS_COMMIT adds the span's accumulated
-// increment to y_m
-enum { LTS_STAGE = 0, LTS_COMMIT = 1 };
 // stage flags: bit 0 odd stage, bit 1 first stage of a span (the increment
 // restarts), bit 2 the span ends here (the new y_m is written)
 enum { LTS_ODD = 1, LTS_RESET = 2, LTS_END = 4 };
-struct LtsLaunch {
-  int kind, e0, e1, flags, input;
-  double hb, w;  // substep h_b; weight of the stage's h_b k in the span increment
+
+// One tile (at most 64 elements of one class) of a launch: elements [e0, e1),
+// the class's stage flags, its level b (substep h / 2^b) and the weight w of
+// the stage's h_b k in the span increment.
+struct LtsTile {
+  int e0, e1, flags, b;
+  double w;
 };
 
-// the launches of one step of size h, in stream order
-void lts_schedule(int nlev, const std::vector<LtsClass>& cls, double h, std::vector<LtsLaunch>& out);
+// The launches of one step: stage i evaluates the residual of the tiles
+// tiles[off[i] .. off[i+1]) on the stage input (even stages: the span-start
+// states y_m; odd stages: the predictors) and updates them; then, at odd
+// stages, the tiles ctiles[coff[i] .. coff[i+1]) (classes whose span ends at a
+// stage they skip) add the span's increment to y_m.  One launch each, in this
+// order; the plan does not depend on the step size.
+struct LtsPlan {
+  int S = 0;
+  std::vector<LtsTile> tiles, ctiles;
+  std::vector<int> off, coff;
+};
+void lts_plan(int nlev, const std::vector<LtsClass>& cls, LtsPlan& plan);
 
 }  // namespace dgp

@@ -17,12 +17,10 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return (b % 8) * (nb / 8) + b / 8;
 }
 
-template <int P, class LAW, typename CT, typename IT, typename OT, int NW, bool MF = false>
-__global__ __launch_bounds__(NW * 64) void tri_stage_tiled(StageArgs a) {
-  using K = TiledStage<P, LAW, CT, IT, OT, NW, MF>;
-  __shared__ typename K::Smem sm;
+// the phases of one tile, separated by workgroup barriers
+template <class K, bool MF>
+__device__ __forceinline__ void tiled_body(const StageArgs& a, typename K::Smem& sm, int tile) {
   typename K::Regs g;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   K::phaseA(a, sm, tid, tile, g);
   __syncthreads();
@@ -56,6 +54,13 @@ __global__ __launch_bounds__(NW * 64) void tri_stage_tiled(StageArgs a) {
     }
     if (tid == 0) a.resid_part[tile] = sm.red[0];
   }
+}
+
+template <int P, class LAW, typename CT, typename IT, typename OT, int NW, bool MF = false>
+__global__ __launch_bounds__(NW * 64) void tri_stage_tiled(StageArgs a) {
+  using K = TiledStage<P, LAW, CT, IT, OT, NW, MF>;
+  __shared__ typename K::Smem sm;
+  tiled_body<K, MF>(a, sm, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // tiles of the launch's element range [e0, e1) (e1 = 0: the whole mesh)

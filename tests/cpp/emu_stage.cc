@@ -323,26 +323,31 @@ extern "C" int emu_tri_lts_steps(const dg_mesh_desc* mesh, const dg_law_desc* la
   }
   if (levels) std::memcpy(levels, lev.data(), (size_t)E * 4);
   m.to_soa(U);
-  std::vector<LtsLaunch> plan;
-  lts_schedule(nlev, cls, h, plan);
+  LtsPlan plan;
+  lts_plan(nlev, cls, plan);
   const int nrow = m.S.NF * m.S.NP;
   for (int s = 0; s < nsteps; s++) {
-    StageArgs a = m.base();
-    a.scheme = SCHEME_LTS;
-    for (const LtsLaunch& L : plan) {
-      if (L.kind == LTS_COMMIT) {
-        for (int r = 0; r < nrow; r++)
-          for (int e = L.e0; e < L.e1; e++) m.Un[(size_t)r * E + e] += m.acc[(size_t)r * E + e];
-        continue;
+    for (int i = 0; i < plan.S; i++) {
+      const bool odd = i & 1;
+      // the stage launch: every tile as the kernel sets it up from its entry
+      for (int k = plan.off[i]; k < plan.off[i + 1]; k++) {
+        const LtsTile& tl = plan.tiles[k];
+        StageArgs a = m.base();
+        a.scheme = SCHEME_LTS;
+        a.e0 = tl.e0;
+        a.e1 = tl.e1;
+        a.stage = tl.flags;
+        a.dt = h / (double)(1 << tl.b);
+        a.lts_w = tl.w;
+        a.Uin = odd ? (const void*)m.A64.data() : (const void*)m.Un.data();
+        a.Uout = odd ? (void*)m.Un.data() : (void*)m.A64.data();
+        if (run_variant(a, m.S.law, 0)) return -2;
       }
-      a.e0 = L.e0;
-      a.e1 = L.e1;
-      a.stage = L.flags;
-      a.dt = L.hb;
-      a.lts_w = L.w;
-      a.Uin = L.input ? (const void*)m.A64.data() : (const void*)m.Un.data();
-      a.Uout = (L.flags & LTS_ODD) ? (void*)m.Un.data() : (void*)m.A64.data();
-      if (run_variant(a, m.S.law, 0)) return -2;
+      for (int k = plan.coff[i]; k < plan.coff[i + 1]; k++) {
+        const LtsTile& tl = plan.ctiles[k];
+        for (int r = 0; r < nrow; r++)
+          for (int e = tl.e0; e < tl.e1; e++) m.Un[(size_t)r * E + e] += m.acc[(size_t)r * E + e];
+      }
     }
   }
   m.from_soa(m.Un.data(), U);
