@@ -247,6 +247,7 @@ static int vol_rule(int dim, int degree, double (*pts)[4]) {
 typedef struct {
   int eL, lL, eR, lR; /* eR = -1: boundary */
   int tag;
+  int32_t ha, hb;      /* a half of the coarse element eR's face (ha, hb): hanging face (else -1) */
   int nq;
   int off_pts;        /* into face point arrays */
   double n[3];        /* unit outward normal of eL */
@@ -266,6 +267,7 @@ struct or_ctx {
   int32_t* vids;
   or_face* faces;
   int* ef;             /* [E][nfl] face index; sign via eside */
+  int* ef2;            /* [E][nfl] the second half of a hanging face, or -1 */
   int* eside;          /* [E][nfl] 0 = left side, 1 = right side */
   /* face point data */
   double* fx;          /* [P][3] physical point (left frame) */
@@ -377,6 +379,7 @@ void or_destroy(or_ctx* c) {
   free(c->vids);
   free(c->faces);
   free(c->ef);
+  free(c->ef2);
   free(c->eside);
   free(c->fx);
   free(c->fjw);
@@ -384,6 +387,135 @@ void or_destroy(or_ctx* c) {
   free(c->fphiR);
   free(c->fcontrib);
   free(c);
+}
+
+/* Hanging faces (triangles, 1-irregular meshes): a face (a, b) of element K
+ * without a neighbour, where some vertex m closes two other faces without a
+ * neighbour, (a, m) and (m, b), is K's face split on the other side (the
+ * reference's red refinement, mDGMesh::splitEdge / splitTriangle,
+ * mDGMesh.cc:54-337).  Each half becomes a face with the child on the left and
+ * K on the right -- the reference's boundary cells of the leaf edges, which
+ * evaluate both sides at the physical points of the leaf edge's Gauss points
+ * (DGCell.cc:423-555).  K's face keeps both halves (ef, ef2). */
+typedef struct {
+  int32_t v;
+  int fi;
+} vface;
+static int vface_cmp(const void* a, const void* b) {
+  const vface *x = (const vface*)a, *y = (const vface*)b;
+  if (x->v != y->v) return x->v < y->v ? -1 : 1;
+  return x->fi - y->fi;
+}
+/* the vector from vertex id va to vb of face fi, in the coordinates of the
+ * face's left element */
+static void face_vec(const or_ctx* c, const or_mesh* m, int fi, int32_t va, int32_t vb, double* d) {
+  const or_face* f = &c->faces[fi];
+  const double* X = m->xyz + (long)f->eL * 6;
+  int ia = 0, ib = 0;
+  for (int v = 0; v < 3; v++) {
+    if (m->vids[f->eL * 3 + v] == va) ia = v;
+    if (m->vids[f->eL * 3 + v] == vb) ib = v;
+  }
+  d[0] = X[ib * 2] - X[ia * 2];
+  d[1] = X[ib * 2 + 1] - X[ia * 2 + 1];
+}
+
+static int hanging_faces(or_ctx* c, const or_mesh* m) {
+  const int nfl = c->nfl, nv = c->nv;
+  int nb = 0;
+  for (int fi = 0; fi < c->nf; fi++)
+    if (c->faces[fi].eR < 0) nb++;
+  if (nb == 0) return 0;
+  vface* vf = (vface*)malloc(sizeof(vface) * 2 * nb);
+  int32_t (*fv)[2] = (int32_t (*)[2])malloc(sizeof(int32_t) * 2 * c->nf);
+  int k = 0;
+  for (int fi = 0; fi < c->nf; fi++) {
+    const or_face* f = &c->faces[fi];
+    int lv[3];
+    face_local_verts(2, f->lL, lv);
+    fv[fi][0] = m->vids[f->eL * nv + lv[0]];
+    fv[fi][1] = m->vids[f->eL * nv + lv[1]];
+    if (f->eR >= 0) continue;
+    vf[k].v = fv[fi][0];
+    vf[k++].fi = fi;
+    vf[k].v = fv[fi][1];
+    vf[k++].fi = fi;
+  }
+  qsort(vf, k, sizeof(vface), vface_cmp);
+  int* dead = (int*)calloc(c->nf, sizeof(int));
+  int nhang = 0;
+  for (int fi = 0; fi < c->nf; fi++) {
+    or_face* F = &c->faces[fi];
+    if (F->eR >= 0 || dead[fi] || F->ha >= 0) continue;
+    const int32_t a = fv[fi][0], b = fv[fi][1];
+    /* boundary faces at a: (a, x); a half needs (x, b) without neighbour too */
+    int lo = 0, hi = k;
+    while (lo < hi) {
+      int mid = (lo + hi) / 2;
+      if (vf[mid].v < a) lo = mid + 1; else hi = mid;
+    }
+    int h1 = -1, h2 = -1;
+    for (int i = lo; i < k && vf[i].v == a && h1 < 0; i++) {
+      int g = vf[i].fi;
+      if (g == fi || c->faces[g].eL == F->eL || c->faces[g].ha >= 0) continue;
+      int32_t x = fv[g][0] == a ? fv[g][1] : fv[g][0];
+      int l2 = 0, h2i = k;
+      while (l2 < h2i) {
+        int mid = (l2 + h2i) / 2;
+        if (vf[mid].v < x) l2 = mid + 1; else h2i = mid;
+      }
+      for (int j = l2; j < k && vf[j].v == x; j++) {
+        int g2 = vf[j].fi;
+        if (g2 == g || c->faces[g2].eL == F->eL || c->faces[g2].ha >= 0) continue;
+        int32_t y = fv[g2][0] == x ? fv[g2][1] : fv[g2][0];
+        /* x is the midpoint: x - a (in g's element) equals b - x (in g2's);
+         * two other faces of a split neighbour also close a path a-x-b */
+        double d1[2], d2[2];
+        face_vec(c, m, g, a, x, d1);
+        face_vec(c, m, g2, x, b, d2);
+        const double L = fabs(d1[0]) + fabs(d1[1]);
+        if (y == b && fabs(d1[0] - d2[0]) + fabs(d1[1] - d2[1]) <= 1e-9 * L) {
+          h1 = g;
+          h2 = g2;
+          break;
+        }
+      }
+    }
+    if (h1 < 0) continue;
+    int half[2] = {h1, h2};
+    for (int t = 0; t < 2; t++) {
+      or_face* H = &c->faces[half[t]];
+      H->eR = F->eL;
+      H->lR = F->lL;
+      H->tag = 0;
+      H->ha = a;
+      H->hb = b;
+    }
+    c->ef[F->eL * nfl + F->lL] = h1;
+    c->ef2[F->eL * nfl + F->lL] = h2;
+    c->eside[F->eL * nfl + F->lL] = 1;
+    dead[fi] = 1;
+    nhang++;
+  }
+  if (nhang) {
+    /* drop the coarse faces; renumber */
+    int* nw = (int*)malloc(sizeof(int) * c->nf);
+    int n2 = 0;
+    for (int fi = 0; fi < c->nf; fi++) {
+      nw[fi] = dead[fi] ? -1 : n2;
+      if (!dead[fi]) c->faces[n2++] = c->faces[fi];
+    }
+    for (long i = 0; i < (long)c->E * nfl; i++) {
+      c->ef[i] = nw[c->ef[i]];
+      if (c->ef2[i] >= 0) c->ef2[i] = nw[c->ef2[i]];
+    }
+    c->nf = n2;
+    free(nw);
+  }
+  free(dead);
+  free(vf);
+  free(fv);
+  return 0;
 }
 
 or_ctx* or_create(const or_mesh* m, const or_law* law) {
@@ -521,6 +653,14 @@ or_ctx* or_create(const or_mesh* m, const or_law* law) {
   }
   free(keys);
   c->nf = nf;
+  c->ef2 = (int*)malloc(sizeof(int) * E * nfl);
+  for (long i = 0; i < (long)E * nfl; i++) c->ef2[i] = -1;
+  for (int fi = 0; fi < nf; fi++) c->faces[fi].ha = c->faces[fi].hb = -1;
+  if (dim == 2 && hanging_faces(c, m) != 0) {
+    or_destroy(c);
+    return NULL;
+  }
+  nf = c->nf;
   /* face quadrature (DGCell.cc:408-555): degree 2*max(pL,pR)+1 */
   long tot = 0;
   for (int fi = 0; fi < nf; fi++) {
@@ -557,15 +697,20 @@ or_ctx* or_create(const or_mesh* m, const or_law* law) {
       fv[i] = m->vids[f->eL * nv + lv[i]];
       for (int d = 0; d < dim; d++) A[i][d] = XL[lv[i] * dim + d];
     }
-    /* the same corners in the right element's frame (translation periodicity) */
+    /* the same corners in the right element's frame (translation periodicity);
+     * on a hanging face the midpoint of the coarse face is the mean of its ends */
     double B[3][3] = {{0}};
     if (f->eR >= 0) {
       const double* XR = m->xyz + (long)f->eR * nv * dim;
       for (int i = 0; i < nfv; i++) {
-        int found = -1;
-        for (int v = 0; v < nv; v++)
+        int found = -1, fa = -1, fb = -1;
+        for (int v = 0; v < nv; v++) {
           if (m->vids[f->eR * nv + v] == fv[i]) found = v;
-        for (int d = 0; d < dim; d++) B[i][d] = XR[found * dim + d];
+          if (m->vids[f->eR * nv + v] == f->ha) fa = v;
+          if (m->vids[f->eR * nv + v] == f->hb) fb = v;
+        }
+        for (int d = 0; d < dim; d++)
+          B[i][d] = found >= 0 ? XR[found * dim + d] : 0.5 * (XR[fa * dim + d] + XR[fb * dim + d]);
       }
     }
     /* opposite vertex of the left element */
@@ -1228,6 +1373,11 @@ static void* worker(void* arg) {
         int fi = c->ef[e * c->nfl + l];
         const double* src = c->fcontrib + (long)fi * 2 * Nf * MAXNP + c->eside[e * c->nfl + l] * Nf * MAXNP;
         for (int i = 0; i < Nf * Np; i++) Re[i] += src[i];
+        int f2 = c->ef2[e * c->nfl + l];
+        if (f2 >= 0) {
+          src = c->fcontrib + (long)f2 * 2 * Nf * MAXNP + Nf * MAXNP; /* the coarse side is the right side */
+          for (int i = 0; i < Nf * Np; i++) Re[i] += src[i];
+        }
       }
     }
   }
@@ -1591,19 +1741,22 @@ static int elem_nonphysical(or_ctx* c, const double* Ue, int e) {
     }
     if (!law_physical(L, val)) return 1;
   }
-  for (int l = 0; l < c->nfl; l++) {
-    const or_face* fc = &c->faces[c->ef[e * c->nfl + l]];
-    const double* tab = c->eside[e * c->nfl + l] ? c->fphiR : c->fphiL;
-    for (int q = 0; q < fc->nq; q++) {
-      const double* ph = tab + (long)(fc->off_pts + q) * MAXNP;
-      for (int f = 0; f < Nf; f++) {
-        double s = 0;
-        for (int k = 0; k < Np; k++) s += Ue[f * Np + k] * ph[k];
-        val[f] = s;
+  for (int l = 0; l < c->nfl; l++)
+    for (int half = 0; half < 2; half++) {
+      const int fi = half ? c->ef2[e * c->nfl + l] : c->ef[e * c->nfl + l];
+      if (fi < 0) continue;
+      const or_face* fc = &c->faces[fi];
+      const double* tab = c->eside[e * c->nfl + l] ? c->fphiR : c->fphiL;
+      for (int q = 0; q < fc->nq; q++) {
+        const double* ph = tab + (long)(fc->off_pts + q) * MAXNP;
+        for (int f = 0; f < Nf; f++) {
+          double s = 0;
+          for (int k = 0; k < Np; k++) s += Ue[f * Np + k] * ph[k];
+          val[f] = s;
+        }
+        if (!law_physical(L, val)) return 1;
       }
-      if (!law_physical(L, val)) return 1;
     }
-  }
   return 0;
 }
 

@@ -195,3 +195,45 @@ def graded_nodes(n, lo, hi, ratios):
     w = np.concatenate([w, np.full(n - len(w), ratios[-1], float)])
     x = np.concatenate([[0.0], np.cumsum(w)])
     return lo + (hi - lo) * x / x[-1]
+
+
+def refine_tri(vids, xyz, bc, mask):
+    """Red refinement (each marked triangle into 4 through its edge midpoints),
+    written independently of the library's refinement: corners P0, P1, P2 and
+    midpoints M01, M12, M20 give the children (P0, M01, M20), (M01, P1, M12),
+    (M20, M12, P2), (M12, M20, M01), in place of the parent, in order.  A
+    midpoint's topological id is new per edge (shared by both sides' images);
+    its coordinates come from each element's own corners.  Halves of a parent's
+    face keep its tag.  Unmarked neighbours keep their faces (hanging nodes).
+    Returns (vids, xyz, bc, parent) with parent[i] the input element of output i."""
+    vids = np.asarray(vids)
+    xyz = np.asarray(xyz, float)
+    bc = np.zeros_like(vids) if bc is None else np.asarray(bc)
+    nxt = int(vids.max()) + 1
+    mids = {}
+
+    def mid(a, b):
+        nonlocal nxt
+        k = (min(a, b), max(a, b))
+        if k not in mids:
+            mids[k] = nxt
+            nxt += 1
+        return mids[k]
+
+    ov, ox, ob, par = [], [], [], []
+    for e in range(len(vids)):
+        if not mask[e]:
+            ov.append(vids[e]), ox.append(xyz[e]), ob.append(bc[e]), par.append(e)
+            continue
+        P = list(vids[e])
+        X = list(xyz[e])
+        m01, m12, m20 = mid(P[0], P[1]), mid(P[1], P[2]), mid(P[2], P[0])
+        X01, X12, X20 = 0.5 * (X[0] + X[1]), 0.5 * (X[1] + X[2]), 0.5 * (X[2] + X[0])
+        t = bc[e]
+        kids = [((P[0], m01, m20), (X[0], X01, X20), (t[0], 0, t[2])),
+                ((m01, P[1], m12), (X01, X[1], X12), (t[0], t[1], 0)),
+                ((m20, m12, P[2]), (X20, X12, X[2]), (0, t[1], t[2])),
+                ((m12, m20, m01), (X12, X20, X01), (0, 0, 0))]
+        for v, x, b in kids:
+            ov.append(v), ox.append(np.array(x)), ob.append(b), par.append(e)
+    return (np.array(ov, np.int32), np.array(ox), np.array(ob, np.int32), np.array(par, np.int32))
