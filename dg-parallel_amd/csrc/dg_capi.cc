@@ -199,6 +199,30 @@ static dg_status ctx_init_tet(dg_ctx* c, const dg_mesh_desc* m, const dg_law_des
   return DG_OK;
 }
 
+// Faces of half an edge (meshes with hanging nodes): per owning element (internal
+// numbering) its records for the hanging-face pass, and the basis at the
+// Gauss points of each face's halves (the points of the order-P face rule
+// mapped to [0, 1/2] and [1/2, 1] of the face, each half's points in the
+// face's direction)
+static dg_status hang_setup(dg_ctx* c, const TriSetup& S) {
+  if (c->mixed || c->prec != DG_PREC_FP64 || c->kernel != 0)
+    return fail(c, DG_ERR_UNSUPPORTED, "meshes with hanging nodes run one order in the FP64 form of the tiled kernel");
+  std::vector<int32_t> elem, rstart, rec;
+  std::vector<double> ph;
+  build_hang_tables(S, elem, rstart, rec, ph);
+  dg_status s;
+  if ((s = dalloc(c, &c->hang_elem, elem.size())) || (s = dalloc(c, &c->hang_rstart, rstart.size())) ||
+      (s = dalloc(c, &c->hang_rec, rec.size())) || (s = dalloc(c, &c->phih, ph.size())))
+    return s;
+  if (hipMemcpy(c->hang_elem, elem.data(), elem.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->hang_rstart, rstart.data(), rstart.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->hang_rec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->phih, ph.data(), ph.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(c, DG_ERR_HIP, "hanging-face table upload failed");
+  c->hang_n = (int)elem.size();
+  return DG_OK;
+}
+
 static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt) {
   auto bail = [&](dg_status s) { return s; };
   if (m && m->dim == 3) return ctx_init_tet(c, m, law, opt);
@@ -346,6 +370,7 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
     return bail(fail(c, DG_ERR_HIP, "table upload failed"));
   if (hipMemset(c->Un, 0, nd * 8) != hipSuccess || hipMemset(c->acc, 0, nd * 8) != hipSuccess)
     return bail(fail(c, DG_ERR_HIP, "memset failed"));
+  if (!S.topo.hang.empty() && (s = hang_setup(c, S))) return bail(s);
   return DG_OK;
 }
 
@@ -520,6 +545,27 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
   } else {
     r = c->kernel == 1 ? launch_tri_stage(a, c->law, variant, c->stream)
                        : launch_tri_stage_tiled(a, c->law, variant | c->shape << 2, c->stream);
+    if (r == 0 && c->hang_n) {
+      HangArgs h{};
+      h.E = c->E;
+      h.P = c->P;
+      h.n = c->hang_n;
+      h.elem = c->hang_elem;
+      h.rstart = c->hang_rstart;
+      h.rec = c->hang_rec;
+      h.Uin = (const double*)a.Uin;
+      h.Uout = (double*)a.Uout;
+      h.acc = a.acc;
+      h.R = a.R;
+      h.tab = c->tab;
+      h.phih = c->phih;
+      h.el = a.el;
+      h.law = c->lp;
+      h.dt = a.dt;
+      h.scheme = a.scheme;
+      h.stage = a.stage;
+      r = launch_hang(h, c->law, c->stream);
+    }
   }
   if (r == 0 && c->lim_on && a.scheme != MODE_RESIDUAL) {
     const LimiterArgs la = limiter_args(c, (double*)a.Uout);
@@ -615,6 +661,7 @@ static int enqueue_dt(dg_ctx* c, double cfl, double* part) {
 // point), uploaded as [NQV + 3 NQF][2][E] in the internal element order
 dg_status dg_set_velocity_field(dg_ctx* c, dg_field_fn fn, void* user) {
   if (!ready(c) || !fn) return fail(c, DG_ERR_ARG, "dg_set_velocity_field: bad argument");
+  if (c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "a velocity field runs meshes without hanging nodes");
   if (c->dim != 2 || c->law != DG_LAW_SCALAR)
     return fail(c, DG_ERR_UNSUPPORTED, "a velocity field serves the scalar law on triangles");
   if (c->mixed || c->kernel != 0)
@@ -1013,6 +1060,7 @@ dg_status dg_set_limiter(dg_ctx* c, int enabled, double tvb_m, double nu) {
     return fail(c, DG_ERR_UNSUPPORTED, "the limiter is built for the scalar and Euler laws");
   if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "built for triangles");
   if (enabled && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the limiter runs in the FP64 form only");
+  if (enabled && c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "the limiter runs meshes without hanging nodes");
   c->lim_on = enabled ? 1 : 0;
   c->lim_m = tvb_m;
   c->lim_nu = nu;
@@ -1034,6 +1082,7 @@ dg_status dg_set_recovery(dg_ctx* c, int enabled) {
   if (enabled && (c->dim != 2 || c->law != DG_LAW_EULER2D))
     return fail(c, DG_ERR_UNSUPPORTED, "recovery serves the Euler law on triangles (other laws are always physical)");
   if (enabled && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "recovery runs in the FP64 form only");
+  if (enabled && c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "recovery runs meshes without hanging nodes");
   c->rec_on = enabled ? 1 : 0;
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured steps change
   c->gexec = nullptr;
@@ -1064,6 +1113,7 @@ dg_status dg_limit(dg_ctx* c, uint8_t* flags_out) {
   if (!ready(c)) return fail(c, DG_ERR_ARG, "bad context");
   if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "built for triangles");
   if (c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the limiter runs in the FP64 form only");
+  if (c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "the limiter runs meshes without hanging nodes");
   HIPCHECK(c, hipSetDevice(c->device));
   if (launch_limiter(limiter_args(c, c->Un), c->law, c->stream)) return fail(c, DG_ERR_HIP, "limiter launch failed");
   if (flags_out) return fetch_flags(c, c->flags, flags_out);
@@ -1201,6 +1251,52 @@ static dg_status ctx_relayout(dg_ctx* c, const std::vector<int32_t>& neword, con
   return dg_set_state(c, V.data(), V.size());
 }
 
+// a context built anew on another triangle mesh of the same order (caller
+// arrays: vids, xyz, bc may be empty) holding the state U (caller layout), with
+// the limiter, recovery and schedule settings carried over when the new mesh
+// allows them; the refinement history is the caller's to set
+extern "C++" dg_status ctx_remesh(dg_ctx* c, const std::vector<int32_t>& vids, const std::vector<double>& xyz,
+                                  const std::vector<int32_t>& bc, const std::vector<double>& U) {
+  dg_ctx* t = new (std::nothrow) dg_ctx();
+  if (!t) return fail(c, DG_ERR_OOM, "out of memory");
+  dg_mesh_desc m{};
+  m.dim = 2;
+  m.n_elems = (int)(vids.size() / 3);
+  m.vids = vids.data();
+  m.xyz = xyz.data();
+  m.bc = bc.empty() ? nullptr : bc.data();
+  dg_options opt = c->m_opt;
+  opt.order = c->P;
+  dg_status s = ctx_init(t, &m, &c->m_law, &opt);
+  if (s != DG_OK) {
+    c->err = t->err;
+    free_all(t);
+    delete t;
+    return s;
+  }
+  t->m_opt = c->m_opt;
+  t->adj_nd = c->adj_nd;
+  const dg_schedule sc{c->shape, c->mix_grid, {0, 0, 0, 0, 0, 0}};
+  if (schedule_ok(t, sc)) apply_schedule(t, sc);
+  if (!t->hang_n) {
+    t->lim_on = c->lim_on;
+    t->lim_m = c->lim_m;
+    t->lim_nu = c->lim_nu;
+    t->rec_on = c->rec_on;
+  }
+  // the history arrays move with the context
+  t->h_level.swap(c->h_level);
+  t->h_fam.swap(c->h_fam);
+  t->h_ci.swap(c->h_ci);
+  t->hfam.swap(c->hfam);
+  t->h_mid.swap(c->h_mid);
+  t->h_nextid = c->h_nextid;
+  std::swap(*c, *t);
+  free_all(t);
+  delete t;
+  return dg_set_state(c, U.data(), U.size());
+}
+
 extern "C++" dg_status ctx_rebucket(dg_ctx* c, const std::vector<int32_t>& neword) {
   dg_status st;
   if (rebucket_in_place(c, neword, st)) return st;
@@ -1215,6 +1311,7 @@ extern "C++" dg_status ctx_rebucket(dg_ctx* c, const std::vector<int32_t>& newor
 dg_status dg_lts_setup(dg_ctx* c, int max_levels, int32_t* levels_out) {
   if (!ready(c) || max_levels < 1 || max_levels > LTS_MAX_LEVELS)
     return fail(c, DG_ERR_ARG, "dg_lts_setup: bad argument (max_levels 1..%d)", LTS_MAX_LEVELS);
+  if (c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "local time stepping runs meshes without hanging nodes");
   if (c->dim != 2 || c->mixed) return fail(c, DG_ERR_UNSUPPORTED, "local time stepping runs triangles of one order");
   if (c->prec != DG_PREC_FP64 || c->kernel != 0)
     return fail(c, DG_ERR_UNSUPPORTED, "local time stepping runs the FP64 form of the tiled kernel");
@@ -1284,6 +1381,7 @@ static int enqueue_lts_step(dg_ctx* c, double h) {
 
 dg_status dg_select_orders(dg_ctx* c, const dg_sensor_opts* o, int apply, int32_t* orders_out) {
   if (!ready(c) || !o) return fail(c, DG_ERR_ARG, "dg_select_orders: bad argument");
+  if (apply && c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "order changes run meshes without hanging nodes");
   if (c->dim != 2) return fail(c, DG_ERR_UNSUPPORTED, "built for triangles");
   if (o->pmin < 1 || o->pmax > MAXP_TRI || o->pmin > o->pmax) return fail(c, DG_ERR_ARG, "orders must lie in 1..%d", MAXP_TRI);
   if (apply && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "order changes run in the FP64 form only");
@@ -1354,6 +1452,7 @@ static AdjArgs adj_args(dg_ctx* c) {
 
 dg_status dg_residual_jvp(dg_ctx* c, const double* U, const double* dU, double* dR) {
   if (!ready(c) || !U || !dU || !dR) return fail(c, DG_ERR_ARG, "dg_residual_jvp: bad argument");
+  if (c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "residual derivatives run meshes without hanging nodes");
   if (c->mixed || c->dim != 2 || c->pvel)
     return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform-order triangles (constant velocity)");
   HIPCHECK(c, hipSetDevice(c->device));
@@ -1369,6 +1468,7 @@ dg_status dg_residual_jvp(dg_ctx* c, const double* U, const double* dU, double* 
 
 dg_status dg_residual_vjp(dg_ctx* c, const double* U, const double* z, double* w) {
   if (!ready(c) || !U || !z || !w) return fail(c, DG_ERR_ARG, "dg_residual_vjp: bad argument");
+  if (c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "residual derivatives run meshes without hanging nodes");
   if (c->mixed || c->dim != 2 || c->pvel)
     return fail(c, DG_ERR_UNSUPPORTED, "derivatives are built for uniform-order triangles (constant velocity)");
   HIPCHECK(c, hipSetDevice(c->device));
@@ -1424,6 +1524,7 @@ static double density_misfit(const dg_ctx* c, const double* U, const double* tar
 dg_status dg_adjoint_forward(dg_ctx* c, double dt, int nsteps, int ckpt_every, const double* target, double* J_out) {
   if (!ready(c) || !(dt > 0) || nsteps <= 0 || ckpt_every <= 0)
     return fail(c, DG_ERR_ARG, "dg_adjoint_forward: bad argument");
+  if (c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "the adjoint runs meshes without hanging nodes");
   if (c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the FP64 form");
   if (c->mixed || c->lim_on || c->dim != 2 || c->pvel)
     return fail(c, DG_ERR_UNSUPPORTED, "the adjoint is built for the unlimited uniform-order triangle residual");

@@ -6,6 +6,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -86,6 +87,24 @@ struct dg_ctx {
   LtsPlan lts_plan;
   LtsTile *lts_tiles = nullptr, *lts_ctiles = nullptr;  // the plan's tile lists (device)
   double* dtscale = nullptr;
+  // hanging nodes (faces of half an edge): the elements that own such faces,
+  // their records (HangArgs) and the half-face basis tables of order P
+  int hang_n = 0;
+  int32_t *hang_elem = nullptr, *hang_rstart = nullptr, *hang_rec = nullptr;
+  double* phih = nullptr;
+  // h-adaptivity (dg_adapt_h): per current element its refinement level, its
+  // family (index into hfam; -1: an element of the mesh given to dg_create) and
+  // its place among the family's four children; each family keeps its parent's
+  // corners, tags and own history; the midpoint id of every edge ever split
+  // (sorted end ids -> id), and the next free vertex id
+  struct HFam {
+    int32_t vids[3], bc[3], level, fam, ci;
+    double xyz[6];
+  };
+  std::vector<int32_t> h_level, h_fam, h_ci;
+  std::vector<HFam> hfam;
+  std::map<uint64_t, int32_t> h_mid;
+  int32_t h_nextid = -1;
   uint8_t* npflags = nullptr;
   unsigned* status = nullptr;
   std::vector<void*> allocs;
@@ -124,6 +143,10 @@ inline dg_status fail(dg_ctx* c, dg_status s, const char* fmt, ...) {
   } while (0)
 
 inline bool ready(const dg_ctx* c) { return c && c->stream && c->Un; }
+
+// a new triangle mesh of the same order for the context (dg_adapt_h)
+dg_status ctx_remesh(dg_ctx* c, const std::vector<int32_t>& vids, const std::vector<double>& xyz,
+                     const std::vector<int32_t>& bc, const std::vector<double>& U);
 
 // new per-element orders (caller numbering) for a triangle context, carrying
 // the state over (raised orders get zero modes, lowered orders drop the top ones)

@@ -97,6 +97,32 @@ int launch_min_reduce(const double* part, int n, double* out, hipStream_t s);
 // *status |= 4 when any of U[0 .. n) is NaN or Inf (one atomic per wave that meets one)
 int launch_nonfinite_scan(const double* U, long n, unsigned* status, hipStream_t s);
 int launch_fill_one(double* out, double v, hipStream_t s);
+// Meshes with hanging nodes: after each stage kernel, one work-item per element
+// with faces of half an edge integrates them (both sides evaluated at the
+// half's Gauss points, the interior two-point flux, FP64) and adds the
+// contribution to the residual (MODE_RESIDUAL) or, through the stage update's
+// coefficient of dt R / |J|, to the stage output and the RK4 accumulator.
+struct HangArgs {
+  int E, P, n;            // n: elements with faces of half an edge
+  const int32_t* elem;    // [n] internal element
+  const int32_t* rstart;  // [n + 1] into rec
+  // per face of half an edge and side: own local face, the other side's
+  // internal element, its local face, flags: bit 0 the own side is the coarse
+  // cell, bit 1 the child runs against the coarse face's direction, bit 2 the half
+  const int32_t* rec;     // [nrec][4]
+  const double* Uin;      // stage input (FP64)
+  double* Uout;
+  double* acc;
+  double* R;
+  const double* tab;      // the order-P block (FP64)
+  const double* phih;     // [3][2][NQF][NP]: the basis at the Gauss points of each face's halves
+  TriElemTable el;
+  LawParams law;
+  double dt;
+  int scheme, stage;
+};
+int launch_hang(const HangArgs& a, int law, hipStream_t s);
+
 // local time stepping: Un += acc on rows [0, nrow) of the tiles' elements
 int launch_lts_commit(int E, int nrow, const LtsTile* tiles, int ntiles, const double* acc, double* Un,
                       hipStream_t s);

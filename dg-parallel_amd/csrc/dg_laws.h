@@ -37,7 +37,10 @@ enum { FLUX_UPWIND = 0, FLUX_LF = 1, FLUX_ROE = 2, FLUX_CENTRAL = 3, FLUX_ROE_EF
 // BC_CHAR: the Euler characteristic far field (tag 30000), whose flux is the
 // physical flux of a boundary state; the two-point flux takes that state on
 // both sides (FP64 form)
-enum { BC_FARFIELD = 1, BC_WALL = 2, BC_PEC = 3, BC_FARFIELD2 = 4, BC_CHAR = 5 };
+// BC_HANG: a face of a mesh with hanging nodes (a coarse cell's face split on
+// the other side, or a half of such a face): the stage kernels add nothing for
+// it; the hanging-face pass integrates it over its halves
+enum { BC_FARFIELD = 1, BC_WALL = 2, BC_PEC = 3, BC_FARFIELD2 = 4, BC_CHAR = 5, BC_HANG = 6 };
 
 struct LawParams {
   double gamma;

@@ -2,6 +2,8 @@
 #include "dg_setup.h"
 
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -19,6 +21,99 @@ static dg_status failS(std::string& err, dg_status s, const char* fmt, ...) {
   va_end(ap);
   err = buf;
   return s;
+}
+
+// faces of half an edge among the faces without a neighbour (see dg_setup.h).
+// A face split into more than two pieces is not found (its pieces are no
+// halves): such faces stay boundary faces; dg_adapt_h keeps meshes 1-irregular.
+static void find_hanging(const dg_mesh_desc* m, TriTopo& T) {
+  T.hang.clear();
+  const int E = m->n_elems;
+  // faces without a neighbour: (vertex, e*3+l) for both ends, sorted
+  std::vector<std::pair<int32_t, int32_t>> at;
+  for (int e = 0; e < E; e++)
+    for (int l = 0; l < 3; l++)
+      if (T.nbe[e * 3 + l] < 0) {
+        at.push_back({m->vids[e * 3 + l], e * 3 + l});
+        at.push_back({m->vids[e * 3 + (l + 1) % 3], e * 3 + l});
+      }
+  if (at.empty()) return;
+  std::sort(at.begin(), at.end());
+  auto range = [&](int32_t v) {
+    auto lo = std::lower_bound(at.begin(), at.end(), std::make_pair(v, (int32_t)INT32_MIN));
+    auto hi = std::upper_bound(at.begin(), at.end(), std::make_pair(v, (int32_t)INT32_MAX));
+    return std::make_pair(lo, hi);
+  };
+  auto vert = [&](int f, int i) { return m->vids[(f / 3) * 3 + (f % 3 + i) % 3]; };
+  // vector from the face's vertex i to vertex j (0 or 1) in its element's coordinates
+  auto vec = [&](int f, int i, int j, double* d) {
+    const double* X = m->xyz + (size_t)(f / 3) * 6;
+    const int a = (f % 3 + i) % 3, b = (f % 3 + j) % 3;
+    d[0] = X[b * 2] - X[a * 2];
+    d[1] = X[b * 2 + 1] - X[a * 2 + 1];
+  };
+  std::vector<char> used((size_t)E * 3, 0);
+  std::vector<std::pair<int, int>> halves;  // per coarse face found: the two half faces
+  for (int e = 0; e < E; e++)
+    for (int l = 0; l < 3; l++) {
+      const int F = e * 3 + l;
+      if (T.nbe[F] >= 0 || used[F]) continue;
+      const int32_t a = vert(F, 0), b = vert(F, 1);
+      int h1 = -1, h2 = -1;
+      auto ra = range(a);
+      for (auto it = ra.first; it != ra.second && h1 < 0; ++it) {
+        const int g = it->second;
+        if (g / 3 == e || used[g]) continue;
+        const int ia = vert(g, 0) == a ? 0 : 1;
+        const int32_t x = vert(g, 1 - ia);
+        auto rx = range(x);
+        for (auto jt = rx.first; jt != rx.second; ++jt) {
+          const int g2 = jt->second;
+          if (g2 == g || g2 / 3 == e || used[g2]) continue;
+          const int ix = vert(g2, 0) == x ? 0 : 1;
+          if (vert(g2, 1 - ix) != b) continue;
+          // x is the midpoint of (a, b): x - a equals b - x
+          double d1[2], d2[2];
+          vec(g, ia, 1 - ia, d1);
+          vec(g2, ix, 1 - ix, d2);
+          if (std::fabs(d1[0] - d2[0]) + std::fabs(d1[1] - d2[1]) > 1e-9 * (std::fabs(d1[0]) + std::fabs(d1[1])))
+            continue;
+          h1 = g;
+          h2 = g2;
+          break;
+        }
+      }
+      if (h1 < 0) continue;
+      used[F] = used[h1] = used[h2] = 1;
+      for (int t = 0; t < 2; t++) {
+        const int g = t ? h2 : h1;
+        HangFace H;
+        H.child = g / 3;
+        H.lc = g % 3;
+        H.K = e;
+        H.lK = l;
+        // the half holding K's vertex l (= a) is half 0
+        const int32_t ga = vert(g, 0), gb = vert(g, 1);
+        H.h = (ga == a || gb == a) ? 0 : 1;
+        // K runs a -> b; the child runs ga -> gb: along K's direction when it
+        // starts at a (half 0) or at the midpoint (half 1)
+        const bool along = H.h == 0 ? ga == a : gb == b;
+        H.rev = along ? 0 : 1;
+        T.hang.push_back(H);
+        T.bcode[g] = BC_HANG;
+      }
+      T.bcode[F] = BC_HANG;
+    }
+  // faces list: each coarse face becomes its two halves (child left, K right)
+  std::vector<int32_t> nf;
+  nf.reserve(T.faces.size() + 4 * T.hang.size());
+  for (size_t i = 0; i < T.faces.size(); i += 4) {
+    const int Le = T.faces[i], Ll = T.faces[i + 1];
+    if (T.faces[i + 2] < 0 && used[Le * 3 + Ll]) continue;  // a coarse face or a half: listed below
+    nf.insert(nf.end(), T.faces.begin() + i, T.faces.begin() + i + 4);
+  }
+  for (const HangFace& H : T.hang) nf.insert(nf.end(), {H.child, H.lc, H.K, H.lK});
+  T.faces.swap(nf);
 }
 
 dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err) {
@@ -61,6 +156,7 @@ dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err) {
     i = j;
   }
   T.bcode.assign((size_t)E * 3, 0);
+  find_hanging(m, T);
   T.sig.assign(E, 0);
   for (int e = 0; e < E; e++) {
     int sg = 0;
@@ -69,6 +165,11 @@ dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err) {
       if (T.nbe[e * 3 + l] >= 0) {
         v = T.nbl[e * 3 + l] | (T.same[e * 3 + l] << 2);
       } else {
+        if (T.bcode[e * 3 + l] == BC_HANG) {
+          v = 8 + BC_HANG;
+          sg = sg * 16 + v;
+          continue;
+        }
         const int tag = m->bc ? m->bc[e * 3 + l] : DG_TAG_FARFIELD;
         // 20000, the reference's curved wall, is the 40000 wall on straight faces
         const int code = (tag == DG_TAG_WALL || tag == DG_TAG_CURVED_WALL) ? BC_WALL
@@ -177,6 +278,42 @@ void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* gr
                                            : -(int32_t)T.bcode[e * 3 + l];
     }
   }
+}
+
+void build_hang_tables(const TriSetup& S, std::vector<int32_t>& elem, std::vector<int32_t>& rstart,
+                       std::vector<int32_t>& rec, std::vector<double>& ph) {
+  const int E = S.E;
+  std::vector<int32_t> iperm(E);
+  for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
+  std::vector<std::array<int32_t, 5>> r;  // own, local face, other, its local face, flags
+  for (const HangFace& H : S.topo.hang) {
+    const int fl = (H.rev ? 2 : 0) | (H.h << 2);
+    r.push_back({iperm[H.child], H.lc, iperm[H.K], H.lK, fl});
+    r.push_back({iperm[H.K], H.lK, iperm[H.child], H.lc, 1 | fl});
+  }
+  std::sort(r.begin(), r.end());
+  for (size_t k = 0; k < r.size(); k++) {
+    if (k == 0 || r[k][0] != r[k - 1][0]) {
+      elem.push_back(r[k][0]);
+      rstart.push_back((int32_t)k);
+    }
+    rec.insert(rec.end(), {r[k][1], r[k][2], r[k][3], r[k][4]});
+  }
+  rstart.push_back((int32_t)r.size());
+  const int P = S.P, NP = np_tri(P), NQF = nqf_tri(P);
+  ph.assign((size_t)3 * 2 * NQF * NP, 0.0);
+  const auto gl = gl_rule(NQF);
+  double phi[64], xi[3];
+  for (int l = 0; l < 3; l++)
+    for (int h = 0; h < 2; h++)
+      for (int q = 0; q < NQF; q++) {
+        // the face rule's point q lies at fraction (1 - s_q) / 2 from vertex l
+        // (edge_point); on half h at h / 2 + (1 - s_q) / 4
+        const double t = 0.5 * h + 0.25 * (1.0 - gl[q].x[0]);
+        edge_point(l, 1.0 - 2.0 * t, xi);
+        basis(2, P, xi, phi, nullptr);
+        for (int k = 0; k < NP; k++) ph[((l * 2 + h) * NQF + q) * NP + k] = phi[k];
+      }
 }
 
 dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const dg_options* opt, TriSetup& S,

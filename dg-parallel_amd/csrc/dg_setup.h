@@ -14,8 +14,16 @@ namespace dgp {
 
 // What order changes keep: face connectivity, boundary codes and geometry of
 // every element, in the caller's numbering.
+// A face of half an edge on a mesh with hanging nodes: the child's local face
+// lc lies on half h of the coarse cell K's local face lK (half 0 holds K's
+// vertex lK); rev: the child runs along the face against K's direction.
+struct HangFace {
+  int child, lc, K, lK, h, rev;
+};
+
 struct TriTopo {
   int E = 0;
+  std::vector<HangFace> hang;           // faces of half an edge (1-irregular meshes)
   std::vector<int32_t> nbe, nbl, same;  // [E*3]: neighbour, its local face, equal first vertex (-1: none)
   std::vector<int8_t> bcode;            // [E*3]: BC_* of a face without neighbour
   std::vector<int32_t> sig;             // [E]: link signature (faces' neighbour face, orientation, boundary)
@@ -28,7 +36,12 @@ struct TriTopo {
 double tri_cell_size(const double* X);
 
 // Caller topology and geometry (mMesh.cc:321-378, Mapping.cc:442-566,
-// DGCell.cc:1209-1229); validates the mesh.
+// DGCell.cc:1209-1229); validates the mesh.  A face (a, b) without a
+// neighbour, where a vertex m closes two other such faces (a, m) and (m, b)
+// with m - a = b - m (each in its own element's coordinates), is a coarse
+// cell's face split on the other side (red refinement, mDGMesh.cc:54-337): the
+// two halves become faces of half an edge (hang, faces list: child left, coarse
+// cell right), and the three faces get the code BC_HANG.
 dg_status build_tri_topo(const dg_mesh_desc* m, TriTopo& T, std::string& err);
 
 struct TriSetup {
@@ -75,6 +88,15 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
 // a group key ahead of the blocks (the classes of local time stepping), so each
 // group is one internal range.
 void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* grp = nullptr);
+
+// The hanging-face pass's tables for S (one order): the owning elements of faces
+// of half an edge (internal numbering), records [4] per face and side (own
+// local face, the other side, its local face, flags: bit 0 own side coarse,
+// bit 1 reversed points, bit 2 the half) grouped by element (rstart), and the
+// order-P basis at the Gauss points of each face's halves [3][2][NQF][NP]
+// (each half's points in the face's direction).
+void build_hang_tables(const TriSetup& S, std::vector<int32_t>& elem, std::vector<int32_t>& rstart,
+                       std::vector<int32_t>& rec, std::vector<double>& ph);
 
 // Tetrahedra (Maxwell, one order): the same roles as TriSetup.  mg holds
 // |J| J^-1 as [9][E] (row-major r*3+d), size the shortest edge (DGCell.cc:1230-1262),

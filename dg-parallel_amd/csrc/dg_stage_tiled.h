@@ -410,9 +410,13 @@ struct TiledStage {
       const int it = wave + NW * j;
       if (it < 3 * NQF) {
         const int l = it / NQF, q = it - l * NQF;
-        const int link = s.Ls[l][t];
+        // a face of a mesh with hanging nodes adds nothing here (the hanging-face
+        // pass integrates its halves): the flux of the own trace on both sides,
+        // weight zero
+        const bool hang = s.Ls[l][t] == -BC_HANG;
+        const int link = hang ? 0 : s.Ls[l][t];
         const CT n[2] = {s.Ns[l][0][t], s.Ns[l][1][t]};
-        const CT len = s.Ns[l][2][t];
+        const CT len = hang ? CT(0) : s.Ns[l][2][t];
         CT uL[NF], uR[NF], Fh[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
@@ -420,7 +424,7 @@ struct TiledStage {
 #pragma unroll
           for (int k = 0; k < NP; k++) acc += tb[TriBlock::off_phif(P) + (l * NQF + q) * NP + k] * s.Us[f * NP + k][t];
           uL[f] = acc;
-          uR[f] = s.W[(l * NQF + q) * NF + f][t];
+          uR[f] = hang ? acc : s.W[(l * NQF + q) * NF + f][t];
         }
         // boundary faces: the law's ghost state, then the same two-point flux (one
         // inlined flux for interior and boundary lanes alike)

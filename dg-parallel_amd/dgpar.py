@@ -92,6 +92,8 @@ _advance = _sig("dg_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double,
 _time_advance = _sig("dg_time_advance", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                        ctypes.POINTER(ctypes.c_float)])
 _lts_setup = _sig("dg_lts_setup", ctypes.c_int, [_vp, ctypes.c_int, _p_i32])
+_adapt_h = _sig("dg_adapt_h", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int8), ctypes.c_int, _p_i32])
+_get_mesh = _sig("dg_get_mesh", ctypes.c_int, [_vp, _p_i32, _p_i32, _p_f64, _p_i32, _p_i32])
 class SensorOpts(ctypes.Structure):
     _fields_ = [("s_hi", ctypes.c_double * 4), ("s_lo", ctypes.c_double * 4), ("pmin", ctypes.c_int),
                 ("pmax", ctypes.c_int)]
@@ -403,6 +405,33 @@ class DGAnalysis:
         return lev
 
     lts_setup = sortCellsBySize
+
+    def adaptH(self, marks, max_level=1):
+        """h-adaptivity (DGAnalysis::adaptH's mesh operations): cells marked > 0 are
+        split into four, families of four children marked < 0 merged; the state
+        follows by L2 projection.  Returns the new cell count; refinedMesh() gives
+        the new mesh (get_state/set_state use its cells)."""
+        marks = np.ascontiguousarray(marks, np.int8)
+        if marks.shape != (self.mesh_E,):
+            raise ValueError("one mark per current cell (%d)" % self.mesh_E)
+        n = ctypes.c_int32()
+        self._check(_adapt_h(self._h, marks.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), max_level, ctypes.byref(n)))
+        self.mesh_E = n.value
+        self.ndof = _ndof(self._h)
+        return n.value
+
+    def refinedMesh(self):
+        """the current mesh (after adaptH): (vids [E,3], xyz [E,3,2], bc [E,3], levels [E])"""
+        n = ctypes.c_int32()
+        self._check(_get_mesh(self._h, ctypes.byref(n), None, None, None, None))
+        E = n.value
+        vids = np.zeros((E, 3), np.int32)
+        xyz = np.zeros((E, 3, 2))
+        bc = np.zeros((E, 3), np.int32)
+        lev = np.zeros(E, np.int32)
+        self._check(_get_mesh(self._h, None, _ptr(vids, _p_i32), _ptr(xyz, _p_f64), _ptr(bc, _p_i32),
+                              _ptr(lev, _p_i32)))
+        return vids, xyz, bc, lev
 
     def time_advance(self, scheme, dt, nsteps):
         ms = ctypes.c_float()

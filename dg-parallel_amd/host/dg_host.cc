@@ -377,6 +377,26 @@ DGAnalysis::DGAnalysis(const Mesh& mesh, ConservationLaw* law, DGLimiter* limite
     integrator_ = new RungeKutta4(this);
 }
 
+int DGAnalysis::adaptH(const std::vector<int8_t>& marks, int max_level) {
+  if ((int)marks.size() != n_elems_) throw DGError(DG_ERR_ARG, "adaptH: one mark per cell");
+  int32_t n = 0;
+  check(dg_adapt_h(ctx_, marks.data(), max_level, &n));
+  n_elems_ = n;
+  return n;
+}
+
+Mesh DGAnalysis::refinedMesh() const {
+  Mesh m;
+  m.dim = 2;
+  int32_t n = 0;
+  check(dg_get_mesh(ctx_, &n, nullptr, nullptr, nullptr, nullptr));
+  m.vids.resize((size_t)n * 3);
+  m.xyz.resize((size_t)n * 6);
+  m.bc.resize((size_t)n * 3);
+  check(dg_get_mesh(ctx_, nullptr, m.vids.data(), m.xyz.data(), m.bc.data(), nullptr));
+  return m;
+}
+
 std::vector<int32_t> DGAnalysis::sortCellsBySize(int max_levels) {
   std::vector<int32_t> lev(n_elems_);
   check(dg_lts_setup(ctx_, max_levels, lev.data()));

@@ -319,6 +319,13 @@ class DGAnalysis {
   // RungeKuttaTVD2adaptive; computeDt then returns the coarsest level's step.
   // Returns each cell's level (caller order).
   std::vector<int32_t> sortCellsBySize(int max_levels = 6);
+  // h-adaptivity (DGAnalysis::adaptH's mesh operations, DGPref.cc:78-192): cells
+  // marked > 0 are split into four (at most max_level times, LEVEL_OF_REFINEMENT),
+  // families of four children marked < 0 merged, the state L2-projected; the
+  // cells then follow the new mesh (refinedMesh()).  Returns the new cell count.
+  int adaptH(const std::vector<int8_t>& marks, int max_level = 1);
+  // the current mesh after adaptH (the reference exports it as refined-N.msh)
+  Mesh refinedMesh() const;
   // launch schedule of the stage kernel (no counterpart in the reference): the
   // tuner times every candidate over `steps` steps of the integrator's scheme
   // with step dt on the current state (restored after) and keeps the fastest;

@@ -194,6 +194,35 @@ dg_status dg_advance(dg_ctx* ctx, int rk_scheme, double t, double dt, int nsteps
  * every cell on level 0. */
 dg_status dg_lts_setup(dg_ctx* ctx, int max_levels, int32_t* levels_out);
 
+/* ---- h-adaptivity (triangles, FP64 form, one order) -----------------------
+ * DGAnalysis::adaptH's mesh operations (DGPref.cc:78-192): cells marked > 0 are
+ * split into four through their edge midpoints (mDGMesh::splitTriangle,
+ * mDGMesh.cc:234-337) while below max_level refinements (LEVEL_OF_REFINEMENT),
+ * and families of four children all marked < 0 are merged back into their
+ * parent (mDGMesh::unsplit, mDGMesh.cc:463-552); the state moves by exact L2
+ * projection (DGAnalysis::splitCell / unsplitCell, DGPref.cc:632-671).  The
+ * reference chooses the marks from its error estimate (modifyP, DGPref.cc:
+ * 27-76); here the caller passes them (one per current cell, caller order).
+ * Neighbours of split cells stay unsplit (hanging nodes: each of their split
+ * faces is integrated over its two halves, as the reference's boundary cells of
+ * leaf edges do, DGCell.cc:423-555); a coarser neighbour of a split cell is
+ * split too, and a merge that would put a cell beside one two levels finer is
+ * skipped, so every face meets at most two halves.  The context is rebuilt on
+ * the new mesh: split cells give their place to their four children
+ * (corners (P0, M01, M20), (M01, P1, M12), (M20, M12, P2), (M12, M20, M01)),
+ * a merged family's parent takes its first child's place, the state, orders
+ * and dg_get_state's layout follow the new cells; a new midpoint takes the next
+ * free vertex id (the same id for the edge's other side and for later splits).
+ * n_elems_out (may be NULL) receives the new cell count.  Meshes with hanging
+ * nodes run the scalar, Euler, Maxwell2d and Burgers2D laws with every RK
+ * scheme; the limiter, recovery, local time stepping, order changes, velocity
+ * fields and residual derivatives refuse them (DG_ERR_UNSUPPORTED). */
+dg_status dg_adapt_h(dg_ctx* ctx, const int8_t* marks, int max_level, int32_t* n_elems_out);
+/* the current mesh (after dg_adapt_h: caller order): the cell count, corner ids
+ * [E][3], coordinates [E][3][2], face tags [E][3] and refinement levels [E];
+ * any pointer may be NULL */
+dg_status dg_get_mesh(const dg_ctx* ctx, int32_t* n_elems, int32_t* vids, double* xyz, int32_t* bc, int32_t* levels);
+
 /* Device-resident variants for callers that keep state on the GPU (torch tensors):
  * pointers are device pointers holding dg_ndof() doubles in the boundary layout. */
 dg_status dg_set_state_device(dg_ctx* ctx, const double* dU);

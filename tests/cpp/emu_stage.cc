@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../dg-parallel_amd/csrc/dg_adjoint.h"
+#include "../../dg-parallel_amd/csrc/dg_hang.h"
 #include "../../dg-parallel_amd/csrc/dg_mixed.h"
 #include "../../dg-parallel_amd/csrc/dg_mixed_tiled.h"
 #include "../../dg-parallel_amd/csrc/dg_tet.h"
@@ -91,6 +92,9 @@ struct Emu {
   TriSetup S;
   std::vector<double> Un, acc, R, A64, B64, part;
   std::vector<float> A32, B32;
+  // the hanging-face pass's tables (meshes with hanging nodes)
+  std::vector<int32_t> helem, hrstart, hrec;
+  std::vector<double> hph;
   StageArgs base() {
     StageArgs a{};
     a.E = S.E;
@@ -212,6 +216,51 @@ int make(Emu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int order, co
   m.A32.assign(n, 0.0f);
   m.B32.assign(n, 0.0f);
   m.part.assign((m.S.E + TILE_ELEMS - 1) / TILE_ELEMS + 1, 0.0);
+  if (!m.S.topo.hang.empty()) build_hang_tables(m.S, m.helem, m.hrstart, m.hrec, m.hph);
+  return 0;
+}
+
+template <class LAW>
+void hang_law(const HangArgs& h) {
+  for (int i = 0; i < h.n; i++) switch (h.P) {
+      case 1: hang_element<1, LAW>(h, i); break;
+      case 2: hang_element<2, LAW>(h, i); break;
+      case 3: hang_element<3, LAW>(h, i); break;
+      case 4: hang_element<4, LAW>(h, i); break;
+    }
+}
+
+// the library's stage launch: the stage kernel, then (FP64, meshes with hanging
+// nodes) the hanging-face pass on the same arguments
+int run_stage(Emu& m, const StageArgs& a, int variant) {
+  if (run_variant(a, m.S.law, variant)) return -2;
+  if (m.helem.empty()) return 0;
+  if ((variant & 3) != 0) return -3;
+  HangArgs h{};
+  h.E = m.S.E;
+  h.P = m.S.P;
+  h.n = (int)m.helem.size();
+  h.elem = m.helem.data();
+  h.rstart = m.hrstart.data();
+  h.rec = m.hrec.data();
+  h.Uin = (const double*)a.Uin;
+  h.Uout = (double*)a.Uout;
+  h.acc = a.acc;
+  h.R = a.R;
+  h.tab = m.S.tab.data();
+  h.phih = m.hph.data();
+  h.el = a.el;
+  h.law = m.S.lp;
+  h.dt = a.dt;
+  h.scheme = a.scheme;
+  h.stage = a.stage;
+  const int law = m.S.law;
+  if (law == LAW_SCALAR) hang_law<ScalarLawD>(h);
+  else if (law == LAW_EULER2D && h.law.flux == FLUX_EXACT) hang_law<EulerExactLawD>(h);
+  else if (law == LAW_EULER2D) hang_law<EulerLawD>(h);
+  else if (law == LAW_MAXWELL2D) hang_law<Maxwell2dLawD>(h);
+  else if (law == LAW_BURGERS2D) hang_law<Burgers2DLawD>(h);
+  else return -4;
   return 0;
 }
 
@@ -228,7 +277,7 @@ extern "C" int emu_tri_residual(const dg_mesh_desc* mesh, const dg_law_desc* law
   a.Uin = m.Un.data();
   a.R = m.R.data();
   a.scheme = MODE_RESIDUAL;
-  if (run_variant(a, m.S.law, (fp32 ? 1 : 0) | shape << 2)) return -2;
+  if (run_stage(m, a, (fp32 ? 1 : 0) | shape << 2)) return -2;
   m.from_soa(m.R.data(), R);
   return 0;
 }
@@ -285,7 +334,7 @@ extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, i
       if (first && last) a.Uout = bufA;
       a.resid_part = last ? m.part.data() : nullptr;
       const int variant = (fp32 ? (first ? 1 : (last ? 3 : 2)) : 0) | shape << 2;
-      if (run_variant(a, m.S.law, variant)) return -2;
+      if (run_stage(m, a, variant)) return -2;
     }
     if (ns == 1) m.Un = m.A64;
   }

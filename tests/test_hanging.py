@@ -99,3 +99,69 @@ def test_linear_field_exact(p, levels):
     inner = ~boundary
     assert inner.sum() > len(v) // 2
     assert np.abs(R[inner] - expect[inner]).max() <= 1e-12 * np.abs(expect).max()
+
+
+# ---- the stage kernel and the hanging-face pass, CPU emulation vs the oracle ---
+def _emu():
+    import ctypes
+    import os
+
+    import dgpar
+    from helpers import ROOT
+    path = os.path.join(ROOT, "tests", "cpp", "libemu.so")
+    if not os.path.exists(path):
+        pytest.skip("tests/cpp/libemu.so not built")
+    lib = ctypes.CDLL(path)
+    p_md, p_ld = ctypes.POINTER(dgpar.MeshDesc), ctypes.POINTER(dgpar.LawDesc)
+    pd = ctypes.POINTER(ctypes.c_double)
+    lib.emu_tri_residual.argtypes = [p_md, p_ld, ctypes.c_int, ctypes.c_int, pd, pd]
+    lib.emu_tri_steps.argtypes = [p_md, p_ld, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_int, pd, pd]
+    return lib, dgpar, ctypes
+
+
+def hang_case(law, p, periodic=True):
+    import dgpar
+    v, x, b = split_mesh(lx=10.0, periodic=periodic)
+    ff = (1.0, 0.1, 0.05, 2.5)
+    if law == "scalar":
+        o = po.Oracle(2, v, x, p, po.LAW_SCALAR, po.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.2,), bc=b)
+        lw = dgpar.Law(dgpar.LAW_SCALAR, dgpar.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.2,))
+        U = o.project_fn(lambda X, Y, Z, t: (np.exp(-((X - 5) ** 2 + (Y - 5) ** 2) / 4.0),))
+    else:
+        fl = {"lf": (po.FLUX_LF, dgpar.FLUX_LF), "roe": (po.FLUX_ROE, dgpar.FLUX_ROE)}[law]
+        o = po.Oracle(2, v, x, p, po.LAW_EULER2D, fl[0], gamma=1.4, farfield=ff, bc=b)
+        lw = dgpar.Law(dgpar.LAW_EULER2D, fl[1], gamma=1.4, farfield=ff)
+        U = o.project(1, VORTEX)
+    U = U + 0.002 * np.random.default_rng(11).standard_normal(U.shape)
+    return dgpar.Mesh(2, v, x, b), lw, o, U
+
+
+@pytest.mark.parametrize("law,p,periodic", [("scalar", 1, True), ("scalar", 3, False), ("lf", 1, True),
+                                             ("lf", 2, False), ("roe", 3, True), ("roe", 4, True)])
+def test_emulated_residual_matches_oracle(law, p, periodic):
+    lib, dgpar, ctypes = _emu()
+    mesh, lw, o, U = hang_case(law, p, periodic)
+    Ro, _ = o.residual(U)
+    R = np.zeros_like(U)
+    md, ld = mesh.desc(), lw.desc()
+    pd = ctypes.POINTER(ctypes.c_double)
+    assert lib.emu_tri_residual(ctypes.byref(md), ctypes.byref(ld), p, 0, U.ctypes.data_as(pd), R.ctypes.data_as(pd)) == 0
+    assert np.abs(R - Ro).max() <= 1e-12 * np.abs(Ro).max()
+
+
+@pytest.mark.parametrize("scheme", [po.RK4, po.RK_TVD3, po.RK_TVD2, po.RK_FE])
+def test_emulated_steps_match_oracle(scheme):
+    lib, dgpar, ctypes = _emu()
+    mesh, lw, o, U = hang_case("lf", 2)
+    dt = o.dt(U, 0.1)
+    V = U
+    for k in range(2):
+        V, _ = o.step(V, scheme, k * dt, dt)
+    W = np.array(U, copy=True)
+    md, ld = mesh.desc(), lw.desc()
+    pd = ctypes.POINTER(ctypes.c_double)
+    r = ctypes.c_double()
+    assert lib.emu_tri_steps(ctypes.byref(md), ctypes.byref(ld), 2, 0, scheme, dt, 2, W.ctypes.data_as(pd),
+                             ctypes.byref(r)) == 0
+    assert np.abs(W - V).max() <= 1e-12 * np.abs(V).max()
