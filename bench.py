@@ -550,6 +550,42 @@ def run_lts(dgpar, steps=40, n=256, levels=4):
     return out
 
 
+def run_hadapt(dgpar, steps=100, n=256, radius=2.0):
+    """h-adaptivity (dg_adapt_h): the P3 LF isentropic vortex on the 256x256x2
+    jittered periodic mesh of [0,10]^2; the cells within `radius` of the vortex
+    centre are split twice (two passes, 1-irregular closure), which leaves
+    hanging nodes on the patch's rim.  Reported: the wall time of each adapt
+    pass (host refinement, projection and context rebuild), RK4 throughput on
+    the adapted mesh (stage kernels + hanging-face passes) and the DOF share of
+    each level; the totals' drift through the passes."""
+    mesh = dgpar.mesh_tri_structured(n, n, lx=10.0, ly=10.0, jitter=0.1, seed=SEED)
+    law = dgpar.EulerLaw(dgpar.FLUX_LF, 1.4)
+    with dgpar.DGAnalysis(mesh, law, ORDER, use_graph=True) as g:
+        g.project(dgpar.FIELD_VORTEX, VORTEX)
+        t0 = g.norms()["totals"]
+        adapt_s = []
+        for _ in range(2):
+            v, x, b, lev = g.refinedMesh()
+            c = x.mean(axis=1)
+            marks = (np.hypot(c[:, 0] - 5.0, c[:, 1] - 5.0) < radius).astype(np.int8)
+            w0 = time.perf_counter()
+            g.adaptH(marks, 2)
+            adapt_s.append(time.perf_counter() - w0)
+        v, x, b, lev = g.refinedMesh()
+        drift = float(np.abs(g.norms()["totals"] - t0).max() / np.abs(t0).max())
+        dt = g.computeDt(CFL)
+        g.time_advance(dgpar.RK4, dt, 5)  # warm-up (graph capture)
+        dgpar.device_synchronize()
+        ms = g.time_advance(dgpar.RK4, dt, steps)
+        ndof = g.ndof
+    E = len(v)
+    return {"workload": "euler2d-isentropic-vortex P3 LF RK4, 256x256x2 periodic [0,10]^2 jitter 0.1h, cells within "
+                        "%.1f of the centre split twice (hanging nodes on the rim)" % radius,
+            "elements": E, "dof": ndof, "levels_histogram": np.bincount(lev).tolist(),
+            "adapt_pass_s": adapt_s, "totals_rel_drift": drift, "dt": dt, "steps": steps,
+            "ms_per_step": ms / steps, "value": ndof * 4 * steps / (ms / 1e3), "unit": "DOF*stage/s"}
+
+
 def cpu_model():
     """the host CPU's model name and the cores this process may use"""
     name = "unknown"
@@ -601,6 +637,8 @@ def main():
                     help="SSP-RK2 steps of the ScalarPb leg (rotating velocity field; 0: skip it)")
     ap.add_argument("--lts-steps", type=int, default=40,
                     help="local-time-stepping steps of the graded-mesh leg (0: skip it)")
+    ap.add_argument("--hadapt-steps", type=int, default=100,
+                    help="RK4 steps on the h-adapted vortex mesh (0: skip it)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch schedules (no tuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 256 MiB device copy (reference point for HBM byte counters)")
@@ -675,6 +713,8 @@ def main():
         out["maxwell3d_tets"] = run_maxwell(dgpar, args.maxwell_steps, tune=tune)
     if args.lts_steps > 0:
         out["local_time_stepping"] = run_lts(dgpar, args.lts_steps)
+    if args.hadapt_steps > 0:
+        out["h_adaptivity"] = run_hadapt(dgpar, args.hadapt_steps)
     if not args.no_cpu:
         # the reference's own serial code on the same mesh (its build travels with
         # the repository as oracle/_ref); the FP64 oracle port on this host's
