@@ -13,6 +13,12 @@
 //   scalarpb the reference's ScalarPb: ScalarLaw with the rotating velocity
 //            FieldEvaluator, P1 on a periodic jittered 12x12x2 mesh of [-1,1]^2
 //   restart  the vortex written, plotted, read back and continued (see restart())
+//   lts      the vortex (Euler2dLF, P1) on a graded periodic mesh: sortCellsBySize(3)
+//            (integrator RungeKuttaTVD2adaptive), two steps of the coarsest level's dt;
+//            .L (levels as doubles), .U2
+//   adapt    the vortex (Euler2dLF, P2): adaptH splits the cells within 2 of the
+//            centre, two RK3 steps on the refined mesh; .M (refined corners
+//            [E][3][2]), .U2
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -173,6 +179,74 @@ static int restart(const std::string& out) {
   return 0;
 }
 
+// local time stepping: a tensor grid of [0,10]^2 with cell widths 4:2:1:1:2:4 in
+// both directions (blocks of 2 cells), P1 vortex, SSP-RK2 with local steps
+static int lts(const std::string& out) {
+  const int n = 12;
+  std::vector<double> xs(n + 1, 0.0);
+  const double w[6] = {4, 2, 1, 1, 2, 4};
+  for (int i = 0; i < n; i++) xs[i + 1] = xs[i] + w[i / 2];
+  for (double& x : xs) x *= 10.0 / 28.0;
+  dgpar::Mesh mesh;
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      const int c[2][3][2] = {{{0, 0}, {1, 0}, {1, 1}}, {{0, 0}, {1, 1}, {0, 1}}};
+      for (int t = 0; t < 2; t++)
+        for (int v = 0; v < 3; v++) {
+          const int a = i + c[t][v][0], b = j + c[t][v][1];
+          mesh.vids.push_back((b % n) * (n + 1) + (a % n));
+          mesh.xyz.push_back(xs[a]);
+          mesh.xyz.push_back(xs[b]);
+        }
+    }
+  dgpar::ConstantField far({1.0, 1.0, 1.0, 3.5});
+  dgpar::Euler2dLF law(&far, 1.4);
+  dgpar::DGAnalysis an(mesh, &law, nullptr, 1);
+  const double vortex[6] = {5.0, 5.0, 5.0, 1.0, 1.0, 10.0};
+  an.project(DG_FIELD_VORTEX, vortex);
+  const std::vector<int32_t> lev = an.sortCellsBySize(3);
+  dump(out + ".U0", an.getState());
+  dump(out + ".L", std::vector<double>(lev.begin(), lev.end()));
+  const double dt = an.computeDt(0.1);
+  const double r1 = an.integrator()->advanceInTime(0.0, dt);
+  const double r2 = an.integrator()->advanceInTime(dt, dt);
+  dump(out + ".U2", an.getState());
+  printf("dt %.17e\nresid %.17e %.17e\n", dt, r1, r2);
+  return 0;
+}
+
+// h-adaptivity: the P2 vortex on the periodic jittered 10x10x2 mesh of [0,10]^2,
+// the cells whose centroid lies within 2 of the centre split (adaptH, at most one
+// level), then two SSP-RK3 steps on the refined mesh
+static int adapt(const std::string& out) {
+  const int32_t tags[4] = {0, 0, 0, 0};
+  dgpar::Mesh mesh = dgpar::Mesh::structuredTri(10, 10, 0.0, 0.0, 10.0, 10.0, true, true, 0.1, 77, tags);
+  dgpar::ConstantField far({1.0, 1.0, 1.0, 3.5});
+  dgpar::Euler2dLF law(&far, 1.4);
+  dgpar::DGAnalysis an(mesh, &law, nullptr, 2);
+  const double vortex[6] = {5.0, 5.0, 5.0, 1.0, 1.0, 10.0};
+  an.project(DG_FIELD_VORTEX, vortex);
+  std::vector<int8_t> marks(mesh.n_elems(), 0);
+  for (int e = 0; e < mesh.n_elems(); e++) {
+    double cx = 0, cy = 0;
+    for (int v = 0; v < 3; v++) {
+      cx += mesh.xyz[e * 6 + v * 2] / 3.0;
+      cy += mesh.xyz[e * 6 + v * 2 + 1] / 3.0;
+    }
+    marks[e] = std::hypot(cx - 5.0, cy - 5.0) < 2.0 ? 1 : 0;
+  }
+  an.adaptH(marks, 1);
+  const dgpar::Mesh fine = an.refinedMesh();
+  dump(out + ".M", fine.xyz);
+  dump(out + ".U0", an.getState());
+  const double dt = an.computeDt(0.1);
+  const double r1 = an.integrator()->advanceInTime(0.0, dt);
+  const double r2 = an.integrator()->advanceInTime(dt, dt);
+  dump(out + ".U2", an.getState());
+  printf("dt %.17e\nresid %.17e %.17e\n", dt, r1, r2);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
   std::string out = argv[1], mode = argc > 2 ? argv[2] : "vortex";
@@ -181,6 +255,8 @@ int main(int argc, char** argv) {
     if (mode == "maxwell") return maxwell(out);
     if (mode == "restart") return restart(out);
     if (mode == "scalarpb") return scalarpb(out);
+    if (mode == "lts") return lts(out);
+    if (mode == "adapt") return adapt(out);
     return vortex(out);
   } catch (const dgpar::DGError& e) {
     fprintf(stderr, "DGError %d: %s\n", (int)e.status, e.what());

@@ -146,3 +146,40 @@ def test_cpp_dropin_scalarpb_with_a_velocity_field():
         U, r = o.step(U, po.RK_TVD2, s * dt, dt)
     assert np.abs(U2 - U).max() / np.abs(U).max() < 1e-12
     assert abs(res[1] - r) <= 1e-10 * abs(r)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="C++ driver not built")
+def test_cpp_dropin_local_time_stepping():
+    print("\n[gpu case] C++ drop-in local time stepping", flush=True)
+    from helpers import graded_nodes, mesh_tri_graded
+    pre, dt, res = run_exe("lts")
+    U0, L, U2 = (np.fromfile(pre + x) for x in (".U0", ".L", ".U2"))
+    xs = graded_nodes(12, 0.0, 10.0, (4, 2, 1, 1, 2, 4))
+    vids, xyz = mesh_tri_graded(xs, xs)
+    o = po.Oracle(2, vids, xyz, 1, po.LAW_EULER2D, po.FLUX_LF, gamma=1.4, farfield=(1.0, 1.0, 1.0, 3.5))
+    lev, n = o.lts_levels(3)
+    np.testing.assert_array_equal(L.astype(np.int32), lev)
+    assert abs(dt - o.lts_dt(U0, 0.1, lev)) <= 1e-14 * dt
+    assert res == [0.0, 0.0]  # RungeKuttaTVD2adaptive::advanceInTime returns 0
+    U = U0
+    for _ in range(2):
+        U = o.lts_step(U, lev, dt)
+    assert np.abs(U2 - U).max() / np.abs(U).max() < 1e-12
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="C++ driver not built")
+def test_cpp_dropin_h_adaptation():
+    print("\n[gpu case] C++ drop-in h-adaptation", flush=True)
+    from helpers import refine_tri
+    pre, dt, res = run_exe("adapt")
+    M, U0, U2 = (np.fromfile(pre + x) for x in (".M", ".U0", ".U2"))
+    vids, xyz, bc = mesh_tri(10, 10, lx=10.0, ly=10.0, jitter=0.1, seed=77)
+    c = xyz.mean(axis=1)
+    v, x, b, par = refine_tri(vids, xyz, bc, np.hypot(c[:, 0] - 5.0, c[:, 1] - 5.0) < 2.0)
+    np.testing.assert_allclose(M.reshape(-1, 3, 2), x, rtol=0, atol=1e-15)
+    o = po.Oracle(2, v, x, 2, po.LAW_EULER2D, po.FLUX_LF, gamma=1.4, farfield=(1.0, 1.0, 1.0, 3.5))
+    assert abs(dt - o.dt(U0, 0.1)) <= 1e-14 * dt
+    U = U0
+    for s in range(2):
+        U, _ = o.step(U, po.RK_TVD3, s * dt, dt)
+    assert np.abs(U2 - U).max() / np.abs(U).max() < 1e-12
