@@ -210,6 +210,11 @@ static dg_status hang_setup(dg_ctx* c, const TriSetup& S) {
   std::vector<int32_t> elem, rstart, rec;
   std::vector<double> ph;
   build_hang_tables(S, elem, rstart, rec, ph);
+  // the pass gives each element one lane per (face half, point): at most two
+  // halves on each of its three faces (1-irregular meshes)
+  for (size_t i = 0; i + 1 < rstart.size(); i++)
+    if (rstart[i + 1] - rstart[i] > HANG_MAXREC)
+      return fail(c, DG_ERR_UNSUPPORTED, "an element has more than %d faces of half an edge", HANG_MAXREC);
   dg_status s;
   if ((s = dalloc(c, &c->hang_elem, elem.size())) || (s = dalloc(c, &c->hang_rstart, rstart.size())) ||
       (s = dalloc(c, &c->hang_rec, rec.size())) || (s = dalloc(c, &c->phih, ph.size())))

@@ -1,5 +1,5 @@
-// The hanging-face pass (element functions in dg_hang.h): one work-item per
-// element with faces of half an edge.
+// The hanging-face pass (phases in dg_hang.h): one workgroup per element with
+// faces of half an edge.
 #include <hip/hip_runtime.h>
 
 #include "dg_hang.h"
@@ -7,14 +7,20 @@
 namespace dgp {
 
 template <int P, class LAW>
-__global__ __launch_bounds__(64) void tri_hang_kernel(HangArgs a) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i < a.n) hang_element<P, LAW>(a, i);
+__global__ __launch_bounds__(HANG_LANES) void tri_hang_kernel(HangArgs a) {
+  using K = HangBlock<P, LAW>;
+  __shared__ typename K::Smem sm;
+  const int i = blockIdx.x, lane = threadIdx.x;
+  K::hangA(a, i, lane, sm);
+  __syncthreads();
+  K::hangB(a, i, lane, sm);
+  __syncthreads();
+  K::hangC(a, i, lane, sm);
 }
 
 template <class LAW>
 static int launch_hang_law(const HangArgs& a, hipStream_t s) {
-  dim3 grid((a.n + 63) / 64), block(64);
+  dim3 grid(a.n), block(HANG_LANES);
   switch (a.P) {
     case 1: hipLaunchKernelGGL((tri_hang_kernel<1, LAW>), grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL((tri_hang_kernel<2, LAW>), grid, block, 0, s, a); break;

@@ -97,11 +97,13 @@ int launch_min_reduce(const double* part, int n, double* out, hipStream_t s);
 // *status |= 4 when any of U[0 .. n) is NaN or Inf (one atomic per wave that meets one)
 int launch_nonfinite_scan(const double* U, long n, unsigned* status, hipStream_t s);
 int launch_fill_one(double* out, double v, hipStream_t s);
-// Meshes with hanging nodes: after each stage kernel, one work-item per element
+// Meshes with hanging nodes: after each stage kernel, one workgroup per element
 // with faces of half an edge integrates them (both sides evaluated at the
 // half's Gauss points, the interior two-point flux, FP64) and adds the
 // contribution to the residual (MODE_RESIDUAL) or, through the stage update's
 // coefficient of dt R / |J|, to the stage output and the RK4 accumulator.
+constexpr int HANG_LANES = 64;   // work-items per element of the list
+constexpr int HANG_MAXREC = 6;   // face halves per element: two on each face
 struct HangArgs {
   int E, P, n;            // n: elements with faces of half an edge
   const int32_t* elem;    // [n] internal element

@@ -220,14 +220,25 @@ int make(Emu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int order, co
   return 0;
 }
 
+template <int P, class LAW>
+void hang_p(const HangArgs& h) {
+  using K = HangBlock<P, LAW>;
+  typename K::Smem sm;
+  for (int i = 0; i < h.n; i++) {
+    for (int l = 0; l < HANG_LANES; l++) K::hangA(h, i, l, sm);
+    for (int l = 0; l < HANG_LANES; l++) K::hangB(h, i, l, sm);
+    for (int l = 0; l < HANG_LANES; l++) K::hangC(h, i, l, sm);
+  }
+}
+
 template <class LAW>
 void hang_law(const HangArgs& h) {
-  for (int i = 0; i < h.n; i++) switch (h.P) {
-      case 1: hang_element<1, LAW>(h, i); break;
-      case 2: hang_element<2, LAW>(h, i); break;
-      case 3: hang_element<3, LAW>(h, i); break;
-      case 4: hang_element<4, LAW>(h, i); break;
-    }
+  switch (h.P) {
+    case 1: hang_p<1, LAW>(h); break;
+    case 2: hang_p<2, LAW>(h); break;
+    case 3: hang_p<3, LAW>(h); break;
+    case 4: hang_p<4, LAW>(h); break;
+  }
 }
 
 // the library's stage launch: the stage kernel, then (FP64, meshes with hanging
