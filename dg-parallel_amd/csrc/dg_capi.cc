@@ -123,7 +123,7 @@ static int max_chunks(int E) { return (E + MIX_CHUNK - 1) / MIX_CHUNK + MAXP_TRI
 static size_t part_size(const TriSetup& S) {
   const int E = S.E;
   int n = std::max(tri_stage_blocks(E), tri_tiled_blocks(E));
-  n = std::max(n, mixed_grid(max_chunks(E)));
+  n = std::max(n, max_chunks(E));  // the bucket-by-bucket mixed launches: one per chunk
   n = std::max(n, mixed_dt_blocks(E));
   return (size_t)n + 1;
 }
@@ -546,7 +546,24 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
     m.scheme = a.scheme;
     m.stage = a.stage;
     m.resid_part = a.resid_part;
-    r = launch_mixed_stage(m, c->law, c->mix_grid > 0 ? c->mix_grid : mixed_grid(c->nchunks), c->stream);
+    if (c->shape == 1) {
+      // bucket by bucket: orders 2..4 on the side streams, order 1 on the
+      // context stream, joined before anything that follows on it
+      // residual partial sums: order 4's tiles first, then 3, 2, 1
+      int part0[MAXP_TRI + 1] = {0};
+      for (int p = MAXP_TRI - 1; p >= 1; p--) part0[p] = part0[p + 1] + mixed_bucket_tiles(c->bstart, p + 1);
+      if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return -2;
+      for (int j = 0; j < 3; j++)
+        if (hipStreamWaitEvent(c->side[j], c->ev_fork, 0) != hipSuccess ||
+            launch_mixed_bucket(m, c->law, MAXP_TRI - j, part0[MAXP_TRI - j], c->side[j]) ||
+            hipEventRecord(c->ev_join[j], c->side[j]) != hipSuccess)
+          return -2;
+      r = launch_mixed_bucket(m, c->law, 1, part0[1], c->stream);
+      for (int j = 0; j < 3; j++)
+        if (hipStreamWaitEvent(c->stream, c->ev_join[j], 0) != hipSuccess) return -2;
+    } else {
+      r = launch_mixed_stage(m, c->law, c->mix_grid > 0 ? c->mix_grid : mixed_grid(c->nchunks), c->stream);
+    }
   } else {
     r = c->kernel == 1 ? launch_tri_stage(a, c->law, variant, c->stream)
                        : launch_tri_stage_tiled(a, c->law, variant | c->shape << 2, c->stream);
@@ -597,6 +614,11 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
 
 static int stage_blocks(const dg_ctx* c) {
   if (c->dim == 3) return tet_tiles(c->E);
+  if (c->mixed && c->shape == 1) {
+    int n = 0;
+    for (int p = 1; p <= MAXP_TRI; p++) n += mixed_bucket_tiles(c->bstart, p);
+    return n;
+  }
   if (c->mixed) return c->mix_grid > 0 ? c->mix_grid : mixed_grid(c->nchunks);
   return c->kernel == 1 ? tri_stage_blocks(c->E) : tri_tiled_blocks(c->E);
 }
@@ -1666,7 +1688,12 @@ static int tri_shapes(const dg_ctx* c) {
 static bool schedule_ok(const dg_ctx* c, const dg_schedule& s) {
   const bool tiled = c->dim == 3 || (!c->mixed && c->kernel == 0);
   const int nshape = c->dim == 3 ? TET_SHAPES : tri_shapes(c);
-  if (s.shape < 0 || s.shape >= nshape || (s.shape && !tiled)) return false;
+  // mixed orders: shape 1 = one launch per order bucket (no persistent grid)
+  if (c->mixed) {
+    if (s.shape < 0 || s.shape > 1 || (s.shape == 1 && (s.grid || !c->side[0]))) return false;
+  } else if (s.shape < 0 || s.shape >= nshape || (s.shape && !tiled)) {
+    return false;
+  }
   if (s.grid < 0 || (s.grid && !c->mixed)) return false;
   if (s.grid && ((size_t)s.grid + 1 > c->part_cap || s.grid > c->nchunks)) return false;
   return true;
@@ -1712,6 +1739,9 @@ static std::vector<dg_schedule> tune_candidates(const dg_ctx* c) {
       s.grid = g;
       if (schedule_ok(c, s)) v.push_back(s);
     }
+    dg_schedule sb{};
+    sb.shape = 1;  // bucket by bucket
+    if (schedule_ok(c, sb)) v.push_back(sb);
   } else {
     for (int sh = 0; sh < (c->dim == 3 ? TET_SHAPES : tri_shapes(c)); sh++) {
       dg_schedule s{};

@@ -509,6 +509,11 @@ int launch_remap(const RemapArgs& a, hipStream_t s);
 int launch_recover(const RecoverArgs& a, int law, int nf, hipStream_t s);
 int mixed_grid(int nchunks);
 int launch_mixed_stage(const MixedArgs& a, int law, int grid, hipStream_t s);
+// the bucket-by-bucket form: one launch over bucket p (1..4), one workgroup per
+// 64 of its elements, with order p's own registers and LDS; residual partial
+// sums at resid_part[part0 ..)
+inline int mixed_bucket_tiles(const int* bstart, int p) { return (bstart[p + 1] - bstart[p] + MIX_CHUNK - 1) / MIX_CHUNK; }
+int launch_mixed_bucket(const MixedArgs& a, int law, int p, int part0, hipStream_t s);
 // part receives one minimum per workgroup: mixed_dt_blocks(E) values
 int mixed_dt_blocks(int E);
 int launch_mixed_dt(const MixedArgs& a, int law, double cfl, double* part, hipStream_t s);

@@ -17,9 +17,9 @@ constexpr int MIX_BLOCK = 256;
 
 int mixed_grid(int nchunks) { return nchunks < 2048 ? nchunks : 2048; }
 
-template <int P, class LAW>
-__device__ __forceinline__ void mixed_tile(const MixedArgs& a, MixSmem<LAW>& s, int tid, int i0, double& rsum) {
-  using K = TiledMixed<P, LAW>;
+template <int P, class LAW, int PU, bool PF>
+__device__ __forceinline__ void mixed_tile(const MixedArgs& a, MixSmem<LAW, PU>& s, int tid, int i0, double& rsum) {
+  using K = TiledMixed<P, LAW, PU, PF>;
   typename K::Regs g;
   K::phaseA(a, s, tid, i0, g);
   __syncthreads();
@@ -29,7 +29,7 @@ __device__ __forceinline__ void mixed_tile(const MixedArgs& a, MixSmem<LAW>& s, 
     K::phaseD1(a, s, tid, q0, g);
     __syncthreads();
   }
-  K::phaseC1(a, s, tid, i0);
+  K::phaseC1(a, s, tid, i0, g);
   __syncthreads();
   K::phaseC2(a, s, tid);
   __syncthreads();
@@ -52,10 +52,10 @@ __global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_stage(MixedArgs a) {
   for (int w = lw; w < a.nchunks; w += nb) {
     const int p = a.chunks[2 * w], i0 = a.chunks[2 * w + 1];
     switch (p) {
-      case 1: mixed_tile<1, LAW>(a, sm, tid, i0, rsum); break;
-      case 2: mixed_tile<2, LAW>(a, sm, tid, i0, rsum); break;
-      case 3: mixed_tile<3, LAW>(a, sm, tid, i0, rsum); break;
-      default: mixed_tile<4, LAW>(a, sm, tid, i0, rsum); break;
+      case 1: mixed_tile<1, LAW, MAXP_TRI, false>(a, sm, tid, i0, rsum); break;
+      case 2: mixed_tile<2, LAW, MAXP_TRI, false>(a, sm, tid, i0, rsum); break;
+      case 3: mixed_tile<3, LAW, MAXP_TRI, false>(a, sm, tid, i0, rsum); break;
+      default: mixed_tile<4, LAW, MAXP_TRI, false>(a, sm, tid, i0, rsum); break;
     }
     __syncthreads();  // the next tile overwrites the LDS
   }
@@ -67,6 +67,30 @@ __global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_stage(MixedArgs a) {
       __syncthreads();
     }
     if (tid == 0) a.resid_part[blockIdx.x] = sm.red[0];
+  }
+}
+
+// One order bucket per launch: workgroup b takes the bucket's elements
+// [bstart[P] + 64 b, + 64); the LDS and registers are those of order P alone.
+// Residual partial sums at resid_part[part0 + b].
+template <class LAW, int P>
+__global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_stage_p(MixedArgs a, int part0) {
+  __shared__ MixSmem<LAW, P> sm;
+  const int tid = threadIdx.x;
+  double rsum = 0.0;
+  TiledMixed<P, LAW, P, true>::stage_tables(a, sm, tid);
+  __syncthreads();
+  const int i0 = a.bstart[P] + blockIdx.x * MIX_TILE;
+  mixed_tile<P, LAW, P, true>(a, sm, tid, i0, rsum);
+  if (a.resid_part) {
+    __syncthreads();
+    sm.red[tid] = rsum;
+    __syncthreads();
+    for (int s = MIX_BLOCK / 2; s > 0; s >>= 1) {
+      if (tid < s) sm.red[tid] += sm.red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) a.resid_part[part0 + blockIdx.x] = sm.red[0];
   }
 }
 
@@ -175,6 +199,28 @@ int launch_mixed_stage(const MixedArgs& a, int law, int ngrid, hipStream_t s) {
   else
     return -1;
   return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+template <class LAW>
+static int launch_mixed_bucket_t(const MixedArgs& a, int p, int part0, hipStream_t s) {
+  void (*kern[4])(MixedArgs, int) = {tri_mixed_stage_p<LAW, 1>, tri_mixed_stage_p<LAW, 2>, tri_mixed_stage_p<LAW, 3>,
+                                     tri_mixed_stage_p<LAW, 4>};
+  const int n = mixed_bucket_tiles(a.bstart, p);
+  if (n > 0) hipLaunchKernelGGL(kern[p - 1], dim3(n), dim3(MIX_BLOCK), 0, s, a, part0);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_mixed_bucket(const MixedArgs& a, int law, int p, int part0, hipStream_t s) {
+  if (p < 1 || p > MAXP_TRI) return -1;
+  if (law == LAW_SCALAR) return launch_mixed_bucket_t<ScalarLawD>(a, p, part0, s);
+  if (law == LAW_EULER2D && a.law.char_bc && a.law.flux == FLUX_EXACT)
+    return launch_mixed_bucket_t<EulerExactCharLawD>(a, p, part0, s);
+  if (law == LAW_EULER2D && a.law.char_bc) return launch_mixed_bucket_t<EulerCharLawD>(a, p, part0, s);
+  if (law == LAW_EULER2D && a.law.flux == FLUX_EXACT) return launch_mixed_bucket_t<EulerExactLawD>(a, p, part0, s);
+  if (law == LAW_EULER2D) return launch_mixed_bucket_t<EulerLawD>(a, p, part0, s);
+  if (law == LAW_MAXWELL2D) return launch_mixed_bucket_t<Maxwell2dLawD>(a, p, part0, s);
+  if (law == LAW_BURGERS2D) return launch_mixed_bucket_t<Burgers2DLawD>(a, p, part0, s);
+  return -1;
 }
 
 int mixed_dt_blocks(int E) { return (E + MIX_BLOCK - 1) / MIX_BLOCK; }

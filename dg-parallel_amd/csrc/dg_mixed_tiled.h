@@ -37,10 +37,12 @@ DG_HD bool wave_any(bool pred) {
 #endif
 }
 
-template <class LAW>
+// PU: the largest order of the tiles that share it (MAXP_TRI for the launch
+// over every bucket, P for a launch over bucket P alone)
+template <class LAW, int PU = MAXP_TRI>
 struct MixSmem {
   static constexpr int NF = LAW::NF;
-  static constexpr int UROWS = NF * np_tri(MAXP_TRI);
+  static constexpr int UROWS = NF * np_tri(PU);
   static constexpr int GW = MIX_QC * NF * 2;     // volume-flux rows of one pass
   static constexpr int FW = 3 * MIX_NQF * NF;    // face trace / flux rows
   static constexpr int WROWS = GW > FW ? GW : FW;
@@ -55,6 +57,7 @@ struct MixSmem {
   int Ls[3][MIX_TILE];
   int Nq[3][MIX_TILE];
   double Tn[MIX_NTAB];  // neighbour trace tables, staged once per workgroup
+  double Gw[GLW_OFF(MIX_NQF + 1)];  // Gauss-Legendre half weights of every face rule size
 };
 
 // bucket quantities of a per-lane order p (1..4) by select chains
@@ -72,12 +75,17 @@ DG_HD int trace_off(int p, int nq) {
   return nq == p + 1 ? TriBlock::off_phif(p) : TriBlock::off_phifx(p, nq);
 }
 
-template <int P, class LAW>
+// PF: C1's neighbour rows and the stage update's operands are loaded early (in
+// phase A and before the lift) at the cost of registers: the launches over one
+// bucket (PF = true) have them to spare, the persistent launch (every order's
+// code in one kernel) keeps its register budget
+template <int P, class LAW, int PU = MAXP_TRI, bool PF = false>
 struct TiledMixed {
   static constexpr int NP = np_tri(P), NQ = nqv_tri(P), NF = LAW::NF;
   static constexpr int T = MIX_TILE;
   static constexpr int NROW = NF * NP;
-  using Smem = MixSmem<LAW>;
+  static_assert(P <= PU, "the LDS rows hold the tile's order");
+  using Smem = MixSmem<LAW, PU>;
   // output rows of a wave as in the uniform kernel: whole fields for Euler
   // (f = wave), modes k = wave, wave+4, .. of the single scalar field
   static constexpr bool BY_FIELD = NF >= 4;
@@ -86,9 +94,14 @@ struct TiledMixed {
   static constexpr int QPP = MIX_QC / 4;             // volume points per wave per pass
   static constexpr int CPW = (3 * NF + 3) / 4;       // neighbour-trace tasks per wave
   static constexpr int FPPW = (3 * MIX_NQF + 3) / 4; // face-point items per wave
+  // the output rows, and C1's neighbour rows and links, loaded in phase A so
+  // their latency hides behind B and D1 (rows past the neighbour's order are
+  // loaded at an in-bounds index and discarded)
   struct Regs {
     double R[FPW * KPW];
     double rsum;
+    double nv[PF ? CPW : 1][MIX_MAXNP];
+    int lk[CPW];
   };
 
   DG_HD static int row_f(int wave, int fi) { return BY_FIELD ? wave + 4 * fi : fi; }
@@ -102,6 +115,20 @@ struct TiledMixed {
   // the neighbour trace tables into LDS (once per workgroup, before any tile)
   DG_HD static void stage_tables(const MixedArgs& a, Smem& s, int tid) {
     for (int j = tid; j < MIX_NTAB; j += TILE_THREADS) s.Tn[j] = a.ntab[j];
+    if (tid < GLW_OFF(MIX_NQF + 1)) s.Gw[tid] = a.glw[tid];
+  }
+  // the coefficients of field f of the neighbour across link (rows of bucket
+  // pn are stride doubles apart), every load issued at an in-bounds index
+  // before any is used: boundary faces read the own element is, rows past the
+  // neighbour's order repeat its first row (both discarded by C1)
+  DG_HD static void load_nb(const MixedArgs& a, int link, int is, int f, double* v) {
+    const int nb = link >= 0 ? link >> 3 : is;
+    const int pn = mixed_order_of(a, nb), npn = (pn + 1) * (pn + 2) / 2;
+    const int b0 = sel_bstart(a, pn), b1 = sel_bstart(a, pn + 1);
+    const long stride = b1 - b0;
+    const long base = sel_boff(a, pn) + (long)f * npn * stride + (nb - b0);
+#pragma unroll
+    for (int k = 0; k < MIX_MAXNP; k++) v[k] = a.Uin[base + (long)(k < npn ? k : 0) * stride];
   }
 
   // ---- A ------------------------------------------------------------------------
@@ -110,7 +137,11 @@ struct TiledMixed {
     const int E = a.E, i = i0 + t;
     const bool valid = i < a.bstart[P + 1];
     const int is = valid ? i : a.bstart[P];  // padding lanes load the bucket's first element and discard it
-    // every load is issued, unconditionally, before any is used
+    // every load is issued, unconditionally, before any is used: the links
+    // first (C1's addresses need them), the rows, |J| J^-1, then C1's rows
+    int lk3[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) lk3[l] = valid ? a.el.link[(long)l * E + is] : -BC_FARFIELD;
     constexpr int RA = (NROW + 3) / 4;
     double u[RA];
 #pragma unroll
@@ -121,7 +152,14 @@ struct TiledMixed {
     double m[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) m[r] = a.el.mg[(long)r * E + is];
-    const int lk = a.el.link[(long)(wave < 3 ? wave : 0) * E + is];
+#pragma unroll
+    for (int j = 0; j < CPW; j++) {
+      const int it = wave + 4 * j;
+      const int l = it < 3 * NF ? it / NF : 0, f = it < 3 * NF ? it - l * NF : 0;
+      const int link = lk3[l];
+      g.lk[j] = link;
+      if constexpr (PF) load_nb(a, link, is, f, g.nv[j]);
+    }
 #pragma unroll
     for (int j = 0; j < RA; j++) {
       const int r = wave + 4 * j;
@@ -133,7 +171,7 @@ struct TiledMixed {
     s.Ms[wave][t] = m[wave];
     if (wave < 3) {
       const int l = wave;
-      const int link = valid ? lk : -BC_FARFIELD;
+      const int link = lk3[l];
       s.Ls[l][t] = link;
       double mg[2][2];
       for (int r = 0; r < 2; r++)
@@ -206,43 +244,42 @@ struct TiledMixed {
   }
 
   // ---- C1: neighbour traces at the face points ------------------------------------
-  DG_HD static void phaseC1(const MixedArgs& a, Smem& s, int tid, int i0) {
+  DG_HD static void phaseC1(const MixedArgs& a, Smem& s, int tid, int i0, const Regs& g) {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
-    const bool valid = i0 + t < a.bstart[P + 1];
+    const int i = i0 + t;
+    const int is = i < a.bstart[P + 1] ? i : a.bstart[P];
+#pragma unroll
     for (int j = 0; j < CPW; j++) {
       const int it = wave + 4 * j;
       if (it >= 3 * NF) break;
       const int l = it / NF, f = it - l * NF;
-      const int link = s.Ls[l][t], nq = s.Nq[l][t];
-      double out[MIX_NQF];
+      const int link = g.lk[j], nq = s.Nq[l][t];
+      double vl[MIX_MAXNP];
+      if constexpr (!PF) load_nb(a, link, is, f, vl);
+      const double* v = PF ? g.nv[PF ? j : 0] : vl;
+      // branch-free over the lanes' neighbour orders, rule sizes, faces and
+      // orientations: every lane reads MIX_NQF x MIX_MAXNP table entries at
+      // in-bounds places (points past nq repeat the last, rows past the
+      // neighbour's order its first) and the products it does not need are
+      // zero, so the LDS reads issue back to back
+      const bool has = link >= 0;
+      const int nb = has ? link >> 3 : is, lb = has ? link & 3 : 0;
+      const bool same = (link >> 2) & 1;
+      const int pn = mixed_order_of(a, nb), npn = (pn + 1) * (pn + 2) / 2;
+      const double* tn = s.Tn + ntab_off(pn, nq) + lb * nq * npn;
+      double vz[MIX_MAXNP];
 #pragma unroll
-      for (int q = 0; q < MIX_NQF; q++) out[q] = 0.0;
-      if (valid && link >= 0) {
-        const int nb = link >> 3, lb = link & 3;
-        const bool same = (link >> 2) & 1;
-        const int pn = mixed_order_of(a, nb), npn = (pn + 1) * (pn + 2) / 2;
-        const int b0 = sel_bstart(a, pn), b1 = sel_bstart(a, pn + 1);
-        const long stride = b1 - b0;  // rows of bucket pn are stride doubles apart
-        const long base = sel_boff(a, pn) + (long)f * npn * stride + (nb - b0);
-        double v[MIX_MAXNP];
+      for (int k = 0; k < MIX_MAXNP; k++) vz[k] = has && k < npn ? v[k] : 0.0;
 #pragma unroll
-        for (int k = 0; k < MIX_MAXNP; k++) v[k] = k < npn ? a.Uin[base + (long)k * stride] : 0.0;
-        const double* tn = s.Tn + ntab_off(pn, nq) + lb * nq * npn;
+      for (int q = 0; q < MIX_NQF; q++) {
+        const int qv = q < nq ? q : nq - 1;
+        const int qq = same ? qv : nq - 1 - qv;
+        const double* row = tn + qq * npn;
+        double acc = 0.0;
 #pragma unroll
-        for (int q = 0; q < MIX_NQF; q++) {
-          if (q < nq) {
-            const int qq = same ? q : nq - 1 - q;
-            const double* row = tn + qq * npn;
-            double acc = 0.0;
-#pragma unroll
-            for (int k = 0; k < MIX_MAXNP; k++)
-              if (k < npn) acc += row[k] * v[k];
-            out[q] = acc;
-          }
-        }
+        for (int k = 0; k < MIX_MAXNP; k++) acc += row[k < npn ? k : 0] * vz[k];
+        s.W[(l * MIX_NQF + q) * NF + f][t] = q < nq ? acc : 0.0;
       }
-#pragma unroll
-      for (int q = 0; q < MIX_NQF; q++) s.W[(l * MIX_NQF + q) * NF + f][t] = out[q];
     }
   }
 
@@ -284,10 +321,33 @@ struct TiledMixed {
 #pragma unroll
         for (int f = 0; f < NF; f++) uR[f] = s.W[(l * MIX_NQF + q) * NF + f][t];
         face_flux<LAW>(a.law, link, n, uL, uR, Fh);
-        const double sw = s.Ns[l][2][t] * a.glw[GLW_OFF(nq) + q];
+        const double sw = s.Ns[l][2][t] * s.Gw[GLW_OFF(nq) + q];
 #pragma unroll
         for (int f = 0; f < NF; f++) s.W[(l * MIX_NQF + q) * NF + f][t] = Fh[f] * sw;
       }
+    }
+  }
+
+  // the lift of face l's fluxes at the NV points of the rule of that size, on
+  // the lanes whose face l has that size
+  template <int NV>
+  DG_HD static void lift_nv(const MixedArgs& a, const Smem& s, int t, int wave, int l, int f, int fi, int nq,
+                            Regs& g) {
+    static_assert(NV <= MIX_NQF, "face rule size");
+    if constexpr (NV >= P + 1) {
+      const bool mine = nq == NV;
+      if (!wave_any(mine)) return;
+      const double* to = block(a) + trace_off(P, NV) + l * NV * NP;
+      double Fq[NV];
+#pragma unroll
+      for (int q = 0; q < NV; q++) Fq[q] = mine ? s.W[(l * MIX_NQF + q) * NF + f][t] : 0.0;
+#pragma unroll
+      for (int q = 0; q < NV; q++)
+#pragma unroll
+        for (int ki = 0; ki < KPW; ki++) {
+          const int k = row_k(wave, ki);
+          if (k < NP) g.R[fi * KPW + ki] -= to[q * NP + k] * Fq[q];
+        }
     }
   }
 
@@ -296,52 +356,65 @@ struct TiledMixed {
     const int t = tid & (T - 1), wave = wave_uniform(tid >> 6);
     const int i = i0 + t;
     const bool valid = i < a.bstart[P + 1];
+    const int is = valid ? i : a.bstart[P];
     const double* tb = block(a);
+    const int sc = a.scheme * 4 + a.stage;
+    const bool residual_only = a.scheme == MODE_RESIDUAL;
+    const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
+    const bool reads_acc = sc == 3 * 4 + 1 || sc == 3 * 4 + 2 || sc == 3 * 4 + 3;
+    const double dtj = a.dt / a.el.detj[is];
+    g.rsum = 0.0;
 #pragma unroll
     for (int fi = 0; fi < FPW; fi++) {
       const int f = row_f(wave, fi);
       if (f >= NF) continue;
+      // PF: issue the stage update's loads first; the lift below hides them
+      double un[PF ? KPW : 1], ac[PF ? KPW : 1], us[PF ? KPW : 1];
+      if constexpr (PF) {
+#pragma unroll
+        for (int ki = 0; ki < KPW; ki++) {
+          const int k = row_k(wave, ki);
+          const long ix = idx(a, is, f * NP + (k < NP ? k : 0));
+          un[ki] = residual_only ? 0.0 : a.Un[ix];
+          ac[ki] = reads_acc ? a.acc[ix] : 0.0;
+          us[ki] = reads_input ? a.Uin[ix] : 0.0;
+        }
+      }
 #pragma unroll 1
       for (int l = 0; l < 3; l++) {
         const int nq = s.Nq[l][t];
-#pragma unroll 1
-        for (int nv = P + 1; nv <= MIX_NQF; nv++) {
-          const bool mine = nq == nv;
-          if (!wave_any(mine)) continue;
-          const double* to = tb + trace_off(P, nv) + l * nv * NP;
-          for (int q = 0; q < nv; q++) {
-            const double Fq = mine ? s.W[(l * MIX_NQF + q) * NF + f][t] : 0.0;
-#pragma unroll
-            for (int ki = 0; ki < KPW; ki++) {
-              const int k = row_k(wave, ki);
-              if (k < NP) g.R[fi * KPW + ki] -= to[q * NP + k] * Fq;
-            }
-          }
+        // one pass per rule size present in the wave, each with its points
+        // unrolled (the flux reads and table rows issue together)
+        switch (P + 1) {
+          case 2: lift_nv<2>(a, s, t, wave, l, f, fi, nq, g); [[fallthrough]];
+          case 3: lift_nv<3>(a, s, t, wave, l, f, fi, nq, g); [[fallthrough]];
+          case 4: lift_nv<4>(a, s, t, wave, l, f, fi, nq, g); [[fallthrough]];
+          default: lift_nv<5>(a, s, t, wave, l, f, fi, nq, g);
         }
       }
-    }
-    g.rsum = 0.0;
-    if (!valid) return;
-    const int sc = a.scheme * 4 + a.stage;
-    const bool reads_input = sc == 2 * 4 + 1 || sc == 2 * 4 + 2 || sc == 1 * 4 + 1;
-    const bool reads_acc = sc == 3 * 4 + 1 || sc == 3 * 4 + 2 || sc == 3 * 4 + 3;
-    const double dtj = a.dt / a.el.detj[i];
-#pragma unroll
-    for (int fi = 0; fi < FPW; fi++) {
-      const int f = row_f(wave, fi);
-      if (f >= NF) continue;
+      if (!valid) continue;
 #pragma unroll
       for (int ki = 0; ki < KPW; ki++) {
         const int k = row_k(wave, ki);
         if (k >= NP) continue;
         const long ix = idx(a, i, f * NP + k);
         const double r = g.R[fi * KPW + ki];
-        if (a.scheme == MODE_RESIDUAL) {
+        if (residual_only) {
           a.R[ix] = r;
           continue;
         }
-        double accv = reads_acc ? a.acc[ix] : 0.0;
-        const double o = stage_value_hd(sc, r * dtj, a.Un[ix], reads_input ? a.Uin[ix] : 0.0, &accv, g.rsum);
+        const int kp = PF ? ki : 0;
+        double accv, unv, usv;
+        if constexpr (PF) {
+          accv = ac[kp];
+          unv = un[kp];
+          usv = us[kp];
+        } else {
+          accv = reads_acc ? a.acc[ix] : 0.0;
+          unv = a.Un[ix];
+          usv = reads_input ? a.Uin[ix] : 0.0;
+        }
+        const double o = stage_value_hd(sc, r * dtj, unv, usv, &accv, g.rsum);
         if (sc >= 3 * 4 && sc < 3 * 4 + 3) a.acc[ix] = accv;
         a.Uout[ix] = o;
       }

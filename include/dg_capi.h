@@ -322,8 +322,11 @@ dg_status dg_adjoint_timing(const dg_ctx* ctx, float* fwd_ms, float* bwd_ms);
  *          contraction and the face lift on the matrix cores
  *          (v_mfma_f32_16x16x4_f32): its sums run in the matrix cores' order,
  *          so the state follows shape 0 to FP32 rounding, not bit for bit;
- *   grid:  the persistent mixed-order launch, workgroups walking the chunk list
- *          (0: one per chunk, at most 2048). */
+ *          Mixed orders: 0 = one persistent launch over every order bucket,
+ *          1 = one launch per order bucket (each with its order's registers
+ *          and LDS), the buckets side by side on the context's side streams;
+ *   grid:  the persistent mixed-order launch (shape 0), workgroups walking the
+ *          chunk list (0: one per chunk, at most 2048). */
 typedef struct {
   int shape;
   int grid;

@@ -594,17 +594,19 @@ int mix_make(MixEmu& m, const dg_mesh_desc* mesh, const dg_law_desc* law, int or
   return 0;
 }
 
-// one tile of the persistent mixed-order kernel, the phases in the kernel's order
-template <int P, class LAW>
-void mix_tile(const MixedArgs& a, MixSmem<LAW>& sm, int i0, double& rsum) {
-  using K = TiledMixed<P, LAW>;
+// one tile of the mixed-order kernels, the phases in the kernels' order; PU is
+// the largest order the LDS rows hold (MAXP_TRI: the persistent launch over
+// every bucket; P: the launch over bucket P alone)
+template <int P, class LAW, int PU, bool PF>
+void mix_tile(const MixedArgs& a, MixSmem<LAW, PU>& sm, int i0, double& rsum) {
+  using K = TiledMixed<P, LAW, PU, PF>;
   std::vector<typename K::Regs> g(TILE_THREADS);
   for (int t = 0; t < TILE_THREADS; t++) K::phaseA(a, sm, t, i0, g[t]);
   for (int q0 = 0; q0 < K::NQ; q0 += MIX_QC) {
     for (int t = 0; t < TILE_THREADS; t++) K::phaseB(a, sm, t, q0);
     for (int t = 0; t < TILE_THREADS; t++) K::phaseD1(a, sm, t, q0, g[t]);
   }
-  for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, sm, t, i0);
+  for (int t = 0; t < TILE_THREADS; t++) K::phaseC1(a, sm, t, i0, g[t]);
   for (int t = 0; t < TILE_THREADS; t++) K::phaseC2(a, sm, t);
   for (int t = 0; t < TILE_THREADS; t++) {
     K::phaseD2(a, sm, t, i0, g[t]);
@@ -612,21 +614,35 @@ void mix_tile(const MixedArgs& a, MixSmem<LAW>& sm, int i0, double& rsum) {
   }
 }
 
-// every chunk of 64 elements of every bucket, as the persistent launch walks them
+template <int P, class LAW, int PU, bool PF>
+void mix_tiles_of(const MixedArgs& a, double& rsum) {
+  std::unique_ptr<MixSmem<LAW, PU>> sm(new MixSmem<LAW, PU>());
+  for (int i0 = a.bstart[P]; i0 < a.bstart[P + 1]; i0 += MIX_CHUNK) {
+    std::memset(sm.get(), 0, sizeof(MixSmem<LAW, PU>));
+    for (int t = 0; t < TILE_THREADS; t++) TiledMixed<P, LAW, PU, PF>::stage_tables(a, *sm, t);
+    mix_tile<P, LAW, PU, PF>(a, *sm, i0, rsum);
+  }
+}
+
+// 0: the tiles as the persistent launch runs them (LDS rows of the top order);
+// 1: as the bucket-by-bucket launches run them (each bucket's own LDS rows)
+static int g_mix_mode = 0;
+extern "C" void emu_mixed_mode(int mode) { g_mix_mode = mode; }
+
+// every chunk of 64 elements of every bucket
 template <class LAW>
 void mix_stage_all(const MixedArgs& a, double& rsum) {
-  std::unique_ptr<MixSmem<LAW>> sm(new MixSmem<LAW>());
-  for (int p = MAXP_TRI; p >= 1; p--)
-    for (int i0 = a.bstart[p]; i0 < a.bstart[p + 1]; i0 += MIX_CHUNK) {
-      std::memset(sm.get(), 0, sizeof(MixSmem<LAW>));
-      for (int t = 0; t < TILE_THREADS; t++) TiledMixed<1, LAW>::stage_tables(a, *sm, t);
-      switch (p) {
-        case 1: mix_tile<1, LAW>(a, *sm, i0, rsum); break;
-        case 2: mix_tile<2, LAW>(a, *sm, i0, rsum); break;
-        case 3: mix_tile<3, LAW>(a, *sm, i0, rsum); break;
-        default: mix_tile<4, LAW>(a, *sm, i0, rsum); break;
-      }
-    }
+  if (g_mix_mode == 1) {
+    mix_tiles_of<4, LAW, 4, true>(a, rsum);
+    mix_tiles_of<3, LAW, 3, true>(a, rsum);
+    mix_tiles_of<2, LAW, 2, true>(a, rsum);
+    mix_tiles_of<1, LAW, 1, true>(a, rsum);
+  } else {
+    mix_tiles_of<4, LAW, MAXP_TRI, false>(a, rsum);
+    mix_tiles_of<3, LAW, MAXP_TRI, false>(a, rsum);
+    mix_tiles_of<2, LAW, MAXP_TRI, false>(a, rsum);
+    mix_tiles_of<1, LAW, MAXP_TRI, false>(a, rsum);
+  }
 }
 
 void mix_stage(const MixedArgs& a, int law, double& rsum) {

@@ -4,7 +4,7 @@
   of the tetrahedron kernel (P1-P3) against the oracle within 1e-12;
 - a schedule changes no state: RK steps under every candidate schedule are bit
   for bit those of the default schedule (triangles, tetrahedra, the persistent
-  mixed-order launch); the step residual norm, a sum over workgroups and their
+  mixed-order launch and its bucket-by-bucket form); the step residual norm, a sum over workgroups and their
   threads, agrees within 1e-14 (its summation order follows the launch);
 - the tuner restores the state it timed on, picks one of its candidates, and a
   second context of the same configuration reads the choice back from the cache
@@ -131,6 +131,9 @@ def test_mixed_grids_give_identical_steps():
     for grid in (1, 3, 7):
         Ug, rg = steps_under(g, U, dt, po.RK_TVD3, 3, grid=grid)
         assert np.array_equal(Ug, ref) and abs(rg - r0) <= 1e-14 * abs(r0), grid
+    # one launch per order bucket, the buckets side by side on the side streams
+    Ug, rg = steps_under(g, U, dt, po.RK_TVD3, 3, shape=1)
+    assert np.array_equal(Ug, ref) and abs(rg - r0) <= 1e-14 * abs(r0)
 
 
 def test_autotune_restores_state_and_uses_cache(tmp_path):
@@ -177,8 +180,9 @@ def test_autotune_mixed_and_limiter_flags_restored():
     U = g.get_state()
     dt = 0.5 * g.computeDt(0.1)
     rep = g.autotune(po.RK_TVD3, dt, steps=2)
-    grids = [c[1] for c in rep["candidates"]]
-    assert grids[0] == 0 and len(grids) >= 2 and all(c[0] == 0 for c in rep["candidates"])
+    cand = [tuple(c[:2]) for c in rep["candidates"]]
+    assert cand[0] == (0, 0) and cand[-1] == (1, 0) and len(cand) >= 3
+    assert all(c[0] == 0 for c in cand[:-1])
     assert np.array_equal(g.get_state(), U)
     assert np.array_equal(g.limiter_flags(), flags)
 
@@ -193,7 +197,9 @@ def test_bad_schedules_refused():
         g.set_schedule(shape=2)
     gm = mixed_ctx()
     with pytest.raises(dgpar.DGError):
-        gm.set_schedule(shape=1)          # the mixed kernel has one shape
+        gm.set_schedule(shape=2)          # mixed orders: persistent (0) or bucket by bucket (1)
+    with pytest.raises(dgpar.DGError):
+        gm.set_schedule(shape=1, grid=4)  # the bucket launches have no persistent grid
     with pytest.raises(dgpar.DGError):
         gm.set_schedule(grid=10 ** 6)     # more workgroups than chunks
     assert g.schedule() == (0, 0) and gm.schedule() == (0, 0)

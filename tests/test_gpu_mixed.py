@@ -1,4 +1,5 @@
-"""GPU: mixed orders P1-P4 (one persistent launch over order buckets), the
+"""GPU: mixed orders P1-P4 (one persistent launch over order buckets, or one
+launch per bucket side by side on side streams: schedule shape 1), the
 triangle limiter and the order sensor, through the C ABI, against the oracle.
 
 - mixed-order residual, dt and limited SSP-RK3 / RK4 steps within 1e-12
@@ -76,7 +77,11 @@ def test_mixed_residual_and_dt(flux, seed):
     vids, xyz, bc, orders, o, g = tube(seed, flux=flux)
     U = smooth_state(orders, seed)
     g.set_state(U)
-    assert rel(g.residual(), o.residual(U)[0]) < 1e-12
+    R = o.residual(U)[0]
+    assert rel(g.residual(), R) < 1e-12
+    g.set_schedule(shape=1)  # one launch per order bucket
+    assert rel(g.residual(), R) < 1e-12
+    g.set_schedule(shape=0)
     assert abs(g.computeDt(0.1) - o.dt(U, 0.1)) <= 1e-13 * o.dt(U, 0.1)  # vertex sums of up to 15 modes
 
 
@@ -90,6 +95,8 @@ def test_mixed_scalar_residual():
                          dgpar.Law(dgpar.LAW_SCALAR, dgpar.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.2,)), 1)
     U = np.random.default_rng(3).standard_normal(o.ndof)
     g.set_state(U)
+    assert rel(g.residual(), o.residual(U)[0]) < 1e-12
+    g.set_schedule(shape=1)
     assert rel(g.residual(), o.residual(U)[0]) < 1e-12
 
 
@@ -109,11 +116,13 @@ def test_limiter_flags_bit_exact(seed, M, nu):
     assert rel(g.get_state(), Uo) < 1e-14
 
 
-@pytest.mark.parametrize("seed,scheme,use_graph", [(7, po.RK_TVD3, True), (7, po.RK_TVD3, False),
-                                                   (None, po.RK_TVD3, True), (9, po.RK4, True)])
-def test_limited_steps(seed, scheme, use_graph):
-    announce("limited steps seed %s scheme %d graph %d" % (seed, scheme, use_graph))
+@pytest.mark.parametrize("seed,scheme,use_graph,shape", [(7, po.RK_TVD3, True, 0), (7, po.RK_TVD3, False, 0),
+                                                         (None, po.RK_TVD3, True, 0), (9, po.RK4, True, 0),
+                                                         (7, po.RK_TVD3, True, 1), (9, po.RK4, False, 1)])
+def test_limited_steps(seed, scheme, use_graph, shape):
+    announce("limited steps seed %s scheme %d graph %d shape %d" % (seed, scheme, use_graph, shape))
     vids, xyz, bc, orders, o, g = tube(seed, p=2, use_graph=use_graph)
+    g.set_schedule(shape=shape)
     U, _ = o.limit(o.project(2, (0.53,)))
     dt = o.dt(U, 0.1)
     o.set_limiter(True)

@@ -184,9 +184,13 @@ def test_emulated_mixed_residual(flux, seed):
 
 
 @pytestmark_emu
-def test_emulated_mixed_residual_many_tiles():
+@pytest.mark.parametrize("mode", [0, 1])
+def test_emulated_mixed_residual_many_tiles(mode):
     """about 180 elements per order bucket: whole tiles of 64 and a partial last
-    tile in every bucket of the persistent launch"""
+    tile in every bucket; mode 0 the tiles of the persistent launch (LDS rows of
+    the top order), mode 1 those of the bucket-by-bucket launches (each bucket's
+    own LDS rows)"""
+    LIBEMU.emu_mixed_mode(mode)
     vids, xyz, bc, orders, o, _ = sod_case(3, nx=36, ny=10)
     assert all(np.bincount(orders, minlength=5)[1:] > 128)
     rng = np.random.default_rng(3)
@@ -196,7 +200,11 @@ def test_emulated_mixed_residual_many_tiles():
     mesh, lw = descs(vids, xyz, bc, orders, po.FLUX_ROE_EF)
     md, ld = mesh.desc(), lw.desc()
     R = np.zeros_like(U)
-    assert LIBEMU.emu_mixed_residual(ctypes.byref(md), ctypes.byref(ld), U.ctypes.data_as(PD), R.ctypes.data_as(PD)) == 0
+    try:
+        assert LIBEMU.emu_mixed_residual(ctypes.byref(md), ctypes.byref(ld), U.ctypes.data_as(PD),
+                                         R.ctypes.data_as(PD)) == 0
+    finally:
+        LIBEMU.emu_mixed_mode(0)
     Ro, _ = o.residual(U)
     assert np.abs(R - Ro).max() <= 1e-12 * np.abs(Ro).max()
 
