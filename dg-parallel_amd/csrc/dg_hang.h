@@ -16,7 +16,7 @@ namespace dgp {
 
 // the coefficient of v = dt R / |J| in the stage output and in the RK4
 // accumulator of each stage update (dg_stage_tiled.h stage_value_hd)
-DG_HD inline void stage_coeffs(int scheme, int stage, double& cout, double& cacc) {
+DG_HD void stage_coeffs(int scheme, int stage, double& cout, double& cacc) {
   cacc = 0.0;
   switch (scheme * 4 + stage) {
     case 3 * 4 + 0: cout = 0.5; cacc = 1.0; break;

@@ -1,6 +1,7 @@
-// CPU emulation of the tiled triangle stage kernel and of the residual-derivative
-// kernels: runs the kernels' own phase functions (dg-parallel_amd/csrc/
-// dg_stage_tiled.h, dg_adjoint.h) thread by thread on the host,
+// CPU emulation of the tiled triangle stage kernel, the element-per-work-item
+// stage kernel and the residual-derivative kernels: runs the kernels' own phase
+// and element functions (dg-parallel_amd/csrc/dg_stage_tiled.h, dg_element.h,
+// dg_adjoint.h) thread by thread on the host,
 // with the same host-built tables the library uploads.  Test infrastructure:
 // lets the CPU test suite check the kernel arithmetic and indexing against the
 // oracle without a GPU.
@@ -10,6 +11,7 @@
 #include <vector>
 
 #include "../../dg-parallel_amd/csrc/dg_adjoint.h"
+#include "../../dg-parallel_amd/csrc/dg_element.h"
 #include "../../dg-parallel_amd/csrc/dg_hang.h"
 #include "../../dg-parallel_amd/csrc/dg_mixed.h"
 #include "../../dg-parallel_amd/csrc/dg_mixed_tiled.h"
@@ -289,6 +291,47 @@ extern "C" int emu_tri_residual(const dg_mesh_desc* mesh, const dg_law_desc* law
   a.R = m.R.data();
   a.scheme = MODE_RESIDUAL;
   if (run_stage(m, a, (fp32 ? 1 : 0) | shape << 2)) return -2;
+  m.from_soa(m.R.data(), R);
+  return 0;
+}
+
+template <int P, class LAW>
+void element_residual(const StageArgs& a) {
+  double rsum = 0.0;
+  for (int e = 0; e < a.E; e++) element_stage<P, LAW, double, double, double>(a, e, rsum);
+}
+
+template <class LAW>
+int element_residual_law(const StageArgs& a) {
+  switch (a.P) {
+    case 1: element_residual<1, LAW>(a); return 0;
+    case 2: element_residual<2, LAW>(a); return 0;
+    case 3: element_residual<3, LAW>(a); return 0;
+    case 4: element_residual<4, LAW>(a); return 0;
+  }
+  return -1;
+}
+
+// the FP64 residual of the element-per-work-item kernel (every law, including
+// those the library runs on the tiled kernels only)
+extern "C" int emu_tri_residual_element(const dg_mesh_desc* mesh, const dg_law_desc* law, int order,
+                                        const double* U, double* R) {
+  Emu m;
+  if (make(m, mesh, law, order)) return -1;
+  m.to_soa(U);
+  StageArgs a = m.base();
+  a.Uin = m.Un.data();
+  a.R = m.R.data();
+  a.scheme = MODE_RESIDUAL;
+  const int lw = m.S.law;
+  int r;
+  if (lw == LAW_SCALAR) r = element_residual_law<ScalarLawD>(a);
+  else if (lw == LAW_EULER2D && a.law.flux == FLUX_EXACT) r = element_residual_law<EulerExactLawD>(a);
+  else if (lw == LAW_EULER2D) r = element_residual_law<EulerLawD>(a);
+  else if (lw == LAW_MAXWELL2D) r = element_residual_law<Maxwell2dLawD>(a);
+  else if (lw == LAW_BURGERS2D) r = element_residual_law<Burgers2DLawD>(a);
+  else return -3;
+  if (r) return -2;
   m.from_soa(m.R.data(), R);
   return 0;
 }

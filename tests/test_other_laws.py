@@ -52,6 +52,22 @@ def test_emulated_residual(law, periodic, p):
     assert rel(emu_residual(mesh, lw, p, U, fp32=1), Ro) < 5e-5   # FP32 form
 
 
+@pytest.mark.parametrize("law", ["maxwell2d", "burgers"])
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_emulated_element_kernel_residual(law, periodic, p):
+    """the element-per-work-item kernel's code (dg_element.h), run on the CPU:
+    the library refuses this kernel for these laws on the GPU (Maxwell2d gave
+    wrong values there at P3 and P4), while the same source on the host matches
+    the oracle"""
+    mesh, lw, o, U = case(law, p, periodic)
+    md, ld = mesh.desc(), lw.desc()
+    R = np.zeros_like(U)
+    assert LIBEMU.emu_tri_residual_element(ctypes.byref(md), ctypes.byref(ld), p, U.ctypes.data_as(PD),
+                                           R.ctypes.data_as(PD)) == 0
+    assert rel(R, o.residual(U)[0]) < 1e-12
+
+
 @pytest.mark.parametrize("law,p,scheme", [("maxwell2d", 3, po.RK4), ("burgers", 2, po.RK_TVD3)])
 def test_emulated_steps(law, p, scheme):
     mesh, lw, o, U = case(law, p, periodic=False)
