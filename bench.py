@@ -660,8 +660,11 @@ def main():
     X = mesh.xyz
     detj = np.abs((X[:, 1, 0] - X[:, 0, 0]) * (X[:, 2, 1] - X[:, 0, 1]) -
                   (X[:, 2, 0] - X[:, 0, 0]) * (X[:, 1, 1] - X[:, 0, 1]))
-    d = (r32["state"] - r64["state"]).reshape(E, 4, np_)[:, 0, :]
-    l2_rho_diff = float(np.sqrt((d ** 2 * detj[:, None]).sum()))
+    # per-field L2 of FP32 - FP64: the orthonormal modal basis on the reference
+    # triangle makes int_K u^2 = |J| sum_k c_k^2
+    dd = (r32["state"] - r64["state"]).reshape(E, 4, np_)
+    l2_diff = [float(np.sqrt((dd[:, f, :] ** 2 * detj[:, None]).sum())) for f in range(4)]
+    l2_rho_diff = l2_diff[0]
 
     def thr(r):
         return ndof * 4 * args.steps / (r["ms"] / 1e3)
@@ -698,6 +701,10 @@ def main():
         "ratio_fp32_over_fp64": v32 / v64,
         "l2_rho_vs_exact": {"fp32": float(r32["err"][0]), "fp64": float(r64["err"][0])},
         "l2_rho_fp32_vs_fp64": l2_rho_diff,
+        "l2_per_field": {"fields": ["rho", "rho u", "rho v", "rho E"],
+                         "fp32_vs_exact": [float(x) for x in r32["err"]],
+                         "fp64_vs_exact": [float(x) for x in r64["err"]],
+                         "fp32_vs_fp64": l2_diff},
         "schedule": {"fp32": r32["schedule"], "fp64": r64["schedule"]},
         "wall_s_timed": r32["wall"],
     }
