@@ -325,6 +325,8 @@ dg_status dg_adjoint_timing(const dg_ctx* ctx, float* fwd_ms, float* bwd_ms);
  *          Mixed orders: 0 = one persistent launch over every order bucket,
  *          1 = one launch per order bucket (each with its order's registers
  *          and LDS), the buckets side by side on the context's side streams;
+ *          compiled apart from shape 0, its states follow shape 0 to FP64
+ *          rounding, not bit for bit;
  *   grid:  the persistent mixed-order launch (shape 0), workgroups walking the
  *          chunk list (0: one per chunk, at most 2048). */
 typedef struct {

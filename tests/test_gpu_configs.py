@@ -187,15 +187,15 @@ def test_c3_tube_limited_step_flags_and_orders_vs_oracle():
         assert abs(g.computeDt(0.1) - dt) <= 1e-13 * dt
         o.set_limiter(True)
         Uo, ro = o.step(U, po.RK_TVD3, 0.0, dt, nthreads=THREADS)
-        # the bucket-by-bucket launches (schedule shape 1) give the persistent
-        # launch's state bit for bit
+        # the bucket-by-bucket launches (schedule shape 1, compiled apart from the
+        # persistent launch) give its state to FP64 rounding
         g.set_schedule(shape=1)
         r1 = g.advanceInTime(0.0, dt, scheme=dgpar.RK_TVD3, nsteps=1)
         U1 = g.get_state()
         g.set_state(U)
         g.set_schedule(shape=0)
         r = g.advanceInTime(0.0, dt, scheme=dgpar.RK_TVD3, nsteps=1)
-        assert np.array_equal(U1, g.get_state()) and abs(r1 - r) <= 1e-14 * abs(r)
+        assert rel(U1, g.get_state()) < 1e-14 and abs(r1 - r) <= 1e-12 * abs(r)
         es = rel(g.get_state(), Uo)
         # limiter flags and sensor orders on one state, bit for bit
         Ug = g.get_state()
