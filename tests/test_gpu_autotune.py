@@ -135,7 +135,7 @@ def test_mixed_grids_give_identical_steps():
     # one launch per order bucket, the buckets side by side on the side streams:
     # compiled apart from the persistent launch, so equal to FP64 rounding
     Ug, rg = steps_under(g, U, dt, po.RK_TVD3, 3, shape=1)
-    assert rel(Ug, ref) < 1e-14 and abs(rg - r0) <= 1e-12 * abs(r0)
+    assert rel(Ug, ref) < 1e-12 and abs(rg - r0) <= 1e-12 * abs(r0)
 
 
 def test_autotune_restores_state_and_uses_cache(tmp_path):

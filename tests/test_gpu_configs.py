@@ -195,7 +195,7 @@ def test_c3_tube_limited_step_flags_and_orders_vs_oracle():
         g.set_state(U)
         g.set_schedule(shape=0)
         r = g.advanceInTime(0.0, dt, scheme=dgpar.RK_TVD3, nsteps=1)
-        assert rel(U1, g.get_state()) < 1e-14 and abs(r1 - r) <= 1e-12 * abs(r)
+        assert rel(U1, g.get_state()) < 1e-12 and abs(r1 - r) <= 1e-12 * abs(r)
         es = rel(g.get_state(), Uo)
         # limiter flags and sensor orders on one state, bit for bit
         Ug = g.get_state()
