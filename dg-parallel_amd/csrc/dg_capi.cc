@@ -318,6 +318,9 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
       (s = dalloc(c, &c->tabf, h_tabf.size())) || (s = dalloc(c, &c->vtx, h_vtx.size())) ||
       (s = dalloc(c, &c->part, c->part_cap = part_size(S))))
     return bail(s);
+  if (c->mixed && ((s = dalloc(c, &c->mg2, (size_t)4 * E)) || (s = dalloc(c, &c->detj2, E)) ||
+                   (s = dalloc(c, &c->size2, E))))
+    return bail(s);
   if (c->prec == DG_PREC_FP64) {
     if ((s = dalloc(c, &c->A64, ns)) || (s = dalloc(c, &c->B64, ns))) return bail(s);
   } else {
@@ -1170,7 +1173,7 @@ dg_status dg_get_orders(dg_ctx* c, int32_t* orders_out) {
 // layout does not fit.
 static bool rebucket_in_place(dg_ctx* c, const std::vector<int32_t>& neword, dg_status& st) {
   st = DG_OK;
-  if (!c->mixed || c->topo.E != c->E || !c->remap) return false;
+  if (!c->mixed || c->topo.E != c->E || !c->remap || !c->mg2) return false;
   bool mixed = false;
   for (int e = 1; e < c->E; e++)
     if (neword[e] != neword[0]) mixed = true;
@@ -1180,7 +1183,7 @@ static bool rebucket_in_place(dg_ctx* c, const std::vector<int32_t>& neword, dg_
   L.E = E;
   L.NF = c->Nf;
   const std::vector<int> ord(neword.begin(), neword.end());
-  tri_layout(c->topo, ord.data(), L);
+  tri_layout(c->topo, ord.data(), L, nullptr, false);  // the geometry moves on the device
   const std::vector<int> chunks = mixed_chunk_list(L.bstart, L.perm);
   if ((size_t)L.boff[5] > c->state_cap || (int)chunks.size() / 2 > c->chunk_cap) return false;
   // new internal element i was old internal element iold[i]
@@ -1199,19 +1202,26 @@ static bool rebucket_in_place(dg_ctx* c, const std::vector<int32_t>& neword, dg_
   r.iold = c->remap;
   r.src = c->Un;
   r.dst = c->A64;
+  r.mg0 = c->mg;
+  r.detj0 = c->detj;
+  r.size0 = c->size;
+  r.mg1 = c->mg2;
+  r.detj1 = c->detj2;
+  r.size1 = c->size2;
   auto up = [&](void* d, const void* h, size_t bytes) {
     return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) == hipSuccess;
   };
   if (!up(c->remap, iold.data(), (size_t)E * 4) || launch_remap(r, c->stream) ||
-      !up(c->mg, L.mg.data(), L.mg.size() * 8) || !up(c->detj, L.detj.data(), (size_t)E * 8) ||
-      !up(c->size, L.size.data(), (size_t)E * 8) || !up(c->link, L.link.data(), L.link.size() * 4) ||
-      !up(c->chunks, chunks.data(), chunks.size() * 4) ||
+      !up(c->link, L.link.data(), L.link.size() * 4) || !up(c->chunks, chunks.data(), chunks.size() * 4) ||
       hipMemsetAsync(c->flags, 0, (size_t)E, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
     st = fail(c, DG_ERR_HIP, "rebucketing failed: %s", hipGetErrorString(hipGetLastError()));
     return true;
   }
   std::swap(c->Un, c->A64);
+  std::swap(c->mg, c->mg2);
+  std::swap(c->detj, c->detj2);
+  std::swap(c->size, c->size2);
   c->perm = std::move(L.perm);
   c->order = std::move(L.order);
   c->coff = std::move(L.coff);

@@ -66,6 +66,9 @@ struct dg_ctx {
   double *Un = nullptr, *acc = nullptr, *A64 = nullptr, *B64 = nullptr, *R = nullptr;
   float *A32 = nullptr, *B32 = nullptr;
   double *mg = nullptr, *detj = nullptr, *size = nullptr, *tab = nullptr, *vtx = nullptr, *part = nullptr;
+  // mixed orders: second geometry buffers, the targets of re-bucketing's device
+  // gather (swapped with mg, detj, size after it)
+  double *mg2 = nullptr, *detj2 = nullptr, *size2 = nullptr;
   float* tabf = nullptr;
   int32_t* link = nullptr;
   double *mtab = nullptr, *glw = nullptr, *mvtx = nullptr, *ntab = nullptr;

@@ -413,6 +413,10 @@ struct RemapArgs {
   const int32_t* iold;
   const double* src;
   double* dst;
+  // the element geometry carried to the new internal numbering on the device
+  // (null: not moved): |J| J^-1 [4][E], |J| [E], size [E]
+  const double *mg0, *detj0, *size0;
+  double *mg1, *detj1, *size1;
 };
 
 // ---------------------------------------------------------------------------

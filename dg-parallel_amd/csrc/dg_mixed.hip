@@ -170,6 +170,11 @@ __global__ __launch_bounds__(256) void tri_remap(RemapArgs a) {
   const long d = o1 + (i - s1), r = o0 + (j - s0);
   for (int f = 0; f < a.NF; f++)
     for (int k = 0; k < np1; k++) a.dst[d + (long)(f * np1 + k) * n1] = k < np0 ? a.src[r + (long)(f * np0 + k) * n0] : 0.0;
+  if (a.mg1) {
+    for (int q = 0; q < 4; q++) a.mg1[(long)q * a.E + i] = a.mg0[(long)q * a.E + j];
+    a.detj1[i] = a.detj0[j];
+    a.size1[i] = a.size0[j];
+  }
 }
 
 int launch_remap(const RemapArgs& a, hipStream_t s) {

@@ -8,6 +8,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <stdexcept>
+#include <thread>
+#include <vector>
 
 #include "dg_tables.h"
 
@@ -213,35 +215,81 @@ double tri_cell_size(const double* X) {
   return std::sqrt(hp * (hp - a) * (hp - b) * (hp - cc)) / hp * 2.;
 }
 
-void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* grp) {
+// f(i0, i1) over [0, n) split among up to 8 host threads (one thread below
+// 16384 items); every range is handled by exactly one call
+template <class F>
+static void par_for(int n, F f) {
+  const unsigned hc = std::thread::hardware_concurrency();
+  const int nt = n < 16384 ? 1 : (int)std::min(8u, hc ? hc : 1u);
+  if (nt <= 1) {
+    f(0, n);
+    return;
+  }
+  const int chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) {
+    const int i0 = t * chunk, i1 = std::min(n, i0 + chunk);
+    if (i0 < i1) th.emplace_back(f, i0, i1);
+  }
+  f(0, std::min(n, chunk));
+  for (auto& x : th) x.join();
+}
+
+void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* grp, bool geom) {
   const int E = T.E, NF = S.NF;
   S.mixed = false;
-  for (int e = 1; e < E; e++)
-    if (ord[e] != ord[0]) S.mixed = true;
+  for (int e = 1; e < E && !S.mixed; e++) S.mixed = ord[e] != ord[0];
   S.P = S.mixed ? 0 : ord[0];
   S.NP = S.mixed ? 0 : np_tri(ord[0]);
-  S.coff.assign(E + 1, 0);
+  S.coff.resize(E + 1);
+  S.coff[0] = 0;
   for (int e = 0; e < E; e++) S.coff[e + 1] = S.coff[e] + (long)NF * np_tri(ord[e]);
   // order buckets, blocks of BLK caller elements, signatures: each block's
-  // elements of one order are few, so a stable sort per block is cheap
+  // elements of one order are few, so a stable sort per block is cheap.  The
+  // blocks' places in each bucket come from per-block counts, so the blocks
+  // are placed independently (in parallel).
   const int BLK = 2 * 64;
-  int cnt[6] = {0, 0, 0, 0, 0, 0};
-  for (int e = 0; e < E; e++) cnt[ord[e]]++;
-  int at[6] = {0, 0, 0, 0, 0, 0};  // next free slot of each order's run in perm
-  for (int p = 2; p <= MAXP_TRI; p++) at[p] = at[p - 1] + cnt[p - 1];
-  S.perm.resize(E);
-  for (int b0 = 0; b0 < E; b0 += BLK) {
-    int run[6];  // where this block's elements of each order begin
-    for (int p = 1; p <= MAXP_TRI; p++) run[p] = at[p];
-    const int b1 = std::min(E, b0 + BLK);
-    for (int e = b0; e < b1; e++) {
-      // stable insertion by signature into this block's elements of order ord[e]
-      const int p = ord[e], sg = T.sig[e];
-      int k = at[p]++;
-      for (; k > run[p] && T.sig[S.perm[k - 1]] > sg; k--) S.perm[k] = S.perm[k - 1];
-      S.perm[k] = e;
+  const int nblk = (E + BLK - 1) / BLK;
+  std::vector<std::array<int, MAXP_TRI + 1>> bcnt(nblk);
+  par_for(nblk, [&](int k0, int k1) {
+    for (int k = k0; k < k1; k++) {
+      bcnt[k].fill(0);
+      for (int e = k * BLK; e < std::min(E, (k + 1) * BLK); e++) bcnt[k][ord[e]]++;
     }
+  });
+  int cnt[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < nblk; k++)
+    for (int p = 1; p <= MAXP_TRI; p++) cnt[p] += bcnt[k][p];
+  {
+    // bcnt becomes each block's first slot in each order's run
+    int at[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = 2; p <= MAXP_TRI; p++) at[p] = at[p - 1] + cnt[p - 1];
+    for (int k = 0; k < nblk; k++)
+      for (int p = 1; p <= MAXP_TRI; p++) {
+        const int c = bcnt[k][p];
+        bcnt[k][p] = at[p];
+        at[p] += c;
+      }
   }
+  S.perm.resize(E);
+  par_for(nblk, [&](int k0, int k1) {
+    int el[MAXP_TRI + 1][BLK], sgs[MAXP_TRI + 1][BLK];
+    for (int k = k0; k < k1; k++) {
+      int n[MAXP_TRI + 1] = {0};
+      for (int e = k * BLK; e < std::min(E, (k + 1) * BLK); e++) {
+        // stable insertion by signature into this block's elements of order ord[e]
+        const int p = ord[e], sg = T.sig[e];
+        int i = n[p]++;
+        for (; i > 0 && sgs[p][i - 1] > sg; i--) {
+          sgs[p][i] = sgs[p][i - 1];
+          el[p][i] = el[p][i - 1];
+        }
+        sgs[p][i] = sg;
+        el[p][i] = e;
+      }
+      for (int p = 1; p <= MAXP_TRI; p++) std::copy(el[p], el[p] + n[p], S.perm.begin() + bcnt[k][p]);
+    }
+  });
   if (grp && !S.mixed) {
     // one order: groups first, then the same blocks and signatures
     for (int e = 0; e < E; e++) S.perm[e] = e;
@@ -252,9 +300,13 @@ void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* gr
     });
   }
   std::vector<int32_t> iperm(E);
-  for (int i = 0; i < E; i++) iperm[S.perm[i]] = i;
   S.order.resize(E);
-  for (int i = 0; i < E; i++) S.order[i] = (int8_t)ord[S.perm[i]];
+  par_for(E, [&](int i0, int i1) {
+    for (int i = i0; i < i1; i++) {
+      iperm[S.perm[i]] = i;
+      S.order[i] = (int8_t)ord[S.perm[i]];
+    }
+  });
   S.bstart[0] = 0;
   S.boff[0] = 0;
   S.bstart[1] = 0;
@@ -263,21 +315,27 @@ void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* gr
     S.bstart[p + 1] = S.bstart[p] + cnt[p];
     S.boff[p + 1] = S.boff[p] + (long)NF * np_tri(p) * cnt[p];
   }
-  S.mg.resize((size_t)4 * E);
-  S.detj.resize(E);
-  S.size.resize(E);
-  S.link.resize((size_t)3 * E);
-  for (int i = 0; i < E; i++) {
-    const int e = S.perm[i];
-    for (int r = 0; r < 4; r++) S.mg[(size_t)r * E + i] = T.mg[(size_t)e * 4 + r];
-    S.detj[i] = T.detj[e];
-    S.size[i] = T.size[e];
-    for (int l = 0; l < 3; l++) {
-      const int nb = T.nbe[e * 3 + l];
-      S.link[(size_t)l * E + i] = nb >= 0 ? (iperm[nb] << 3) | (T.same[e * 3 + l] << 2) | T.nbl[e * 3 + l]
-                                           : -(int32_t)T.bcode[e * 3 + l];
-    }
+  if (geom) {
+    S.mg.resize((size_t)4 * E);
+    S.detj.resize(E);
+    S.size.resize(E);
   }
+  S.link.resize((size_t)3 * E);
+  par_for(E, [&](int i0, int i1) {
+    for (int i = i0; i < i1; i++) {
+      const int e = S.perm[i];
+      if (geom) {
+        for (int r = 0; r < 4; r++) S.mg[(size_t)r * E + i] = T.mg[(size_t)e * 4 + r];
+        S.detj[i] = T.detj[e];
+        S.size[i] = T.size[e];
+      }
+      for (int l = 0; l < 3; l++) {
+        const int nb = T.nbe[e * 3 + l];
+        S.link[(size_t)l * E + i] = nb >= 0 ? (iperm[nb] << 3) | (T.same[e * 3 + l] << 2) | T.nbl[e * 3 + l]
+                                             : -(int32_t)T.bcode[e * 3 + l];
+      }
+    }
+  });
 }
 
 void build_hang_tables(const TriSetup& S, std::vector<int32_t>& elem, std::vector<int32_t>& rstart,

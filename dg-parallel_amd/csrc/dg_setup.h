@@ -86,8 +86,9 @@ dg_status build_tri_setup(const dg_mesh_desc* m, const dg_law_desc* law, const d
 // then link signature (stable), so a wavefront takes one path and face
 // neighbours sit in nearby tiles.  grp (caller numbering, uniform orders only):
 // a group key ahead of the blocks (the classes of local time stepping), so each
-// group is one internal range.
-void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* grp = nullptr);
+// group is one internal range.  geom = false leaves S.mg, S.detj and S.size
+// unset (a caller that moves the geometry itself, as re-bucketing on the device).
+void tri_layout(const TriTopo& T, const int* ord, TriSetup& S, const int32_t* grp = nullptr, bool geom = true);
 
 // The hanging-face pass's tables for S (one order): the owning elements of faces
 // of half an edge (internal numbering), records [4] per face and side (own
