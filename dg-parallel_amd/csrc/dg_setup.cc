@@ -216,7 +216,9 @@ double tri_cell_size(const double* X) {
 }
 
 // f(i0, i1) over [0, n) split among up to 8 host threads (one thread below
-// 16384 items); every range is handled by exactly one call
+// 16384 items); every range is handled by exactly one call.  A thread that
+// cannot be started leaves its range, and the ones after it, to the caller's
+// thread.
 template <class F>
 static void par_for(int n, F f) {
   const unsigned hc = std::thread::hardware_concurrency();
@@ -227,11 +229,21 @@ static void par_for(int n, F f) {
   }
   const int chunk = (n + nt - 1) / nt;
   std::vector<std::thread> th;
-  for (int t = 1; t < nt; t++) {
+  int t = 1;
+  for (; t < nt; t++) {
     const int i0 = t * chunk, i1 = std::min(n, i0 + chunk);
-    if (i0 < i1) th.emplace_back(f, i0, i1);
+    if (i0 >= i1) break;
+    try {
+      th.emplace_back(f, i0, i1);
+    } catch (...) {
+      break;
+    }
   }
   f(0, std::min(n, chunk));
+  for (; t < nt; t++) {
+    const int i0 = t * chunk, i1 = std::min(n, i0 + chunk);
+    if (i0 < i1) f(i0, i1);
+  }
   for (auto& x : th) x.join();
 }
 
