@@ -31,6 +31,8 @@ def laws(law):
     if law == "maxwell2d":
         return (dgpar.Maxwell2dLaw(farfield=(0.1, 0.0, 0.2)),
                 dict(law=po.LAW_MAXWELL2D, flux=po.FLUX_UPWIND, farfield=(0.1, 0.0, 0.2)))
+    if law == "burgers":
+        return (dgpar.Burgers2DLaw((0.4,)), dict(law=po.LAW_BURGERS2D, flux=po.FLUX_CENTRAL, farfield=(0.4,)))
     fl = {"lf": (dgpar.FLUX_LF, po.FLUX_LF), "roe": (dgpar.FLUX_ROE, po.FLUX_ROE),
           "exact": (dgpar.FLUX_EXACT, po.FLUX_EXACT)}[law]
     return (dgpar.Law(dgpar.LAW_EULER2D, fl[0], gamma=1.4, farfield=FF),
@@ -47,6 +49,8 @@ def state(o, law, seed=3):
         U = o.project_fn(lambda X, Y, Z, t: (np.exp(-((X - 5) ** 2 + (Y - 5) ** 2) / 4.0),))
     elif law == "maxwell2d":
         U = o.project_fn(lambda X, Y, Z, t: (np.sin(0.6 * Y), np.cos(0.5 * X), np.sin(0.4 * (X + Y))))
+    elif law == "burgers":
+        U = o.project_fn(lambda X, Y, Z, t: (0.5 + 0.3 * np.sin(0.6 * X) * np.cos(0.4 * Y),))
     else:
         U = o.project(1, VORTEX)
     return U + 0.002 * np.random.default_rng(seed).standard_normal(U.shape)
@@ -54,7 +58,8 @@ def state(o, law, seed=3):
 
 @pytest.mark.parametrize("law,p,periodic", [("scalar", 1, True), ("scalar", 3, False), ("lf", 1, False),
                                              ("lf", 3, True), ("roe", 2, True), ("roe", 4, False),
-                                             ("exact", 2, True), ("maxwell2d", 3, False)])
+                                             ("exact", 2, True), ("maxwell2d", 3, False), ("burgers", 2, True),
+                                             ("burgers", 3, False)])
 def test_residual_matches_oracle(law, p, periodic):
     print("hanging residual", law, p, periodic, flush=True)
     v, x, b, _ = split_mesh(periodic=periodic)

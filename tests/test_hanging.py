@@ -128,6 +128,14 @@ def hang_case(law, p, periodic=True):
         o = po.Oracle(2, v, x, p, po.LAW_SCALAR, po.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.2,), bc=b)
         lw = dgpar.Law(dgpar.LAW_SCALAR, dgpar.FLUX_UPWIND, vel=(1.0, 0.5), farfield=(0.2,))
         U = o.project_fn(lambda X, Y, Z, t: (np.exp(-((X - 5) ** 2 + (Y - 5) ** 2) / 4.0),))
+    elif law == "maxwell2d":
+        o = po.Oracle(2, v, x, p, po.LAW_MAXWELL2D, po.FLUX_UPWIND, farfield=(0.1, 0.0, 0.2), bc=b)
+        lw = dgpar.Maxwell2dLaw(farfield=(0.1, 0.0, 0.2))
+        U = o.project_fn(lambda X, Y, Z, t: (np.sin(0.6 * Y), np.cos(0.5 * X), np.sin(0.4 * (X + Y))))
+    elif law == "burgers":
+        o = po.Oracle(2, v, x, p, po.LAW_BURGERS2D, po.FLUX_CENTRAL, farfield=(0.4,), bc=b)
+        lw = dgpar.Burgers2DLaw((0.4,))
+        U = o.project_fn(lambda X, Y, Z, t: (0.5 + 0.3 * np.sin(0.6 * X) * np.cos(0.4 * Y),))
     else:
         fl = {"lf": (po.FLUX_LF, dgpar.FLUX_LF), "roe": (po.FLUX_ROE, dgpar.FLUX_ROE)}[law]
         o = po.Oracle(2, v, x, p, po.LAW_EULER2D, fl[0], gamma=1.4, farfield=ff, bc=b)
@@ -138,7 +146,9 @@ def hang_case(law, p, periodic=True):
 
 
 @pytest.mark.parametrize("law,p,periodic", [("scalar", 1, True), ("scalar", 3, False), ("lf", 1, True),
-                                             ("lf", 2, False), ("roe", 3, True), ("roe", 4, True)])
+                                             ("lf", 2, False), ("roe", 3, True), ("roe", 4, True),
+                                             ("maxwell2d", 2, False), ("maxwell2d", 4, True), ("burgers", 3, True),
+                                             ("burgers", 2, False)])
 def test_emulated_residual_matches_oracle(law, p, periodic):
     lib, dgpar, ctypes = _emu()
     mesh, lw, o, U = hang_case(law, p, periodic)
