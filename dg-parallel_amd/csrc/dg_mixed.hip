@@ -70,7 +70,7 @@ __global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_stage(MixedArgs a) {
   }
 }
 
-// One order bucket per launch: workgroup b takes the bucket's elements
+// One order bucket per launch: tile b takes the bucket's elements
 // [bstart[P] + 64 b, + 64); the LDS and registers are those of order P alone.
 // Residual partial sums at resid_part[part0 + b].
 template <class LAW, int P>
@@ -80,7 +80,11 @@ __global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_stage_p(MixedArgs a, int 
   double rsum = 0.0;
   TiledMixed<P, LAW, P, true>::stage_tables(a, sm, tid);
   __syncthreads();
-  const int i0 = a.bstart[P] + blockIdx.x * MIX_TILE;
+  // consecutive tiles to one XCD (the hardware deals workgroups to the 8 XCDs
+  // in turn), so the neighbour rows its tiles gather stay in that XCD's L2
+  const int nb = gridDim.x, full = (nb / 8) * 8, bx = blockIdx.x;
+  const int b = bx >= full ? bx : (bx % 8) * (nb / 8) + bx / 8;
+  const int i0 = a.bstart[P] + b * MIX_TILE;
   mixed_tile<P, LAW, P, true>(a, sm, tid, i0, rsum);
   if (a.resid_part) {
     __syncthreads();
@@ -90,7 +94,7 @@ __global__ __launch_bounds__(MIX_BLOCK) void tri_mixed_stage_p(MixedArgs a, int 
       if (tid < s) sm.red[tid] += sm.red[tid + s];
       __syncthreads();
     }
-    if (tid == 0) a.resid_part[part0 + blockIdx.x] = sm.red[0];
+    if (tid == 0) a.resid_part[part0 + b] = sm.red[0];
   }
 }
 
