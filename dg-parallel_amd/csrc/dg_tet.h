@@ -371,9 +371,11 @@ struct TiledTet {
 };
 
 // shape: 0 for 4 waves per tile, 1 for 8 (traces over the coefficient rows);
-// 2 for 8, 3 for 16 (coefficient rows kept).  prec: 0 FP64; 1..3 the FP32 form
-// (1: FP64 input, FP32 output; 2: FP32 both; 3: FP32 input, FP64 output).
-constexpr int TET_SHAPES = 4;
+// 2 for 8, 3 for 16, 4 for 4 (coefficient rows kept: no global re-reads of
+// the own rows; FP32 LDS 49 KB at P3, three workgroups per CU).  prec: 0 FP64;
+// 1..3 the FP32 form (1: FP64 input, FP32 output; 2: FP32 both; 3: FP32 input,
+// FP64 output).
+constexpr int TET_SHAPES = 5;
 int launch_tet_stage(const TetArgs& a, int shape, int prec, hipStream_t s);
 int launch_tet_stage32(const TetArgs& a, int shape, int prec, hipStream_t s);
 int tet_tiles(int E);

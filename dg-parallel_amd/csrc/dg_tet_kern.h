@@ -66,6 +66,7 @@ static int launch_tet_shape(const TetArgs& a, int shape, hipStream_t s) {
     case 1: return launch_tet_form<8, true, CT, IT, OT>(a, s);
     case 2: return launch_tet_form<8, false, CT, IT, OT>(a, s);
     case 3: return launch_tet_form<16, false, CT, IT, OT>(a, s);
+    case 4: return launch_tet_form<4, false, CT, IT, OT>(a, s);
   }
   return -1;
 }

@@ -322,6 +322,9 @@ dg_status dg_adjoint_timing(const dg_ctx* ctx, float* fwd_ms, float* bwd_ms);
  *          contraction and the face lift on the matrix cores
  *          (v_mfma_f32_16x16x4_f32): its sums run in the matrix cores' order,
  *          so the state follows shape 0 to FP32 rounding, not bit for bit;
+ *          Tetrahedra: 0 / 1 = 4 / 8 waves with the face traces over the
+ *          coefficient rows in LDS, 2 / 3 / 4 = 8 / 16 / 4 waves with the rows
+ *          kept (own traces from LDS, more LDS per tile);
  *          Mixed orders: 0 = one persistent launch over every order bucket,
  *          1 = one launch per order bucket (each with its order's registers
  *          and LDS), the buckets side by side on the context's side streams;

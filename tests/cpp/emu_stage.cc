@@ -893,6 +893,7 @@ int tet_run32_io(const TetArgs& a, int shape, double* rsum) {
     case 1: return tet_run_nw_form<8, true, float, IT, OT>(a, rsum);
     case 2: return tet_run_nw_form<8, false, float, IT, OT>(a, rsum);
     case 3: return tet_run_nw_form<16, false, float, IT, OT>(a, rsum);
+    case 4: return tet_run_nw_form<4, false, float, IT, OT>(a, rsum);
   }
   return -1;
 }
@@ -912,6 +913,7 @@ int tet_run(const TetArgs& a, int shape, double* rsum) {
     case 1: return tet_run_nw<8, true>(a, rsum);
     case 2: return tet_run_nw<8, false>(a, rsum);
     case 3: return tet_run_nw<16, false>(a, rsum);
+    case 4: return tet_run_nw<4, false>(a, rsum);
   }
   return -1;
 }

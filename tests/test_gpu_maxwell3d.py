@@ -73,7 +73,7 @@ def test_tet_rejects_what_it_does_not_build():
         g.set_limiter(True)
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("p", [1, 2, 3])
 def test_tet_fp32_form(p, shape):
     """the FP32 form on tetrahedra (FP32 arithmetic, tables and stage copies;

@@ -105,7 +105,7 @@ emu = pytest.mark.skipif(LIBEMU is None, reason="emulator not built")
 
 
 @emu
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("flux", [po.FLUX_UPWIND, po.FLUX_CENTRAL])
 @pytest.mark.parametrize("p", [1, 2, 3])
 def test_emulated_tet_residual(flux, p, shape):
@@ -123,7 +123,7 @@ def test_emulated_tet_residual(flux, p, shape):
 
 
 @emu
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 def test_emulated_tet_rk4_steps(shape):
     p = 3
     vids, xyz, o = oracle(2, p, po.FLUX_UPWIND)
@@ -149,7 +149,7 @@ if LIBEMU is not None:
 
 
 @emu
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("p", [1, 2, 3])
 def test_emulated_tet_fp32_form_rk4_steps(p, shape):
     """the FP32 form (FP32 arithmetic, tables and stage copies; FP64 step-start
