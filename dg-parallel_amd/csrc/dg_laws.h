@@ -215,19 +215,19 @@ struct EulerLawD {
     T roe[4] = {T(0), T(0), T(0), T(0)};
     if (qq > T(0)) {
       T c_A = dg_sqrt((gamma - T(1)) * qq);
+      T inv_c_A = T(1) / c_A;
       T vn = u_A * n[0] + v_A * n[1];
       T eg0 = dg_abs(vn), eg2 = dg_abs(vn + c_A), eg3 = dg_abs(vn - c_A);
       if (L.flux == FLUX_ROE_EF) {
         // Harten's entropy fix (Harten & Hyman 1983), delta = 0.2 c: a new option;
         // the reference's Roe flux has none, and its zero dissipation of the
         // linear waves on faces with vn ~ 0 lets a shock-tube run blow up
-        const T dl = T(0.2) * c_A, inv2 = T(0.5) / dl;
+        const T dl = T(0.2) * c_A, inv2 = T(2.5) * inv_c_A;  // 1 / (2 delta)
         if (eg0 < dl) eg0 = (eg0 * eg0 + dl * dl) * inv2;
         if (eg2 < dl) eg2 = (eg2 * eg2 + dl * dl) * inv2;
         if (eg3 < dl) eg3 = (eg3 * eg3 + dl * dl) * inv2;
       }
-      T k = T(0.5) * rho_A / c_A;
-      T inv_c_A = T(1) / c_A;
+      T k = T(0.5) * rho_A * inv_c_A;
       T drho = rho_1 - rho_0, dp = p_1 - p_0, du = u_1 - u_0, dv = v_1 - v_0;
       T ch0 = drho - dp * inv_c_A * inv_c_A;
       T ch1 = n[1] * du - n[0] * dv;
@@ -241,12 +241,13 @@ struct EulerLawD {
       roe[3] = a0 * (T(0.5) * (u_A * u_A + v_A * v_A)) + a1 * (rho_A * (u_A * n[1] - v_A * n[0])) +
                a2 * (k * (H_A + c_A * vn)) + a3 * (k * (H_A - c_A * vn));
     }
-    T f1[4][2], f2[4][2];
-    flux(L, u0, f1);
-    flux(L, u1, f2);
+    // F(u).n of both states from the velocities and pressures above:
+    // (rho vn, rho u vn + p nx, rho v vn + p ny, (E + p) vn)
+    const T vn0 = u_0 * n[0] + v_0 * n[1], vn1 = u_1 * n[0] + v_1 * n[1];
+    const T fn[4] = {u0[0] * vn0 + u1[0] * vn1, u0[1] * vn0 + u1[1] * vn1 + (p_0 + p_1) * n[0],
+                     u0[2] * vn0 + u1[2] * vn1 + (p_0 + p_1) * n[1], (u0[3] + p_0) * vn0 + (u1[3] + p_1) * vn1};
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-      flux_out[i] = T(0.5) * ((f1[i][0] + f2[i][0]) * n[0] + (f1[i][1] + f2[i][1]) * n[1] - roe[i]);
+    for (int i = 0; i < 4; i++) flux_out[i] = T(0.5) * (fn[i] - roe[i]);
   }
   // Exact Riemann problem in the face frame (u normal, v tangential velocity),
   // sampled at x/t = 0, in the steps of Euler::riemann (EulerLaw.cc:602-851):
