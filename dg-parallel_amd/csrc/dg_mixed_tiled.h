@@ -1,8 +1,10 @@
-// Tiled mixed-order (P1-P4) triangle stage: one persistent launch whose work
-// items are tiles of 64 elements of one order bucket (dg_mixed.hip walks the
-// chunk list, heaviest order first).  Per tile of order P, 256 threads, lane =
-// element, the four waves share rows (field, mode) or points as the uniform
-// kernel does (dg_stage_tiled.h):
+// Tiled mixed-order (P1-P4) triangle stage.  Work items are tiles of 64
+// elements of one order bucket, run either by one persistent launch over every
+// bucket (dg_mixed.hip walks the chunk list) or by one launch per bucket (LDS
+// rows and registers of that order; PF: C1's rows and the stage update's
+// operands loaded early).  Per tile of order P, 256 threads, lane = element,
+// the four waves share rows (field, mode) or points as the uniform kernel does
+// (dg_stage_tiled.h):
 //   A   the tile's coefficients into LDS (rows of 64 consecutive doubles of
 //       bucket P), |J| J^-1, face normals and lengths, links, and per face the
 //       rule size nq = max(P, P_nb) + 1 (boundary faces: P + 1; DGCell.cc:408-423);
@@ -10,13 +12,15 @@
 //       the basis gradients into the register rows (DGCell.cc:184-249);
 //   C1  neighbour traces at the nq points of each face in the neighbour's own
 //       basis (its order, local face and orientation vary per lane: the trace
-//       rows are read per lane from the tables staged in LDS);
+//       rows are read per lane from the tables staged in LDS, every lane over
+//       all points and rows at in-bounds places, without branches);
 //   C2  own traces and the numerical flux times length and weight, written over
 //       the neighbour trace of that point (DGCell.cc:639-732);
 //   D2  lift into the rows and the RK stage update.
 // C2's own traces and D2's lift read the order-P trace table of rule size nq in
 // wave-uniform passes over the sizes present in the wave (scalar loads, lanes of
-// other sizes idle), since nq takes at most MAXP_TRI + 1 - P values.
+// other sizes idle; D2's passes with their points unrolled), since nq takes at
+// most MAXP_TRI + 1 - P values.
 // Per-lane choices among the buckets (neighbour order) go through select chains,
 // never through a per-lane index into the argument block.
 #pragma once
