@@ -6,7 +6,9 @@
 // order first, and each workgroup walks the chunk list with a stride of the
 // grid size.  A chunk never spans buckets, so the order switch is uniform per
 // workgroup.  All workgroups reach the end of the list; the residual norm is
-// reduced per workgroup in a fixed order.
+// reduced per workgroup in a fixed order.  tri_mixed_stage_p<LAW, P> is the
+// other form: one launch per bucket (schedule shape 1), one tile per
+// workgroup, with order P's own LDS rows and registers.
 #include <hip/hip_runtime.h>
 
 #include "dg_mixed_tiled.h"
