@@ -720,6 +720,12 @@ def main():
         out["maxwell3d_tets"] = run_maxwell(dgpar, args.maxwell_steps, tune=tune)
     if args.lts_steps > 0:
         out["local_time_stepping"] = run_lts(dgpar, args.lts_steps)
+        # the same mesh with at most three levels: the finest cells join level
+        # 2, so the step h halves, and the step holds 8 stage launches, not 16
+        r3 = run_lts(dgpar, 2 * args.lts_steps, levels=3)
+        out["local_time_stepping"]["three_levels"] = {
+            k: r3[k] for k in ("levels_histogram", "h", "lts_steps", "lts_ms_per_step", "simulated_time",
+                               "global_steps", "global_ms_per_step", "speedup_vs_global_ssp_rk2", "l2_rho_vs_exact")}
     if args.hadapt_steps > 0:
         out["h_adaptivity"] = run_hadapt(dgpar, args.hadapt_steps)
     if not args.no_cpu:
