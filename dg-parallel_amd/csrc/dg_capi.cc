@@ -54,9 +54,17 @@ static void release(dg_ctx* c, void* p) {
     }
 }
 
-static void free_all(dg_ctx* c) {
+// the captured step graphs hold the context's buffers, launch schedule and
+// stage options: dropped whenever one of them changes
+static void drop_graphs(dg_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  c->gexec = nullptr;
+  if (c->gexec1) (void)hipGraphExecDestroy(c->gexec1);
+  c->gexec = c->gexec1 = nullptr;
+  c->g_scheme = -1;
+}
+
+static void free_all(dg_ctx* c) {
+  drop_graphs(c);
   for (int i = 0; i < 3; i++) {
     if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
     if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
@@ -705,8 +713,7 @@ dg_status dg_set_velocity_field(dg_ctx* c, dg_field_fn fn, void* user) {
     if (s) return s;
   }
   HIPCHECK(c, hipMemcpy(c->pvel, h.data(), h.size() * 8, hipMemcpyHostToDevice));
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured steps change
-  c->gexec = nullptr;
+  drop_graphs(c);  // captured steps change
   return DG_OK;
 }
 
@@ -810,23 +817,43 @@ static dg_status check_scheme(dg_ctx* c, int scheme) {
   return DG_OK;
 }
 
-// the captured step graph of (scheme, dt), captured, instantiated and uploaded
-// to the device when missing; advances nothing
-static dg_status ensure_graph(dg_ctx* c, int scheme, double dt) {
-  if (c->gexec && c->g_scheme == scheme && c->g_dt == dt) return DG_OK;
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  c->gexec = nullptr;
+// Steps captured in one graph: one graph launch per GRAPH_STEPS steps, so the
+// launch gap between graphs (a few microseconds) comes once per GRAPH_STEPS
+// steps; the remainder runs the one-step graph.
+constexpr int GRAPH_STEPS = 8;
+
+// (scheme, dt)'s steps captured into one graph, instantiated and uploaded to
+// the device; advances nothing
+static dg_status capture_steps(dg_ctx* c, int scheme, double dt, int n, hipGraphExec_t* out) {
   hipGraph_t g;
   HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_step(c, scheme, dt, false);
+  int rc = 0;
+  for (int i = 0; i < n && !rc; i++) rc = enqueue_step(c, scheme, dt, false);
   hipError_t ec = hipStreamEndCapture(c->stream, &g);
   if (rc || ec != hipSuccess) return fail(c, DG_ERR_HIP, "graph capture failed");
-  hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
-  if (ei != hipSuccess) return fail(c, DG_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+  if (ei != hipSuccess) {
+    *out = nullptr;
+    return fail(c, DG_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+  }
+  HIPCHECK(c, hipGraphUpload(*out, c->stream));
+  return DG_OK;
+}
+
+// the step graphs of (scheme, dt) (GRAPH_STEPS steps, and one step), built
+// when missing
+static dg_status ensure_graph(dg_ctx* c, int scheme, double dt) {
+  if (c->gexec && c->gexec1 && c->g_scheme == scheme && c->g_dt == dt) return DG_OK;
+  drop_graphs(c);
+  dg_status s = capture_steps(c, scheme, dt, GRAPH_STEPS, &c->gexec);
+  if (s) return s;
+  if ((s = capture_steps(c, scheme, dt, 1, &c->gexec1))) {
+    drop_graphs(c);
+    return s;
+  }
   c->g_scheme = scheme;
   c->g_dt = dt;
-  HIPCHECK(c, hipGraphUpload(c->gexec, c->stream));
   return DG_OK;
 }
 
@@ -835,7 +862,8 @@ static dg_status run_steps(dg_ctx* c, int scheme, double dt, int nsteps) {
   if (c->use_graph && scheme != DG_RK_FE) {
     dg_status s = ensure_graph(c, scheme, dt);
     if (s) return s;
-    for (int i = 0; i < nsteps; i++) HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
+    for (int i = 0; i < nsteps / GRAPH_STEPS; i++) HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
+    for (int i = 0; i < nsteps % GRAPH_STEPS; i++) HIPCHECK(c, hipGraphLaunch(c->gexec1, c->stream));
   } else {
     for (int s = 0; s < nsteps; s++)
       if (enqueue_step(c, scheme, dt, false)) return fail(c, DG_ERR_HIP, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -1094,8 +1122,7 @@ dg_status dg_set_limiter(dg_ctx* c, int enabled, double tvb_m, double nu) {
   c->lim_on = enabled ? 1 : 0;
   c->lim_m = tvb_m;
   c->lim_nu = nu;
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured steps change
-  c->gexec = nullptr;
+  drop_graphs(c);  // captured steps change
   return DG_OK;
 }
 
@@ -1114,8 +1141,7 @@ dg_status dg_set_recovery(dg_ctx* c, int enabled) {
   if (enabled && c->prec != DG_PREC_FP64) return fail(c, DG_ERR_UNSUPPORTED, "recovery runs in the FP64 form only");
   if (enabled && c->hang_n) return fail(c, DG_ERR_UNSUPPORTED, "recovery runs meshes without hanging nodes");
   c->rec_on = enabled ? 1 : 0;
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured steps change
-  c->gexec = nullptr;
+  drop_graphs(c);  // captured steps change
   return DG_OK;
 }
 
@@ -1234,8 +1260,7 @@ static bool rebucket_in_place(dg_ctx* c, const std::vector<int32_t>& neword, dg_
   c->m_order = neword;
   // the captured step holds the old buffers and chunk count
   const dg_schedule sc{c->shape, c->mix_grid, {0, 0, 0, 0, 0, 0}};
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  c->gexec = nullptr;
+  drop_graphs(c);
   c->shape = 0;
   c->mix_grid = 0;
   if (schedule_ok(c, sc)) apply_schedule(c, sc);
@@ -1386,6 +1411,7 @@ dg_status dg_lts_setup(dg_ctx* c, int max_levels, int32_t* levels_out) {
   if (!plan.ctiles.empty())
     HIPCHECK(c, hipMemcpy(c->lts_ctiles, plan.ctiles.data(), plan.ctiles.size() * sizeof(LtsTile),
                           hipMemcpyHostToDevice));
+  drop_graphs(c);  // captured local-time-stepping steps hold the old plan's tiles
   c->lts_plan = std::move(plan);
   c->lts_nlev = nlev;
   c->lts_level = lev;
@@ -1711,8 +1737,7 @@ static bool schedule_ok(const dg_ctx* c, const dg_schedule& s) {
 
 static void apply_schedule(dg_ctx* c, const dg_schedule& s) {
   if (c->shape != s.shape || c->mix_grid != s.grid) {
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);  // captured with the old launch
-    c->gexec = nullptr;
+    drop_graphs(c);  // captured with the old launch
   }
   c->shape = s.shape;
   c->mix_grid = s.grid;

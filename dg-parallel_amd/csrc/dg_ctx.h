@@ -111,8 +111,8 @@ struct dg_ctx {
   uint8_t* npflags = nullptr;
   unsigned* status = nullptr;
   std::vector<void*> allocs;
-  // graph cache
-  hipGraphExec_t gexec = nullptr;
+  // captured step graphs: gexec holds several steps, gexec1 one step
+  hipGraphExec_t gexec = nullptr, gexec1 = nullptr;
   int g_scheme = -1;
   double g_dt = 0.0;
   // discrete adjoint (FP64, RK4): checkpoints every adj_every steps, the stage
