@@ -42,12 +42,14 @@ HBM_PEAK_GBS = 8000.0
 def bytes_per_dof_stage(fp32, nf, np_):
     """Algorithmic HBM bytes per DOF and RK4 stage of the fused stage kernel:
     stage input read once, FP64 step-start state and accumulator, stage output,
-    plus the element table (|J| J^-1: 4 doubles, |J|, 3 int32 links) per element."""
+    plus the element table (|J| J^-1: 4 doubles, |J|, 3 int32 links) per element.
+    The FP32 form's first stage reads U^n rounded to FP32 (written by the step
+    before's last stage) as its input and U^n in FP64 for the update."""
     s = 4 if fp32 else 8
-    st = [8 + 8 + s,            # stage 1: read U^n, write acc, write U1
-          s + 8 + 16 + s,       # stage 2: read U1, U^n, acc; write acc, U2
-          s + 8 + 16 + s,       # stage 3
-          s + 8 + 8 + 8]        # stage 4: read U3, U^n, acc; write U^{n+1}
+    st = [(4 + 8 if fp32 else 8) + 8 + s,  # stage 1: read U^n (FP32 copy, FP64), write acc, write U1
+          s + 8 + 16 + s,                   # stage 2: read U1, U^n, acc; write acc, U2
+          s + 8 + 16 + s,                   # stage 3
+          s + 8 + 8 + 8 + (4 if fp32 else 0)]  # stage 4: read U3, U^n, acc; write U^{n+1} (and its FP32 copy)
     table = (4 * 8 + 8 + 3 * 4) / (nf * np_)
     return sum(st) / 4.0 + table
 

@@ -332,7 +332,7 @@ static dg_status ctx_init(dg_ctx* c, const dg_mesh_desc* m, const dg_law_desc* l
   if (c->prec == DG_PREC_FP64) {
     if ((s = dalloc(c, &c->A64, ns)) || (s = dalloc(c, &c->B64, ns))) return bail(s);
   } else {
-    if ((s = dalloc(c, &c->A32, nd)) || (s = dalloc(c, &c->B32, nd))) return bail(s);
+    if ((s = dalloc(c, &c->A32, nd)) || (s = dalloc(c, &c->B32, nd)) || (s = dalloc(c, &c->M32, nd))) return bail(s);
   }
   // every order's tables (limiter, sensor, mixed kernels) and the chunk list of
   // the persistent mixed launch (heaviest order first)
@@ -775,6 +775,19 @@ static int nstages(int scheme) {
 // enqueue one full step on the stream; returns 0 on success
 static int enqueue_lts_step(dg_ctx* c, double h);
 
+// The FP32 form's steps on uniform-order triangles read their first stage from
+// M32 (the step-start state rounded to FP32: the same values the FP64-input
+// kernel rounds on load), so that stage runs the FP32-input kernel (fewer
+// registers in flight, one more wave per SIMD); the last stage writes M32 beside
+// U^n.  run_steps and dg_advance fill M32 from U^n before their steps.
+static bool uses_m32(const dg_ctx* c) {
+  return c->M32 && c->prec != DG_PREC_FP64 && c->dim == 2 && !c->mixed && !c->hang_n;
+}
+
+static int round_state_f32(dg_ctx* c) {
+  return uses_m32(c) ? launch_round_f32(c->Un, c->M32, (long)c->ndof, c->stream) : 0;
+}
+
 static int enqueue_step(dg_ctx* c, int scheme, double dt, bool want_resid) {
   if (scheme == DG_RK_TVD2_LTS) return enqueue_lts_step(c, dt);
   const int ns = nstages(scheme);
@@ -782,20 +795,22 @@ static int enqueue_step(dg_ctx* c, int scheme, double dt, bool want_resid) {
   a.dt = dt;
   a.scheme = scheme;
   const bool f64 = c->prec == DG_PREC_FP64;
+  const bool m32 = !f64 && ns > 1 && uses_m32(c);
   void* bufA = f64 ? (void*)c->A64 : (void*)c->A32;
   void* bufB = f64 ? (void*)c->B64 : (void*)c->B32;
   for (int st = 0; st < ns; st++) {
     a.stage = st;
     const bool first = st == 0, last = st == ns - 1;
-    a.Uin = first ? (const void*)c->Un : ((st % 2) ? bufA : bufB);
+    a.Uin = first ? (m32 ? (const void*)c->M32 : (const void*)c->Un) : ((st % 2) ? bufA : bufB);
     a.Uout = last ? (void*)c->Un : ((st % 2) ? bufB : bufA);
     if (first && last) a.Uout = bufA;  // forward Euler: never write the array being read
+    a.Uout32 = (last && m32) ? c->M32 : nullptr;
     a.resid_part = (last && want_resid) ? c->part : nullptr;
     int variant;
     if (f64)
       variant = 0;
     else
-      variant = first ? 1 : (last ? 3 : 2);
+      variant = first ? (m32 ? 2 : 1) : (last ? 3 : 2);
     if (launch_stage(c, a, variant)) return -1;
   }
   if (ns == 1) {
@@ -859,6 +874,7 @@ static dg_status ensure_graph(dg_ctx* c, int scheme, double dt) {
 
 static dg_status run_steps(dg_ctx* c, int scheme, double dt, int nsteps) {
   if (nsteps <= 0) return DG_OK;
+  if (round_state_f32(c)) return fail(c, DG_ERR_HIP, "state rounding launch failed");
   if (c->use_graph && scheme != DG_RK_FE) {
     dg_status s = ensure_graph(c, scheme, dt);
     if (s) return s;
@@ -915,7 +931,7 @@ dg_status dg_advance(dg_ctx* c, int scheme, double t, double dt, int nsteps, dou
   int body = resid_out ? nsteps - 1 : nsteps;
   if ((s = run_steps(c, scheme, dt, body))) return s;
   if (resid_out && nsteps > 0) {
-    if (enqueue_step(c, scheme, dt, true)) return fail(c, DG_ERR_HIP, "stage launch failed");
+    if (round_state_f32(c) || enqueue_step(c, scheme, dt, true)) return fail(c, DG_ERR_HIP, "stage launch failed");
     int nb = stage_blocks(c);
     std::vector<double> h(nb);
     HIPCHECK(c, hipMemcpyAsync(h.data(), c->part, nb * 8, hipMemcpyDeviceToHost, c->stream));

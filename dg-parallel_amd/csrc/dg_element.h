@@ -239,6 +239,9 @@ DG_HD void element_stage(const StageArgs& a, int e, double& rsum) {
             }
           }
           Uout[i] = OT(o);
+          if constexpr (sizeof(CT) == 4 && sizeof(OT) == 8) {
+            if (a.Uout32) a.Uout32[i] = float(o);
+          }
         }
     }
   }

@@ -52,6 +52,9 @@ struct StageArgs {
   // local time stepping: the launch's tiles (launch_tri_stage_lts), each with
   // its own range, flags (stage), level (dt / 2^b) and weight (lts_w)
   const LtsTile* lts_tiles;
+  // FP32 form, a stage with FP64 output (the step's last): the output rounded
+  // to FP32 as well, the next step's first-stage input (null: not written)
+  float* Uout32;
 };
 
 // variant: 0 = FP64 throughout; 1 = FP32 arithmetic, FP64 input, FP32 output;
@@ -63,6 +66,7 @@ int launch_tri_stage_tiled(const StageArgs& a, int law, int variant, hipStream_t
 int tri_tiled_blocks(int E);
 
 int launch_copy_probe(const void* src, void* dst, long n_doubles, hipStream_t s);
+int launch_round_f32(const double* in, float* out, long n, hipStream_t s);
 
 struct AdjArgs;
 int launch_tri_vjp(const AdjArgs& a, int law, int nd, hipStream_t s);

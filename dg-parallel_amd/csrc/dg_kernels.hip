@@ -242,6 +242,17 @@ __global__ __launch_bounds__(256) void copy_probe_kernel(const double4* __restri
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n4; i += (long)gridDim.x * 256) dst[i] = src[i];
 }
 
+// out[i] = in[i] rounded to FP32 (the FP32 form's first-stage input)
+__global__ __launch_bounds__(256) void round_f32_kernel(const double* __restrict__ in, float* __restrict__ out, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) out[i] = float(in[i]);
+}
+
+int launch_round_f32(const double* in, float* out, long n, hipStream_t s) {
+  const long nb = std::min<long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(round_f32_kernel, dim3(nb > 0 ? nb : 1), dim3(256), 0, s, in, out, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int launch_copy_probe(const void* src, void* dst, long n_doubles, hipStream_t s) {
   long n4 = n_doubles / 4;
   hipLaunchKernelGGL(copy_probe_kernel, dim3(4096), dim3(256), 0, s, (const double4*)src, (double4*)dst, n4);

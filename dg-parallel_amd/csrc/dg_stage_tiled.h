@@ -551,6 +551,9 @@ struct TiledStage {
               const double o = stage_value_hd(sc, double(r) * dtj, un[ki], us[ki], &accv, g.rsum);
               if (sc >= 3 * 4 && sc < 3 * 4 + 3) a.acc[i] = accv;
               ((OT*)a.Uout)[i] = OT(o);
+              if constexpr (sizeof(CT) == 4 && sizeof(OT) == 8) {
+                if (a.Uout32) a.Uout32[i] = float(o);
+              }
             }
           }
         }

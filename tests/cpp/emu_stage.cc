@@ -368,12 +368,20 @@ extern "C" int emu_tri_residual_rotating(const dg_mesh_desc* mesh, const dg_law_
 // the library (dg_capi.cc enqueue_step); resid: last step's residual norm
 extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int form, int scheme,
                              double dt, int nsteps, double* U, double* resid) {
-  const int fp32 = form & 1, shape = (form >> 1) & 1;
+  // form: bit 0 the FP32 form, bit 1 the 8-wave tile, bit 2 (FP32 form) the
+  // library's step sequence: the first stage reads the state rounded to FP32
+  // (the FP32-input kernel), the last stage writes that rounding beside U^n
+  const int fp32 = form & 1, shape = (form >> 1) & 1, m32 = fp32 && ((form >> 2) & 1);
   Emu m;
   if (make(m, mesh, law, order)) return -1;
   m.to_soa(U);
   const int ns = scheme == 0 ? 1 : scheme == 1 ? 2 : scheme == 2 ? 3 : 4;
   if (fp32 && ns == 1) return -3;
+  std::vector<float> M32;
+  if (m32) {
+    M32.resize(m.Un.size());
+    for (size_t i = 0; i < M32.size(); i++) M32[i] = float(m.Un[i]);
+  }
   void* bufA = fp32 ? (void*)m.A32.data() : (void*)m.A64.data();
   void* bufB = fp32 ? (void*)m.B32.data() : (void*)m.B64.data();
   for (int s = 0; s < nsteps; s++) {
@@ -383,11 +391,12 @@ extern "C" int emu_tri_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, i
     for (int st = 0; st < ns; st++) {
       a.stage = st;
       const bool first = st == 0, last = st == ns - 1;
-      a.Uin = first ? (const void*)m.Un.data() : ((st % 2) ? bufA : bufB);
+      a.Uin = first ? (m32 ? (const void*)M32.data() : (const void*)m.Un.data()) : ((st % 2) ? bufA : bufB);
       a.Uout = last ? (void*)m.Un.data() : ((st % 2) ? bufB : bufA);
       if (first && last) a.Uout = bufA;
+      a.Uout32 = (last && m32) ? M32.data() : nullptr;
       a.resid_part = last ? m.part.data() : nullptr;
-      const int variant = (fp32 ? (first ? 1 : (last ? 3 : 2)) : 0) | shape << 2;
+      const int variant = (fp32 ? (first ? (m32 ? 2 : 1) : (last ? 3 : 2)) : 0) | shape << 2;
       if (run_stage(m, a, variant)) return -2;
     }
     if (ns == 1) m.Un = m.A64;

@@ -134,6 +134,24 @@ def test_emulated_8_wave_steps_and_fp32_form():
     assert rel(U32, Uo) < 1e-5
 
 
+@pytest.mark.parametrize("law,flux,p,scheme,shape", [("euler", "lf", 3, po.RK4, 0), ("euler", "roe", 2, po.RK_TVD3, 2),
+                                                     ("scalar", None, 1, po.RK_TVD2, 0), ("euler", "lf", 4, po.RK4, 2)])
+def test_fp32_form_first_stage_from_rounded_state(law, flux, p, scheme, shape):
+    """The library's FP32 step sequence: the last stage also writes the new state
+    rounded to FP32, and the next step's first stage reads that copy through the
+    FP32-input kernel.  The FP64-input kernel rounds the same values on load, so
+    the steps are bit for bit those of the FP64-input first stage."""
+    mesh, lw, o, U = case(law, flux, p, periodic=(law == "scalar"))
+    dt = o.dt(U, 0.1)
+    Ua, ra = emu_steps(mesh, lw, p, U, scheme, dt, 3, fp32=1 | shape)
+    Ub, rb = emu_steps(mesh, lw, p, U, scheme, dt, 3, fp32=1 | shape | 4)
+    np.testing.assert_array_equal(Ua, Ub)
+    assert ra == rb
+    # the FP32 form is not the FP64 one (the comparison above is not vacuous)
+    U64, _ = emu_steps(mesh, lw, p, U, scheme, dt, 3)
+    assert 0 < rel(Ub, U64) < 1e-5
+
+
 def test_emulated_fp32_form():
     mesh, lw, o, U = case("euler", "lf", 3, nx=12)
     R64 = o.residual(U)[0]
