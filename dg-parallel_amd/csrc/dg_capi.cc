@@ -834,8 +834,12 @@ static dg_status check_scheme(dg_ctx* c, int scheme) {
 
 // Steps captured in one graph: one graph launch per GRAPH_STEPS steps, so the
 // launch gap between graphs (a few microseconds) comes once per GRAPH_STEPS
-// steps; the remainder runs the one-step graph.
+// steps; the remainder runs the one-step graph.  Calls of fewer than
+// GRAPH_MULTI_MIN steps replay the one-step graph only: capturing the longer
+// graph costs more host time than it saves over a few launches (a mixed-order
+// context captures again after every re-bucketing pass).
 constexpr int GRAPH_STEPS = 8;
+constexpr int GRAPH_MULTI_MIN = 64;
 
 // (scheme, dt)'s steps captured into one graph, instantiated and uploaded to
 // the device; advances nothing
@@ -856,19 +860,17 @@ static dg_status capture_steps(dg_ctx* c, int scheme, double dt, int n, hipGraph
   return DG_OK;
 }
 
-// the step graphs of (scheme, dt) (GRAPH_STEPS steps, and one step), built
-// when missing
-static dg_status ensure_graph(dg_ctx* c, int scheme, double dt) {
-  if (c->gexec && c->gexec1 && c->g_scheme == scheme && c->g_dt == dt) return DG_OK;
-  drop_graphs(c);
-  dg_status s = capture_steps(c, scheme, dt, GRAPH_STEPS, &c->gexec);
-  if (s) return s;
-  if ((s = capture_steps(c, scheme, dt, 1, &c->gexec1))) {
+// the step graphs of (scheme, dt), built when missing: the one-step graph, and
+// for a call of nsteps >= GRAPH_MULTI_MIN steps the GRAPH_STEPS-step graph
+static dg_status ensure_graph(dg_ctx* c, int scheme, double dt, int nsteps) {
+  if (!(c->gexec1 && c->g_scheme == scheme && c->g_dt == dt)) {
     drop_graphs(c);
-    return s;
+    dg_status s = capture_steps(c, scheme, dt, 1, &c->gexec1);
+    if (s) return s;
+    c->g_scheme = scheme;
+    c->g_dt = dt;
   }
-  c->g_scheme = scheme;
-  c->g_dt = dt;
+  if (!c->gexec && nsteps >= GRAPH_MULTI_MIN) return capture_steps(c, scheme, dt, GRAPH_STEPS, &c->gexec);
   return DG_OK;
 }
 
@@ -876,10 +878,12 @@ static dg_status run_steps(dg_ctx* c, int scheme, double dt, int nsteps) {
   if (nsteps <= 0) return DG_OK;
   if (round_state_f32(c)) return fail(c, DG_ERR_HIP, "state rounding launch failed");
   if (c->use_graph && scheme != DG_RK_FE) {
-    dg_status s = ensure_graph(c, scheme, dt);
+    dg_status s = ensure_graph(c, scheme, dt, nsteps);
     if (s) return s;
-    for (int i = 0; i < nsteps / GRAPH_STEPS; i++) HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
-    for (int i = 0; i < nsteps % GRAPH_STEPS; i++) HIPCHECK(c, hipGraphLaunch(c->gexec1, c->stream));
+    int i = 0;
+    if (c->gexec)
+      for (; i + GRAPH_STEPS <= nsteps; i += GRAPH_STEPS) HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
+    for (; i < nsteps; i++) HIPCHECK(c, hipGraphLaunch(c->gexec1, c->stream));
   } else {
     for (int s = 0; s < nsteps; s++)
       if (enqueue_step(c, scheme, dt, false)) return fail(c, DG_ERR_HIP, "stage launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -952,7 +956,7 @@ dg_status dg_time_advance(dg_ctx* c, int scheme, double dt, int nsteps, float* m
   HIPCHECK(c, hipEventCreate(&e0));
   HIPCHECK(c, hipEventCreate(&e1));
   // build the graph (if any) outside the timed region, without taking a step
-  if (c->use_graph && scheme != DG_RK_FE && (s = ensure_graph(c, scheme, dt))) return s;
+  if (c->use_graph && scheme != DG_RK_FE && (s = ensure_graph(c, scheme, dt, nsteps))) return s;
   HIPCHECK(c, hipEventRecord(e0, c->stream));
   s = run_steps(c, scheme, dt, nsteps);
   HIPCHECK(c, hipEventRecord(e1, c->stream));

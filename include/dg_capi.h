@@ -100,8 +100,8 @@ typedef struct {
 typedef struct {
   int order;              /* polynomial order when mesh.order is NULL */
   int precision;          /* DG_PREC_FP64: plain form; DG_PREC_FP32: FP32 residual, FP64 stage accumulation */
-  int use_graph;          /* capture dg_advance's steps in hipGraphs (eight steps per graph, one-step graphs
-                             for the remainder) and replay them */
+  int use_graph;          /* capture dg_advance's steps in hipGraphs and replay them (calls of 64 steps or
+                             more: eight steps per graph, one-step graphs for the remainder) */
   int device;             /* HIP device ordinal */
   int reserved[8];        /* reserved[0]: stage kernel, 0 tiled (default), 1 one element per work-item;
                              reserved[1]: adjoint flux tangent directions per pass, 0 (default), 1, 2, 4 */
