@@ -281,10 +281,11 @@ def run_maxwell(dgpar, steps, n=48, tune=True):
     r32 = run_maxwell_form(dgpar, mesh, dgpar.PREC_FP32, steps, tune)
     ndof = r64["ndof"]
     # algorithmic bytes per DOF*stage: RK4 state traffic and the element table
-    # (|J| J^-1, |J|, 4 links per element over 6 x 20 DOF); FP32 stage copies
+    # (|J| J^-1, |J|, 4 links per element over 6 x 20 DOF); FP32 stage copies,
+    # the first stage reading U^n's FP32 copy and U^n, the last writing both
     table = (9 * 8 + 8 + 4 * 4) / (6 * 20.0)
     b64 = 34.0 + table
-    b32 = (20 + 32 + 32 + 28) / 4.0 + table
+    b32 = (24 + 32 + 32 + 32) / 4.0 + table
 
     def leg(r, b):
         ach = b * ndof / (r["ms"] / (4 * steps) / 1e3) / 1e9

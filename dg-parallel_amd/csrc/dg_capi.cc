@@ -193,7 +193,7 @@ static dg_status ctx_init_tet(dg_ctx* c, const dg_mesh_desc* m, const dg_law_des
   if (c->prec == DG_PREC_FP64) {
     if ((s = dalloc(c, &c->A64, nd)) || (s = dalloc(c, &c->B64, nd))) return s;
   } else {
-    if ((s = dalloc(c, &c->A32, nd)) || (s = dalloc(c, &c->B32, nd))) return s;
+    if ((s = dalloc(c, &c->A32, nd)) || (s = dalloc(c, &c->B32, nd)) || (s = dalloc(c, &c->M32, nd))) return s;
   }
   const std::vector<float> tabf(S.tab.begin(), S.tab.end());
   if (hipMemcpy(c->mg, S.mg.data(), S.mg.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
@@ -544,6 +544,7 @@ static int launch_stage(dg_ctx* c, const StageArgs& a, int variant) {
     t.scheme = a.scheme;
     t.stage = a.stage;
     t.resid_part = a.resid_part;
+    t.Uout32 = a.Uout32;
     return launch_tet_stage(t, c->shape, variant & 3, c->stream);
   }
   if (c->mixed) {
@@ -775,13 +776,13 @@ static int nstages(int scheme) {
 // enqueue one full step on the stream; returns 0 on success
 static int enqueue_lts_step(dg_ctx* c, double h);
 
-// The FP32 form's steps on uniform-order triangles read their first stage from
+// The FP32 form's steps (tetrahedra, uniform-order triangles) read their first stage from
 // M32 (the step-start state rounded to FP32: the same values the FP64-input
 // kernel rounds on load), so that stage runs the FP32-input kernel (fewer
 // registers in flight, one more wave per SIMD); the last stage writes M32 beside
 // U^n.  run_steps and dg_advance fill M32 from U^n before their steps.
 static bool uses_m32(const dg_ctx* c) {
-  return c->M32 && c->prec != DG_PREC_FP64 && c->dim == 2 && !c->mixed && !c->hang_n;
+  return c->M32 && c->prec != DG_PREC_FP64 && !c->mixed && !c->hang_n;
 }
 
 static int round_state_f32(dg_ctx* c) {

@@ -65,7 +65,7 @@ struct dg_ctx {
   // device memory
   double *Un = nullptr, *acc = nullptr, *A64 = nullptr, *B64 = nullptr, *R = nullptr;
   float *A32 = nullptr, *B32 = nullptr;
-  // FP32 form on triangles: the step-start state rounded to FP32, written by
+  // FP32 form (tetrahedra, uniform-order triangles): the step-start state rounded to FP32, written by
   // each step's last stage and read by the next step's first stage
   float* M32 = nullptr;
   double *mg = nullptr, *detj = nullptr, *size = nullptr, *tab = nullptr, *vtx = nullptr, *part = nullptr;

@@ -51,6 +51,9 @@ struct TetArgs {
   double dt;
   int scheme, stage;     // scheme MODE_RESIDUAL: write R only
   double* resid_part;    // per tile (or null)
+  // FP32 form, the last stage: the FP64 output rounded to FP32 as well, the
+  // next step's first-stage input (null: not written)
+  float* Uout32;
 };
 
 // d F[f][d] / d U[g] of MaxwellLawD::flux as (d, g, sign) pairs per field
@@ -366,6 +369,9 @@ struct TiledTet {
                                         &accv, g.rsum);
         if (sc >= 3 * 4 && sc < 3 * 4 + 3) a.acc[i] = accv;
         ((OT*)a.Uout)[i] = OT(o);
+        if constexpr (sizeof(CT) == 4 && sizeof(OT) == 8) {
+          if (a.Uout32) a.Uout32[i] = float(o);
+        }
       }
   }
 };

@@ -1021,14 +1021,23 @@ extern "C" int emu_tet_steps(const dg_mesh_desc* mesh, const dg_law_desc* law, i
   return 0;
 }
 
+// 1: emu_tet_steps32 runs the library's step sequence, whose first stage reads
+// the step-start state rounded to FP32 (written by the step before's last
+// stage); 0: the first stage reads the FP64 state
+static int g_tet_m32 = 0;
+extern "C" void emu_tet_m32(int on) { g_tet_m32 = on; }
+
 // RK steps of the tetrahedron kernel's FP32 form with the library's buffer
-// sequence: stage 1 reads the FP64 step-start state and writes an FP32 copy,
-// the middle stages run FP32 to FP32, the last writes the FP64 state
+// sequence: stage 1 reads the step-start state and writes an FP32 copy, the
+// middle stages run FP32 to FP32, the last writes the FP64 state
 extern "C" int emu_tet_steps32(const dg_mesh_desc* mesh, const dg_law_desc* law, int order, int shape, int scheme,
                                double dt, int nsteps, double* U) {
   TetEmu m;
   if (tet_make(m, mesh, law, order)) return -1;
   m.to_soa(U, m.Un.data());
+  const bool m32 = g_tet_m32 != 0;
+  std::vector<float> M32(m32 ? m.Un.size() : 0);
+  for (size_t i = 0; i < M32.size(); i++) M32[i] = float(m.Un[i]);
   const int ns = scheme == 1 ? 2 : scheme == 2 ? 3 : 4;
   if (scheme == 0) return -3;
   for (int st = 0; st < nsteps; st++) {
@@ -1040,10 +1049,12 @@ extern "C" int emu_tet_steps32(const dg_mesh_desc* mesh, const dg_law_desc* law,
     for (int s = 0; s < ns; s++) {
       a.stage = s;
       const bool first = s == 0, last = s == ns - 1;
-      a.Uin = first ? (const void*)m.Un.data() : (const void*)((s % 2) ? m.A32.data() : m.B32.data());
+      a.Uin = first ? (m32 ? (const void*)M32.data() : (const void*)m.Un.data())
+                    : (const void*)((s % 2) ? m.A32.data() : m.B32.data());
       a.Uout = last ? (void*)m.Un.data() : (void*)((s % 2) ? m.B32.data() : m.A32.data());
+      a.Uout32 = (last && m32) ? M32.data() : nullptr;
       double rs = 0;
-      if (tet_run32(a, shape, first ? 1 : (last ? 3 : 2), &rs)) return -2;
+      if (tet_run32(a, shape, first ? (m32 ? 2 : 1) : (last ? 3 : 2), &rs)) return -2;
     }
   }
   m.from_soa(m.Un.data(), U);

@@ -171,3 +171,29 @@ def test_emulated_tet_fp32_form_rk4_steps(p, shape):
     err = np.abs(Ue - Uo).max() / np.abs(Uo).max()
     assert err <= 2e-6
     assert err < 1e-2 * np.abs(Uo - U).max() / np.abs(Uo).max()
+
+
+@emu
+@pytest.mark.parametrize("shape", [0, 2, 4])
+def test_emulated_tet_fp32_first_stage_from_rounded_state(shape):
+    """The library's FP32 step sequence on tetrahedra: the last stage also writes
+    the new state rounded to FP32, and the next step's first stage reads that copy
+    through the FP32-input kernel.  The FP64-input kernel rounds the same values
+    on load, so the steps are bit for bit those of the FP64-input first stage."""
+    vids, xyz, o = oracle(2, 2, po.FLUX_UPWIND)
+    U = o.project(3)
+    dt = o.dt(U, 0.1)
+    md = dgpar.Mesh(3, vids, xyz).desc()
+    ld = dgpar.MaxwellLaw(po.FLUX_UPWIND).desc()
+    out = []
+    for m32 in (0, 1):
+        LIBEMU.emu_tet_m32(m32)
+        Ue = U.copy()
+        try:
+            assert LIBEMU.emu_tet_steps32(ctypes.byref(md), ctypes.byref(ld), 2, shape, po.RK4, dt, 3,
+                                          Ue.ctypes.data_as(PD)) == 0
+        finally:
+            LIBEMU.emu_tet_m32(0)
+        out.append(Ue)
+    np.testing.assert_array_equal(out[0], out[1])
+    assert np.abs(out[1] - U).max() > 0
